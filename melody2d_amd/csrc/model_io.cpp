@@ -1,0 +1,829 @@
+// Model-file parsing (ASCII and binary) into mel::Model.  See model_io.hpp.
+#include "model_io.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+
+namespace mel {
+
+namespace {
+
+bool starts(const std::string& line, const char* kw) {
+    size_t n = std::strlen(kw);
+    return line.size() >= n && line.compare(0, n, kw) == 0;
+}
+
+int material_type_of(const std::string& t) {
+    if (t == "ElasticLinear") return MAT_ELASTIC_LINEAR;
+    if (t == "NeoHookean") return MAT_NEOHOOKEAN;
+    return -1;
+}
+
+// Number of parameters each law reads (Code/IO.h:721-793).
+int law_nparams(int type) {
+    switch (type) {
+        case LAW_FRICTIONLESS: return 1;
+        case LAW_COHESION: return 4;
+        case LAW_MOHR_COULOMB: return 5;
+        case LAW_DAMPED_MOHR_COULOMB: return 6;
+        case LAW_TWO_SLOPES: return 6;
+        case LAW_CZM_LINEAR: return 6;
+        case LAW_CZM_FATIGUE: return 9;
+        case LAW_BONDED_MOHR_COULOMB: return 11;
+    }
+    return 0;
+}
+
+int law_type_of(const std::string& t) {
+    static const char* names[] = {"Frictionless", "Cohesion", "MohrCoulomb", "DampedMohrCoulomb",
+                                  "TwoSlopesMohrCoulomb", "BondedMohrCoulomb", "CZMlinear", "CZMfatigue"};
+    for (int i = 0; i < 8; i++)
+        if (t == names[i]) return i;
+    return -1;
+}
+
+int periodicity_of(const std::string& p) {
+    if (p == "Periodic") return PER_PERIODIC;
+    if (p == "Simple") return PER_SIMPLE;
+    return PER_CLOSED;
+}
+
+int status_of(const std::string& s) {
+    if (s == "inactive") return ST_INACTIVE;
+    if (s == "nogravity") return ST_NOGRAVITY;
+    return ST_ACTIVE;
+}
+
+// mu, lambda | kappa conversion of Code/IO.h:678 and Code/IO.h:687
+void convert_material(Material& mat) {
+    const double* r = mat.raw;
+    mat.p[0] = r[0];
+    mat.p[1] = r[1];
+    mat.p[2] = r[2];
+    mat.p[3] = r[3] / (2. * (1. + r[4]));
+    if (mat.type == MAT_ELASTIC_LINEAR)
+        mat.p[4] = r[3] * r[4] / ((1. + r[4]) * (1. - 2. * r[4]));
+    else
+        mat.p[4] = r[3] / (3. * (1. - 2. * r[4]));
+    mat.p[5] = r[5];
+}
+
+void build_table(Model& m) {
+    size_t nm = m.materials.size();
+    m.contacts_table.assign(nm * nm, -1);
+    // later laws overwrite earlier ones, both orders (Code/IO.h:797-813)
+    for (size_t i = 0; i < m.laws.size(); i++) {
+        int nn = 0, pp = 0;
+        for (size_t n = 0; n < nm; n++)
+            if (m.materials[n].name == m.laws[i].m1) nn = (int)n;
+        for (size_t p = 0; p < nm; p++)
+            if (m.materials[p].name == m.laws[i].m2) pp = (int)p;
+        m.contacts_table[nn * nm + pp] = (int)i;
+        m.contacts_table[pp * nm + nn] = (int)i;
+    }
+}
+
+void read_bc(std::istream& in, BorderBC& bc) {
+    std::string direction, type1, type2, token;
+    in >> direction >> type1;
+    if (type1 == "None") {
+        bc.type = BC_NONE;
+        std::getline(in, token);
+        return;
+    }
+    in >> type2;
+    if (type1 == "Dirichlet") {
+        if (type2 == "Soft") {
+            bc.type = BC_SOFT;
+            in >> bc.stiffness >> bc.damping;
+        } else if (type2 == "Driven") {
+            bc.type = BC_DRIVEN;
+        }
+    } else if (type1 == "Neumann") {
+        bc.type = type2 == "Following" ? BC_FOLLOWING : BC_ORIENTED;
+    }
+    std::getline(in, token);
+    int n = 0;
+    in >> n;
+    std::getline(in, token);
+    for (int j = 0; j < n; j++) {
+        double t, v;
+        in >> t >> v;
+        std::getline(in, token);
+        bc.hist.t.push_back(t);
+        bc.hist.v.push_back(v);
+    }
+}
+
+void finish_body_state(Model& m, Body& b) {
+    size_t n = b.n_nodes;
+    auto z = [&](std::vector<double>& v) { if (v.size() != n) v.assign(n, 0.0); };
+    z(b.ux); z(b.uy); z(b.vx); z(b.vy); z(b.ax); z(b.ay);
+    z(b.upx); z(b.upy); z(b.vpx); z(b.vpy); z(b.apx); z(b.apy);
+    (void)m;
+}
+
+}  // namespace
+
+void initialize_segments(Model& m, Body& body) {
+    double period = m.xmax_period - m.xmin_period;
+    int nbr = (int)body.borders.size();
+    for (int i = 0; i < nbr; i++) {
+        Border& B = body.borders[i];
+        int n = (int)B.nodes.size();
+        B.node0.assign(n, 0); B.node1.assign(n, 0); B.node2.assign(n, 0); B.node3.assign(n, 0);
+        B.shift0.assign(n, 0); B.shift1.assign(n, 0); B.shift2.assign(n, 0); B.shift3.assign(n, 0);
+        B.initial_length.assign(n, 0.0);
+        if (n < 3) continue;
+        const Border& bb = body.borders[B.before];
+        const Border& ba = body.borders[B.after];
+        bool per = body.borders[i].periodicity == PER_PERIODIC;
+        // first border node
+        B.node0[0] = bb.nodes[bb.nodes.size() - 2];
+        B.node1[0] = B.nodes[0];
+        B.node2[0] = B.nodes[1];
+        B.node3[0] = B.nodes[2];
+        if (per) B.shift0[0] = (i == 0) ? -1 : 1;
+        for (int j = 1; j < n - 2; j++) {
+            B.node0[j] = B.nodes[j - 1];
+            B.node1[j] = B.nodes[j];
+            B.node2[j] = B.nodes[j + 1];
+            B.node3[j] = B.nodes[j + 2];
+        }
+        // second to last
+        B.node0[n - 2] = B.nodes[n - 3];
+        B.node1[n - 2] = B.nodes[n - 2];
+        B.node2[n - 2] = B.nodes[n - 1];
+        B.node3[n - 2] = ba.nodes[1];
+        if (per) {
+            int s = (i == 0) ? 1 : -1;
+            B.shift2[n - 2] = s;
+            B.shift3[n - 2] = s;
+        }
+        // last
+        B.node0[n - 1] = B.nodes[n - 2];
+        B.node1[n - 1] = B.nodes[n - 1];
+        B.node2[n - 1] = ba.nodes[1];
+        B.node3[n - 1] = ba.nodes[2];
+        if (per) {
+            int s = (i == 0) ? 1 : -1;
+            B.shift1[n - 1] = s;
+            B.shift2[n - 1] = s;
+            B.shift3[n - 1] = s;
+        }
+        for (int j = 0; j < n; j++) {
+            double x0 = body.X[B.node0[j]] + B.shift0[j] * period, y0 = body.Y[B.node0[j]];
+            double x1 = body.X[B.node1[j]] + B.shift1[j] * period, y1 = body.Y[B.node1[j]];
+            double x2 = body.X[B.node2[j]] + B.shift2[j] * period, y2 = body.Y[B.node2[j]];
+            B.initial_length[j] = 0.5 * std::sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0)) +
+                                  0.5 * std::sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
+        }
+    }
+}
+
+bool load_model_files(const std::string& dir, int save_index, Model& m, std::string& err) {
+    m = Model();
+    std::string line, token;
+    int index_body = 0;
+    // ---------------- STATIC_CONTROL.asc (Code/IO.h:646-1487) ----------------
+    {
+        std::ifstream in(dir + "/STATIC_CONTROL.asc");
+        if (!in) { err = "cannot open " + dir + "/STATIC_CONTROL.asc"; return false; }
+        while (std::getline(in, line)) {
+            if (starts(line, "SIMULATION_NAME")) {
+                in >> m.name;
+                std::getline(in, token);
+            }
+            if (starts(line, "MATERIALS")) {
+                int nm = 0;
+                in >> nm;
+                std::getline(in, token);
+                std::getline(in, token);
+                for (int i = 0; i < nm; i++) {
+                    Material mat;
+                    std::string type;
+                    in >> mat.name;
+                    std::getline(in, token);
+                    in >> type;
+                    std::getline(in, token);
+                    mat.type = material_type_of(type);
+                    if (mat.type < 0) { err = "unsupported material type " + type; return false; }
+                    for (int k = 0; k < 6; k++) in >> mat.raw[k];
+                    std::getline(in, token);
+                    convert_material(mat);
+                    m.materials.push_back(mat);
+                    std::getline(in, token);
+                }
+            }
+            if (starts(line, "CONTACT_LAWS")) {
+                int nl = 0;
+                in >> nl;
+                std::getline(in, token);
+                std::getline(in, token);
+                for (int i = 0; i < nl; i++) {
+                    ContactLaw law;
+                    std::string type, lev;
+                    in >> law.m1; std::getline(in, token);
+                    in >> law.m2; std::getline(in, token);
+                    in >> type; std::getline(in, token);
+                    in >> lev; std::getline(in, token);
+                    law.type = law_type_of(type);
+                    if (law.type < 0) { err = "unsupported contact law " + type; return false; }
+                    law.fixed_length = (lev == "Fixed");
+                    law.p.resize(law_nparams(law.type));
+                    for (auto& x : law.p) in >> x;
+                    std::getline(in, token);
+                    m.laws.push_back(law);
+                    std::getline(in, token);
+                }
+            }
+            if (starts(line, "SOLVER")) {
+                double target, ctrl, ratio;
+                in >> m.solver;
+                std::getline(in, token);
+                if (m.solver != "Euler") { err = "only the Euler solver is supported, found " + m.solver; return false; }
+                in >> m.tini >> m.dt >> m.tend;
+                std::getline(in, token);
+                in >> target >> ctrl >> ratio;
+                std::getline(in, token);
+                in >> m.save_period >> m.print_period >> m.contact_update_period;
+                std::getline(in, token);
+            }
+            if (starts(line, "PERIODIC_BOUNDARIES")) {
+                in >> m.xmin_period >> m.xmax_period;
+                std::getline(in, token);
+            }
+            if (starts(line, "PENALTY_PARAMETER")) {
+                double pen;
+                in >> pen;
+                std::getline(in, token);
+            }
+            if (starts(line, "GRAVITY")) {
+                in >> m.gravity_x >> m.gravity_y;
+                std::getline(in, token);
+            }
+            if (starts(line, "PLOTTING")) {
+                std::getline(in, token);
+                std::getline(in, token);
+            }
+            if (starts(line, "NUMBER_BODIES")) {
+                int nb;
+                in >> nb;
+                std::getline(in, token);
+            }
+            if (starts(line, "MONITORING") || starts(line, "DEACTIVATION") || starts(line, "SPIES")) {
+                int n = 0;
+                in >> n;
+                std::getline(in, token);
+                if (n != 0) { err = "monitoring, spies and deactivation entries are not supported"; return false; }
+            }
+            if (starts(line, "GRAPHIC")) {
+                for (int i = 0; i < 41; i++) {
+                    int j;
+                    in >> j >> token;
+                    std::getline(in, token);
+                }
+            }
+            bool def = starts(line, "DEFORMABLE"), rig = starts(line, "RIGID");
+            if (def || rig) {
+                Body b;
+                b.rigid = rig;
+                std::string mat, per;
+                in >> index_body; std::getline(in, token);
+                in >> mat; std::getline(in, token);
+                in >> per; std::getline(in, token);
+                in >> b.nodal_distance >> b.detection_distance >> b.contact_distance;
+                std::getline(in, token);
+                b.periodicity = periodicity_of(per);
+                b.material = -1;
+                for (size_t i = 0; i < m.materials.size(); i++)
+                    if (m.materials[i].name == mat) { b.material = (int)i; b.density = m.materials[i].p[0]; break; }
+                if (b.material < 0) { err = "unknown material " + mat; return false; }
+                if (index_body != (int)m.bodies.size()) { err = "bodies must be listed in index order"; return false; }
+                m.bodies.push_back(b);
+            }
+            if (starts(line, "BORDERS")) {
+                int nbr = 0;
+                in >> nbr;
+                std::getline(in, token);
+                std::getline(in, token);
+                Body& b = m.bodies.at(index_body);
+                b.borders.resize(nbr);
+                for (int i = 0; i < nbr; i++) {
+                    std::string per, interp;
+                    int nn;
+                    in >> per >> interp;
+                    std::getline(in, token);
+                    in >> nn;
+                    std::getline(in, token);
+                    b.borders[i].periodicity = periodicity_of(per);
+                    b.borders[i].linear = interp != "Spline";
+                    if (!b.borders[i].linear) { err = "Spline interpolant is not supported"; return false; }
+                    b.borders[i].nodes.resize(nn);
+                    read_bc(in, b.borders[i].xbc);
+                    read_bc(in, b.borders[i].ybc);
+                    std::getline(in, token);
+                }
+            }
+            if (starts(line, "ALID")) { err = "ALID absorbing layers are not supported"; return false; }
+        }
+    }
+    // ---------------- STATIC_DATA.asc (Code/IO.h:1491-1852) ----------------
+    {
+        std::ifstream in(dir + "/STATIC_DATA.asc");
+        if (!in) { err = "cannot open " + dir + "/STATIC_DATA.asc"; return false; }
+        bool rigid = false;
+        while (std::getline(in, line)) {
+            if (starts(line, "DEFORMABLE")) {
+                rigid = false;
+                in >> index_body;
+                std::getline(in, token);
+            }
+            if (starts(line, "RIGID")) {
+                rigid = true;
+                in >> index_body;
+                std::getline(in, token);
+                Body& b = m.bodies.at(index_body);
+                in >> b.xc >> b.yc;
+                std::getline(in, token);
+                in >> b.mass >> b.inertia >> b.inverse_mass >> b.inverse_inertia;
+                std::getline(in, token);
+            }
+            if (starts(line, "NODES")) {
+                Body& b = m.bodies.at(index_body);
+                int n;
+                in >> n;
+                std::getline(in, token);
+                std::getline(in, token);
+                b.n_nodes = n;
+                b.X.resize(n); b.Y.resize(n); b.domain.assign(n, 0); b.mx.assign(n, 0); b.my.assign(n, 0);
+                b.imx.assign(n, 0); b.imy.assign(n, 0);
+                for (int i = 0; i < n; i++) {
+                    double j;
+                    if (!rigid) in >> j >> b.X[i] >> b.Y[i] >> b.domain[i] >> b.mx[i] >> b.my[i] >> b.imx[i] >> b.imy[i];
+                    else in >> j >> b.X[i] >> b.Y[i];
+                    std::getline(in, token);
+                }
+            }
+            if (starts(line, "BORDERS")) {
+                Body& b = m.bodies.at(index_body);
+                int nbr;
+                in >> nbr;
+                std::getline(in, token);
+                std::getline(in, token);
+                if (nbr != (int)b.borders.size()) { err = "border count differs between control and data files"; return false; }
+                for (int i = 0; i < nbr; i++) {
+                    int nn, before, after;
+                    std::getline(in, token);
+                    in >> nn >> before >> after;
+                    std::getline(in, token);
+                    Border& B = b.borders[i];
+                    B.nodes.resize(nn);
+                    for (int j = 0; j < nn; j++) {
+                        in >> B.nodes[j];
+                        std::getline(in, token);
+                    }
+                    B.before = before;
+                    B.after = after;
+                    std::getline(in, token);
+                }
+                initialize_segments(m, b);
+            }
+            if (starts(line, "INTEGRATION")) {
+                Body& b = m.bodies.at(index_body);
+                std::string integ;
+                std::getline(in, integ);
+                if (!starts(integ, "Gauss")) { err = "only Gauss integration is supported"; return false; }
+                int G, R;
+                in >> G >> R;
+                std::getline(in, token);
+                std::getline(in, token);
+                b.n_gauss = G;
+                b.n_regions = R;
+                b.gx.resize(G); b.gy.resize(G); b.gw.resize(G); b.gjac.resize(G);
+                b.gptr.assign(G + 1, 0);
+                for (int g = 0; g < G; g++) {
+                    std::getline(in, token);
+                    in >> b.gx[g] >> b.gy[g] >> b.gw[g] >> b.gjac[g];
+                    std::getline(in, token);
+                    int k;
+                    in >> k;
+                    std::getline(in, token);
+                    for (int j = 0; j < k; j++) {
+                        int node, n1, n2;
+                        double f, fx, fy;
+                        in >> node >> n1 >> n2 >> f >> fx >> fy;
+                        std::getline(in, token);
+                        b.gidx.push_back(node);
+                        b.gphi.push_back(f);
+                        b.gphix.push_back(fx);
+                        b.gphiy.push_back(fy);
+                    }
+                    b.gptr[g + 1] = b.gptr[g] + k;
+                    std::getline(in, token);
+                }
+            }
+            if (starts(line, "TRIANGULATION")) {
+                Body& b = m.bodies.at(index_body);
+                int nc;
+                in >> nc;
+                std::getline(in, token);
+                b.cells.resize(3 * nc);
+                for (int i = 0; i < nc; i++) {
+                    int n0;
+                    in >> n0 >> b.cells[3 * i] >> b.cells[3 * i + 1] >> b.cells[3 * i + 2];
+                    std::getline(in, token);
+                }
+                std::getline(in, token);
+            }
+            if (starts(line, "DOF_TO_DISP")) {
+                Body& b = m.bodies.at(index_body);
+                b.dptr.assign(b.n_nodes + 1, 0);
+                for (int i = 0; i < b.n_nodes; i++) {
+                    std::getline(in, token);
+                    std::getline(in, token);
+                    int k;
+                    in >> k;
+                    std::getline(in, token);
+                    for (int j = 0; j < k; j++) {
+                        int nn;
+                        double ss, sx, sy;
+                        in >> nn >> ss >> sx >> sy;
+                        std::getline(in, token);
+                        b.didx.push_back(nn);
+                        b.dphi.push_back(ss);
+                        b.dphix.push_back(sx);
+                        b.dphiy.push_back(sy);
+                    }
+                    b.dptr[i + 1] = b.dptr[i] + k;
+                }
+            }
+            if (starts(line, "MATRICES")) {
+                Body& b = m.bodies.at(index_body);
+                int nmat;
+                in >> nmat;
+                std::getline(in, token);
+                std::getline(in, token);
+                for (int i = 0; i < nmat; i++) {
+                    int d0, d1;
+                    double s, d;
+                    in >> d0 >> d1 >> s >> d;
+                    std::getline(in, token);
+                    b.mat_dof.push_back(d0);
+                    b.mat_dof.push_back(d1);
+                    b.mat_stiffness.push_back(s);
+                    b.mat_damping.push_back(d);
+                }
+            }
+        }
+    }
+    build_table(m);
+    // ---------------- DYNAMIC_nnnnn.asc (Code/IO.h:1871-2343) ----------------
+    {
+        char name[64];
+        std::snprintf(name, sizeof name, "/DYNAMIC_%05d.asc", save_index);
+        std::ifstream in(dir + name);
+        if (!in) { err = "cannot open " + dir + name; return false; }
+        for (auto& b : m.bodies) finish_body_state(m, b);
+        bool rigid = false;
+        static const char* flag_names[F_COUNT] = {"KILL_VELOCITY", "MONITOR_ENERGY", "MONITOR_BOUNDARIES",
+                                                  "INITIALIZE_CZM", "UPDATE_MASS_MATRIX", "UPDATE_STIFFNESS_MATRIX",
+                                                  "UPDATE_DAMPING_MATRIX", "NO_LOG", "NO_MONITORING",
+                                                  "NO_SELF_CONTACT", "RESET_WORK"};
+        while (std::getline(in, line)) {
+            for (int f = 0; f < F_COUNT; f++)
+                if (starts(line, flag_names[f])) m.flags[f] = 1;
+            if (starts(line, "SOLVER")) {
+                std::string solver;
+                in >> solver;
+                std::getline(in, token);
+                in >> m.number_save >> m.number_iteration >> m.time >> m.number_print;
+                std::getline(in, token);
+                in >> m.dt;
+                std::getline(in, token);
+                in >> m.next_save >> m.next_print >> m.next_contact_update;
+                std::getline(in, token);
+            }
+            bool def = starts(line, "DEFORMABLE"), rig = starts(line, "RIGID");
+            if (def || rig) {
+                std::string st;
+                in >> st;
+                std::getline(in, token);
+                in >> index_body;
+                std::getline(in, token);
+                m.bodies.at(index_body).status = status_of(st);
+                rigid = rig;
+            }
+            if (starts(line, "KINEMATICS")) {
+                Body& b = m.bodies.at(index_body);
+                if (!rigid) {
+                    int n;
+                    in >> n;
+                    std::getline(in, token);
+                    for (int i = 0; i < n && i < b.n_nodes; i++) {
+                        double j, xc, yc;
+                        in >> j >> xc >> yc >> b.ux[i] >> b.uy[i] >> b.vx[i] >> b.vy[i] >> b.ax[i] >> b.ay[i] >>
+                            b.upx[i] >> b.upy[i] >> b.vpx[i] >> b.vpy[i] >> b.apx[i] >> b.apy[i];
+                        std::getline(in, token);
+                        if (m.flags[F_KILL_VELOCITY]) { b.vx[i] = b.vy[i] = b.vpx[i] = b.vpy[i] = 0.0; }
+                    }
+                } else {
+                    in >> b.rigid_cur[0] >> b.rigid_cur[1] >> b.rigid_cur[2];
+                    std::getline(in, token);
+                    in >> b.rigid_disp[0] >> b.rigid_disp[1] >> b.rigid_disp[2];
+                    std::getline(in, token);
+                    in >> b.rigid_vel[0] >> b.rigid_vel[1] >> b.rigid_vel[2];
+                    std::getline(in, token);
+                    if (m.flags[F_KILL_VELOCITY]) b.rigid_vel[0] = b.rigid_vel[1] = b.rigid_vel[2] = 0.0;
+                }
+            }
+            if (starts(line, "WORKS")) {
+                Body& b = m.bodies.at(index_body);
+                double w[7];
+                for (int k = 0; k < 7; k++) in >> w[k];
+                std::getline(in, token);
+                for (int k = 0; k < 7; k++) b.works[k] = m.flags[F_RESET_WORK] ? 0.0 : w[k];
+            }
+            if (starts(line, "NEIGHBOURS")) {
+                Body& b = m.bodies.at(index_body);
+                int n;
+                in >> n;
+                std::getline(in, token);
+                b.neighbours.resize(3 * n);
+                for (int i = 0; i < n; i++) {
+                    in >> b.neighbours[3 * i] >> b.neighbours[3 * i + 1] >> b.neighbours[3 * i + 2];
+                    std::getline(in, token);
+                }
+            }
+            if (starts(line, "BORDERS")) {
+                Body& b = m.bodies.at(index_body);
+                for (size_t i = 0; i < b.borders.size(); i++) {
+                    std::getline(in, token);
+                    std::getline(in, token);
+                    int n;
+                    in >> n;
+                    std::getline(in, token);
+                    for (int j = 0; j < n; j++) std::getline(in, token);
+                }
+            }
+            if (starts(line, "CONTACTS")) {
+                Body& b = m.bodies.at(index_body);
+                int n;
+                in >> n;
+                std::getline(in, token);
+                b.contacts.assign(n, ContactElement());
+                for (int i = 0; i < n; i++) {
+                    ContactElement& c = b.contacts[i];
+                    int ni;
+                    double N[4];
+                    int nd[4], sh[4];
+                    in >> c.borderS >> c.border_nodeS >> c.nodeS >> c.bodyM >> c.borderM >> c.shiftM >> c.segmentM >>
+                        c.gapn >> c.gapt >> c.xsi;
+                    in >> c.xnorm >> c.ynorm >> c.xtan >> c.ytan;
+                    in >> c.length >> c.fx >> c.fy;
+                    for (int k = 0; k < 4; k++) in >> nd[k] >> N[k] >> sh[k];
+                    in >> c.effective_mass >> ni;
+                    // Code/IO.h:2270-2281 stores all four master nodes into slot 0;
+                    // Update_contacts rebuilds them from segmentM before use.
+                    c.nodeM[0] = nd[3]; c.shapeM[0] = N[3]; c.shiftMk[0] = sh[3];
+                    c.nb_internal = ni;
+                    if (ni == 0) c.internal.assign(1, 0.0);
+                    for (int k = 0; k < ni; k++) {
+                        double v;
+                        in >> v;
+                        c.internal.push_back(v);
+                    }
+                }
+            }
+        }
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// binary container
+// ---------------------------------------------------------------------------
+
+namespace {
+struct Rec {
+    int kind = 0;
+    std::vector<double> d;
+    std::vector<int> i;
+};
+}  // namespace
+
+bool load_model_binary(const std::string& path, Model& m, std::string& err) {
+    m = Model();
+    std::ifstream in(path, std::ios::binary);
+    if (!in) { err = "cannot open " + path; return false; }
+    std::map<std::string, Rec> recs;
+    while (true) {
+        uint32_t l;
+        if (!in.read((char*)&l, 4)) break;
+        std::string name(l, '\0');
+        in.read(&name[0], l);
+        uint8_t kind;
+        uint64_t n;
+        in.read((char*)&kind, 1);
+        in.read((char*)&n, 8);
+        Rec& r = recs[name];
+        r.kind = kind;
+        if (kind == 0) { r.d.resize(n); in.read((char*)r.d.data(), 8 * n); }
+        else { r.i.resize(n); in.read((char*)r.i.data(), 4 * n); }
+        if (!in) { err = "truncated record " + name; return false; }
+    }
+    auto D = [&](const std::string& k) -> std::vector<double>& {
+        auto it = recs.find(k);
+        if (it == recs.end()) throw std::runtime_error("missing record " + k);
+        return it->second.d;
+    };
+    auto I = [&](const std::string& k) -> std::vector<int>& {
+        auto it = recs.find(k);
+        if (it == recs.end()) throw std::runtime_error("missing record " + k);
+        return it->second.i;
+    };
+    try {
+        auto& h = D("model.header");
+        m.tini = h[0]; m.dt = h[1]; m.tend = h[2]; m.save_period = h[3]; m.print_period = h[4];
+        m.contact_update_period = h[5]; m.xmin_period = h[6]; m.xmax_period = h[7];
+        m.gravity_x = h[8]; m.gravity_y = h[9]; m.time = h[10]; m.number_save = (int)h[11];
+        m.number_iteration = (int)h[12]; m.number_print = (int)h[13]; m.next_save = h[14];
+        m.next_print = h[15]; m.next_contact_update = h[16];
+        auto& fl = I("model.flags");
+        for (int f = 0; f < F_COUNT; f++) m.flags[f] = fl[f];
+        auto& mt = I("model.material_type");
+        auto& mr = D("model.material_raw");
+        for (size_t k = 0; k < mt.size(); k++) {
+            Material mat;
+            mat.name = "Material" + std::to_string(k);
+            mat.type = mt[k];
+            for (int q = 0; q < 6; q++) mat.raw[q] = mr[6 * k + q];
+            convert_material(mat);
+            m.materials.push_back(mat);
+        }
+        auto& lm = I("model.law_materials");
+        auto& lt = I("model.law_type");
+        auto& lf = I("model.law_fixed");
+        auto& lp = I("model.law_ptr");
+        auto& lv = D("model.law_params");
+        for (size_t k = 0; k < lt.size(); k++) {
+            ContactLaw law;
+            law.m1 = m.materials.at(lm[2 * k]).name;
+            law.m2 = m.materials.at(lm[2 * k + 1]).name;
+            law.type = lt[k];
+            law.fixed_length = lf[k] != 0;
+            law.p.assign(lv.begin() + lp[k], lv.begin() + lp[k + 1]);
+            if ((int)law.p.size() != law_nparams(law.type)) { err = "law parameter count"; return false; }
+            m.laws.push_back(law);
+        }
+        build_table(m);
+        int nb = I("model.n_bodies")[0];
+        m.bodies.resize(nb);
+        for (int bi = 0; bi < nb; bi++) {
+            std::string p = "b" + std::to_string(bi) + ".";
+            Body& b = m.bodies[bi];
+            auto& info = I(p + "info");
+            b.rigid = info[0] != 0;
+            b.material = info[1];
+            b.periodicity = info[2];
+            b.status = info[3];
+            b.n_nodes = info[4];
+            b.n_gauss = info[5];
+            b.n_regions = info[6];
+            b.density = m.materials.at(b.material).p[0];
+            auto& dist = D(p + "dist");
+            b.nodal_distance = dist[0]; b.detection_distance = dist[1]; b.contact_distance = dist[2];
+            b.X = D(p + "X");
+            b.Y = D(p + "Y");
+            auto& bptr = I(p + "border_ptr");
+            auto& bnod = I(p + "border_nodes");
+            auto& meta = I(p + "border_meta");
+            auto& bcp = D(p + "border_bcparams");
+            auto& hptr = I(p + "border_hist_ptr");
+            auto& ht = D(p + "hist_t");
+            auto& hv = D(p + "hist_v");
+            int nbr = (int)bptr.size() - 1;
+            b.borders.resize(nbr);
+            for (int k = 0; k < nbr; k++) {
+                Border& B = b.borders[k];
+                B.nodes.assign(bnod.begin() + bptr[k], bnod.begin() + bptr[k + 1]);
+                B.periodicity = meta[5 * k];
+                B.before = meta[5 * k + 1];
+                B.after = meta[5 * k + 2];
+                B.xbc.type = meta[5 * k + 3];
+                B.ybc.type = meta[5 * k + 4];
+                B.xbc.stiffness = bcp[4 * k]; B.xbc.damping = bcp[4 * k + 1];
+                B.ybc.stiffness = bcp[4 * k + 2]; B.ybc.damping = bcp[4 * k + 3];
+                B.xbc.hist.t.assign(ht.begin() + hptr[2 * k], ht.begin() + hptr[2 * k + 1]);
+                B.xbc.hist.v.assign(hv.begin() + hptr[2 * k], hv.begin() + hptr[2 * k + 1]);
+                B.ybc.hist.t.assign(ht.begin() + hptr[2 * k + 1], ht.begin() + hptr[2 * k + 2]);
+                B.ybc.hist.v.assign(hv.begin() + hptr[2 * k + 1], hv.begin() + hptr[2 * k + 2]);
+            }
+            initialize_segments(m, b);
+            b.cells = I(p + "cells");
+            if (!b.rigid) {
+                b.domain = D(p + "domain");
+                b.mx = D(p + "mx"); b.my = D(p + "my");
+                b.imx.resize(b.n_nodes); b.imy.resize(b.n_nodes);
+                for (int i = 0; i < b.n_nodes; i++) { b.imx[i] = 1.0 / b.mx[i]; b.imy[i] = 1.0 / b.my[i]; }
+                auto& gq = D(p + "gauss");
+                b.gx.resize(b.n_gauss); b.gy.resize(b.n_gauss); b.gw.resize(b.n_gauss); b.gjac.resize(b.n_gauss);
+                for (int g = 0; g < b.n_gauss; g++) {
+                    b.gx[g] = gq[4 * g]; b.gy[g] = gq[4 * g + 1]; b.gw[g] = gq[4 * g + 2]; b.gjac[g] = gq[4 * g + 3];
+                }
+                b.gptr = I(p + "gptr"); b.gidx = I(p + "gidx");
+                b.gphi = D(p + "gphi"); b.gphix = D(p + "gphix"); b.gphiy = D(p + "gphiy");
+                b.dptr = I(p + "dptr"); b.didx = I(p + "didx");
+                b.dphi = D(p + "dphi"); b.dphix = D(p + "dphix"); b.dphiy = D(p + "dphiy");
+                finish_body_state(m, b);
+                auto& kin = D(p + "kin");
+                std::vector<double>* f[12] = {&b.ux, &b.uy, &b.vx, &b.vy, &b.ax, &b.ay,
+                                              &b.upx, &b.upy, &b.vpx, &b.vpy, &b.apx, &b.apy};
+                for (int q = 0; q < 12; q++)
+                    for (int i = 0; i < b.n_nodes; i++) (*f[q])[i] = kin[(size_t)q * b.n_nodes + i];
+            } else {
+                auto& r = D(p + "rigid");
+                b.xc = r[0]; b.yc = r[1]; b.mass = r[2]; b.inertia = r[3]; b.inverse_mass = r[4]; b.inverse_inertia = r[5];
+                auto& rk = D(p + "rkin");
+                for (int q = 0; q < 3; q++) { b.rigid_cur[q] = rk[q]; b.rigid_disp[q] = rk[3 + q]; b.rigid_vel[q] = rk[6 + q]; }
+                b.domain.assign(b.n_nodes, 0.0);
+                b.mx.assign(b.n_nodes, 0.0); b.my.assign(b.n_nodes, 0.0);
+                b.imx.assign(b.n_nodes, 0.0); b.imy.assign(b.n_nodes, 0.0);
+                finish_body_state(m, b);
+            }
+        }
+    } catch (std::exception& e) {
+        err = e.what();
+        return false;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// DYNAMIC writer (layout of Code/IO.h:2351-2636, full precision)
+// ---------------------------------------------------------------------------
+
+bool write_dynamic_text(const std::string& path, const Model& m, std::string& err) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) { err = "cannot write " + path; return false; }
+    auto g = [&](double x) { std::fprintf(f, "%.17g ", x); };
+    std::fprintf(f, " \nSIMULATION_NAME\n%s\n \nSOLVER\n%s\n%d %d %.17g %d \n%.17g\n%.17g %.17g %.17g \n",
+                 m.name.c_str(), m.solver.c_str(), m.number_save, m.number_iteration, m.time, m.number_print,
+                 m.dt, m.next_save, m.next_print, m.next_contact_update);
+    static const char* flag_names[F_COUNT] = {"KILL_VELOCITY", "MONITOR_ENERGY", "MONITOR_BOUNDARIES",
+                                              "INITIALIZE_CZM", "UPDATE_MASS_MATRIX", "UPDATE_STIFFNESS_MATRIX",
+                                              "UPDATE_DAMPING_MATRIX", "NO_LOG", "NO_MONITORING",
+                                              "NO_SELF_CONTACT", "RESET_WORK"};
+    for (int k = 0; k < F_COUNT; k++)
+        if (m.flags[k] && k != F_INITIALIZE_CZM && k != F_KILL_VELOCITY) std::fprintf(f, "%s\n", flag_names[k]);
+    std::fprintf(f, " \n");
+    for (size_t bi = 0; bi < m.bodies.size(); bi++) {
+        const Body& b = m.bodies[bi];
+        const char* st = b.status == ST_INACTIVE ? "inactive" : b.status == ST_NOGRAVITY ? "nogravity" : "active";
+        std::fprintf(f, " \n%s\n%s\n%zu\n \nKINEMATICS\n", b.rigid ? "RIGID" : "DEFORMABLE", st, bi);
+        if (!b.rigid) {
+            std::fprintf(f, "%d\n", b.n_nodes);
+            for (int i = 0; i < b.n_nodes; i++) {
+                std::fprintf(f, "%d ", i);
+                g(b.X[i] + b.ux[i]); g(b.Y[i] + b.uy[i]);
+                g(b.ux[i]); g(b.uy[i]); g(b.vx[i]); g(b.vy[i]); g(b.ax[i]); g(b.ay[i]);
+                g(b.upx[i]); g(b.upy[i]); g(b.vpx[i]); g(b.vpy[i]); g(b.apx[i]); g(b.apy[i]);
+                std::fprintf(f, "\n");
+            }
+        } else {
+            for (int q = 0; q < 3; q++) g(b.rigid_cur[q]);
+            std::fprintf(f, "\n");
+            for (int q = 0; q < 3; q++) g(b.rigid_disp[q]);
+            std::fprintf(f, "\n");
+            for (int q = 0; q < 3; q++) g(b.rigid_vel[q]);
+            std::fprintf(f, "\n");
+        }
+        std::fprintf(f, " \nWORKS\n");
+        for (int k = 0; k < 7; k++) g(b.works[k]);
+        std::fprintf(f, "\n \nNEIGHBOURS\n%zu\n", b.neighbours.size() / 3);
+        for (size_t k = 0; k < b.neighbours.size() / 3; k++)
+            std::fprintf(f, "%d %d %d\n", b.neighbours[3 * k], b.neighbours[3 * k + 1], b.neighbours[3 * k + 2]);
+        std::fprintf(f, " \nCONTACTS\n%zu\n", b.contacts.size());
+        for (const ContactElement& c : b.contacts) {
+            std::fprintf(f, "%d %d %d %d %d %d %d ", c.borderS, c.border_nodeS, c.nodeS, c.bodyM, c.borderM, c.shiftM,
+                         c.segmentM);
+            g(c.gapn); g(c.gapt); g(c.xsi); g(c.xnorm); g(c.ynorm); g(c.xtan); g(c.ytan);
+            g(c.length); g(c.fx); g(c.fy);
+            for (int k = 0; k < 4; k++) { std::fprintf(f, "%d ", c.nodeM[k]); g(c.shapeM[k]); std::fprintf(f, "%d ", c.shiftMk[k]); }
+            g(c.effective_mass);
+            std::fprintf(f, "%d ", c.nb_internal);
+            for (int k = 0; k < c.nb_internal; k++) g(c.internal[k]);
+            std::fprintf(f, "\n");
+        }
+        std::fprintf(f, " \n");
+    }
+    std::fclose(f);
+    return true;
+}
+
+}  // namespace mel

@@ -1,0 +1,146 @@
+// Host-side model: the reference's model files parsed into flat per-body
+// arrays (SoA), plus a binary container for models too large for ASCII.
+//
+// Format semantics follow the reference loaders: Load_static
+// (Code/IO.h:598-1862) and Load_dynamic (Code/IO.h:1871-2343).  Material
+// parameters are converted exactly as Code/IO.h:678 / :687 do, the Gauss
+// points of a body are cut into ceil(G/R) contiguous regions
+// (Code/IO.h:1743-1755), the four-node segment stencils are built as
+// Body::Initialize_segments does (Code/Body.h:403-605).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace mel {
+
+enum BcType { BC_NONE = 0, BC_DRIVEN = 1, BC_SOFT = 2, BC_ORIENTED = 3, BC_FOLLOWING = 4 };
+enum Periodicity { PER_CLOSED = 0, PER_SIMPLE = 1, PER_PERIODIC = 2 };
+enum MaterialType { MAT_ELASTIC_LINEAR = 0, MAT_NEOHOOKEAN = 1 };
+enum LawType {
+    LAW_FRICTIONLESS = 0, LAW_COHESION = 1, LAW_MOHR_COULOMB = 2, LAW_DAMPED_MOHR_COULOMB = 3,
+    LAW_TWO_SLOPES = 4, LAW_BONDED_MOHR_COULOMB = 5, LAW_CZM_LINEAR = 6, LAW_CZM_FATIGUE = 7
+};
+enum Status { ST_ACTIVE = 0, ST_INACTIVE = 1, ST_NOGRAVITY = 2 };
+
+// flag slots of the DYNAMIC file, Code/IO.h:1912-1933
+enum Flag {
+    F_KILL_VELOCITY = 0, F_MONITOR_ENERGY, F_MONITOR_BOUNDARIES, F_INITIALIZE_CZM, F_UPDATE_MASS,
+    F_UPDATE_STIFFNESS, F_UPDATE_DAMPING, F_NO_LOG, F_NO_MONITORING, F_NO_SELF_CONTACT, F_RESET_WORK,
+    F_COUNT
+};
+
+struct History {
+    std::vector<double> t, v;
+};
+
+struct BorderBC {
+    int type = BC_NONE;
+    History hist;             // Dirichlet (velocity) or Neumann (pressure) history
+    double stiffness = 0, damping = 0;
+};
+
+struct Border {
+    int periodicity = PER_CLOSED;
+    bool linear = true;       // interpolant "Linear" (Spline is not supported)
+    int before = 0, after = 0;
+    std::vector<int> nodes;   // a closed border repeats its first node at the end
+    BorderBC xbc, ybc;
+    // four-node stencils (Body::Initialize_segments)
+    std::vector<int> node0, node1, node2, node3, shift0, shift1, shift2, shift3;
+    std::vector<double> initial_length;
+};
+
+struct Material {
+    std::string name;
+    int type = MAT_NEOHOOKEAN;
+    double raw[6] = {0, 0, 0, 0, 0, 0};     // rho alpha beta E nu c as written
+    double p[6] = {0, 0, 0, 0, 0, 0};       // rho alpha beta mu lambda|kappa c
+};
+
+struct ContactLaw {
+    std::string m1, m2;
+    int type = LAW_FRICTIONLESS;
+    bool fixed_length = false;               // "Fixed" vs "Evolutive"
+    std::vector<double> p;
+};
+
+struct ContactElement {                      // Contact_element (Code/Contact_Element.h:11-44)
+    int borderS = 0, border_nodeS = 0, nodeS = 0, bodyM = 0, borderM = 0, shiftM = 0, segmentM = 0;
+    int nodeM[4] = {0, 0, 0, 0}, shiftMk[4] = {0, 0, 0, 0};
+    double shapeM[4] = {0, 0, 0, 0};
+    double gapn = 0, vgapn = 0, gapt = 0, vgapt = 0, xsi = 0, xnorm = 0, ynorm = 0, xtan = 0, ytan = 0;
+    double effective_mass = 0, length = 0, fx = 0, fy = 0;
+    int nb_internal = 0;
+    std::vector<double> internal;
+};
+
+struct Body {
+    bool rigid = false;
+    int material = 0;
+    int periodicity = PER_SIMPLE;
+    int status = ST_ACTIVE;
+    double nodal_distance = 0, detection_distance = 0, contact_distance = 0;
+    double density = 0;
+    int n_nodes = 0;
+    std::vector<double> X, Y, domain, mx, my, imx, imy;
+    std::vector<Border> borders;
+    // deformable
+    int n_gauss = 0, n_regions = 1;
+    std::vector<double> gx, gy, gw, gjac;
+    std::vector<int> gptr, gidx;
+    std::vector<double> gphi, gphix, gphiy;
+    std::vector<int> dptr, didx;
+    std::vector<double> dphi, dphix, dphiy;
+    std::vector<int> cells;                  // 3 per triangle
+    std::vector<int> mat_dof;                // 2 per matrix entry (upper triangle)
+    std::vector<double> mat_stiffness, mat_damping;
+    // rigid
+    double xc = 0, yc = 0, mass = 0, inertia = 0, inverse_mass = 0, inverse_inertia = 0;
+    // kinematic state as loaded (deformable: per node; rigid: body level)
+    std::vector<double> ux, uy, vx, vy, ax, ay, upx, upy, vpx, vpy, apx, apy;
+    double rigid_cur[3] = {0, 0, 0}, rigid_disp[3] = {0, 0, 0}, rigid_vel[3] = {0, 0, 0};
+    double works[7] = {0, 0, 0, 0, 0, 0, 0};
+    std::vector<int> neighbours;             // 3 per neighbour
+    std::vector<ContactElement> contacts;
+};
+
+struct Model {
+    std::string name;
+    std::vector<Material> materials;
+    std::vector<ContactLaw> laws;
+    std::vector<int> contacts_table;          // n_materials^2, -1 = no law
+    std::string solver = "Euler";
+    double tini = 0, dt = 0, tend = 0;
+    double save_period = 0, print_period = 0, contact_update_period = 0;
+    double xmin_period = 0, xmax_period = 0, gravity_x = 0, gravity_y = 0;
+    int flags[F_COUNT] = {0};
+    double time = 0;
+    int number_save = 0, number_iteration = 0, number_print = 0;
+    double next_save = 0, next_print = 0, next_contact_update = 0;
+    std::vector<Body> bodies;
+
+    int law_index(int mat_s, int mat_m) const {
+        return contacts_table[(size_t)mat_s * materials.size() + mat_m];
+    }
+    size_t total_nodes() const {
+        size_t n = 0;
+        for (auto& b : bodies) n += b.n_nodes;
+        return n;
+    }
+};
+
+// Parse STATIC_CONTROL.asc, STATIC_DATA.asc and DYNAMIC_<save>.asc from dir.
+bool load_model_files(const std::string& dir, int save_index, Model& m, std::string& err);
+
+// Read the binary record container written by the preprocessor package.
+bool load_model_binary(const std::string& path, Model& m, std::string& err);
+
+// Segment stencils and initial lengths of every border (Code/Body.h:403-605).
+void initialize_segments(Model& m, Body& b);
+
+// Write DYNAMIC_nnnnn-format text of the current state (Code/IO.h:2351-2636 layout,
+// 17 significant digits).
+bool write_dynamic_text(const std::string& path, const Model& m, std::string& err);
+
+}  // namespace mel

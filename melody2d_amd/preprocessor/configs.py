@@ -1,0 +1,204 @@
+"""Seeded workload generators.
+
+Seeds: workload k uses seed 1000 + k; inside a workload one numpy PCG64
+stream per body in creation order (seed sequence spawned from the workload
+seed), so a body's node cloud does not depend on how many bodies follow it.
+
+Workloads
+  tiny        two deformable discs (~30 nodes each) touching, no platens
+  tiny_platen two small discs between two rigid platens (driven)
+  c1          two elastic discs R = 1 mm (64 border nodes, Poisson-disc
+              interior at 0.1 mm), neo-Hookean, between rigid platens; top
+              platen driven down at 0.1 m/s; penalty Mohr-Coulomb mu = 0.3
+  c2          biaxial compression of polydisperse deformable grains
+              (R ~ U[0.8, 1.2] mm, ~200 MLS nodes each) placed by random
+              sequential deposition, rigid walls, mu = 0.5, density x100
+              (mass scaling expressed as density).  1000 grains cannot fit
+              a 40 x 40 mm box (their area is ~3200 mm^2), so the box side
+              is chosen for a 0.45 area fraction (~84 mm for 1000 grains).
+"""
+from __future__ import annotations
+
+import math
+import numpy as np
+
+from .bodies import BC, deformable_body, disc_nodes, rigid_rectangle
+from .export import Model
+
+E_GRAIN = 1.0e7
+NU_GRAIN = 0.3
+RHO_GRAIN = 1000.0
+
+
+def _streams(seed, n):
+    ss = np.random.SeedSequence(seed)
+    return [np.random.Generator(np.random.PCG64(s)) for s in ss.spawn(n)]
+
+
+def _critical_dt(model, kn):
+    """0.2 x the smaller of the elastic (h / c) and contact (2 sqrt(m / (kn L)))
+    stability limits over the deformable bodies."""
+    dts = []
+    for b in model.bodies:
+        if b.rigid:
+            continue
+        mat = [m for m in model.materials if m[0] == b.material][0]
+        rho, E = mat[2][0], mat[2][3]
+        c = math.sqrt(E / rho)
+        h = b.nodal_distance
+        mmin = float(np.min(b.mass[:, 0]))
+        dts.append(h / c)
+        dts.append(2.0 * math.sqrt(mmin / (kn * h)))
+    return 0.2 * min(dts)
+
+
+def tiny(seed=1000, h=0.35e-3, R=1.0e-3, with_platens=False, nregions=2, overlap=2e-6):
+    """Two small discs pressed together by `overlap` (so contact forces act
+    from the first step), optionally between rigid platens that also overlap."""
+    rngs = _streams(seed, 4)
+    nb = 2 * max(int(round(math.pi * R / h)), 4)
+    bodies = []
+    for i, yc in enumerate([R, 3 * R - overlap]):
+        bxy, ixy = disc_nodes(rngs[i], 0.0, yc, R, nb, h)
+        bodies.append(deformable_body(bxy, ixy, "Grain", h, RHO_GRAIN, detection=0.5 * h,
+                                      contact=0.5 * h, nregions=nregions,
+                                      velocity=(0.0, -0.05 * i)))
+    mats = [("Grain", "NeoHookean", [RHO_GRAIN, 0.0, 0.0, E_GRAIN, NU_GRAIN, 1.0])]
+    laws = [("Grain", "Grain", "MohrCoulomb", "Evolutive", [1e13, 1e13, 0.3, 0.0, 0.0])]
+    if with_platens:
+        mats.append(("Steel", "ElasticLinear", [7800.0, 0.0, 0.0, 2e11, 0.3, 1.0]))
+        laws.append(("Grain", "Steel", "MohrCoulomb", "Evolutive", [1e13, 1e13, 0.3, 0.0, 0.0]))
+        t = 0.3e-3
+        bodies.append(rigid_rectangle(-1.5e-3, -t + overlap, 1.5e-3, overlap, "Steel", h, 7800.0,
+                                      detection=0.5 * h, contact=0.5 * h,
+                                      xbc=BC("Driven", [(0.0, 0.0), (1e3, 0.0)]),
+                                      ybc=BC("Driven", [(0.0, 0.0), (1e3, 0.0)])))
+        top = 4 * R - 2 * overlap
+        bodies.append(rigid_rectangle(-1.5e-3, top, 1.5e-3, top + t, "Steel", h, 7800.0,
+                                      detection=0.5 * h, contact=0.5 * h,
+                                      xbc=BC("Driven", [(0.0, 0.0), (1e3, 0.0)]),
+                                      ybc=BC("Driven", [(0.0, -0.1), (1e3, -0.1)])))
+    m = Model(name="tiny_platen" if with_platens else "tiny", materials=mats, contact_laws=laws,
+              bodies=bodies, periodic=(-1.0, 1.0))
+    m.dt = _critical_dt(m, 1e13)
+    m.contact_update_period = 10 * m.dt
+    m.tend = 1.0
+    return m
+
+
+def c1(seed=1001, h=1.0e-4, R=1.0e-3, nregions=4):
+    """Two elastic discs between rigid platens, FP64 on the CPU."""
+    rngs = _streams(seed, 4)
+    bodies = []
+    for i, yc in enumerate([R, 3 * R]):
+        bxy, ixy = disc_nodes(rngs[i], 0.0, yc, R, 64, 0.8 * h)
+        bodies.append(deformable_body(bxy, ixy, "Grain", h, RHO_GRAIN, detection=0.5 * h,
+                                      contact=0.5 * h, nregions=nregions))
+    t = 0.3e-3
+    bodies.append(rigid_rectangle(-1.5e-3, -t, 1.5e-3, 0.0, "Steel", h, 7800.0, detection=0.5 * h,
+                                  contact=0.5 * h, xbc=BC("Driven", [(0.0, 0.0), (1e3, 0.0)]),
+                                  ybc=BC("Driven", [(0.0, 0.0), (1e3, 0.0)])))
+    bodies.append(rigid_rectangle(-1.5e-3, 4 * R, 1.5e-3, 4 * R + t, "Steel", h, 7800.0,
+                                  detection=0.5 * h, contact=0.5 * h,
+                                  xbc=BC("Driven", [(0.0, 0.0), (1e3, 0.0)]),
+                                  ybc=BC("Driven", [(0.0, -0.1), (1e3, -0.1)])))
+    mats = [("Grain", "NeoHookean", [RHO_GRAIN, 0.0, 0.0, E_GRAIN, NU_GRAIN, 1.0]),
+            ("Steel", "ElasticLinear", [7800.0, 0.0, 0.0, 2e11, 0.3, 1.0])]
+    laws = [("Grain", "Grain", "MohrCoulomb", "Evolutive", [1e13, 1e13, 0.3, 0.0, 0.0]),
+            ("Grain", "Steel", "MohrCoulomb", "Evolutive", [1e13, 1e13, 0.3, 0.0, 0.0])]
+    m = Model(name="c1_two_discs", materials=mats, contact_laws=laws, bodies=bodies, periodic=(-1.0, 1.0))
+    m.dt = _critical_dt(m, 1e13)
+    m.contact_update_period = 10 * m.dt
+    m.tend = 10000 * m.dt
+    return m
+
+
+def _rsa(rng, n, rmin, rmax, W, H, gap, max_tries=20000000):
+    """Random sequential deposition of n discs (radius ~ U[rmin, rmax]) in a
+    W x H box, rejecting overlaps (with a clearance `gap`)."""
+    cell = 2 * rmax + gap
+    nx, ny = int(W / cell) + 1, int(H / cell) + 1
+    grid = [[[] for _ in range(ny)] for _ in range(nx)]
+    out = []
+    tries = 0
+    while len(out) < n:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("random sequential deposition stalled at %d grains" % len(out))
+        r = rmin + (rmax - rmin) * rng.random()
+        x = r + (W - 2 * r) * rng.random()
+        y = r + (H - 2 * r) * rng.random()
+        i, j = int(x / cell), int(y / cell)
+        ok = True
+        for a in range(max(i - 1, 0), min(i + 2, nx)):
+            for b in range(max(j - 1, 0), min(j + 2, ny)):
+                for (xo, yo, ro) in grid[a][b]:
+                    if (x - xo) ** 2 + (y - yo) ** 2 < (r + ro + gap) ** 2:
+                        ok = False
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if ok:
+            out.append((x, y, r))
+            grid[i][j].append((x, y, r))
+    return out
+
+
+def c2(n_grains=1000, seed=1002, nodes_per_grain=200, area_fraction=0.45, mu=0.5,
+       mass_scaling=100.0, confinement=1e5, wall_velocity=0.05, nregions=1):
+    """Biaxial compression of polydisperse deformable grains with rigid walls."""
+    rng_place = np.random.Generator(np.random.PCG64(np.random.SeedSequence(seed).spawn(1)[0]))
+    rmin, rmax = 0.8e-3, 1.2e-3
+    mean_area = math.pi * (rmin * rmin + rmin * rmax + rmax * rmax) / 3.0
+    side = math.sqrt(n_grains * mean_area / area_fraction)
+    grains = _rsa(rng_place, n_grains, rmin, rmax, side, side, gap=0.02e-3)
+    rngs = _streams(seed + 7, n_grains)
+    rho = RHO_GRAIN * mass_scaling
+    bodies = []
+    hmax = 0.0
+    for k, (x, y, r) in enumerate(grains):
+        # spacing so that border + Poisson interior gives ~nodes_per_grain nodes
+        h = r * 0.108 * math.sqrt(200.0 / nodes_per_grain)
+        hmax = max(hmax, h)
+        nb = int(round(2 * math.pi * r / h))
+        bxy, ixy = disc_nodes(rngs[k], x, y, r, nb, h)
+        bodies.append(deformable_body(bxy, ixy, "Grain", h, rho, detection=0.5 * h,
+                                      contact=0.5 * h, nregions=nregions))
+    t = 2.0e-3
+    hw = hmax
+    # bottom and left walls fixed; top wall driven down; right wall under confinement
+    fixed = lambda: BC("Driven", [(0.0, 0.0), (1e3, 0.0)])
+    walls = [
+        rigid_rectangle(-t, -t, side + t, 0.0, "Steel", hw, 7800.0, detection=0.5 * hw, contact=0.5 * hw,
+                        xbc=fixed(), ybc=fixed()),
+        rigid_rectangle(-t, 0.0, 0.0, side, "Steel", hw, 7800.0, detection=0.5 * hw, contact=0.5 * hw,
+                        xbc=fixed(), ybc=fixed()),
+        rigid_rectangle(-t, side, side + t, side + t, "Steel", hw, 7800.0, detection=0.5 * hw,
+                        contact=0.5 * hw, xbc=fixed(),
+                        ybc=BC("Driven", [(0.0, -wall_velocity), (1e3, -wall_velocity)])),
+    ]
+    # a rigid border takes the Neumann value at every border node times its
+    # tributary length (Code/Body.h:854-858): scale so the net push equals
+    # confinement x the wall's face length
+    rw = rigid_rectangle(side, 0.0, side + t, side, "Steel", hw, 7800.0, detection=0.5 * hw,
+                         contact=0.5 * hw, ybc=fixed())
+    perim = 2 * (t + side)
+    p = -confinement * side / perim
+    rw.borders[0].xbc = BC("Oriented", [(0.0, p), (1e3, p)])
+    walls.append(rw)
+    bodies += walls
+    mats = [("Grain", "NeoHookean", [rho, 0.0, 0.0, E_GRAIN, NU_GRAIN, 1.0]),
+            ("Steel", "ElasticLinear", [7800.0 * mass_scaling, 0.0, 0.0, 2e11, 0.3, 1.0])]
+    laws = [("Grain", "Grain", "MohrCoulomb", "Evolutive", [1e13, 1e13, mu, 0.0, 0.0]),
+            ("Grain", "Steel", "MohrCoulomb", "Evolutive", [1e13, 1e13, mu, 0.0, 0.0])]
+    m = Model(name="c2_biaxial_%d" % n_grains, materials=mats, contact_laws=laws, bodies=bodies,
+              periodic=(-1.0, 1.0))
+    m.dt = _critical_dt(m, 1e13)
+    m.contact_update_period = 10 * m.dt
+    m.tend = 5000 * m.dt
+    return m
+
+
+WORKLOADS = {"tiny": tiny, "tiny_platen": lambda: tiny(with_platens=True), "c1": c1, "c2": c2}
