@@ -1,0 +1,126 @@
+/*
+ * melody2d.h - C-ABI of the MI355X (gfx950) implementation of the MELODY 2D
+ * explicit dynamics step.
+ *
+ * The reference (GMollon/MELODY2D) has no plugin or FFI layer: its step sits
+ * behind C++ free functions over vector<Body>& called from Code/main.cpp.
+ * Each entry point below names the reference function or call sequence it
+ * replaces.  No C++ or HIP types cross this boundary: plain pointers, sizes
+ * and status codes.  Device memory is owned by the context; host buffers are
+ * owned by the caller.  A context is not re-entrant; distinct contexts may be
+ * used from different threads.
+ */
+#ifndef MELODY2D_H
+#define MELODY2D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mel_ctx mel_ctx;
+
+typedef enum {
+    MEL_OK = 0,
+    MEL_E_ARG = 1,          /* bad argument (index out of range, buffer too small) */
+    MEL_E_IO = 2,           /* model file missing / unreadable */
+    MEL_E_PARSE = 3,        /* model file content not understood */
+    MEL_E_OOM = 4,          /* device allocation failed */
+    MEL_E_HIP = 5,          /* HIP runtime error */
+    MEL_E_NAN = 6,          /* non-finite kinematic state (Code/main.cpp:227-237 stops the run) */
+    MEL_E_CAPACITY = 7,     /* a fixed-capacity device buffer overflowed */
+    MEL_E_UNSUPPORTED = 8   /* model uses a feature this library does not implement */
+} mel_status;
+
+typedef enum {
+    MEL_FP64 = 0,           /* plain FP64 drop-in */
+    MEL_FP32_MIXED = 1      /* FP32 shape-function products / stresses / laws, FP64 state */
+} mel_precision;
+
+/* node fields, in the order of the dump records used by the tests */
+typedef enum {
+    MEL_X_CURRENT = 0, MEL_Y_CURRENT, MEL_X_DISP, MEL_Y_DISP, MEL_X_VEL, MEL_Y_VEL, MEL_X_ACC, MEL_Y_ACC,
+    MEL_X_DISP_PARAM, MEL_Y_DISP_PARAM, MEL_X_VEL_PARAM, MEL_Y_VEL_PARAM, MEL_X_ACC_PARAM, MEL_Y_ACC_PARAM,
+    MEL_X_INTERNAL_FORCE, MEL_Y_INTERNAL_FORCE, MEL_X_CONTACT_FORCE, MEL_Y_CONTACT_FORCE,
+    MEL_X_BODY_FORCE, MEL_Y_BODY_FORCE, MEL_X_DIRICHLET_FORCE, MEL_Y_DIRICHLET_FORCE,
+    MEL_X_NEUMANN_FORCE, MEL_Y_NEUMANN_FORCE, MEL_X_DAMPING_FORCE, MEL_Y_DAMPING_FORCE,
+    MEL_X_FORCE, MEL_Y_FORCE, MEL_X_MASS, MEL_Y_MASS,
+    MEL_NODE_FIELD_COUNT
+} mel_node_field;
+
+typedef struct {
+    int n_bodies, n_nodes, n_gauss, n_borders, n_border_nodes, n_rigid;
+    int n_contacts, contact_capacity;
+    long long nnz_gauss;    /* Gauss-point support entries */
+    long long nnz_d2d;      /* DOF_TO_DISP entries */
+    double time, dt;
+    int iteration;
+    int precision;
+} mel_info_t;
+
+/* Load_static (Code/IO.h:598) + Load_dynamic (Code/IO.h:1871) on the files
+ * STATIC_CONTROL.asc, STATIC_DATA.asc, DYNAMIC_<save_index>.asc of `dir`,
+ * then the initialisation block of Code/main.cpp:103-126 (borders, BCs,
+ * Update_proximity, Initialize_CZM, Update_material masses, Update_proximity,
+ * Initialize_CZM) on HIP device `hip_device`. */
+mel_status mel_load(const char* dir, int save_index, mel_precision precision, int hip_device, mel_ctx** out);
+
+/* Same as mel_load for a model held in the binary record container written by
+ * melody2d_amd.preprocessor (for models too large for the ASCII files). */
+mel_status mel_load_binary(const char* path, mel_precision precision, int hip_device, mel_ctx** out);
+
+/* Update_proximity (Code/Proximity.h:48-473): neighbour lists, contact
+ * candidates with history transfer, self-contact unless NO_MONITORING. */
+mel_status mel_update_proximity(mel_ctx* ctx);
+
+/* nsteps x [Euler_step (Code/Solver.h:10-88) + the contact-update cadence of
+ * Code/main.cpp:191-196].  Asynchronous with respect to the host except at
+ * contact updates; returns MEL_E_NAN / MEL_E_CAPACITY when detected. */
+mel_status mel_euler_steps(mel_ctx* ctx, int nsteps);
+
+/* wait for queued device work; reports deferred errors */
+mel_status mel_synchronize(mel_ctx* ctx);
+
+/* node field of one body (Node members, Code/Node.h:10-99) */
+mel_status mel_get_node_field(mel_ctx* ctx, int body, int field, double* out, size_t n);
+mel_status mel_set_node_field(mel_ctx* ctx, int body, int field, const double* in, size_t n);
+
+/* contact elements of slave body `body` in the reference's order: 16 ints
+ * (borderS border_nodeS nodeS bodyM borderM shiftM segmentM nodeM0..3
+ * shiftM0..3 nb_internal) and 18 doubles (shapeM0..3 gapn vgapn gapt vgapt
+ * xsi xnorm ynorm xtan ytan effective_mass length fx fy internal0) each,
+ * node ids local to their body (Contact_element, Code/Contact_Element.h:11-44) */
+mel_status mel_get_contacts(mel_ctx* ctx, int body, int32_t* ids, double* state, size_t cap, size_t* count);
+
+/* neighbour triples (body2 border2 shift) of body (Code/Proximity.h:160) */
+mel_status mel_get_neighbours(mel_ctx* ctx, int body, int32_t* out, size_t cap, size_t* count);
+
+/* 26 body-level values (rigid kinematics and resultants; mass, inertia) and
+ * the 7 work accumulators internal contact body dirichlet neumann damping alid
+ * (Code/Body.h:112-118, accumulated as in Code/Body.h:2058-2081) */
+mel_status mel_get_body_state(mel_ctx* ctx, int body, double* rigid26, double* works7);
+
+/* per border-node slot: 0 tributary length (Code/Body.h:614-661),
+ * 1 / 2 x / y contact pressure (Update_contact_pressures, Code/Proximity.h:1014-1069) */
+mel_status mel_get_border_field(mel_ctx* ctx, int body, int border, int field, double* out, size_t n);
+
+mel_status mel_get_info(mel_ctx* ctx, mel_info_t* info);
+
+/* state in the DYNAMIC_nnnnn.asc layout (Write_dynamic, Code/IO.h:2351-2636),
+ * 17 significant digits */
+mel_status mel_write_dynamic(mel_ctx* ctx, const char* path);
+
+/* the HIP stream (hipStream_t) all of the context's kernels run on, for
+ * event timing by the caller */
+mel_status mel_get_stream(mel_ctx* ctx, void** stream);
+
+void mel_destroy(mel_ctx* ctx);
+const char* mel_last_error(const mel_ctx* ctx);
+const char* mel_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MELODY2D_H */

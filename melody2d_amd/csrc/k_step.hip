@@ -1,0 +1,661 @@
+// Kernels of one explicit step (FP64 drop-in).  Each kernel names the
+// reference function it replaces.  Compiled with -ffp-contract=off: the
+// expression and summation orders below follow the reference wherever the
+// result can be made bit-identical (internal forces, Neumann/Dirichlet
+// spreading, damping, kinematics); contact point forces are summed in a
+// fixed element order instead of the reference's body/send order.
+#include "mel_device.hpp"
+#include "k_step.hpp"
+
+namespace mel {
+
+// ---------------------------------------------------------------------------
+// Body::Update_borders (Code/Body.h:614-661): tributary length per slot.
+// ---------------------------------------------------------------------------
+__global__ void k_border_length(DevModel m) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m.n_slots) return;
+    int br = m.sl_border[s];
+    int b = m.br_body[br];
+    if (m.body_rigid_flag[b] || m.body_status[b] == 1) return;
+    int j = m.sl_j[s];
+    int n = m.br_nslots[br];
+    double P = m.period;
+    int a0 = m.sl_n0[s], a1 = m.sl_n1[s], a2 = m.sl_n2[s];
+    double x0 = m.xc[a0] + P * m.sl_s0[s], y0 = m.yc[a0];
+    double x1 = m.xc[a1] + P * m.sl_s1[s], y1 = m.yc[a1];
+    double x2 = m.xc[a2] + P * m.sl_s2[s], y2 = m.yc[a2];
+    double l21 = __dsqrt_rn((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
+    double l10 = __dsqrt_rn((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
+    bool simple = m.br_periodicity[br] == 1;
+    double L;
+    if (j == 0 && simple) L = l21 * 0.5;
+    else if (j == n - 1 && simple) L = l10 * 0.5;
+    else L = (l21 + l10) * 0.5;
+    m.sl_len[s] = L;
+}
+
+// ---------------------------------------------------------------------------
+// Body::Update_contacts (Code/Body.h:1341-1634): one thread per element.
+// ---------------------------------------------------------------------------
+__global__ void k_contact_project(DevModel m) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= *m.c_count) return;
+    int bS = m.C.bodyS[e];
+    if (m.body_status[bS] == 1) return;
+    project_element(m, m.C, e);
+}
+
+// ---------------------------------------------------------------------------
+// Contact laws, Code/Model.h:81-335 (0.5 factors applied by the caller as in
+// Code/Body.h:1703-1789)
+// ---------------------------------------------------------------------------
+__device__ inline void law_mohr_coulomb(double kn, double kt, double fric, double coh, double tens, double gapn,
+                                        double& gapt, double& Pn, double& Pt) {
+    Pn = -kn * gapn;
+    Pt = -kt * gapt;
+    if (Pn < -tens) { Pn = 0.; Pt = 0.; gapt = 0.; }
+    else if (fabs(Pt) > coh + fric * Pn) {
+        if (gapt > 0.) Pt = -coh - fric * Pn;
+        else Pt = coh + fric * Pn;
+        gapt = -Pt / kt;
+    }
+}
+
+__device__ inline void law_czm_linear(double kini, double Pnlim, double gapnlim, double Pnres, double NT, double gapn,
+                                      double& gapt, double& D, double& Pn, double& Pt) {
+    if (D >= 1.) {
+        Pn = -kini * gapn;
+        Pt = -kini * gapt;
+        if (Pn < -Pnres) { Pn = 0.; Pt = 0.; gapt = 0.; }
+        else if (fabs(Pt) > Pnres * NT) {
+            Pt = gapt > 0. ? -Pnres * NT : Pnres * NT;
+            gapt = -Pt / kini;
+        }
+        return;
+    }
+    double gnmax = Pnlim / kini * (1. - D) + gapnlim * D;
+    double gtmax = gnmax * NT;
+    double kmax;
+    bool n_in = gapn < gnmax, n_out = gapn > gnmax;
+    bool t_in = fabs(gapt) < gtmax, t_out = fabs(gapt) > gtmax;
+    if (n_in && t_in) {
+        kmax = (Pnlim * (1. - D) + Pnres * D) / gnmax;
+    } else if (n_out && t_in) {
+        gnmax = gapn; gtmax = gnmax * NT;
+        D = (gnmax - Pnlim / kini) / (gapnlim - Pnlim / kini);
+        kmax = (Pnlim * (1. - D) + Pnres * D) / gnmax;
+    } else if (n_in && t_out) {
+        gtmax = fabs(gapt); gnmax = gtmax / NT;
+        D = (gnmax - Pnlim / kini) / (gapnlim - Pnlim / kini);
+        kmax = (Pnlim * (1. - D) + Pnres * D) / gnmax;
+    } else if (n_out && t_out) {
+        if (gapn / gnmax > fabs(gapt) / gtmax) { gnmax = gapn; gtmax = gnmax * NT; }
+        else { gtmax = fabs(gapt); gnmax = gtmax / NT; }
+        D = (gnmax - Pnlim / kini) / (gapnlim - Pnlim / kini);
+        kmax = (Pnlim * (1. - D) + Pnres * D) / gnmax;
+    } else {
+        return;
+    }
+    if (gapn < 0.) Pn = -kini * gapn;
+    else Pn = -kmax * gapn;
+    Pt = -kmax * gapt;
+}
+
+// Body::Update_contact_forces (Code/Body.h:1642-1941) minus the scatter: the
+// element's force and its point-force records.
+__global__ void k_contact_law(DevModel m) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= *m.c_count) return;
+    const ContactArrays& C = m.C;
+    m.rec_bn[3 * e] = -1; m.rec_bn[3 * e + 1] = -1; m.rec_bn[3 * e + 2] = -1;
+    for (int q = 0; q < 6; q++) m.rig_rec[6 * e + q] = 0.;
+    int bS = C.bodyS[e];
+    if (m.body_status[bS] == 1) return;
+    int bM = C.bodyM[e];
+    int li = m.law_table[m.body_material[bS] * m.n_materials + m.body_material[bM]];
+    if (li < 0) return;
+    int slot = slot_of(m, bS, C.borderS[e], C.bnodeS[e]);
+    double length = m.sl_len[slot];
+    double length0 = m.sl_len0[slot];
+    double gapn = C.gapn[e], gapt = C.gapt[e];
+    const double* p = m.law_p + 12 * li;
+    double Pn = 0., Pt = 0., W = 0.;
+    int lt = m.law_type[li];
+    if (lt == 0) {                      // Frictionless
+        Pn = gapn > 0. ? 0. : -p[0] * gapn;
+        Pt = 0.;
+        gapt = 0.;
+    } else if (lt == 1) {               // Cohesion
+        double gtang = p[2] * 0.5, gtens = p[3] * 0.5;
+        Pn = -p[0] * gapn;
+        Pt = -p[1] * gapt;
+        if (Pn < -gtens) { Pn = 0.; Pt = 0.; gapt = 0.; }
+        else if (fabs(Pt) > gtang) { Pt = gapt > 0. ? -gtang : gtang; gapt = -Pt / p[1]; }
+    } else if (lt == 2) {               // MohrCoulomb
+        law_mohr_coulomb(p[0], p[1], p[2], p[3] * 0.5, p[4] * 0.5, gapn, gapt, Pn, Pt);
+    } else if (lt == 3) {               // DampedMohrCoulomb
+        double kn = p[0], kt = p[1], fric = p[2], coh = p[3] * 0.5, tens = p[4] * 0.5, damp = p[5];
+        double mass = C.emass[e], L0 = length0, vgn = C.vgapn[e], vgt = C.vgapt[e];
+        Pn = 0.;
+        if (gapn < 0.) { Pn = -damp * vgn * 2. * __dsqrt_rn(mass * kn / L0); W += Pn * vgn; }
+        Pn -= kn * gapn;
+        if (Pn < -tens) { Pn = 0.; Pt = 0.; gapt = 0.; }
+        else {
+            double Ptd = -damp * vgt * 2. * __dsqrt_rn(mass * kt / L0);
+            Pt = -kt * gapt + Ptd;
+            if (fabs(Pt) > coh + fric * Pn) {
+                Pt = gapt > 0. ? -coh - fric * Pn : coh + fric * Pn;
+                gapt = -Pt / kt;
+                W += Pt * vgt;
+            } else W += Ptd * vgt;
+        }
+    } else if (lt == 4) {               // TwoSlopesMohrCoulomb
+        double kn = p[0], kt = p[1], fric = p[2], coh = p[3] * 0.5, tens = p[4] * 0.5, rati = p[5];
+        Pn = C.vgapn[e] <= 0. ? -kn * gapn : -kn * gapn * rati;
+        if (Pn < -tens) { Pn = 0.; Pt = 0.; gapt = 0.; }
+        else {
+            Pt = -kt * gapt;
+            if (fabs(Pt) > coh + fric * Pn) { Pt = gapt > 0. ? -coh - fric * Pn : coh + fric * Pn; gapt = -Pt / kt; }
+        }
+    } else if (lt == 6) {               // CZMlinear
+        double D = C.int0[e];
+        if (C.nbint[e] == 0) { D = 1.; C.nbint[e] = 1; }
+        law_czm_linear(p[0], p[1] * 0.5, p[2], p[3] * 0.5, p[4], gapn, gapt, D, Pn, Pt);
+        C.int0[e] = D;
+    }
+    double Px = C.xn[e] * Pn + C.xt[e] * Pt;
+    double Py = C.yn[e] * Pn + C.yt[e] * Pt;
+    double L = m.law_fixed[li] ? length0 : length;
+    double Fx = Px * L, Fy = Py * L;
+    W *= L * m.dt;
+    C.gapn[e] = gapn; C.gapt[e] = gapt; C.len[e] = length; C.fx[e] = Fx; C.fy[e] = Fy;
+    if (W != 0.) {
+        atomicAdd(&m.body_rigid[bS * RIGID_FIELDS + R_WORK + W_CONTACT], 0.5 * W);
+        atomicAdd(&m.body_rigid[bM * RIGID_FIELDS + R_WORK + W_CONTACT], 0.5 * W);
+    }
+    int nodeS = C.nodeS[e];
+    if (!m.body_rigid_flag[bS]) {
+        int bn = m.sl_bn[slot];
+        m.rec_bn[3 * e] = bn; m.rec_fx[3 * e] = Fx; m.rec_fy[3 * e] = Fy;
+        atomicAdd(&m.bk_count[bn], 1);
+    } else {
+        const double* R = m.body_rigid + bS * RIGID_FIELDS;
+        double dx = m.xc[nodeS] - R[R_CUR], dy = m.yc[nodeS] - R[R_CUR + 1];
+        m.rig_rec[6 * e] = Fx; m.rig_rec[6 * e + 1] = Fy; m.rig_rec[6 * e + 2] = -Fx * dy + Fy * dx;
+    }
+    if (!m.body_rigid_flag[bM]) {
+        int sl = slot_of(m, bM, C.borderM[e], C.segM[e]);
+        int bn1 = m.sl_bn[sl], bn2 = m.sl_bn[sl + 1];
+        m.rec_bn[3 * e + 1] = bn1; m.rec_fx[3 * e + 1] = -Fx * C.N1[e]; m.rec_fy[3 * e + 1] = -Fy * C.N1[e];
+        m.rec_bn[3 * e + 2] = bn2; m.rec_fx[3 * e + 2] = -Fx * C.N2[e]; m.rec_fy[3 * e + 2] = -Fy * C.N2[e];
+        atomicAdd(&m.bk_count[bn1], 1);
+        atomicAdd(&m.bk_count[bn2], 1);
+    } else {
+        const double* R = m.body_rigid + bM * RIGID_FIELDS;
+        double dx = m.xc[nodeS] - R[R_CUR] - C.shiftM[e] * m.period;
+        double dy = m.yc[nodeS] - R[R_CUR + 1];
+        m.rig_rec[6 * e + 3] = -Fx; m.rig_rec[6 * e + 4] = -Fy; m.rig_rec[6 * e + 5] = Fx * dy - Fy * dx;
+    }
+}
+
+// fixed-capacity buckets: record ids per border node
+__global__ void k_bucket_fill(DevModel m) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= 3 * *m.c_count) return;
+    int bn = m.rec_bn[r];
+    if (bn < 0) return;
+    int pos = atomicAdd(&m.bk_cursor[bn], 1);
+    if (pos < BUCKET_CAP) m.bk_rec[bn * BUCKET_CAP + pos] = r;
+    else atomicOr(&m.status_flags[0], 1);
+}
+
+// point contact force per border node: records summed in record-id order
+__global__ void k_bucket_sum(DevModel m) {
+    int bn = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bn >= m.n_bnodes) return;
+    int n = min(m.bk_cursor[bn], BUCKET_CAP);
+    int ids[BUCKET_CAP];
+    for (int k = 0; k < n; k++) {
+        int v = m.bk_rec[bn * BUCKET_CAP + k];
+        int q = k;
+        while (q > 0 && ids[q - 1] > v) { ids[q] = ids[q - 1]; q--; }
+        ids[q] = v;
+    }
+    double px = 0., py = 0.;
+    for (int k = 0; k < n; k++) { px += m.rec_fx[ids[k]]; py += m.rec_fy[ids[k]]; }
+    m.bn_pcx[bn] = px;
+    m.bn_pcy[bn] = py;
+    m.bk_cursor[bn] = 0;
+    m.bk_count[bn] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Border::Update_bc (Code/Border.h:120-221): current BC values at time t
+// ---------------------------------------------------------------------------
+__device__ void bc_value(const DevModel& m, int type, int h0, int h1, double t, double& value, double& vel) {
+    if (type == 1) return;
+    if (type == 2) {
+        value = 0.;
+        for (int n = h0; n + 1 < h1; n++) {
+            double t0 = m.hist_t[n], p0 = m.hist_v[n], t1 = m.hist_t[n + 1], p1 = m.hist_v[n + 1];
+            if (t >= t1) value = value + (t1 - t0) * (p1 + p0) * 0.5;
+            else {
+                value = value + (t - t0) * (p0 + (p1 - p0) / (t1 - t0) * ((t - t0) * 0.5));
+                vel = p0 + (p1 - p0) / (t1 - t0) * (t - t0);
+                break;
+            }
+        }
+    } else if (type == 3 || type == 4) {
+        for (int n = h0; n + 1 < h1; n++) {
+            double t0 = m.hist_t[n], p0 = m.hist_v[n], t1 = m.hist_t[n + 1], p1 = m.hist_v[n + 1];
+            if ((t0 <= t) & (t <= t1)) { value = p0 + (p1 - p0) / (t1 - t0) * (t - t0); break; }
+        }
+    } else {
+        value = 0.;
+    }
+}
+
+__global__ void k_bc_eval(DevModel m, int plus_dt) {
+    int br = blockIdx.x * blockDim.x + threadIdx.x;
+    if (br >= m.n_borders) return;
+    double t = *m.time;
+    if (plus_dt) t = t + m.dt;
+    const int* hp = m.br_hist_ptr + 2 * br;
+    bc_value(m, m.br_xtype[br], hp[0], hp[1], t, m.br_xval[br], m.br_xvel[br]);
+    bc_value(m, m.br_ytype[br], hp[1], hp[2], t, m.br_yval[br], m.br_yvel[br]);
+}
+
+// Body::Update_bc_forces deformable branch (Code/Body.h:672-815): point
+// Neumann / Dirichlet values per border node (one slot per border node in
+// practice; several slots are summed in slot order).
+__global__ void k_bc_point_forces(DevModel m) {
+    int bn = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bn >= m.n_bnodes) return;
+    double cnx = 0., cny = 0., cdx = 0., cdy = 0.;
+    int node = m.bn_node[bn];
+    int b = m.node_body[node];
+    if (!m.body_rigid_flag[b] && m.body_status[b] != 1) {
+        for (int q = m.bn_slot_ptr[bn]; q < m.bn_slot_ptr[bn + 1]; q++) {
+            int s = m.bn_slots[q];
+            int br = m.sl_border[s];
+            int xt = m.br_xtype[br], yt = m.br_ytype[br];
+            if (xt == 0 && yt == 0) continue;
+            if (m.sl_j[s] >= m.br_nslots[br] - 1) continue;
+            double L = m.sl_len[s];
+            double px = 0., py = 0.;
+            double tx = 0, ty = 0, tpx = 0, tpy = 0;
+            if (xt == 4 || yt == 4) {
+                int a = m.sl_n0[s], c = m.sl_n2[s];
+                double x0 = m.X[a], y0 = m.Y[a], x2 = m.X[c], y2 = m.Y[c];
+                double xp0 = m.xc[a], yp0 = m.yc[a], xp2 = m.xc[c], yp2 = m.yc[c];
+                double invl = inv_sqrt((x2 - x0) * (x2 - x0) + (y2 - y0) * (y2 - y0));
+                double invlp = inv_sqrt((xp2 - xp0) * (xp2 - xp0) + (yp2 - yp0) * (yp2 - yp0));
+                tx = (x2 - x0) * invl; ty = (y2 - y0) * invl;
+                tpx = (xp2 - xp0) * invlp; tpy = (yp2 - yp0) * invlp;
+            }
+            if (xt == 3) px = m.br_xval[br];
+            else if (xt == 4) { px += m.br_xval[br] * (tx * tpx + ty * tpy); py += m.br_xval[br] * (tx * tpy - ty * tpx); }
+            else if (xt == 2) px = m.br_xparam[2 * br] * (m.br_xval[br] - m.ux[node]) - m.br_xparam[2 * br + 1] * (m.br_xvel[br] - m.vx[node]);
+            if (yt == 3) py = m.br_yval[br];
+            else if (yt == 4) { px += m.br_yval[br] * (ty * tpx - tx * tpy); py += m.br_yval[br] * (ty * tpy + tx * tpx); }
+            else if (yt == 2) py = m.br_yparam[2 * br] * (m.br_yval[br] - m.uy[node]) - m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node]);
+            double a_nx = 0., a_ny = 0., a_dx = 0., a_dy = 0.;
+            if (xt == 3) a_nx = px * L;
+            else if (xt == 4) { a_nx = px * L; a_ny = py * L; }
+            else if (xt == 2) a_dx = px * L;
+            if (yt == 3) a_ny = py * L;
+            else if (yt == 4) { a_nx = px * L; a_ny = py * L; }
+            else if (yt == 2) a_dy = py * L;
+            cnx += a_nx; cny += a_ny; cdx += a_dx; cdy += a_dy;
+        }
+    }
+    m.bn_pnx[bn] = cnx; m.bn_pny[bn] = cny; m.bn_pdx[bn] = cdx; m.bn_pdy[bn] = cdy;
+}
+
+// ---------------------------------------------------------------------------
+// Body::Update_internal_forces, per Gauss point part (Code/Body.h:969-1048,
+// Code/Model.h:19-73): stress terms with c0 kept separate so that the node
+// gather reproduces c0 * (phix * A + phiy * B) exactly.
+// ---------------------------------------------------------------------------
+__global__ void k_gp_stress(DevModel m) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= m.n_gauss) return;
+    int b = m.g_body[g];
+    int mat = m.body_material[b];
+    double Mu = m.mat_p[6 * mat + 3], K2 = m.mat_p[6 * mat + 4];
+    int k0 = m.g_ptr[g], k1 = m.g_ptr[g + 1];
+    double A, B, Cc, D;
+    if (m.mat_type[mat] == 0) {
+        double E11 = 0., E12 = 0., E22 = 0.;
+        for (int k = k0; k < k1; k++) {
+            int j = m.g_idx[k];
+            double fx = m.g_phix[k], fy = m.g_phiy[k];
+            double ux = m.upx[j], uy = m.upy[j];
+            E11 += fx * ux;
+            E12 += fx * uy + fy * ux;
+            E22 += fy * uy;
+        }
+        E12 = 0.5 * E12;
+        double J = 1. + E11 + E22;
+        double S33 = K2 * (J - 1.);
+        double M2 = 2 * Mu;
+        A = S33 + M2 * E11;         // S11
+        B = M2 * E12;               // S12
+        Cc = B;                     // S12
+        D = S33 + M2 * E22;         // S22
+    } else {
+        double F11 = 1., F12 = 0., F21 = 0., F22 = 1.;
+        for (int k = k0; k < k1; k++) {
+            int j = m.g_idx[k];
+            double fx = m.g_phix[k], fy = m.g_phiy[k];
+            double ux = m.upx[j], uy = m.upy[j];
+            F11 += fx * ux;
+            F12 += fy * ux;
+            F21 += fx * uy;
+            F22 += fy * uy;
+        }
+        double F11F12 = F11 * F11 + F12 * F12;
+        double F21F22 = F21 * F21 + F22 * F22;
+        double J = F11 * F22 - F12 * F21;
+        double invJ = 1. / J;
+        double K = K2 * (J - 1.);
+        double MuJ53 = Mu * pow(J, -1.666666666667);
+        double trB3 = 0.333333333333 * (F11F12 + F21F22 + 1.);
+        double s11 = K + MuJ53 * (F11F12 - trB3);
+        double s22 = K + MuJ53 * (F21F22 - trB3);
+        double s12 = MuJ53 * (F11 * F21 + F12 * F22);
+        double S11 = (F12 * (F12 * s22 - F22 * s12) - F22 * (F12 * s12 - F22 * s11)) * invJ;
+        double S22 = (F11 * (F11 * s22 - F21 * s12) - F21 * (F11 * s12 - F21 * s11)) * invJ;
+        double S12 = (F21 * (F12 * s12 - F22 * s11) - F11 * (F12 * s22 - F22 * s12)) * invJ;
+        A = F11 * S11 + F12 * S12;
+        B = F12 * S22 + F11 * S12;
+        Cc = F21 * S11 + F22 * S12;
+        D = F22 * S22 + F21 * S12;
+    }
+    double4 out = make_double4(A, B, Cc, D);
+    reinterpret_cast<double4*>(m.g_P)[g] = out;
+}
+
+// ---------------------------------------------------------------------------
+// Node-centric gather + Sum_up_forces + Apply_Newton + Apply_Euler
+// (Code/Body.h:1975-2068, Code/Node.h:212-245).  Internal forces are summed
+// per region in Gauss-point order, then across regions, as the per-region
+// accumulators of Code/Body.h:996-997 / Code/Node.h:216-222 do.
+// ---------------------------------------------------------------------------
+__global__ void k_node_update(DevModel m) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m.n_nodes) return;
+    int b = m.node_body[j];
+    if (m.body_rigid_flag[b] || m.body_status[b] == 1) return;
+    int gp0 = m.gp_off[b];
+    int per = m.body_per_region[b];
+    double tx = 0., ty = 0., rx = 0., ry = 0.;
+    int cur = -1;
+    for (int e = m.t_ptr[j]; e < m.t_ptr[j + 1]; e++) {
+        int g = m.t_gp[e];
+        int r = (g - gp0) / per;
+        if (r != cur) {
+            if (cur >= 0) { tx += rx; ty += ry; }
+            rx = 0.; ry = 0.; cur = r;
+        }
+        double4 P = reinterpret_cast<const double4*>(m.g_P)[g];
+        double c0 = m.g_c0[g];
+        double fx = m.t_phix[e], fy = m.t_phiy[e];
+        rx -= c0 * (fx * P.x + fy * P.y);
+        ry -= c0 * (fx * P.z + fy * P.w);
+    }
+    if (cur >= 0) { tx += rx; ty += ry; }
+    // point forces (contact, Dirichlet, Neumann) through DOF_TO_DISP rows
+    double cx = 0., cy = 0., dx = 0., dy = 0., nx = 0., ny = 0.;
+    for (int e = m.bt_ptr[j]; e < m.bt_ptr[j + 1]; e++) {
+        int bn = m.bt_bn[e];
+        double f = m.bt_phi[e];
+        cx += m.bn_pcx[bn] * f; cy += m.bn_pcy[bn] * f;
+        dx += m.bn_pdx[bn] * f; dy += m.bn_pdy[bn] * f;
+        nx += m.bn_pnx[bn] * f; ny += m.bn_pny[bn] * f;
+    }
+    double gx = m.gravity_x, gy = m.gravity_y;
+    if (m.body_status[b] == 2) { gx = 0.; gy = 0.; }
+    double bx = m.mx[j] * gx, by = m.my[j] * gy;
+    double kx = m.fkx[j], ky = m.fky[j];
+    double fx = tx + cx + 0. + bx + dx + nx + kx + 0.;
+    double fy = ty + cy + 0. + by + dy + ny + ky + 0.;
+    double apx = fx * m.imx[j], apy = fy * m.imy[j];
+    double dt = m.dt;
+    double vpx = m.vpx[j], vpy = m.vpy[j];
+    double hx = vpx + 0.5 * dt * apx, hy = vpy + 0.5 * dt * apy;
+    double* w = m.node_work + 7 * (size_t)j;
+    w[W_INTERNAL] += dt * (hx * tx + hy * ty);
+    w[W_BODY] += dt * (hx * bx + hy * by);
+    w[W_DIRICHLET] += dt * (hx * dx + hy * dy);
+    w[W_NEUMANN] += dt * (hx * nx + hy * ny);
+    w[W_DAMPING] += dt * (hx * kx + hy * ky);
+    vpx = vpx + dt * apx;
+    vpy = vpy + dt * apy;
+    m.upx[j] = m.upx[j] + dt * vpx;
+    m.upy[j] = m.upy[j] + dt * vpy;
+    m.vpx[j] = vpx; m.vpy[j] = vpy;
+    m.apx[j] = apx; m.apy[j] = apy;
+    m.fix[j] = tx; m.fiy[j] = ty;
+    m.fcx[j] = cx; m.fcy[j] = cy;
+    m.fdx[j] = dx; m.fdy[j] = dy;
+    m.fnx[j] = nx; m.fny[j] = ny;
+    m.fbx[j] = bx; m.fby[j] = by;
+    m.fx[j] = fx; m.fy[j] = fy;
+}
+
+// Body::Update_damping_forces (Code/Body.h:1057-1154): node gather over the
+// COO entries in entry order, reading velocity parameters.
+__global__ void k_damping(DevModel m) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m.n_nodes) return;
+    double fx = 0., fy = 0.;
+    for (int q = m.mt_ptr[j]; q < m.mt_ptr[j + 1]; q++) {
+        int code = m.mt_entry[q];
+        int i = code >> 1, side = code & 1;
+        int dof0 = m.m_dof[2 * i], dof1 = m.m_dof[2 * i + 1];
+        double d = m.m_damp[i];
+        int n0 = dof0 / 2, r0 = dof0 - 2 * n0, n1 = dof1 / 2, r1 = dof1 - 2 * n1;
+        if (side == 0) {          // contribution to node0's direction r0 from node1's r1 velocity
+            double v = r1 == 0 ? m.vpx[n1] : m.vpy[n1];
+            if (r0 == 0) fx -= v * d; else fy -= v * d;
+        } else {                  // contribution to node1's direction r1 from node0's r0 velocity
+            double v = r0 == 0 ? m.vpx[n0] : m.vpy[n0];
+            if (r1 == 0) fx -= v * d; else fy -= v * d;
+        }
+    }
+    m.fkx[j] = fx;
+    m.fky[j] = fy;
+}
+
+// Body::Update_kinematics + Update_current_positions, deformable
+// (Code/Body.h:2176-2212, Code/Node.h:233-237)
+__global__ void k_kinematics(DevModel m) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.n_nodes) return;
+    int b = m.node_body[i];
+    if (m.body_rigid_flag[b] || m.body_status[b] == 1) return;
+    double dx = 0., dy = 0., vx = 0., vy = 0., ax = 0., ay = 0.;
+    for (int k = m.d_ptr[i]; k < m.d_ptr[i + 1]; k++) {
+        int j = m.d_idx[k];
+        double f = m.d_phi[k];
+        dx += f * m.upx[j]; dy += f * m.upy[j];
+        vx += f * m.vpx[j]; vy += f * m.vpy[j];
+        ax += f * m.apx[j]; ay += f * m.apy[j];
+    }
+    m.ux[i] = dx; m.uy[i] = dy; m.vx[i] = vx; m.vy[i] = vy; m.ax[i] = ax; m.ay[i] = ay;
+    m.xc[i] = m.X[i] + dx;
+    m.yc[i] = m.Y[i] + dy;
+}
+
+// ---------------------------------------------------------------------------
+// rigid bodies: one thread per body for the serial parts
+// ---------------------------------------------------------------------------
+// Body::Update_bc_forces rigid branch (Code/Body.h:816-899) + Update_body_forces
+// + the contact resultant (slave records and the serial sends of
+// Code/Body.h:1948-1967), reduced in element order.
+__global__ void k_rigid_forces(DevModel m) {
+    int b = m.rigid_ids[blockIdx.x];
+    if (m.body_status[b] == 1) return;
+    double* R = m.body_rigid + b * RIGID_FIELDS;
+    __shared__ double red[256][3];
+    int tid = threadIdx.x;
+    double sx = 0., sy = 0., sm = 0.;
+    int nc = *m.c_count;
+    for (int e = tid; e < nc; e += blockDim.x) {
+        if (m.C.bodyS[e] == b) { sx += m.rig_rec[6 * e]; sy += m.rig_rec[6 * e + 1]; sm += m.rig_rec[6 * e + 2]; }
+        if (m.C.bodyM[e] == b) { sx += m.rig_rec[6 * e + 3]; sy += m.rig_rec[6 * e + 4]; sm += m.rig_rec[6 * e + 5]; }
+    }
+    red[tid][0] = sx; red[tid][1] = sy; red[tid][2] = sm;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (tid < w) { red[tid][0] += red[tid + w][0]; red[tid][1] += red[tid + w][1]; red[tid][2] += red[tid + w][2]; }
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    R[R_FCON] = red[0][0]; R[R_FCON + 1] = red[0][1]; R[R_FCON + 2] = red[0][2];
+    // boundary conditions
+    double fn[3] = {0., 0., 0.}, fd[3] = {0., 0., 0.};
+    int gb0 = m.border_off[b];
+    for (int k = 0; k < m.body_nborders[b]; k++) {
+        int br = gb0 + k;
+        int xt = m.br_xtype[br], yt = m.br_ytype[br];
+        if (xt == 0 && yt == 0) continue;
+        for (int j = 0; j < m.br_nslots[br]; j++) {
+            int s = m.br_slot0[br] + j;
+            int node = m.sl_node[s];
+            double L = m.sl_len[s];
+            double fxn = 0., fyn = 0., fxd = 0., fyd = 0.;
+            if (xt == 3 || xt == 4) fxn = m.br_xval[br] * L;
+            else if (xt == 2) fxd = (m.br_xparam[2 * br] * (m.br_xval[br] - m.ux[node]) + m.br_xparam[2 * br + 1] * (m.br_xvel[br] - m.vx[node])) * L;
+            if (yt == 3 || yt == 4) fyn = m.br_yval[br] * L;
+            else if (yt == 2) fyd = (m.br_yparam[2 * br] * (m.br_yval[br] - m.uy[node]) + m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node])) * L;
+            double dx = m.xc[node] - R[R_CUR], dy = m.yc[node] - R[R_CUR + 1];
+            fn[0] += fxn; fn[1] += fyn; fn[2] += -fxn * dy + fyn * dx;
+            fd[0] += fxd; fd[1] += fyd; fd[2] += -fxd * dy + fyd * dx;
+        }
+    }
+    for (int q = 0; q < 3; q++) { R[R_FNEU + q] = fn[q]; R[R_FDIR + q] = fd[q]; }
+    double gx = m.gravity_x, gy = m.gravity_y;
+    if (m.body_status[b] == 2) { gx = 0.; gy = 0.; }
+    R[R_FBODY] = m.body_mass[b] * gx;
+    R[R_FBODY + 1] = m.body_mass[b] * gy;
+}
+
+// driven DOFs of Body::Update_bc (Code/Body.h:340-393) at time t
+__device__ void driven(const DevModel& m, int h0, int h1, double t, double* d) {
+    d[0] = 1.;
+    d[1] = 0.;
+    for (int n = h0; n + 1 < h1; n++) {
+        double t0 = m.hist_t[n], p0 = m.hist_v[n], t1 = m.hist_t[n + 1], p1 = m.hist_v[n + 1];
+        if (t >= t1) d[1] += (t1 - t0) * (p1 + p0) * 0.5;
+        else {
+            d[1] += (t - t0) * (p0 + (p1 - p0) / (t1 - t0) * ((t - t0) * 0.5));
+            d[2] = (t - t0) * (p1 - p0) / (t1 - t0);
+            d[3] = (p1 - p0) / (t1 - t0);
+            break;
+        }
+    }
+}
+
+__device__ void rigid_driven_dofs(const DevModel& m, int b, double t, double* dv) {
+    int gb0 = m.border_off[b];
+    for (int k = 0; k < m.body_nborders[b]; k++) {
+        int br = gb0 + k;
+        const int* hp = m.br_hist_ptr + 2 * br;
+        if (m.br_xtype[br] == 1) driven(m, hp[0], hp[1], t, dv);
+        if (m.br_ytype[br] == 1) driven(m, hp[1], hp[2], t, dv + 4);
+    }
+}
+
+// Sum_up_forces, Apply_Newton, Update_bc(t+dt), Apply_Euler for a rigid body
+// (Code/Body.h:1984-2014, 2032-2037, 2071-2117)
+__global__ void k_rigid_step(DevModel m) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.n_bodies) return;
+    if (!m.body_rigid_flag[b] || m.body_status[b] == 1) return;
+    double* R = m.body_rigid + b * RIGID_FIELDS;
+    double dt = m.dt;
+    for (int q = 0; q < 3; q++)
+        R[R_FTOT + q] = R[R_FCON + q] + R[R_FBODY + q] + R[R_FDIR + q] + R[R_FNEU + q] + R[R_FDAMP + q];
+    R[R_ACC] = R[R_FTOT] * m.body_inv_mass[b];
+    R[R_ACC + 1] = R[R_FTOT + 1] * m.body_inv_mass[b];
+    R[R_ACC + 2] = R[R_FTOT + 2] * m.body_inv_inertia[b];
+    double nacc_x = R[R_ACC], nacc_y = R[R_ACC + 1];
+    R[R_NODE_ACC] = nacc_x; R[R_NODE_ACC + 1] = nacc_y;   // node copies of Body::Apply_Newton (Code/Body.h:2038-2042)
+    R[R_FDIR_NODE] = R[R_FDIR]; R[R_FDIR_NODE + 1] = R[R_FDIR + 1];   // copied to nodes by Code/Body.h:2005-2006
+    double* dv = R + R_DRIVEN;
+    rigid_driven_dofs(m, b, *m.time + dt, dv);
+    double hx = R[R_VEL] + 0.5 * dt * R[R_ACC];
+    double hy = R[R_VEL + 1] + 0.5 * dt * R[R_ACC + 1];
+    double hr = R[R_VEL + 2] + 0.5 * dt * R[R_ACC + 2];
+    R[R_WORK + W_BODY] += dt * (hx * R[R_FBODY] + hy * R[R_FBODY + 1] + hr * R[R_FBODY + 2]);
+    R[R_WORK + W_DIRICHLET] += dt * (hx * R[R_FDIR] + hy * R[R_FDIR + 1] + hr * R[R_FDIR + 2]);
+    R[R_WORK + W_NEUMANN] += dt * (hx * R[R_FNEU] + hy * R[R_FNEU + 1] + hr * R[R_FNEU + 2]);
+    R[R_WORK + W_DAMPING] += dt * (hx * R[R_FDAMP] + hy * R[R_FDAMP + 1] + hr * R[R_FDAMP + 2]);
+    double mass = m.body_mass[b];
+    if (dv[0] == 0.) {
+        R[R_VEL] += dt * R[R_ACC];
+        R[R_DISP] = R[R_DISP] + dt * R[R_VEL];
+    } else if (dv[0] == 1.) {
+        R[R_ACC] = dv[3]; R[R_VEL] = dv[2]; R[R_DISP] = dv[1];
+        R[R_FDIR] = mass * R[R_ACC] - R[R_FTOT];
+    }
+    if (dv[4] == 0.) {
+        R[R_VEL + 1] += dt * R[R_ACC + 1];
+        R[R_DISP + 1] = R[R_DISP + 1] + dt * R[R_VEL + 1];
+    } else if (dv[4] == 1.) {
+        R[R_ACC + 1] = dv[7]; R[R_VEL + 1] = dv[6]; R[R_DISP + 1] = dv[5];
+        R[R_FDIR + 1] = mass * R[R_ACC + 1] - R[R_FTOT + 1];
+    }
+    if (dv[0] == 0. && dv[4] == 0. && m.body_periodicity[b] != 2) {
+        R[R_VEL + 2] += dt * R[R_ACC + 2];
+        R[R_DISP + 2] = R[R_DISP + 2] + dt * R[R_VEL + 2];
+    } else {
+        R[R_VEL + 2] = 0.;
+        R[R_DISP + 2] = 0.;
+    }
+    R[R_CUR] = R[R_XC] + R[R_DISP];
+    R[R_CUR + 1] = R[R_YC] + R[R_DISP + 1];
+    R[R_CUR + 2] = 0. + R[R_DISP + 2];
+}
+
+// rigid node kinematics and positions (Code/Body.h:2214-2226, 2281-2301);
+// node force fields mirror the body totals as Code/Body.h:1999-2013 does.
+__global__ void k_rigid_nodes(DevModel m, int with_forces) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.n_nodes) return;
+    int b = m.node_body[i];
+    if (!m.body_rigid_flag[b] || m.body_status[b] == 1) return;
+    const double* R = m.body_rigid + b * RIGID_FIELDS;
+    double dr = R[R_DISP + 2] - 0.;
+    double dx = m.X[i] - R[R_XC], dy = m.Y[i] - R[R_YC];
+    double c = cos(dr), s = sin(dr);
+    m.ux[i] = R[R_DISP] + dx * c - dy * s - dx;
+    m.uy[i] = R[R_DISP + 1] + dx * s + dy * c - dy;
+    double xx = R[R_CUR] + dx * c - dy * s;
+    double yy = R[R_CUR + 1] + dx * s + dy * c;
+    m.xc[i] = xx;
+    m.yc[i] = yy;
+    double ex = xx - R[R_CUR], ey = yy - R[R_CUR + 1];
+    m.vx[i] = -ey * R[R_VEL + 2] + R[R_VEL];
+    m.vy[i] = ex * R[R_VEL + 2] + R[R_VEL + 1];
+    if (with_forces) {
+        m.ax[i] = R[R_NODE_ACC]; m.ay[i] = R[R_NODE_ACC + 1];
+        m.fcx[i] = R[R_FCON]; m.fcy[i] = R[R_FCON + 1];
+        m.fbx[i] = R[R_FBODY]; m.fby[i] = R[R_FBODY + 1];
+        m.fnx[i] = R[R_FNEU]; m.fny[i] = R[R_FNEU + 1];
+        m.fkx[i] = R[R_FDAMP]; m.fky[i] = R[R_FDAMP + 1];
+        m.fx[i] = R[R_FTOT]; m.fy[i] = R[R_FTOT + 1];
+        // Body::Sum_up_forces copies the Dirichlet resultant before Apply_Euler
+        // overwrites it for driven directions
+        m.fdx[i] = R[R_FDIR_NODE]; m.fdy[i] = R[R_FDIR_NODE + 1];
+    }
+}
+
+__global__ void k_advance_time(DevModel m) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *m.time = *m.time + m.dt;
+}
+
+}  // namespace mel

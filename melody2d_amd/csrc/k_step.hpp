@@ -1,0 +1,40 @@
+// Kernel declarations shared by the host orchestration (mel_context.cpp).
+#pragma once
+#include "mel_device.hpp"
+
+namespace mel {
+
+constexpr int BUCKET_CAP = 32;     // point-force records per border node per step
+constexpr int BLOCK = 256;
+
+__global__ void k_border_length(DevModel m);
+__global__ void k_contact_project(DevModel m);
+__global__ void k_contact_law(DevModel m);
+__global__ void k_bucket_fill(DevModel m);
+__global__ void k_bucket_sum(DevModel m);
+__global__ void k_bc_eval(DevModel m, int plus_dt);
+__global__ void k_bc_point_forces(DevModel m);
+__global__ void k_gp_stress(DevModel m);
+__global__ void k_node_update(DevModel m);
+__global__ void k_damping(DevModel m);
+__global__ void k_kinematics(DevModel m);
+__global__ void k_rigid_forces(DevModel m);
+__global__ void k_rigid_step(DevModel m);
+__global__ void k_rigid_nodes(DevModel m, int with_forces);
+__global__ void k_advance_time(DevModel m);
+
+// proximity (k_proximity.hip)
+__global__ void k_prox_aabb(DevModel m);
+__global__ void k_prox_neighbours(DevModel m);
+__global__ void k_prox_rows(DevModel m);
+__global__ void k_prox_count(DevModel m, int n_rows);
+__global__ void k_prox_write(DevModel m, int n_rows);
+__global__ void k_prox_clear_old(DevModel m);
+__global__ void k_prox_old_index(DevModel m);
+__global__ void k_prox_history(DevModel m);
+__global__ void k_prox_chain(DevModel m);
+__global__ void k_prox_finish(DevModel m, int n_rows);
+__global__ void k_init_czm(DevModel m);
+__global__ void k_project_all(DevModel m);
+
+}  // namespace mel

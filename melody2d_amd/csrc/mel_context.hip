@@ -1,0 +1,1061 @@
+// Host orchestration and C-ABI: builds the flattened device model from a
+// parsed mel::Model, runs the initialisation block of Code/main.cpp:103-126
+// and the Euler_step sequence of Code/Solver.h:10-88 as kernel launches on
+// one HIP stream.
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/melody2d.h"
+#include "k_step.hpp"
+#include "mel_device.hpp"
+#include "model_io.hpp"
+#include "scan.hpp"
+
+using namespace mel;
+
+namespace {
+
+struct HipError {
+    std::string msg;
+};
+
+#define HIP_OK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) throw HipError{std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+struct CapacityError {
+    std::string msg;
+};
+
+inline int blocks(long long n, int b = BLOCK) { return (int)std::max<long long>(1, (n + b - 1) / b); }
+
+}  // namespace
+
+struct mel_ctx {
+    Model model;
+    DevModel d{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int precision = MEL_FP64;
+    std::vector<void*> allocs;
+    std::string err;
+    // host mirrors
+    std::vector<int> h_node_off, h_border_off, h_gp_off, h_rigid_ids;
+    std::vector<int> h_br_slot0, h_br_nslots;
+    long long nnz_g = 0, nnz_d = 0;
+    int n_contacts = 0;          // host mirror of *d.c_count after the last proximity update
+    int row_cap = 0;
+    int* d_row_count = nullptr;
+    int* d_row_off = nullptr;
+    int* d_scan_tmp = nullptr;
+    int scan_tmp_cap = 0;
+    int* d_rigid_ids = nullptr;
+    double time = 0.0;
+    double next_contact_update = 0.0;
+    int iteration = 0;
+    ContactArrays bufA{}, bufB{};
+
+    template <typename T>
+    T* alloc(size_t n) {
+        void* p = nullptr;
+        size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        if (hipMalloc(&p, bytes) != hipSuccess) throw HipError{"hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+        allocs.push_back(p);
+        HIP_OK(hipMemsetAsync(p, 0, bytes, stream));
+        return (T*)p;
+    }
+    template <typename T>
+    T* upload(const std::vector<T>& v) {
+        T* p = alloc<T>(v.size());
+        if (!v.empty()) HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+        return p;
+    }
+    template <typename T>
+    void download(std::vector<T>& v, const T* p, size_t n) {
+        v.resize(n);
+        if (n) HIP_OK(hipMemcpyAsync(v.data(), p, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+    ~mel_ctx() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void alloc_contacts(mel_ctx& c, ContactArrays& A, int cap) {
+    int** ints[] = {&A.bodyS, &A.borderS, &A.bnodeS, &A.nodeS, &A.bodyM, &A.borderM, &A.shiftM, &A.segM, &A.nbint,
+                    &A.fresh, &A.nM0, &A.nM1, &A.nM2, &A.nM3, &A.sM0, &A.sM1, &A.sM2, &A.sM3};
+    for (int** p : ints) *p = c.alloc<int>(cap);
+    double** dbl[] = {&A.N0, &A.N1, &A.N2, &A.N3, &A.gapn, &A.vgapn, &A.gapt, &A.vgapt, &A.xsi, &A.xn, &A.yn,
+                      &A.xt, &A.yt, &A.emass, &A.len, &A.fx, &A.fy, &A.int0};
+    for (double** p : dbl) *p = c.alloc<double>(cap);
+}
+
+// Body::Update_material masses (Code/Body.h:2751-2783) and rigid mass and
+// inertia (Code/Body.h:2877-2905), in the reference's summation order.
+void compute_masses(Model& m, std::vector<double>& body_mass, std::vector<double>& body_inv_mass,
+                    std::vector<double>& body_inertia, std::vector<double>& body_inv_inertia) {
+    size_t NB = m.bodies.size();
+    body_mass.assign(NB, 0.); body_inv_mass.assign(NB, 0.); body_inertia.assign(NB, 0.); body_inv_inertia.assign(NB, 0.);
+    for (size_t bi = 0; bi < NB; bi++) {
+        Body& b = m.bodies[bi];
+        double rho = m.materials[b.material].p[0];
+        if (!b.rigid) {
+            if (m.flags[F_UPDATE_MASS]) {
+                std::vector<double> mx(b.n_nodes, 0.), my(b.n_nodes, 0.);
+                for (int g = 0; g < b.n_gauss; g++)
+                    for (int j = b.gptr[g]; j < b.gptr[g + 1]; j++)
+                        for (int k = b.gptr[g]; k < b.gptr[g + 1]; k++) {
+                            double dm = b.gw[g] * b.gjac[g] * b.gphi[j] * b.gphi[k];
+                            mx[b.gidx[j]] += dm;
+                            my[b.gidx[j]] += dm;
+                        }
+                double mass = 0., inv = 0.;
+                for (int i = 0; i < b.n_nodes; i++) {
+                    mx[i] *= rho;
+                    my[i] *= rho;
+                    b.imx[i] = 1. / mx[i];
+                    b.imy[i] = 1. / my[i];
+                    mass += std::sqrt(mx[i] * mx[i] + my[i] * my[i]);
+                    inv = 1 / mass;
+                }
+                b.mx = mx; b.my = my;
+                body_mass[bi] = mass; body_inv_mass[bi] = inv;
+            }
+            // Rayleigh damping rebuilt from the stored stiffness (Code/Body.h:2862-2875)
+            if (m.flags[F_UPDATE_DAMPING]) {
+                double alpha = m.materials[b.material].p[1], beta = m.materials[b.material].p[2];
+                for (size_t i = 0; i < b.mat_damping.size(); i++) {
+                    if (b.mat_dof[2 * i] == b.mat_dof[2 * i + 1])
+                        b.mat_damping[i] = alpha * b.mat_stiffness[i] + beta * b.mx[b.mat_dof[2 * i] / 2];
+                    else
+                        b.mat_damping[i] = alpha * b.mat_stiffness[i];
+                }
+            }
+        } else {
+            double mass = 0., inertia = 0.;
+            auto surf = [](double a0, double b0, double a1, double b1, double a2, double b2) {
+                return 0.5 * std::abs((a1 - a0) * (b2 - b0) - (a2 - a0) * (b1 - b0));
+            };
+            for (size_t c = 0; c < b.cells.size() / 3; c++) {
+                int n0 = b.cells[3 * c], n1 = b.cells[3 * c + 1], n2 = b.cells[3 * c + 2];
+                double x0 = b.X[n0], y0 = b.Y[n0], x1 = b.X[n1], y1 = b.Y[n1], x2 = b.X[n2], y2 = b.Y[n2];
+                double xc = (x0 + x1 + x2) / 3., yc = (y0 + y1 + y2) / 3.;
+                double sf = surf(x0, y0, x1, y1, x2, y2);
+                mass += sf;
+                inertia += sf * ((xc - b.xc) * (xc - b.xc) + (yc - b.yc) * (yc - b.yc));
+                double px[3] = {x0, x1, x2}, py[3] = {y0, y1, y2};
+                double ti = 0.;
+                for (int e = 0; e < 3; e++) {
+                    double ax = px[e], ay = py[e], bx = px[(e + 1) % 3], by = py[(e + 1) % 3];
+                    double area = surf(xc, yc, ax, ay, bx, by);
+                    double PP = (ax - xc) * (ax - xc) + (ay - yc) * (ay - yc);
+                    double PQ = (ax - xc) * (bx - xc) + (ay - yc) * (by - yc);
+                    double QQ = (bx - xc) * (bx - xc) + (by - yc) * (by - yc);
+                    ti += area * (PP + PQ + QQ);
+                }
+                inertia += std::abs(ti / 6.);
+            }
+            body_mass[bi] = mass * rho;
+            body_inertia[bi] = inertia * rho;
+            body_inv_mass[bi] = 1. / body_mass[bi];
+            body_inv_inertia[bi] = 1. / body_inertia[bi];
+        }
+    }
+}
+
+void ensure_rows(mel_ctx& c, int n_rows) {
+    if (n_rows + 1 > c.row_cap) {
+        int cap = std::max(n_rows + 1, c.row_cap * 2);
+        if (c.d_row_count) { (void)hipFree(c.d_row_count); (void)hipFree(c.d_row_off); }
+        c.allocs.erase(std::remove(c.allocs.begin(), c.allocs.end(), (void*)c.d_row_count), c.allocs.end());
+        c.allocs.erase(std::remove(c.allocs.begin(), c.allocs.end(), (void*)c.d_row_off), c.allocs.end());
+        c.d_row_count = c.alloc<int>(cap);
+        c.d_row_off = c.alloc<int>(cap);
+        c.row_cap = cap;
+    }
+    int need = scan_tmp_ints(std::max(n_rows, c.d.n_bodies));
+    if (need > c.scan_tmp_cap) {
+        if (c.d_scan_tmp) {
+            (void)hipFree(c.d_scan_tmp);
+            c.allocs.erase(std::remove(c.allocs.begin(), c.allocs.end(), (void*)c.d_scan_tmp), c.allocs.end());
+        }
+        c.d_scan_tmp = c.alloc<int>(need * 2);
+        c.scan_tmp_cap = need * 2;
+    }
+    c.d.row_count = c.d_row_count;
+    c.d.row_off = c.d_row_off;
+}
+
+void check_flags(mel_ctx& c) {
+    int f[2];
+    HIP_OK(hipMemcpyAsync(f, c.d.status_flags, sizeof f, hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    if (f[0]) {
+        std::string what;
+        if (f[0] & 1) what += " point-force bucket";
+        if (f[0] & 2) what += " neighbour list";
+        if (f[0] & 4) what += " history index";
+        if (f[0] & 8) what += " contact elements";
+        throw CapacityError{"device capacity exceeded:" + what};
+    }
+}
+
+// Update_proximity (Code/Proximity.h:48-473)
+void update_proximity(mel_ctx& c) {
+    DevModel& d = c.d;
+    hipStream_t s = c.stream;
+    k_prox_aabb<<<blocks(d.n_borders), BLOCK, 0, s>>>(d);
+    k_prox_neighbours<<<d.n_bodies, 64, 0, s>>>(d);
+    k_prox_rows<<<blocks(d.n_bodies), BLOCK, 0, s>>>(d);
+    ensure_rows(c, 0);
+    exclusive_scan_int(d.body_rows, d.body_row_off, d.n_bodies, c.d_scan_tmp, s);
+    int n_rows = 0;
+    HIP_OK(hipMemcpyAsync(&n_rows, d.body_row_off + d.n_bodies, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    ensure_rows(c, n_rows);
+    // previous elements become the history source
+    std::swap(c.bufA, c.bufB);
+    d.C = c.bufA;
+    d.Cold = c.bufB;
+    k_prox_clear_old<<<blocks(d.n_slots), BLOCK, 0, s>>>(d);
+    int old = c.n_contacts;
+    if (old > 0) k_prox_old_index<<<blocks(old), BLOCK, 0, s>>>(d);
+    if (n_rows > 0) k_prox_count<<<blocks(n_rows), BLOCK, 0, s>>>(d, n_rows);
+    exclusive_scan_int(d.row_count, d.row_off, n_rows, c.d_scan_tmp, s);
+    k_prox_finish<<<1, 64, 0, s>>>(d, n_rows);
+    if (n_rows > 0) k_prox_write<<<blocks(n_rows), BLOCK, 0, s>>>(d, n_rows);
+    int n = 0;
+    HIP_OK(hipMemcpyAsync(&n, d.c_count, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    c.n_contacts = n;
+    if (n > 0) {
+        k_prox_history<<<blocks(n), BLOCK, 0, s>>>(d);
+        k_prox_chain<<<blocks(n), BLOCK, 0, s>>>(d);
+    }
+    check_flags(c);
+}
+
+// Initialize_CZM (Code/Proximity.h:481-541); it clears the INITIALIZE_CZM
+// flag (Code/Proximity.h:486), so the second call of Code/main.cpp:124 is a no-op
+void initialize_czm(mel_ctx& c) {
+    c.model.flags[F_INITIALIZE_CZM] = 0;
+    if (c.n_contacts == 0) return;
+    k_project_all<<<blocks(c.n_contacts), BLOCK, 0, c.stream>>>(c.d);
+    k_init_czm<<<blocks(c.n_contacts), BLOCK, 0, c.stream>>>(c.d);
+}
+
+// Euler_step (Code/Solver.h:10-88)
+void euler_step(mel_ctx& c) {
+    DevModel& d = c.d;
+    hipStream_t s = c.stream;
+    int nc = c.n_contacts;
+    k_border_length<<<blocks(d.n_slots), BLOCK, 0, s>>>(d);
+    if (nc > 0) k_contact_project<<<blocks(nc), BLOCK, 0, s>>>(d);
+    k_bc_point_forces<<<blocks(d.n_bnodes), BLOCK, 0, s>>>(d);
+    if (d.n_matrix > 0) k_damping<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d);
+    if (nc > 0) {
+        k_contact_law<<<blocks(nc), BLOCK, 0, s>>>(d);
+        k_bucket_fill<<<blocks(3LL * nc), BLOCK, 0, s>>>(d);
+    }
+    k_bucket_sum<<<blocks(d.n_bnodes), BLOCK, 0, s>>>(d);
+    if (d.n_rigid > 0) k_rigid_forces<<<d.n_rigid, 256, 0, s>>>(d);   // one block per rigid body
+    if (d.n_gauss > 0) k_gp_stress<<<blocks(d.n_gauss), BLOCK, 0, s>>>(d);
+    k_node_update<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d);
+    k_bc_eval<<<blocks(d.n_borders), BLOCK, 0, s>>>(d, 1);
+    if (d.n_rigid > 0) k_rigid_step<<<blocks(d.n_bodies), BLOCK, 0, s>>>(d);
+    k_kinematics<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d);
+    if (d.n_rigid > 0) k_rigid_nodes<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d, 1);
+    k_advance_time<<<1, 64, 0, s>>>(d);
+    c.time += d.dt;
+}
+
+// flatten the parsed model into device arrays
+void build(mel_ctx& c) {
+    Model& m = c.model;
+    DevModel& d = c.d;
+    int NB = (int)m.bodies.size();
+    d.n_bodies = NB;
+    d.n_materials = (int)m.materials.size();
+    d.n_laws = (int)m.laws.size();
+    d.period = m.xmax_period - m.xmin_period;
+    d.xmin_period = m.xmin_period;
+    d.gravity_x = m.gravity_x;
+    d.gravity_y = m.gravity_y;
+    d.dt = m.dt;
+    d.self_contact = m.flags[F_NO_MONITORING] ? 0 : 1;
+    for (auto& b : m.bodies) {
+        if (b.status == ST_INACTIVE) throw CapacityError{"inactive bodies are not supported"};
+        if (b.borders.empty()) throw CapacityError{"every body needs at least one border"};
+    }
+    for (auto& l : m.laws)
+        if (l.type == LAW_BONDED_MOHR_COULOMB || l.type == LAW_CZM_FATIGUE)
+            throw CapacityError{"BondedMohrCoulomb and CZMfatigue contact laws are not supported"};
+    for (auto& b : m.bodies)
+        if (m.flags[F_UPDATE_STIFFNESS] && !b.mat_dof.empty())
+            throw CapacityError{"UPDATE_STIFFNESS_MATRIX is not supported"};
+
+    std::vector<double> body_mass, body_inv_mass, body_inertia, body_inv_inertia;
+    compute_masses(m, body_mass, body_inv_mass, body_inertia, body_inv_inertia);
+
+    // bodies
+    std::vector<int> rigid(NB), material(NB), periodicity(NB), status(NB), node_off(NB + 1, 0), nn(NB), gp_off(NB + 1, 0),
+        ng(NB), nreg(NB), perreg(NB), border_off(NB + 1, 0), nbr(NB);
+    std::vector<double> detect(NB), contact(NB);
+    for (int b = 0; b < NB; b++) {
+        Body& B = m.bodies[b];
+        rigid[b] = B.rigid;
+        material[b] = B.material;
+        periodicity[b] = B.periodicity;
+        status[b] = B.status;
+        nn[b] = B.n_nodes;
+        node_off[b + 1] = node_off[b] + B.n_nodes;
+        ng[b] = B.rigid ? 0 : B.n_gauss;
+        gp_off[b + 1] = gp_off[b] + ng[b];
+        nreg[b] = std::max(1, B.n_regions);
+        perreg[b] = ng[b] > 0 ? (int)std::ceil((double)ng[b] / (double)nreg[b]) : 1;
+        nbr[b] = (int)B.borders.size();
+        border_off[b + 1] = border_off[b] + nbr[b];
+        detect[b] = B.detection_distance;
+        contact[b] = B.contact_distance;
+        if (B.rigid) c.h_rigid_ids.push_back(b);
+    }
+    int N = node_off[NB], G = gp_off[NB], NBR = border_off[NB];
+    d.n_nodes = N;
+    d.n_gauss = G;
+    d.n_borders = NBR;
+    d.n_rigid = (int)c.h_rigid_ids.size();
+    c.h_node_off = node_off;
+    c.h_gp_off = gp_off;
+    c.h_border_off = border_off;
+
+    // nodes
+    std::vector<int> node_body(N);
+    std::vector<double> X(N), Y(N), ux(N), uy(N), vx(N), vy(N), ax(N), ay(N), upx(N), upy(N), vpx(N), vpy(N), apx(N),
+        apy(N), mx(N), my(N), imx(N), imy(N), xc(N), yc(N);
+    for (int b = 0; b < NB; b++) {
+        Body& B = m.bodies[b];
+        for (int i = 0; i < B.n_nodes; i++) {
+            int g = node_off[b] + i;
+            node_body[g] = b;
+            X[g] = B.X[i]; Y[g] = B.Y[i];
+            ux[g] = B.ux[i]; uy[g] = B.uy[i]; vx[g] = B.vx[i]; vy[g] = B.vy[i]; ax[g] = B.ax[i]; ay[g] = B.ay[i];
+            upx[g] = B.upx[i]; upy[g] = B.upy[i]; vpx[g] = B.vpx[i]; vpy[g] = B.vpy[i];
+            apx[g] = B.apx[i]; apy[g] = B.apy[i];
+            if (!B.rigid) {
+                mx[g] = B.mx[i]; my[g] = B.my[i]; imx[g] = B.imx[i]; imy[g] = B.imy[i];
+                xc[g] = B.X[i] + B.ux[i]; yc[g] = B.Y[i] + B.uy[i];
+            }
+        }
+    }
+    // Gauss points and the transposed gather
+    std::vector<int> g_ptr(G + 1, 0), g_idx, g_body(G);
+    std::vector<double> g_phix, g_phiy, g_c0(G);
+    for (int b = 0; b < NB; b++) {
+        Body& B = m.bodies[b];
+        if (B.rigid) continue;
+        for (int g = 0; g < B.n_gauss; g++) {
+            int gg = gp_off[b] + g;
+            g_body[gg] = b;
+            g_c0[gg] = B.gw[g] * B.gjac[g];
+            for (int k = B.gptr[g]; k < B.gptr[g + 1]; k++) {
+                g_idx.push_back(node_off[b] + B.gidx[k]);
+                g_phix.push_back(B.gphix[k]);
+                g_phiy.push_back(B.gphiy[k]);
+            }
+            g_ptr[gg + 1] = (int)g_idx.size();
+        }
+    }
+    c.nnz_g = (long long)g_idx.size();
+    std::vector<int> t_ptr(N + 1, 0), t_gp(g_idx.size());
+    std::vector<double> t_phix(g_idx.size()), t_phiy(g_idx.size());
+    for (int j : g_idx) t_ptr[j + 1]++;
+    for (int i = 0; i < N; i++) t_ptr[i + 1] += t_ptr[i];
+    {
+        std::vector<int> cur(t_ptr.begin(), t_ptr.end() - 1);
+        for (int g = 0; g < G; g++)
+            for (int k = g_ptr[g]; k < g_ptr[g + 1]; k++) {
+                int q = cur[g_idx[k]]++;
+                t_gp[q] = g;
+                t_phix[q] = g_phix[k];
+                t_phiy[q] = g_phiy[k];
+            }
+    }
+    // DOF_TO_DISP
+    std::vector<int> d_ptr(N + 1, 0), d_idx;
+    std::vector<double> d_phi;
+    for (int b = 0; b < NB; b++) {
+        Body& B = m.bodies[b];
+        for (int i = 0; i < B.n_nodes; i++) {
+            int g = node_off[b] + i;
+            if (!B.rigid)
+                for (int k = B.dptr[i]; k < B.dptr[i + 1]; k++) {
+                    d_idx.push_back(node_off[b] + B.didx[k]);
+                    d_phi.push_back(B.dphi[k]);
+                }
+            d_ptr[g + 1] = (int)d_idx.size();
+        }
+    }
+    c.nnz_d = (long long)d_idx.size();
+    // damping matrices
+    std::vector<int> m_dof;
+    std::vector<double> m_damp;
+    for (int b = 0; b < NB; b++) {
+        Body& B = m.bodies[b];
+        for (size_t i = 0; i < B.mat_damping.size(); i++) {
+            m_dof.push_back(2 * node_off[b] + B.mat_dof[2 * i]);
+            m_dof.push_back(2 * node_off[b] + B.mat_dof[2 * i + 1]);
+            m_damp.push_back(B.mat_damping[i]);
+        }
+    }
+    int NM = (int)m_damp.size();
+    std::vector<int> mt_ptr(N + 1, 0), mt_entry;
+    {
+        std::vector<std::vector<int>> lists(NM > 0 ? N : 0);
+        for (int i = 0; i < NM; i++) {
+            int dof0 = m_dof[2 * i], dof1 = m_dof[2 * i + 1];
+            lists[dof0 / 2].push_back(2 * i);
+            if (dof0 != dof1) lists[dof1 / 2].push_back(2 * i + 1);
+        }
+        for (int j = 0; j < (NM > 0 ? N : 0); j++) {
+            mt_ptr[j + 1] = mt_ptr[j] + (int)lists[j].size();
+            mt_entry.insert(mt_entry.end(), lists[j].begin(), lists[j].end());
+        }
+    }
+    d.n_matrix = NM;
+    // borders and slots
+    std::vector<int> br_body(NBR), br_slot0(NBR), br_nslots(NBR), br_before(NBR), br_after(NBR), br_per(NBR),
+        br_xt(NBR), br_yt(NBR), br_hist_ptr(2 * NBR + 1, 0);
+    std::vector<double> br_xparam(2 * NBR), br_yparam(2 * NBR), hist_t, hist_v;
+    std::vector<int> sl_border, sl_j, sl_node, sl_n0, sl_n1, sl_n2, sl_n3, sl_s0, sl_s1, sl_s2, sl_s3;
+    std::vector<double> sl_len0;
+    for (int b = 0; b < NB; b++) {
+        Body& B = m.bodies[b];
+        for (int r = 0; r < nbr[b]; r++) {
+            Border& R = B.borders[r];
+            int gb = border_off[b] + r;
+            br_body[gb] = b;
+            br_slot0[gb] = (int)sl_node.size();
+            br_nslots[gb] = (int)R.nodes.size();
+            br_before[gb] = R.before;
+            br_after[gb] = R.after;
+            br_per[gb] = R.periodicity;
+            br_xt[gb] = R.xbc.type;
+            br_yt[gb] = R.ybc.type;
+            br_xparam[2 * gb] = R.xbc.stiffness; br_xparam[2 * gb + 1] = R.xbc.damping;
+            br_yparam[2 * gb] = R.ybc.stiffness; br_yparam[2 * gb + 1] = R.ybc.damping;
+            hist_t.insert(hist_t.end(), R.xbc.hist.t.begin(), R.xbc.hist.t.end());
+            hist_v.insert(hist_v.end(), R.xbc.hist.v.begin(), R.xbc.hist.v.end());
+            br_hist_ptr[2 * gb + 1] = (int)hist_t.size();
+            hist_t.insert(hist_t.end(), R.ybc.hist.t.begin(), R.ybc.hist.t.end());
+            hist_v.insert(hist_v.end(), R.ybc.hist.v.begin(), R.ybc.hist.v.end());
+            br_hist_ptr[2 * gb + 2] = (int)hist_t.size();
+            if ((int)R.nodes.size() < 3) throw CapacityError{"borders need at least 3 node positions"};
+            for (size_t j = 0; j < R.nodes.size(); j++) {
+                int o = node_off[b];
+                sl_border.push_back(gb);
+                sl_j.push_back((int)j);
+                sl_node.push_back(o + R.nodes[j]);
+                sl_n0.push_back(o + R.node0[j]); sl_n1.push_back(o + R.node1[j]);
+                sl_n2.push_back(o + R.node2[j]); sl_n3.push_back(o + R.node3[j]);
+                sl_s0.push_back(R.shift0[j]); sl_s1.push_back(R.shift1[j]);
+                sl_s2.push_back(R.shift2[j]); sl_s3.push_back(R.shift3[j]);
+                sl_len0.push_back(R.initial_length[j]);
+            }
+        }
+    }
+    int NS = (int)sl_node.size();
+    d.n_slots = NS;
+    c.h_br_slot0 = br_slot0;
+    c.h_br_nslots = br_nslots;
+    // distinct border nodes in first-appearance order
+    std::vector<int> node_bn(N, -1), bn_node, sl_bn(NS);
+    for (int s = 0; s < NS; s++) {
+        int nd = sl_node[s];
+        if (node_bn[nd] < 0) { node_bn[nd] = (int)bn_node.size(); bn_node.push_back(nd); }
+        sl_bn[s] = node_bn[nd];
+    }
+    int NBN = (int)bn_node.size();
+    d.n_bnodes = NBN;
+    std::vector<int> bn_slot_ptr(NBN + 1, 0), bn_slots(NS);
+    for (int s = 0; s < NS; s++) bn_slot_ptr[sl_bn[s] + 1]++;
+    for (int i = 0; i < NBN; i++) bn_slot_ptr[i + 1] += bn_slot_ptr[i];
+    {
+        std::vector<int> cur(bn_slot_ptr.begin(), bn_slot_ptr.end() - 1);
+        for (int s = 0; s < NS; s++) bn_slots[cur[sl_bn[s]]++] = s;
+    }
+    // transposed DOF_TO_DISP rows of border nodes: node j -> (bn, phi_j(x_bn))
+    std::vector<int> bt_ptr(N + 1, 0), bt_bn;
+    std::vector<double> bt_phi;
+    {
+        std::vector<int> cnt(N, 0);
+        for (int q = 0; q < NBN; q++) {
+            int i = bn_node[q];
+            for (int k = d_ptr[i]; k < d_ptr[i + 1]; k++) cnt[d_idx[k]]++;
+        }
+        for (int j = 0; j < N; j++) bt_ptr[j + 1] = bt_ptr[j] + cnt[j];
+        bt_bn.resize(bt_ptr[N]);
+        bt_phi.resize(bt_ptr[N]);
+        std::vector<int> cur(bt_ptr.begin(), bt_ptr.end() - 1);
+        for (int q = 0; q < NBN; q++) {
+            int i = bn_node[q];
+            for (int k = d_ptr[i]; k < d_ptr[i + 1]; k++) {
+                int p = cur[d_idx[k]]++;
+                bt_bn[p] = q;
+                bt_phi[p] = d_phi[k];
+            }
+        }
+    }
+    // materials and laws
+    std::vector<int> mat_type(d.n_materials), law_type(d.n_laws), law_fixed(d.n_laws);
+    std::vector<double> mat_p(6 * d.n_materials), law_p(12 * std::max(1, d.n_laws), 0.);
+    for (int i = 0; i < d.n_materials; i++) {
+        mat_type[i] = m.materials[i].type;
+        for (int k = 0; k < 6; k++) mat_p[6 * i + k] = m.materials[i].p[k];
+    }
+    for (int i = 0; i < d.n_laws; i++) {
+        law_type[i] = m.laws[i].type;
+        law_fixed[i] = m.laws[i].fixed_length;
+        for (size_t k = 0; k < m.laws[i].p.size() && k < 12; k++) law_p[12 * i + k] = m.laws[i].p[k];
+    }
+    std::vector<int> law_first(d.n_materials * d.n_materials, -1);
+    for (int a = 0; a < d.n_materials; a++)
+        for (int bm = 0; bm < d.n_materials; bm++)
+            for (int i = 0; i < d.n_laws; i++) {
+                const ContactLaw& L = m.laws[i];
+                const std::string &ma = m.materials[a].name, &mb = m.materials[bm].name;
+                if ((L.m1 == ma && L.m2 == mb) || (L.m2 == ma && L.m1 == mb)) { law_first[a * d.n_materials + bm] = i; break; }
+            }
+    // rigid / per-body state
+    std::vector<double> body_rigid((size_t)NB * RIGID_FIELDS, 0.);
+    for (int b = 0; b < NB; b++) {
+        Body& B = m.bodies[b];
+        double* R = &body_rigid[(size_t)b * RIGID_FIELDS];
+        for (int q = 0; q < 7; q++) R[R_WORK + q] = B.works[q];
+        if (!B.rigid) continue;
+        for (int q = 0; q < 3; q++) { R[R_DISP + q] = B.rigid_disp[q]; R[R_VEL + q] = B.rigid_vel[q]; }
+        R[R_XC] = B.xc;
+        R[R_YC] = B.yc;
+        // Update_current_positions at load (Code/IO.h:2050)
+        R[R_CUR] = B.xc + R[R_DISP];
+        R[R_CUR + 1] = B.yc + R[R_DISP + 1];
+        R[R_CUR + 2] = 0. + R[R_DISP + 2];
+    }
+    // node work accumulators start from the loaded body works on the first node
+    std::vector<double> node_work((size_t)7 * N, 0.);
+    for (int b = 0; b < NB; b++) {
+        Body& B = m.bodies[b];
+        if (B.rigid || B.n_nodes == 0) continue;
+        for (int q = 0; q < 7; q++)
+            if (q != W_CONTACT) node_work[7 * (size_t)node_off[b] + q] = B.works[q];
+        body_rigid[(size_t)b * RIGID_FIELDS + R_WORK + W_INTERNAL] = 0.;
+        for (int q = 0; q < 7; q++)
+            if (q != W_CONTACT) body_rigid[(size_t)b * RIGID_FIELDS + R_WORK + q] = 0.;
+    }
+
+    // ---- upload ----
+    d.body_rigid_flag = c.upload(rigid);
+    d.body_material = c.upload(material);
+    d.body_periodicity = c.upload(periodicity);
+    d.body_status = c.upload(status);
+    d.node_off = c.upload(node_off);
+    d.body_nnodes = c.upload(nn);
+    d.gp_off = c.upload(gp_off);
+    d.body_ngauss = c.upload(ng);
+    d.body_nregions = c.upload(nreg);
+    d.body_per_region = c.upload(perreg);
+    d.border_off = c.upload(border_off);
+    d.body_nborders = c.upload(nbr);
+    d.body_detect = c.upload(detect);
+    d.body_contact = c.upload(contact);
+    // masses are uploaded in two stages by the init sequence
+    std::vector<double> mass0(NB, 0.), inv0(NB, 0.), inertia0(NB, 0.), invi0(NB, 0.);
+    for (int b = 0; b < NB; b++)
+        if (m.bodies[b].rigid) {
+            mass0[b] = m.bodies[b].mass; inv0[b] = m.bodies[b].inverse_mass;
+            inertia0[b] = m.bodies[b].inertia; invi0[b] = m.bodies[b].inverse_inertia;
+        }
+    d.body_mass = c.upload(mass0);
+    d.body_inv_mass = c.upload(inv0);
+    d.body_inertia = c.upload(inertia0);
+    d.body_inv_inertia = c.upload(invi0);
+    d.body_rigid = c.upload(body_rigid);
+    d.mat_type = c.upload(mat_type);
+    d.mat_p = c.upload(mat_p);
+    d.law_table = c.upload(m.contacts_table);
+    d.law_first = c.upload(law_first);
+    d.law_type = c.upload(law_type);
+    d.law_fixed = c.upload(law_fixed);
+    d.law_p = c.upload(law_p);
+    d.node_body = c.upload(node_body);
+    d.X = c.upload(X); d.Y = c.upload(Y);
+    d.xc = c.upload(xc); d.yc = c.upload(yc);
+    d.ux = c.upload(ux); d.uy = c.upload(uy); d.vx = c.upload(vx); d.vy = c.upload(vy);
+    d.ax = c.upload(ax); d.ay = c.upload(ay);
+    d.upx = c.upload(upx); d.upy = c.upload(upy); d.vpx = c.upload(vpx); d.vpy = c.upload(vpy);
+    d.apx = c.upload(apx); d.apy = c.upload(apy);
+    d.mx = c.upload(mx); d.my = c.upload(my); d.imx = c.upload(imx); d.imy = c.upload(imy);
+    double** forces[] = {&d.fix, &d.fiy, &d.fcx, &d.fcy, &d.fbx, &d.fby, &d.fdx, &d.fdy, &d.fnx, &d.fny, &d.fkx,
+                         &d.fky, &d.fx, &d.fy};
+    for (double** p : forces) *p = c.alloc<double>(N);
+    d.node_work = c.upload(node_work);
+    d.g_ptr = c.upload(g_ptr);
+    d.g_idx = c.upload(g_idx);
+    d.g_phix = c.upload(g_phix);
+    d.g_phiy = c.upload(g_phiy);
+    d.g_c0 = c.upload(g_c0);
+    d.g_body = c.upload(g_body);
+    d.g_P = c.alloc<double>(4 * (size_t)std::max(G, 1));
+    d.t_ptr = c.upload(t_ptr);
+    d.t_gp = c.upload(t_gp);
+    d.t_phix = c.upload(t_phix);
+    d.t_phiy = c.upload(t_phiy);
+    d.d_ptr = c.upload(d_ptr);
+    d.d_idx = c.upload(d_idx);
+    d.d_phi = c.upload(d_phi);
+    d.m_dof = c.upload(m_dof);
+    d.m_damp = c.upload(m_damp);
+    d.mt_ptr = c.upload(mt_ptr);
+    d.mt_entry = c.upload(mt_entry);
+    d.br_body = c.upload(br_body);
+    d.br_slot0 = c.upload(br_slot0);
+    d.br_nslots = c.upload(br_nslots);
+    d.br_before = c.upload(br_before);
+    d.br_after = c.upload(br_after);
+    d.br_periodicity = c.upload(br_per);
+    d.br_xtype = c.upload(br_xt);
+    d.br_ytype = c.upload(br_yt);
+    d.br_xparam = c.upload(br_xparam);
+    d.br_yparam = c.upload(br_yparam);
+    d.br_hist_ptr = c.upload(br_hist_ptr);
+    d.hist_t = c.upload(hist_t);
+    d.hist_v = c.upload(hist_v);
+    d.br_xval = c.alloc<double>(NBR); d.br_yval = c.alloc<double>(NBR);
+    d.br_xvel = c.alloc<double>(NBR); d.br_yvel = c.alloc<double>(NBR);
+    d.sl_border = c.upload(sl_border); d.sl_j = c.upload(sl_j); d.sl_node = c.upload(sl_node);
+    d.sl_n0 = c.upload(sl_n0); d.sl_n1 = c.upload(sl_n1); d.sl_n2 = c.upload(sl_n2); d.sl_n3 = c.upload(sl_n3);
+    d.sl_s0 = c.upload(sl_s0); d.sl_s1 = c.upload(sl_s1); d.sl_s2 = c.upload(sl_s2); d.sl_s3 = c.upload(sl_s3);
+    d.sl_bn = c.upload(sl_bn);
+    d.sl_len = c.upload(sl_len0);
+    d.sl_len0 = c.upload(sl_len0);
+    d.sl_xcp = c.alloc<double>(NS);
+    d.sl_ycp = c.alloc<double>(NS);
+    d.bn_node = c.upload(bn_node);
+    d.bn_slot_ptr = c.upload(bn_slot_ptr);
+    d.bn_slots = c.upload(bn_slots);
+    double** bnp[] = {&d.bn_pcx, &d.bn_pcy, &d.bn_pdx, &d.bn_pdy, &d.bn_pnx, &d.bn_pny};
+    for (double** p : bnp) *p = c.alloc<double>(NBN);
+    d.bt_ptr = c.upload(bt_ptr);
+    d.bt_bn = c.upload(bt_bn);
+    d.bt_phi = c.upload(bt_phi);
+    d.bk_count = c.alloc<int>(NBN);
+    d.bk_cursor = c.alloc<int>(NBN);
+    d.bk_rec = c.alloc<int>((size_t)NBN * BUCKET_CAP);
+    // contacts
+    d.c_cap = 4 * NS + 1024;
+    d.c_count = c.alloc<int>(1);
+    d.c_count_old = c.alloc<int>(1);
+    alloc_contacts(c, c.bufA, d.c_cap);
+    alloc_contacts(c, c.bufB, d.c_cap);
+    d.C = c.bufA;
+    d.Cold = c.bufB;
+    d.rec_bn = c.alloc<int>(3 * (size_t)d.c_cap);
+    d.rec_fx = c.alloc<double>(3 * (size_t)d.c_cap);
+    d.rec_fy = c.alloc<double>(3 * (size_t)d.c_cap);
+    d.rig_rec = c.alloc<double>(6 * (size_t)d.c_cap);
+    // proximity scratch
+    d.pb_xmin = c.alloc<double>(NBR); d.pb_xmax = c.alloc<double>(NBR);
+    d.pb_ymin = c.alloc<double>(NBR); d.pb_ymax = c.alloc<double>(NBR);
+    d.pb_shmin = c.alloc<int>(NBR); d.pb_shmax = c.alloc<int>(NBR);
+    d.nb_cap = 256;
+    d.nb_count = c.alloc<int>(NB);
+    d.nb_list = c.alloc<int>((size_t)NB * d.nb_cap * 3);
+    d.body_rows = c.alloc<int>(NB);
+    d.body_row_off = c.alloc<int>(NB + 1);
+    d.slot_old_list = c.alloc<int>((size_t)NS * 16);
+    d.slot_old_cursor = c.alloc<int>(NS);
+    d.status_flags = c.alloc<int>(4);
+    d.time = c.alloc<double>(1);
+    HIP_OK(hipMemcpyAsync(d.time, &m.time, sizeof(double), hipMemcpyHostToDevice, c.stream));
+    c.d_rigid_ids = c.upload(c.h_rigid_ids);
+    d.rigid_ids = c.d_rigid_ids;
+    HIP_OK(hipStreamSynchronize(c.stream));
+
+    c.time = m.time;
+    c.iteration = m.number_iteration;
+    c.next_contact_update = m.next_contact_update;
+
+    // rigid node positions, Update_current_positions at load (Code/IO.h:2050)
+    if (d.n_rigid > 0) k_rigid_nodes<<<blocks(N), BLOCK, 0, c.stream>>>(d, 0);
+    // ---- initialisation block, Code/main.cpp:103-126 ----
+    k_border_length<<<blocks(NS), BLOCK, 0, c.stream>>>(d);          // Update_borders
+    k_bc_eval<<<blocks(NBR), BLOCK, 0, c.stream>>>(d, 0);            // Update_bc(Time)
+    update_proximity(c);
+    if (m.flags[F_INITIALIZE_CZM]) initialize_czm(c);
+    // Update_material: masses of deformable bodies, rigid mass and inertia
+    {
+        std::vector<double> mx2(N), my2(N), imx2(N), imy2(N);
+        for (int b = 0; b < NB; b++) {
+            Body& B = m.bodies[b];
+            for (int i = 0; i < B.n_nodes; i++) {
+                int g = node_off[b] + i;
+                if (!B.rigid) { mx2[g] = B.mx[i]; my2[g] = B.my[i]; imx2[g] = B.imx[i]; imy2[g] = B.imy[i]; }
+            }
+        }
+        HIP_OK(hipMemcpyAsync(d.mx, mx2.data(), N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        HIP_OK(hipMemcpyAsync(d.my, my2.data(), N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        HIP_OK(hipMemcpyAsync(d.imx, imx2.data(), N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        HIP_OK(hipMemcpyAsync(d.imy, imy2.data(), N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        std::vector<double> bm(NB), bim(NB), bi(NB), bii(NB);
+        for (int b = 0; b < NB; b++) {
+            bool upd = m.bodies[b].rigid || m.flags[F_UPDATE_MASS];
+            bm[b] = upd ? body_mass[b] : mass0[b];
+            bim[b] = upd ? body_inv_mass[b] : inv0[b];
+            bi[b] = upd ? body_inertia[b] : inertia0[b];
+            bii[b] = upd ? body_inv_inertia[b] : invi0[b];
+        }
+        HIP_OK(hipMemcpyAsync(d.body_mass, bm.data(), NB * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        HIP_OK(hipMemcpyAsync(d.body_inv_mass, bim.data(), NB * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        HIP_OK(hipMemcpyAsync(d.body_inertia, bi.data(), NB * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        HIP_OK(hipMemcpyAsync(d.body_inv_inertia, bii.data(), NB * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    }
+    update_proximity(c);
+    if (m.flags[F_INITIALIZE_CZM]) initialize_czm(c);
+    HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+thread_local std::string g_load_err;
+
+mel_status create(mel_ctx** out, int device, int precision, const std::function<bool(Model&, std::string&)>& loader) {
+    if (!out) return MEL_E_ARG;
+    *out = nullptr;
+    if (precision != MEL_FP64) { g_load_err = "only the FP64 path is available"; return MEL_E_UNSUPPORTED; }
+    std::unique_ptr<mel_ctx> c(new mel_ctx());
+    c->device = device;
+    c->precision = precision;
+    std::string err;
+    if (!loader(c->model, err)) {
+        g_load_err = err;
+        return err.rfind("cannot open", 0) == 0 ? MEL_E_IO : MEL_E_PARSE;
+    }
+    try {
+        HIP_OK(hipSetDevice(device));
+        HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        build(*c);
+    } catch (HipError& e) {
+        g_load_err = e.msg;
+        return e.msg.find("hipMalloc") != std::string::npos ? MEL_E_OOM : MEL_E_HIP;
+    } catch (CapacityError& e) {
+        g_load_err = e.msg;
+        return e.msg.find("capacity") != std::string::npos ? MEL_E_CAPACITY : MEL_E_UNSUPPORTED;
+    }
+    *out = c.release();
+    return MEL_OK;
+}
+
+template <typename F>
+mel_status guarded(mel_ctx* c, F&& f) {
+    if (!c) return MEL_E_ARG;
+    try {
+        HIP_OK(hipSetDevice(c->device));
+        return f();
+    } catch (HipError& e) {
+        c->err = e.msg;
+        return MEL_E_HIP;
+    } catch (CapacityError& e) {
+        c->err = e.msg;
+        return MEL_E_CAPACITY;
+    }
+}
+
+double* node_field_ptr(mel_ctx& c, int field) {
+    DevModel& d = c.d;
+    double* f[] = {d.xc, d.yc, d.ux, d.uy, d.vx, d.vy, d.ax, d.ay, d.upx, d.upy, d.vpx, d.vpy, d.apx, d.apy,
+                   d.fix, d.fiy, d.fcx, d.fcy, d.fbx, d.fby, d.fdx, d.fdy, d.fnx, d.fny, d.fkx, d.fky,
+                   d.fx, d.fy, d.mx, d.my};
+    if (field < 0 || field >= MEL_NODE_FIELD_COUNT) return nullptr;
+    return f[field];
+}
+}  // namespace
+
+extern "C" {
+
+const char* mel_version(void) { return "melody2d-mi355x 0.1"; }
+
+mel_status mel_load(const char* dir, int save_index, mel_precision precision, int hip_device, mel_ctx** out) {
+    if (!dir) return MEL_E_ARG;
+    std::string d(dir);
+    return create(out, hip_device, precision,
+                  [&](Model& m, std::string& err) { return load_model_files(d, save_index, m, err); });
+}
+
+mel_status mel_load_binary(const char* path, mel_precision precision, int hip_device, mel_ctx** out) {
+    if (!path) return MEL_E_ARG;
+    std::string p(path);
+    return create(out, hip_device, precision,
+                  [&](Model& m, std::string& err) { return load_model_binary(p, m, err); });
+}
+
+mel_status mel_update_proximity(mel_ctx* c) {
+    return guarded(c, [&]() { update_proximity(*c); return MEL_OK; });
+}
+
+mel_status mel_euler_steps(mel_ctx* c, int nsteps) {
+    if (nsteps < 0) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        for (int k = 0; k < nsteps; k++) {
+            c->iteration++;
+            euler_step(*c);
+            // contact-update cadence, Code/main.cpp:191-196
+            if (c->time > c->next_contact_update - 1.e-6 * c->d.dt) {
+                c->next_contact_update = c->next_contact_update + c->model.contact_update_period;
+                update_proximity(*c);
+            }
+        }
+        HIP_OK(hipGetLastError());
+        return MEL_OK;
+    });
+}
+
+mel_status mel_synchronize(mel_ctx* c) {
+    return guarded(c, [&]() {
+        HIP_OK(hipStreamSynchronize(c->stream));
+        check_flags(*c);
+        return MEL_OK;
+    });
+}
+
+mel_status mel_get_node_field(mel_ctx* c, int body, int field, double* out, size_t n) {
+    if (!c || !out || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
+    double* p = node_field_ptr(*c, field);
+    size_t nn = c->model.bodies[body].n_nodes;
+    if (!p || n < nn) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        HIP_OK(hipMemcpyAsync(out, p + c->h_node_off[body], nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        return MEL_OK;
+    });
+}
+
+mel_status mel_set_node_field(mel_ctx* c, int body, int field, const double* in, size_t n) {
+    if (!c || !in || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
+    double* p = node_field_ptr(*c, field);
+    size_t nn = c->model.bodies[body].n_nodes;
+    if (!p || n < nn) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        HIP_OK(hipMemcpyAsync(p + c->h_node_off[body], in, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        return MEL_OK;
+    });
+}
+
+mel_status mel_get_contacts(mel_ctx* c, int body, int32_t* ids, double* state, size_t cap, size_t* count) {
+    if (!c || !count || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        int n = c->n_contacts;
+        ContactArrays& A = c->bufA;
+        std::vector<int> I[18];
+        int* ip[18] = {A.bodyS, A.borderS, A.bnodeS, A.nodeS, A.bodyM, A.borderM, A.shiftM, A.segM, A.nM0, A.nM1,
+                       A.nM2, A.nM3, A.sM0, A.sM1, A.sM2, A.sM3, A.nbint, A.fresh};
+        for (int k = 0; k < 18; k++) c->download(I[k], ip[k], n);
+        std::vector<double> D[18];
+        double* dp[18] = {A.N0, A.N1, A.N2, A.N3, A.gapn, A.vgapn, A.gapt, A.vgapt, A.xsi, A.xn, A.yn, A.xt, A.yt,
+                          A.emass, A.len, A.fx, A.fy, A.int0};
+        for (int k = 0; k < 18; k++) c->download(D[k], dp[k], n);
+        size_t w = 0;
+        for (int e = 0; e < n; e++) {
+            if (I[0][e] != body) continue;
+            if (ids && state && w < cap) {
+                int oS = c->h_node_off[body], oM = c->h_node_off[I[4][e]];
+                int32_t* o = ids + 16 * w;
+                o[0] = I[1][e]; o[1] = I[2][e]; o[2] = I[3][e] - oS; o[3] = I[4][e]; o[4] = I[5][e]; o[5] = I[6][e];
+                o[6] = I[7][e];
+                for (int k = 0; k < 4; k++) { o[7 + k] = I[8 + k][e] - oM; o[11 + k] = I[12 + k][e]; }
+                o[15] = I[16][e];
+                for (int k = 0; k < 18; k++) state[18 * w + k] = D[k][e];
+            }
+            w++;
+        }
+        *count = w;
+        return w > cap && ids ? MEL_E_ARG : MEL_OK;
+    });
+}
+
+mel_status mel_get_neighbours(mel_ctx* c, int body, int32_t* out, size_t cap, size_t* count) {
+    if (!c || !count || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        std::vector<int> nb, lst;
+        c->download(nb, c->d.nb_count + body, 1);
+        c->download(lst, c->d.nb_list + (size_t)body * c->d.nb_cap * 3, 3 * (size_t)nb[0]);
+        *count = nb[0];
+        if (out) {
+            if ((size_t)nb[0] > cap) return MEL_E_ARG;
+            std::copy(lst.begin(), lst.end(), out);
+        }
+        return MEL_OK;
+    });
+}
+
+mel_status mel_get_body_state(mel_ctx* c, int body, double* rigid26, double* works7) {
+    if (!c || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        std::vector<double> R, bm, bi;
+        c->download(R, c->d.body_rigid + (size_t)body * RIGID_FIELDS, RIGID_FIELDS);
+        c->download(bm, c->d.body_mass + body, 1);
+        c->download(bi, c->d.body_inertia + body, 1);
+        if (rigid26) {
+            double v[26] = {R[R_CUR], R[R_CUR + 1], R[R_CUR + 2], R[R_DISP], R[R_DISP + 1], R[R_DISP + 2],
+                            R[R_VEL], R[R_VEL + 1], R[R_VEL + 2], R[R_ACC], R[R_ACC + 1], R[R_ACC + 2],
+                            R[R_FCON], R[R_FCON + 1], R[R_FCON + 2], R[R_FTOT], R[R_FTOT + 1], R[R_FTOT + 2],
+                            bm[0], bi[0], R[R_FDIR], R[R_FDIR + 1], R[R_FDIR + 2], R[R_FNEU], R[R_FNEU + 1],
+                            R[R_FNEU + 2]};
+            std::memcpy(rigid26, v, sizeof v);
+        }
+        if (works7) {
+            for (int q = 0; q < 7; q++) works7[q] = R[R_WORK + q];
+            const Body& B = c->model.bodies[body];
+            if (!B.rigid && B.n_nodes > 0) {
+                std::vector<double> w;
+                c->download(w, c->d.node_work + 7 * (size_t)c->h_node_off[body], 7 * (size_t)B.n_nodes);
+                // compensated (Neumaier) sum over the body's nodes
+                for (int q = 0; q < 7; q++) {
+                    if (q == W_CONTACT) continue;
+                    double s = 0., comp = 0.;
+                    for (int i = 0; i < B.n_nodes; i++) {
+                        double x = w[7 * (size_t)i + q], t = s + x;
+                        comp += std::fabs(s) >= std::fabs(x) ? (s - t) + x : (x - t) + s;
+                        s = t;
+                    }
+                    works7[q] = s + comp;
+                }
+            }
+        }
+        return MEL_OK;
+    });
+}
+
+mel_status mel_get_border_field(mel_ctx* c, int body, int border, int field, double* out, size_t n) {
+    if (!c || !out || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
+    if (border < 0 || border >= (int)c->model.bodies[body].borders.size() || field < 0 || field > 2) return MEL_E_ARG;
+    int gb = c->h_border_off[body] + border;
+    size_t ns = c->h_br_nslots[gb];
+    if (n < ns) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        if (field == 0) {
+            HIP_OK(hipMemcpyAsync(out, c->d.sl_len + c->h_br_slot0[gb], ns * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIP_OK(hipStreamSynchronize(c->stream));
+            return MEL_OK;
+        }
+        // Update_contact_pressures (Code/Proximity.h:1014-1069): evaluated on
+        // request from the current elements, in element order
+        int nc = c->n_contacts;
+        ContactArrays& A = c->bufA;
+        std::vector<int> bS, brS, jS, bM, brM, seg;
+        std::vector<double> fx, fy, len, N1, N2;
+        c->download(bS, A.bodyS, nc); c->download(brS, A.borderS, nc); c->download(jS, A.bnodeS, nc);
+        c->download(bM, A.bodyM, nc); c->download(brM, A.borderM, nc); c->download(seg, A.segM, nc);
+        c->download(fx, A.fx, nc); c->download(fy, A.fy, nc); c->download(len, A.len, nc);
+        c->download(N1, A.N1, nc); c->download(N2, A.N2, nc);
+        std::vector<double> p(ns, 0.);
+        for (int e = 0; e < nc; e++) {
+            double f = field == 1 ? fx[e] : fy[e];
+            double il = 1. / len[e];
+            if (bS[e] == body && brS[e] == border) p[jS[e]] += f * il;
+            if (bM[e] == body && brM[e] == border) {
+                p[seg[e]] -= f * il * N1[e];
+                p[seg[e] + 1] -= f * il * N2[e];
+            }
+        }
+        std::copy(p.begin(), p.end(), out);
+        return MEL_OK;
+    });
+}
+
+mel_status mel_get_info(mel_ctx* c, mel_info_t* info) {
+    if (!c || !info) return MEL_E_ARG;
+    info->n_bodies = c->d.n_bodies;
+    info->n_nodes = c->d.n_nodes;
+    info->n_gauss = c->d.n_gauss;
+    info->n_borders = c->d.n_borders;
+    info->n_border_nodes = c->d.n_bnodes;
+    info->n_rigid = c->d.n_rigid;
+    info->n_contacts = c->n_contacts;
+    info->contact_capacity = c->d.c_cap;
+    info->nnz_gauss = c->nnz_g;
+    info->nnz_d2d = c->nnz_d;
+    info->time = c->time;
+    info->dt = c->d.dt;
+    info->iteration = c->iteration;
+    info->precision = c->precision;
+    return MEL_OK;
+}
+
+mel_status mel_write_dynamic(mel_ctx* c, const char* path) {
+    if (!c || !path) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        Model& m = c->model;
+        m.time = c->time;
+        m.number_iteration = c->iteration;
+        m.next_contact_update = c->next_contact_update;
+        for (int b = 0; b < c->d.n_bodies; b++) {
+            Body& B = m.bodies[b];
+            size_t n = B.n_nodes;
+            std::vector<double>* f[12] = {&B.ux, &B.uy, &B.vx, &B.vy, &B.ax, &B.ay, &B.upx, &B.upy, &B.vpx, &B.vpy, &B.apx, &B.apy};
+            int fld[12] = {MEL_X_DISP, MEL_Y_DISP, MEL_X_VEL, MEL_Y_VEL, MEL_X_ACC, MEL_Y_ACC, MEL_X_DISP_PARAM,
+                           MEL_Y_DISP_PARAM, MEL_X_VEL_PARAM, MEL_Y_VEL_PARAM, MEL_X_ACC_PARAM, MEL_Y_ACC_PARAM};
+            for (int k = 0; k < 12; k++) c->download(*f[k], node_field_ptr(*c, fld[k]) + c->h_node_off[b], n);
+            double r[26], w[7];
+            mel_get_body_state(c, b, r, w);
+            for (int q = 0; q < 3; q++) { B.rigid_cur[q] = r[q]; B.rigid_disp[q] = r[3 + q]; B.rigid_vel[q] = r[6 + q]; }
+            for (int q = 0; q < 7; q++) B.works[q] = w[q];
+            size_t cnt = 0;
+            mel_get_neighbours(c, b, nullptr, 0, &cnt);
+            B.neighbours.assign(3 * cnt, 0);
+            mel_get_neighbours(c, b, B.neighbours.data(), cnt, &cnt);
+            mel_get_contacts(c, b, nullptr, nullptr, 0, &cnt);
+            std::vector<int32_t> ids(16 * cnt);
+            std::vector<double> st(18 * cnt);
+            mel_get_contacts(c, b, ids.data(), st.data(), cnt, &cnt);
+            B.contacts.assign(cnt, ContactElement());
+            for (size_t e = 0; e < cnt; e++) {
+                ContactElement& E = B.contacts[e];
+                int32_t* o = &ids[16 * e];
+                double* s = &st[18 * e];
+                E.borderS = o[0]; E.border_nodeS = o[1]; E.nodeS = o[2]; E.bodyM = o[3]; E.borderM = o[4];
+                E.shiftM = o[5]; E.segmentM = o[6];
+                for (int k = 0; k < 4; k++) { E.nodeM[k] = o[7 + k]; E.shiftMk[k] = o[11 + k]; E.shapeM[k] = s[k]; }
+                E.nb_internal = o[15];
+                E.gapn = s[4]; E.vgapn = s[5]; E.gapt = s[6]; E.vgapt = s[7]; E.xsi = s[8]; E.xnorm = s[9];
+                E.ynorm = s[10]; E.xtan = s[11]; E.ytan = s[12]; E.effective_mass = s[13]; E.length = s[14];
+                E.fx = s[15]; E.fy = s[16];
+                E.internal.assign(std::max(1, E.nb_internal), s[17]);
+            }
+        }
+        std::string err;
+        if (!write_dynamic_text(path, m, err)) { c->err = err; return MEL_E_IO; }
+        return MEL_OK;
+    });
+}
+
+mel_status mel_get_stream(mel_ctx* c, void** stream) {
+    if (!c || !stream) return MEL_E_ARG;
+    *stream = (void*)c->stream;
+    return MEL_OK;
+}
+
+void mel_destroy(mel_ctx* c) { delete c; }
+
+const char* mel_last_error(const mel_ctx* c) { return c ? c->err.c_str() : g_load_err.c_str(); }
+
+}  // extern "C"

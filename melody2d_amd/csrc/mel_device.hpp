@@ -1,0 +1,352 @@
+// Device-side model state and the arithmetic shared by the kernels.
+//
+// Layout (all arrays in HBM, structure of arrays, one allocation each):
+//   nodes      global index g = node_off[body] + local id; FP64 fields
+//   Gauss pts  global index in body order; CSR gp -> support (global node ids)
+//   gtrans     transposed CSR node -> (gp, dphi/dx, dphi/dy) for the
+//              deterministic node-centric internal-force gather
+//   d2d        DOF_TO_DISP CSR node -> (support node, phi)
+//   borders    global border id = border_off[body] + local border
+//   slots      one per border-node position (closed borders repeat their first
+//              node), with the four-node segment stencil of Initialize_segments
+//   bnodes     distinct border nodes; point forces (contact, Dirichlet,
+//              Neumann) are summed per border node and spread through the
+//              transposed DOF_TO_DISP rows restricted to border nodes (btrans)
+//   contacts   structure of arrays with a fixed capacity, double-buffered
+//              across proximity updates
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace mel {
+
+constexpr int RIGID_FIELDS = 64;
+// per-body state slots in body_rigid[b * RIGID_FIELDS + k]; the work slots are
+// used by every body, the rest by rigid bodies
+enum RigidSlot {
+    R_CUR = 0, R_DISP = 3, R_VEL = 6, R_ACC = 9, R_FCON = 12, R_FBODY = 15, R_FDIR = 18, R_FNEU = 21,
+    R_FDAMP = 24, R_FTOT = 27, R_DRIVEN = 30, R_XC = 38, R_YC = 39, R_WORK = 40, R_NODE_ACC = 48,
+    R_FDIR_NODE = 50
+};
+enum WorkSlot { W_INTERNAL = 0, W_CONTACT, W_BODY, W_DIRICHLET, W_NEUMANN, W_DAMPING, W_ALID };
+
+struct ContactArrays {
+    int *bodyS, *borderS, *bnodeS, *nodeS, *bodyM, *borderM, *shiftM, *segM, *nbint, *fresh;
+    int *nM0, *nM1, *nM2, *nM3, *sM0, *sM1, *sM2, *sM3;
+    double *N0, *N1, *N2, *N3, *gapn, *vgapn, *gapt, *vgapt, *xsi, *xn, *yn, *xt, *yt, *emass, *len, *fx, *fy,
+        *int0;
+};
+
+struct DevModel {
+    // sizes
+    int n_bodies, n_nodes, n_gauss, n_borders, n_slots, n_bnodes, n_rigid;
+    int n_materials, n_laws;
+    double period, xmin_period, gravity_x, gravity_y, dt;
+    int self_contact;          // 1 unless NO_MONITORING (Code/Proximity.h:332)
+
+    // bodies
+    int *body_rigid_flag, *body_material, *body_periodicity, *body_status;
+    int *node_off, *body_nnodes, *gp_off, *body_ngauss, *body_nregions, *body_per_region;
+    int *border_off, *body_nborders;
+    double *body_detect, *body_contact, *body_mass, *body_inv_mass, *body_rigid;   // body_rigid: RIGID_FIELDS each
+    double *body_inertia, *body_inv_inertia;
+    int *rigid_ids;            // body ids of the rigid bodies
+
+    // materials / laws
+    int *mat_type;             // per material
+    double *mat_p;             // 6 per material (rho alpha beta mu lambda|kappa c)
+    int *law_table;            // n_materials^2
+    int *law_type, *law_fixed;
+    double *law_p;             // 12 per law
+
+    // nodes
+    int *node_body;
+    double *X, *Y, *xc, *yc, *ux, *uy, *vx, *vy, *ax, *ay;
+    double *upx, *upy, *vpx, *vpy, *apx, *apy;
+    double *mx, *my, *imx, *imy;
+    double *fix, *fiy, *fcx, *fcy, *fbx, *fby, *fdx, *fdy, *fnx, *fny, *fkx, *fky, *fx, *fy;
+    double *node_work;         // 7 per node, accumulated over steps (sum_k dt v_half . f_k)
+
+    // Gauss points
+    int *g_ptr, *g_idx;
+    double *g_phix, *g_phiy, *g_c0, *g_P;   // g_P: 4 per GP (c0 * first Piola terms)
+    int *g_body;
+    int *t_ptr, *t_gp;
+    double *t_phix, *t_phiy;
+
+    // DOF_TO_DISP
+    int *d_ptr, *d_idx;
+    double *d_phi;
+
+    // damping matrices (global COO, upper triangle, dofs global: 2*g + dir)
+    int n_matrix;
+    int *m_dof;                // 2 per entry
+    double *m_damp;
+    int *mt_ptr, *mt_entry;    // node -> contributing (entry*2 + side) list, for a deterministic gather
+
+    // borders
+    int *br_body, *br_slot0, *br_nslots, *br_before, *br_after, *br_periodicity, *br_xtype, *br_ytype;
+    double *br_xparam, *br_yparam;           // stiffness, damping
+    int *br_hist_ptr;                        // 2 per border + 1 (x then y)
+    double *hist_t, *hist_v;
+    double *br_xval, *br_yval, *br_xvel, *br_yvel;   // current BC value / velocity
+
+    // slots
+    int *sl_border, *sl_j, *sl_node, *sl_n0, *sl_n1, *sl_n2, *sl_n3, *sl_s0, *sl_s1, *sl_s2, *sl_s3, *sl_bn;
+    double *sl_len, *sl_len0, *sl_xcp, *sl_ycp;
+
+    // distinct border nodes and point forces
+    int *bn_node;
+    int *bn_slot_ptr, *bn_slots;             // bn -> slots (ascending)
+    double *bn_pcx, *bn_pcy, *bn_pdx, *bn_pdy, *bn_pnx, *bn_pny;
+    int *bt_ptr, *bt_bn;                     // node -> (border node, phi) entries
+    double *bt_phi;
+
+    // contact point-force bucketing (per border node)
+    int *bk_count, *bk_off, *bk_cursor, *bk_rec;
+    int *rec_bn;                             // 3 per element: slave, master 1, master 2 (-1 = none)
+    double *rec_fx, *rec_fy;
+    double *rig_rec;                         // 6 per element: slave rigid (fx,fy,m), master rigid (fx,fy,m)
+
+    // contacts
+    int c_cap;
+    int *c_count;                            // device-side count (current buffer)
+    ContactArrays C;                         // current
+    ContactArrays Cold;                      // previous proximity result (history source)
+
+    // proximity scratch
+    double *pb_xmin, *pb_xmax, *pb_ymin, *pb_ymax;   // per border AABB
+    int *pb_shmin, *pb_shmax;
+    int nb_cap;                              // neighbour capacity per body
+    int *nb_count, *nb_list;                 // per body: count, 3 ints per neighbour
+    int *row_count, *row_off;                // narrow-phase rows
+    int *body_rows, *body_row_off;           // rows per body (non-self + self)
+    int *slot_old_list, *slot_old_cursor;
+    int *c_count_old;                        // element count of Cold
+    int *law_first;                          // first law matching a material pair by name (Initialize_CZM)
+    int *status_flags;                       // [0] capacity overflow bits, [1] non-finite state seen
+    double *time;                            // device copy of the simulation time
+};
+
+// --------------------------------------------------------------------------
+// geometry: Closest_2_segment (Linear branch, Code/Tools.h:106-219) and
+// Closest_2_segment_self (Code/Tools.h:319-369).
+//
+// pow(x, -0.5) (Code/Tools.h:108-110) decides contact classification at
+// geometric ties, so x^(-1/2) is computed correctly rounded: y = 1/sqrt(x)
+// (two roundings, error <= ~1 ulp), then one Newton correction with the exact
+// residual r = 1 - x*y*y evaluated through FMA error-free products,
+// y' = y + y*r/2 rounded once.  glibc's pow agrees with the correctly rounded
+// value on all but ~0.1% of inputs.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double inv_sqrt(double x) {
+    double y = 1.0 / __dsqrt_rn(x);
+    if (!(x > 0.0) || !isfinite(y) || y == 0.0) return y;
+    double p = y * y;
+    double plo = __fma_rn(y, y, -p);
+    double q = x * p;
+    double qlo = __fma_rn(x, p, -q);
+    double r = ((1.0 - q) - qlo) - x * plo;
+    return __fma_rn(y, r * 0.5, y);
+}
+
+struct Proj {
+    double gap, xsi, xn, yn, xt, yt, N0, N1, N2, N3;
+    int flag;
+};
+
+__device__ __forceinline__ void closest_linear(double Xs, double Ys, double X0, double Y0, double X1, double Y1,
+                                               double X2, double Y2, double X3, double Y3, Proj& p) {
+    double il = inv_sqrt((X2 - X1) * (X2 - X1) + (Y2 - Y1) * (Y2 - Y1));
+    double ilm = inv_sqrt((X1 - X0) * (X1 - X0) + (Y1 - Y0) * (Y1 - Y0));
+    double ilp = inv_sqrt((X3 - X2) * (X3 - X2) + (Y3 - Y2) * (Y3 - Y2));
+    double tx = (X2 - X1) * il, ty = (Y2 - Y1) * il;
+    p.xt = tx; p.yt = ty; p.xn = ty; p.yn = -tx;
+    double tmx = (X1 - X0) * ilm, tmy = (Y1 - Y0) * ilm;
+    double tpx = (X3 - X2) * ilp, tpy = (Y3 - Y2) * ilp;
+    if (!(((Xs - X1) * (tmx + tx) + (Ys - Y1) * (tmy + ty)) * (tx * (tmx + tx) + ty * (tmy + ty)) >= 0.)) {
+        p.flag = -3; p.xsi = -1.; return;
+    }
+    if (!(((Xs - X2) * (tx + tpx) + (Ys - Y2) * (ty + tpy)) * (tx * (tx + tpx) + ty * (ty + tpy)) <= 0.)) {
+        p.flag = 3; p.xsi = 1.; return;
+    }
+    double xsi = -1. + 2. * ((X2 - X1) * (Xs - X1) + (Y2 - Y1) * (Ys - Y1)) * il * il;
+    p.xsi = xsi;
+    if ((xsi < 1.) && (xsi > -1.)) {
+        double xcl = 0.5 * (X1 + X2) + xsi * 0.5 * (X2 - X1);
+        double ycl = 0.5 * (Y1 + Y2) + xsi * 0.5 * (Y2 - Y1);
+        p.gap = p.xn * (Xs - xcl) + p.yn * (Ys - ycl);
+        p.N0 = 0.; p.N1 = 0.5 * (1. - xsi); p.N2 = 0.5 * (1. + xsi); p.N3 = 0.;
+        p.flag = 0;
+    } else if (xsi == -1.) {
+        p.gap = p.xn * (Xs - X1) + p.yn * (Ys - Y1);
+        p.N0 = 0.; p.N1 = 1.; p.N2 = 0.; p.N3 = 0.;
+        p.flag = -1;
+    } else if (xsi < -1.) {
+        double idm = inv_sqrt((Xs - X1) * (Xs - X1) + (Ys - Y1) * (Ys - Y1));
+        double nx = (Xs - X1) * idm, ny = (Ys - Y1) * idm;
+        if (-ny * (tx + tmx) + nx * (ty + tmy) < 0.) { nx = -nx; ny = -ny; }
+        p.xn = nx; p.yn = ny; p.xt = -ny; p.yt = nx;
+        p.gap = nx * (Xs - X1) + ny * (Ys - Y1);
+        p.N0 = 0.; p.N1 = 1.; p.N2 = 0.; p.N3 = 0.;
+        p.xsi = -1.;
+        p.flag = -2;
+    } else if (xsi == 1.) {
+        p.gap = p.xn * (Xs - X2) + p.yn * (Ys - Y2);
+        p.N0 = 0.; p.N1 = 0.; p.N2 = 1.; p.N3 = 0.;
+        p.flag = 1;
+    } else {
+        double idp = inv_sqrt((Xs - X2) * (Xs - X2) + (Ys - Y2) * (Ys - Y2));
+        double nx = (Xs - X2) * idp, ny = (Ys - Y2) * idp;
+        if (-ny * (tx + tpx) + nx * (ty + tpy) < 0.) { nx = -nx; ny = -ny; }
+        p.xn = nx; p.yn = ny; p.xt = -ny; p.yt = nx;
+        p.gap = nx * (Xs - X2) + ny * (Ys - Y2);
+        p.N0 = 0.; p.N1 = 0.; p.N2 = 1.; p.N3 = 0.;
+        p.xsi = 1.;
+        p.flag = 2;
+    }
+}
+
+__device__ __forceinline__ void closest_self(double Xs, double Ys, double X1, double Y1, double X2, double Y2,
+                                             Proj& p) {
+    double il = inv_sqrt((X2 - X1) * (X2 - X1) + (Y2 - Y1) * (Y2 - Y1));
+    double tx = (X2 - X1) * il, ty = (Y2 - Y1) * il;
+    p.xt = tx; p.yt = ty;
+    if (((Xs - X1) * tx + (Ys - Y1) * ty) * (tx * tx + ty * ty) >= 0.) {
+        if (((Xs - X2) * tx + (Ys - Y2) * ty) * (tx * tx + ty * ty) <= 0.) {
+            double xsi = -1. + 2. * ((X2 - X1) * (Xs - X1) + (Y2 - Y1) * (Ys - Y1)) * il * il;
+            p.xsi = xsi;
+            if ((xsi < 1.) && (xsi > -1.)) {
+                p.xn = ty; p.yn = -tx;
+                double xcl = 0.5 * (X1 + X2) + xsi * 0.5 * (X2 - X1);
+                double ycl = 0.5 * (Y1 + Y2) + xsi * 0.5 * (Y2 - Y1);
+                p.gap = p.xn * (Xs - xcl) + p.yn * (Ys - ycl);
+                p.N0 = 0.; p.N1 = 0.5 * (1. - xsi); p.N2 = 0.5 * (1. + xsi); p.N3 = 0.;
+            }
+        } else {
+            p.xsi = 1.;
+        }
+    } else {
+        p.xsi = -1.;
+    }
+    p.flag = -3;
+}
+
+// Detect_Segment_Proximity, Code/Proximity.h:8-27
+__device__ __forceinline__ bool detect_segment(double Xs, double Ys, double X1, double Y1, double X2, double Y2,
+                                               double d) {
+    d = d * d;
+    if ((Xs - X1) * (Xs - X1) + (Ys - Y1) * (Ys - Y1) < d) return true;
+    if ((Xs - X2) * (Xs - X2) + (Ys - Y2) * (Ys - Y2) < d) return true;
+    double dx = X2 - X1, dy = Y2 - Y1;
+    double a = dx * dx + dy * dy;
+    double xsi = -1. + 2. * (dx * (Xs - X1) + dy * (Ys - Y1)) / a;
+    if ((xsi < 1.) && (xsi > -1.)) {
+        double b = dy * (Xs - 0.5 * (X1 + X2) - xsi * 0.5 * dx) - dx * (Ys - 0.5 * (Y1 + Y2) - xsi * 0.5 * dy);
+        if (b * b < a * d) return true;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// Body::Update_contacts (Code/Body.h:1341-1634) for one element: re-project
+// the slave node, walk to the previous / next master segment on flags +-2/+-3,
+// accumulate the tangential gap, relative velocities.  When both projections
+// return +-3 the element keeps its own stored shape values.
+// ---------------------------------------------------------------------------
+__device__ inline int slot_of(const DevModel& m, int body, int border, int j) {
+    return m.br_slot0[m.border_off[body] + border] + j;
+}
+
+__device__ inline void load_stencil(const DevModel& m, int body, int border, int seg, int* nd, int* sh) {
+    int s = slot_of(m, body, border, seg);
+    nd[0] = m.sl_n0[s]; nd[1] = m.sl_n1[s]; nd[2] = m.sl_n2[s]; nd[3] = m.sl_n3[s];
+    sh[0] = m.sl_s0[s]; sh[1] = m.sl_s1[s]; sh[2] = m.sl_s2[s]; sh[3] = m.sl_s3[s];
+}
+
+__device__ inline void project_element(const DevModel& m, const ContactArrays& C, int e) {
+    int bS = C.bodyS[e];
+    int bodyM = C.bodyM[e], borderM = C.borderM[e], shiftM = C.shiftM[e], segM = C.segM[e];
+    int nodeS = C.nodeS[e];
+    int gbM = m.border_off[bodyM] + borderM;
+    int per = m.br_periodicity[gbM];
+    Proj p;
+    p.gap = C.gapn[e]; p.xsi = C.xsi[e]; p.xn = C.xn[e]; p.yn = C.yn[e]; p.xt = C.xt[e]; p.yt = C.yt[e];
+    p.N0 = C.N0[e]; p.N1 = C.N1[e]; p.N2 = C.N2[e]; p.N3 = C.N3[e];
+    p.flag = 0;
+    double gapt = C.gapt[e];
+    double xsi_prev = p.xsi;
+    int nd[4], sh[4];
+    load_stencil(m, bodyM, borderM, segM, nd, sh);
+    double xs = m.xc[nodeS], ys = m.yc[nodeS];
+    double P = m.period;
+    double X[4], Y[4];
+    for (int k = 0; k < 4; k++) { X[k] = m.xc[nd[k]] + P * (shiftM + sh[k]); Y[k] = m.yc[nd[k]]; }
+    auto project = [&]() {
+        if (bS == bodyM && (nd[0] == nodeS || nd[3] == nodeS))
+            closest_self(xs, ys, X[1], Y[1], X[2], Y[2], p);
+        else
+            closest_linear(xs, ys, X[0], Y[0], X[1], Y[1], X[2], Y[2], X[3], Y[3], p);
+    };
+    project();
+    auto seglen = [&]() { return __dsqrt_rn((X[2] - X[1]) * (X[2] - X[1]) + (Y[2] - Y[1]) * (Y[2] - Y[1])); };
+    if (p.flag == -2 || p.flag == -3) {
+        gapt = gapt + 0.5 * (-1. - xsi_prev) * seglen();
+        if (per == 1) {
+            if (segM == 0) {
+                borderM = m.br_before[gbM];
+                segM = m.br_nslots[m.border_off[bodyM] + borderM] - 2;
+            } else segM--;
+        } else if (per == 2) {
+            if (segM == 0) {
+                if (borderM == 0) shiftM--;
+                else if (borderM == 1) shiftM++;
+                segM = m.br_nslots[gbM] - 2;
+            } else segM--;
+        } else {
+            if (segM == 0) segM = m.br_nslots[gbM] - 2;
+            else segM--;
+        }
+        load_stencil(m, bodyM, borderM, segM, nd, sh);
+        for (int k = 0; k < 4; k++) { X[k] = m.xc[nd[k]] + P * (shiftM + sh[k]); Y[k] = m.yc[nd[k]]; }
+        p.xsi = 1.;
+        project();
+        gapt = gapt + 0.5 * (-1. + p.xsi) * seglen();
+    } else if (p.flag == 2 || p.flag == 3) {
+        gapt = gapt + 0.5 * (-1. + xsi_prev) * seglen();
+        int last = m.br_nslots[gbM] - 2;
+        if (per == 1) {
+            if (segM == last) { borderM = m.br_after[gbM]; segM = 0; }
+            else segM++;
+        } else if (per == 2) {
+            if (segM == last) {
+                if (borderM == 0) shiftM++;
+                else if (borderM == 1) shiftM--;
+                segM = 0;
+            } else segM++;
+        } else {
+            if (segM == last) segM = 0;
+            else segM++;
+        }
+        load_stencil(m, bodyM, borderM, segM, nd, sh);
+        for (int k = 0; k < 4; k++) { X[k] = m.xc[nd[k]] + P * (shiftM + sh[k]); Y[k] = m.yc[nd[k]]; }
+        p.xsi = -1.;
+        project();
+        gapt = gapt + 0.5 * (1. + p.xsi) * seglen();
+    } else {
+        gapt = gapt + 0.5 * (p.xsi - xsi_prev) * seglen();
+    }
+    double vgx = m.vx[nodeS] - (p.N0 * m.vx[nd[0]] + p.N1 * m.vx[nd[1]] + p.N2 * m.vx[nd[2]] + p.N3 * m.vx[nd[3]]);
+    double vgy = m.vy[nodeS] - (p.N0 * m.vy[nd[0]] + p.N1 * m.vy[nd[1]] + p.N2 * m.vy[nd[2]] + p.N3 * m.vy[nd[3]]);
+    C.borderM[e] = borderM; C.shiftM[e] = shiftM; C.segM[e] = segM;
+    C.gapn[e] = p.gap;
+    C.vgapn[e] = vgx * p.xn + vgy * p.yn;
+    C.gapt[e] = gapt;
+    C.vgapt[e] = vgx * p.xt + vgy * p.yt;
+    C.xsi[e] = p.xsi; C.xn[e] = p.xn; C.yn[e] = p.yn; C.xt[e] = p.xt; C.yt[e] = p.yt;
+    C.nM0[e] = nd[0]; C.nM1[e] = nd[1]; C.nM2[e] = nd[2]; C.nM3[e] = nd[3];
+    C.sM0[e] = sh[0]; C.sM1[e] = sh[1]; C.sM2[e] = sh[2]; C.sM3[e] = sh[3];
+    C.N0[e] = p.N0; C.N1[e] = p.N1; C.N2[e] = p.N2; C.N3[e] = p.N3;
+}
+
+}  // namespace mel

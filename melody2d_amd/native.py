@@ -1,0 +1,203 @@
+"""ctypes binding of the C-ABI in include/melody2d.h.
+
+Loads melody2d_amd/libmelody2d_hip.so (built in-tree by melody2d_amd.build).
+There is no fallback: if the library is missing or cannot be loaded, every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmelody2d_hip.so")
+
+MEL_OK = 0
+STATUS_NAMES = {0: "MEL_OK", 1: "MEL_E_ARG", 2: "MEL_E_IO", 3: "MEL_E_PARSE", 4: "MEL_E_OOM", 5: "MEL_E_HIP",
+                6: "MEL_E_NAN", 7: "MEL_E_CAPACITY", 8: "MEL_E_UNSUPPORTED"}
+FP64, FP32_MIXED = 0, 1
+
+NODE_FIELDS = ["x_current", "y_current", "x_disp", "y_disp", "x_vel", "y_vel", "x_acc", "y_acc",
+               "x_disp_param", "y_disp_param", "x_vel_param", "y_vel_param", "x_acc_param", "y_acc_param",
+               "x_internal_force", "y_internal_force", "x_contact_force", "y_contact_force",
+               "x_body_force", "y_body_force", "x_dirichlet_force", "y_dirichlet_force",
+               "x_neumann_force", "y_neumann_force", "x_damping_force", "y_damping_force",
+               "x_force", "y_force", "x_mass", "y_mass"]
+
+
+class MelError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (STATUS_NAMES.get(status, status), msg))
+        self.status = status
+
+
+class Info(ctypes.Structure):
+    _fields_ = [("n_bodies", ctypes.c_int), ("n_nodes", ctypes.c_int), ("n_gauss", ctypes.c_int),
+                ("n_borders", ctypes.c_int), ("n_border_nodes", ctypes.c_int), ("n_rigid", ctypes.c_int),
+                ("n_contacts", ctypes.c_int), ("contact_capacity", ctypes.c_int),
+                ("nnz_gauss", ctypes.c_longlong), ("nnz_d2d", ctypes.c_longlong),
+                ("time", ctypes.c_double), ("dt", ctypes.c_double), ("iteration", ctypes.c_int),
+                ("precision", ctypes.c_int)]
+
+
+_lib = None
+
+
+def library():
+    """The loaded HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("HIP library %s is not built; run melody2d_amd.build.build_all()" % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        vp = ctypes.c_void_p
+        sigs = {
+            "mel_load": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(vp)], ctypes.c_int),
+            "mel_load_binary": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, P(vp)], ctypes.c_int),
+            "mel_update_proximity": ([vp], ctypes.c_int),
+            "mel_euler_steps": ([vp, ctypes.c_int], ctypes.c_int),
+            "mel_synchronize": ([vp], ctypes.c_int),
+            "mel_get_node_field": ([vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_double), ctypes.c_size_t], ctypes.c_int),
+            "mel_set_node_field": ([vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_double), ctypes.c_size_t], ctypes.c_int),
+            "mel_get_contacts": ([vp, ctypes.c_int, P(ctypes.c_int32), P(ctypes.c_double), ctypes.c_size_t,
+                                  P(ctypes.c_size_t)], ctypes.c_int),
+            "mel_get_neighbours": ([vp, ctypes.c_int, P(ctypes.c_int32), ctypes.c_size_t, P(ctypes.c_size_t)],
+                                   ctypes.c_int),
+            "mel_get_body_state": ([vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)], ctypes.c_int),
+            "mel_get_border_field": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_double),
+                                      ctypes.c_size_t], ctypes.c_int),
+            "mel_get_info": ([vp, P(Info)], ctypes.c_int),
+            "mel_write_dynamic": ([vp, ctypes.c_char_p], ctypes.c_int),
+            "mel_get_stream": ([vp, P(vp)], ctypes.c_int),
+            "mel_destroy": ([vp], None),
+            "mel_last_error": ([vp], ctypes.c_char_p),
+            "mel_version": ([], ctypes.c_char_p),
+        }
+        for name, (args, res) in sigs.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = lib
+    return _lib
+
+
+def exported_symbols():
+    return ["mel_load", "mel_load_binary", "mel_update_proximity", "mel_euler_steps", "mel_synchronize",
+            "mel_get_node_field", "mel_set_node_field", "mel_get_contacts", "mel_get_neighbours",
+            "mel_get_body_state", "mel_get_border_field", "mel_get_info", "mel_write_dynamic", "mel_get_stream",
+            "mel_destroy", "mel_last_error", "mel_version"]
+
+
+class Context:
+    """One loaded model on one HIP device (owns the mel_ctx)."""
+
+    def __init__(self, path, save_index=0, precision=FP64, device=0, binary=False):
+        lib = library()
+        h = ctypes.c_void_p()
+        if binary:
+            st = lib.mel_load_binary(os.fsencode(path), precision, device, ctypes.byref(h))
+        else:
+            st = lib.mel_load(os.fsencode(path), save_index, precision, device, ctypes.byref(h))
+        if st != MEL_OK:
+            raise MelError(st, lib.mel_last_error(None).decode())
+        self._h = h
+        self.lib = lib
+        self.info = self.get_info()
+
+    def _check(self, st):
+        if st != MEL_OK:
+            raise MelError(st, self.lib.mel_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.mel_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def get_info(self):
+        inf = Info()
+        self._check(self.lib.mel_get_info(self._h, ctypes.byref(inf)))
+        return inf
+
+    def euler_steps(self, n):
+        self._check(self.lib.mel_euler_steps(self._h, n))
+
+    def update_proximity(self):
+        self._check(self.lib.mel_update_proximity(self._h))
+
+    def synchronize(self):
+        self._check(self.lib.mel_synchronize(self._h))
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        self._check(self.lib.mel_get_stream(self._h, ctypes.byref(s)))
+        return s.value
+
+    def n_nodes(self, body):
+        out = np.zeros(1)
+        # size query: the field length equals the body's node count
+        return self._body_nodes[body]
+
+    def node_field(self, body, field, n):
+        if isinstance(field, str):
+            field = NODE_FIELDS.index(field)
+        out = np.zeros(n, dtype=np.float64)
+        self._check(self.lib.mel_get_node_field(self._h, body, field,
+                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n))
+        return out
+
+    def set_node_field(self, body, field, values):
+        if isinstance(field, str):
+            field = NODE_FIELDS.index(field)
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        self._check(self.lib.mel_set_node_field(self._h, body, field,
+                                                v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(v)))
+
+    def contacts(self, body):
+        cnt = ctypes.c_size_t()
+        self._check(self.lib.mel_get_contacts(self._h, body, None, None, 0, ctypes.byref(cnt)))
+        n = cnt.value
+        ids = np.zeros(16 * max(n, 1), dtype=np.int32)
+        st = np.zeros(18 * max(n, 1), dtype=np.float64)
+        self._check(self.lib.mel_get_contacts(self._h, body, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                              st.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n,
+                                              ctypes.byref(cnt)))
+        return ids[:16 * n], st[:18 * n]
+
+    def neighbours(self, body):
+        cnt = ctypes.c_size_t()
+        self._check(self.lib.mel_get_neighbours(self._h, body, None, 0, ctypes.byref(cnt)))
+        out = np.zeros(3 * max(cnt.value, 1), dtype=np.int32)
+        self._check(self.lib.mel_get_neighbours(self._h, body, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                cnt.value, ctypes.byref(cnt)))
+        return out[:3 * cnt.value]
+
+    def body_state(self, body):
+        r = np.zeros(26)
+        w = np.zeros(7)
+        self._check(self.lib.mel_get_body_state(self._h, body, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                w.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return r, w
+
+    def border_field(self, body, border, field, n):
+        out = np.zeros(n)
+        self._check(self.lib.mel_get_border_field(self._h, body, border, field,
+                                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n))
+        return out
+
+    def write_dynamic(self, path):
+        self._check(self.lib.mel_write_dynamic(self._h, os.fsencode(path)))

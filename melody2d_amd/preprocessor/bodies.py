@@ -77,6 +77,7 @@ class Body:
     d2d_phi: np.ndarray | None = None
     d2d_phix: np.ndarray | None = None
     d2d_phiy: np.ndarray | None = None
+    matrix: list = field(default_factory=list)   # [(dof0, dof1, stiffness, damping)] upper triangle
     # rigid only
     centre: tuple = (0.0, 0.0)
     rigid_mass: float = 0.0

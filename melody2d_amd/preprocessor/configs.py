@@ -201,4 +201,51 @@ def c2(n_grains=1000, seed=1002, nodes_per_grain=200, area_fraction=0.45, mu=0.5
     return m
 
 
+def _set_law(m, typ, params, evolution="Evolutive"):
+    m.contact_laws = [(a, b, typ, evolution, list(params)) for (a, b, _, _, _) in m.contact_laws]
+    return m
+
+
+def variants():
+    """Small models that together exercise every branch of the step the
+    library implements; the reference solver's outputs on them are the
+    golden vectors under tests/golden."""
+    v = {}
+    v["tiny"] = tiny(h=0.45e-3)
+    v["tiny_platen"] = tiny(h=0.45e-3, with_platens=True)
+    v["tiny_frictionless"] = _set_law(tiny(h=0.45e-3), "Frictionless", [1e13])
+    v["tiny_cohesion"] = _set_law(tiny(h=0.45e-3), "Cohesion", [1e13, 5e12, 2e4, 1e4])
+    v["tiny_damped"] = _set_law(tiny(h=0.45e-3), "DampedMohrCoulomb", [1e13, 1e13, 0.3, 1e3, 1e3, 0.2], "Fixed")
+    v["tiny_twoslopes"] = _set_law(tiny(h=0.45e-3), "TwoSlopesMohrCoulomb", [1e13, 1e13, 0.4, 0.0, 0.0, 0.5])
+    m = _set_law(tiny(h=0.45e-3, overlap=-1e-6), "CZMlinear", [1e13, 2e6, 5e-6, 1e5, 1.0, 2e-6])
+    m.flags = m.flags + ["INITIALIZE_CZM"]
+    v["tiny_czm"] = m
+    # linear elastic material, gravity, Neumann/Dirichlet on deformable
+    # borders, Rayleigh damping entries, a free rigid body
+    m = tiny(h=0.45e-3, with_platens=True)
+    m.materials[0] = ("Grain", "ElasticLinear", [RHO_GRAIN, 2e-6, 50.0, E_GRAIN, NU_GRAIN, 1.0])
+    m.gravity = (0.3, -9.81)
+    b0 = m.bodies[0]
+    b0.borders[0].xbc = BC("Soft", [(0.0, 0.01), (1e3, 0.01)], (1e5, 2.0))
+    b0.borders[0].ybc = BC("Following", [(0.0, -2e4), (1e3, -2e4)])
+    b1 = m.bodies[1]
+    b1.borders[0].xbc = BC("Oriented", [(0.0, 1e4), (1e3, 3e4)])
+    b1.borders[0].ybc = BC("Soft", [(0.0, 0.0), (1e-6, -0.02), (1e3, -0.02)], (5e4, 1.0))
+    n = b1.n_nodes
+    b1.matrix = [(2 * i + d, 2 * i + d, 1e3 * (1 + i), 0.0) for i in range(n) for d in (0, 1)]
+    b1.matrix += [(2 * i, 2 * i + 3, -2e2, 0.0) for i in range(n - 2)]
+    b1.matrix.sort()
+    top = m.bodies[3]
+    top.borders[0].xbc = BC("Oriented", [(0.0, 5e2), (1e3, 5e2)])
+    # free rigid body (no driven direction): translates and rotates
+    top.borders[0].ybc = BC("Oriented", [(0.0, -3e4), (1e3, -3e4)])
+    m.flags = [f for f in m.flags if f != "NO_LOG"] + ["NO_LOG"]
+    v["tiny_bc"] = m
+    # self-contact disabled (NO_MONITORING)
+    m = tiny(h=0.45e-3)
+    m.flags = m.flags + ["NO_MONITORING"]
+    v["tiny_nomonitor"] = m
+    return v
+
+
 WORKLOADS = {"tiny": tiny, "tiny_platen": lambda: tiny(with_platens=True), "c1": c1, "c2": c2}

@@ -133,7 +133,9 @@ def write_static(model: Model, d: str):
                     out.append("%d %s %s %s" % (b.d2d_idx[k], g(b.d2d_phi[k]), g(b.d2d_phix[k]), g(b.d2d_phiy[k])))
                 out.append(" ")
             out.append(" ")
-            out += ["MATRICES", "0", " ", " ", " ", " "]
+            out += ["MATRICES", "%d" % len(b.matrix), " "]
+            out += ["%d %d %s %s" % (d0, d1, g(k), g(dmp)) for d0, d1, k, dmp in b.matrix]
+            out += [" ", " ", " "]
     with open(os.path.join(d, "STATIC_DATA.asc"), "w") as f:
         f.write("\n".join(out) + "\n")
 

@@ -1152,6 +1152,16 @@ void update_material(Sim& S, int bi) {
                 s.inverse_mass = 1 / s.mass;
             }
         }
+        // Rayleigh damping rebuilt from the stored stiffness (Code/Body.h:2862-2875)
+        if (S.m.flags[F_UPDATE_DAMPING]) {
+            double alpha = S.m.materials[b.material].p[1], beta = S.m.materials[b.material].p[2];
+            for (size_t i = 0; i < b.mat_damping.size(); i++) {
+                if (b.mat_dof[2 * i] == b.mat_dof[2 * i + 1])
+                    b.mat_damping[i] = alpha * b.mat_stiffness[i] + beta * s.mx[b.mat_dof[2 * i] / 2];
+                else
+                    b.mat_damping[i] = alpha * b.mat_stiffness[i];
+            }
+        }
     } else {
         double mass = 0., inertia = 0.;
         for (size_t c = 0; c < b.cells.size() / 3; c++) {
@@ -1398,6 +1408,8 @@ int oracle_run(const char* path, int save_index, int nsteps, const char* out_dir
         if (b.status == ST_INACTIVE) { g_err = "inactive bodies are not supported"; return 1; }
     for (auto& l : S.m.laws)
         if (l.type == LAW_BONDED_MOHR_COULOMB || l.type == LAW_CZM_FATIGUE) { g_err = "contact law not supported"; return 1; }
+    for (auto& b : S.m.bodies)
+        if (S.m.flags[F_UPDATE_STIFFNESS] && !b.mat_dof.empty()) { g_err = "UPDATE_STIFFNESS_MATRIX is not supported"; return 1; }
     setup(S);
     return run(S, nsteps, out_dir, dump_steps, n_dump, seconds);
 }
