@@ -1,0 +1,179 @@
+"""Benchmark of the MELODY 2D explicit step on one MI355X.
+
+Workload: biaxial compression of 1000 polydisperse deformable grains
+(~200 MLS nodes each, ~209k nodes, ~1.04M Gauss points, neo-Hookean,
+penalty Mohr-Coulomb mu = 0.5, rigid walls, density x100), generated with
+fixed seeds on the host and handed to the library as a binary model
+container.  A "step" is one Euler_step (internal forces, contact
+re-projection and laws, BCs, integration, kinematics) plus the contact-update
+cadence (a proximity update every 10 steps), all on the device.
+
+Printed: one JSON line with whole-job node-steps/s, the roofline of the
+dominant kernel (algorithmic bytes over its HIP-event time in the timed
+region), and the CPU baseline (the FP64 OpenMP oracle on a bounded sample
+of the same workload, on this host's cores).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(kernel, info):
+    """Bytes a launch must move at minimum, each array counted once (FP64
+    values).  N = nodes, G = Gauss points, nnz = Gauss-point support
+    entries."""
+    N, G, nnz = info.n_nodes, info.n_gauss, info.nnz_gauss
+    if kernel == "block_force_update":
+        # per support entry: dphi/dx, dphi/dy (16) + local node index (2) +
+        # transposed position (2); per Gauss point: c0 (8), region (4),
+        # entry offset (2); per node: displacement parameters read + write
+        # (32), velocity parameters read + write (32), acceleration
+        # parameters and internal force written (32), masses, inverse masses
+        # and damping force read (48), node id and transposed offset (6)
+        return 20 * nnz + 14 * G + 150 * N
+    if kernel == "kinematics":
+        # per border node: DOF_TO_DISP row (index 4 + phi 8 per entry, read
+        # with the 6 parameters of each entry's node), 8 outputs
+        return None
+    return None
+
+
+def cpu_baseline(model_path, n_nodes, steps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+    lib.oracle_run.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                               ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_double)]
+    lib.oracle_last_error.restype = ctypes.c_char_p
+    sec = ctypes.c_double()
+    r = lib.oracle_run(os.fsencode(model_path), -1, steps, None, None, 0, threads, ctypes.byref(sec))
+    if r != 0:
+        raise RuntimeError(lib.oracle_last_error().decode())
+    return n_nodes * steps / sec.value, sec.value
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--grains", type=int, default=1000)
+    ap.add_argument("--cpu-steps", type=int, default=2000)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--breakdown", action="store_true", help="also time every kernel in a separate window")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (samples and grains would be "
+              "sharded across GPUs by HBM footprint; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    from melody2d_amd import native
+    from melody2d_amd.preprocessor import c2
+    from melody2d_amd.preprocessor.binary import write_binary
+
+    t0 = time.time()
+    model = c2(n_grains=args.grains)
+    log("generated %d bodies, %d nodes, %d Gauss points in %.1fs" % (len(model.bodies), model.n_nodes,
+                                                                     model.n_gauss, time.time() - t0))
+    tmpdir = tempfile.mkdtemp(prefix="mel_bench_")
+    path = os.path.join(tmpdir, "c2.bin")
+    write_binary(model, path)
+    t0 = time.time()
+    ctx = native.Context(path, binary=True)
+    info = ctx.get_info()
+    log("loaded on device in %.1fs: nodes %d, gauss %d, nnz %d, contacts %d" % (
+        time.time() - t0, info.n_nodes, info.n_gauss, info.nnz_gauss, info.n_contacts))
+
+    ctx.euler_steps(args.warmup)
+    ctx.synchronize()
+    log("warmup done")
+    dominant = ["block_force_update"]
+    ctx.set_kernel_timing(dominant)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.euler_steps(args.steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    kt = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    sec = t1 - t0
+    info = ctx.get_info()
+    value = info.n_nodes * args.steps / sec
+    log("timed %d steps in %.3fs: %.3e node-steps/s, %.3f ms/step" % (args.steps, sec, value, 1e3 * sec / args.steps))
+
+    roof = None
+    best = max(dominant, key=lambda k: kt[k][0])
+    ms, n = kt[best]
+    if n:
+        avg_s = ms / n * 1e-3
+        b = algorithmic_bytes(best, info)
+        achieved = b / avg_s / 1e9
+        roof = {"kernel": best, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes": b}
+        for k in dominant:
+            log("kernel %-18s %8.2f us/launch" % (k, kt[k][0] / max(kt[k][1], 1) * 1e3))
+
+    breakdown = None
+    if args.breakdown:
+        ctx.set_kernel_timing(True)
+        ctx.euler_steps(min(args.steps, 50))
+        kt2 = ctx.kernel_timing()
+        ctx.set_kernel_timing(False)
+        breakdown = {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in kt2.items() if v[1]}
+        log("per-launch us: " + json.dumps(breakdown))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        t0 = time.time()
+        v, s = cpu_baseline(path, info.n_nodes, args.cpu_steps, threads)
+        log("cpu baseline: %.3e node-steps/s over %d steps (%.1fs stepping, %.1fs total)" % (
+            v, args.cpu_steps, s, time.time() - t0))
+        cpu = {"value": round(v, 1), "unit": "node-steps/s", "cores": threads, "kind": "port",
+               "sample": "%d Euler steps of the same %d-node workload after its initialisation, "
+                         "FP64 OpenMP oracle" % (args.cpu_steps, info.n_nodes)}
+    ctx.close()
+    os.remove(path)
+    os.rmdir(tmpdir)
+
+    out = {
+        "metric": "node-steps/s, FP32-mixed vs FP64 at ≤1e-5 rel force err; p50/p99 job latency",
+        "value": round(value, 1),
+        "unit": "node-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * sec / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": "biaxial compression, %d deformable grains + 4 rigid walls (FP64 drop-in)" % args.grains,
+                   "nodes": info.n_nodes, "gauss_points": info.n_gauss, "support_entries": info.nnz_gauss,
+                   "contacts": info.n_contacts, "precision": "fp64", "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if breakdown:
+        out["kernel_us"] = breakdown
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

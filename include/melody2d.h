@@ -116,6 +116,18 @@ mel_status mel_write_dynamic(mel_ctx* ctx, const char* path);
  * event timing by the caller */
 mel_status mel_get_stream(mel_ctx* ctx, void** stream);
 
+/* Optional HIP-event instrumentation of kernel launches of the step (and of
+ * each contact update as a whole).  `on` = 0 off, -1 every kernel, otherwise a
+ * bit mask of kernel ids.  Setting it resets the accumulators;
+ * mel_get_kernel_timing waits for the stream and returns the summed elapsed
+ * time and launch count of kernel `kernel` (0 <= kernel < mel_kernel_count()).
+ * No reference counterpart (the reference has no timers); used by the
+ * benchmark's roofline figures. */
+mel_status mel_set_kernel_timing(mel_ctx* ctx, int on);
+mel_status mel_get_kernel_timing(mel_ctx* ctx, int kernel, double* total_ms, long long* launches);
+int mel_kernel_count(void);
+const char* mel_kernel_name(int kernel);
+
 void mel_destroy(mel_ctx* ctx);
 const char* mel_last_error(const mel_ctx* ctx);
 const char* mel_version(void);

@@ -313,138 +313,6 @@ __global__ void k_bc_point_forces(DevModel m) {
     m.bn_pnx[bn] = cnx; m.bn_pny[bn] = cny; m.bn_pdx[bn] = cdx; m.bn_pdy[bn] = cdy;
 }
 
-// ---------------------------------------------------------------------------
-// Body::Update_internal_forces, per Gauss point part (Code/Body.h:969-1048,
-// Code/Model.h:19-73): stress terms with c0 kept separate so that the node
-// gather reproduces c0 * (phix * A + phiy * B) exactly.
-// ---------------------------------------------------------------------------
-__global__ void k_gp_stress(DevModel m) {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= m.n_gauss) return;
-    int b = m.g_body[g];
-    int mat = m.body_material[b];
-    double Mu = m.mat_p[6 * mat + 3], K2 = m.mat_p[6 * mat + 4];
-    int k0 = m.g_ptr[g], k1 = m.g_ptr[g + 1];
-    double A, B, Cc, D;
-    if (m.mat_type[mat] == 0) {
-        double E11 = 0., E12 = 0., E22 = 0.;
-        for (int k = k0; k < k1; k++) {
-            int j = m.g_idx[k];
-            double fx = m.g_phix[k], fy = m.g_phiy[k];
-            double ux = m.upx[j], uy = m.upy[j];
-            E11 += fx * ux;
-            E12 += fx * uy + fy * ux;
-            E22 += fy * uy;
-        }
-        E12 = 0.5 * E12;
-        double J = 1. + E11 + E22;
-        double S33 = K2 * (J - 1.);
-        double M2 = 2 * Mu;
-        A = S33 + M2 * E11;         // S11
-        B = M2 * E12;               // S12
-        Cc = B;                     // S12
-        D = S33 + M2 * E22;         // S22
-    } else {
-        double F11 = 1., F12 = 0., F21 = 0., F22 = 1.;
-        for (int k = k0; k < k1; k++) {
-            int j = m.g_idx[k];
-            double fx = m.g_phix[k], fy = m.g_phiy[k];
-            double ux = m.upx[j], uy = m.upy[j];
-            F11 += fx * ux;
-            F12 += fy * ux;
-            F21 += fx * uy;
-            F22 += fy * uy;
-        }
-        double F11F12 = F11 * F11 + F12 * F12;
-        double F21F22 = F21 * F21 + F22 * F22;
-        double J = F11 * F22 - F12 * F21;
-        double invJ = 1. / J;
-        double K = K2 * (J - 1.);
-        double MuJ53 = Mu * pow(J, -1.666666666667);
-        double trB3 = 0.333333333333 * (F11F12 + F21F22 + 1.);
-        double s11 = K + MuJ53 * (F11F12 - trB3);
-        double s22 = K + MuJ53 * (F21F22 - trB3);
-        double s12 = MuJ53 * (F11 * F21 + F12 * F22);
-        double S11 = (F12 * (F12 * s22 - F22 * s12) - F22 * (F12 * s12 - F22 * s11)) * invJ;
-        double S22 = (F11 * (F11 * s22 - F21 * s12) - F21 * (F11 * s12 - F21 * s11)) * invJ;
-        double S12 = (F21 * (F12 * s12 - F22 * s11) - F11 * (F12 * s22 - F22 * s12)) * invJ;
-        A = F11 * S11 + F12 * S12;
-        B = F12 * S22 + F11 * S12;
-        Cc = F21 * S11 + F22 * S12;
-        D = F22 * S22 + F21 * S12;
-    }
-    double4 out = make_double4(A, B, Cc, D);
-    reinterpret_cast<double4*>(m.g_P)[g] = out;
-}
-
-// ---------------------------------------------------------------------------
-// Node-centric gather + Sum_up_forces + Apply_Newton + Apply_Euler
-// (Code/Body.h:1975-2068, Code/Node.h:212-245).  Internal forces are summed
-// per region in Gauss-point order, then across regions, as the per-region
-// accumulators of Code/Body.h:996-997 / Code/Node.h:216-222 do.
-// ---------------------------------------------------------------------------
-__global__ void k_node_update(DevModel m) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= m.n_nodes) return;
-    int b = m.node_body[j];
-    if (m.body_rigid_flag[b] || m.body_status[b] == 1) return;
-    int gp0 = m.gp_off[b];
-    int per = m.body_per_region[b];
-    double tx = 0., ty = 0., rx = 0., ry = 0.;
-    int cur = -1;
-    for (int e = m.t_ptr[j]; e < m.t_ptr[j + 1]; e++) {
-        int g = m.t_gp[e];
-        int r = (g - gp0) / per;
-        if (r != cur) {
-            if (cur >= 0) { tx += rx; ty += ry; }
-            rx = 0.; ry = 0.; cur = r;
-        }
-        double4 P = reinterpret_cast<const double4*>(m.g_P)[g];
-        double c0 = m.g_c0[g];
-        double fx = m.t_phix[e], fy = m.t_phiy[e];
-        rx -= c0 * (fx * P.x + fy * P.y);
-        ry -= c0 * (fx * P.z + fy * P.w);
-    }
-    if (cur >= 0) { tx += rx; ty += ry; }
-    // point forces (contact, Dirichlet, Neumann) through DOF_TO_DISP rows
-    double cx = 0., cy = 0., dx = 0., dy = 0., nx = 0., ny = 0.;
-    for (int e = m.bt_ptr[j]; e < m.bt_ptr[j + 1]; e++) {
-        int bn = m.bt_bn[e];
-        double f = m.bt_phi[e];
-        cx += m.bn_pcx[bn] * f; cy += m.bn_pcy[bn] * f;
-        dx += m.bn_pdx[bn] * f; dy += m.bn_pdy[bn] * f;
-        nx += m.bn_pnx[bn] * f; ny += m.bn_pny[bn] * f;
-    }
-    double gx = m.gravity_x, gy = m.gravity_y;
-    if (m.body_status[b] == 2) { gx = 0.; gy = 0.; }
-    double bx = m.mx[j] * gx, by = m.my[j] * gy;
-    double kx = m.fkx[j], ky = m.fky[j];
-    double fx = tx + cx + 0. + bx + dx + nx + kx + 0.;
-    double fy = ty + cy + 0. + by + dy + ny + ky + 0.;
-    double apx = fx * m.imx[j], apy = fy * m.imy[j];
-    double dt = m.dt;
-    double vpx = m.vpx[j], vpy = m.vpy[j];
-    double hx = vpx + 0.5 * dt * apx, hy = vpy + 0.5 * dt * apy;
-    double* w = m.node_work + 7 * (size_t)j;
-    w[W_INTERNAL] += dt * (hx * tx + hy * ty);
-    w[W_BODY] += dt * (hx * bx + hy * by);
-    w[W_DIRICHLET] += dt * (hx * dx + hy * dy);
-    w[W_NEUMANN] += dt * (hx * nx + hy * ny);
-    w[W_DAMPING] += dt * (hx * kx + hy * ky);
-    vpx = vpx + dt * apx;
-    vpy = vpy + dt * apy;
-    m.upx[j] = m.upx[j] + dt * vpx;
-    m.upy[j] = m.upy[j] + dt * vpy;
-    m.vpx[j] = vpx; m.vpy[j] = vpy;
-    m.apx[j] = apx; m.apy[j] = apy;
-    m.fix[j] = tx; m.fiy[j] = ty;
-    m.fcx[j] = cx; m.fcy[j] = cy;
-    m.fdx[j] = dx; m.fdy[j] = dy;
-    m.fnx[j] = nx; m.fny[j] = ny;
-    m.fbx[j] = bx; m.fby[j] = by;
-    m.fx[j] = fx; m.fy[j] = fy;
-}
-
 // Body::Update_damping_forces (Code/Body.h:1057-1154): node gather over the
 // COO entries in entry order, reading velocity parameters.
 __global__ void k_damping(DevModel m) {
@@ -467,26 +335,6 @@ __global__ void k_damping(DevModel m) {
     }
     m.fkx[j] = fx;
     m.fky[j] = fy;
-}
-
-// Body::Update_kinematics + Update_current_positions, deformable
-// (Code/Body.h:2176-2212, Code/Node.h:233-237)
-__global__ void k_kinematics(DevModel m) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m.n_nodes) return;
-    int b = m.node_body[i];
-    if (m.body_rigid_flag[b] || m.body_status[b] == 1) return;
-    double dx = 0., dy = 0., vx = 0., vy = 0., ax = 0., ay = 0.;
-    for (int k = m.d_ptr[i]; k < m.d_ptr[i + 1]; k++) {
-        int j = m.d_idx[k];
-        double f = m.d_phi[k];
-        dx += f * m.upx[j]; dy += f * m.upy[j];
-        vx += f * m.vpx[j]; vy += f * m.vpy[j];
-        ax += f * m.apx[j]; ay += f * m.apy[j];
-    }
-    m.ux[i] = dx; m.uy[i] = dy; m.vx[i] = vx; m.vy[i] = vy; m.ax[i] = ax; m.ay[i] = ay;
-    m.xc[i] = m.X[i] + dx;
-    m.yc[i] = m.Y[i] + dy;
 }
 
 // ---------------------------------------------------------------------------

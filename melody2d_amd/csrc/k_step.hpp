@@ -14,10 +14,7 @@ __global__ void k_bucket_fill(DevModel m);
 __global__ void k_bucket_sum(DevModel m);
 __global__ void k_bc_eval(DevModel m, int plus_dt);
 __global__ void k_bc_point_forces(DevModel m);
-__global__ void k_gp_stress(DevModel m);
-__global__ void k_node_update(DevModel m);
 __global__ void k_damping(DevModel m);
-__global__ void k_kinematics(DevModel m);
 __global__ void k_rigid_forces(DevModel m);
 __global__ void k_rigid_step(DevModel m);
 __global__ void k_rigid_nodes(DevModel m, int with_forces);

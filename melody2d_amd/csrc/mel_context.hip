@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/melody2d.h"
+#include "k_block.hpp"
 #include "k_step.hpp"
 #include "mel_device.hpp"
 #include "model_io.hpp"
@@ -37,10 +38,61 @@ struct CapacityError {
 
 inline int blocks(long long n, int b = BLOCK) { return (int)std::max<long long>(1, (n + b - 1) / b); }
 
+#define TIMED(ctx, id, launch)                     \
+    do {                                           \
+        if ((ctx).timing && ((ctx).timing_mask >> (id) & 1u)) { \
+            int ev_ = (ctx).ev_begin(id);          \
+            launch;                                \
+            (ctx).ev_end(ev_);                     \
+        } else {                                   \
+            launch;                                \
+        }                                          \
+    } while (0)
+
 }  // namespace
+
+// kernels timed individually by the optional event instrumentation
+enum KernelId {
+    KT_BORDER_LENGTH = 0, KT_CONTACT_PROJECT, KT_BC_POINT_FORCES, KT_DAMPING, KT_CONTACT_LAW, KT_BUCKET_FILL,
+    KT_BUCKET_SUM, KT_RIGID_FORCES, KT_BLOCK_UPDATE, KT_BC_EVAL, KT_RIGID_STEP, KT_KINEMATICS,
+    KT_RIGID_NODES, KT_ADVANCE_TIME, KT_PROXIMITY, KT_COUNT
+};
+static const char* kKernelNames[KT_COUNT] = {
+    "border_length", "contact_project", "bc_point_forces", "damping", "contact_law", "bucket_fill", "bucket_sum",
+    "rigid_forces", "block_force_update", "bc_eval", "rigid_step", "kinematics", "rigid_nodes",
+    "advance_time", "proximity_update"};
 
 struct mel_ctx {
     Model model;
+    // event instrumentation (off by default)
+    bool timing = false;
+    unsigned timing_mask = 0;     // bit k set: kernel k is instrumented
+    struct EvPair { int id; hipEvent_t a, b; };
+    std::vector<EvPair> pending;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+    double kt_ms[KT_COUNT] = {0};
+    long long kt_n[KT_COUNT] = {0};
+    int ev_begin(int id) {
+        std::pair<hipEvent_t, hipEvent_t> p;
+        if (!pool.empty()) { p = pool.back(); pool.pop_back(); }
+        else { HIP_OK(hipEventCreate(&p.first)); HIP_OK(hipEventCreate(&p.second)); }
+        pending.push_back({id, p.first, p.second});
+        HIP_OK(hipEventRecord(p.first, stream));
+        return (int)pending.size() - 1;
+    }
+    void ev_end(int k) { HIP_OK(hipEventRecord(pending[k].b, stream)); }
+    void harvest() {
+        if (pending.empty()) return;
+        HIP_OK(hipStreamSynchronize(stream));
+        for (auto& e : pending) {
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, e.a, e.b));
+            kt_ms[e.id] += ms;
+            kt_n[e.id]++;
+            pool.push_back({e.a, e.b});
+        }
+        pending.clear();
+    }
     DevModel d{};
     hipStream_t stream = nullptr;
     int device = 0;
@@ -62,6 +114,12 @@ struct mel_ctx {
     double next_contact_update = 0.0;
     int iteration = 0;
     ContactArrays bufA{}, bufB{};
+    BlockArrays blocks_dev{};
+    size_t block_lds = 0;
+    int* d_kin_all = nullptr;     // every deformable node (kinematics on request)
+    int n_kin_all = 0;
+    bool outputs_stale = false;   // interior kinematics / force fields lag the last steps
+    std::vector<int> h_blk_body;  // body of each block (work sums)
 
     template <typename T>
     T* alloc(size_t n) {
@@ -86,6 +144,8 @@ struct mel_ctx {
     }
     ~mel_ctx() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& e : pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+        for (auto& e : pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         for (void* p : allocs) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -212,6 +272,209 @@ void check_flags(mel_ctx& c) {
     }
 }
 
+
+size_t block_lds_size(const BlockArrays& B) {
+    auto r = [](size_t b) { return (b + 15) & ~size_t(15); };
+    return 2 * r(8 * (size_t)B.max_touch) + 2 * r(8 * (size_t)B.e_cap) + r(32 * (size_t)B.ch_cap) +
+           r(8 * (size_t)B.ch_cap) + r(4 * (size_t)B.ch_cap) + 3 * r(2 * (size_t)B.e_cap) +
+           r(2 * ((size_t)B.max_own + 1)) + r(2 * ((size_t)B.ch_cap + 1));
+}
+
+// Node blocks of the fused internal-force kernel: whole bodies when they fit,
+// otherwise groups of nodes in Morton order; chunked Gauss-point streams and
+// the transposed (node -> entry position) index per chunk.
+void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<int>& g_idx,
+                  const std::vector<double>& g_phix, const std::vector<double>& g_phiy,
+                  const std::vector<double>& g_c0, const std::vector<int>& t_ptr, const std::vector<int>& t_gp,
+                  const std::vector<int>& bn_node, const std::vector<int>& node_off, const std::vector<int>& gp_off,
+                  const std::vector<int>& perreg) {
+    Model& m = c.model;
+    const int ECAP = 2560, CHCAP = 128;
+    int N = node_off.back();
+    std::vector<int> blk_body, blk_own_off, blk_nown, blk_touch_off, blk_ntouch, blk_chunk_off, blk_nchunk;
+    std::vector<int> own_nodes, touch_nodes;
+    std::vector<int> ch_gp_start, ch_ngp, ch_ne, ch_tptr_off, ch_tpos_off, ch_gptr_off;
+    std::vector<long long> ch_e_start;
+    std::vector<double> bgp_c0, be_fx, be_fy, blk_work;
+    std::vector<int> bgp_region;
+    std::vector<uint16_t> gptr, be_idx, tptr, tpos;
+    std::vector<int> local(N, -1), blk_split, split_pos;
+    int max_touch = 1, max_own = 1;
+    // MELODY2D_BLOCK_NODES lowers the nodes per block (exercises the split
+    // path on small models); default and maximum BLOCK_MAX_OWN
+    int own_cap = BLOCK_MAX_OWN;
+    if (const char* e = std::getenv("MELODY2D_BLOCK_NODES")) {
+        int v = std::atoi(e);
+        if (v >= 1 && v < BLOCK_MAX_OWN) own_cap = v;
+    }
+    for (size_t b = 0; b < m.bodies.size(); b++) {
+        const Body& Bd = m.bodies[b];
+        if (Bd.rigid) continue;
+        int n = Bd.n_nodes, o = node_off[b];
+        std::vector<std::vector<int>> groups;
+        if (n <= own_cap) {
+            std::vector<int> g(n);
+            for (int i = 0; i < n; i++) g[i] = o + i;
+            groups.push_back(g);
+        } else {
+            double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+            for (int i = 0; i < n; i++) {
+                xmin = std::min(xmin, Bd.X[i]); xmax = std::max(xmax, Bd.X[i]);
+                ymin = std::min(ymin, Bd.Y[i]); ymax = std::max(ymax, Bd.Y[i]);
+            }
+            double sc = std::max(xmax - xmin, ymax - ymin);
+            if (sc <= 0) sc = 1;
+            std::vector<std::pair<uint64_t, int>> key(n);
+            for (int i = 0; i < n; i++) {
+                uint32_t qx = (uint32_t)std::min(65535.0, (Bd.X[i] - xmin) / sc * 65535.0);
+                uint32_t qy = (uint32_t)std::min(65535.0, (Bd.Y[i] - ymin) / sc * 65535.0);
+                uint64_t z = 0;
+                for (int bit = 0; bit < 16; bit++) {
+                    z |= (uint64_t)((qx >> bit) & 1u) << (2 * bit);
+                    z |= (uint64_t)((qy >> bit) & 1u) << (2 * bit + 1);
+                }
+                key[i] = {z, i};
+            }
+            std::sort(key.begin(), key.end());
+            for (int s0 = 0; s0 < n; s0 += own_cap) {
+                std::vector<int> g;
+                for (int i = s0; i < std::min(n, s0 + own_cap); i++) g.push_back(o + key[i].second);
+                std::sort(g.begin(), g.end());
+                groups.push_back(g);
+            }
+        }
+        bool first = true;
+        for (auto& owned : groups) {
+            int blk = (int)blk_body.size();
+            blk_body.push_back((int)b);
+            blk_split.push_back(groups.size() > 1 ? 1 : 0);
+            if (groups.size() > 1)
+                for (size_t i = 0; i < owned.size(); i++) split_pos.push_back((int)(own_nodes.size() + i));
+            blk_own_off.push_back((int)own_nodes.size());
+            blk_nown.push_back((int)owned.size());
+            max_own = std::max(max_own, (int)owned.size());
+            std::vector<int> gps;
+            for (int j : owned)
+                for (int e = t_ptr[j]; e < t_ptr[j + 1]; e++) gps.push_back(t_gp[e]);
+            std::sort(gps.begin(), gps.end());
+            gps.erase(std::unique(gps.begin(), gps.end()), gps.end());
+            std::vector<int> touched = owned;
+            for (size_t i = 0; i < owned.size(); i++) local[owned[i]] = (int)i;
+            for (int g : gps)
+                for (int k = g_ptr[g]; k < g_ptr[g + 1]; k++) {
+                    int j = g_idx[k];
+                    if (local[j] < 0) { local[j] = (int)touched.size(); touched.push_back(j); }
+                }
+            if (touched.size() > 65535) throw CapacityError{"node block touches more than 65535 nodes"};
+            max_touch = std::max(max_touch, (int)touched.size());
+            blk_touch_off.push_back((int)touch_nodes.size());
+            blk_ntouch.push_back((int)touched.size());
+            own_nodes.insert(own_nodes.end(), owned.begin(), owned.end());
+            touch_nodes.insert(touch_nodes.end(), touched.begin(), touched.end());
+            blk_chunk_off.push_back((int)ch_ngp.size());
+            int nown = (int)owned.size();
+            size_t q = 0;
+            while (q < gps.size()) {
+                size_t q1 = q;
+                int ne = 0;
+                while (q1 < gps.size() && (int)(q1 - q) < CHCAP) {
+                    int cnt = g_ptr[gps[q1] + 1] - g_ptr[gps[q1]];
+                    if (cnt > ECAP) throw CapacityError{"Gauss point support larger than the chunk capacity"};
+                    if (ne + cnt > ECAP) break;
+                    ne += cnt;
+                    q1++;
+                }
+                ch_gp_start.push_back((int)bgp_c0.size());
+                ch_ngp.push_back((int)(q1 - q));
+                ch_ne.push_back(ne);
+                ch_e_start.push_back((long long)be_idx.size());
+                ch_gptr_off.push_back((int)gptr.size());
+                std::vector<std::vector<uint16_t>> lst(nown);
+                int pos = 0;
+                gptr.push_back(0);
+                for (size_t a = q; a < q1; a++) {
+                    int g = gps[a];
+                    bgp_c0.push_back(g_c0[g]);
+                    bgp_region.push_back((g - gp_off[b]) / perreg[b]);
+                    for (int k = g_ptr[g]; k < g_ptr[g + 1]; k++) {
+                        int li = local[g_idx[k]];
+                        be_idx.push_back((uint16_t)li);
+                        be_fx.push_back(g_phix[k]);
+                        be_fy.push_back(g_phiy[k]);
+                        if (li < nown) lst[li].push_back((uint16_t)pos);
+                        pos++;
+                    }
+                    gptr.push_back((uint16_t)pos);
+                }
+                ch_tptr_off.push_back((int)tptr.size());
+                ch_tpos_off.push_back((int)tpos.size());
+                int acc = 0;
+                tptr.push_back(0);
+                for (int i = 0; i < nown; i++) {
+                    acc += (int)lst[i].size();
+                    tptr.push_back((uint16_t)acc);
+                    tpos.insert(tpos.end(), lst[i].begin(), lst[i].end());
+                }
+                q = q1;
+            }
+            blk_nchunk.push_back((int)ch_ngp.size() - blk_chunk_off.back());
+            for (int j : touched) local[j] = -1;
+            std::vector<double> w(14, 0.);
+            if (first)
+                for (int k = 0; k < 7; k++)
+                    if (k != W_CONTACT) w[2 * k] = Bd.works[k];
+            first = false;
+            blk_work.insert(blk_work.end(), w.begin(), w.end());
+            c.h_blk_body.push_back((int)b);
+            (void)blk;
+        }
+    }
+    std::vector<int> kin, kin_all;
+    for (int nd : bn_node)
+        if (!m.bodies[c.d.node_body_host[nd]].rigid) kin.push_back(nd);
+    for (size_t b = 0; b < m.bodies.size(); b++)
+        if (!m.bodies[b].rigid)
+            for (int i = 0; i < m.bodies[b].n_nodes; i++) kin_all.push_back(node_off[b] + i);
+    BlockArrays& B = c.blocks_dev;
+    B.n_blocks = (int)blk_body.size();
+    B.max_touch = max_touch;
+    B.max_own = max_own;
+    B.e_cap = ECAP;
+    B.ch_cap = CHCAP;
+    B.blk_body = c.upload(blk_body); B.blk_own_off = c.upload(blk_own_off); B.blk_nown = c.upload(blk_nown);
+    B.blk_touch_off = c.upload(blk_touch_off); B.blk_ntouch = c.upload(blk_ntouch);
+    B.blk_chunk_off = c.upload(blk_chunk_off); B.blk_nchunk = c.upload(blk_nchunk);
+    B.own_nodes = c.upload(own_nodes); B.touch_nodes = c.upload(touch_nodes);
+    B.ch_gp_start = c.upload(ch_gp_start); B.ch_ngp = c.upload(ch_ngp); B.ch_ne = c.upload(ch_ne);
+    B.ch_tptr_off = c.upload(ch_tptr_off); B.ch_tpos_off = c.upload(ch_tpos_off);
+    B.ch_gptr_off = c.upload(ch_gptr_off); B.ch_e_start = c.upload(ch_e_start);
+    B.bgp_c0 = c.upload(bgp_c0); B.bgp_region = c.upload(bgp_region);
+    B.gptr = c.upload(gptr); B.be_idx = c.upload(be_idx); B.tptr = c.upload(tptr); B.tpos = c.upload(tpos);
+    B.be_fx = c.upload(be_fx); B.be_fy = c.upload(be_fy);
+    B.blk_work = c.upload(blk_work);
+    B.blk_split = c.upload(blk_split);
+    B.split_pos = c.upload(split_pos);
+    B.n_split = (int)split_pos.size();
+    B.stage_u = B.n_split ? c.alloc<double>(2 * own_nodes.size()) : nullptr;
+    B.kin_nodes = c.upload(kin);
+    B.n_kin = (int)kin.size();
+    c.d_kin_all = c.upload(kin_all);
+    c.n_kin_all = (int)kin_all.size();
+    c.block_lds = block_lds_size(B);
+    if (c.block_lds > 160 * 1024) throw CapacityError{"node block needs more LDS than a workgroup has"};
+    HIP_OK(hipFuncSetAttribute((const void*)k_block_force_update, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)c.block_lds));
+}
+
+// interior kinematics and force fields for output, after steps have run
+void refresh_outputs(mel_ctx& c) {
+    if (!c.outputs_stale) return;
+    if (c.n_kin_all > 0) k_kinematics_list<<<blocks(c.n_kin_all), BLOCK, 0, c.stream>>>(c.d, c.d_kin_all, c.n_kin_all);
+    k_node_outputs<<<blocks(c.d.n_nodes), BLOCK, 0, c.stream>>>(c.d);
+    HIP_OK(hipGetLastError());
+    c.outputs_stale = false;
+}
+
 // Update_proximity (Code/Proximity.h:48-473)
 void update_proximity(mel_ctx& c) {
     DevModel& d = c.d;
@@ -261,24 +524,36 @@ void euler_step(mel_ctx& c) {
     DevModel& d = c.d;
     hipStream_t s = c.stream;
     int nc = c.n_contacts;
-    k_border_length<<<blocks(d.n_slots), BLOCK, 0, s>>>(d);
-    if (nc > 0) k_contact_project<<<blocks(nc), BLOCK, 0, s>>>(d);
-    k_bc_point_forces<<<blocks(d.n_bnodes), BLOCK, 0, s>>>(d);
-    if (d.n_matrix > 0) k_damping<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d);
+    // Initialize_contact_forces + Update_borders + Update_contacts
+    TIMED(c, KT_BORDER_LENGTH, (k_border_length<<<blocks(d.n_slots), BLOCK, 0, s>>>(d)));
+    if (nc > 0) TIMED(c, KT_CONTACT_PROJECT, (k_contact_project<<<blocks(nc), BLOCK, 0, s>>>(d)));
+    // Update_bc_forces, Update_body_forces (in node_update), Update_damping_forces
+    TIMED(c, KT_BC_POINT_FORCES, (k_bc_point_forces<<<blocks(d.n_bnodes), BLOCK, 0, s>>>(d)));
+    if (d.n_matrix > 0) TIMED(c, KT_DAMPING, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d)));
+    // Update_contact_forces + Send_contact_forces
     if (nc > 0) {
-        k_contact_law<<<blocks(nc), BLOCK, 0, s>>>(d);
-        k_bucket_fill<<<blocks(3LL * nc), BLOCK, 0, s>>>(d);
+        TIMED(c, KT_CONTACT_LAW, (k_contact_law<<<blocks(nc), BLOCK, 0, s>>>(d)));
+        TIMED(c, KT_BUCKET_FILL, (k_bucket_fill<<<blocks(3LL * nc), BLOCK, 0, s>>>(d)));
     }
-    k_bucket_sum<<<blocks(d.n_bnodes), BLOCK, 0, s>>>(d);
-    if (d.n_rigid > 0) k_rigid_forces<<<d.n_rigid, 256, 0, s>>>(d);   // one block per rigid body
-    if (d.n_gauss > 0) k_gp_stress<<<blocks(d.n_gauss), BLOCK, 0, s>>>(d);
-    k_node_update<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d);
-    k_bc_eval<<<blocks(d.n_borders), BLOCK, 0, s>>>(d, 1);
-    if (d.n_rigid > 0) k_rigid_step<<<blocks(d.n_bodies), BLOCK, 0, s>>>(d);
-    k_kinematics<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d);
-    if (d.n_rigid > 0) k_rigid_nodes<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d, 1);
-    k_advance_time<<<1, 64, 0, s>>>(d);
+    TIMED(c, KT_BUCKET_SUM, (k_bucket_sum<<<blocks(d.n_bnodes), BLOCK, 0, s>>>(d)));
+    if (d.n_rigid > 0) TIMED(c, KT_RIGID_FORCES, (k_rigid_forces<<<d.n_rigid, 256, 0, s>>>(d)));
+    // Update_internal_forces, Sum_up_forces, Apply_Newton, Apply_Euler
+    if (c.blocks_dev.n_blocks > 0)
+        TIMED(c, KT_BLOCK_UPDATE,
+              (k_block_force_update<<<c.blocks_dev.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, c.blocks_dev)));
+    if (c.blocks_dev.n_split > 0)
+        k_commit_split<<<blocks(c.blocks_dev.n_split), BLOCK, 0, s>>>(d, c.blocks_dev);
+    // Update_bc(t + dt), rigid bodies, Update_kinematics, Update_current_positions
+    TIMED(c, KT_BC_EVAL, (k_bc_eval<<<blocks(d.n_borders), BLOCK, 0, s>>>(d, 1)));
+    if (d.n_rigid > 0) TIMED(c, KT_RIGID_STEP, (k_rigid_step<<<blocks(d.n_bodies), BLOCK, 0, s>>>(d)));
+    // kinematics of border nodes only: the step reads positions and
+    // velocities of border nodes; interior ones are rebuilt on request
+    if (c.blocks_dev.n_kin > 0)
+        TIMED(c, KT_KINEMATICS, (k_kinematics_list<<<blocks(c.blocks_dev.n_kin), BLOCK, 0, s>>>(d, c.blocks_dev.kin_nodes, c.blocks_dev.n_kin)));
+    if (d.n_rigid > 0) TIMED(c, KT_RIGID_NODES, (k_rigid_nodes<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d, 1)));
+    TIMED(c, KT_ADVANCE_TIME, (k_advance_time<<<1, 64, 0, s>>>(d)));
     c.time += d.dt;
+    c.outputs_stale = true;
 }
 
 // flatten the parsed model into device arrays
@@ -552,14 +827,11 @@ void build(mel_ctx& c) {
         R[R_CUR + 1] = B.yc + R[R_DISP + 1];
         R[R_CUR + 2] = 0. + R[R_DISP + 2];
     }
-    // node work accumulators start from the loaded body works on the first node
-    std::vector<double> node_work((size_t)7 * N, 0.);
+    // works of deformable bodies accumulate per node block (the loaded values
+    // are put into the body's first block by build_blocks); the contact work
+    // stays in the body slot
     for (int b = 0; b < NB; b++) {
-        Body& B = m.bodies[b];
-        if (B.rigid || B.n_nodes == 0) continue;
-        for (int q = 0; q < 7; q++)
-            if (q != W_CONTACT) node_work[7 * (size_t)node_off[b] + q] = B.works[q];
-        body_rigid[(size_t)b * RIGID_FIELDS + R_WORK + W_INTERNAL] = 0.;
+        if (m.bodies[b].rigid) continue;
         for (int q = 0; q < 7; q++)
             if (q != W_CONTACT) body_rigid[(size_t)b * RIGID_FIELDS + R_WORK + q] = 0.;
     }
@@ -609,18 +881,9 @@ void build(mel_ctx& c) {
     double** forces[] = {&d.fix, &d.fiy, &d.fcx, &d.fcy, &d.fbx, &d.fby, &d.fdx, &d.fdy, &d.fnx, &d.fny, &d.fkx,
                          &d.fky, &d.fx, &d.fy};
     for (double** p : forces) *p = c.alloc<double>(N);
-    d.node_work = c.upload(node_work);
-    d.g_ptr = c.upload(g_ptr);
-    d.g_idx = c.upload(g_idx);
-    d.g_phix = c.upload(g_phix);
-    d.g_phiy = c.upload(g_phiy);
-    d.g_c0 = c.upload(g_c0);
-    d.g_body = c.upload(g_body);
-    d.g_P = c.alloc<double>(4 * (size_t)std::max(G, 1));
-    d.t_ptr = c.upload(t_ptr);
-    d.t_gp = c.upload(t_gp);
-    d.t_phix = c.upload(t_phix);
-    d.t_phiy = c.upload(t_phiy);
+    c.d.node_body_host = node_body.data();
+    build_blocks(c, g_ptr, g_idx, g_phix, g_phiy, g_c0, t_ptr, t_gp, bn_node, node_off, gp_off, perreg);
+    c.d.node_body_host = nullptr;
     d.d_ptr = c.upload(d_ptr);
     d.d_idx = c.upload(d_idx);
     d.d_phi = c.upload(d_phi);
@@ -820,7 +1083,13 @@ mel_status mel_euler_steps(mel_ctx* c, int nsteps) {
             // contact-update cadence, Code/main.cpp:191-196
             if (c->time > c->next_contact_update - 1.e-6 * c->d.dt) {
                 c->next_contact_update = c->next_contact_update + c->model.contact_update_period;
-                update_proximity(*c);
+                if (c->timing && (c->timing_mask >> KT_PROXIMITY & 1u)) {
+                    int ev = c->ev_begin(KT_PROXIMITY);
+                    update_proximity(*c);
+                    c->ev_end(ev);
+                } else {
+                    update_proximity(*c);
+                }
             }
         }
         HIP_OK(hipGetLastError());
@@ -842,6 +1111,7 @@ mel_status mel_get_node_field(mel_ctx* c, int body, int field, double* out, size
     size_t nn = c->model.bodies[body].n_nodes;
     if (!p || n < nn) return MEL_E_ARG;
     return guarded(c, [&]() {
+        refresh_outputs(*c);
         HIP_OK(hipMemcpyAsync(out, p + c->h_node_off[body], nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
         return MEL_OK;
@@ -854,6 +1124,7 @@ mel_status mel_set_node_field(mel_ctx* c, int body, int field, const double* in,
     size_t nn = c->model.bodies[body].n_nodes;
     if (!p || n < nn) return MEL_E_ARG;
     return guarded(c, [&]() {
+        refresh_outputs(*c);
         HIP_OK(hipMemcpyAsync(p + c->h_node_off[body], in, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
         return MEL_OK;
@@ -925,17 +1196,20 @@ mel_status mel_get_body_state(mel_ctx* c, int body, double* rigid26, double* wor
         if (works7) {
             for (int q = 0; q < 7; q++) works7[q] = R[R_WORK + q];
             const Body& B = c->model.bodies[body];
-            if (!B.rigid && B.n_nodes > 0) {
+            if (!B.rigid && c->blocks_dev.n_blocks > 0) {
                 std::vector<double> w;
-                c->download(w, c->d.node_work + 7 * (size_t)c->h_node_off[body], 7 * (size_t)B.n_nodes);
-                // compensated (Neumaier) sum over the body's nodes
+                c->download(w, c->blocks_dev.blk_work, 14 * (size_t)c->blocks_dev.n_blocks);
+                // compensated (Neumaier) sum over the body's blocks
                 for (int q = 0; q < 7; q++) {
                     if (q == W_CONTACT) continue;
                     double s = 0., comp = 0.;
-                    for (int i = 0; i < B.n_nodes; i++) {
-                        double x = w[7 * (size_t)i + q], t = s + x;
-                        comp += std::fabs(s) >= std::fabs(x) ? (s - t) + x : (x - t) + s;
-                        s = t;
+                    for (size_t k = 0; k < c->h_blk_body.size(); k++) {
+                        if (c->h_blk_body[k] != body) continue;
+                        for (int h = 0; h < 2; h++) {
+                            double x = w[14 * k + 2 * q + h], t = s + x;
+                            comp += std::fabs(s) >= std::fabs(x) ? (s - t) + x : (x - t) + s;
+                            s = t;
+                        }
                     }
                     works7[q] = s + comp;
                 }
@@ -1004,6 +1278,7 @@ mel_status mel_get_info(mel_ctx* c, mel_info_t* info) {
 mel_status mel_write_dynamic(mel_ctx* c, const char* path) {
     if (!c || !path) return MEL_E_ARG;
     return guarded(c, [&]() {
+        refresh_outputs(*c);
         Model& m = c->model;
         m.time = c->time;
         m.number_iteration = c->iteration;
@@ -1053,6 +1328,30 @@ mel_status mel_get_stream(mel_ctx* c, void** stream) {
     *stream = (void*)c->stream;
     return MEL_OK;
 }
+
+mel_status mel_set_kernel_timing(mel_ctx* c, int on) {
+    return guarded(c, [&]() {
+        c->harvest();
+        for (int k = 0; k < KT_COUNT; k++) { c->kt_ms[k] = 0.; c->kt_n[k] = 0; }
+        c->timing = on != 0;
+        c->timing_mask = on == -1 ? 0xffffffffu : (unsigned)on;
+        return MEL_OK;
+    });
+}
+
+mel_status mel_get_kernel_timing(mel_ctx* c, int kernel, double* total_ms, long long* launches) {
+    if (!c || kernel < 0 || kernel >= KT_COUNT) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        c->harvest();
+        if (total_ms) *total_ms = c->kt_ms[kernel];
+        if (launches) *launches = c->kt_n[kernel];
+        return MEL_OK;
+    });
+}
+
+int mel_kernel_count(void) { return KT_COUNT; }
+
+const char* mel_kernel_name(int kernel) { return kernel >= 0 && kernel < KT_COUNT ? kKernelNames[kernel] : ""; }
 
 void mel_destroy(mel_ctx* c) { delete c; }
 

@@ -65,7 +65,7 @@ struct DevModel {
     double *upx, *upy, *vpx, *vpy, *apx, *apy;
     double *mx, *my, *imx, *imy;
     double *fix, *fiy, *fcx, *fcy, *fbx, *fby, *fdx, *fdy, *fnx, *fny, *fkx, *fky, *fx, *fy;
-    double *node_work;         // 7 per node, accumulated over steps (sum_k dt v_half . f_k)
+    const int* node_body_host; // host copy used while building (null on the device)
 
     // Gauss points
     int *g_ptr, *g_idx;

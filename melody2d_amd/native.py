@@ -72,6 +72,10 @@ def library():
             "mel_get_info": ([vp, P(Info)], ctypes.c_int),
             "mel_write_dynamic": ([vp, ctypes.c_char_p], ctypes.c_int),
             "mel_get_stream": ([vp, P(vp)], ctypes.c_int),
+            "mel_set_kernel_timing": ([vp, ctypes.c_int], ctypes.c_int),
+            "mel_get_kernel_timing": ([vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_longlong)], ctypes.c_int),
+            "mel_kernel_count": ([], ctypes.c_int),
+            "mel_kernel_name": ([ctypes.c_int], ctypes.c_char_p),
             "mel_destroy": ([vp], None),
             "mel_last_error": ([vp], ctypes.c_char_p),
             "mel_version": ([], ctypes.c_char_p),
@@ -88,6 +92,7 @@ def exported_symbols():
     return ["mel_load", "mel_load_binary", "mel_update_proximity", "mel_euler_steps", "mel_synchronize",
             "mel_get_node_field", "mel_set_node_field", "mel_get_contacts", "mel_get_neighbours",
             "mel_get_body_state", "mel_get_border_field", "mel_get_info", "mel_write_dynamic", "mel_get_stream",
+            "mel_set_kernel_timing", "mel_get_kernel_timing", "mel_kernel_count", "mel_kernel_name",
             "mel_destroy", "mel_last_error", "mel_version"]
 
 
@@ -197,6 +202,32 @@ class Context:
         out = np.zeros(n)
         self._check(self.lib.mel_get_border_field(self._h, body, border, field,
                                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n))
+        return out
+
+    def kernel_names(self):
+        return [self.lib.mel_kernel_name(k).decode() for k in range(self.lib.mel_kernel_count())]
+
+    def set_kernel_timing(self, kernels):
+        """kernels: False/None (off), True (all) or an iterable of kernel names."""
+        if kernels is True:
+            mask = -1
+        elif not kernels:
+            mask = 0
+        else:
+            names = self.kernel_names()
+            mask = 0
+            for k in kernels:
+                mask |= 1 << names.index(k)
+        self._check(self.lib.mel_set_kernel_timing(self._h, mask))
+
+    def kernel_timing(self):
+        """{kernel name: (total ms, launches)} of the instrumented launches."""
+        out = {}
+        for k in range(self.lib.mel_kernel_count()):
+            ms = ctypes.c_double()
+            n = ctypes.c_longlong()
+            self._check(self.lib.mel_get_kernel_timing(self._h, k, ctypes.byref(ms), ctypes.byref(n)))
+            out[self.lib.mel_kernel_name(k).decode()] = (ms.value, n.value)
         return out
 
     def write_dynamic(self, path):

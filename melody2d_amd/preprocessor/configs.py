@@ -20,6 +20,8 @@ Workloads
 from __future__ import annotations
 
 import math
+import os
+
 import numpy as np
 
 from .bodies import BC, deformable_body, disc_nodes, rigid_rectangle
@@ -146,26 +148,42 @@ def _rsa(rng, n, rmin, rmax, W, H, gap, max_tries=20000000):
     return out
 
 
+def _grain(job):
+    """One seeded polydisperse grain: border + Poisson-disc interior, MLS."""
+    ss, x, y, r, nodes_per_grain, rho, nregions = job
+    rng = np.random.Generator(np.random.PCG64(ss))
+    # spacing so that border + Poisson interior gives ~nodes_per_grain nodes
+    h = r * 0.108 * math.sqrt(200.0 / nodes_per_grain)
+    nb = int(round(2 * math.pi * r / h))
+    bxy, ixy = disc_nodes(rng, x, y, r, nb, h)
+    return deformable_body(bxy, ixy, "Grain", h, rho, detection=0.5 * h, contact=0.5 * h, nregions=nregions)
+
+
+def _map(fn, jobs, workers):
+    """Build bodies in worker processes; the result does not depend on the
+    worker count (each job carries its own seed)."""
+    if workers is None:
+        workers = min(16, os.cpu_count() or 1)
+    if workers <= 1 or len(jobs) < 64:
+        return [fn(j) for j in jobs]
+    import concurrent.futures as cf
+    with cf.ProcessPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(fn, jobs, chunksize=max(1, len(jobs) // (4 * workers))))
+
+
 def c2(n_grains=1000, seed=1002, nodes_per_grain=200, area_fraction=0.45, mu=0.5,
-       mass_scaling=100.0, confinement=1e5, wall_velocity=0.05, nregions=1):
+       mass_scaling=100.0, confinement=1e5, wall_velocity=0.05, nregions=1, workers=None):
     """Biaxial compression of polydisperse deformable grains with rigid walls."""
     rng_place = np.random.Generator(np.random.PCG64(np.random.SeedSequence(seed).spawn(1)[0]))
     rmin, rmax = 0.8e-3, 1.2e-3
     mean_area = math.pi * (rmin * rmin + rmin * rmax + rmax * rmax) / 3.0
     side = math.sqrt(n_grains * mean_area / area_fraction)
     grains = _rsa(rng_place, n_grains, rmin, rmax, side, side, gap=0.02e-3)
-    rngs = _streams(seed + 7, n_grains)
+    seeds = np.random.SeedSequence(seed + 7).spawn(n_grains)
     rho = RHO_GRAIN * mass_scaling
-    bodies = []
-    hmax = 0.0
-    for k, (x, y, r) in enumerate(grains):
-        # spacing so that border + Poisson interior gives ~nodes_per_grain nodes
-        h = r * 0.108 * math.sqrt(200.0 / nodes_per_grain)
-        hmax = max(hmax, h)
-        nb = int(round(2 * math.pi * r / h))
-        bxy, ixy = disc_nodes(rngs[k], x, y, r, nb, h)
-        bodies.append(deformable_body(bxy, ixy, "Grain", h, rho, detection=0.5 * h,
-                                      contact=0.5 * h, nregions=nregions))
+    jobs = [(s, x, y, r, nodes_per_grain, rho, nregions) for s, (x, y, r) in zip(seeds, grains)]
+    bodies = _map(_grain, jobs, workers)
+    hmax = max(b.nodal_distance for b in bodies)
     t = 2.0e-3
     hw = hmax
     # bottom and left walls fixed; top wall driven down; right wall under confinement

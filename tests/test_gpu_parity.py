@@ -58,3 +58,17 @@ def test_internal_forces_bitwise_on_contact_free_start(golden_dir, tmp_path):
             k = "b%d.%s" % (b, f)
             eq = np.mean(dev[k] == ref[k])
             assert eq > 0.9, (k, eq)
+
+
+@pytest.mark.parametrize("name", ["tiny_bc", "tiny_platen"])
+def test_split_node_blocks_match_oracle(name, golden_dir, tmp_path, monkeypatch):
+    """Bodies cut into several node blocks (64 nodes each here) exchange
+    displacement parameters through the staged commit: results are the same
+    as with one block per body."""
+    monkeypatch.setenv("MELODY2D_BLOCK_NODES", "64")
+    model, _ = golden_dir(name)
+    print("case split", name, flush=True)
+    for s, (dev, ref) in _run_pair(model, [1, 10, 50], tmp_path).items():
+        res = compare(dev, ref)
+        bad = {k: v[0] for k, v in res.items() if not v[0] <= TOL}
+        assert not bad, "step %d: %s" % (s, sorted(bad.items(), key=lambda kv: -kv[1])[:8])
