@@ -7,13 +7,14 @@
 //   Body::Apply_Euler (works)     Code/Body.h:2051-2068, Node::Apply_Euler Code/Node.h:239-245
 //
 // Data path per block: the displacement parameters of the touched nodes are
-// staged in LDS once; the block's Gauss-point support entries (local node
-// index + dphi/dx, dphi/dy) stream from HBM in chunks with coalesced loads
-// into LDS; one thread per Gauss point forms F and the stress terms from
-// LDS; then one thread per owned node walks the chunk-local transposed index
-// (positions of its entries, in Gauss-point order) and accumulates
-// -c0 (dphi . P) per region, exactly in the order of the reference's
-// per-region accumulators.  The per-entry HBM traffic is 2 + 16 + 2 bytes.
+// staged in LDS once.  The block's support entries stream from HBM chunk by
+// chunk with 16-byte loads into registers; the loads of chunk c + 1 are in
+// flight while chunk c is computed.  Per chunk, one thread per Gauss point
+// forms F and the stress terms from LDS and overwrites its entries with their
+// contributions c0 * (dphi . P); then one thread per owned node walks its
+// chunk-local transposed index and subtracts the contributions per region, in
+// the order of the reference's per-region accumulators.  HBM traffic per
+// entry: 16 (dphi/dx, dphi/dy) + 2 (node index) + 2 (transposed position).
 #include "mel_device.hpp"
 #include "k_block.hpp"
 
@@ -21,147 +22,191 @@ namespace mel {
 
 namespace {
 
-struct Lds {
-    double *ux, *uy, *P, *c0;
-    double *ex, *ey;
-    int* greg;
-    uint16_t *eidx, *egp, *tpos, *tptr, *gptr;
-};
+constexpr int VEC16 = CHUNK_ENTRIES / 8;    // uint16x8 vectors per chunk array
+// native vector types (LLVM vectors) keep the in-flight chunk in registers
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+static_assert(VEC16 <= BLOCK_THREADS, "one uint16x8 vector per thread");
+static_assert(CHUNK_GPS <= BLOCK_THREADS, "one Gauss point per thread");
 
-__device__ inline Lds carve(unsigned char* smem, const BlockArrays& B) {
-    Lds L;
-    unsigned char* p = smem;
-    auto take = [&](size_t bytes) { unsigned char* q = p; p += (bytes + 15) & ~size_t(15); return q; };
-    L.ux = (double*)take(sizeof(double) * B.max_touch);
-    L.uy = (double*)take(sizeof(double) * B.max_touch);
-    L.ex = (double*)take(sizeof(double) * B.e_cap);
-    L.ey = (double*)take(sizeof(double) * B.e_cap);
-    L.P = (double*)take(sizeof(double) * 4 * B.ch_cap);
-    L.c0 = (double*)take(sizeof(double) * B.ch_cap);
-    L.greg = (int*)take(sizeof(int) * B.ch_cap);
-    L.eidx = (uint16_t*)take(2 * B.e_cap);
-    L.egp = (uint16_t*)take(2 * B.e_cap);
-    L.tpos = (uint16_t*)take(2 * B.e_cap);
-    L.tptr = (uint16_t*)take(2 * (B.max_own + 1));
-    L.gptr = (uint16_t*)take(2 * (B.ch_cap + 1));
-    return L;
-}
-
-// stress terms of Code/Model.h:19-73 for one Gauss point, written so that
-// the gather computes c0 * (phix * A + phiy * B) and c0 * (phix * C + phiy * D)
-__device__ inline void gp_terms(int mat_type, double Mu, double K2, const Lds& L, int k0, int k1, double4& out) {
+// stress terms of Code/Model.h:19-73 for one Gauss point, written so that a
+// support entry contributes c0 * (phix * P.x + phiy * P.y), c0 * (phix * P.z + phiy * P.w)
+__device__ inline double4 gp_terms(int mat_type, double Mu, double K2, const uint16_t* eidx, const double2* exy,
+                                   const double* ux, const double* uy, int k0, int k1) {
     if (mat_type == 0) {
         double E11 = 0., E12 = 0., E22 = 0.;
         for (int k = k0; k < k1; k++) {
-            int j = L.eidx[k];
-            double fx = L.ex[k], fy = L.ey[k], ux = L.ux[j], uy = L.uy[j];
-            E11 += fx * ux;
-            E12 += fx * uy + fy * ux;
-            E22 += fy * uy;
+            int j = eidx[k];
+            double2 f = exy[k];
+            double u = ux[j], v = uy[j];
+            E11 += f.x * u;
+            E12 += f.x * v + f.y * u;
+            E22 += f.y * v;
         }
         E12 = 0.5 * E12;
         double J = 1. + E11 + E22;
         double S33 = K2 * (J - 1.);
         double M2 = 2 * Mu;
         double S11 = S33 + M2 * E11, S22 = S33 + M2 * E22, S12 = M2 * E12;
-        out = make_double4(S11, S12, S12, S22);
-    } else {
-        double F11 = 1., F12 = 0., F21 = 0., F22 = 1.;
-        for (int k = k0; k < k1; k++) {
-            int j = L.eidx[k];
-            double fx = L.ex[k], fy = L.ey[k], ux = L.ux[j], uy = L.uy[j];
-            F11 += fx * ux;
-            F12 += fy * ux;
-            F21 += fx * uy;
-            F22 += fy * uy;
+        return make_double4(S11, S12, S12, S22);
+    }
+    double F11 = 1., F12 = 0., F21 = 0., F22 = 1.;
+    for (int k = k0; k < k1; k++) {
+        int j = eidx[k];
+        double2 f = exy[k];
+        double u = ux[j], v = uy[j];
+        F11 += f.x * u;
+        F12 += f.y * u;
+        F21 += f.x * v;
+        F22 += f.y * v;
+    }
+    double F11F12 = F11 * F11 + F12 * F12;
+    double F21F22 = F21 * F21 + F22 * F22;
+    double J = F11 * F22 - F12 * F21;
+    double invJ = 1. / J;
+    double K = K2 * (J - 1.);
+    double MuJ53 = Mu * pow(J, -1.666666666667);
+    double trB3 = 0.333333333333 * (F11F12 + F21F22 + 1.);
+    double s11 = K + MuJ53 * (F11F12 - trB3);
+    double s22 = K + MuJ53 * (F21F22 - trB3);
+    double s12 = MuJ53 * (F11 * F21 + F12 * F22);
+    double S11 = (F12 * (F12 * s22 - F22 * s12) - F22 * (F12 * s12 - F22 * s11)) * invJ;
+    double S22 = (F11 * (F11 * s22 - F21 * s12) - F21 * (F11 * s12 - F21 * s11)) * invJ;
+    double S12 = (F21 * (F12 * s12 - F22 * s11) - F11 * (F12 * s22 - F22 * s12)) * invJ;
+    return make_double4(F11 * S11 + F12 * S12, F12 * S22 + F11 * S12, F21 * S11 + F22 * S12, F22 * S22 + F21 * S12);
+}
+
+// one chunk's data in flight, held in registers until it is written to LDS
+struct ChunkRegs {
+    f64x2 f[ENTRIES_PER_THREAD];
+    u32x4 iv, tv;
+    int gk0, gk1, greg;
+    double gc0;
+    int tp0[NODES_PER_THREAD], tp1[NODES_PER_THREAD];
+};
+
+__device__ inline void load_chunk(const BlockArrays& B, int4 meta, int2 po, int nown, int t, ChunkRegs& R) {
+    const int ngp = meta.y, ne8 = (meta.z + 7) & ~7;
+    const size_t E = 8 * (size_t)meta.w;
+    // small per-thread items first (the counter of outstanding loads is in
+    // order); int32 offsets need no unpacking, so nothing waits here
+    if (t < ngp) {
+        R.gk0 = B.gptr[po.x + t];
+        R.gk1 = B.gptr[po.x + t + 1];
+        R.gc0 = B.bgp_c0[meta.x + t];
+        R.greg = B.bgp_region[meta.x + t];
+    }
+#pragma unroll
+    for (int q = 0; q < NODES_PER_THREAD; q++) {
+        int o = t + q * BLOCK_THREADS;
+        if (o < nown) {
+            R.tp0[q] = B.tptr[po.y + o];
+            R.tp1[q] = B.tptr[po.y + o + 1];
         }
-        double F11F12 = F11 * F11 + F12 * F12;
-        double F21F22 = F21 * F21 + F22 * F22;
-        double J = F11 * F22 - F12 * F21;
-        double invJ = 1. / J;
-        double K = K2 * (J - 1.);
-        double MuJ53 = Mu * pow(J, -1.666666666667);
-        double trB3 = 0.333333333333 * (F11F12 + F21F22 + 1.);
-        double s11 = K + MuJ53 * (F11F12 - trB3);
-        double s22 = K + MuJ53 * (F21F22 - trB3);
-        double s12 = MuJ53 * (F11 * F21 + F12 * F22);
-        double S11 = (F12 * (F12 * s22 - F22 * s12) - F22 * (F12 * s12 - F22 * s11)) * invJ;
-        double S22 = (F11 * (F11 * s22 - F21 * s12) - F21 * (F11 * s12 - F21 * s11)) * invJ;
-        double S12 = (F21 * (F12 * s12 - F22 * s11) - F11 * (F12 * s22 - F22 * s12)) * invJ;
-        out = make_double4(F11 * S11 + F12 * S12, F12 * S22 + F11 * S12, F21 * S11 + F22 * S12,
-                           F22 * S22 + F21 * S12);
+    }
+    if (t < ne8 / 8) {
+        R.iv = reinterpret_cast<const u32x4*>(B.be_idx + E)[t];
+        R.tv = reinterpret_cast<const u32x4*>(B.tpos + E)[t];
+    }
+#pragma unroll
+    for (int q = 0; q < ENTRIES_PER_THREAD; q++) {
+        int i = t + q * BLOCK_THREADS;
+        if (i < ne8) R.f[q] = reinterpret_cast<const f64x2*>(B.be_fxy)[E + i];
     }
 }
 
 }  // namespace
 
-__global__ void __launch_bounds__(BLOCK_THREADS) k_block_force_update(DevModel m, BlockArrays B) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_block_force_update(DevModel m, BlockArrays B) {
+    extern __shared__ __align__(16) double s_u[];          // ux[max_touch], uy[max_touch]
+    __shared__ __align__(16) double2 s_exy[CHUNK_ENTRIES];
+    __shared__ __align__(16) uint16_t s_eidx[CHUNK_ENTRIES];
+    __shared__ __align__(16) uint16_t s_tpos[CHUNK_ENTRIES];
+    __shared__ __align__(16) uint16_t s_ereg[CHUNK_ENTRIES];
+    __shared__ int4 s_meta[MAX_BLOCK_CHUNKS];
+    __shared__ int2 s_po[MAX_BLOCK_CHUNKS];
     __shared__ double red[BLOCK_THREADS / 64][5];
     const int blk = blockIdx.x;
     const int t = threadIdx.x;
     const int body = B.blk_body[blk];
     if (m.body_status[body] == 1) return;   // inactive body: no update (uniform over the block)
     const bool split = B.blk_split[blk] != 0;
+    const bool single = B.blk_single_region[blk] != 0;
     const int nown = B.blk_nown[blk];
     const int ntouch = B.blk_ntouch[blk];
     const int* own = B.own_nodes + B.blk_own_off[blk];
     const int* touch = B.touch_nodes + B.blk_touch_off[blk];
-    Lds L = carve(smem, B);
+    double* s_ux = s_u;
+    double* s_uy = s_u + B.max_touch;
     const int mat = m.body_material[body];
     const int mtype = m.mat_type[mat];
     const double Mu = m.mat_p[6 * mat + 3], K2 = m.mat_p[6 * mat + 4];
 
+    const int nch = B.blk_nchunk[blk];
+    const int4* gmeta = B.ch_meta + B.blk_chunk_off[blk];
+    const int2* gpo = B.ch_ptr_off + B.blk_chunk_off[blk];
+    ChunkRegs R;
+    if (nch > 0) load_chunk(B, gmeta[0], gpo[0], nown, t, R);
+    // chunk descriptors in LDS: read without waiting on the entry stream
+    for (int i = t; i < nch; i += BLOCK_THREADS) {
+        s_meta[i] = gmeta[i];
+        s_po[i] = gpo[i];
+    }
     for (int i = t; i < ntouch; i += BLOCK_THREADS) {
         int g = touch[i];
-        L.ux[i] = m.upx[g];
-        L.uy[i] = m.upy[g];
+        s_ux[i] = m.upx[g];
+        s_uy[i] = m.upy[g];
     }
     double tx[NODES_PER_THREAD], ty[NODES_PER_THREAD], rx[NODES_PER_THREAD], ry[NODES_PER_THREAD];
     int cur[NODES_PER_THREAD];
+#pragma unroll
     for (int q = 0; q < NODES_PER_THREAD; q++) { tx[q] = ty[q] = rx[q] = ry[q] = 0.; cur[q] = -1; }
 
-    const int c0i = B.blk_chunk_off[blk], c1i = c0i + B.blk_nchunk[blk];
-    for (int c = c0i; c < c1i; c++) {
-        const int ngp = B.ch_ngp[c], ne = B.ch_ne[c];
-        const int gs = B.ch_gp_start[c];
-        const size_t es = B.ch_e_start[c];
-        __syncthreads();
-        for (int i = t; i < ne; i += BLOCK_THREADS) {
-            L.eidx[i] = B.be_idx[es + i];
-            L.ex[i] = B.be_fx[es + i];
-            L.ey[i] = B.be_fy[es + i];
-            L.tpos[i] = B.tpos[B.ch_tpos_off[c] + i];
+    for (int c = 0; c < nch; c++) {
+        __syncthreads();                      // previous chunk's LDS reads are done
+        const int4 meta = s_meta[c];
+        const int ngp = meta.y, ne8 = (meta.z + 7) & ~7;
+#pragma unroll
+        for (int q = 0; q < ENTRIES_PER_THREAD; q++) {
+            int i = t + q * BLOCK_THREADS;
+            if (i < ne8) reinterpret_cast<f64x2*>(s_exy)[i] = R.f[q];
         }
-        for (int i = t; i <= nown; i += BLOCK_THREADS) L.tptr[i] = B.tptr[B.ch_tptr_off[c] + i];
-        for (int i = t; i <= ngp; i += BLOCK_THREADS) L.gptr[i] = B.gptr[B.ch_gptr_off[c] + i];
-        for (int i = t; i < ngp; i += BLOCK_THREADS) {
-            L.c0[i] = B.bgp_c0[gs + i];
-            L.greg[i] = B.bgp_region[gs + i];
+        if (t < ne8 / 8) {
+            reinterpret_cast<u32x4*>(s_eidx)[t] = R.iv;
+            reinterpret_cast<u32x4*>(s_tpos)[t] = R.tv;
+        }
+        const int gk0 = R.gk0, gk1 = R.gk1, greg = R.greg;
+        const double gc0 = R.gc0;
+        int tp0[NODES_PER_THREAD], tp1[NODES_PER_THREAD];
+#pragma unroll
+        for (int q = 0; q < NODES_PER_THREAD; q++) { tp0[q] = R.tp0[q]; tp1[q] = R.tp1[q]; }
+        __syncthreads();
+        // next chunk's loads stay in flight during this chunk's work
+        if (c + 1 < nch) load_chunk(B, s_meta[c + 1], s_po[c + 1], nown, t, R);
+        if (t < ngp) {
+            double4 P = gp_terms(mtype, Mu, K2, s_eidx, s_exy, s_ux, s_uy, gk0, gk1);
+            for (int k = gk0; k < gk1; k++) {
+                double2 f = s_exy[k];
+                s_exy[k] = make_double2(gc0 * (f.x * P.x + f.y * P.y), gc0 * (f.x * P.z + f.y * P.w));
+                if (!single) s_ereg[k] = (uint16_t)greg;
+            }
         }
         __syncthreads();
-        for (int gl = t; gl < ngp; gl += BLOCK_THREADS) {
-            int k0 = L.gptr[gl], k1 = L.gptr[gl + 1];
-            double4 P;
-            gp_terms(mtype, Mu, K2, L, k0, k1, P);
-            L.P[4 * gl] = P.x; L.P[4 * gl + 1] = P.y; L.P[4 * gl + 2] = P.z; L.P[4 * gl + 3] = P.w;
-            for (int k = k0; k < k1; k++) L.egp[k] = (uint16_t)gl;
-        }
-        __syncthreads();
+#pragma unroll
         for (int q = 0; q < NODES_PER_THREAD; q++) {
             int o = t + q * BLOCK_THREADS;
             if (o >= nown) break;
-            for (int p = L.tptr[o]; p < L.tptr[o + 1]; p++) {
-                int k = L.tpos[p];
-                int gl = L.egp[k];
-                int r = L.greg[gl];
+            for (int p = tp0[q]; p < tp1[q]; p++) {
+                int k = s_tpos[p];
+                double2 v = s_exy[k];
+                int r = single ? 0 : (int)s_ereg[k];
                 if (r != cur[q]) {
                     if (cur[q] >= 0) { tx[q] += rx[q]; ty[q] += ry[q]; }
                     rx[q] = 0.; ry[q] = 0.; cur[q] = r;
                 }
-                double c0 = L.c0[gl], fx = L.ex[k], fy = L.ey[k];
-                rx[q] -= c0 * (fx * L.P[4 * gl] + fy * L.P[4 * gl + 1]);
-                ry[q] -= c0 * (fx * L.P[4 * gl + 2] + fy * L.P[4 * gl + 3]);
+                rx[q] -= v.x;
+                ry[q] -= v.y;
             }
         }
     }
@@ -171,6 +216,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_block_force_update(DevModel m
     double gx = m.gravity_x, gy = m.gravity_y;
     if (m.body_status[body] == 2) { gx = 0.; gy = 0.; }
     const double dt = m.dt;
+#pragma unroll
     for (int q = 0; q < NODES_PER_THREAD; q++) {
         int o = t + q * BLOCK_THREADS;
         if (o >= nown) break;
@@ -214,6 +260,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_block_force_update(DevModel m
     }
     // block work increments: fixed-order reduction (wavefront butterflies,
     // then the waves in order), accumulated with compensation across steps
+#pragma unroll
     for (int q = 0; q < 5; q++) {
         double v = w[q];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -221,7 +268,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_block_force_update(DevModel m
     }
     __syncthreads();
     if (t == 0) {
-        static const int slot[5] = {W_INTERNAL, W_BODY, W_DIRICHLET, W_NEUMANN, W_DAMPING};
+        const int slot[5] = {W_INTERNAL, W_BODY, W_DIRICHLET, W_NEUMANN, W_DAMPING};
         double* acc = B.blk_work + 14 * (size_t)blk;
         for (int q = 0; q < 5; q++) {
             double v = 0.;

@@ -15,22 +15,56 @@ namespace mel {
 
 constexpr int OLD_CAP = 16;    // previous elements per slave slot
 
+// per chunk of SEG_CHUNK border slots: the box of the slot nodes inflated by
+// the detection distance (its part of the border AABB of
+// Code/Proximity.h:63-109) and the box of the chunk's segments, used to skip
+// whole chunks of master segments in the narrow phase
+__global__ void k_prox_chunk_boxes(DevModel m) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_seg_chunks) return;
+    int br = m.sc_border[c];
+    int b = m.br_body[br];
+    if (m.body_status[b] == 1) return;
+    int s0 = m.br_slot0[br], ns = m.br_nslots[br];
+    int first = (c - m.br_chunk0[br]) * SEG_CHUNK;
+    double d = m.body_detect[b];
+    double xmin = 1.e100, xmax = -1.e100, ymin = 1.e100, ymax = -1.e100;
+    for (int j = first; j < min(first + SEG_CHUNK, ns); j++) {
+        int nd = m.sl_node[s0 + j];
+        double x = m.xc[nd], y = m.yc[nd];
+        if (x - d < xmin) xmin = x - d;
+        if (x + d > xmax) xmax = x + d;
+        if (y - d < ymin) ymin = y - d;
+        if (y + d > ymax) ymax = y + d;
+    }
+    m.sc_nxmin[c] = xmin; m.sc_nxmax[c] = xmax; m.sc_nymin[c] = ymin; m.sc_nymax[c] = ymax;
+    double P = m.period;
+    xmin = 1.e100; xmax = -1.e100; ymin = 1.e100; ymax = -1.e100;
+    for (int seg = first; seg < min(first + SEG_CHUNK, ns - 1); seg++) {
+        int s = s0 + seg;
+        int n1 = m.sl_n1[s], n2 = m.sl_n2[s];
+        double x1 = m.xc[n1] + P * m.sl_s1[s], x2 = m.xc[n2] + P * m.sl_s2[s];
+        double y1 = m.yc[n1], y2 = m.yc[n2];
+        xmin = fmin(xmin, fmin(x1, x2)); xmax = fmax(xmax, fmax(x1, x2));
+        ymin = fmin(ymin, fmin(y1, y2)); ymax = fmax(ymax, fmax(y1, y2));
+    }
+    m.sc_xmin[c] = xmin; m.sc_xmax[c] = xmax; m.sc_ymin[c] = ymin; m.sc_ymax[c] = ymax;
+}
+
 // border AABBs inflated by the detection distance + periodic shift range
-// (Code/Proximity.h:63-109)
+// (Code/Proximity.h:63-109): min / max over the chunk boxes, which equals
+// the reference's running min / max over the border nodes
 __global__ void k_prox_aabb(DevModel m) {
     int br = blockIdx.x * blockDim.x + threadIdx.x;
     if (br >= m.n_borders) return;
     int b = m.br_body[br];
     if (m.body_status[b] == 1) return;
     double xmin = 1.e100, xmax = -1.e100, ymin = 1.e100, ymax = -1.e100;
-    double d = m.body_detect[b];
-    for (int s = m.br_slot0[br]; s < m.br_slot0[br] + m.br_nslots[br]; s++) {
-        int nd = m.sl_node[s];
-        double x = m.xc[nd], y = m.yc[nd];
-        if (x - d < xmin) xmin = x - d;
-        if (x + d > xmax) xmax = x + d;
-        if (y - d < ymin) ymin = y - d;
-        if (y + d > ymax) ymax = y + d;
+    for (int c = m.br_chunk0[br]; c < m.br_chunk0[br + 1]; c++) {
+        if (m.sc_nxmin[c] < xmin) xmin = m.sc_nxmin[c];
+        if (m.sc_nxmax[c] > xmax) xmax = m.sc_nxmax[c];
+        if (m.sc_nymin[c] < ymin) ymin = m.sc_nymin[c];
+        if (m.sc_nymax[c] > ymax) ymax = m.sc_nymax[c];
     }
     double inv = 1. / m.period;
     m.pb_xmin[br] = xmin; m.pb_xmax[br] = xmax; m.pb_ymin[br] = ymin; m.pb_ymax[br] = ymax;
@@ -217,7 +251,17 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base) {
         emass = m.body_mass[bS];
     }
     int cnt = 0;
-    for (int seg = 0; seg < nM - 1; seg++) {
+    // chunks whose segment box lies farther than the detection distance are
+    // skipped: detect_segment only accepts points within that distance of a
+    // segment; the margin covers rounding of the shifted coordinates
+    const double Xq = Xs - P * shM;
+    const double dd = 1.000001 * detect + 1e-10 * (1. + fabs(Xs) + fabs(Ys) + fabs(P * shM));
+    const int cM0 = m.br_chunk0[gM], cM1 = m.br_chunk0[gM + 1];
+    for (int ch = cM0; ch < cM1; ch++) {
+    if (!(Xq > m.sc_xmin[ch] - dd && Xq < m.sc_xmax[ch] + dd && Ys > m.sc_ymin[ch] - dd && Ys < m.sc_ymax[ch] + dd))
+        continue;
+    const int sg0 = (ch - cM0) * SEG_CHUNK, sg1 = min(sg0 + SEG_CHUNK, nM - 1);
+    for (int seg = sg0; seg < sg1; seg++) {
         int s = s0 + seg;
         int n1 = m.sl_n1[s], n2 = m.sl_n2[s];
         if (ri.self && (n1 == nodeS || n2 == nodeS)) continue;
@@ -259,6 +303,7 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base) {
             }
         }
         cnt++;
+    }
     }
     return cnt;
 }

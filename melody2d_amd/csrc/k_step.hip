@@ -361,29 +361,45 @@ __global__ void k_rigid_forces(DevModel m) {
         if (tid < w) { red[tid][0] += red[tid + w][0]; red[tid][1] += red[tid + w][1]; red[tid][2] += red[tid + w][2]; }
         __syncthreads();
     }
-    if (tid != 0) return;
-    R[R_FCON] = red[0][0]; R[R_FCON + 1] = red[0][1]; R[R_FCON + 2] = red[0][2];
-    // boundary conditions
+    if (tid == 0) { R[R_FCON] = red[0][0]; R[R_FCON + 1] = red[0][1]; R[R_FCON + 2] = red[0][2]; }
+    // boundary conditions: every thread forms the terms of its border slots,
+    // thread 0 adds them in slot order (the reference's serial loop)
+    __shared__ double terms[256][6];
     double fn[3] = {0., 0., 0.}, fd[3] = {0., 0., 0.};
+    double xc0 = R[R_CUR], yc0 = R[R_CUR + 1];
     int gb0 = m.border_off[b];
     for (int k = 0; k < m.body_nborders[b]; k++) {
         int br = gb0 + k;
         int xt = m.br_xtype[br], yt = m.br_ytype[br];
         if (xt == 0 && yt == 0) continue;
-        for (int j = 0; j < m.br_nslots[br]; j++) {
-            int s = m.br_slot0[br] + j;
-            int node = m.sl_node[s];
-            double L = m.sl_len[s];
-            double fxn = 0., fyn = 0., fxd = 0., fyd = 0.;
-            if (xt == 3 || xt == 4) fxn = m.br_xval[br] * L;
-            else if (xt == 2) fxd = (m.br_xparam[2 * br] * (m.br_xval[br] - m.ux[node]) + m.br_xparam[2 * br + 1] * (m.br_xvel[br] - m.vx[node])) * L;
-            if (yt == 3 || yt == 4) fyn = m.br_yval[br] * L;
-            else if (yt == 2) fyd = (m.br_yparam[2 * br] * (m.br_yval[br] - m.uy[node]) + m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node])) * L;
-            double dx = m.xc[node] - R[R_CUR], dy = m.yc[node] - R[R_CUR + 1];
-            fn[0] += fxn; fn[1] += fyn; fn[2] += -fxn * dy + fyn * dx;
-            fd[0] += fxd; fd[1] += fyd; fd[2] += -fxd * dy + fyd * dx;
+        int ns = m.br_nslots[br];
+        for (int base = 0; base < ns; base += 256) {
+            int j = base + tid;
+            __syncthreads();
+            if (j < ns) {
+                int s = m.br_slot0[br] + j;
+                int node = m.sl_node[s];
+                double L = m.sl_len[s];
+                double fxn = 0., fyn = 0., fxd = 0., fyd = 0.;
+                if (xt == 3 || xt == 4) fxn = m.br_xval[br] * L;
+                else if (xt == 2) fxd = (m.br_xparam[2 * br] * (m.br_xval[br] - m.ux[node]) + m.br_xparam[2 * br + 1] * (m.br_xvel[br] - m.vx[node])) * L;
+                if (yt == 3 || yt == 4) fyn = m.br_yval[br] * L;
+                else if (yt == 2) fyd = (m.br_yparam[2 * br] * (m.br_yval[br] - m.uy[node]) + m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node])) * L;
+                double dx = m.xc[node] - xc0, dy = m.yc[node] - yc0;
+                terms[tid][0] = fxn; terms[tid][1] = fyn; terms[tid][2] = -fxn * dy + fyn * dx;
+                terms[tid][3] = fxd; terms[tid][4] = fyd; terms[tid][5] = -fxd * dy + fyd * dx;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                int n = min(256, ns - base);
+                for (int q = 0; q < n; q++) {
+                    fn[0] += terms[q][0]; fn[1] += terms[q][1]; fn[2] += terms[q][2];
+                    fd[0] += terms[q][3]; fd[1] += terms[q][4]; fd[2] += terms[q][5];
+                }
+            }
         }
     }
+    if (tid != 0) return;
     for (int q = 0; q < 3; q++) { R[R_FNEU + q] = fn[q]; R[R_FDIR + q] = fd[q]; }
     double gx = m.gravity_x, gy = m.gravity_y;
     if (m.body_status[b] == 2) { gx = 0.; gy = 0.; }

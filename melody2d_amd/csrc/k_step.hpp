@@ -6,6 +6,7 @@ namespace mel {
 
 constexpr int BUCKET_CAP = 32;     // point-force records per border node per step
 constexpr int BLOCK = 256;
+constexpr int SEG_CHUNK = 32;     // border slots per proximity culling chunk
 
 __global__ void k_border_length(DevModel m);
 __global__ void k_contact_project(DevModel m);
@@ -21,6 +22,7 @@ __global__ void k_rigid_nodes(DevModel m, int with_forces);
 __global__ void k_advance_time(DevModel m);
 
 // proximity (k_proximity.hip)
+__global__ void k_prox_chunk_boxes(DevModel m);
 __global__ void k_prox_aabb(DevModel m);
 __global__ void k_prox_neighbours(DevModel m);
 __global__ void k_prox_rows(DevModel m);

@@ -273,12 +273,8 @@ void check_flags(mel_ctx& c) {
 }
 
 
-size_t block_lds_size(const BlockArrays& B) {
-    auto r = [](size_t b) { return (b + 15) & ~size_t(15); };
-    return 2 * r(8 * (size_t)B.max_touch) + 2 * r(8 * (size_t)B.e_cap) + r(32 * (size_t)B.ch_cap) +
-           r(8 * (size_t)B.ch_cap) + r(4 * (size_t)B.ch_cap) + 3 * r(2 * (size_t)B.e_cap) +
-           r(2 * ((size_t)B.max_own + 1)) + r(2 * ((size_t)B.ch_cap + 1));
-}
+// dynamic LDS of the block kernel: displacement parameters of the touched nodes
+size_t block_lds_size(const BlockArrays& B) { return 16 * (size_t)B.max_touch; }
 
 // Node blocks of the fused internal-force kernel: whole bodies when they fit,
 // otherwise groups of nodes in Morton order; chunked Gauss-point streams and
@@ -289,15 +285,16 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
                   const std::vector<int>& bn_node, const std::vector<int>& node_off, const std::vector<int>& gp_off,
                   const std::vector<int>& perreg) {
     Model& m = c.model;
-    const int ECAP = 2560, CHCAP = 128;
+    const int ECAP = CHUNK_ENTRIES, CHCAP = CHUNK_GPS;
     int N = node_off.back();
     std::vector<int> blk_body, blk_own_off, blk_nown, blk_touch_off, blk_ntouch, blk_chunk_off, blk_nchunk;
-    std::vector<int> own_nodes, touch_nodes;
-    std::vector<int> ch_gp_start, ch_ngp, ch_ne, ch_tptr_off, ch_tpos_off, ch_gptr_off;
-    std::vector<long long> ch_e_start;
-    std::vector<double> bgp_c0, be_fx, be_fy, blk_work;
+    std::vector<int> blk_single, own_nodes, touch_nodes;
+    std::vector<int4> ch_meta;
+    std::vector<int2> ch_ptr_off;
+    std::vector<double> bgp_c0, be_fxy, blk_work;
     std::vector<int> bgp_region;
-    std::vector<uint16_t> gptr, be_idx, tptr, tpos;
+    std::vector<int> gptr, tptr;
+    std::vector<uint16_t> be_idx, tpos;
     std::vector<int> local(N, -1), blk_split, split_pos;
     int max_touch = 1, max_own = 1;
     // MELODY2D_BLOCK_NODES lowers the nodes per block (exercises the split
@@ -371,7 +368,12 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
             blk_ntouch.push_back((int)touched.size());
             own_nodes.insert(own_nodes.end(), owned.begin(), owned.end());
             touch_nodes.insert(touch_nodes.end(), touched.begin(), touched.end());
-            blk_chunk_off.push_back((int)ch_ngp.size());
+            blk_chunk_off.push_back((int)ch_meta.size());
+            {
+                int r0 = gps.empty() ? 0 : (gps.front() - gp_off[b]) / perreg[b];
+                int r1 = gps.empty() ? 0 : (gps.back() - gp_off[b]) / perreg[b];
+                blk_single.push_back(r0 == r1 ? 1 : 0);
+            }
             int nown = (int)owned.size();
             size_t q = 0;
             while (q < gps.size()) {
@@ -384,11 +386,11 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
                     ne += cnt;
                     q1++;
                 }
-                ch_gp_start.push_back((int)bgp_c0.size());
-                ch_ngp.push_back((int)(q1 - q));
-                ch_ne.push_back(ne);
-                ch_e_start.push_back((long long)be_idx.size());
-                ch_gptr_off.push_back((int)gptr.size());
+                if (be_idx.size() / 8 > (size_t)INT32_MAX) throw CapacityError{"support entries exceed the chunk index range"};
+                ch_meta.push_back(make_int4((int)bgp_c0.size(), (int)(q1 - q), ne, (int)(be_idx.size() / 8)));
+                ch_ptr_off.push_back(make_int2((int)gptr.size(), (int)tptr.size()));
+                const size_t ebase = be_idx.size();
+                const int ne8 = (ne + 7) & ~7;
                 std::vector<std::vector<uint16_t>> lst(nown);
                 int pos = 0;
                 gptr.push_back(0);
@@ -399,25 +401,30 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
                     for (int k = g_ptr[g]; k < g_ptr[g + 1]; k++) {
                         int li = local[g_idx[k]];
                         be_idx.push_back((uint16_t)li);
-                        be_fx.push_back(g_phix[k]);
-                        be_fy.push_back(g_phiy[k]);
+                        be_fxy.push_back(g_phix[k]);
+                        be_fxy.push_back(g_phiy[k]);
                         if (li < nown) lst[li].push_back((uint16_t)pos);
                         pos++;
                     }
-                    gptr.push_back((uint16_t)pos);
+                    gptr.push_back(pos);
                 }
-                ch_tptr_off.push_back((int)tptr.size());
-                ch_tpos_off.push_back((int)tpos.size());
                 int acc = 0;
                 tptr.push_back(0);
                 for (int i = 0; i < nown; i++) {
                     acc += (int)lst[i].size();
-                    tptr.push_back((uint16_t)acc);
+                    tptr.push_back(acc);
                     tpos.insert(tpos.end(), lst[i].begin(), lst[i].end());
                 }
+                // pad the entry arrays of the chunk to a multiple of 8 entries
+                be_idx.resize(ebase + ne8, 0);
+                be_fxy.resize(2 * (ebase + ne8), 0.);
+                tpos.resize(ebase + ne8, 0);
                 q = q1;
             }
-            blk_nchunk.push_back((int)ch_ngp.size() - blk_chunk_off.back());
+            blk_nchunk.push_back((int)ch_meta.size() - blk_chunk_off.back());
+            if (blk_nchunk.back() > MAX_BLOCK_CHUNKS)
+                throw CapacityError{"node block streams more than " + std::to_string(MAX_BLOCK_CHUNKS) +
+                                    " chunks (set MELODY2D_BLOCK_NODES lower)"};
             for (int j : touched) local[j] = -1;
             std::vector<double> w(14, 0.);
             if (first)
@@ -439,18 +446,15 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     B.n_blocks = (int)blk_body.size();
     B.max_touch = max_touch;
     B.max_own = max_own;
-    B.e_cap = ECAP;
-    B.ch_cap = CHCAP;
     B.blk_body = c.upload(blk_body); B.blk_own_off = c.upload(blk_own_off); B.blk_nown = c.upload(blk_nown);
     B.blk_touch_off = c.upload(blk_touch_off); B.blk_ntouch = c.upload(blk_ntouch);
     B.blk_chunk_off = c.upload(blk_chunk_off); B.blk_nchunk = c.upload(blk_nchunk);
+    B.blk_single_region = c.upload(blk_single);
     B.own_nodes = c.upload(own_nodes); B.touch_nodes = c.upload(touch_nodes);
-    B.ch_gp_start = c.upload(ch_gp_start); B.ch_ngp = c.upload(ch_ngp); B.ch_ne = c.upload(ch_ne);
-    B.ch_tptr_off = c.upload(ch_tptr_off); B.ch_tpos_off = c.upload(ch_tpos_off);
-    B.ch_gptr_off = c.upload(ch_gptr_off); B.ch_e_start = c.upload(ch_e_start);
+    B.ch_meta = c.upload(ch_meta); B.ch_ptr_off = c.upload(ch_ptr_off);
     B.bgp_c0 = c.upload(bgp_c0); B.bgp_region = c.upload(bgp_region);
     B.gptr = c.upload(gptr); B.be_idx = c.upload(be_idx); B.tptr = c.upload(tptr); B.tpos = c.upload(tpos);
-    B.be_fx = c.upload(be_fx); B.be_fy = c.upload(be_fy);
+    B.be_fxy = reinterpret_cast<double2*>(c.upload(be_fxy));
     B.blk_work = c.upload(blk_work);
     B.blk_split = c.upload(blk_split);
     B.split_pos = c.upload(split_pos);
@@ -461,7 +465,8 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     c.d_kin_all = c.upload(kin_all);
     c.n_kin_all = (int)kin_all.size();
     c.block_lds = block_lds_size(B);
-    if (c.block_lds > 160 * 1024) throw CapacityError{"node block needs more LDS than a workgroup has"};
+    if (c.block_lds + 24 * (size_t)CHUNK_ENTRIES + 256 > 160 * 1024)
+        throw CapacityError{"node block needs more LDS than a workgroup has"};
     HIP_OK(hipFuncSetAttribute((const void*)k_block_force_update, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)c.block_lds));
 }
@@ -479,6 +484,7 @@ void refresh_outputs(mel_ctx& c) {
 void update_proximity(mel_ctx& c) {
     DevModel& d = c.d;
     hipStream_t s = c.stream;
+    if (d.n_seg_chunks > 0) k_prox_chunk_boxes<<<blocks(d.n_seg_chunks), BLOCK, 0, s>>>(d);
     k_prox_aabb<<<blocks(d.n_borders), BLOCK, 0, s>>>(d);
     k_prox_neighbours<<<d.n_bodies, 64, 0, s>>>(d);
     k_prox_rows<<<blocks(d.n_bodies), BLOCK, 0, s>>>(d);
@@ -941,6 +947,22 @@ void build(mel_ctx& c) {
     d.pb_xmin = c.alloc<double>(NBR); d.pb_xmax = c.alloc<double>(NBR);
     d.pb_ymin = c.alloc<double>(NBR); d.pb_ymax = c.alloc<double>(NBR);
     d.pb_shmin = c.alloc<int>(NBR); d.pb_shmax = c.alloc<int>(NBR);
+    {
+        std::vector<int> chunk0(NBR + 1, 0), sc_border;
+        for (int br = 0; br < NBR; br++) {
+            int nch = (br_nslots[br] + SEG_CHUNK - 1) / SEG_CHUNK;
+            chunk0[br + 1] = chunk0[br] + nch;
+            for (int q = 0; q < nch; q++) sc_border.push_back(br);
+        }
+        d.n_seg_chunks = chunk0[NBR];
+        d.br_chunk0 = c.upload(chunk0);
+        d.sc_border = c.upload(sc_border);
+        size_t nc = (size_t)d.n_seg_chunks;
+        d.sc_nxmin = c.alloc<double>(nc); d.sc_nxmax = c.alloc<double>(nc);
+        d.sc_nymin = c.alloc<double>(nc); d.sc_nymax = c.alloc<double>(nc);
+        d.sc_xmin = c.alloc<double>(nc); d.sc_xmax = c.alloc<double>(nc);
+        d.sc_ymin = c.alloc<double>(nc); d.sc_ymax = c.alloc<double>(nc);
+    }
     d.nb_cap = 256;
     d.nb_count = c.alloc<int>(NB);
     d.nb_list = c.alloc<int>((size_t)NB * d.nb_cap * 3);

@@ -117,6 +117,10 @@ struct DevModel {
     // proximity scratch
     double *pb_xmin, *pb_xmax, *pb_ymin, *pb_ymax;   // per border AABB
     int *pb_shmin, *pb_shmax;
+    int n_seg_chunks;                        // chunks of SEG_CHUNK border slots
+    int *br_chunk0, *sc_border;              // first chunk per border (n_borders + 1), border per chunk
+    double *sc_nxmin, *sc_nxmax, *sc_nymin, *sc_nymax;   // slot-node box inflated by detection
+    double *sc_xmin, *sc_xmax, *sc_ymin, *sc_ymax;       // segment box (slot shifts applied)
     int nb_cap;                              // neighbour capacity per body
     int *nb_count, *nb_list;                 // per body: count, 3 ints per neighbour
     int *row_count, *row_off;                // narrow-phase rows
