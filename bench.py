@@ -50,6 +50,24 @@ def algorithmic_bytes(kernel, info):
     return None
 
 
+def measured_traffic(kernel, info):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary, used only when it was measured on this workload with the
+    current kernel source; otherwise None."""
+    import hashlib
+    path = os.path.join(ROOT, "profiles", "%s_1000grains.json" % kernel)
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        rec = json.load(f)
+    with open(os.path.join(ROOT, "melody2d_amd", "csrc", "k_block.hip"), "rb") as f:
+        src = hashlib.sha256(f.read()).hexdigest()[:16]
+    if rec.get("k_block_sha256_16") != src or rec.get("nodes") != info.n_nodes or \
+            rec.get("support_entries") != info.nnz_gauss:
+        return None, None
+    return rec["traffic_bytes"], os.path.relpath(path[:-5] + ".md", ROOT)
+
+
 def cpu_baseline(model_path, n_nodes, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import ctypes
@@ -134,9 +152,12 @@ def main():
         avg_s = ms / n * 1e-3
         b = algorithmic_bytes(best, info)
         achieved = b / avg_s / 1e9
+        traffic, src = measured_traffic(best, info)
         roof = {"kernel": best, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes": b}
+        if src:
+            roof["traffic_source"] = src
         for k in dominant:
             log("kernel %-18s %8.2f us/launch" % (k, kt[k][0] / max(kt[k][1], 1) * 1e3))
 

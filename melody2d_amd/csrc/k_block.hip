@@ -32,13 +32,15 @@ static_assert(CHUNK_GPS <= BLOCK_THREADS, "one Gauss point per thread");
 // stress terms of Code/Model.h:19-73 for one Gauss point, written so that a
 // support entry contributes c0 * (phix * P.x + phiy * P.y), c0 * (phix * P.z + phiy * P.w)
 __device__ inline double4 gp_terms(int mat_type, double Mu, double K2, const uint16_t* eidx, const double2* exy,
-                                   const double* ux, const double* uy, int k0, int k1) {
+                                   const double2* uv, int k0, int k1) {
     if (mat_type == 0) {
         double E11 = 0., E12 = 0., E22 = 0.;
+#pragma unroll 4
         for (int k = k0; k < k1; k++) {
             int j = eidx[k];
             double2 f = exy[k];
-            double u = ux[j], v = uy[j];
+            double2 w = uv[j];
+            double u = w.x, v = w.y;
             E11 += f.x * u;
             E12 += f.x * v + f.y * u;
             E22 += f.y * v;
@@ -51,10 +53,12 @@ __device__ inline double4 gp_terms(int mat_type, double Mu, double K2, const uin
         return make_double4(S11, S12, S12, S22);
     }
     double F11 = 1., F12 = 0., F21 = 0., F22 = 1.;
+#pragma unroll 4
     for (int k = k0; k < k1; k++) {
         int j = eidx[k];
         double2 f = exy[k];
-        double u = ux[j], v = uy[j];
+        double2 w = uv[j];
+        double u = w.x, v = w.y;
         F11 += f.x * u;
         F12 += f.y * u;
         F21 += f.x * v;
@@ -119,7 +123,7 @@ __device__ inline void load_chunk(const BlockArrays& B, int4 meta, int2 po, int 
 
 __global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2)))
 k_block_force_update(DevModel m, BlockArrays B) {
-    extern __shared__ __align__(16) double s_u[];          // ux[max_touch], uy[max_touch]
+    extern __shared__ __align__(16) double2 s_uv[];        // (ux, uy) of the touched nodes
     __shared__ __align__(16) double2 s_exy[CHUNK_ENTRIES];
     __shared__ __align__(16) uint16_t s_eidx[CHUNK_ENTRIES];
     __shared__ __align__(16) uint16_t s_tpos[CHUNK_ENTRIES];
@@ -137,8 +141,6 @@ k_block_force_update(DevModel m, BlockArrays B) {
     const int ntouch = B.blk_ntouch[blk];
     const int* own = B.own_nodes + B.blk_own_off[blk];
     const int* touch = B.touch_nodes + B.blk_touch_off[blk];
-    double* s_ux = s_u;
-    double* s_uy = s_u + B.max_touch;
     const int mat = m.body_material[body];
     const int mtype = m.mat_type[mat];
     const double Mu = m.mat_p[6 * mat + 3], K2 = m.mat_p[6 * mat + 4];
@@ -155,8 +157,7 @@ k_block_force_update(DevModel m, BlockArrays B) {
     }
     for (int i = t; i < ntouch; i += BLOCK_THREADS) {
         int g = touch[i];
-        s_ux[i] = m.upx[g];
-        s_uy[i] = m.upy[g];
+        s_uv[i] = make_double2(m.upx[g], m.upy[g]);
     }
     double tx[NODES_PER_THREAD], ty[NODES_PER_THREAD], rx[NODES_PER_THREAD], ry[NODES_PER_THREAD];
     int cur[NODES_PER_THREAD];
@@ -185,7 +186,8 @@ k_block_force_update(DevModel m, BlockArrays B) {
         // next chunk's loads stay in flight during this chunk's work
         if (c + 1 < nch) load_chunk(B, s_meta[c + 1], s_po[c + 1], nown, t, R);
         if (t < ngp) {
-            double4 P = gp_terms(mtype, Mu, K2, s_eidx, s_exy, s_ux, s_uy, gk0, gk1);
+            double4 P = gp_terms(mtype, Mu, K2, s_eidx, s_exy, s_uv, gk0, gk1);
+#pragma unroll 4
             for (int k = gk0; k < gk1; k++) {
                 double2 f = s_exy[k];
                 s_exy[k] = make_double2(gc0 * (f.x * P.x + f.y * P.y), gc0 * (f.x * P.z + f.y * P.w));
@@ -197,6 +199,7 @@ k_block_force_update(DevModel m, BlockArrays B) {
         for (int q = 0; q < NODES_PER_THREAD; q++) {
             int o = t + q * BLOCK_THREADS;
             if (o >= nown) break;
+#pragma unroll 4
             for (int p = tp0[q]; p < tp1[q]; p++) {
                 int k = s_tpos[p];
                 double2 v = s_exy[k];

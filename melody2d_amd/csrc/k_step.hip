@@ -390,17 +390,23 @@ __global__ void k_rigid_forces(DevModel m) {
                 terms[tid][3] = fxd; terms[tid][4] = fyd; terms[tid][5] = -fxd * dy + fyd * dx;
             }
             __syncthreads();
-            if (tid == 0) {
+            // six serial sums, one per component, each in slot order
+            if (tid < 6) {
                 int n = min(256, ns - base);
-                for (int q = 0; q < n; q++) {
-                    fn[0] += terms[q][0]; fn[1] += terms[q][1]; fn[2] += terms[q][2];
-                    fd[0] += terms[q][3]; fd[1] += terms[q][4]; fd[2] += terms[q][5];
-                }
+                double acc = tid < 3 ? fn[tid] : fd[tid - 3];
+#pragma unroll 16
+                for (int q = 0; q < n; q++) acc += terms[q][tid];
+                if (tid < 3) fn[tid] = acc;
+                else fd[tid - 3] = acc;
             }
         }
     }
+    __shared__ double fsum[6];
+    if (tid < 3) fsum[tid] = fn[tid];
+    else if (tid < 6) fsum[tid] = fd[tid - 3];
+    __syncthreads();
     if (tid != 0) return;
-    for (int q = 0; q < 3; q++) { R[R_FNEU + q] = fn[q]; R[R_FDIR + q] = fd[q]; }
+    for (int q = 0; q < 3; q++) { R[R_FNEU + q] = fsum[q]; R[R_FDIR + q] = fsum[3 + q]; }
     double gx = m.gravity_x, gy = m.gravity_y;
     if (m.body_status[b] == 2) { gx = 0.; gy = 0.; }
     R[R_FBODY] = m.body_mass[b] * gx;
