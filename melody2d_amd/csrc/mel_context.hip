@@ -304,6 +304,10 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
         int v = std::atoi(e);
         if (v >= 1 && v < BLOCK_MAX_OWN) own_cap = v;
     }
+    // MELODY2D_EXACT_SUMS=1: internal forces summed in the reference's order
+    // (bit-identical where pow rounds alike); default: balanced order
+    const char* ex = std::getenv("MELODY2D_EXACT_SUMS");
+    const bool exact = ex && std::atoi(ex) != 0;
     for (size_t b = 0; b < m.bodies.size(); b++) {
         const Body& Bd = m.bodies[b];
         if (Bd.rigid) continue;
@@ -370,32 +374,65 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
             touch_nodes.insert(touch_nodes.end(), touched.begin(), touched.end());
             blk_chunk_off.push_back((int)ch_meta.size());
             {
+                // regions only shape the rounding order; the balanced order
+                // sums all of a node's entries in one accumulator
                 int r0 = gps.empty() ? 0 : (gps.front() - gp_off[b]) / perreg[b];
                 int r1 = gps.empty() ? 0 : (gps.back() - gp_off[b]) / perreg[b];
-                blk_single.push_back(r0 == r1 ? 1 : 0);
+                blk_single.push_back(!exact || r0 == r1 ? 1 : 0);
             }
             int nown = (int)owned.size();
-            size_t q = 0;
-            while (q < gps.size()) {
-                size_t q1 = q;
-                int ne = 0;
-                while (q1 < gps.size() && (int)(q1 - q) < CHCAP) {
-                    int cnt = g_ptr[gps[q1] + 1] - g_ptr[gps[q1]];
-                    if (cnt > ECAP) throw CapacityError{"Gauss point support larger than the chunk capacity"};
-                    if (ne + cnt > ECAP) break;
-                    ne += cnt;
-                    q1++;
+            for (int g : gps)
+                if (g_ptr[g + 1] - g_ptr[g] > ECAP) throw CapacityError{"Gauss point support larger than the chunk capacity"};
+            std::vector<std::vector<int>> chunks;
+            if (exact) {
+                // contiguous Gauss-point ranges: every node sums its entries
+                // in Gauss-point order, region by region, as the reference does
+                size_t q = 0;
+                while (q < gps.size()) {
+                    std::vector<int> ch;
+                    int ne = 0;
+                    while (q < gps.size() && (int)ch.size() < CHCAP) {
+                        int cnt = g_ptr[gps[q] + 1] - g_ptr[gps[q]];
+                        if (ne + cnt > ECAP) break;
+                        ne += cnt;
+                        ch.push_back(gps[q++]);
+                    }
+                    chunks.push_back(ch);
                 }
+            } else {
+                // Gauss points dealt round-robin over the chunks: a node's
+                // entries spread evenly over the chunks, which balances the
+                // per-node gather (sum order: chunk by chunk, Gauss-point
+                // order inside a chunk)
+                long long tot = 0;
+                for (int g : gps) tot += g_ptr[g + 1] - g_ptr[g];
+                int nch = (int)std::max<long long>({1, (tot + ECAP * 15 / 16 - 1) / (ECAP * 15 / 16),
+                                                    ((long long)gps.size() + CHCAP - 1) / CHCAP});
+                for (;; nch++) {
+                    chunks.assign(nch, {});
+                    std::vector<int> ne(nch, 0);
+                    bool ok = true;
+                    for (size_t a = 0; a < gps.size() && ok; a++) {
+                        int k = (int)(a % nch), g = gps[a];
+                        ne[k] += g_ptr[g + 1] - g_ptr[g];
+                        chunks[k].push_back(g);
+                        ok = ne[k] <= ECAP && (int)chunks[k].size() <= CHCAP;
+                    }
+                    if (ok) break;
+                }
+            }
+            for (const std::vector<int>& chunk : chunks) {
+                int ne = 0;
+                for (int g : chunk) ne += g_ptr[g + 1] - g_ptr[g];
                 if (be_idx.size() / 8 > (size_t)INT32_MAX) throw CapacityError{"support entries exceed the chunk index range"};
-                ch_meta.push_back(make_int4((int)bgp_c0.size(), (int)(q1 - q), ne, (int)(be_idx.size() / 8)));
+                ch_meta.push_back(make_int4((int)bgp_c0.size(), (int)chunk.size(), ne, (int)(be_idx.size() / 8)));
                 ch_ptr_off.push_back(make_int2((int)gptr.size(), (int)tptr.size()));
                 const size_t ebase = be_idx.size();
                 const int ne8 = (ne + 7) & ~7;
                 std::vector<std::vector<uint16_t>> lst(nown);
                 int pos = 0;
                 gptr.push_back(0);
-                for (size_t a = q; a < q1; a++) {
-                    int g = gps[a];
+                for (int g : chunk) {
                     bgp_c0.push_back(g_c0[g]);
                     bgp_region.push_back((g - gp_off[b]) / perreg[b]);
                     for (int k = g_ptr[g]; k < g_ptr[g + 1]; k++) {
@@ -408,18 +445,19 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
                     }
                     gptr.push_back(pos);
                 }
+                // destination slot of every entry in owned-node order: node i
+                // owns slots tptr[i] .. tptr[i+1]-1, its entries in Gauss-point order
                 int acc = 0;
                 tptr.push_back(0);
+                tpos.resize(ebase + ne8, (uint16_t)CHUNK_ENTRIES);   // sink slot for entries of other blocks' nodes
                 for (int i = 0; i < nown; i++) {
+                    for (size_t r = 0; r < lst[i].size(); r++) tpos[ebase + lst[i][r]] = (uint16_t)(acc + r);
                     acc += (int)lst[i].size();
                     tptr.push_back(acc);
-                    tpos.insert(tpos.end(), lst[i].begin(), lst[i].end());
                 }
                 // pad the entry arrays of the chunk to a multiple of 8 entries
                 be_idx.resize(ebase + ne8, 0);
                 be_fxy.resize(2 * (ebase + ne8), 0.);
-                tpos.resize(ebase + ne8, 0);
-                q = q1;
             }
             blk_nchunk.push_back((int)ch_meta.size() - blk_chunk_off.back());
             if (blk_nchunk.back() > MAX_BLOCK_CHUNKS)
@@ -465,7 +503,7 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     c.d_kin_all = c.upload(kin_all);
     c.n_kin_all = (int)kin_all.size();
     c.block_lds = block_lds_size(B);
-    if (c.block_lds + 24 * (size_t)CHUNK_ENTRIES + 256 > 160 * 1024)
+    if (c.block_lds + 40 * (size_t)CHUNK_ENTRIES + 2048 > 160 * 1024)
         throw CapacityError{"node block needs more LDS than a workgroup has"};
     HIP_OK(hipFuncSetAttribute((const void*)k_block_force_update, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)c.block_lds));

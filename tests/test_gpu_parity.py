@@ -47,10 +47,12 @@ def test_device_matches_oracle(name, golden_dir, tmp_path):
         assert not bad, "step %d: %s" % (s, sorted(bad.items(), key=lambda kv: -kv[1])[:8])
 
 
-def test_internal_forces_bitwise_on_contact_free_start(golden_dir, tmp_path):
-    """Internal forces are gathered per region in Gauss-point order as the
-    reference does: at step 1 (same displacement parameters on both sides)
-    they are bit-identical wherever pow(J, -5/3) rounds as glibc does."""
+def test_internal_forces_bitwise_on_contact_free_start(golden_dir, tmp_path, monkeypatch):
+    """With MELODY2D_EXACT_SUMS=1 internal forces are gathered per region in
+    Gauss-point order as the reference does: at step 1 (same displacement
+    parameters on both sides) they are bit-identical wherever
+    pow(J, -5/3) rounds as glibc does."""
+    monkeypatch.setenv("MELODY2D_EXACT_SUMS", "1")
     model, _ = golden_dir("tiny")
     dev, ref = _run_pair(model, [1], tmp_path)[1]
     for b in range(2):
@@ -69,6 +71,19 @@ def test_split_node_blocks_match_oracle(name, golden_dir, tmp_path, monkeypatch)
     model, _ = golden_dir(name)
     print("case split", name, flush=True)
     for s, (dev, ref) in _run_pair(model, [1, 10, 50], tmp_path).items():
+        res = compare(dev, ref)
+        bad = {k: v[0] for k, v in res.items() if not v[0] <= TOL}
+        assert not bad, "step %d: %s" % (s, sorted(bad.items(), key=lambda kv: -kv[1])[:8])
+
+
+@pytest.mark.parametrize("name", ["tiny_bc", "tiny_czm"])
+def test_exact_sum_order_matches_oracle(name, golden_dir, tmp_path, monkeypatch):
+    """The reference-order gather (contiguous Gauss-point chunks, per-region
+    partial sums) meets the same bar as the default balanced order."""
+    monkeypatch.setenv("MELODY2D_EXACT_SUMS", "1")
+    model, _ = golden_dir(name)
+    print("case exact", name, flush=True)
+    for s, (dev, ref) in _run_pair(model, [1, 10, 50, 200], tmp_path).items():
         res = compare(dev, ref)
         bad = {k: v[0] for k, v in res.items() if not v[0] <= TOL}
         assert not bad, "step %d: %s" % (s, sorted(bad.items(), key=lambda kv: -kv[1])[:8])
