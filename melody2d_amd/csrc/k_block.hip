@@ -8,15 +8,15 @@
 //
 // Data path per block: the displacement parameters of the touched nodes are
 // staged in LDS once.  The block's support entries stream from HBM chunk by
-// chunk with 16-byte loads into registers; a chunk's loads are issued two
-// chunks ahead and written to LDS while the previous chunk is being summed.
-// Per chunk, one thread per Gauss point forms F and the stress terms from LDS
-// and writes each entry's contribution c0 * (dphi . P) to the entry's slot in
-// owned-node order (its destination position); then one thread per owned
-// node reads its contiguous slots - its entries in Gauss-point order - and
-// subtracts them per region, in the order of the reference's per-region
-// accumulators.  HBM traffic per entry: 16 (dphi/dx, dphi/dy) + 2 (node
-// index) + 2 (destination position).
+// chunk with 16-byte loads into registers; the loads of chunk c + 1 are in
+// flight while chunk c is computed.  Per chunk, one thread per Gauss point
+// forms F and the stress terms from LDS and overwrites its entries with their
+// contributions c0 * (dphi . P); then each thread sums, for its owned nodes,
+// the contributions listed by the chunk's transposed index (a node's entries
+// in Gauss-point order), per region when the reference order is requested.
+// A block is 128 threads and needs < 40 KB of LDS, so four blocks share a CU
+// and a 1000-grain model is resident at once.  HBM traffic per entry: 16
+// (dphi/dx, dphi/dy) + 2 (node index) + 2 (transposed position).
 #include "mel_device.hpp"
 #include "k_block.hpp"
 
@@ -25,10 +25,10 @@ namespace mel {
 namespace {
 
 constexpr int VEC16 = CHUNK_ENTRIES / 8;    // uint16x8 vectors per chunk array
+constexpr int VEC_PER_THREAD = (VEC16 + BLOCK_THREADS - 1) / BLOCK_THREADS;
 // native vector types (LLVM vectors) keep the in-flight chunk in registers
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-static_assert(VEC16 <= BLOCK_THREADS, "one uint16x8 vector per thread");
 static_assert(CHUNK_GPS <= BLOCK_THREADS, "one Gauss point per thread");
 
 // stress terms of Code/Model.h:19-73 for one Gauss point, written so that a
@@ -85,7 +85,7 @@ __device__ inline double4 gp_terms(int mat_type, double Mu, double K2, const uin
 // one chunk's data in flight, held in registers until it is written to LDS
 struct ChunkRegs {
     f64x2 f[ENTRIES_PER_THREAD];
-    u32x4 iv, tv;
+    u32x4 iv[VEC_PER_THREAD], tv[VEC_PER_THREAD];
     int gk0, gk1, greg;
     double gc0;
     int tp0[NODES_PER_THREAD], tp1[NODES_PER_THREAD];
@@ -110,9 +110,13 @@ __device__ inline void load_chunk(const BlockArrays& B, int4 meta, int2 po, int 
             R.tp1[q] = B.tptr[po.y + o + 1];
         }
     }
-    if (t < ne8 / 8) {
-        R.iv = reinterpret_cast<const u32x4*>(B.be_idx + E)[t];
-        R.tv = reinterpret_cast<const u32x4*>(B.tpos + E)[t];
+#pragma unroll
+    for (int q = 0; q < VEC_PER_THREAD; q++) {
+        int v = t + q * BLOCK_THREADS;
+        if (v < ne8 / 8) {
+            R.iv[q] = reinterpret_cast<const u32x4*>(B.be_idx + E)[v];
+            R.tv[q] = reinterpret_cast<const u32x4*>(B.tpos + E)[v];
+        }
     }
 #pragma unroll
     for (int q = 0; q < ENTRIES_PER_THREAD; q++) {
@@ -126,11 +130,10 @@ __device__ inline void load_chunk(const BlockArrays& B, int4 meta, int2 po, int 
 __global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2)))
 k_block_force_update(DevModel m, BlockArrays B) {
     extern __shared__ __align__(16) double2 s_uv[];        // (ux, uy) of the touched nodes
-    __shared__ __align__(16) double2 s_exy[CHUNK_ENTRIES];  // dphi/dx, dphi/dy in Gauss-point order
-    __shared__ __align__(16) double2 s_cxy[CHUNK_ENTRIES + 1];  // contributions in owned-node order (+ sink slot)
+    __shared__ __align__(16) double2 s_exy[CHUNK_ENTRIES];  // dphi/dx, dphi/dy, then contributions
     __shared__ __align__(16) uint16_t s_eidx[CHUNK_ENTRIES];
-    __shared__ __align__(16) uint16_t s_epos[CHUNK_ENTRIES];
-    __shared__ __align__(16) uint16_t s_creg[CHUNK_ENTRIES + 8];
+    __shared__ __align__(16) uint16_t s_tpos[CHUNK_ENTRIES];
+    __shared__ __align__(16) uint16_t s_ereg[CHUNK_ENTRIES];
     __shared__ int4 s_meta[MAX_BLOCK_CHUNKS];
     __shared__ int2 s_po[MAX_BLOCK_CHUNKS];
     __shared__ double red[BLOCK_THREADS / 64][5];
@@ -167,63 +170,73 @@ k_block_force_update(DevModel m, BlockArrays B) {
 #pragma unroll
     for (int q = 0; q < NODES_PER_THREAD; q++) { tx[q] = ty[q] = rx[q] = ry[q] = 0.; cur[q] = -1; }
 
-    // chunk c's per-thread items, copied out of the in-flight registers at commit
-    int ngp = 0, gk0 = 0, gk1 = 0, greg = 0;
-    double gc0 = 0.;
-    int tp0[NODES_PER_THREAD], tp1[NODES_PER_THREAD];
-    auto commit = [&](int c) {
+    for (int c = 0; c < nch; c++) {
+        __syncthreads();                      // previous chunk's LDS reads are done
         const int4 meta = s_meta[c];
-        const int ne8 = (meta.z + 7) & ~7;
+        const int ngp = meta.y, ne8 = (meta.z + 7) & ~7;
 #pragma unroll
         for (int q = 0; q < ENTRIES_PER_THREAD; q++) {
             int i = t + q * BLOCK_THREADS;
             if (i < ne8) reinterpret_cast<f64x2*>(s_exy)[i] = R.f[q];
         }
-        if (t < ne8 / 8) {
-            reinterpret_cast<u32x4*>(s_eidx)[t] = R.iv;
-            reinterpret_cast<u32x4*>(s_epos)[t] = R.tv;
+#pragma unroll
+        for (int q = 0; q < VEC_PER_THREAD; q++) {
+            int v = t + q * BLOCK_THREADS;
+            if (v < ne8 / 8) {
+                reinterpret_cast<u32x4*>(s_eidx)[v] = R.iv[q];
+                reinterpret_cast<u32x4*>(s_tpos)[v] = R.tv[q];
+            }
         }
-        ngp = meta.y;
-        gk0 = R.gk0; gk1 = R.gk1; greg = R.greg; gc0 = R.gc0;
+        const int gk0 = R.gk0, gk1 = R.gk1, greg = R.greg;
+        const double gc0 = R.gc0;
+        int tp0[NODES_PER_THREAD], tp1[NODES_PER_THREAD];
 #pragma unroll
         for (int q = 0; q < NODES_PER_THREAD; q++) { tp0[q] = R.tp0[q]; tp1[q] = R.tp1[q]; }
-    };
-    __syncthreads();                          // chunk descriptors staged
-    if (nch > 0) commit(0);
-    if (nch > 1) load_chunk(B, s_meta[1], s_po[1], nown, t, R);
-
-    for (int c = 0; c < nch; c++) {
-        __syncthreads();      // chunk c committed; the node pass of chunk c - 1 is done with s_cxy
+        __syncthreads();
+        // next chunk's loads stay in flight during this chunk's work
+        if (c + 1 < nch) load_chunk(B, s_meta[c + 1], s_po[c + 1], nown, t, R);
         if (t < ngp) {
             double4 P = gp_terms(mtype, Mu, K2, s_eidx, s_exy, s_uv, gk0, gk1);
-            // entries of nodes the block does not own go to the sink slot
-            // CHUNK_ENTRIES, so the loop has no branch and its loads batch
+            // contributions overwrite the entries in place (each thread owns
+            // its Gauss point's entries)
 #pragma unroll 4
             for (int k = gk0; k < gk1; k++) {
                 double2 f = s_exy[k];
-                int e = s_epos[k];
-                s_cxy[e] = make_double2(gc0 * (f.x * P.x + f.y * P.y), gc0 * (f.x * P.z + f.y * P.w));
+                s_exy[k] = make_double2(gc0 * (f.x * P.x + f.y * P.y), gc0 * (f.x * P.z + f.y * P.w));
             }
             if (!single)
-                for (int k = gk0; k < gk1; k++) s_creg[s_epos[k]] = (uint16_t)greg;
+                for (int k = gk0; k < gk1; k++) s_ereg[k] = (uint16_t)greg;
         }
-        __syncthreads();      // contributions visible; chunk c's input arrays are free
-#pragma unroll
-        for (int q = 0; q < NODES_PER_THREAD; q++) {
-            int o = t + q * BLOCK_THREADS;
-            if (o >= nown) break;
-            if (single) {
-                cur[q] = 0;
-#pragma unroll 4
-                for (int p = tp0[q]; p < tp1[q]; p++) {
-                    double2 v = s_cxy[p];
-                    rx[q] -= v.x;
-                    ry[q] -= v.y;
+        __syncthreads();
+        if (single) {
+            // both owned nodes of the thread advance together
+            const int n0 = t < nown ? tp1[0] - tp0[0] : 0;
+            const int n1 = NODES_PER_THREAD > 1 && t + BLOCK_THREADS < nown ? tp1[NODES_PER_THREAD - 1] - tp0[NODES_PER_THREAD - 1] : 0;
+            if (n0 > 0) cur[0] = 0;
+            if (n1 > 0) cur[NODES_PER_THREAD - 1] = 0;
+            const int nmax = max(n0, n1);
+#pragma unroll 2
+            for (int i = 0; i < nmax; i++) {
+                if (i < n0) {
+                    double2 v = s_exy[s_tpos[tp0[0] + i]];
+                    rx[0] -= v.x;
+                    ry[0] -= v.y;
                 }
-            } else {
+                if (i < n1) {
+                    double2 v = s_exy[s_tpos[tp0[NODES_PER_THREAD - 1] + i]];
+                    rx[NODES_PER_THREAD - 1] -= v.x;
+                    ry[NODES_PER_THREAD - 1] -= v.y;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NODES_PER_THREAD; q++) {
+                int o = t + q * BLOCK_THREADS;
+                if (o >= nown) break;
                 for (int p = tp0[q]; p < tp1[q]; p++) {
-                    double2 v = s_cxy[p];
-                    int r = s_creg[p];
+                    int k = s_tpos[p];
+                    double2 v = s_exy[k];
+                    int r = s_ereg[k];
                     if (r != cur[q]) {
                         if (cur[q] >= 0) { tx[q] += rx[q]; ty[q] += ry[q]; }
                         rx[q] = 0.; ry[q] = 0.; cur[q] = r;
@@ -232,12 +245,6 @@ k_block_force_update(DevModel m, BlockArrays B) {
                     ry[q] -= v.y;
                 }
             }
-        }
-        // stage chunk c + 1 while other waves still sum chunk c, then put
-        // chunk c + 2 in flight
-        if (c + 1 < nch) {
-            commit(c + 1);
-            if (c + 2 < nch) load_chunk(B, s_meta[c + 2], s_po[c + 2], nown, t, R);
         }
     }
 

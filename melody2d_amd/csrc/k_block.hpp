@@ -8,10 +8,8 @@
 // uint16) and are padded to a multiple of 8:
 //   be_fxy  dphi/dx, dphi/dy per support entry (double2)
 //   be_idx  block-local touched-node index per entry
-//   tpos    per entry, its destination slot in owned-node order (owned node o
-//           owns slots tptr[o] .. tptr[o+1] - 1, its entries in Gauss-point
-//           order); CHUNK_ENTRIES (a sink slot) for entries of nodes the
-//           block does not own
+//   tpos    per owned node (in tptr order), the chunk-local positions of its
+//           entries in Gauss-point order: the transposed index of the gather
 // Per Gauss point: gptr (entry offsets, chunk local), c0 = w * jac, region;
 // per owned node: tptr (offsets into tpos, chunk local).
 // A body larger than one block is split; its blocks read displacement
@@ -24,10 +22,10 @@
 
 namespace mel {
 
-constexpr int BLOCK_THREADS = 256;
+constexpr int BLOCK_THREADS = 128;
 constexpr int NODES_PER_THREAD = 2;
 constexpr int BLOCK_MAX_OWN = BLOCK_THREADS * NODES_PER_THREAD;
-constexpr int ENTRIES_PER_THREAD = 7;
+constexpr int ENTRIES_PER_THREAD = 12;
 constexpr int CHUNK_ENTRIES = BLOCK_THREADS * ENTRIES_PER_THREAD;   // entries per chunk
 constexpr int CHUNK_GPS = BLOCK_THREADS;                            // Gauss points per chunk
 constexpr int MAX_BLOCK_CHUNKS = 64;                                // chunks per node block

@@ -445,16 +445,17 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
                     }
                     gptr.push_back(pos);
                 }
-                // destination slot of every entry in owned-node order: node i
-                // owns slots tptr[i] .. tptr[i+1]-1, its entries in Gauss-point order
+                // transposed index: per owned node, the positions of its
+                // entries in Gauss-point order
                 int acc = 0;
                 tptr.push_back(0);
-                tpos.resize(ebase + ne8, (uint16_t)CHUNK_ENTRIES);   // sink slot for entries of other blocks' nodes
+                tpos.resize(ebase, 0);
                 for (int i = 0; i < nown; i++) {
-                    for (size_t r = 0; r < lst[i].size(); r++) tpos[ebase + lst[i][r]] = (uint16_t)(acc + r);
+                    tpos.insert(tpos.end(), lst[i].begin(), lst[i].end());
                     acc += (int)lst[i].size();
                     tptr.push_back(acc);
                 }
+                tpos.resize(ebase + ne8, 0);
                 // pad the entry arrays of the chunk to a multiple of 8 entries
                 be_idx.resize(ebase + ne8, 0);
                 be_fxy.resize(2 * (ebase + ne8), 0.);
@@ -503,7 +504,7 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     c.d_kin_all = c.upload(kin_all);
     c.n_kin_all = (int)kin_all.size();
     c.block_lds = block_lds_size(B);
-    if (c.block_lds + 40 * (size_t)CHUNK_ENTRIES + 2048 > 160 * 1024)
+    if (c.block_lds + 22 * (size_t)CHUNK_ENTRIES + 2048 > 160 * 1024)
         throw CapacityError{"node block needs more LDS than a workgroup has"};
     HIP_OK(hipFuncSetAttribute((const void*)k_block_force_update, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)c.block_lds));
