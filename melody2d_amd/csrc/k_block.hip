@@ -7,16 +7,19 @@
 //   Body::Apply_Euler (works)     Code/Body.h:2051-2068, Node::Apply_Euler Code/Node.h:239-245
 //
 // Data path per block: the displacement parameters of the touched nodes are
-// staged in LDS once.  The block's support entries stream from HBM chunk by
-// chunk with 16-byte loads into registers; the loads of chunk c + 1 are in
-// flight while chunk c is computed.  Per chunk, one thread per Gauss point
-// forms F and the stress terms from LDS and overwrites its entries with their
-// contributions c0 * (dphi . P); then each thread sums, for its owned nodes,
-// the contributions listed by the chunk's transposed index (a node's entries
-// in Gauss-point order), per region when the reference order is requested.
-// A block is 128 threads and needs < 40 KB of LDS, so four blocks share a CU
-// and a 1000-grain model is resident at once.  HBM traffic per entry: 16
-// (dphi/dx, dphi/dy) + 2 (node index) + 2 (transposed position).
+// staged in LDS once.  Chunk by chunk, every Gauss point is a lane that holds
+// its own support entries (dphi/dx, dphi/dy, node index, destination slot)
+// in registers, loaded with coalesced 16-byte loads from the wave's
+// entry-major stream while the previous chunk is summed.  The lane gathers
+// all its displacements from LDS at once, forms F and the stress terms
+// (Code/Model.h:19-73) summing in the reference's entry order, and writes
+// each entry's contribution c0 * (dphi . P) to the entry's slot in
+// owned-node order.  Then each thread sums, for its owned nodes, their
+// contiguous slots - a node's entries in Gauss-point order - per region when
+// the reference order is requested.  A block is 128 threads and needs < 40 KB
+// of LDS, so four blocks share a CU and a 1000-grain model is resident at
+// once.  HBM traffic per entry: 16 (dphi/dx, dphi/dy) + 4 (index and slot),
+// times the padding of each wave to its largest support.
 #include "mel_device.hpp"
 #include "k_block.hpp"
 
@@ -24,48 +27,25 @@ namespace mel {
 
 namespace {
 
-constexpr int VEC16 = CHUNK_ENTRIES / 8;    // uint16x8 vectors per chunk array
-constexpr int VEC_PER_THREAD = (VEC16 + BLOCK_THREADS - 1) / BLOCK_THREADS;
-// native vector types (LLVM vectors) keep the in-flight chunk in registers
+static_assert(BLOCK_THREADS == 128, "chunk descriptors describe two waves of lanes");
+// native vector types (LLVM vectors) keep in-flight loads in registers
 typedef double f64x2 __attribute__((ext_vector_type(2)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-static_assert(CHUNK_GPS <= BLOCK_THREADS, "one Gauss point per thread");
 
-// stress terms of Code/Model.h:19-73 for one Gauss point, written so that a
-// support entry contributes c0 * (phix * P.x + phiy * P.y), c0 * (phix * P.z + phiy * P.w)
-__device__ inline double4 gp_terms(int mat_type, double Mu, double K2, const uint16_t* eidx, const double2* exy,
-                                   const double2* uv, int k0, int k1) {
-    if (mat_type == 0) {
-        double E11 = 0., E12 = 0., E22 = 0.;
-#pragma unroll 4
-        for (int k = k0; k < k1; k++) {
-            int j = eidx[k];
-            double2 f = exy[k];
-            double2 w = uv[j];
-            double u = w.x, v = w.y;
-            E11 += f.x * u;
-            E12 += f.x * v + f.y * u;
-            E22 += f.y * v;
-        }
-        E12 = 0.5 * E12;
+// stress terms of Code/Model.h:19-73 for one Gauss point from its gradient
+// sums (Code/Body.h:975-1014): linear material (a, b, c) = (E11, 2 E12, E22),
+// neo-Hookean (a, b, c, d) = (F11, F12, F21, F22).  Written so that a support
+// entry contributes c0 * (phix * P.x + phiy * P.y), c0 * (phix * P.z + phiy * P.w).
+template <int MAT>
+__device__ inline double4 stress_terms(double Mu, double K2, double a, double b, double c, double d) {
+    if (MAT == 0) {
+        double E11 = a, E12 = 0.5 * b, E22 = c;
         double J = 1. + E11 + E22;
         double S33 = K2 * (J - 1.);
         double M2 = 2 * Mu;
         double S11 = S33 + M2 * E11, S22 = S33 + M2 * E22, S12 = M2 * E12;
         return make_double4(S11, S12, S12, S22);
     }
-    double F11 = 1., F12 = 0., F21 = 0., F22 = 1.;
-#pragma unroll 4
-    for (int k = k0; k < k1; k++) {
-        int j = eidx[k];
-        double2 f = exy[k];
-        double2 w = uv[j];
-        double u = w.x, v = w.y;
-        F11 += f.x * u;
-        F12 += f.y * u;
-        F21 += f.x * v;
-        F22 += f.y * v;
-    }
+    double F11 = a, F12 = b, F21 = c, F22 = d;
     double F11F12 = F11 * F11 + F12 * F12;
     double F21F22 = F21 * F21 + F22 * F22;
     double J = F11 * F22 - F12 * F21;
@@ -82,26 +62,60 @@ __device__ inline double4 gp_terms(int mat_type, double Mu, double K2, const uin
     return make_double4(F11 * S11 + F12 * S12, F12 * S22 + F11 * S12, F21 * S11 + F22 * S12, F22 * S22 + F21 * S12);
 }
 
-// one chunk's data in flight, held in registers until it is written to LDS
-struct ChunkRegs {
-    f64x2 f[ENTRIES_PER_THREAD];
-    u32x4 iv[VEC_PER_THREAD], tv[VEC_PER_THREAD];
-    int gk0, gk1, greg;
-    double gc0;
+// one support entry added in the reference's entry order; a padding entry
+// has zero derivatives and adds +-0, which leaves every sum unchanged (none
+// of them is ever -0)
+template <int MAT>
+__device__ inline void grad_add(double& a, double& b, double& c, double& d, f64x2 f, f64x2 w) {
+    const double u = w.x, v = w.y;
+    if (MAT == 0) {
+        a += f.x * u;
+        b += f.x * v + f.y * u;
+        c += f.y * v;
+    } else {
+        a += f.x * u;
+        b += f.y * u;
+        c += f.x * v;
+        d += f.y * v;
+    }
+}
+
+__device__ inline double2 contribution(double c0, f64x2 f, const double4& P) {
+    return make_double2(c0 * (f.x * P.x + f.y * P.y), c0 * (f.x * P.z + f.y * P.w));
+}
+
+// where a lane's entries live in the chunk's stream
+struct LaneMap {
+    int valid;      // lane holds a Gauss point
+    int lane;       // lane index within the chunk
+    int K;          // the wave's largest support: entries per lane
+    int stride;     // lanes in the wave
+    size_t base;    // entry 0 of this lane
+};
+
+__device__ inline LaneMap lane_map(int4 meta, int t) {
+    const int w = t >> 6, lw = t & 63;
+    const int K0 = meta.w & 255, K1 = (meta.w >> 8) & 255, n0 = (meta.w >> 16) & 255, n1 = (meta.w >> 24) & 255;
+    LaneMap L;
+    L.stride = w ? n1 : n0;
+    L.K = w ? K1 : K0;
+    L.valid = lw < L.stride;
+    L.lane = 64 * w + lw;
+    L.base = (size_t)meta.z + (w ? (size_t)K0 * n0 : 0) + lw;
+    return L;
+}
+
+// one lane's chunk data in flight, held in registers until its Gauss-point work
+struct LaneRegs {
+    f64x2 f[KREG_ENTRIES];
+    unsigned pk[KREG_ENTRIES];
+    double c0;
+    int reg;
     int tp0[NODES_PER_THREAD], tp1[NODES_PER_THREAD];
 };
 
-__device__ inline void load_chunk(const BlockArrays& B, int4 meta, int2 po, int nown, int t, ChunkRegs& R) {
-    const int ngp = meta.y, ne8 = (meta.z + 7) & ~7;
-    const size_t E = 8 * (size_t)meta.w;
-    // small per-thread items first (the counter of outstanding loads is in
-    // order); int32 offsets need no unpacking, so nothing waits here
-    if (t < ngp) {
-        R.gk0 = B.gptr[po.x + t];
-        R.gk1 = B.gptr[po.x + t + 1];
-        R.gc0 = B.bgp_c0[meta.x + t];
-        R.greg = B.bgp_region[meta.x + t];
-    }
+__device__ inline void load_lane(const BlockArrays& B, int4 meta, int2 po, int nown, int t, LaneRegs& R) {
+    const LaneMap L = lane_map(meta, t);
 #pragma unroll
     for (int q = 0; q < NODES_PER_THREAD; q++) {
         int o = t + q * BLOCK_THREADS;
@@ -110,18 +124,60 @@ __device__ inline void load_chunk(const BlockArrays& B, int4 meta, int2 po, int 
             R.tp1[q] = B.tptr[po.y + o + 1];
         }
     }
+    if (L.valid) {
+        R.c0 = B.bgp_c0[meta.x + L.lane];
+        R.reg = B.bgp_region[meta.x + L.lane];
 #pragma unroll
-    for (int q = 0; q < VEC_PER_THREAD; q++) {
-        int v = t + q * BLOCK_THREADS;
-        if (v < ne8 / 8) {
-            R.iv[q] = reinterpret_cast<const u32x4*>(B.be_idx + E)[v];
-            R.tv[q] = reinterpret_cast<const u32x4*>(B.tpos + E)[v];
+        for (int i = 0; i < KREG_ENTRIES; i++) {
+            if (i < L.K) {
+                size_t e = L.base + (size_t)i * L.stride;
+                R.f[i] = reinterpret_cast<const f64x2*>(B.be_fxy)[e];
+                R.pk[i] = B.be_pk[e];
+            }
         }
     }
+}
+
+// the Gauss-point work of one lane: gradient, stress, contributions to slots
+template <int MAT>
+__device__ inline void gp_lane(const BlockArrays& B, const LaneMap& L, const LaneRegs& R, const f64x2* uv,
+                               double2* s_cxy, uint16_t* s_creg, bool single, double Mu, double K2) {
+    constexpr int HALF = KREG_ENTRIES / 2;
+    double a = MAT == 0 ? 0. : 1., b = 0., c = 0., d = MAT == 0 ? 0. : 1.;
+    // displacement gathers in two batches of register entries
+    f64x2 u[HALF];
 #pragma unroll
-    for (int q = 0; q < ENTRIES_PER_THREAD; q++) {
-        int i = t + q * BLOCK_THREADS;
-        if (i < ne8) R.f[q] = reinterpret_cast<const f64x2*>(B.be_fxy)[E + i];
+    for (int i = 0; i < HALF; i++)
+        if (i < L.K) u[i] = uv[R.pk[i] & 0xFFFF];
+#pragma unroll
+    for (int i = 0; i < HALF; i++)
+        if (i < L.K) grad_add<MAT>(a, b, c, d, R.f[i], u[i]);
+#pragma unroll
+    for (int i = 0; i < HALF; i++)
+        if (HALF + i < L.K) u[i] = uv[R.pk[HALF + i] & 0xFFFF];
+#pragma unroll
+    for (int i = 0; i < HALF; i++)
+        if (HALF + i < L.K) grad_add<MAT>(a, b, c, d, R.f[HALF + i], u[i]);
+    // supports longer than the register window: straight from memory
+    const f64x2* fxy = reinterpret_cast<const f64x2*>(B.be_fxy);
+    for (int i = KREG_ENTRIES; i < L.K; i++) {
+        size_t e = L.base + (size_t)i * L.stride;
+        grad_add<MAT>(a, b, c, d, fxy[e], uv[B.be_pk[e] & 0xFFFF]);
+    }
+    const double4 P = stress_terms<MAT>(Mu, K2, a, b, c, d);
+#pragma unroll
+    for (int i = 0; i < KREG_ENTRIES; i++)
+        if (i < L.K) s_cxy[R.pk[i] >> 16] = contribution(R.c0, R.f[i], P);
+    for (int i = KREG_ENTRIES; i < L.K; i++) {
+        size_t e = L.base + (size_t)i * L.stride;
+        s_cxy[B.be_pk[e] >> 16] = contribution(R.c0, fxy[e], P);
+    }
+    if (!single) {
+#pragma unroll
+        for (int i = 0; i < KREG_ENTRIES; i++)
+            if (i < L.K) s_creg[R.pk[i] >> 16] = (uint16_t)R.reg;
+        for (int i = KREG_ENTRIES; i < L.K; i++)
+            s_creg[B.be_pk[L.base + (size_t)i * L.stride] >> 16] = (uint16_t)R.reg;
     }
 }
 
@@ -129,11 +185,8 @@ __device__ inline void load_chunk(const BlockArrays& B, int4 meta, int2 po, int 
 
 __global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2)))
 k_block_force_update(DevModel m, BlockArrays B) {
-    extern __shared__ __align__(16) double2 s_uv[];        // (ux, uy) of the touched nodes
-    __shared__ __align__(16) double2 s_exy[CHUNK_ENTRIES];  // dphi/dx, dphi/dy, then contributions
-    __shared__ __align__(16) uint16_t s_eidx[CHUNK_ENTRIES];
-    __shared__ __align__(16) uint16_t s_tpos[CHUNK_ENTRIES];
-    __shared__ __align__(16) uint16_t s_ereg[CHUNK_ENTRIES];
+    extern __shared__ __align__(16) double2 s_dyn[];       // (ux, uy) of the touched nodes [, slot regions]
+    __shared__ __align__(16) double2 s_cxy[CHUNK_ENTRIES + 1];  // contributions by slot (+ sink)
     __shared__ int4 s_meta[MAX_BLOCK_CHUNKS];
     __shared__ int2 s_po[MAX_BLOCK_CHUNKS];
     __shared__ double red[BLOCK_THREADS / 64][5];
@@ -150,12 +203,14 @@ k_block_force_update(DevModel m, BlockArrays B) {
     const int mat = m.body_material[body];
     const int mtype = m.mat_type[mat];
     const double Mu = m.mat_p[6 * mat + 3], K2 = m.mat_p[6 * mat + 4];
+    double2* s_uv = s_dyn;
+    uint16_t* s_creg = reinterpret_cast<uint16_t*>(s_dyn + B.max_touch);
 
     const int nch = B.blk_nchunk[blk];
     const int4* gmeta = B.ch_meta + B.blk_chunk_off[blk];
     const int2* gpo = B.ch_ptr_off + B.blk_chunk_off[blk];
-    ChunkRegs R;
-    if (nch > 0) load_chunk(B, gmeta[0], gpo[0], nown, t, R);
+    LaneRegs R;
+    if (nch > 0) load_lane(B, gmeta[0], gpo[0], nown, t, R);
     // chunk descriptors in LDS: read without waiting on the entry stream
     for (int i = t; i < nch; i += BLOCK_THREADS) {
         s_meta[i] = gmeta[i];
@@ -171,61 +226,31 @@ k_block_force_update(DevModel m, BlockArrays B) {
     for (int q = 0; q < NODES_PER_THREAD; q++) { tx[q] = ty[q] = rx[q] = ry[q] = 0.; cur[q] = -1; }
 
     for (int c = 0; c < nch; c++) {
-        __syncthreads();                      // previous chunk's LDS reads are done
+        __syncthreads();      // descriptors and displacements staged; previous chunk's slots read
         const int4 meta = s_meta[c];
-        const int ngp = meta.y, ne8 = (meta.z + 7) & ~7;
-#pragma unroll
-        for (int q = 0; q < ENTRIES_PER_THREAD; q++) {
-            int i = t + q * BLOCK_THREADS;
-            if (i < ne8) reinterpret_cast<f64x2*>(s_exy)[i] = R.f[q];
+        const LaneMap L = lane_map(meta, t);
+        if (L.valid) {
+            const f64x2* uv = reinterpret_cast<const f64x2*>(s_uv);
+            if (mtype == 0) gp_lane<0>(B, L, R, uv, s_cxy, s_creg, single, Mu, K2);
+            else gp_lane<1>(B, L, R, uv, s_cxy, s_creg, single, Mu, K2);
         }
-#pragma unroll
-        for (int q = 0; q < VEC_PER_THREAD; q++) {
-            int v = t + q * BLOCK_THREADS;
-            if (v < ne8 / 8) {
-                reinterpret_cast<u32x4*>(s_eidx)[v] = R.iv[q];
-                reinterpret_cast<u32x4*>(s_tpos)[v] = R.tv[q];
-            }
-        }
-        const int gk0 = R.gk0, gk1 = R.gk1, greg = R.greg;
-        const double gc0 = R.gc0;
         int tp0[NODES_PER_THREAD], tp1[NODES_PER_THREAD];
 #pragma unroll
         for (int q = 0; q < NODES_PER_THREAD; q++) { tp0[q] = R.tp0[q]; tp1[q] = R.tp1[q]; }
-        __syncthreads();
-        // next chunk's loads stay in flight during this chunk's work
-        if (c + 1 < nch) load_chunk(B, s_meta[c + 1], s_po[c + 1], nown, t, R);
-        if (t < ngp) {
-            double4 P = gp_terms(mtype, Mu, K2, s_eidx, s_exy, s_uv, gk0, gk1);
-            // contributions overwrite the entries in place (each thread owns
-            // its Gauss point's entries)
-#pragma unroll 4
-            for (int k = gk0; k < gk1; k++) {
-                double2 f = s_exy[k];
-                s_exy[k] = make_double2(gc0 * (f.x * P.x + f.y * P.y), gc0 * (f.x * P.z + f.y * P.w));
-            }
-            if (!single)
-                for (int k = gk0; k < gk1; k++) s_ereg[k] = (uint16_t)greg;
-        }
-        __syncthreads();
+        // next chunk's entries load while this chunk is summed
+        if (c + 1 < nch) load_lane(B, s_meta[c + 1], s_po[c + 1], nown, t, R);
+        __syncthreads();      // contributions visible
         if (single) {
-            // both owned nodes of the thread advance together
-            const int n0 = t < nown ? tp1[0] - tp0[0] : 0;
-            const int n1 = NODES_PER_THREAD > 1 && t + BLOCK_THREADS < nown ? tp1[NODES_PER_THREAD - 1] - tp0[NODES_PER_THREAD - 1] : 0;
-            if (n0 > 0) cur[0] = 0;
-            if (n1 > 0) cur[NODES_PER_THREAD - 1] = 0;
-            const int nmax = max(n0, n1);
-#pragma unroll 2
-            for (int i = 0; i < nmax; i++) {
-                if (i < n0) {
-                    double2 v = s_exy[s_tpos[tp0[0] + i]];
-                    rx[0] -= v.x;
-                    ry[0] -= v.y;
-                }
-                if (i < n1) {
-                    double2 v = s_exy[s_tpos[tp0[NODES_PER_THREAD - 1] + i]];
-                    rx[NODES_PER_THREAD - 1] -= v.x;
-                    ry[NODES_PER_THREAD - 1] -= v.y;
+#pragma unroll
+            for (int q = 0; q < NODES_PER_THREAD; q++) {
+                int o = t + q * BLOCK_THREADS;
+                if (o >= nown) break;
+                if (tp1[q] > tp0[q]) cur[q] = 0;
+#pragma unroll 4
+                for (int p = tp0[q]; p < tp1[q]; p++) {
+                    double2 v = s_cxy[p];
+                    rx[q] -= v.x;
+                    ry[q] -= v.y;
                 }
             }
         } else {
@@ -234,9 +259,8 @@ k_block_force_update(DevModel m, BlockArrays B) {
                 int o = t + q * BLOCK_THREADS;
                 if (o >= nown) break;
                 for (int p = tp0[q]; p < tp1[q]; p++) {
-                    int k = s_tpos[p];
-                    double2 v = s_exy[k];
-                    int r = s_ereg[k];
+                    double2 v = s_cxy[p];
+                    int r = s_creg[p];
                     if (r != cur[q]) {
                         if (cur[q] >= 0) { tx[q] += rx[q]; ty[q] += ry[q]; }
                         rx[q] = 0.; ry[q] = 0.; cur[q] = r;

@@ -2,16 +2,19 @@
 //
 // A block owns up to BLOCK_THREADS * NODES_PER_THREAD nodes of one body (a
 // whole grain when it fits) and lists every Gauss point of that body whose
-// support touches an owned node, in ascending Gauss-point order, cut into
-// chunks of at most CHUNK_GPS points / CHUNK_ENTRIES support entries.  Every
-// chunk's entry arrays start at a multiple of 8 entries (16-byte vectors of
-// uint16) and are padded to a multiple of 8:
-//   be_fxy  dphi/dx, dphi/dy per support entry (double2)
-//   be_idx  block-local touched-node index per entry
-//   tpos    per owned node (in tptr order), the chunk-local positions of its
-//           entries in Gauss-point order: the transposed index of the gather
-// Per Gauss point: gptr (entry offsets, chunk local), c0 = w * jac, region;
-// per owned node: tptr (offsets into tpos, chunk local).
+// support touches an owned node, cut into chunks of at most BLOCK_THREADS
+// Gauss points and CHUNK_ENTRIES owned-node entries (contiguous Gauss-point
+// ranges in the reference order, or dealt round-robin for balance).  In a
+// chunk every Gauss point is a lane; lanes are sorted by support size and
+// grouped in waves of 64.  A wave's support entries are stored entry-major
+// (entry i of all its lanes contiguous, 16-byte aligned loads coalesce),
+// every lane padded to the wave's largest support with zero derivatives:
+//   be_fxy  dphi/dx, dphi/dy (double2)
+//   be_pk   touched-node index | destination slot << 16, where owned node o
+//           owns slots tptr[o] .. tptr[o+1] - 1 (its entries in Gauss-point
+//           order) and CHUNK_ENTRIES is a sink for entries of other blocks'
+//           nodes and for padding
+// Per lane: support size gl_cnt, c0 = w * jac, region.
 // A body larger than one block is split; its blocks read displacement
 // parameters owned by sibling blocks, so they write their updated ones to
 // stage_u (per own-node position) and k_commit_split publishes them after
@@ -25,9 +28,8 @@ namespace mel {
 constexpr int BLOCK_THREADS = 128;
 constexpr int NODES_PER_THREAD = 2;
 constexpr int BLOCK_MAX_OWN = BLOCK_THREADS * NODES_PER_THREAD;
-constexpr int ENTRIES_PER_THREAD = 12;
-constexpr int CHUNK_ENTRIES = BLOCK_THREADS * ENTRIES_PER_THREAD;   // entries per chunk
-constexpr int CHUNK_GPS = BLOCK_THREADS;                            // Gauss points per chunk
+constexpr int CHUNK_ENTRIES = 2048;                                 // owned-node entry slots per chunk
+constexpr int KREG_ENTRIES = 20;                                    // support entries held in registers per lane
 constexpr int MAX_BLOCK_CHUNKS = 64;                                // chunks per node block
 
 struct BlockArrays {
@@ -36,12 +38,14 @@ struct BlockArrays {
     int *blk_body, *blk_own_off, *blk_nown, *blk_touch_off, *blk_ntouch, *blk_chunk_off, *blk_nchunk;
     int* blk_single_region;    // 1 when every Gauss point of the block is in one region
     int *own_nodes, *touch_nodes;
-    int4* ch_meta;             // per chunk: (gp start, n gp, n entries, entry start / 8)
-    int2* ch_ptr_off;          // per chunk: (gptr offset, tptr offset)
+    int any_multi_region;
+    int4* ch_meta;             // per chunk: (lane start, n lanes, entry start, K0 | K1 << 8 | n0 << 16 | n1 << 24)
+    int2* ch_ptr_off;          // per chunk: (slots used, tptr offset)
     double* bgp_c0;
     int* bgp_region;
-    int *gptr, *tptr;          // chunk-local entry offsets per Gauss point / per owned node
-    uint16_t *be_idx, *tpos;
+    int* gl_cnt;               // support size per lane
+    int* tptr;                 // chunk-local slot offsets per owned node
+    uint32_t* be_pk;
     double2* be_fxy;
     double* blk_work;          // 14 per block: (sum, compensation) for the 7 works
     int* blk_split;            // 1 when the block's body spans several blocks

@@ -273,8 +273,11 @@ void check_flags(mel_ctx& c) {
 }
 
 
-// dynamic LDS of the block kernel: displacement parameters of the touched nodes
-size_t block_lds_size(const BlockArrays& B) { return 16 * (size_t)B.max_touch; }
+// dynamic LDS of the block kernel: displacement parameters of the touched
+// nodes, then the region of every contribution slot (reference order only)
+size_t block_lds_size(const BlockArrays& B) {
+    return 16 * (size_t)B.max_touch + (B.any_multi_region ? 2 * ((size_t)CHUNK_ENTRIES + 8) : 0);
+}
 
 // Node blocks of the fused internal-force kernel: whole bodies when they fit,
 // otherwise groups of nodes in Morton order; chunked Gauss-point streams and
@@ -285,7 +288,7 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
                   const std::vector<int>& bn_node, const std::vector<int>& node_off, const std::vector<int>& gp_off,
                   const std::vector<int>& perreg) {
     Model& m = c.model;
-    const int ECAP = CHUNK_ENTRIES, CHCAP = CHUNK_GPS;
+    const int ECAP = CHUNK_ENTRIES, CHCAP = BLOCK_THREADS;   // slots and Gauss points per chunk
     int N = node_off.back();
     std::vector<int> blk_body, blk_own_off, blk_nown, blk_touch_off, blk_ntouch, blk_chunk_off, blk_nchunk;
     std::vector<int> blk_single, own_nodes, touch_nodes;
@@ -293,8 +296,8 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     std::vector<int2> ch_ptr_off;
     std::vector<double> bgp_c0, be_fxy, blk_work;
     std::vector<int> bgp_region;
-    std::vector<int> gptr, tptr;
-    std::vector<uint16_t> be_idx, tpos;
+    std::vector<int> tptr, gl_cnt;
+    std::vector<uint32_t> be_pk;
     std::vector<int> local(N, -1), blk_split, split_pos;
     int max_touch = 1, max_own = 1;
     // MELODY2D_BLOCK_NODES lowers the nodes per block (exercises the split
@@ -422,43 +425,71 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
                 }
             }
             for (const std::vector<int>& chunk : chunks) {
-                int ne = 0;
-                for (int g : chunk) ne += g_ptr[g + 1] - g_ptr[g];
-                if (be_idx.size() / 8 > (size_t)INT32_MAX) throw CapacityError{"support entries exceed the chunk index range"};
-                ch_meta.push_back(make_int4((int)bgp_c0.size(), (int)chunk.size(), ne, (int)(be_idx.size() / 8)));
-                ch_ptr_off.push_back(make_int2((int)gptr.size(), (int)tptr.size()));
-                const size_t ebase = be_idx.size();
-                const int ne8 = (ne + 7) & ~7;
-                std::vector<std::vector<uint16_t>> lst(nown);
-                int pos = 0;
-                gptr.push_back(0);
-                for (int g : chunk) {
-                    bgp_c0.push_back(g_c0[g]);
-                    bgp_region.push_back((g - gp_off[b]) / perreg[b]);
-                    for (int k = g_ptr[g]; k < g_ptr[g + 1]; k++) {
-                        int li = local[g_idx[k]];
-                        be_idx.push_back((uint16_t)li);
-                        be_fxy.push_back(g_phix[k]);
-                        be_fxy.push_back(g_phiy[k]);
-                        if (li < nown) lst[li].push_back((uint16_t)pos);
-                        pos++;
-                    }
-                    gptr.push_back(pos);
-                }
-                // transposed index: per owned node, the positions of its
+                // lanes: the chunk's Gauss points by decreasing support size
+                // (stable), 64 per wave; a wave's entries are stored
+                // entry-major (entry i of its n_w lanes contiguous), each
+                // lane padded to the wave's largest support K_w with zero
+                // derivatives and the sink slot
+                const int ngp = (int)chunk.size();
+                std::vector<int> order(ngp);
+                for (int i = 0; i < ngp; i++) order[i] = i;
+                std::stable_sort(order.begin(), order.end(), [&](int a, int b2) {
+                    return g_ptr[chunk[a] + 1] - g_ptr[chunk[a]] > g_ptr[chunk[b2] + 1] - g_ptr[chunk[b2]];
+                });
+                // destination slot of every entry: per owned node, its
                 // entries in Gauss-point order
+                std::vector<int> first_entry(ngp + 1, 0);
+                for (int i = 0; i < ngp; i++)
+                    first_entry[i + 1] = first_entry[i] + g_ptr[chunk[i] + 1] - g_ptr[chunk[i]];
+                const int ne = first_entry[ngp];
+                std::vector<int> cnt(nown, 0), dst(ne, CHUNK_ENTRIES);
+                for (int i = 0; i < ngp; i++)
+                    for (int k = g_ptr[chunk[i]]; k < g_ptr[chunk[i] + 1]; k++) {
+                        int li = local[g_idx[k]];
+                        if (li < nown) cnt[li]++;
+                    }
                 int acc = 0;
                 tptr.push_back(0);
-                tpos.resize(ebase, 0);
-                for (int i = 0; i < nown; i++) {
-                    tpos.insert(tpos.end(), lst[i].begin(), lst[i].end());
-                    acc += (int)lst[i].size();
-                    tptr.push_back(acc);
+                std::vector<int> next(nown);
+                for (int i = 0; i < nown; i++) { next[i] = acc; acc += cnt[i]; tptr.push_back(acc); }
+                for (int i = 0; i < ngp; i++)
+                    for (int k = g_ptr[chunk[i]]; k < g_ptr[chunk[i] + 1]; k++) {
+                        int li = local[g_idx[k]];
+                        if (li < nown) dst[first_entry[i] + k - g_ptr[chunk[i]]] = next[li]++;
+                    }
+                int Kw[2] = {0, 0}, nw[2] = {0, 0};
+                for (int l = 0; l < ngp; l++) {
+                    int gi = order[l];
+                    nw[l / 64]++;
+                    Kw[l / 64] = std::max(Kw[l / 64], g_ptr[chunk[gi] + 1] - g_ptr[chunk[gi]]);
                 }
-                tpos.resize(ebase + ne8, 0);
-                // pad the entry arrays of the chunk to a multiple of 8 entries
-                be_idx.resize(ebase + ne8, 0);
-                be_fxy.resize(2 * (ebase + ne8), 0.);
+                if (Kw[0] > 255 || Kw[1] > 255) throw CapacityError{"Gauss point support larger than 255 entries"};
+                if (be_pk.size() > (size_t)INT32_MAX) throw CapacityError{"support entries exceed the chunk index range"};
+                ch_meta.push_back(make_int4((int)bgp_c0.size(), ngp, (int)be_pk.size(),
+                                            Kw[0] | (Kw[1] << 8) | (nw[0] << 16) | (nw[1] << 24)));
+                ch_ptr_off.push_back(make_int2(ne, (int)tptr.size() - nown - 1));
+                for (int l = 0; l < ngp; l++) {
+                    int g = chunk[order[l]];
+                    bgp_c0.push_back(g_c0[g]);
+                    bgp_region.push_back((g - gp_off[b]) / perreg[b]);
+                    gl_cnt.push_back(g_ptr[g + 1] - g_ptr[g]);
+                }
+                for (int w = 0; w < 2; w++) {
+                    const size_t base = be_pk.size();
+                    be_pk.resize(base + (size_t)Kw[w] * nw[w], (uint32_t)CHUNK_ENTRIES << 16);
+                    be_fxy.resize(2 * (base + (size_t)Kw[w] * nw[w]), 0.);
+                    for (int lw = 0; lw < nw[w]; lw++) {
+                        int gi = order[64 * w + lw], g = chunk[gi];
+                        for (int k = g_ptr[g]; k < g_ptr[g + 1]; k++) {
+                            int i = k - g_ptr[g];
+                            size_t e = base + (size_t)i * nw[w] + lw;
+                            int li = local[g_idx[k]];
+                            be_pk[e] = (uint32_t)li | ((uint32_t)dst[first_entry[gi] + i] << 16);
+                            be_fxy[2 * e] = g_phix[k];
+                            be_fxy[2 * e + 1] = g_phiy[k];
+                        }
+                    }
+                }
             }
             blk_nchunk.push_back((int)ch_meta.size() - blk_chunk_off.back());
             if (blk_nchunk.back() > MAX_BLOCK_CHUNKS)
@@ -489,10 +520,16 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     B.blk_touch_off = c.upload(blk_touch_off); B.blk_ntouch = c.upload(blk_ntouch);
     B.blk_chunk_off = c.upload(blk_chunk_off); B.blk_nchunk = c.upload(blk_nchunk);
     B.blk_single_region = c.upload(blk_single);
+    B.any_multi_region = 0;
+    for (int v : blk_single) B.any_multi_region |= v ? 0 : 1;
     B.own_nodes = c.upload(own_nodes); B.touch_nodes = c.upload(touch_nodes);
     B.ch_meta = c.upload(ch_meta); B.ch_ptr_off = c.upload(ch_ptr_off);
     B.bgp_c0 = c.upload(bgp_c0); B.bgp_region = c.upload(bgp_region);
-    B.gptr = c.upload(gptr); B.be_idx = c.upload(be_idx); B.tptr = c.upload(tptr); B.tpos = c.upload(tpos);
+    // slack so that a lane reading a full register window past the end of
+    // the stream stays inside the allocation
+    be_pk.resize(be_pk.size() + 64 * KREG_ENTRIES, (uint32_t)CHUNK_ENTRIES << 16);
+    be_fxy.resize(2 * be_pk.size(), 0.);
+    B.gl_cnt = c.upload(gl_cnt); B.be_pk = c.upload(be_pk); B.tptr = c.upload(tptr);
     B.be_fxy = reinterpret_cast<double2*>(c.upload(be_fxy));
     B.blk_work = c.upload(blk_work);
     B.blk_split = c.upload(blk_split);
@@ -504,7 +541,7 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     c.d_kin_all = c.upload(kin_all);
     c.n_kin_all = (int)kin_all.size();
     c.block_lds = block_lds_size(B);
-    if (c.block_lds + 22 * (size_t)CHUNK_ENTRIES + 2048 > 160 * 1024)
+    if (c.block_lds + 16 * ((size_t)CHUNK_ENTRIES + 1) + 2048 > 160 * 1024)
         throw CapacityError{"node block needs more LDS than a workgroup has"};
     HIP_OK(hipFuncSetAttribute((const void*)k_block_force_update, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)c.block_lds));
