@@ -30,32 +30,28 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(kernel, info):
-    """Bytes a launch must move at minimum, each array counted once (FP64
-    values).  N = nodes, G = Gauss points, nnz = Gauss-point support
-    entries."""
+def algorithmic_bytes(kernel, info, fp32=False):
+    """Bytes a launch must move at minimum, each array counted once.
+    N = nodes, G = Gauss points, nnz = Gauss-point support entries."""
     N, G, nnz = info.n_nodes, info.n_gauss, info.nnz_gauss
     if kernel == "block_force_update":
-        # per support entry: dphi/dx, dphi/dy (16) + local node index (2) +
-        # transposed position (2); per Gauss point: c0 (8), region (4),
-        # entry offset (2); per node: displacement parameters read + write
-        # (32), velocity parameters read + write (32), acceleration
-        # parameters and internal force written (32), masses, inverse masses
-        # and damping force read (48), node id and transposed offset (6)
-        return 20 * nnz + 14 * G + 150 * N
-    if kernel == "kinematics":
-        # per border node: DOF_TO_DISP row (index 4 + phi 8 per entry, read
-        # with the 6 parameters of each entry's node), 8 outputs
-        return None
+        # per support entry: dphi/dx, dphi/dy (16 B in FP64, 8 B in the
+        # FP32-mixed form) + node index and destination slot (4); per Gauss
+        # point: c0 (8) and region (4); per node: displacement parameters
+        # read + write (32), velocity parameters read + write (32),
+        # acceleration parameters and internal force written (32), masses,
+        # inverse masses and damping force read (48), node id and slot
+        # offset (6)
+        return (12 if fp32 else 20) * nnz + 12 * G + 150 * N
     return None
 
 
-def measured_traffic(kernel, info):
+def measured_traffic(kernel, info, fp32=False):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary, used only when it was measured on this workload with the
     current kernel source; otherwise None."""
     import hashlib
-    path = os.path.join(ROOT, "profiles", "%s_1000grains.json" % kernel)
+    path = os.path.join(ROOT, "profiles", "%s%s_1000grains.json" % (kernel, "32" if fp32 else ""))
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
@@ -93,6 +89,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="also time every kernel in a separate window")
+    ap.add_argument("--error-step", type=int, default=3000,
+                    help="FP64 step at which the FP32-mixed force error is evaluated")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (scale reference for HBM byte counters)")
     args = ap.parse_args()
@@ -122,70 +120,105 @@ def main():
     tmpdir = tempfile.mkdtemp(prefix="mel_bench_")
     path = os.path.join(tmpdir, "c2.bin")
     write_binary(model, path)
-    t0 = time.time()
-    ctx = native.Context(path, binary=True)
-    info = ctx.get_info()
-    log("loaded on device in %.1fs: nodes %d, gauss %d, nnz %d, contacts %d" % (
-        time.time() - t0, info.n_nodes, info.n_gauss, info.nnz_gauss, info.n_contacts))
 
-    ctx.euler_steps(args.warmup)
-    ctx.synchronize()
-    log("warmup done")
-    dominant = ["block_force_update"]
-    ctx.set_kernel_timing(dominant)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.euler_steps(args.steps)
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    kt = ctx.kernel_timing()
-    ctx.set_kernel_timing(False)
-    sec = t1 - t0
-    info = ctx.get_info()
-    value = info.n_nodes * args.steps / sec
-    log("timed %d steps in %.3fs: %.3e node-steps/s, %.3f ms/step" % (args.steps, sec, value, 1e3 * sec / args.steps))
-
-    roof = None
-    best = max(dominant, key=lambda k: kt[k][0])
-    ms, n = kt[best]
-    if n:
-        avg_s = ms / n * 1e-3
-        b = algorithmic_bytes(best, info)
-        achieved = b / avg_s / 1e9
-        traffic, src = measured_traffic(best, info)
-        roof = {"kernel": best, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes": b}
-        if src:
-            roof["traffic_source"] = src
-        for k in dominant:
-            log("kernel %-18s %8.2f us/launch" % (k, kt[k][0] / max(kt[k][1], 1) * 1e3))
-
-    breakdown = None
-    if args.breakdown:
-        ctx.set_kernel_timing(True)
-        ctx.euler_steps(min(args.steps, 50))
-        kt2 = ctx.kernel_timing()
+    def run(precision):
+        name = "fp64" if precision == native.FP64 else "fp32-mixed"
+        t0 = time.time()
+        ctx = native.Context(path, binary=True, precision=precision)
+        info = ctx.get_info()
+        log("[%s] loaded in %.1fs: nodes %d, gauss %d, nnz %d, contacts %d" % (
+            name, time.time() - t0, info.n_nodes, info.n_gauss, info.nnz_gauss, info.n_contacts))
+        ctx.euler_steps(args.warmup)
+        ctx.synchronize()
+        dominant = "block_force_update"
+        ctx.set_kernel_timing([dominant])
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.euler_steps(args.steps)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        kt = ctx.kernel_timing()
         ctx.set_kernel_timing(False)
-        breakdown = {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in kt2.items() if v[1]}
-        log("per-launch us: " + json.dumps(breakdown))
+        sec = t1 - t0
+        info = ctx.get_info()
+        value = info.n_nodes * args.steps / sec
+        log("[%s] timed %d steps in %.3fs: %.3e node-steps/s, %.3f ms/step" % (
+            name, args.steps, sec, value, 1e3 * sec / args.steps))
+        roof = None
+        ms, n = kt[dominant]
+        if n:
+            avg_s = ms / n * 1e-3
+            b = algorithmic_bytes(dominant, info, fp32=precision != native.FP64)
+            achieved = b / avg_s / 1e9
+            traffic, src = measured_traffic(dominant, info, fp32=precision != native.FP64)
+            roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes": b}
+            if src:
+                roof["traffic_source"] = src
+            log("[%s] kernel %s %.2f us/launch, %.1f GB/s" % (name, dominant, avg_s * 1e6, achieved))
+        breakdown = None
+        if args.breakdown:
+            ctx.set_kernel_timing(True)
+            ctx.euler_steps(min(args.steps, 50))
+            kt2 = ctx.kernel_timing()
+            ctx.set_kernel_timing(False)
+            breakdown = {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in kt2.items() if v[1]}
+            log("[%s] per-launch us: %s" % (name, json.dumps(breakdown)))
+        return ctx, info, value, sec, roof, breakdown
+
+    ctx64, info, value, sec, roof, breakdown = run(native.FP64)
+    ctx32, _, value32, sec32, roof32, breakdown32 = run(native.FP32_MIXED)
+
+    # relative internal-force error of the FP32-mixed form, from the FP64
+    # state: the FP64 run continues to step --error-step (until then the
+    # walls have not reached the grains, whose internal forces are pure
+    # rounding residue of the reference formula), then both forms take one
+    # step from the FP64 displacement parameters
+    t0 = time.time()
+    done = args.warmup + args.steps + (min(args.steps, 50) if args.breakdown else 0)
+    if args.error_step > done:
+        ctx64.euler_steps(args.error_step - done)
+    counts = [b.n_nodes for b in model.bodies]
+    rigid = [b.rigid for b in model.bodies]
+    fields = ["x_disp_param", "y_disp_param", "x_vel_param", "y_vel_param"]
+    for bi, n in enumerate(counts):
+        if rigid[bi]:
+            continue
+        for f in fields:
+            ctx32.set_node_field(bi, f, ctx64.node_field(bi, f, n))
+    ctx64.euler_steps(1)
+    ctx32.euler_steps(1)
+    num = den = 0.0
+    for bi, n in enumerate(counts):
+        if rigid[bi]:
+            continue
+        for f in ("x_internal_force", "y_internal_force"):
+            a = ctx64.node_field(bi, f, n)
+            d = ctx32.node_field(bi, f, n) - a
+            num += float(d @ d)
+            den += float(a @ a)
+    rel_err = (num / den) ** 0.5 if den > 0 else None
+    log("fp32-mixed internal-force error vs fp64 from the same state at step %d: %.3e (%.1fs)" % (
+        max(args.error_step, done), rel_err, time.time() - t0))
+    ctx64.close()
+    ctx32.close()
 
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         t0 = time.time()
-        v, s = cpu_baseline(path, info.n_nodes, args.cpu_steps, threads)
+        v, sc = cpu_baseline(path, info.n_nodes, args.cpu_steps, threads)
         log("cpu baseline: %.3e node-steps/s over %d steps (%.1fs stepping, %.1fs total)" % (
-            v, args.cpu_steps, s, time.time() - t0))
+            v, args.cpu_steps, sc, time.time() - t0))
         cpu = {"value": round(v, 1), "unit": "node-steps/s", "cores": threads, "kind": "port",
                "sample": "%d Euler steps of the same %d-node workload after its initialisation, "
                          "FP64 OpenMP oracle" % (args.cpu_steps, info.n_nodes)}
-    ctx.close()
     os.remove(path)
     os.rmdir(tmpdir)
 
     out = {
-        "metric": "node-steps/s, FP32-mixed vs FP64 at ≤1e-5 rel force err; p50/p99 job latency",
+        "metric": "node-steps/s, FP32-mixed vs FP64 at \u22641e-5 rel force err; p50/p99 job latency",
         "value": round(value, 1),
         "unit": "node-steps/s",
         "n_gpus": 1,
@@ -202,9 +235,17 @@ def main():
                    "contacts": info.n_contacts, "precision": "fp64", "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "fp32_mixed": {"value": round(value32, 1), "unit": "node-steps/s",
+                       "ms_per_step": round(1e3 * sec32 / args.steps, 4),
+                       "speedup_vs_fp64": round(value32 / value, 3),
+                       "rel_internal_force_err_vs_fp64": rel_err,
+                       "error_state_step": max(args.error_step, args.warmup + args.steps),
+                       "dtype": "f32 gradients/stresses/contributions, f64 state and sums",
+                       "roofline": roof32},
     }
     if breakdown:
         out["kernel_us"] = breakdown
+        out["fp32_mixed"]["kernel_us"] = breakdown32
     print(json.dumps(out), flush=True)
 
 

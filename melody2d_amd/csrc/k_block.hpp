@@ -9,7 +9,7 @@
 // grouped in waves of 64.  A wave's support entries are stored entry-major
 // (entry i of all its lanes contiguous, 16-byte aligned loads coalesce),
 // every lane padded to the wave's largest support with zero derivatives:
-//   be_fxy  dphi/dx, dphi/dy (double2)
+//   be_fxy  dphi/dx, dphi/dy (double2; float2 in be_fxy32 for the FP32-mixed form)
 //   be_pk   touched-node index | destination slot << 16, where owned node o
 //           owns slots tptr[o] .. tptr[o+1] - 1 (its entries in Gauss-point
 //           order) and CHUNK_ENTRIES is a sink for entries of other blocks'
@@ -29,7 +29,8 @@ constexpr int BLOCK_THREADS = 128;
 constexpr int NODES_PER_THREAD = 2;
 constexpr int BLOCK_MAX_OWN = BLOCK_THREADS * NODES_PER_THREAD;
 constexpr int CHUNK_ENTRIES = 2048;                                 // owned-node entry slots per chunk
-constexpr int KREG_ENTRIES = 20;                                    // support entries held in registers per lane
+constexpr int KREG_ENTRIES = 20;                                    // support entries held in registers per lane (FP64)
+constexpr int KREG_ENTRIES_F32 = 24;                                // the same for the FP32-mixed form
 constexpr int MAX_BLOCK_CHUNKS = 64;                                // chunks per node block
 
 struct BlockArrays {
@@ -46,7 +47,8 @@ struct BlockArrays {
     int* gl_cnt;               // support size per lane
     int* tptr;                 // chunk-local slot offsets per owned node
     uint32_t* be_pk;
-    double2* be_fxy;
+    double2* be_fxy;           // FP64 form
+    float2* be_fxy32;          // FP32-mixed form
     double* blk_work;          // 14 per block: (sum, compensation) for the 7 works
     int* blk_split;            // 1 when the block's body spans several blocks
     double* stage_u;           // 2 per own-node position (split blocks only)
@@ -57,6 +59,7 @@ struct BlockArrays {
 };
 
 __global__ void k_block_force_update(DevModel m, BlockArrays B);
+__global__ void k_block_force_update32(DevModel m, BlockArrays B);
 __global__ void k_commit_split(DevModel m, BlockArrays B);
 __global__ void k_kinematics_list(DevModel m, const int* nodes, int n);
 __global__ void k_node_outputs(DevModel m);

@@ -274,9 +274,12 @@ void check_flags(mel_ctx& c) {
 
 
 // dynamic LDS of the block kernel: displacement parameters of the touched
-// nodes, then the region of every contribution slot (reference order only)
-size_t block_lds_size(const BlockArrays& B) {
-    return 16 * (size_t)B.max_touch + (B.any_multi_region ? 2 * ((size_t)CHUNK_ENTRIES + 8) : 0);
+// nodes (FP64 pairs, or FP32 pairs in the FP32-mixed form), then the region
+// of every contribution slot (reference order only)
+size_t block_lds_size(const BlockArrays& B, bool f32) {
+    size_t u = (f32 ? 8 : 16) * (size_t)B.max_touch;
+    u = (u + 15) & ~size_t(15);
+    return u + (B.any_multi_region ? 2 * ((size_t)CHUNK_ENTRIES + 8) : 0);
 }
 
 // Node blocks of the fused internal-force kernel: whole bodies when they fit,
@@ -530,7 +533,14 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     be_pk.resize(be_pk.size() + 64 * KREG_ENTRIES, (uint32_t)CHUNK_ENTRIES << 16);
     be_fxy.resize(2 * be_pk.size(), 0.);
     B.gl_cnt = c.upload(gl_cnt); B.be_pk = c.upload(be_pk); B.tptr = c.upload(tptr);
-    B.be_fxy = reinterpret_cast<double2*>(c.upload(be_fxy));
+    if (c.precision == MEL_FP32_MIXED) {
+        std::vector<float> f32(be_fxy.begin(), be_fxy.end());
+        B.be_fxy32 = reinterpret_cast<float2*>(c.upload(f32));
+        B.be_fxy = nullptr;
+    } else {
+        B.be_fxy = reinterpret_cast<double2*>(c.upload(be_fxy));
+        B.be_fxy32 = nullptr;
+    }
     B.blk_work = c.upload(blk_work);
     B.blk_split = c.upload(blk_split);
     B.split_pos = c.upload(split_pos);
@@ -540,11 +550,12 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     B.n_kin = (int)kin.size();
     c.d_kin_all = c.upload(kin_all);
     c.n_kin_all = (int)kin_all.size();
-    c.block_lds = block_lds_size(B);
+    c.block_lds = block_lds_size(B, c.precision == MEL_FP32_MIXED);
     if (c.block_lds + 16 * ((size_t)CHUNK_ENTRIES + 1) + 2048 > 160 * 1024)
         throw CapacityError{"node block needs more LDS than a workgroup has"};
-    HIP_OK(hipFuncSetAttribute((const void*)k_block_force_update, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)c.block_lds));
+    HIP_OK(hipFuncSetAttribute(c.precision == MEL_FP32_MIXED ? (const void*)k_block_force_update32
+                                                             : (const void*)k_block_force_update,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.block_lds));
 }
 
 // interior kinematics and force fields for output, after steps have run
@@ -622,7 +633,9 @@ void euler_step(mel_ctx& c) {
     // Update_internal_forces, Sum_up_forces, Apply_Newton, Apply_Euler
     if (c.blocks_dev.n_blocks > 0)
         TIMED(c, KT_BLOCK_UPDATE,
-              (k_block_force_update<<<c.blocks_dev.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, c.blocks_dev)));
+              (c.precision == MEL_FP32_MIXED
+                   ? k_block_force_update32<<<c.blocks_dev.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, c.blocks_dev)
+                   : k_block_force_update<<<c.blocks_dev.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, c.blocks_dev)));
     if (c.blocks_dev.n_split > 0)
         k_commit_split<<<blocks(c.blocks_dev.n_split), BLOCK, 0, s>>>(d, c.blocks_dev);
     // Update_bc(t + dt), rigid bodies, Update_kinematics, Update_current_positions
@@ -1101,7 +1114,7 @@ thread_local std::string g_load_err;
 mel_status create(mel_ctx** out, int device, int precision, const std::function<bool(Model&, std::string&)>& loader) {
     if (!out) return MEL_E_ARG;
     *out = nullptr;
-    if (precision != MEL_FP64) { g_load_err = "only the FP64 path is available"; return MEL_E_UNSUPPORTED; }
+    if (precision != MEL_FP64 && precision != MEL_FP32_MIXED) { g_load_err = "unknown precision"; return MEL_E_ARG; }
     std::unique_ptr<mel_ctx> c(new mel_ctx());
     c->device = device;
     c->precision = precision;
