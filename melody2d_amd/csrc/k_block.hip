@@ -512,18 +512,7 @@ __global__ void k_commit_split(DevModel m, BlockArrays B) {
 __global__ void k_kinematics_list(DevModel m, const int* nodes, int n) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    int i = nodes[q];
-    double dx = 0., dy = 0., vx = 0., vy = 0., ax = 0., ay = 0.;
-    for (int k = m.d_ptr[i]; k < m.d_ptr[i + 1]; k++) {
-        int j = m.d_idx[k];
-        double f = m.d_phi[k];
-        dx += f * m.upx[j]; dy += f * m.upy[j];
-        vx += f * m.vpx[j]; vy += f * m.vpy[j];
-        ax += f * m.apx[j]; ay += f * m.apy[j];
-    }
-    m.ux[i] = dx; m.uy[i] = dy; m.vx[i] = vx; m.vy[i] = vy; m.ax[i] = ax; m.ay[i] = ay;
-    m.xc[i] = m.X[i] + dx;
-    m.yc[i] = m.Y[i] + dy;
+    kinematics_node(m, nodes[q]);
 }
 
 // force fields of deformable nodes rebuilt from the last step's components

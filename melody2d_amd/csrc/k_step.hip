@@ -12,12 +12,8 @@ namespace mel {
 // ---------------------------------------------------------------------------
 // Body::Update_borders (Code/Body.h:614-661): tributary length per slot.
 // ---------------------------------------------------------------------------
-__global__ void k_border_length(DevModel m) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= m.n_slots) return;
+__device__ inline double slot_length(const DevModel& m, int s) {
     int br = m.sl_border[s];
-    int b = m.br_body[br];
-    if (m.body_rigid_flag[b] || m.body_status[b] == 1) return;
     int j = m.sl_j[s];
     int n = m.br_nslots[br];
     double P = m.period;
@@ -28,22 +24,25 @@ __global__ void k_border_length(DevModel m) {
     double l21 = __dsqrt_rn((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
     double l10 = __dsqrt_rn((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
     bool simple = m.br_periodicity[br] == 1;
-    double L;
-    if (j == 0 && simple) L = l21 * 0.5;
-    else if (j == n - 1 && simple) L = l10 * 0.5;
-    else L = (l21 + l10) * 0.5;
-    m.sl_len[s] = L;
+    if (j == 0 && simple) return l21 * 0.5;
+    if (j == n - 1 && simple) return l10 * 0.5;
+    return (l21 + l10) * 0.5;
 }
 
-// ---------------------------------------------------------------------------
-// Body::Update_contacts (Code/Body.h:1341-1634): one thread per element.
-// ---------------------------------------------------------------------------
-__global__ void k_contact_project(DevModel m) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= *m.c_count) return;
-    int bS = m.C.bodyS[e];
-    if (m.body_status[bS] == 1) return;
-    project_element(m, m.C, e);
+// current tributary length of a slot: deformable borders from the current
+// positions (the step recomputes them where they are used), rigid borders
+// keep their stored value
+__device__ inline double slot_len_now(const DevModel& m, int s) {
+    int b = m.br_body[m.sl_border[s]];
+    return m.body_rigid_flag[b] ? m.sl_len[s] : slot_length(m, s);
+}
+
+__global__ void k_border_length(DevModel m) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m.n_slots) return;
+    int b = m.br_body[m.sl_border[s]];
+    if (m.body_rigid_flag[b] || m.body_status[b] == 1) return;
+    m.sl_len[s] = slot_length(m, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -102,11 +101,16 @@ __device__ inline void law_czm_linear(double kini, double Pnlim, double gapnlim,
     Pt = -kmax * gapt;
 }
 
-// Body::Update_contact_forces (Code/Body.h:1642-1941) minus the scatter: the
-// element's force and its point-force records.
-__global__ void k_contact_law(DevModel m) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= *m.c_count) return;
+__device__ inline void bucket_append(const DevModel& m, int bn, int r) {
+    int pos = atomicAdd(&m.bk_cursor[bn], 1);
+    if (pos < BUCKET_CAP) m.bk_rec[bn * BUCKET_CAP + pos] = r;
+    else atomicOr(&m.status_flags[0], 1);
+}
+
+// Body::Update_contact_forces (Code/Body.h:1642-1941) minus the scatter, for
+// one element after its re-projection (Body::Update_contacts,
+// Code/Body.h:1341-1634): the element's force and its point-force records
+__device__ inline void contact_law_element(const DevModel& m, int e) {
     const ContactArrays& C = m.C;
     m.rec_bn[3 * e] = -1; m.rec_bn[3 * e + 1] = -1; m.rec_bn[3 * e + 2] = -1;
     for (int q = 0; q < 6; q++) m.rig_rec[6 * e + q] = 0.;
@@ -116,7 +120,7 @@ __global__ void k_contact_law(DevModel m) {
     int li = m.law_table[m.body_material[bS] * m.n_materials + m.body_material[bM]];
     if (li < 0) return;
     int slot = slot_of(m, bS, C.borderS[e], C.bnodeS[e]);
-    double length = m.sl_len[slot];
+    double length = slot_len_now(m, slot);
     double length0 = m.sl_len0[slot];
     double gapn = C.gapn[e], gapt = C.gapt[e];
     const double* p = m.law_p + 12 * li;
@@ -178,7 +182,7 @@ __global__ void k_contact_law(DevModel m) {
     if (!m.body_rigid_flag[bS]) {
         int bn = m.sl_bn[slot];
         m.rec_bn[3 * e] = bn; m.rec_fx[3 * e] = Fx; m.rec_fy[3 * e] = Fy;
-        atomicAdd(&m.bk_count[bn], 1);
+        bucket_append(m, bn, 3 * e);
     } else {
         const double* R = m.body_rigid + bS * RIGID_FIELDS;
         double dx = m.xc[nodeS] - R[R_CUR], dy = m.yc[nodeS] - R[R_CUR + 1];
@@ -189,8 +193,8 @@ __global__ void k_contact_law(DevModel m) {
         int bn1 = m.sl_bn[sl], bn2 = m.sl_bn[sl + 1];
         m.rec_bn[3 * e + 1] = bn1; m.rec_fx[3 * e + 1] = -Fx * C.N1[e]; m.rec_fy[3 * e + 1] = -Fy * C.N1[e];
         m.rec_bn[3 * e + 2] = bn2; m.rec_fx[3 * e + 2] = -Fx * C.N2[e]; m.rec_fy[3 * e + 2] = -Fy * C.N2[e];
-        atomicAdd(&m.bk_count[bn1], 1);
-        atomicAdd(&m.bk_count[bn2], 1);
+        bucket_append(m, bn1, 3 * e + 1);
+        bucket_append(m, bn2, 3 * e + 2);
     } else {
         const double* R = m.body_rigid + bM * RIGID_FIELDS;
         double dx = m.xc[nodeS] - R[R_CUR] - C.shiftM[e] * m.period;
@@ -199,21 +203,8 @@ __global__ void k_contact_law(DevModel m) {
     }
 }
 
-// fixed-capacity buckets: record ids per border node
-__global__ void k_bucket_fill(DevModel m) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= 3 * *m.c_count) return;
-    int bn = m.rec_bn[r];
-    if (bn < 0) return;
-    int pos = atomicAdd(&m.bk_cursor[bn], 1);
-    if (pos < BUCKET_CAP) m.bk_rec[bn * BUCKET_CAP + pos] = r;
-    else atomicOr(&m.status_flags[0], 1);
-}
-
 // point contact force per border node: records summed in record-id order
-__global__ void k_bucket_sum(DevModel m) {
-    int bn = blockIdx.x * blockDim.x + threadIdx.x;
-    if (bn >= m.n_bnodes) return;
+__device__ inline void bucket_sum_node(const DevModel& m, int bn) {
     int n = min(m.bk_cursor[bn], BUCKET_CAP);
     int ids[BUCKET_CAP];
     for (int k = 0; k < n; k++) {
@@ -227,7 +218,6 @@ __global__ void k_bucket_sum(DevModel m) {
     m.bn_pcx[bn] = px;
     m.bn_pcy[bn] = py;
     m.bk_cursor[bn] = 0;
-    m.bk_count[bn] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -256,9 +246,7 @@ __device__ void bc_value(const DevModel& m, int type, int h0, int h1, double t, 
     }
 }
 
-__global__ void k_bc_eval(DevModel m, int plus_dt) {
-    int br = blockIdx.x * blockDim.x + threadIdx.x;
-    if (br >= m.n_borders) return;
+__device__ inline void bc_eval_border(const DevModel& m, int br, int plus_dt) {
     double t = *m.time;
     if (plus_dt) t = t + m.dt;
     const int* hp = m.br_hist_ptr + 2 * br;
@@ -266,16 +254,26 @@ __global__ void k_bc_eval(DevModel m, int plus_dt) {
     bc_value(m, m.br_ytype[br], hp[1], hp[2], t, m.br_yval[br], m.br_yvel[br]);
 }
 
+__global__ void k_bc_eval(DevModel m, int plus_dt) {
+    int br = blockIdx.x * blockDim.x + threadIdx.x;
+    if (br >= m.n_borders) return;
+    bc_eval_border(m, br, plus_dt);
+}
+
 // Body::Update_bc_forces deformable branch (Code/Body.h:672-815): point
 // Neumann / Dirichlet values per border node (one slot per border node in
 // practice; several slots are summed in slot order).
-__global__ void k_bc_point_forces(DevModel m) {
-    int bn = blockIdx.x * blockDim.x + threadIdx.x;
-    if (bn >= m.n_bnodes) return;
+__device__ inline void bc_point_force_node(const DevModel& m, int bn) {
     double cnx = 0., cny = 0., cdx = 0., cdy = 0.;
     int node = m.bn_node[bn];
     int b = m.node_body[node];
     if (!m.body_rigid_flag[b] && m.body_status[b] != 1) {
+        // Body::Update_borders (Code/Body.h:614-661) for this node's slots:
+        // the lengths the step and the outputs use
+        for (int q = m.bn_slot_ptr[bn]; q < m.bn_slot_ptr[bn + 1]; q++) {
+            int s = m.bn_slots[q];
+            m.sl_len[s] = slot_length(m, s);
+        }
         for (int q = m.bn_slot_ptr[bn]; q < m.bn_slot_ptr[bn + 1]; q++) {
             int s = m.bn_slots[q];
             int br = m.sl_border[s];
@@ -343,8 +341,8 @@ __global__ void k_damping(DevModel m) {
 // Body::Update_bc_forces rigid branch (Code/Body.h:816-899) + Update_body_forces
 // + the contact resultant (slave records and the serial sends of
 // Code/Body.h:1948-1967), reduced in element order.
-__global__ void k_rigid_forces(DevModel m) {
-    int b = m.rigid_ids[blockIdx.x];
+__device__ inline void rigid_forces_block(const DevModel& m, int rigid_index) {
+    int b = m.rigid_ids[rigid_index];
     if (m.body_status[b] == 1) return;
     double* R = m.body_rigid + b * RIGID_FIELDS;
     __shared__ double red[256][3];
@@ -441,10 +439,8 @@ __device__ void rigid_driven_dofs(const DevModel& m, int b, double t, double* dv
 
 // Sum_up_forces, Apply_Newton, Update_bc(t+dt), Apply_Euler for a rigid body
 // (Code/Body.h:1984-2014, 2032-2037, 2071-2117)
-__global__ void k_rigid_step(DevModel m) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= m.n_bodies) return;
-    if (!m.body_rigid_flag[b] || m.body_status[b] == 1) return;
+__device__ inline void rigid_step_body(const DevModel& m, int b) {
+    if (m.body_status[b] == 1) return;
     double* R = m.body_rigid + b * RIGID_FIELDS;
     double dt = m.dt;
     for (int q = 0; q < 3; q++)
@@ -493,9 +489,7 @@ __global__ void k_rigid_step(DevModel m) {
 
 // rigid node kinematics and positions (Code/Body.h:2214-2226, 2281-2301);
 // node force fields mirror the body totals as Code/Body.h:1999-2013 does.
-__global__ void k_rigid_nodes(DevModel m, int with_forces) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m.n_nodes) return;
+__device__ inline void rigid_node(const DevModel& m, int i, int with_forces) {
     int b = m.node_body[i];
     if (!m.body_rigid_flag[b] || m.body_status[b] == 1) return;
     const double* R = m.body_rigid + b * RIGID_FIELDS;
@@ -524,8 +518,63 @@ __global__ void k_rigid_nodes(DevModel m, int with_forces) {
     }
 }
 
-__global__ void k_advance_time(DevModel m) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *m.time = *m.time + m.dt;
+__global__ void k_rigid_nodes(DevModel m, int with_forces) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m.n_rigid_nodes) return;
+    rigid_node(m, m.rigid_node_ids[q], with_forces);
+}
+
+// ---------------------------------------------------------------------------
+// The step's launches.  Work of one dependency level shares a launch, each
+// kind of work on its own index range:
+//   k_step_contacts   Update_contacts + Update_contact_forces (per element,
+//                     point-force records appended to per-border-node
+//                     buckets) | Update_bc_forces deformable (per border node)
+//   k_step_gather     bucket sums (per border node) | rigid resultants (one
+//                     workgroup per rigid body)
+//   k_step_rigid      Update_bc(t + dt) (per border) | rigid Newton + Euler
+//   k_step_finish     kinematics of border nodes | rigid node kinematics |
+//                     time += dt
+// ---------------------------------------------------------------------------
+__global__ void k_step_contacts(DevModel m, int nc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nc) {
+        if (m.body_status[m.C.bodyS[i]] != 1) project_element(m, m.C, i);
+        contact_law_element(m, i);
+        return;
+    }
+    i -= nc;
+    if (i < m.n_bnodes) bc_point_force_node(m, i);
+}
+
+__global__ void k_step_gather(DevModel m) {
+    if ((int)blockIdx.x < m.n_rigid) {
+        rigid_forces_block(m, blockIdx.x);
+        return;
+    }
+    int bn = (blockIdx.x - m.n_rigid) * blockDim.x + threadIdx.x;
+    if (bn < m.n_bnodes) bucket_sum_node(m, bn);
+}
+
+__global__ void k_step_rigid(DevModel m) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m.n_borders) {
+        bc_eval_border(m, i, 1);
+        return;
+    }
+    i -= m.n_borders;
+    if (i < m.n_rigid) rigid_step_body(m, m.rigid_ids[i]);
+}
+
+__global__ void k_step_finish(DevModel m, const int* kin_nodes, int n_kin) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *m.time = *m.time + m.dt;     // nothing in this launch reads the time
+    if (i < n_kin) {
+        kinematics_node(m, kin_nodes[i]);
+        return;
+    }
+    i -= n_kin;
+    if (i < m.n_rigid_nodes) rigid_node(m, m.rigid_node_ids[i], 1);
 }
 
 }  // namespace mel

@@ -9,17 +9,13 @@ constexpr int BLOCK = 256;
 constexpr int SEG_CHUNK = 32;     // border slots per proximity culling chunk
 
 __global__ void k_border_length(DevModel m);
-__global__ void k_contact_project(DevModel m);
-__global__ void k_contact_law(DevModel m);
-__global__ void k_bucket_fill(DevModel m);
-__global__ void k_bucket_sum(DevModel m);
 __global__ void k_bc_eval(DevModel m, int plus_dt);
-__global__ void k_bc_point_forces(DevModel m);
 __global__ void k_damping(DevModel m);
-__global__ void k_rigid_forces(DevModel m);
-__global__ void k_rigid_step(DevModel m);
 __global__ void k_rigid_nodes(DevModel m, int with_forces);
-__global__ void k_advance_time(DevModel m);
+__global__ void k_step_contacts(DevModel m, int nc);
+__global__ void k_step_gather(DevModel m);
+__global__ void k_step_rigid(DevModel m);
+__global__ void k_step_finish(DevModel m, const int* kin_nodes, int n_kin);
 
 // proximity (k_proximity.hip)
 __global__ void k_prox_chunk_boxes(DevModel m);

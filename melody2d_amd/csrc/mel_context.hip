@@ -53,14 +53,11 @@ inline int blocks(long long n, int b = BLOCK) { return (int)std::max<long long>(
 
 // kernels timed individually by the optional event instrumentation
 enum KernelId {
-    KT_BORDER_LENGTH = 0, KT_CONTACT_PROJECT, KT_BC_POINT_FORCES, KT_DAMPING, KT_CONTACT_LAW, KT_BUCKET_FILL,
-    KT_BUCKET_SUM, KT_RIGID_FORCES, KT_BLOCK_UPDATE, KT_BC_EVAL, KT_RIGID_STEP, KT_KINEMATICS,
-    KT_RIGID_NODES, KT_ADVANCE_TIME, KT_PROXIMITY, KT_COUNT
+    KT_CONTACTS = 0, KT_GATHER, KT_DAMPING, KT_BLOCK_UPDATE, KT_RIGID_STEP, KT_FINISH, KT_PROXIMITY, KT_COUNT
 };
 static const char* kKernelNames[KT_COUNT] = {
-    "border_length", "contact_project", "bc_point_forces", "damping", "contact_law", "bucket_fill", "bucket_sum",
-    "rigid_forces", "block_force_update", "bc_eval", "rigid_step", "kinematics", "rigid_nodes",
-    "advance_time", "proximity_update"};
+    "contacts_bc", "gather_rigid", "damping", "block_force_update", "bc_rigid_step", "kinematics_finish",
+    "proximity_update"};
 
 struct mel_ctx {
     Model model;
@@ -617,19 +614,15 @@ void euler_step(mel_ctx& c) {
     DevModel& d = c.d;
     hipStream_t s = c.stream;
     int nc = c.n_contacts;
-    // Initialize_contact_forces + Update_borders + Update_contacts
-    TIMED(c, KT_BORDER_LENGTH, (k_border_length<<<blocks(d.n_slots), BLOCK, 0, s>>>(d)));
-    if (nc > 0) TIMED(c, KT_CONTACT_PROJECT, (k_contact_project<<<blocks(nc), BLOCK, 0, s>>>(d)));
-    // Update_bc_forces, Update_body_forces (in node_update), Update_damping_forces
-    TIMED(c, KT_BC_POINT_FORCES, (k_bc_point_forces<<<blocks(d.n_bnodes), BLOCK, 0, s>>>(d)));
+    // Initialize_contact_forces, Update_borders, Update_contacts,
+    // Update_contact_forces (records into buckets) | Update_bc_forces
+    if (nc + d.n_bnodes > 0)
+        TIMED(c, KT_CONTACTS, (k_step_contacts<<<blocks(nc + d.n_bnodes), BLOCK, 0, s>>>(d, nc)));
+    // Update_damping_forces
     if (d.n_matrix > 0) TIMED(c, KT_DAMPING, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d)));
-    // Update_contact_forces + Send_contact_forces
-    if (nc > 0) {
-        TIMED(c, KT_CONTACT_LAW, (k_contact_law<<<blocks(nc), BLOCK, 0, s>>>(d)));
-        TIMED(c, KT_BUCKET_FILL, (k_bucket_fill<<<blocks(3LL * nc), BLOCK, 0, s>>>(d)));
-    }
-    TIMED(c, KT_BUCKET_SUM, (k_bucket_sum<<<blocks(d.n_bnodes), BLOCK, 0, s>>>(d)));
-    if (d.n_rigid > 0) TIMED(c, KT_RIGID_FORCES, (k_rigid_forces<<<d.n_rigid, 256, 0, s>>>(d)));
+    // Send_contact_forces (bucket sums) | rigid resultants
+    if (d.n_rigid + d.n_bnodes > 0)
+        TIMED(c, KT_GATHER, (k_step_gather<<<d.n_rigid + blocks(d.n_bnodes), BLOCK, 0, s>>>(d)));
     // Update_internal_forces, Sum_up_forces, Apply_Newton, Apply_Euler
     if (c.blocks_dev.n_blocks > 0)
         TIMED(c, KT_BLOCK_UPDATE,
@@ -638,15 +631,13 @@ void euler_step(mel_ctx& c) {
                    : k_block_force_update<<<c.blocks_dev.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, c.blocks_dev)));
     if (c.blocks_dev.n_split > 0)
         k_commit_split<<<blocks(c.blocks_dev.n_split), BLOCK, 0, s>>>(d, c.blocks_dev);
-    // Update_bc(t + dt), rigid bodies, Update_kinematics, Update_current_positions
-    TIMED(c, KT_BC_EVAL, (k_bc_eval<<<blocks(d.n_borders), BLOCK, 0, s>>>(d, 1)));
-    if (d.n_rigid > 0) TIMED(c, KT_RIGID_STEP, (k_rigid_step<<<blocks(d.n_bodies), BLOCK, 0, s>>>(d)));
-    // kinematics of border nodes only: the step reads positions and
-    // velocities of border nodes; interior ones are rebuilt on request
-    if (c.blocks_dev.n_kin > 0)
-        TIMED(c, KT_KINEMATICS, (k_kinematics_list<<<blocks(c.blocks_dev.n_kin), BLOCK, 0, s>>>(d, c.blocks_dev.kin_nodes, c.blocks_dev.n_kin)));
-    if (d.n_rigid > 0) TIMED(c, KT_RIGID_NODES, (k_rigid_nodes<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d, 1)));
-    TIMED(c, KT_ADVANCE_TIME, (k_advance_time<<<1, 64, 0, s>>>(d)));
+    // Update_bc(t + dt) | rigid bodies
+    if (d.n_borders + d.n_rigid > 0)
+        TIMED(c, KT_RIGID_STEP, (k_step_rigid<<<blocks(d.n_borders + d.n_rigid), BLOCK, 0, s>>>(d)));
+    // Update_kinematics, Update_current_positions (border nodes; interior
+    // ones are rebuilt on request) | rigid nodes | time += dt
+    TIMED(c, KT_FINISH, (k_step_finish<<<blocks(std::max(1, c.blocks_dev.n_kin + d.n_rigid_nodes)), BLOCK, 0, s>>>(
+                            d, c.blocks_dev.kin_nodes, c.blocks_dev.n_kin)));
     c.time += d.dt;
     c.outputs_stale = true;
 }
@@ -1064,6 +1055,13 @@ void build(mel_ctx& c) {
     HIP_OK(hipMemcpyAsync(d.time, &m.time, sizeof(double), hipMemcpyHostToDevice, c.stream));
     c.d_rigid_ids = c.upload(c.h_rigid_ids);
     d.rigid_ids = c.d_rigid_ids;
+    {
+        std::vector<int> rn;
+        for (int b : c.h_rigid_ids)
+            for (int i = 0; i < m.bodies[b].n_nodes; i++) rn.push_back(node_off[b] + i);
+        d.n_rigid_nodes = (int)rn.size();
+        d.rigid_node_ids = c.upload(rn);
+    }
     HIP_OK(hipStreamSynchronize(c.stream));
 
     c.time = m.time;
@@ -1071,7 +1069,7 @@ void build(mel_ctx& c) {
     c.next_contact_update = m.next_contact_update;
 
     // rigid node positions, Update_current_positions at load (Code/IO.h:2050)
-    if (d.n_rigid > 0) k_rigid_nodes<<<blocks(N), BLOCK, 0, c.stream>>>(d, 0);
+    if (d.n_rigid_nodes > 0) k_rigid_nodes<<<blocks(d.n_rigid_nodes), BLOCK, 0, c.stream>>>(d, 0);
     // ---- initialisation block, Code/main.cpp:103-126 ----
     k_border_length<<<blocks(NS), BLOCK, 0, c.stream>>>(d);          // Update_borders
     k_bc_eval<<<blocks(NBR), BLOCK, 0, c.stream>>>(d, 0);            // Update_bc(Time)

@@ -51,6 +51,8 @@ struct DevModel {
     double *body_detect, *body_contact, *body_mass, *body_inv_mass, *body_rigid;   // body_rigid: RIGID_FIELDS each
     double *body_inertia, *body_inv_inertia;
     int *rigid_ids;            // body ids of the rigid bodies
+    int n_rigid_nodes;
+    int *rigid_node_ids;       // global ids of the nodes of rigid bodies
 
     // materials / laws
     int *mat_type;             // per material
@@ -351,6 +353,22 @@ __device__ inline void project_element(const DevModel& m, const ContactArrays& C
     C.nM0[e] = nd[0]; C.nM1[e] = nd[1]; C.nM2[e] = nd[2]; C.nM3[e] = nd[3];
     C.sM0[e] = sh[0]; C.sM1[e] = sh[1]; C.sM2[e] = sh[2]; C.sM3[e] = sh[3];
     C.N0[e] = p.N0; C.N1[e] = p.N1; C.N2[e] = p.N2; C.N3[e] = p.N3;
+}
+
+// DOF_TO_DISP gather of one node (Body::Update_kinematics, Code/Body.h:2176-2212)
+// and its current position (Node::Update_current_positions, Code/Node.h:233-237)
+__device__ inline void kinematics_node(const DevModel& m, int i) {
+    double dx = 0., dy = 0., vx = 0., vy = 0., ax = 0., ay = 0.;
+    for (int k = m.d_ptr[i]; k < m.d_ptr[i + 1]; k++) {
+        int j = m.d_idx[k];
+        double f = m.d_phi[k];
+        dx += f * m.upx[j]; dy += f * m.upy[j];
+        vx += f * m.vpx[j]; vy += f * m.vpy[j];
+        ax += f * m.apx[j]; ay += f * m.apy[j];
+    }
+    m.ux[i] = dx; m.uy[i] = dy; m.vx[i] = vx; m.vy[i] = vy; m.ax[i] = ax; m.ay[i] = ay;
+    m.xc[i] = m.X[i] + dx;
+    m.yc[i] = m.Y[i] + dy;
 }
 
 }  // namespace mel
