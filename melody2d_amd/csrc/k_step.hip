@@ -246,9 +246,7 @@ __device__ void bc_value(const DevModel& m, int type, int h0, int h1, double t, 
     }
 }
 
-__device__ inline void bc_eval_border(const DevModel& m, int br, int plus_dt) {
-    double t = *m.time;
-    if (plus_dt) t = t + m.dt;
+__device__ inline void bc_eval_at(const DevModel& m, int br, double t) {
     const int* hp = m.br_hist_ptr + 2 * br;
     bc_value(m, m.br_xtype[br], hp[0], hp[1], t, m.br_xval[br], m.br_xvel[br]);
     bc_value(m, m.br_ytype[br], hp[1], hp[2], t, m.br_yval[br], m.br_yvel[br]);
@@ -257,7 +255,9 @@ __device__ inline void bc_eval_border(const DevModel& m, int br, int plus_dt) {
 __global__ void k_bc_eval(DevModel m, int plus_dt) {
     int br = blockIdx.x * blockDim.x + threadIdx.x;
     if (br >= m.n_borders) return;
-    bc_eval_border(m, br, plus_dt);
+    double t = *m.time;
+    if (plus_dt) t = t + m.dt;
+    bc_eval_at(m, br, t);
 }
 
 // Body::Update_bc_forces deformable branch (Code/Body.h:672-815): point
@@ -360,48 +360,44 @@ __device__ inline void rigid_forces_block(const DevModel& m, int rigid_index) {
         __syncthreads();
     }
     if (tid == 0) { R[R_FCON] = red[0][0]; R[R_FCON + 1] = red[0][1]; R[R_FCON + 2] = red[0][2]; }
-    // boundary conditions: every thread forms the terms of its border slots,
-    // thread 0 adds them in slot order (the reference's serial loop)
-    __shared__ double terms[256][6];
-    double fn[3] = {0., 0., 0.}, fd[3] = {0., 0., 0.};
-    double xc0 = R[R_CUR], yc0 = R[R_CUR + 1];
-    int gb0 = m.border_off[b];
+    // boundary conditions: every thread adds the terms of a contiguous run of
+    // slots, then the runs are added in slot order (a two-level version of
+    // the reference's serial loop)
+    __shared__ double part[256][6];
+    double acc[6] = {0., 0., 0., 0., 0., 0.};
+    const double xc0 = R[R_CUR], yc0 = R[R_CUR + 1];
+    const int gb0 = m.border_off[b];
     for (int k = 0; k < m.body_nborders[b]; k++) {
-        int br = gb0 + k;
-        int xt = m.br_xtype[br], yt = m.br_ytype[br];
+        const int br = gb0 + k;
+        const int xt = m.br_xtype[br], yt = m.br_ytype[br];
         if (xt == 0 && yt == 0) continue;
-        int ns = m.br_nslots[br];
-        for (int base = 0; base < ns; base += 256) {
-            int j = base + tid;
-            __syncthreads();
-            if (j < ns) {
-                int s = m.br_slot0[br] + j;
-                int node = m.sl_node[s];
-                double L = m.sl_len[s];
-                double fxn = 0., fyn = 0., fxd = 0., fyd = 0.;
-                if (xt == 3 || xt == 4) fxn = m.br_xval[br] * L;
-                else if (xt == 2) fxd = (m.br_xparam[2 * br] * (m.br_xval[br] - m.ux[node]) + m.br_xparam[2 * br + 1] * (m.br_xvel[br] - m.vx[node])) * L;
-                if (yt == 3 || yt == 4) fyn = m.br_yval[br] * L;
-                else if (yt == 2) fyd = (m.br_yparam[2 * br] * (m.br_yval[br] - m.uy[node]) + m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node])) * L;
-                double dx = m.xc[node] - xc0, dy = m.yc[node] - yc0;
-                terms[tid][0] = fxn; terms[tid][1] = fyn; terms[tid][2] = -fxn * dy + fyn * dx;
-                terms[tid][3] = fxd; terms[tid][4] = fyd; terms[tid][5] = -fxd * dy + fyd * dx;
-            }
-            __syncthreads();
-            // six serial sums, one per component, each in slot order
-            if (tid < 6) {
-                int n = min(256, ns - base);
-                double acc = tid < 3 ? fn[tid] : fd[tid - 3];
-#pragma unroll 16
-                for (int q = 0; q < n; q++) acc += terms[q][tid];
-                if (tid < 3) fn[tid] = acc;
-                else fd[tid - 3] = acc;
-            }
+        const int ns = m.br_nslots[br];
+        const int run = (ns + blockDim.x - 1) / blockDim.x;
+        double p6[6] = {0., 0., 0., 0., 0., 0.};
+        for (int j = tid * run; j < min(ns, (tid + 1) * run); j++) {
+            int s = m.br_slot0[br] + j;
+            int node = m.sl_node[s];
+            double L = m.sl_len[s];
+            double fxn = 0., fyn = 0., fxd = 0., fyd = 0.;
+            if (xt == 3 || xt == 4) fxn = m.br_xval[br] * L;
+            else if (xt == 2) fxd = (m.br_xparam[2 * br] * (m.br_xval[br] - m.ux[node]) + m.br_xparam[2 * br + 1] * (m.br_xvel[br] - m.vx[node])) * L;
+            if (yt == 3 || yt == 4) fyn = m.br_yval[br] * L;
+            else if (yt == 2) fyd = (m.br_yparam[2 * br] * (m.br_yval[br] - m.uy[node]) + m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node])) * L;
+            double dx = m.xc[node] - xc0, dy = m.yc[node] - yc0;
+            p6[0] += fxn; p6[1] += fyn; p6[2] += -fxn * dy + fyn * dx;
+            p6[3] += fxd; p6[4] += fyd; p6[5] += -fxd * dy + fyd * dx;
+        }
+        __syncthreads();
+        for (int q = 0; q < 6; q++) part[tid][q] = p6[q];
+        __syncthreads();
+        if (tid < 6) {
+            double a = acc[tid];
+            for (int r = 0; r < (int)blockDim.x; r++) a += part[r][tid];
+            acc[tid] = a;
         }
     }
     __shared__ double fsum[6];
-    if (tid < 3) fsum[tid] = fn[tid];
-    else if (tid < 6) fsum[tid] = fd[tid - 3];
+    if (tid < 6) fsum[tid] = acc[tid];
     __syncthreads();
     if (tid != 0) return;
     for (int q = 0; q < 3; q++) { R[R_FNEU + q] = fsum[q]; R[R_FDIR + q] = fsum[3 + q]; }
@@ -530,11 +526,10 @@ __global__ void k_rigid_nodes(DevModel m, int with_forces) {
 //   k_step_contacts   Update_contacts + Update_contact_forces (per element,
 //                     point-force records appended to per-border-node
 //                     buckets) | Update_bc_forces deformable (per border node)
-//   k_step_gather     bucket sums (per border node) | rigid resultants (one
-//                     workgroup per rigid body)
-//   k_step_rigid      Update_bc(t + dt) (per border) | rigid Newton + Euler
+//   k_step_gather     bucket sums (per border node) | rigid resultants and
+//                     rigid Newton + Euler (one workgroup per rigid body)
 //   k_step_finish     kinematics of border nodes | rigid node kinematics |
-//                     time += dt
+//                     Update_bc(t + dt) (per border) | time = t + dt
 // ---------------------------------------------------------------------------
 __global__ void k_step_contacts(DevModel m, int nc) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -549,32 +544,29 @@ __global__ void k_step_contacts(DevModel m, int nc) {
 
 __global__ void k_step_gather(DevModel m) {
     if ((int)blockIdx.x < m.n_rigid) {
+        // resultants, then Newton + Euler of the same body by thread 0
         rigid_forces_block(m, blockIdx.x);
+        if (threadIdx.x == 0) rigid_step_body(m, m.rigid_ids[blockIdx.x]);
         return;
     }
     int bn = (blockIdx.x - m.n_rigid) * blockDim.x + threadIdx.x;
     if (bn < m.n_bnodes) bucket_sum_node(m, bn);
 }
 
-__global__ void k_step_rigid(DevModel m) {
+__global__ void k_step_finish(DevModel m, const int* kin_nodes, int n_kin, double t_next) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m.n_borders) {
-        bc_eval_border(m, i, 1);
-        return;
-    }
-    i -= m.n_borders;
-    if (i < m.n_rigid) rigid_step_body(m, m.rigid_ids[i]);
-}
-
-__global__ void k_step_finish(DevModel m, const int* kin_nodes, int n_kin) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *m.time = *m.time + m.dt;     // nothing in this launch reads the time
+    if (i == 0) *m.time = t_next;
     if (i < n_kin) {
         kinematics_node(m, kin_nodes[i]);
         return;
     }
     i -= n_kin;
-    if (i < m.n_rigid_nodes) rigid_node(m, m.rigid_node_ids[i], 1);
+    if (i < m.n_rigid_nodes) {
+        rigid_node(m, m.rigid_node_ids[i], 1);
+        return;
+    }
+    i -= m.n_rigid_nodes;
+    if (i < m.n_borders) bc_eval_at(m, i, t_next);
 }
 
 }  // namespace mel

@@ -14,8 +14,7 @@ __global__ void k_damping(DevModel m);
 __global__ void k_rigid_nodes(DevModel m, int with_forces);
 __global__ void k_step_contacts(DevModel m, int nc);
 __global__ void k_step_gather(DevModel m);
-__global__ void k_step_rigid(DevModel m);
-__global__ void k_step_finish(DevModel m, const int* kin_nodes, int n_kin);
+__global__ void k_step_finish(DevModel m, const int* kin_nodes, int n_kin, double t_next);
 
 // proximity (k_proximity.hip)
 __global__ void k_prox_chunk_boxes(DevModel m);

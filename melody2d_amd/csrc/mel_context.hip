@@ -53,11 +53,10 @@ inline int blocks(long long n, int b = BLOCK) { return (int)std::max<long long>(
 
 // kernels timed individually by the optional event instrumentation
 enum KernelId {
-    KT_CONTACTS = 0, KT_GATHER, KT_DAMPING, KT_BLOCK_UPDATE, KT_RIGID_STEP, KT_FINISH, KT_PROXIMITY, KT_COUNT
+    KT_CONTACTS = 0, KT_GATHER, KT_DAMPING, KT_BLOCK_UPDATE, KT_FINISH, KT_PROXIMITY, KT_COUNT
 };
 static const char* kKernelNames[KT_COUNT] = {
-    "contacts_bc", "gather_rigid", "damping", "block_force_update", "bc_rigid_step", "kinematics_finish",
-    "proximity_update"};
+    "contacts_bc", "gather_rigid", "damping", "block_force_update", "kinematics_bc_finish", "proximity_update"};
 
 struct mel_ctx {
     Model model;
@@ -631,14 +630,14 @@ void euler_step(mel_ctx& c) {
                    : k_block_force_update<<<c.blocks_dev.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, c.blocks_dev)));
     if (c.blocks_dev.n_split > 0)
         k_commit_split<<<blocks(c.blocks_dev.n_split), BLOCK, 0, s>>>(d, c.blocks_dev);
-    // Update_bc(t + dt) | rigid bodies
-    if (d.n_borders + d.n_rigid > 0)
-        TIMED(c, KT_RIGID_STEP, (k_step_rigid<<<blocks(d.n_borders + d.n_rigid), BLOCK, 0, s>>>(d)));
     // Update_kinematics, Update_current_positions (border nodes; interior
-    // ones are rebuilt on request) | rigid nodes | time += dt
-    TIMED(c, KT_FINISH, (k_step_finish<<<blocks(std::max(1, c.blocks_dev.n_kin + d.n_rigid_nodes)), BLOCK, 0, s>>>(
-                            d, c.blocks_dev.kin_nodes, c.blocks_dev.n_kin)));
-    c.time += d.dt;
+    // ones are rebuilt on request) | rigid nodes | Update_bc(t + dt) |
+    // time = t + dt (the host's copy of the time is the one the device takes)
+    const double t_next = c.time + d.dt;
+    TIMED(c, KT_FINISH,
+          (k_step_finish<<<blocks(std::max(1, c.blocks_dev.n_kin + d.n_rigid_nodes + d.n_borders)), BLOCK, 0, s>>>(
+              d, c.blocks_dev.kin_nodes, c.blocks_dev.n_kin, t_next)));
+    c.time = t_next;
     c.outputs_stale = true;
 }
 
