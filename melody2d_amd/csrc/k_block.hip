@@ -316,12 +316,10 @@ __device__ __forceinline__ void block_force_update(DevModel& m, const BlockArray
     __shared__ __align__(16) slot_t s_cxy[CHUNK_ENTRIES + 1];  // contributions by slot (+ sink)
     __shared__ int4 s_meta[MAX_BLOCK_CHUNKS];
     __shared__ int2 s_po[MAX_BLOCK_CHUNKS];
-    __shared__ double red[BLOCK_THREADS / 64][5];
     const int blk = blockIdx.x;
     const int t = threadIdx.x;
     const int body = B.blk_body[blk];
     if (m.body_status[body] == 1) return;   // inactive body: no update (uniform over the block)
-    const bool split = B.blk_split[blk] != 0;
     const bool single = B.blk_single_region[blk] != 0;
     const int nown = B.blk_nown[blk];
     const int ntouch = B.blk_ntouch[blk];
@@ -414,74 +412,15 @@ __device__ __forceinline__ void block_force_update(DevModel& m, const BlockArray
         }
     }
 
-    // Sum_up_forces + Apply_Newton + Apply_Euler for the owned nodes
-    double w[5] = {0., 0., 0., 0., 0.};   // internal body dirichlet neumann damping
-    double gx = m.gravity_x, gy = m.gravity_y;
-    if (m.body_status[body] == 2) { gx = 0.; gy = 0.; }
-    const double dt = m.dt;
+    // internal force of the owned nodes (Node::Sum_up_forces region sum)
 #pragma unroll
     for (int q = 0; q < NODES_PER_THREAD; q++) {
         int o = t + q * BLOCK_THREADS;
         if (o >= nown) break;
         int j = own[o];
         if (cur[q] >= 0) { tx[q] += rx[q]; ty[q] += ry[q]; }
-        double ix = tx[q], iy = ty[q];
-        double cx = 0., cy = 0., dx = 0., dy = 0., nx = 0., ny = 0.;
-        for (int e = m.bt_ptr[j]; e < m.bt_ptr[j + 1]; e++) {
-            int bn = m.bt_bn[e];
-            double f = m.bt_phi[e];
-            cx += m.bn_pcx[bn] * f; cy += m.bn_pcy[bn] * f;
-            dx += m.bn_pdx[bn] * f; dy += m.bn_pdy[bn] * f;
-            nx += m.bn_pnx[bn] * f; ny += m.bn_pny[bn] * f;
-        }
-        double bx = m.mx[j] * gx, by = m.my[j] * gy;
-        double kx = m.fkx[j], ky = m.fky[j];
-        double fx = ix + cx + 0. + bx + dx + nx + kx + 0.;
-        double fy = iy + cy + 0. + by + dy + ny + ky + 0.;
-        double apx = fx * m.imx[j], apy = fy * m.imy[j];
-        double vpx = m.vpx[j], vpy = m.vpy[j];
-        double hx = vpx + 0.5 * dt * apx, hy = vpy + 0.5 * dt * apy;
-        w[0] += dt * (hx * ix + hy * iy);
-        w[1] += dt * (hx * bx + hy * by);
-        w[2] += dt * (hx * dx + hy * dy);
-        w[3] += dt * (hx * nx + hy * ny);
-        w[4] += dt * (hx * kx + hy * ky);
-        vpx = vpx + dt * apx;
-        vpy = vpy + dt * apy;
-        double nux = m.upx[j] + dt * vpx, nuy = m.upy[j] + dt * vpy;
-        if (split) {
-            size_t pos = (size_t)B.blk_own_off[blk] + o;
-            B.stage_u[2 * pos] = nux;
-            B.stage_u[2 * pos + 1] = nuy;
-        } else {
-            m.upx[j] = nux;
-            m.upy[j] = nuy;
-        }
-        m.vpx[j] = vpx; m.vpy[j] = vpy;
-        m.apx[j] = apx; m.apy[j] = apy;
-        m.fix[j] = ix; m.fiy[j] = iy;
-    }
-    // block work increments: fixed-order reduction (wavefront butterflies,
-    // then the waves in order), accumulated with compensation across steps
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-        double v = w[q];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if ((t & 63) == 0) red[t >> 6][q] = v;
-    }
-    __syncthreads();
-    if (t == 0) {
-        const int slot[5] = {W_INTERNAL, W_BODY, W_DIRICHLET, W_NEUMANN, W_DAMPING};
-        double* acc = B.blk_work + 14 * (size_t)blk;
-        for (int q = 0; q < 5; q++) {
-            double v = 0.;
-            for (int wv = 0; wv < BLOCK_THREADS / 64; wv++) v += red[wv][q];
-            // Neumaier accumulation: acc[2s] sum, acc[2s+1] compensation
-            int s = slot[q];
-            double sum = acc[2 * s], x = v, nt = sum + x;
-            acc[2 * s + 1] += fabs(sum) >= fabs(x) ? (sum - nt) + x : (x - nt) + sum;
-            acc[2 * s] = nt;
-        }
+        m.fix[j] = tx[q];
+        m.fiy[j] = ty[q];
     }
 }
 
@@ -493,18 +432,6 @@ k_block_force_update(DevModel m, BlockArrays B) {
 __global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2)))
 k_block_force_update32(DevModel m, BlockArrays B) {
     block_force_update<Fp32>(m, B);
-}
-
-// publish the displacement parameters of split bodies once every block of
-// the launch has read the old ones
-__global__ void k_commit_split(DevModel m, BlockArrays B) {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= B.n_split) return;
-    int pos = B.split_pos[q];
-    int j = B.own_nodes[pos];
-    if (m.body_status[m.node_body[j]] == 1) return;
-    m.upx[j] = B.stage_u[2 * (size_t)pos];
-    m.upy[j] = B.stage_u[2 * (size_t)pos + 1];
 }
 
 // DOF_TO_DISP gather (Body::Update_kinematics, Code/Body.h:2176-2212) and

@@ -16,9 +16,9 @@
 //           nodes and for padding
 // Per lane: support size gl_cnt, c0 = w * jac, region.
 // A body larger than one block is split; its blocks read displacement
-// parameters owned by sibling blocks, so they write their updated ones to
-// stage_u (per own-node position) and k_commit_split publishes them after
-// the launch.
+// parameters owned by sibling blocks, which is safe because the kernel only
+// computes internal forces: the parameters are updated by k_node_update, a
+// separate launch.
 #pragma once
 #include <cstdint>
 #include "mel_device.hpp"
@@ -50,17 +50,15 @@ struct BlockArrays {
     double2* be_fxy;           // FP64 form
     float2* be_fxy32;          // FP32-mixed form
     double* blk_work;          // 14 per block: (sum, compensation) for the 7 works
-    int* blk_split;            // 1 when the block's body spans several blocks
-    double* stage_u;           // 2 per own-node position (split blocks only)
-    int* split_pos;            // own-node positions of split blocks
-    int n_split;
-    int* kin_nodes;            // deformable border nodes: kinematics needed every step
-    int n_kin;
+    int *blk_bn_off, *blk_bn_n, *bn_list;   // border nodes finished by each (unsplit) block
+    int *split_bn, n_split_bn;              // border nodes of split bodies
+    int *split_kin, n_split_kin;            // their nodes (kinematics after the update)
+    int *bc_bn, n_bc_bn;                    // deformable border nodes on borders with a BC
 };
 
 __global__ void k_block_force_update(DevModel m, BlockArrays B);
 __global__ void k_block_force_update32(DevModel m, BlockArrays B);
-__global__ void k_commit_split(DevModel m, BlockArrays B);
+__global__ void k_update_finish(DevModel m, BlockArrays B, double t_next);
 __global__ void k_kinematics_list(DevModel m, const int* nodes, int n);
 __global__ void k_node_outputs(DevModel m);
 

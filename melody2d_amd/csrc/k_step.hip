@@ -6,6 +6,7 @@
 // fixed element order instead of the reference's body/send order.
 #include "mel_device.hpp"
 #include "k_step.hpp"
+#include "k_block.hpp"
 
 namespace mel {
 
@@ -268,19 +269,13 @@ __device__ inline void bc_point_force_node(const DevModel& m, int bn) {
     int node = m.bn_node[bn];
     int b = m.node_body[node];
     if (!m.body_rigid_flag[b] && m.body_status[b] != 1) {
-        // Body::Update_borders (Code/Body.h:614-661) for this node's slots:
-        // the lengths the step and the outputs use
-        for (int q = m.bn_slot_ptr[bn]; q < m.bn_slot_ptr[bn + 1]; q++) {
-            int s = m.bn_slots[q];
-            m.sl_len[s] = slot_length(m, s);
-        }
         for (int q = m.bn_slot_ptr[bn]; q < m.bn_slot_ptr[bn + 1]; q++) {
             int s = m.bn_slots[q];
             int br = m.sl_border[s];
             int xt = m.br_xtype[br], yt = m.br_ytype[br];
             if (xt == 0 && yt == 0) continue;
             if (m.sl_j[s] >= m.br_nslots[br] - 1) continue;
-            double L = m.sl_len[s];
+            double L = slot_length(m, s);
             double px = 0., py = 0.;
             double tx = 0, ty = 0, tpx = 0, tpy = 0;
             if (xt == 4 || yt == 4) {
@@ -525,13 +520,16 @@ __global__ void k_rigid_nodes(DevModel m, int with_forces) {
 // kind of work on its own index range:
 //   k_step_contacts   Update_contacts + Update_contact_forces (per element,
 //                     point-force records appended to per-border-node
-//                     buckets) | Update_bc_forces deformable (per border node)
-//   k_step_gather     bucket sums (per border node) | rigid resultants and
-//                     rigid Newton + Euler (one workgroup per rigid body)
-//   k_step_finish     kinematics of border nodes | rigid node kinematics |
-//                     Update_bc(t + dt) (per border) | time = t + dt
+//                     buckets) | Update_bc_forces deformable (per border node
+//                     on a border with a BC)
+//   k_step_gather     rigid resultants and rigid Newton + Euler (one
+//                     workgroup per rigid body) | bucket sums and lengths of
+//                     the border nodes of split bodies
+//   k_update_finish   per node block: bucket sums, Newton + Euler + works,
+//                     lengths, border-node kinematics | rigid nodes |
+//                     Update_bc(t + dt) | time = t + dt
 // ---------------------------------------------------------------------------
-__global__ void k_step_contacts(DevModel m, int nc) {
+__global__ void k_step_contacts(DevModel m, int nc, const int* bc_bn, int n_bc) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nc) {
         if (m.body_status[m.C.bodyS[i]] != 1) project_element(m, m.C, i);
@@ -539,28 +537,125 @@ __global__ void k_step_contacts(DevModel m, int nc) {
         return;
     }
     i -= nc;
-    if (i < m.n_bnodes) bc_point_force_node(m, i);
+    if (i < n_bc) bc_point_force_node(m, bc_bn[i]);
 }
 
-__global__ void k_step_gather(DevModel m) {
+// Body::Update_borders (Code/Body.h:614-661) for the slots of one border node
+__device__ inline void bn_lengths(const DevModel& m, int bn) {
+    for (int q = m.bn_slot_ptr[bn]; q < m.bn_slot_ptr[bn + 1]; q++) {
+        int s = m.bn_slots[q];
+        m.sl_len[s] = slot_length(m, s);
+    }
+}
+
+__global__ void k_step_gather(DevModel m, const int* split_bn, int n_split) {
     if ((int)blockIdx.x < m.n_rigid) {
         // resultants, then Newton + Euler of the same body by thread 0
         rigid_forces_block(m, blockIdx.x);
         if (threadIdx.x == 0) rigid_step_body(m, m.rigid_ids[blockIdx.x]);
         return;
     }
-    int bn = (blockIdx.x - m.n_rigid) * blockDim.x + threadIdx.x;
-    if (bn < m.n_bnodes) bucket_sum_node(m, bn);
+    // border nodes of bodies split over several node blocks: bucket sums and
+    // lengths here (the positions are those of the start of the step)
+    int q = (blockIdx.x - m.n_rigid) * blockDim.x + threadIdx.x;
+    if (q < n_split) {
+        int bn = split_bn[q];
+        bucket_sum_node(m, bn);
+        int b = m.node_body[m.bn_node[bn]];
+        if (!m.body_rigid_flag[b] && m.body_status[b] != 1) bn_lengths(m, bn);
+    }
 }
 
-__global__ void k_step_finish(DevModel m, const int* kin_nodes, int n_kin, double t_next) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *m.time = t_next;
-    if (i < n_kin) {
-        kinematics_node(m, kin_nodes[i]);
+// The rest of the step per node block (a whole body): send of the contact
+// point forces to the block's border nodes (bucket sums), Sum_up_forces +
+// Apply_Newton + Apply_Euler with works for the owned nodes
+// (Code/Body.h:1975-2068, Code/Node.h:212-245), Update_borders for the
+// block's border slots (positions of the start of the step) and
+// Update_kinematics / Update_current_positions for its border nodes.
+// Workgroups past the node blocks move the rigid nodes, evaluate the BCs at
+// t + dt and store the new time.
+__device__ inline void node_block_finish(const DevModel& m, const BlockArrays& B, int blk) {
+    __shared__ double red[BLOCK_THREADS / 64][5];
+    const int t = threadIdx.x;
+    const int body = B.blk_body[blk];
+    if (m.body_status[body] == 1) return;   // inactive body: no update (uniform over the block)
+    const int nown = B.blk_nown[blk];
+    const int* own = B.own_nodes + B.blk_own_off[blk];
+    const int* bns = B.bn_list + B.blk_bn_off[blk];
+    const int nbn = B.blk_bn_n[blk];        // 0 for blocks of split bodies
+    for (int i = t; i < nbn; i += BLOCK_THREADS) bucket_sum_node(m, bns[i]);
+    __syncthreads();
+    double w[5] = {0., 0., 0., 0., 0.};   // internal body dirichlet neumann damping
+    double gx = m.gravity_x, gy = m.gravity_y;
+    if (m.body_status[body] == 2) { gx = 0.; gy = 0.; }
+    const double dt = m.dt;
+#pragma unroll
+    for (int q = 0; q < NODES_PER_THREAD; q++) {
+        int o = t + q * BLOCK_THREADS;
+        if (o >= nown) break;
+        int j = own[o];
+        double ix = m.fix[j], iy = m.fiy[j];
+        double cx = 0., cy = 0., dx = 0., dy = 0., nx = 0., ny = 0.;
+        for (int e = m.bt_ptr[j]; e < m.bt_ptr[j + 1]; e++) {
+            int bn = m.bt_bn[e];
+            double f = m.bt_phi[e];
+            cx += m.bn_pcx[bn] * f; cy += m.bn_pcy[bn] * f;
+            dx += m.bn_pdx[bn] * f; dy += m.bn_pdy[bn] * f;
+            nx += m.bn_pnx[bn] * f; ny += m.bn_pny[bn] * f;
+        }
+        double bx = m.mx[j] * gx, by = m.my[j] * gy;
+        double kx = m.fkx[j], ky = m.fky[j];
+        double fx = ix + cx + 0. + bx + dx + nx + kx + 0.;
+        double fy = iy + cy + 0. + by + dy + ny + ky + 0.;
+        double apx = fx * m.imx[j], apy = fy * m.imy[j];
+        double vpx = m.vpx[j], vpy = m.vpy[j];
+        double hx = vpx + 0.5 * dt * apx, hy = vpy + 0.5 * dt * apy;
+        w[0] += dt * (hx * ix + hy * iy);
+        w[1] += dt * (hx * bx + hy * by);
+        w[2] += dt * (hx * dx + hy * dy);
+        w[3] += dt * (hx * nx + hy * ny);
+        w[4] += dt * (hx * kx + hy * ky);
+        vpx = vpx + dt * apx;
+        vpy = vpy + dt * apy;
+        m.upx[j] = m.upx[j] + dt * vpx;
+        m.upy[j] = m.upy[j] + dt * vpy;
+        m.vpx[j] = vpx; m.vpy[j] = vpy;
+        m.apx[j] = apx; m.apy[j] = apy;
+    }
+    // block work increments: fixed-order reduction (wavefront butterflies,
+    // then the waves in order), accumulated with compensation across steps
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        double v = w[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((t & 63) == 0) red[t >> 6][q] = v;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const int slot[5] = {W_INTERNAL, W_BODY, W_DIRICHLET, W_NEUMANN, W_DAMPING};
+        double* acc = B.blk_work + 14 * (size_t)blk;
+        for (int q = 0; q < 5; q++) {
+            double v = 0.;
+            for (int wv = 0; wv < BLOCK_THREADS / 64; wv++) v += red[wv][q];
+            // Neumaier accumulation: acc[2s] sum, acc[2s+1] compensation
+            int s = slot[q];
+            double sum = acc[2 * s], x = v, nt = sum + x;
+            acc[2 * s + 1] += fabs(sum) >= fabs(x) ? (sum - nt) + x : (x - nt) + sum;
+            acc[2 * s] = nt;
+        }
+    }
+    for (int i = t; i < nbn; i += BLOCK_THREADS) bn_lengths(m, bns[i]);
+    __syncthreads();       // the block's new parameters and old-position lengths are in place
+    for (int i = t; i < nbn; i += BLOCK_THREADS) kinematics_node(m, m.bn_node[bns[i]]);
+}
+
+__global__ void __launch_bounds__(BLOCK_THREADS) k_update_finish(DevModel m, BlockArrays B, double t_next) {
+    if ((int)blockIdx.x < B.n_blocks) {
+        node_block_finish(m, B, blockIdx.x);
         return;
     }
-    i -= n_kin;
+    int i = (blockIdx.x - B.n_blocks) * blockDim.x + threadIdx.x;
+    if (i == 0) *m.time = t_next;
     if (i < m.n_rigid_nodes) {
         rigid_node(m, m.rigid_node_ids[i], 1);
         return;

@@ -12,9 +12,8 @@ __global__ void k_border_length(DevModel m);
 __global__ void k_bc_eval(DevModel m, int plus_dt);
 __global__ void k_damping(DevModel m);
 __global__ void k_rigid_nodes(DevModel m, int with_forces);
-__global__ void k_step_contacts(DevModel m, int nc);
-__global__ void k_step_gather(DevModel m);
-__global__ void k_step_finish(DevModel m, const int* kin_nodes, int n_kin, double t_next);
+__global__ void k_step_contacts(DevModel m, int nc, const int* bc_bn, int n_bc);
+__global__ void k_step_gather(DevModel m, const int* split_bn, int n_split);
 
 // proximity (k_proximity.hip)
 __global__ void k_prox_chunk_boxes(DevModel m);

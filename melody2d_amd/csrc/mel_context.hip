@@ -38,12 +38,12 @@ struct CapacityError {
 
 inline int blocks(long long n, int b = BLOCK) { return (int)std::max<long long>(1, (n + b - 1) / b); }
 
-#define TIMED(ctx, id, launch)                     \
+#define TIMED(ctx, id, st, launch)                 \
     do {                                           \
         if ((ctx).timing && ((ctx).timing_mask >> (id) & 1u)) { \
-            int ev_ = (ctx).ev_begin(id);          \
+            int ev_ = (ctx).ev_begin(id, st);      \
             launch;                                \
-            (ctx).ev_end(ev_);                     \
+            (ctx).ev_end(ev_, st);                 \
         } else {                                   \
             launch;                                \
         }                                          \
@@ -56,7 +56,7 @@ enum KernelId {
     KT_CONTACTS = 0, KT_GATHER, KT_DAMPING, KT_BLOCK_UPDATE, KT_FINISH, KT_PROXIMITY, KT_COUNT
 };
 static const char* kKernelNames[KT_COUNT] = {
-    "contacts_bc", "gather_rigid", "damping", "block_force_update", "kinematics_bc_finish", "proximity_update"};
+    "contacts_bc", "gather_rigid", "damping", "block_force_update", "update_finish", "proximity_update"};
 
 struct mel_ctx {
     Model model;
@@ -68,17 +68,18 @@ struct mel_ctx {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
     double kt_ms[KT_COUNT] = {0};
     long long kt_n[KT_COUNT] = {0};
-    int ev_begin(int id) {
+    int ev_begin(int id, hipStream_t st) {
         std::pair<hipEvent_t, hipEvent_t> p;
         if (!pool.empty()) { p = pool.back(); pool.pop_back(); }
         else { HIP_OK(hipEventCreate(&p.first)); HIP_OK(hipEventCreate(&p.second)); }
         pending.push_back({id, p.first, p.second});
-        HIP_OK(hipEventRecord(p.first, stream));
+        HIP_OK(hipEventRecord(p.first, st));
         return (int)pending.size() - 1;
     }
-    void ev_end(int k) { HIP_OK(hipEventRecord(pending[k].b, stream)); }
+    void ev_end(int k, hipStream_t st) { HIP_OK(hipEventRecord(pending[k].b, st)); }
     void harvest() {
         if (pending.empty()) return;
+        HIP_OK(hipStreamSynchronize(aux));
         HIP_OK(hipStreamSynchronize(stream));
         for (auto& e : pending) {
             float ms = 0.f;
@@ -91,6 +92,8 @@ struct mel_ctx {
     }
     DevModel d{};
     hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;            // contact pipeline, concurrent with internal forces
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int device = 0;
     int precision = MEL_FP64;
     std::vector<void*> allocs;
@@ -139,10 +142,14 @@ struct mel_ctx {
         HIP_OK(hipStreamSynchronize(stream));
     }
     ~mel_ctx() {
+        if (aux) (void)hipStreamSynchronize(aux);
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto& e : pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
         for (auto& e : pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
         for (void* p : allocs) (void)hipFree(p);
+        if (aux) (void)hipStreamDestroy(aux);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -297,7 +304,7 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     std::vector<int> bgp_region;
     std::vector<int> tptr, gl_cnt;
     std::vector<uint32_t> be_pk;
-    std::vector<int> local(N, -1), blk_split, split_pos;
+    std::vector<int> local(N, -1);
     int max_touch = 1, max_own = 1;
     // MELODY2D_BLOCK_NODES lowers the nodes per block (exercises the split
     // path on small models); default and maximum BLOCK_MAX_OWN
@@ -350,9 +357,6 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
         for (auto& owned : groups) {
             int blk = (int)blk_body.size();
             blk_body.push_back((int)b);
-            blk_split.push_back(groups.size() > 1 ? 1 : 0);
-            if (groups.size() > 1)
-                for (size_t i = 0; i < owned.size(); i++) split_pos.push_back((int)(own_nodes.size() + i));
             blk_own_off.push_back((int)own_nodes.size());
             blk_nown.push_back((int)owned.size());
             max_own = std::max(max_own, (int)owned.size());
@@ -505,9 +509,31 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
             (void)blk;
         }
     }
-    std::vector<int> kin, kin_all;
-    for (int nd : bn_node)
-        if (!m.bodies[c.d.node_body_host[nd]].rigid) kin.push_back(nd);
+    // border nodes finished by each block (its own, when the block holds a
+    // whole body) and those of split bodies (finished by separate launches)
+    std::vector<int> node_bn(N, -1), kin_all, blk_bn_off, blk_bn_n, bn_list, split_bn, split_kin;
+    for (size_t q = 0; q < bn_node.size(); q++) node_bn[bn_node[q]] = (int)q;
+    {
+        std::vector<int> nblk_of_body(m.bodies.size(), 0);
+        for (int b : blk_body) nblk_of_body[b]++;
+        for (size_t k = 0; k < blk_body.size(); k++) {
+            blk_bn_off.push_back((int)bn_list.size());
+            int n0 = (int)bn_list.size();
+            if (nblk_of_body[blk_body[k]] == 1)
+                for (int o = 0; o < blk_nown[k]; o++) {
+                    int bn = node_bn[own_nodes[blk_own_off[k] + o]];
+                    if (bn >= 0) bn_list.push_back(bn);
+                }
+            blk_bn_n.push_back((int)bn_list.size() - n0);
+        }
+        for (size_t q = 0; q < bn_node.size(); q++) {
+            int b = c.d.node_body_host[bn_node[q]];
+            if (!m.bodies[b].rigid && nblk_of_body[b] > 1) {
+                split_bn.push_back((int)q);
+                split_kin.push_back(bn_node[q]);
+            }
+        }
+    }
     for (size_t b = 0; b < m.bodies.size(); b++)
         if (!m.bodies[b].rigid)
             for (int i = 0; i < m.bodies[b].n_nodes; i++) kin_all.push_back(node_off[b] + i);
@@ -538,12 +564,9 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
         B.be_fxy32 = nullptr;
     }
     B.blk_work = c.upload(blk_work);
-    B.blk_split = c.upload(blk_split);
-    B.split_pos = c.upload(split_pos);
-    B.n_split = (int)split_pos.size();
-    B.stage_u = B.n_split ? c.alloc<double>(2 * own_nodes.size()) : nullptr;
-    B.kin_nodes = c.upload(kin);
-    B.n_kin = (int)kin.size();
+    B.blk_bn_off = c.upload(blk_bn_off); B.blk_bn_n = c.upload(blk_bn_n); B.bn_list = c.upload(bn_list);
+    B.split_bn = c.upload(split_bn); B.n_split_bn = (int)split_bn.size();
+    B.split_kin = c.upload(split_kin); B.n_split_kin = (int)split_kin.size();
     c.d_kin_all = c.upload(kin_all);
     c.n_kin_all = (int)kin_all.size();
     c.block_lds = block_lds_size(B, c.precision == MEL_FP32_MIXED);
@@ -611,32 +634,40 @@ void initialize_czm(mel_ctx& c) {
 // Euler_step (Code/Solver.h:10-88)
 void euler_step(mel_ctx& c) {
     DevModel& d = c.d;
-    hipStream_t s = c.stream;
+    const BlockArrays& B = c.blocks_dev;
+    hipStream_t s = c.stream, a = c.aux;
     int nc = c.n_contacts;
-    // Initialize_contact_forces, Update_borders, Update_contacts,
-    // Update_contact_forces (records into buckets) | Update_bc_forces
-    if (nc + d.n_bnodes > 0)
-        TIMED(c, KT_CONTACTS, (k_step_contacts<<<blocks(nc + d.n_bnodes), BLOCK, 0, s>>>(d, nc)));
+    // the contact pipeline runs on the second stream, concurrently with the
+    // internal forces, which do not depend on it; its launches are a few
+    // workgroups, which fit beside the internal-force blocks
+    HIP_OK(hipEventRecord(c.ev_fork, s));
+    HIP_OK(hipStreamWaitEvent(a, c.ev_fork, 0));
+    // Initialize_contact_forces, Update_contacts, Update_contact_forces
+    // (records into buckets) | Update_bc_forces
+    if (nc + B.n_bc_bn > 0)
+        TIMED(c, KT_CONTACTS, a, (k_step_contacts<<<blocks(nc + B.n_bc_bn), BLOCK, 0, a>>>(d, nc, B.bc_bn, B.n_bc_bn)));
     // Update_damping_forces
-    if (d.n_matrix > 0) TIMED(c, KT_DAMPING, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d)));
-    // Send_contact_forces (bucket sums) | rigid resultants
-    if (d.n_rigid + d.n_bnodes > 0)
-        TIMED(c, KT_GATHER, (k_step_gather<<<d.n_rigid + blocks(d.n_bnodes), BLOCK, 0, s>>>(d)));
-    // Update_internal_forces, Sum_up_forces, Apply_Newton, Apply_Euler
-    if (c.blocks_dev.n_blocks > 0)
-        TIMED(c, KT_BLOCK_UPDATE,
+    if (d.n_matrix > 0) TIMED(c, KT_DAMPING, a, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, a>>>(d)));
+    // rigid resultants, Newton, Euler | sends to the border nodes of split bodies
+    if (d.n_rigid + B.n_split_bn > 0)
+        TIMED(c, KT_GATHER, a, (k_step_gather<<<d.n_rigid + blocks(B.n_split_bn), BLOCK, 0, a>>>(d, B.split_bn, B.n_split_bn)));
+    HIP_OK(hipEventRecord(c.ev_join, a));
+    // Update_internal_forces
+    if (B.n_blocks > 0)
+        TIMED(c, KT_BLOCK_UPDATE, s,
               (c.precision == MEL_FP32_MIXED
-                   ? k_block_force_update32<<<c.blocks_dev.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, c.blocks_dev)
-                   : k_block_force_update<<<c.blocks_dev.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, c.blocks_dev)));
-    if (c.blocks_dev.n_split > 0)
-        k_commit_split<<<blocks(c.blocks_dev.n_split), BLOCK, 0, s>>>(d, c.blocks_dev);
-    // Update_kinematics, Update_current_positions (border nodes; interior
-    // ones are rebuilt on request) | rigid nodes | Update_bc(t + dt) |
-    // time = t + dt (the host's copy of the time is the one the device takes)
+                   ? k_block_force_update32<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B)
+                   : k_block_force_update<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B)));
+    HIP_OK(hipStreamWaitEvent(s, c.ev_join, 0));
+    // sends, Sum_up_forces, Apply_Newton, Apply_Euler, Update_borders,
+    // Update_kinematics of border nodes (interior ones are rebuilt on request)
+    // | rigid nodes | Update_bc(t + dt) | time = t + dt (the host's copy of the
+    // time is the one the device takes)
     const double t_next = c.time + d.dt;
-    TIMED(c, KT_FINISH,
-          (k_step_finish<<<blocks(std::max(1, c.blocks_dev.n_kin + d.n_rigid_nodes + d.n_borders)), BLOCK, 0, s>>>(
-              d, c.blocks_dev.kin_nodes, c.blocks_dev.n_kin, t_next)));
+    const int extra = std::max(1, (d.n_rigid_nodes + d.n_borders + BLOCK_THREADS - 1) / BLOCK_THREADS);
+    TIMED(c, KT_FINISH, s, (k_update_finish<<<B.n_blocks + extra, BLOCK_THREADS, 0, s>>>(d, B, t_next)));
+    if (B.n_split_kin > 0)
+        k_kinematics_list<<<blocks(B.n_split_kin), BLOCK, 0, s>>>(d, B.split_kin, B.n_split_kin);
     c.time = t_next;
     c.outputs_stale = true;
 }
@@ -968,6 +999,21 @@ void build(mel_ctx& c) {
     for (double** p : forces) *p = c.alloc<double>(N);
     c.d.node_body_host = node_body.data();
     build_blocks(c, g_ptr, g_idx, g_phix, g_phiy, g_c0, t_ptr, t_gp, bn_node, node_off, gp_off, perreg);
+    {
+        // deformable border nodes with a slot on a border that carries a BC
+        std::vector<int> bc_bn;
+        for (int q = 0; q < NBN; q++) {
+            if (m.bodies[node_body[bn_node[q]]].rigid) continue;
+            bool any = false;
+            for (int k = bn_slot_ptr[q]; k < bn_slot_ptr[q + 1]; k++) {
+                int br = sl_border[bn_slots[k]];
+                any |= br_xt[br] != 0 || br_yt[br] != 0;
+            }
+            if (any) bc_bn.push_back(q);
+        }
+        c.blocks_dev.bc_bn = c.upload(bc_bn);
+        c.blocks_dev.n_bc_bn = (int)bc_bn.size();
+    }
     c.d.node_body_host = nullptr;
     d.d_ptr = c.upload(d_ptr);
     d.d_idx = c.upload(d_idx);
@@ -1123,6 +1169,9 @@ mel_status create(mel_ctx** out, int device, int precision, const std::function<
     try {
         HIP_OK(hipSetDevice(device));
         HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
         build(*c);
     } catch (HipError& e) {
         g_load_err = e.msg;
@@ -1192,9 +1241,9 @@ mel_status mel_euler_steps(mel_ctx* c, int nsteps) {
             if (c->time > c->next_contact_update - 1.e-6 * c->d.dt) {
                 c->next_contact_update = c->next_contact_update + c->model.contact_update_period;
                 if (c->timing && (c->timing_mask >> KT_PROXIMITY & 1u)) {
-                    int ev = c->ev_begin(KT_PROXIMITY);
+                    int ev = c->ev_begin(KT_PROXIMITY, c->stream);
                     update_proximity(*c);
-                    c->ev_end(ev);
+                    c->ev_end(ev, c->stream);
                 } else {
                     update_proximity(*c);
                 }
@@ -1207,6 +1256,7 @@ mel_status mel_euler_steps(mel_ctx* c, int nsteps) {
 
 mel_status mel_synchronize(mel_ctx* c) {
     return guarded(c, [&]() {
+        HIP_OK(hipStreamSynchronize(c->aux));
         HIP_OK(hipStreamSynchronize(c->stream));
         check_flags(*c);
         return MEL_OK;

@@ -64,9 +64,9 @@ def test_internal_forces_bitwise_on_contact_free_start(golden_dir, tmp_path, mon
 
 @pytest.mark.parametrize("name", ["tiny_bc", "tiny_platen"])
 def test_split_node_blocks_match_oracle(name, golden_dir, tmp_path, monkeypatch):
-    """Bodies cut into several node blocks (64 nodes each here) exchange
-    displacement parameters through the staged commit: results are the same
-    as with one block per body."""
+    """Bodies cut into several node blocks (64 nodes each here) read
+    displacement parameters owned by sibling blocks: results are the same as
+    with one block per body."""
     monkeypatch.setenv("MELODY2D_BLOCK_NODES", "64")
     model, _ = golden_dir(name)
     print("case split", name, flush=True)
