@@ -32,17 +32,31 @@ def log(*a):
 
 def algorithmic_bytes(kernel, info, fp32=False):
     """Bytes a launch must move at minimum, each array counted once.
-    N = nodes, G = Gauss points, nnz = Gauss-point support entries."""
+    N = nodes, G = Gauss points, nnz = Gauss-point support entries,
+    NB = border nodes, nc = contact elements."""
     N, G, nnz = info.n_nodes, info.n_gauss, info.nnz_gauss
+    NB, nc = info.n_border_nodes, info.n_contacts
+    row = info.nnz_d2d / max(N, 1)
+    # internal forces: per support entry dphi/dx, dphi/dy (16 B in FP64, 8 B
+    # in the FP32-mixed form) + node index and destination slot (4); per
+    # Gauss point c0 (8) and region (4); per node the displacement
+    # parameters read (16), the internal force written (16), node ids (8)
+    forces = (12 if fp32 else 20) * nnz + 12 * G + 40 * N
+    # finish: per node internal force read (16), masses and inverse masses
+    # (32), damping force (16), velocity and displacement parameters read (32),
+    # spreading offset (4), new displacement / velocity / acceleration
+    # parameters written (48); per border node its bucket (count, four
+    # records: 84), six point forces read (48) and two written (16), its
+    # kinematics (X, Y read, 8 doubles written: 80) and DOF_TO_DISP row
+    # (12 B per entry)
+    finish = 148 * N + NB * (84 + 48 + 16 + 80 + 12 * row)
+    contacts = 256 * nc     # element state read and written, point-force records
     if kernel == "block_force_update":
-        # per support entry: dphi/dx, dphi/dy (16 B in FP64, 8 B in the
-        # FP32-mixed form) + node index and destination slot (4); per Gauss
-        # point: c0 (8) and region (4); per node: displacement parameters
-        # read + write (32), velocity parameters read + write (32),
-        # acceleration parameters and internal force written (32), masses,
-        # inverse masses and damping force read (48), node id and slot
-        # offset (6)
-        return (12 if fp32 else 20) * nnz + 12 * G + 150 * N
+        return int(forces)
+    if kernel == "step_forces":
+        return int(forces + contacts)
+    if kernel == "step_fused":
+        return int(forces + contacts + finish - 16 * N)   # internal force stays in registers
     return None
 
 
@@ -130,8 +144,9 @@ def main():
             name, time.time() - t0, info.n_nodes, info.n_gauss, info.nnz_gauss, info.n_contacts))
         ctx.euler_steps(args.warmup)
         ctx.synchronize()
-        dominant = "block_force_update"
-        ctx.set_kernel_timing([dominant])
+        # the step's dominant launch: one-launch step, internal forces of the
+        # two-launch step, or the internal-force kernel of the launch sequence
+        ctx.set_kernel_timing(["step_fused", "step_forces", "block_force_update"])
         ctx.synchronize()
         t0 = time.perf_counter()
         ctx.euler_steps(args.steps)
@@ -145,6 +160,7 @@ def main():
         log("[%s] timed %d steps in %.3fs: %.3e node-steps/s, %.3f ms/step" % (
             name, args.steps, sec, value, 1e3 * sec / args.steps))
         roof = None
+        dominant = next(k for k in ("step_fused", "step_forces", "block_force_update") if kt[k][1])
         ms, n = kt[dominant]
         if n:
             avg_s = ms / n * 1e-3

@@ -54,6 +54,9 @@ struct BlockArrays {
     int *split_bn, n_split_bn;              // border nodes of split bodies
     int *split_kin, n_split_kin;            // their nodes (kinematics after the update)
     int *bc_bn, n_bc_bn;                    // deformable border nodes on borders with a BC
+    uint16_t* bt_lbn;          // per point-force spreading entry: its border node's index in the
+                               // block's list (one-launch step only)
+    int lds_finish_ok;         // every block meets the one-launch step's finish preconditions
 };
 
 __global__ void k_block_force_update(DevModel m, BlockArrays B);

@@ -1,0 +1,436 @@
+// Internal forces of one node block (device code of k_block.hip and the
+// fused step).
+//
+// Replaces, per block of owned nodes (a whole grain when it fits):
+//   Body::Update_internal_forces  Code/Body.h:937-1049 (+ Code/Model.h:19-73)
+//   Node::Sum_up_forces           Code/Node.h:212-225
+//   Node::Apply_Newton            Code/Node.h:227-231
+//   Body::Apply_Euler (works)     Code/Body.h:2051-2068, Node::Apply_Euler Code/Node.h:239-245
+//
+// Data path per block: the displacement parameters of the touched nodes are
+// staged in LDS once.  Chunk by chunk, every Gauss point is a lane that holds
+// its own support entries (dphi/dx, dphi/dy, node index, destination slot)
+// in registers, loaded with coalesced 16-byte loads from the wave's
+// entry-major stream while the previous chunk is summed.  The lane gathers
+// all its displacements from LDS at once, forms F and the stress terms
+// (Code/Model.h:19-73) summing in the reference's entry order, and writes
+// each entry's contribution c0 * (dphi . P) to the entry's slot in
+// owned-node order.  Then each thread sums, for its owned nodes, their
+// contiguous slots - a node's entries in Gauss-point order - per region when
+// the reference order is requested.  A block is 128 threads and needs < 40 KB
+// of LDS, so four blocks share a CU and a 1000-grain model is resident at
+// once.  HBM traffic per entry: 16 (dphi/dx, dphi/dy) + 4 (index and slot),
+// times the padding of each wave to its largest support.
+#pragma once
+#include "mel_device.hpp"
+#include "k_block.hpp"
+
+namespace mel {
+
+namespace blockdev {
+
+static_assert(BLOCK_THREADS == 128, "chunk descriptors describe two waves of lanes");
+// native vector types (LLVM vectors) keep in-flight loads in registers
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// stress terms of Code/Model.h:19-73 for one Gauss point from its gradient
+// sums (Code/Body.h:975-1014): linear material (a, b, c) = (E11, 2 E12, E22),
+// neo-Hookean (a, b, c, d) = (F11, F12, F21, F22).  Written so that a support
+// entry contributes c0 * (phix * P.x + phiy * P.y), c0 * (phix * P.z + phiy * P.w).
+template <int MAT>
+__device__ inline double4 stress_terms(double Mu, double K2, double a, double b, double c, double d) {
+    if (MAT == 0) {
+        double E11 = a, E12 = 0.5 * b, E22 = c;
+        double J = 1. + E11 + E22;
+        double S33 = K2 * (J - 1.);
+        double M2 = 2 * Mu;
+        double S11 = S33 + M2 * E11, S22 = S33 + M2 * E22, S12 = M2 * E12;
+        return make_double4(S11, S12, S12, S22);
+    }
+    double F11 = a, F12 = b, F21 = c, F22 = d;
+    double F11F12 = F11 * F11 + F12 * F12;
+    double F21F22 = F21 * F21 + F22 * F22;
+    double J = F11 * F22 - F12 * F21;
+    double invJ = 1. / J;
+    double K = K2 * (J - 1.);
+    double MuJ53 = Mu * pow(J, -1.666666666667);
+    double trB3 = 0.333333333333 * (F11F12 + F21F22 + 1.);
+    double s11 = K + MuJ53 * (F11F12 - trB3);
+    double s22 = K + MuJ53 * (F21F22 - trB3);
+    double s12 = MuJ53 * (F11 * F21 + F12 * F22);
+    double S11 = (F12 * (F12 * s22 - F22 * s12) - F22 * (F12 * s12 - F22 * s11)) * invJ;
+    double S22 = (F11 * (F11 * s22 - F21 * s12) - F21 * (F11 * s12 - F21 * s11)) * invJ;
+    double S12 = (F21 * (F12 * s12 - F22 * s11) - F11 * (F12 * s22 - F22 * s12)) * invJ;
+    return make_double4(F11 * S11 + F12 * S12, F12 * S22 + F11 * S12, F21 * S11 + F22 * S12, F22 * S22 + F21 * S12);
+}
+
+// one support entry added in the reference's entry order; a padding entry
+// has zero derivatives and adds +-0, which leaves every sum unchanged (none
+// of them is ever -0)
+template <int MAT>
+__device__ inline void grad_add(double& a, double& b, double& c, double& d, f64x2 f, f64x2 w) {
+    const double u = w.x, v = w.y;
+    if (MAT == 0) {
+        a += f.x * u;
+        b += f.x * v + f.y * u;
+        c += f.y * v;
+    } else {
+        a += f.x * u;
+        b += f.y * u;
+        c += f.x * v;
+        d += f.y * v;
+    }
+}
+
+__device__ inline double2 contribution(double c0, f64x2 f, const double4& P) {
+    return make_double2(c0 * (f.x * P.x + f.y * P.y), c0 * (f.x * P.z + f.y * P.w));
+}
+
+// ---- FP32-mixed form --------------------------------------------------
+// The gradient is summed as H = F - I from anchored displacements (the
+// block's first owned node subtracted in FP64 before rounding to FP32), and
+// the stress terms are rewritten in H so that no step takes a small
+// difference of O(1) numbers: J - 1 = H11 + H22 + H11 H22 - H12 H21,
+// F11^2 + F12^2 - 1 = 2 H11 + H11^2 + H12^2, ... .  The reference's literal
+// 0.333333333333 leaves an offset 1 - 3 * 0.333333333333 in F11F12 - trB3,
+// which is kept.
+template <int MAT>
+__device__ inline float4 stress_terms32(float Mu, float K2, float a, float b, float c, float d) {
+    if (MAT == 0) {
+        float E11 = a, E12 = 0.5f * b, E22 = c;
+        float S33 = K2 * (E11 + E22);
+        float M2 = 2.f * Mu;
+        float S11 = S33 + M2 * E11, S22 = S33 + M2 * E22, S12 = M2 * E12;
+        return make_float4(S11, S12, S12, S22);
+    }
+    const float H11 = a, H12 = b, H21 = c, H22 = d;
+    const float jm1 = H11 + H22 + (H11 * H22 - H12 * H21);
+    const float invJ = 1.f / (1.f + jm1);
+    const float K = K2 * jm1;
+    const float MuJ53 = Mu * expf(-1.666666666667f * log1pf(jm1));
+    const float A = fmaf(H11, H11, fmaf(H12, H12, 2.f * H11));     // F11^2 + F12^2 - 1
+    const float Bq = fmaf(H22, H22, fmaf(H21, H21, 2.f * H22));    // F21^2 + F22^2 - 1
+    const float third = 0.333333333333f;
+    const float off = (float)(1.0 - 3.0 * 0.333333333333);
+    const float tsum = third * (A + Bq);
+    const float s11 = K + MuJ53 * ((A - tsum) + off);
+    const float s22 = K + MuJ53 * ((Bq - tsum) + off);
+    const float s12 = MuJ53 * (H21 + H12 + fmaf(H11, H21, H12 * H22));
+    const float F11 = 1.f + H11, F12 = H12, F21 = H21, F22 = 1.f + H22;
+    const float S11 = (F12 * (F12 * s22 - F22 * s12) - F22 * (F12 * s12 - F22 * s11)) * invJ;
+    const float S22 = (F11 * (F11 * s22 - F21 * s12) - F21 * (F11 * s12 - F21 * s11)) * invJ;
+    const float S12 = (F21 * (F12 * s12 - F22 * s11) - F11 * (F12 * s22 - F22 * s12)) * invJ;
+    return make_float4(F11 * S11 + F12 * S12, F12 * S22 + F11 * S12, F21 * S11 + F22 * S12, F22 * S22 + F21 * S12);
+}
+
+template <int MAT>
+__device__ inline void grad_add32(float& a, float& b, float& c, float& d, f32x2 f, f32x2 w) {
+    const float u = w.x, v = w.y;
+    if (MAT == 0) {
+        a = fmaf(f.x, u, a);
+        b = fmaf(f.x, v, fmaf(f.y, u, b));
+        c = fmaf(f.y, v, c);
+    } else {
+        a = fmaf(f.x, u, a);
+        b = fmaf(f.y, u, b);
+        c = fmaf(f.x, v, c);
+        d = fmaf(f.y, v, d);
+    }
+}
+
+__device__ inline float2 contribution32(float c0, f32x2 f, const float4& P) {
+    return make_float2(c0 * fmaf(f.x, P.x, f.y * P.y), c0 * fmaf(f.x, P.z, f.y * P.w));
+}
+
+// precision policies of the block kernel
+struct Fp64 {
+    static constexpr bool f32 = false;
+    static constexpr int KREG = KREG_ENTRIES;
+    typedef f64x2 vec;
+    typedef double2 slot;
+    typedef double2 disp;
+    typedef double real;
+};
+struct Fp32 {
+    static constexpr bool f32 = true;
+    static constexpr int KREG = KREG_ENTRIES_F32;
+    typedef f32x2 vec;
+    typedef float2 slot;
+    typedef float2 disp;
+    typedef float real;
+};
+
+// where a lane's entries live in the chunk's stream
+struct LaneMap {
+    int valid;      // lane holds a Gauss point
+    int lane;       // lane index within the chunk
+    int K;          // the wave's largest support: entries per lane
+    int stride;     // lanes in the wave
+    size_t base;    // entry 0 of this lane
+};
+
+__device__ inline LaneMap lane_map(int4 meta, int t) {
+    const int w = t >> 6, lw = t & 63;
+    const int K0 = meta.w & 255, K1 = (meta.w >> 8) & 255, n0 = (meta.w >> 16) & 255, n1 = (meta.w >> 24) & 255;
+    LaneMap L;
+    L.stride = w ? n1 : n0;
+    L.K = w ? K1 : K0;
+    L.valid = lw < L.stride;
+    L.lane = 64 * w + lw;
+    L.base = (size_t)meta.z + (w ? (size_t)K0 * n0 : 0) + lw;
+    return L;
+}
+
+// one lane's chunk data in flight, held in registers until its Gauss-point work
+template <class PR>
+struct LaneRegs {
+    typename PR::vec f[PR::KREG];
+    unsigned pk[PR::KREG];
+    typename PR::real c0;
+    int reg;
+    int tp0[NODES_PER_THREAD], tp1[NODES_PER_THREAD];
+};
+
+template <class PR>
+__device__ inline const typename PR::vec* entry_stream(const BlockArrays& B) {
+    if constexpr (PR::f32) return reinterpret_cast<const f32x2*>(B.be_fxy32);
+    else return reinterpret_cast<const f64x2*>(B.be_fxy);
+}
+
+template <class PR>
+__device__ inline void load_lane(const BlockArrays& B, int4 meta, int2 po, int nown, int t, LaneRegs<PR>& R) {
+    const LaneMap L = lane_map(meta, t);
+    const typename PR::vec* fxy = entry_stream<PR>(B);
+#pragma unroll
+    for (int q = 0; q < NODES_PER_THREAD; q++) {
+        int o = t + q * BLOCK_THREADS;
+        if (o < nown) {
+            R.tp0[q] = B.tptr[po.y + o];
+            R.tp1[q] = B.tptr[po.y + o + 1];
+        }
+    }
+    if (L.valid) {
+        R.c0 = (typename PR::real)B.bgp_c0[meta.x + L.lane];
+        R.reg = B.bgp_region[meta.x + L.lane];
+#pragma unroll
+        for (int i = 0; i < PR::KREG; i++) {
+            if (i < L.K) {
+                size_t e = L.base + (size_t)i * L.stride;
+                R.f[i] = fxy[e];
+                R.pk[i] = B.be_pk[e];
+            }
+        }
+    }
+}
+
+// the Gauss-point work of one lane: gradient, stress, contributions to slots
+template <int MAT>
+__device__ inline void gp_lane(const BlockArrays& B, const LaneMap& L, const LaneRegs<Fp64>& R, const f64x2* uv,
+                               double2* s_cxy, uint16_t* s_creg, bool single, double Mu, double K2) {
+    constexpr int HALF = KREG_ENTRIES / 2;
+    double a = MAT == 0 ? 0. : 1., b = 0., c = 0., d = MAT == 0 ? 0. : 1.;
+    // displacement gathers in two batches of register entries
+    f64x2 u[HALF];
+#pragma unroll
+    for (int i = 0; i < HALF; i++)
+        if (i < L.K) u[i] = uv[R.pk[i] & 0xFFFF];
+#pragma unroll
+    for (int i = 0; i < HALF; i++)
+        if (i < L.K) grad_add<MAT>(a, b, c, d, R.f[i], u[i]);
+#pragma unroll
+    for (int i = 0; i < HALF; i++)
+        if (HALF + i < L.K) u[i] = uv[R.pk[HALF + i] & 0xFFFF];
+#pragma unroll
+    for (int i = 0; i < HALF; i++)
+        if (HALF + i < L.K) grad_add<MAT>(a, b, c, d, R.f[HALF + i], u[i]);
+    // supports longer than the register window: straight from memory
+    const f64x2* fxy = reinterpret_cast<const f64x2*>(B.be_fxy);
+    for (int i = KREG_ENTRIES; i < L.K; i++) {
+        size_t e = L.base + (size_t)i * L.stride;
+        grad_add<MAT>(a, b, c, d, fxy[e], uv[B.be_pk[e] & 0xFFFF]);
+    }
+    const double4 P = stress_terms<MAT>(Mu, K2, a, b, c, d);
+#pragma unroll
+    for (int i = 0; i < KREG_ENTRIES; i++)
+        if (i < L.K) s_cxy[R.pk[i] >> 16] = contribution(R.c0, R.f[i], P);
+    for (int i = KREG_ENTRIES; i < L.K; i++) {
+        size_t e = L.base + (size_t)i * L.stride;
+        s_cxy[B.be_pk[e] >> 16] = contribution(R.c0, fxy[e], P);
+    }
+    if (!single) {
+#pragma unroll
+        for (int i = 0; i < KREG_ENTRIES; i++)
+            if (i < L.K) s_creg[R.pk[i] >> 16] = (uint16_t)R.reg;
+        for (int i = KREG_ENTRIES; i < L.K; i++)
+            s_creg[B.be_pk[L.base + (size_t)i * L.stride] >> 16] = (uint16_t)R.reg;
+    }
+}
+
+// the same in the FP32-mixed form (H sums from anchored displacements)
+template <int MAT>
+__device__ inline void gp_lane32(const BlockArrays& B, const LaneMap& L, const LaneRegs<Fp32>& R, const f32x2* uv,
+                                 float2* s_cxy, uint16_t* s_creg, bool single, float Mu, float K2) {
+    constexpr int HALF = KREG_ENTRIES_F32 / 2;
+    float a = 0.f, b = 0.f, c = 0.f, d = 0.f;
+    f32x2 u[HALF];
+#pragma unroll
+    for (int i = 0; i < HALF; i++)
+        if (i < L.K) u[i] = uv[R.pk[i] & 0xFFFF];
+#pragma unroll
+    for (int i = 0; i < HALF; i++)
+        if (i < L.K) grad_add32<MAT>(a, b, c, d, R.f[i], u[i]);
+#pragma unroll
+    for (int i = 0; i < HALF; i++)
+        if (HALF + i < L.K) u[i] = uv[R.pk[HALF + i] & 0xFFFF];
+#pragma unroll
+    for (int i = 0; i < HALF; i++)
+        if (HALF + i < L.K) grad_add32<MAT>(a, b, c, d, R.f[HALF + i], u[i]);
+    const f32x2* fxy = reinterpret_cast<const f32x2*>(B.be_fxy32);
+    for (int i = KREG_ENTRIES_F32; i < L.K; i++) {
+        size_t e = L.base + (size_t)i * L.stride;
+        grad_add32<MAT>(a, b, c, d, fxy[e], uv[B.be_pk[e] & 0xFFFF]);
+    }
+    const float4 P = stress_terms32<MAT>(Mu, K2, a, b, c, d);
+#pragma unroll
+    for (int i = 0; i < KREG_ENTRIES_F32; i++)
+        if (i < L.K) s_cxy[R.pk[i] >> 16] = contribution32(R.c0, R.f[i], P);
+    for (int i = KREG_ENTRIES_F32; i < L.K; i++) {
+        size_t e = L.base + (size_t)i * L.stride;
+        s_cxy[B.be_pk[e] >> 16] = contribution32(R.c0, fxy[e], P);
+    }
+    if (!single) {
+#pragma unroll
+        for (int i = 0; i < KREG_ENTRIES_F32; i++)
+            if (i < L.K) s_creg[R.pk[i] >> 16] = (uint16_t)R.reg;
+        for (int i = KREG_ENTRIES_F32; i < L.K; i++)
+            s_creg[B.be_pk[L.base + (size_t)i * L.stride] >> 16] = (uint16_t)R.reg;
+    }
+}
+
+}  // namespace blockdev
+
+// Internal forces of the owned nodes of node block blk, written to fix, fiy
+// and returned in fi[0][q], fi[1][q] for the thread's owned node
+// o = threadIdx.x + q * BLOCK_THREADS.  s_cxy: the caller's LDS array of
+// CHUNK_ENTRIES + 1 contribution slots (the last one is the sink).
+template <class PR>
+__device__ __forceinline__ void block_force_update(DevModel& m, const BlockArrays& B, int blk,
+                                                   typename PR::slot* s_cxy, double (&fi)[2][NODES_PER_THREAD]) {
+    using namespace blockdev;
+    typedef typename PR::disp disp_t;
+    typedef typename PR::slot slot_t;
+    extern __shared__ __align__(16) unsigned char s_dyn[];   // (ux, uy) of the touched nodes [, slot regions]
+    __shared__ int4 s_meta[MAX_BLOCK_CHUNKS];
+    __shared__ int2 s_po[MAX_BLOCK_CHUNKS];
+    const int t = threadIdx.x;
+    const int body = B.blk_body[blk];
+    if (m.body_status[body] == 1) return;   // inactive body: no update (uniform over the block)
+    const bool single = B.blk_single_region[blk] != 0;
+    const int nown = B.blk_nown[blk];
+    const int ntouch = B.blk_ntouch[blk];
+    const int* own = B.own_nodes + B.blk_own_off[blk];
+    const int* touch = B.touch_nodes + B.blk_touch_off[blk];
+    const int mat = m.body_material[body];
+    const int mtype = m.mat_type[mat];
+    const double Mu = m.mat_p[6 * mat + 3], K2 = m.mat_p[6 * mat + 4];
+    disp_t* s_uv = reinterpret_cast<disp_t*>(s_dyn);
+    uint16_t* s_creg = reinterpret_cast<uint16_t*>(s_dyn + sizeof(disp_t) * B.max_touch);
+
+    const int nch = B.blk_nchunk[blk];
+    const int4* gmeta = B.ch_meta + B.blk_chunk_off[blk];
+    const int2* gpo = B.ch_ptr_off + B.blk_chunk_off[blk];
+    LaneRegs<PR> R;
+    if (nch > 0) load_lane<PR>(B, gmeta[0], gpo[0], nown, t, R);
+    // chunk descriptors in LDS: read without waiting on the entry stream
+    for (int i = t; i < nch; i += BLOCK_THREADS) {
+        s_meta[i] = gmeta[i];
+        s_po[i] = gpo[i];
+    }
+    if constexpr (PR::f32) {
+        // FP32 displacements relative to the block's first owned node
+        const double ax = nown > 0 ? m.upx[own[0]] : 0., ay = nown > 0 ? m.upy[own[0]] : 0.;
+        for (int i = t; i < ntouch; i += BLOCK_THREADS) {
+            int g = touch[i];
+            s_uv[i] = make_float2((float)(m.upx[g] - ax), (float)(m.upy[g] - ay));
+        }
+    } else {
+        for (int i = t; i < ntouch; i += BLOCK_THREADS) {
+            int g = touch[i];
+            s_uv[i] = make_double2(m.upx[g], m.upy[g]);
+        }
+    }
+    double tx[NODES_PER_THREAD], ty[NODES_PER_THREAD], rx[NODES_PER_THREAD], ry[NODES_PER_THREAD];
+    int cur[NODES_PER_THREAD];
+#pragma unroll
+    for (int q = 0; q < NODES_PER_THREAD; q++) { tx[q] = ty[q] = rx[q] = ry[q] = 0.; cur[q] = -1; }
+
+    for (int c = 0; c < nch; c++) {
+        __syncthreads();      // descriptors and displacements staged; previous chunk's slots read
+        const int4 meta = s_meta[c];
+        const LaneMap L = lane_map(meta, t);
+        if (L.valid) {
+            if constexpr (PR::f32) {
+                const f32x2* uv = reinterpret_cast<const f32x2*>(s_uv);
+                if (mtype == 0) gp_lane32<0>(B, L, R, uv, s_cxy, s_creg, single, (float)Mu, (float)K2);
+                else gp_lane32<1>(B, L, R, uv, s_cxy, s_creg, single, (float)Mu, (float)K2);
+            } else {
+                const f64x2* uv = reinterpret_cast<const f64x2*>(s_uv);
+                if (mtype == 0) gp_lane<0>(B, L, R, uv, s_cxy, s_creg, single, Mu, K2);
+                else gp_lane<1>(B, L, R, uv, s_cxy, s_creg, single, Mu, K2);
+            }
+        }
+        int tp0[NODES_PER_THREAD], tp1[NODES_PER_THREAD];
+#pragma unroll
+        for (int q = 0; q < NODES_PER_THREAD; q++) { tp0[q] = R.tp0[q]; tp1[q] = R.tp1[q]; }
+        // next chunk's entries load while this chunk is summed
+        if (c + 1 < nch) load_lane<PR>(B, s_meta[c + 1], s_po[c + 1], nown, t, R);
+        __syncthreads();      // contributions visible
+        if (single) {
+#pragma unroll
+            for (int q = 0; q < NODES_PER_THREAD; q++) {
+                int o = t + q * BLOCK_THREADS;
+                if (o >= nown) break;
+                if (tp1[q] > tp0[q]) cur[q] = 0;
+#pragma unroll 4
+                for (int p = tp0[q]; p < tp1[q]; p++) {
+                    slot_t v = s_cxy[p];
+                    rx[q] -= (double)v.x;
+                    ry[q] -= (double)v.y;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NODES_PER_THREAD; q++) {
+                int o = t + q * BLOCK_THREADS;
+                if (o >= nown) break;
+                for (int p = tp0[q]; p < tp1[q]; p++) {
+                    slot_t v = s_cxy[p];
+                    int r = s_creg[p];
+                    if (r != cur[q]) {
+                        if (cur[q] >= 0) { tx[q] += rx[q]; ty[q] += ry[q]; }
+                        rx[q] = 0.; ry[q] = 0.; cur[q] = r;
+                    }
+                    rx[q] -= (double)v.x;
+                    ry[q] -= (double)v.y;
+                }
+            }
+        }
+    }
+
+    // internal force of the owned nodes (Node::Sum_up_forces region sum)
+#pragma unroll
+    for (int q = 0; q < NODES_PER_THREAD; q++) {
+        int o = t + q * BLOCK_THREADS;
+        if (o >= nown) break;
+        int j = own[o];
+        if (cur[q] >= 0) { tx[q] += rx[q]; ty[q] += ry[q]; }
+        m.fix[j] = tx[q];
+        m.fiy[j] = ty[q];
+        fi[0][q] = tx[q];
+        fi[1][q] = ty[q];
+    }
+}
+
+
+}  // namespace mel

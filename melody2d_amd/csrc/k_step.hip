@@ -6,37 +6,9 @@
 // fixed element order instead of the reference's body/send order.
 #include "mel_device.hpp"
 #include "k_step.hpp"
-#include "k_block.hpp"
+#include "k_step_dev.hpp"
 
 namespace mel {
-
-// ---------------------------------------------------------------------------
-// Body::Update_borders (Code/Body.h:614-661): tributary length per slot.
-// ---------------------------------------------------------------------------
-__device__ inline double slot_length(const DevModel& m, int s) {
-    int br = m.sl_border[s];
-    int j = m.sl_j[s];
-    int n = m.br_nslots[br];
-    double P = m.period;
-    int a0 = m.sl_n0[s], a1 = m.sl_n1[s], a2 = m.sl_n2[s];
-    double x0 = m.xc[a0] + P * m.sl_s0[s], y0 = m.yc[a0];
-    double x1 = m.xc[a1] + P * m.sl_s1[s], y1 = m.yc[a1];
-    double x2 = m.xc[a2] + P * m.sl_s2[s], y2 = m.yc[a2];
-    double l21 = __dsqrt_rn((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
-    double l10 = __dsqrt_rn((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
-    bool simple = m.br_periodicity[br] == 1;
-    if (j == 0 && simple) return l21 * 0.5;
-    if (j == n - 1 && simple) return l10 * 0.5;
-    return (l21 + l10) * 0.5;
-}
-
-// current tributary length of a slot: deformable borders from the current
-// positions (the step recomputes them where they are used), rigid borders
-// keep their stored value
-__device__ inline double slot_len_now(const DevModel& m, int s) {
-    int b = m.br_body[m.sl_border[s]];
-    return m.body_rigid_flag[b] ? m.sl_len[s] : slot_length(m, s);
-}
 
 __global__ void k_border_length(DevModel m) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -46,264 +18,12 @@ __global__ void k_border_length(DevModel m) {
     m.sl_len[s] = slot_length(m, s);
 }
 
-// ---------------------------------------------------------------------------
-// Contact laws, Code/Model.h:81-335 (0.5 factors applied by the caller as in
-// Code/Body.h:1703-1789)
-// ---------------------------------------------------------------------------
-__device__ inline void law_mohr_coulomb(double kn, double kt, double fric, double coh, double tens, double gapn,
-                                        double& gapt, double& Pn, double& Pt) {
-    Pn = -kn * gapn;
-    Pt = -kt * gapt;
-    if (Pn < -tens) { Pn = 0.; Pt = 0.; gapt = 0.; }
-    else if (fabs(Pt) > coh + fric * Pn) {
-        if (gapt > 0.) Pt = -coh - fric * Pn;
-        else Pt = coh + fric * Pn;
-        gapt = -Pt / kt;
-    }
-}
-
-__device__ inline void law_czm_linear(double kini, double Pnlim, double gapnlim, double Pnres, double NT, double gapn,
-                                      double& gapt, double& D, double& Pn, double& Pt) {
-    if (D >= 1.) {
-        Pn = -kini * gapn;
-        Pt = -kini * gapt;
-        if (Pn < -Pnres) { Pn = 0.; Pt = 0.; gapt = 0.; }
-        else if (fabs(Pt) > Pnres * NT) {
-            Pt = gapt > 0. ? -Pnres * NT : Pnres * NT;
-            gapt = -Pt / kini;
-        }
-        return;
-    }
-    double gnmax = Pnlim / kini * (1. - D) + gapnlim * D;
-    double gtmax = gnmax * NT;
-    double kmax;
-    bool n_in = gapn < gnmax, n_out = gapn > gnmax;
-    bool t_in = fabs(gapt) < gtmax, t_out = fabs(gapt) > gtmax;
-    if (n_in && t_in) {
-        kmax = (Pnlim * (1. - D) + Pnres * D) / gnmax;
-    } else if (n_out && t_in) {
-        gnmax = gapn; gtmax = gnmax * NT;
-        D = (gnmax - Pnlim / kini) / (gapnlim - Pnlim / kini);
-        kmax = (Pnlim * (1. - D) + Pnres * D) / gnmax;
-    } else if (n_in && t_out) {
-        gtmax = fabs(gapt); gnmax = gtmax / NT;
-        D = (gnmax - Pnlim / kini) / (gapnlim - Pnlim / kini);
-        kmax = (Pnlim * (1. - D) + Pnres * D) / gnmax;
-    } else if (n_out && t_out) {
-        if (gapn / gnmax > fabs(gapt) / gtmax) { gnmax = gapn; gtmax = gnmax * NT; }
-        else { gtmax = fabs(gapt); gnmax = gtmax / NT; }
-        D = (gnmax - Pnlim / kini) / (gapnlim - Pnlim / kini);
-        kmax = (Pnlim * (1. - D) + Pnres * D) / gnmax;
-    } else {
-        return;
-    }
-    if (gapn < 0.) Pn = -kini * gapn;
-    else Pn = -kmax * gapn;
-    Pt = -kmax * gapt;
-}
-
-__device__ inline void bucket_append(const DevModel& m, int bn, int r) {
-    int pos = atomicAdd(&m.bk_cursor[bn], 1);
-    if (pos < BUCKET_CAP) m.bk_rec[bn * BUCKET_CAP + pos] = r;
-    else atomicOr(&m.status_flags[0], 1);
-}
-
-// Body::Update_contact_forces (Code/Body.h:1642-1941) minus the scatter, for
-// one element after its re-projection (Body::Update_contacts,
-// Code/Body.h:1341-1634): the element's force and its point-force records
-__device__ inline void contact_law_element(const DevModel& m, int e) {
-    const ContactArrays& C = m.C;
-    m.rec_bn[3 * e] = -1; m.rec_bn[3 * e + 1] = -1; m.rec_bn[3 * e + 2] = -1;
-    for (int q = 0; q < 6; q++) m.rig_rec[6 * e + q] = 0.;
-    int bS = C.bodyS[e];
-    if (m.body_status[bS] == 1) return;
-    int bM = C.bodyM[e];
-    int li = m.law_table[m.body_material[bS] * m.n_materials + m.body_material[bM]];
-    if (li < 0) return;
-    int slot = slot_of(m, bS, C.borderS[e], C.bnodeS[e]);
-    double length = slot_len_now(m, slot);
-    double length0 = m.sl_len0[slot];
-    double gapn = C.gapn[e], gapt = C.gapt[e];
-    const double* p = m.law_p + 12 * li;
-    double Pn = 0., Pt = 0., W = 0.;
-    int lt = m.law_type[li];
-    if (lt == 0) {                      // Frictionless
-        Pn = gapn > 0. ? 0. : -p[0] * gapn;
-        Pt = 0.;
-        gapt = 0.;
-    } else if (lt == 1) {               // Cohesion
-        double gtang = p[2] * 0.5, gtens = p[3] * 0.5;
-        Pn = -p[0] * gapn;
-        Pt = -p[1] * gapt;
-        if (Pn < -gtens) { Pn = 0.; Pt = 0.; gapt = 0.; }
-        else if (fabs(Pt) > gtang) { Pt = gapt > 0. ? -gtang : gtang; gapt = -Pt / p[1]; }
-    } else if (lt == 2) {               // MohrCoulomb
-        law_mohr_coulomb(p[0], p[1], p[2], p[3] * 0.5, p[4] * 0.5, gapn, gapt, Pn, Pt);
-    } else if (lt == 3) {               // DampedMohrCoulomb
-        double kn = p[0], kt = p[1], fric = p[2], coh = p[3] * 0.5, tens = p[4] * 0.5, damp = p[5];
-        double mass = C.emass[e], L0 = length0, vgn = C.vgapn[e], vgt = C.vgapt[e];
-        Pn = 0.;
-        if (gapn < 0.) { Pn = -damp * vgn * 2. * __dsqrt_rn(mass * kn / L0); W += Pn * vgn; }
-        Pn -= kn * gapn;
-        if (Pn < -tens) { Pn = 0.; Pt = 0.; gapt = 0.; }
-        else {
-            double Ptd = -damp * vgt * 2. * __dsqrt_rn(mass * kt / L0);
-            Pt = -kt * gapt + Ptd;
-            if (fabs(Pt) > coh + fric * Pn) {
-                Pt = gapt > 0. ? -coh - fric * Pn : coh + fric * Pn;
-                gapt = -Pt / kt;
-                W += Pt * vgt;
-            } else W += Ptd * vgt;
-        }
-    } else if (lt == 4) {               // TwoSlopesMohrCoulomb
-        double kn = p[0], kt = p[1], fric = p[2], coh = p[3] * 0.5, tens = p[4] * 0.5, rati = p[5];
-        Pn = C.vgapn[e] <= 0. ? -kn * gapn : -kn * gapn * rati;
-        if (Pn < -tens) { Pn = 0.; Pt = 0.; gapt = 0.; }
-        else {
-            Pt = -kt * gapt;
-            if (fabs(Pt) > coh + fric * Pn) { Pt = gapt > 0. ? -coh - fric * Pn : coh + fric * Pn; gapt = -Pt / kt; }
-        }
-    } else if (lt == 6) {               // CZMlinear
-        double D = C.int0[e];
-        if (C.nbint[e] == 0) { D = 1.; C.nbint[e] = 1; }
-        law_czm_linear(p[0], p[1] * 0.5, p[2], p[3] * 0.5, p[4], gapn, gapt, D, Pn, Pt);
-        C.int0[e] = D;
-    }
-    double Px = C.xn[e] * Pn + C.xt[e] * Pt;
-    double Py = C.yn[e] * Pn + C.yt[e] * Pt;
-    double L = m.law_fixed[li] ? length0 : length;
-    double Fx = Px * L, Fy = Py * L;
-    W *= L * m.dt;
-    C.gapn[e] = gapn; C.gapt[e] = gapt; C.len[e] = length; C.fx[e] = Fx; C.fy[e] = Fy;
-    if (W != 0.) {
-        atomicAdd(&m.body_rigid[bS * RIGID_FIELDS + R_WORK + W_CONTACT], 0.5 * W);
-        atomicAdd(&m.body_rigid[bM * RIGID_FIELDS + R_WORK + W_CONTACT], 0.5 * W);
-    }
-    int nodeS = C.nodeS[e];
-    if (!m.body_rigid_flag[bS]) {
-        int bn = m.sl_bn[slot];
-        m.rec_bn[3 * e] = bn; m.rec_fx[3 * e] = Fx; m.rec_fy[3 * e] = Fy;
-        bucket_append(m, bn, 3 * e);
-    } else {
-        const double* R = m.body_rigid + bS * RIGID_FIELDS;
-        double dx = m.xc[nodeS] - R[R_CUR], dy = m.yc[nodeS] - R[R_CUR + 1];
-        m.rig_rec[6 * e] = Fx; m.rig_rec[6 * e + 1] = Fy; m.rig_rec[6 * e + 2] = -Fx * dy + Fy * dx;
-    }
-    if (!m.body_rigid_flag[bM]) {
-        int sl = slot_of(m, bM, C.borderM[e], C.segM[e]);
-        int bn1 = m.sl_bn[sl], bn2 = m.sl_bn[sl + 1];
-        m.rec_bn[3 * e + 1] = bn1; m.rec_fx[3 * e + 1] = -Fx * C.N1[e]; m.rec_fy[3 * e + 1] = -Fy * C.N1[e];
-        m.rec_bn[3 * e + 2] = bn2; m.rec_fx[3 * e + 2] = -Fx * C.N2[e]; m.rec_fy[3 * e + 2] = -Fy * C.N2[e];
-        bucket_append(m, bn1, 3 * e + 1);
-        bucket_append(m, bn2, 3 * e + 2);
-    } else {
-        const double* R = m.body_rigid + bM * RIGID_FIELDS;
-        double dx = m.xc[nodeS] - R[R_CUR] - C.shiftM[e] * m.period;
-        double dy = m.yc[nodeS] - R[R_CUR + 1];
-        m.rig_rec[6 * e + 3] = -Fx; m.rig_rec[6 * e + 4] = -Fy; m.rig_rec[6 * e + 5] = Fx * dy - Fy * dx;
-    }
-}
-
-// point contact force per border node: records summed in record-id order
-__device__ inline void bucket_sum_node(const DevModel& m, int bn) {
-    int n = min(m.bk_cursor[bn], BUCKET_CAP);
-    int ids[BUCKET_CAP];
-    for (int k = 0; k < n; k++) {
-        int v = m.bk_rec[bn * BUCKET_CAP + k];
-        int q = k;
-        while (q > 0 && ids[q - 1] > v) { ids[q] = ids[q - 1]; q--; }
-        ids[q] = v;
-    }
-    double px = 0., py = 0.;
-    for (int k = 0; k < n; k++) { px += m.rec_fx[ids[k]]; py += m.rec_fy[ids[k]]; }
-    m.bn_pcx[bn] = px;
-    m.bn_pcy[bn] = py;
-    m.bk_cursor[bn] = 0;
-}
-
-// ---------------------------------------------------------------------------
-// Border::Update_bc (Code/Border.h:120-221): current BC values at time t
-// ---------------------------------------------------------------------------
-__device__ void bc_value(const DevModel& m, int type, int h0, int h1, double t, double& value, double& vel) {
-    if (type == 1) return;
-    if (type == 2) {
-        value = 0.;
-        for (int n = h0; n + 1 < h1; n++) {
-            double t0 = m.hist_t[n], p0 = m.hist_v[n], t1 = m.hist_t[n + 1], p1 = m.hist_v[n + 1];
-            if (t >= t1) value = value + (t1 - t0) * (p1 + p0) * 0.5;
-            else {
-                value = value + (t - t0) * (p0 + (p1 - p0) / (t1 - t0) * ((t - t0) * 0.5));
-                vel = p0 + (p1 - p0) / (t1 - t0) * (t - t0);
-                break;
-            }
-        }
-    } else if (type == 3 || type == 4) {
-        for (int n = h0; n + 1 < h1; n++) {
-            double t0 = m.hist_t[n], p0 = m.hist_v[n], t1 = m.hist_t[n + 1], p1 = m.hist_v[n + 1];
-            if ((t0 <= t) & (t <= t1)) { value = p0 + (p1 - p0) / (t1 - t0) * (t - t0); break; }
-        }
-    } else {
-        value = 0.;
-    }
-}
-
-__device__ inline void bc_eval_at(const DevModel& m, int br, double t) {
-    const int* hp = m.br_hist_ptr + 2 * br;
-    bc_value(m, m.br_xtype[br], hp[0], hp[1], t, m.br_xval[br], m.br_xvel[br]);
-    bc_value(m, m.br_ytype[br], hp[1], hp[2], t, m.br_yval[br], m.br_yvel[br]);
-}
-
 __global__ void k_bc_eval(DevModel m, int plus_dt) {
     int br = blockIdx.x * blockDim.x + threadIdx.x;
     if (br >= m.n_borders) return;
     double t = *m.time;
     if (plus_dt) t = t + m.dt;
     bc_eval_at(m, br, t);
-}
-
-// Body::Update_bc_forces deformable branch (Code/Body.h:672-815): point
-// Neumann / Dirichlet values per border node (one slot per border node in
-// practice; several slots are summed in slot order).
-__device__ inline void bc_point_force_node(const DevModel& m, int bn) {
-    double cnx = 0., cny = 0., cdx = 0., cdy = 0.;
-    int node = m.bn_node[bn];
-    int b = m.node_body[node];
-    if (!m.body_rigid_flag[b] && m.body_status[b] != 1) {
-        for (int q = m.bn_slot_ptr[bn]; q < m.bn_slot_ptr[bn + 1]; q++) {
-            int s = m.bn_slots[q];
-            int br = m.sl_border[s];
-            int xt = m.br_xtype[br], yt = m.br_ytype[br];
-            if (xt == 0 && yt == 0) continue;
-            if (m.sl_j[s] >= m.br_nslots[br] - 1) continue;
-            double L = slot_length(m, s);
-            double px = 0., py = 0.;
-            double tx = 0, ty = 0, tpx = 0, tpy = 0;
-            if (xt == 4 || yt == 4) {
-                int a = m.sl_n0[s], c = m.sl_n2[s];
-                double x0 = m.X[a], y0 = m.Y[a], x2 = m.X[c], y2 = m.Y[c];
-                double xp0 = m.xc[a], yp0 = m.yc[a], xp2 = m.xc[c], yp2 = m.yc[c];
-                double invl = inv_sqrt((x2 - x0) * (x2 - x0) + (y2 - y0) * (y2 - y0));
-                double invlp = inv_sqrt((xp2 - xp0) * (xp2 - xp0) + (yp2 - yp0) * (yp2 - yp0));
-                tx = (x2 - x0) * invl; ty = (y2 - y0) * invl;
-                tpx = (xp2 - xp0) * invlp; tpy = (yp2 - yp0) * invlp;
-            }
-            if (xt == 3) px = m.br_xval[br];
-            else if (xt == 4) { px += m.br_xval[br] * (tx * tpx + ty * tpy); py += m.br_xval[br] * (tx * tpy - ty * tpx); }
-            else if (xt == 2) px = m.br_xparam[2 * br] * (m.br_xval[br] - m.ux[node]) - m.br_xparam[2 * br + 1] * (m.br_xvel[br] - m.vx[node]);
-            if (yt == 3) py = m.br_yval[br];
-            else if (yt == 4) { px += m.br_yval[br] * (ty * tpx - tx * tpy); py += m.br_yval[br] * (ty * tpy + tx * tpx); }
-            else if (yt == 2) py = m.br_yparam[2 * br] * (m.br_yval[br] - m.uy[node]) - m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node]);
-            double a_nx = 0., a_ny = 0., a_dx = 0., a_dy = 0.;
-            if (xt == 3) a_nx = px * L;
-            else if (xt == 4) { a_nx = px * L; a_ny = py * L; }
-            else if (xt == 2) a_dx = px * L;
-            if (yt == 3) a_ny = py * L;
-            else if (yt == 4) { a_nx = px * L; a_ny = py * L; }
-            else if (yt == 2) a_dy = py * L;
-            cnx += a_nx; cny += a_ny; cdx += a_dx; cdy += a_dy;
-        }
-    }
-    m.bn_pnx[bn] = cnx; m.bn_pny[bn] = cny; m.bn_pdx[bn] = cdx; m.bn_pdy[bn] = cdy;
 }
 
 // Body::Update_damping_forces (Code/Body.h:1057-1154): node gather over the
@@ -330,185 +50,6 @@ __global__ void k_damping(DevModel m) {
     m.fky[j] = fy;
 }
 
-// ---------------------------------------------------------------------------
-// rigid bodies: one thread per body for the serial parts
-// ---------------------------------------------------------------------------
-// Body::Update_bc_forces rigid branch (Code/Body.h:816-899) + Update_body_forces
-// + the contact resultant (slave records and the serial sends of
-// Code/Body.h:1948-1967), reduced in element order.
-__device__ inline void rigid_forces_block(const DevModel& m, int rigid_index) {
-    int b = m.rigid_ids[rigid_index];
-    if (m.body_status[b] == 1) return;
-    double* R = m.body_rigid + b * RIGID_FIELDS;
-    __shared__ double red[256][3];
-    int tid = threadIdx.x;
-    double sx = 0., sy = 0., sm = 0.;
-    int nc = *m.c_count;
-    for (int e = tid; e < nc; e += blockDim.x) {
-        if (m.C.bodyS[e] == b) { sx += m.rig_rec[6 * e]; sy += m.rig_rec[6 * e + 1]; sm += m.rig_rec[6 * e + 2]; }
-        if (m.C.bodyM[e] == b) { sx += m.rig_rec[6 * e + 3]; sy += m.rig_rec[6 * e + 4]; sm += m.rig_rec[6 * e + 5]; }
-    }
-    red[tid][0] = sx; red[tid][1] = sy; red[tid][2] = sm;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (tid < w) { red[tid][0] += red[tid + w][0]; red[tid][1] += red[tid + w][1]; red[tid][2] += red[tid + w][2]; }
-        __syncthreads();
-    }
-    if (tid == 0) { R[R_FCON] = red[0][0]; R[R_FCON + 1] = red[0][1]; R[R_FCON + 2] = red[0][2]; }
-    // boundary conditions: every thread adds the terms of a contiguous run of
-    // slots, then the runs are added in slot order (a two-level version of
-    // the reference's serial loop)
-    __shared__ double part[256][6];
-    double acc[6] = {0., 0., 0., 0., 0., 0.};
-    const double xc0 = R[R_CUR], yc0 = R[R_CUR + 1];
-    const int gb0 = m.border_off[b];
-    for (int k = 0; k < m.body_nborders[b]; k++) {
-        const int br = gb0 + k;
-        const int xt = m.br_xtype[br], yt = m.br_ytype[br];
-        if (xt == 0 && yt == 0) continue;
-        const int ns = m.br_nslots[br];
-        const int run = (ns + blockDim.x - 1) / blockDim.x;
-        double p6[6] = {0., 0., 0., 0., 0., 0.};
-        for (int j = tid * run; j < min(ns, (tid + 1) * run); j++) {
-            int s = m.br_slot0[br] + j;
-            int node = m.sl_node[s];
-            double L = m.sl_len[s];
-            double fxn = 0., fyn = 0., fxd = 0., fyd = 0.;
-            if (xt == 3 || xt == 4) fxn = m.br_xval[br] * L;
-            else if (xt == 2) fxd = (m.br_xparam[2 * br] * (m.br_xval[br] - m.ux[node]) + m.br_xparam[2 * br + 1] * (m.br_xvel[br] - m.vx[node])) * L;
-            if (yt == 3 || yt == 4) fyn = m.br_yval[br] * L;
-            else if (yt == 2) fyd = (m.br_yparam[2 * br] * (m.br_yval[br] - m.uy[node]) + m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node])) * L;
-            double dx = m.xc[node] - xc0, dy = m.yc[node] - yc0;
-            p6[0] += fxn; p6[1] += fyn; p6[2] += -fxn * dy + fyn * dx;
-            p6[3] += fxd; p6[4] += fyd; p6[5] += -fxd * dy + fyd * dx;
-        }
-        __syncthreads();
-        for (int q = 0; q < 6; q++) part[tid][q] = p6[q];
-        __syncthreads();
-        if (tid < 6) {
-            double a = acc[tid];
-            for (int r = 0; r < (int)blockDim.x; r++) a += part[r][tid];
-            acc[tid] = a;
-        }
-    }
-    __shared__ double fsum[6];
-    if (tid < 6) fsum[tid] = acc[tid];
-    __syncthreads();
-    if (tid != 0) return;
-    for (int q = 0; q < 3; q++) { R[R_FNEU + q] = fsum[q]; R[R_FDIR + q] = fsum[3 + q]; }
-    double gx = m.gravity_x, gy = m.gravity_y;
-    if (m.body_status[b] == 2) { gx = 0.; gy = 0.; }
-    R[R_FBODY] = m.body_mass[b] * gx;
-    R[R_FBODY + 1] = m.body_mass[b] * gy;
-}
-
-// driven DOFs of Body::Update_bc (Code/Body.h:340-393) at time t
-__device__ void driven(const DevModel& m, int h0, int h1, double t, double* d) {
-    d[0] = 1.;
-    d[1] = 0.;
-    for (int n = h0; n + 1 < h1; n++) {
-        double t0 = m.hist_t[n], p0 = m.hist_v[n], t1 = m.hist_t[n + 1], p1 = m.hist_v[n + 1];
-        if (t >= t1) d[1] += (t1 - t0) * (p1 + p0) * 0.5;
-        else {
-            d[1] += (t - t0) * (p0 + (p1 - p0) / (t1 - t0) * ((t - t0) * 0.5));
-            d[2] = (t - t0) * (p1 - p0) / (t1 - t0);
-            d[3] = (p1 - p0) / (t1 - t0);
-            break;
-        }
-    }
-}
-
-__device__ void rigid_driven_dofs(const DevModel& m, int b, double t, double* dv) {
-    int gb0 = m.border_off[b];
-    for (int k = 0; k < m.body_nborders[b]; k++) {
-        int br = gb0 + k;
-        const int* hp = m.br_hist_ptr + 2 * br;
-        if (m.br_xtype[br] == 1) driven(m, hp[0], hp[1], t, dv);
-        if (m.br_ytype[br] == 1) driven(m, hp[1], hp[2], t, dv + 4);
-    }
-}
-
-// Sum_up_forces, Apply_Newton, Update_bc(t+dt), Apply_Euler for a rigid body
-// (Code/Body.h:1984-2014, 2032-2037, 2071-2117)
-__device__ inline void rigid_step_body(const DevModel& m, int b) {
-    if (m.body_status[b] == 1) return;
-    double* R = m.body_rigid + b * RIGID_FIELDS;
-    double dt = m.dt;
-    for (int q = 0; q < 3; q++)
-        R[R_FTOT + q] = R[R_FCON + q] + R[R_FBODY + q] + R[R_FDIR + q] + R[R_FNEU + q] + R[R_FDAMP + q];
-    R[R_ACC] = R[R_FTOT] * m.body_inv_mass[b];
-    R[R_ACC + 1] = R[R_FTOT + 1] * m.body_inv_mass[b];
-    R[R_ACC + 2] = R[R_FTOT + 2] * m.body_inv_inertia[b];
-    double nacc_x = R[R_ACC], nacc_y = R[R_ACC + 1];
-    R[R_NODE_ACC] = nacc_x; R[R_NODE_ACC + 1] = nacc_y;   // node copies of Body::Apply_Newton (Code/Body.h:2038-2042)
-    R[R_FDIR_NODE] = R[R_FDIR]; R[R_FDIR_NODE + 1] = R[R_FDIR + 1];   // copied to nodes by Code/Body.h:2005-2006
-    double* dv = R + R_DRIVEN;
-    rigid_driven_dofs(m, b, *m.time + dt, dv);
-    double hx = R[R_VEL] + 0.5 * dt * R[R_ACC];
-    double hy = R[R_VEL + 1] + 0.5 * dt * R[R_ACC + 1];
-    double hr = R[R_VEL + 2] + 0.5 * dt * R[R_ACC + 2];
-    R[R_WORK + W_BODY] += dt * (hx * R[R_FBODY] + hy * R[R_FBODY + 1] + hr * R[R_FBODY + 2]);
-    R[R_WORK + W_DIRICHLET] += dt * (hx * R[R_FDIR] + hy * R[R_FDIR + 1] + hr * R[R_FDIR + 2]);
-    R[R_WORK + W_NEUMANN] += dt * (hx * R[R_FNEU] + hy * R[R_FNEU + 1] + hr * R[R_FNEU + 2]);
-    R[R_WORK + W_DAMPING] += dt * (hx * R[R_FDAMP] + hy * R[R_FDAMP + 1] + hr * R[R_FDAMP + 2]);
-    double mass = m.body_mass[b];
-    if (dv[0] == 0.) {
-        R[R_VEL] += dt * R[R_ACC];
-        R[R_DISP] = R[R_DISP] + dt * R[R_VEL];
-    } else if (dv[0] == 1.) {
-        R[R_ACC] = dv[3]; R[R_VEL] = dv[2]; R[R_DISP] = dv[1];
-        R[R_FDIR] = mass * R[R_ACC] - R[R_FTOT];
-    }
-    if (dv[4] == 0.) {
-        R[R_VEL + 1] += dt * R[R_ACC + 1];
-        R[R_DISP + 1] = R[R_DISP + 1] + dt * R[R_VEL + 1];
-    } else if (dv[4] == 1.) {
-        R[R_ACC + 1] = dv[7]; R[R_VEL + 1] = dv[6]; R[R_DISP + 1] = dv[5];
-        R[R_FDIR + 1] = mass * R[R_ACC + 1] - R[R_FTOT + 1];
-    }
-    if (dv[0] == 0. && dv[4] == 0. && m.body_periodicity[b] != 2) {
-        R[R_VEL + 2] += dt * R[R_ACC + 2];
-        R[R_DISP + 2] = R[R_DISP + 2] + dt * R[R_VEL + 2];
-    } else {
-        R[R_VEL + 2] = 0.;
-        R[R_DISP + 2] = 0.;
-    }
-    R[R_CUR] = R[R_XC] + R[R_DISP];
-    R[R_CUR + 1] = R[R_YC] + R[R_DISP + 1];
-    R[R_CUR + 2] = 0. + R[R_DISP + 2];
-}
-
-// rigid node kinematics and positions (Code/Body.h:2214-2226, 2281-2301);
-// node force fields mirror the body totals as Code/Body.h:1999-2013 does.
-__device__ inline void rigid_node(const DevModel& m, int i, int with_forces) {
-    int b = m.node_body[i];
-    if (!m.body_rigid_flag[b] || m.body_status[b] == 1) return;
-    const double* R = m.body_rigid + b * RIGID_FIELDS;
-    double dr = R[R_DISP + 2] - 0.;
-    double dx = m.X[i] - R[R_XC], dy = m.Y[i] - R[R_YC];
-    double c = cos(dr), s = sin(dr);
-    m.ux[i] = R[R_DISP] + dx * c - dy * s - dx;
-    m.uy[i] = R[R_DISP + 1] + dx * s + dy * c - dy;
-    double xx = R[R_CUR] + dx * c - dy * s;
-    double yy = R[R_CUR + 1] + dx * s + dy * c;
-    m.xc[i] = xx;
-    m.yc[i] = yy;
-    double ex = xx - R[R_CUR], ey = yy - R[R_CUR + 1];
-    m.vx[i] = -ey * R[R_VEL + 2] + R[R_VEL];
-    m.vy[i] = ex * R[R_VEL + 2] + R[R_VEL + 1];
-    if (with_forces) {
-        m.ax[i] = R[R_NODE_ACC]; m.ay[i] = R[R_NODE_ACC + 1];
-        m.fcx[i] = R[R_FCON]; m.fcy[i] = R[R_FCON + 1];
-        m.fbx[i] = R[R_FBODY]; m.fby[i] = R[R_FBODY + 1];
-        m.fnx[i] = R[R_FNEU]; m.fny[i] = R[R_FNEU + 1];
-        m.fkx[i] = R[R_FDAMP]; m.fky[i] = R[R_FDAMP + 1];
-        m.fx[i] = R[R_FTOT]; m.fy[i] = R[R_FTOT + 1];
-        // Body::Sum_up_forces copies the Dirichlet resultant before Apply_Euler
-        // overwrites it for driven directions
-        m.fdx[i] = R[R_FDIR_NODE]; m.fdy[i] = R[R_FDIR_NODE + 1];
-    }
-}
-
 __global__ void k_rigid_nodes(DevModel m, int with_forces) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m.n_rigid_nodes) return;
@@ -530,29 +71,15 @@ __global__ void k_rigid_nodes(DevModel m, int with_forces) {
 //                     Update_bc(t + dt) | time = t + dt
 // ---------------------------------------------------------------------------
 __global__ void k_step_contacts(DevModel m, int nc, const int* bc_bn, int n_bc) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nc) {
-        if (m.body_status[m.C.bodyS[i]] != 1) project_element(m, m.C, i);
-        contact_law_element(m, i);
-        return;
-    }
-    i -= nc;
-    if (i < n_bc) bc_point_force_node(m, bc_bn[i]);
+    contact_or_bc_item(m, blockIdx.x * blockDim.x + threadIdx.x, nc, bc_bn, n_bc);
 }
 
-// Body::Update_borders (Code/Body.h:614-661) for the slots of one border node
-__device__ inline void bn_lengths(const DevModel& m, int bn) {
-    for (int q = m.bn_slot_ptr[bn]; q < m.bn_slot_ptr[bn + 1]; q++) {
-        int s = m.bn_slots[q];
-        m.sl_len[s] = slot_length(m, s);
-    }
-}
-
-__global__ void k_step_gather(DevModel m, const int* split_bn, int n_split) {
+__global__ void __launch_bounds__(BLOCK_THREADS) k_step_gather(DevModel m, const int* split_bn, int n_split) {
+    __shared__ double sc[6 * BLOCK_THREADS + 8];
     if ((int)blockIdx.x < m.n_rigid) {
         // resultants, then Newton + Euler of the same body by thread 0
-        rigid_forces_block(m, blockIdx.x);
-        if (threadIdx.x == 0) rigid_step_body(m, m.rigid_ids[blockIdx.x]);
+        const RigidForces F = rigid_forces_block(m, blockIdx.x, sc);
+        if (threadIdx.x == 0) rigid_step_body(m, m.rigid_ids[blockIdx.x], F);
         return;
     }
     // border nodes of bodies split over several node blocks: bucket sums and
@@ -564,89 +91,6 @@ __global__ void k_step_gather(DevModel m, const int* split_bn, int n_split) {
         int b = m.node_body[m.bn_node[bn]];
         if (!m.body_rigid_flag[b] && m.body_status[b] != 1) bn_lengths(m, bn);
     }
-}
-
-// The rest of the step per node block (a whole body): send of the contact
-// point forces to the block's border nodes (bucket sums), Sum_up_forces +
-// Apply_Newton + Apply_Euler with works for the owned nodes
-// (Code/Body.h:1975-2068, Code/Node.h:212-245), Update_borders for the
-// block's border slots (positions of the start of the step) and
-// Update_kinematics / Update_current_positions for its border nodes.
-// Workgroups past the node blocks move the rigid nodes, evaluate the BCs at
-// t + dt and store the new time.
-__device__ inline void node_block_finish(const DevModel& m, const BlockArrays& B, int blk) {
-    __shared__ double red[BLOCK_THREADS / 64][5];
-    const int t = threadIdx.x;
-    const int body = B.blk_body[blk];
-    if (m.body_status[body] == 1) return;   // inactive body: no update (uniform over the block)
-    const int nown = B.blk_nown[blk];
-    const int* own = B.own_nodes + B.blk_own_off[blk];
-    const int* bns = B.bn_list + B.blk_bn_off[blk];
-    const int nbn = B.blk_bn_n[blk];        // 0 for blocks of split bodies
-    for (int i = t; i < nbn; i += BLOCK_THREADS) bucket_sum_node(m, bns[i]);
-    __syncthreads();
-    double w[5] = {0., 0., 0., 0., 0.};   // internal body dirichlet neumann damping
-    double gx = m.gravity_x, gy = m.gravity_y;
-    if (m.body_status[body] == 2) { gx = 0.; gy = 0.; }
-    const double dt = m.dt;
-#pragma unroll
-    for (int q = 0; q < NODES_PER_THREAD; q++) {
-        int o = t + q * BLOCK_THREADS;
-        if (o >= nown) break;
-        int j = own[o];
-        double ix = m.fix[j], iy = m.fiy[j];
-        double cx = 0., cy = 0., dx = 0., dy = 0., nx = 0., ny = 0.;
-        for (int e = m.bt_ptr[j]; e < m.bt_ptr[j + 1]; e++) {
-            int bn = m.bt_bn[e];
-            double f = m.bt_phi[e];
-            cx += m.bn_pcx[bn] * f; cy += m.bn_pcy[bn] * f;
-            dx += m.bn_pdx[bn] * f; dy += m.bn_pdy[bn] * f;
-            nx += m.bn_pnx[bn] * f; ny += m.bn_pny[bn] * f;
-        }
-        double bx = m.mx[j] * gx, by = m.my[j] * gy;
-        double kx = m.fkx[j], ky = m.fky[j];
-        double fx = ix + cx + 0. + bx + dx + nx + kx + 0.;
-        double fy = iy + cy + 0. + by + dy + ny + ky + 0.;
-        double apx = fx * m.imx[j], apy = fy * m.imy[j];
-        double vpx = m.vpx[j], vpy = m.vpy[j];
-        double hx = vpx + 0.5 * dt * apx, hy = vpy + 0.5 * dt * apy;
-        w[0] += dt * (hx * ix + hy * iy);
-        w[1] += dt * (hx * bx + hy * by);
-        w[2] += dt * (hx * dx + hy * dy);
-        w[3] += dt * (hx * nx + hy * ny);
-        w[4] += dt * (hx * kx + hy * ky);
-        vpx = vpx + dt * apx;
-        vpy = vpy + dt * apy;
-        m.upx[j] = m.upx[j] + dt * vpx;
-        m.upy[j] = m.upy[j] + dt * vpy;
-        m.vpx[j] = vpx; m.vpy[j] = vpy;
-        m.apx[j] = apx; m.apy[j] = apy;
-    }
-    // block work increments: fixed-order reduction (wavefront butterflies,
-    // then the waves in order), accumulated with compensation across steps
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-        double v = w[q];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if ((t & 63) == 0) red[t >> 6][q] = v;
-    }
-    __syncthreads();
-    if (t == 0) {
-        const int slot[5] = {W_INTERNAL, W_BODY, W_DIRICHLET, W_NEUMANN, W_DAMPING};
-        double* acc = B.blk_work + 14 * (size_t)blk;
-        for (int q = 0; q < 5; q++) {
-            double v = 0.;
-            for (int wv = 0; wv < BLOCK_THREADS / 64; wv++) v += red[wv][q];
-            // Neumaier accumulation: acc[2s] sum, acc[2s+1] compensation
-            int s = slot[q];
-            double sum = acc[2 * s], x = v, nt = sum + x;
-            acc[2 * s + 1] += fabs(sum) >= fabs(x) ? (sum - nt) + x : (x - nt) + sum;
-            acc[2 * s] = nt;
-        }
-    }
-    for (int i = t; i < nbn; i += BLOCK_THREADS) bn_lengths(m, bns[i]);
-    __syncthreads();       // the block's new parameters and old-position lengths are in place
-    for (int i = t; i < nbn; i += BLOCK_THREADS) kinematics_node(m, m.bn_node[bns[i]]);
 }
 
 __global__ void __launch_bounds__(BLOCK_THREADS) k_update_finish(DevModel m, BlockArrays B, double t_next) {

@@ -13,6 +13,7 @@
 
 #include "../../include/melody2d.h"
 #include "k_block.hpp"
+#include "k_fused.hpp"
 #include "k_step.hpp"
 #include "mel_device.hpp"
 #include "model_io.hpp"
@@ -53,10 +54,12 @@ inline int blocks(long long n, int b = BLOCK) { return (int)std::max<long long>(
 
 // kernels timed individually by the optional event instrumentation
 enum KernelId {
-    KT_CONTACTS = 0, KT_GATHER, KT_DAMPING, KT_BLOCK_UPDATE, KT_FINISH, KT_PROXIMITY, KT_COUNT
+    KT_CONTACTS = 0, KT_GATHER, KT_DAMPING, KT_BLOCK_UPDATE, KT_FINISH, KT_PROXIMITY, KT_STEP_FUSED,
+    KT_STEP_FORCES, KT_STEP_FINISH, KT_COUNT
 };
 static const char* kKernelNames[KT_COUNT] = {
-    "contacts_bc", "gather_rigid", "damping", "block_force_update", "update_finish", "proximity_update"};
+    "contacts_bc", "gather_rigid", "damping", "block_force_update", "update_finish", "proximity_update",
+    "step_fused", "step_forces", "step_finish"};
 
 struct mel_ctx {
     Model model;
@@ -115,6 +118,10 @@ struct mel_ctx {
     ContactArrays bufA{}, bufB{};
     BlockArrays blocks_dev{};
     size_t block_lds = 0;
+    // step orchestration: 0 launch sequence on two streams, 1 one launch,
+    // 2 two launches (1 and 2 need every body in one node block)
+    int step_mode = 0;
+    size_t finish_lds = 0;
     int* d_kin_all = nullptr;     // every deformable node (kinematics on request)
     int n_kin_all = 0;
     bool outputs_stale = false;   // interior kinematics / force fields lag the last steps
@@ -271,6 +278,7 @@ void check_flags(mel_ctx& c) {
         if (f[0] & 2) what += " neighbour list";
         if (f[0] & 4) what += " history index";
         if (f[0] & 8) what += " contact elements";
+        if (f[0] & 16) what += " cross-workgroup wait time limit (fused step)";
         throw CapacityError{"device capacity exceeded:" + what};
     }
 }
@@ -292,7 +300,8 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
                   const std::vector<double>& g_phix, const std::vector<double>& g_phiy,
                   const std::vector<double>& g_c0, const std::vector<int>& t_ptr, const std::vector<int>& t_gp,
                   const std::vector<int>& bn_node, const std::vector<int>& node_off, const std::vector<int>& gp_off,
-                  const std::vector<int>& perreg) {
+                  const std::vector<int>& perreg, const std::vector<int>& d_ptr, const std::vector<int>& d_idx,
+                  const std::vector<int>& bt_ptr, const std::vector<int>& bt_bn) {
     Model& m = c.model;
     const int ECAP = CHUNK_ENTRIES, CHCAP = BLOCK_THREADS;   // slots and Gauss points per chunk
     int N = node_off.back();
@@ -537,7 +546,42 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     for (size_t b = 0; b < m.bodies.size(); b++)
         if (!m.bodies[b].rigid)
             for (int i = 0; i < m.bodies[b].n_nodes; i++) kin_all.push_back(node_off[b] + i);
+    // the one-launch step's finish (node_block_finish_lds): owned nodes
+    // contiguous, spreading entries of owned nodes name the block's border
+    // nodes, kinematics rows of its border nodes name owned nodes, and the
+    // staged point forces + parameters fit the contribution buffer
+    std::vector<uint16_t> bt_lbn(std::max<size_t>(bt_bn.size(), 1), 0);
+    bool lds_ok = split_bn.empty() && split_kin.empty();
+    {
+        std::vector<int> bn_local(bn_node.size(), -1);
+        for (size_t k = 0; k < blk_body.size() && lds_ok; k++) {
+            const int* own = own_nodes.data() + blk_own_off[k];
+            const int nown = blk_nown[k], n0 = own[0];
+            const int* bl = bn_list.data() + blk_bn_off[k];
+            const int nbn = blk_bn_n[k];
+            if (6 * (size_t)(nbn + nown) * sizeof(double) > sizeof(double2) * ((size_t)CHUNK_ENTRIES + 1) || nbn > 65535) {
+                lds_ok = false;
+                break;
+            }
+            for (int o = 0; o < nown; o++)
+                if (own[o] != n0 + o) lds_ok = false;
+            for (int i = 0; i < nbn; i++) bn_local[bl[i]] = i;
+            for (int o = 0; o < nown && lds_ok; o++)
+                for (int e = bt_ptr[n0 + o]; e < bt_ptr[n0 + o + 1]; e++) {
+                    if (bn_local[bt_bn[e]] < 0) { lds_ok = false; break; }
+                    bt_lbn[e] = (uint16_t)bn_local[bt_bn[e]];
+                }
+            for (int i = 0; i < nbn && lds_ok; i++) {
+                const int node = bn_node[bl[i]];
+                for (int q = d_ptr[node]; q < d_ptr[node + 1]; q++)
+                    if (d_idx[q] < n0 || d_idx[q] >= n0 + nown) { lds_ok = false; break; }
+            }
+            for (int i = 0; i < nbn; i++) bn_local[bl[i]] = -1;
+        }
+    }
     BlockArrays& B = c.blocks_dev;
+    B.bt_lbn = c.upload(bt_lbn);
+    B.lds_finish_ok = lds_ok ? 1 : 0;
     B.n_blocks = (int)blk_body.size();
     B.max_touch = max_touch;
     B.max_own = max_own;
@@ -572,9 +616,27 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     c.block_lds = block_lds_size(B, c.precision == MEL_FP32_MIXED);
     if (c.block_lds + 16 * ((size_t)CHUNK_ENTRIES + 1) + 2048 > 160 * 1024)
         throw CapacityError{"node block needs more LDS than a workgroup has"};
-    HIP_OK(hipFuncSetAttribute(c.precision == MEL_FP32_MIXED ? (const void*)k_block_force_update32
-                                                             : (const void*)k_block_force_update,
+    const bool f32 = c.precision == MEL_FP32_MIXED;
+    HIP_OK(hipFuncSetAttribute(f32 ? (const void*)k_block_force_update32 : (const void*)k_block_force_update,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.block_lds));
+    // one launch per step unless a body is split over several node blocks
+    // (its blocks read each other's displacement parameters)
+    // MELODY2D_FUSED_STEP: 0 launch sequence, 1 one launch, 2 two launches
+    const char* fe = getenv("MELODY2D_FUSED_STEP");
+    const int want = fe ? std::atoi(fe) : 2;
+    c.step_mode = B.lds_finish_ok && (want == 1 || want == 2) ? want : 0;
+    if (c.step_mode == 1)
+        HIP_OK(hipFuncSetAttribute(f32 ? (const void*)k_step_fused32 : (const void*)k_step_fused,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.block_lds));
+    if (c.step_mode == 2) {
+        HIP_OK(hipFuncSetAttribute(f32 ? (const void*)k_step_forces32 : (const void*)k_step_forces,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.block_lds));
+        int max_bn = 0;
+        for (int v : blk_bn_n) max_bn = std::max(max_bn, v);
+        c.finish_lds = step_finish_lds(max_bn, max_own);
+        HIP_OK(hipFuncSetAttribute((const void*)k_step_finish, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)c.finish_lds));
+    }
 }
 
 // interior kinematics and force fields for output, after steps have run
@@ -637,6 +699,29 @@ void euler_step(mel_ctx& c) {
     const BlockArrays& B = c.blocks_dev;
     hipStream_t s = c.stream, a = c.aux;
     int nc = c.n_contacts;
+    const double t_next = c.time + d.dt;
+    if (c.step_mode != 0) {
+        // Update_damping_forces reads the velocity parameters of the start of
+        // the step, so it runs before the step kernels update them
+        if (d.n_matrix > 0) TIMED(c, KT_DAMPING, s, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d)));
+        FusedStep S{nc, (nc + B.n_bc_bn + BLOCK_THREADS - 1) / BLOCK_THREADS, t_next};
+        const bool f32 = c.precision == MEL_FP32_MIXED;
+        if (c.step_mode == 1) {
+            const int grid = fused_step_grid(d, B, S);
+            TIMED(c, KT_STEP_FUSED, s,
+                  (f32 ? k_step_fused32<<<grid, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)
+                       : k_step_fused<<<grid, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)));
+        } else {
+            const int ga = S.n_cwg + B.n_blocks, gb = d.n_rigid + B.n_blocks + 1;
+            TIMED(c, KT_STEP_FORCES, s,
+                  (f32 ? k_step_forces32<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)
+                       : k_step_forces<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)));
+            TIMED(c, KT_STEP_FINISH, s, (k_step_finish<<<gb, BLOCK_THREADS, c.finish_lds, s>>>(d, B, S)));
+        }
+        c.time = t_next;
+        c.outputs_stale = true;
+        return;
+    }
     // the contact pipeline runs on the second stream, concurrently with the
     // internal forces, which do not depend on it; its launches are a few
     // workgroups, which fit beside the internal-force blocks
@@ -650,7 +735,7 @@ void euler_step(mel_ctx& c) {
     if (d.n_matrix > 0) TIMED(c, KT_DAMPING, a, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, a>>>(d)));
     // rigid resultants, Newton, Euler | sends to the border nodes of split bodies
     if (d.n_rigid + B.n_split_bn > 0)
-        TIMED(c, KT_GATHER, a, (k_step_gather<<<d.n_rigid + blocks(B.n_split_bn), BLOCK, 0, a>>>(d, B.split_bn, B.n_split_bn)));
+        TIMED(c, KT_GATHER, a, (k_step_gather<<<d.n_rigid + blocks(B.n_split_bn, BLOCK_THREADS), BLOCK_THREADS, 0, a>>>(d, B.split_bn, B.n_split_bn)));
     HIP_OK(hipEventRecord(c.ev_join, a));
     // Update_internal_forces
     if (B.n_blocks > 0)
@@ -663,7 +748,6 @@ void euler_step(mel_ctx& c) {
     // Update_kinematics of border nodes (interior ones are rebuilt on request)
     // | rigid nodes | Update_bc(t + dt) | time = t + dt (the host's copy of the
     // time is the one the device takes)
-    const double t_next = c.time + d.dt;
     const int extra = std::max(1, (d.n_rigid_nodes + d.n_borders + BLOCK_THREADS - 1) / BLOCK_THREADS);
     TIMED(c, KT_FINISH, s, (k_update_finish<<<B.n_blocks + extra, BLOCK_THREADS, 0, s>>>(d, B, t_next)));
     if (B.n_split_kin > 0)
@@ -998,7 +1082,8 @@ void build(mel_ctx& c) {
                          &d.fky, &d.fx, &d.fy};
     for (double** p : forces) *p = c.alloc<double>(N);
     c.d.node_body_host = node_body.data();
-    build_blocks(c, g_ptr, g_idx, g_phix, g_phiy, g_c0, t_ptr, t_gp, bn_node, node_off, gp_off, perreg);
+    build_blocks(c, g_ptr, g_idx, g_phix, g_phiy, g_c0, t_ptr, t_gp, bn_node, node_off, gp_off, perreg, d_ptr, d_idx,
+                 bt_ptr, bt_bn);
     {
         // deformable border nodes with a slot on a border that carries a BC
         std::vector<int> bc_bn;
@@ -1056,6 +1141,7 @@ void build(mel_ctx& c) {
     d.bk_count = c.alloc<int>(NBN);
     d.bk_cursor = c.alloc<int>(NBN);
     d.bk_rec = c.alloc<int>((size_t)NBN * BUCKET_CAP);
+    d.bk_val = c.alloc<double2>((size_t)NBN * BUCKET_CAP);
     // contacts
     d.c_cap = 4 * NS + 1024;
     d.c_count = c.alloc<int>(1);
@@ -1096,6 +1182,7 @@ void build(mel_ctx& c) {
     d.slot_old_list = c.alloc<int>((size_t)NS * 16);
     d.slot_old_cursor = c.alloc<int>(NS);
     d.status_flags = c.alloc<int>(4);
+    d.step_sync = c.alloc<unsigned>(4);
     d.time = c.alloc<double>(1);
     HIP_OK(hipMemcpyAsync(d.time, &m.time, sizeof(double), hipMemcpyHostToDevice, c.stream));
     c.d_rigid_ids = c.upload(c.h_rigid_ids);

@@ -106,6 +106,7 @@ struct DevModel {
 
     // contact point-force bucketing (per border node)
     int *bk_count, *bk_off, *bk_cursor, *bk_rec;
+    double2 *bk_val;                         // force of each bucket record
     int *rec_bn;                             // 3 per element: slave, master 1, master 2 (-1 = none)
     double *rec_fx, *rec_fy;
     double *rig_rec;                         // 6 per element: slave rigid (fx,fy,m), master rigid (fx,fy,m)
@@ -132,6 +133,8 @@ struct DevModel {
     int *law_first;                          // first law matching a material pair by name (Initialize_CZM)
     int *status_flags;                       // [0] capacity overflow bits, [1] non-finite state seen
     double *time;                            // device copy of the simulation time
+    unsigned *step_sync;                     // fused step: [0] contact workgroups done, [1] rigid
+                                             // workgroups done, [2] node blocks past their wait
 };
 
 // --------------------------------------------------------------------------
