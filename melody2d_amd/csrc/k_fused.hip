@@ -1,36 +1,37 @@
-// The whole explicit step (Euler_step, Code/Solver.h:10-88) in one launch,
-// for models whose deformable bodies each fit one node block (the build
-// checks the preconditions of node_block_finish_lds).
+// The explicit step (Euler_step, Code/Solver.h:10-88) in two launches, for
+// models whose deformable bodies each fit one node block (the build checks
+// the preconditions of node_block_finish_lds).
 //
-// Workgroups by index range, all of BLOCK_THREADS threads:
-//   [0, n_cwg)                 contact elements (Update_contacts +
-//                              Update_contact_forces, records into buckets)
-//                              and BC point forces of deformable border
-//                              nodes (Update_bc_forces); then signal [0]
-//   next n_rigid               one rigid body each: wait [0]; resultants,
-//                              Sum_up_forces, Apply_Newton, Apply_Euler; signal
-//                              [1]; move the body's nodes
-//   next n_blocks              one node block each: internal forces (no
-//                              dependency on the contacts, so they overlap
-//                              them); wait [0]; bucket sums, update + works,
-//                              lengths, border-node kinematics, with the
-//                              block's point forces and new parameters in
-//                              the LDS of the contribution slots
-//   last                       wait [0] and [1]; Update_bc(t + dt), time =
-//                              t + dt; wait until every node block is past its
-//                              wait, then clear the counters for the next
-//                              launch
+// Launch A, workgroups by index range, all of BLOCK_THREADS threads:
+//   [0, n_cwg)         contact elements (Update_contacts +
+//                      Update_contact_forces, point-force records into the
+//                      border nodes' buckets) and BC point forces of
+//                      deformable border nodes (Update_bc_forces); signal [0]
+//   next n_rigid       one rigid body each: wait [0]; resultants,
+//                      Sum_up_forces, Apply_Newton, Apply_Euler; signal [1];
+//                      move the body's nodes
+//   next n_blocks      one node block each: internal forces
+//                      (Update_internal_forces), which depend on nothing else
+//                      of the step
+//   last               wait [0] and [1] (the readers of the BC values and of
+//                      the time); Update_bc(t + dt), time = t + dt; clear the
+//                      counters for the next launch
+// Launch B, one workgroup per node block: bucket sums (the contact sends),
+// Sum_up_forces + Apply_Newton + Apply_Euler with works, Update_borders and
+// the border-node kinematics, with the block's point forces and new
+// parameters staged in LDS.
+//
 // A workgroup only ever waits for workgroups of lower index, which the
-// hardware dispatches first, so every wait ends.  A wait is also bounded in
-// time: one that runs out sets bit 16 of the status flags (reported to the
-// host as an error) instead of hanging.
-//
-// Counters live in device memory (DevModel::step_sync), start at zero and are
-// zero again when the launch ends.  Publication: every thread fences its own
-// writes at device scope, the workgroup meets at a barrier, then one thread
-// increments the counter.  A waiting workgroup polls with one thread, meets
-// at a barrier, then every thread fences (acquire, device scope) before it
-// reads what the producers wrote.
+// hardware dispatches first, so every wait ends; the internal-force blocks
+// never wait.  A wait is also bounded in time: one that runs out sets bit 16
+// of the status flags (reported to the host as an error) instead of hanging.
+// The counters (DevModel::step_sync) start at zero and are zero again when
+// launch A ends: the last workgroup clears them once the rigid workgroups,
+// the only other waiters, have signalled.  Publication: every thread fences
+// its writes at device scope, the workgroup meets at a barrier, then one
+// thread increments the counter.  A waiting workgroup polls with one thread,
+// meets at a barrier, then every thread fences (acquire, device scope)
+// before it reads what the producers wrote.
 #include <hip/hip_runtime.h>
 #include "mel_device.hpp"
 #include "k_block.hpp"
@@ -66,8 +67,8 @@ __device__ inline void wg_wait(const DevModel& m, unsigned* ctr, unsigned target
 }
 
 template <class PR>
-__device__ __forceinline__ void step_fused(DevModel& m, const BlockArrays& B, const FusedStep& S,
-                                           typename PR::slot* s_cxy) {
+__device__ __forceinline__ void step_forces(DevModel& m, const BlockArrays& B, const StepLaunch& S,
+                                            typename PR::slot* s_cxy) {
     unsigned* sync = m.step_sync;
     const int t = threadIdx.x;
     int w = blockIdx.x;
@@ -88,110 +89,47 @@ __device__ __forceinline__ void step_fused(DevModel& m, const BlockArrays& B, co
     }
     w -= m.n_rigid;
     if (w < B.n_blocks) {
-        double fi[2][NODES_PER_THREAD] = {};
+        double fi[2][NODES_PER_THREAD];
         block_force_update<PR>(m, B, w, s_cxy, fi);
-        wg_wait(m, &sync[0], S.n_cwg);
-        if (t == 0) __hip_atomic_fetch_add(&sync[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        node_block_finish_lds(m, B, w, fi, reinterpret_cast<double*>(s_cxy));
         return;
     }
     // last workgroup
     wg_wait(m, &sync[0], S.n_cwg);
     wg_wait(m, &sync[1], m.n_rigid);
-    for (int br = t; br < m.n_borders; br += BLOCK_THREADS) bc_eval_at(m, br, S.t_next);
+    for (int k = t; k < m.n_bc_timed; k += BLOCK_THREADS) bc_eval_at(m, m.bc_timed[k], S.t_next);
     if (t == 0) {
         *m.time = S.t_next;
-        const long long t0 = wall_clock64();
-        while (__hip_atomic_load(&sync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (unsigned)B.n_blocks) {
-            if (wall_clock64() - t0 > WAIT_LIMIT_TICKS) {
-                atomicOr(&m.status_flags[0], 16);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
-        for (int k = 0; k < 3; k++) __hip_atomic_store(&sync[k], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
-
-// Two-launch form of the same step.  Launch A: contact WGs | node blocks'
-// internal forces, no dependency between them.  Launch B: rigid WGs | node
-// block finish | last WG (waits for the rigid WGs only: they read the BC
-// values it replaces).
-template <class PR>
-__device__ __forceinline__ void step_forces(DevModel& m, const BlockArrays& B, const FusedStep& S,
-                                            typename PR::slot* s_cxy) {
-    const int w = blockIdx.x;
-    if (w < S.n_cwg) {
-        contact_or_bc_item(m, w * BLOCK_THREADS + threadIdx.x, S.nc, B.bc_bn, B.n_bc_bn);
-        return;
-    }
-    double fi[2][NODES_PER_THREAD];
-    block_force_update<PR>(m, B, w - S.n_cwg, s_cxy, fi);
-}
-
-__device__ inline void step_finish(DevModel& m, const BlockArrays& B, const FusedStep& S, double* sc) {
-    unsigned* sync = m.step_sync;
-    const int t = threadIdx.x;
-    int w = blockIdx.x;
-    if (w < m.n_rigid) {
-        const RigidForces F = rigid_forces_block(m, w, sc);
-        const int b = m.rigid_ids[w];
-        if (t == 0) rigid_step_body(m, b, F);
-        wg_signal(&sync[1]);
-        if (m.body_status[b] != 1) rigid_body_nodes(m, b);
-        return;
-    }
-    w -= m.n_rigid;
-    if (w < B.n_blocks) {
-        double fi[2][NODES_PER_THREAD];
-        const int n0 = B.own_nodes[B.blk_own_off[w]], nown = B.blk_nown[w];
-#pragma unroll
-        for (int q = 0; q < NODES_PER_THREAD; q++) {
-            const int o = t + q * BLOCK_THREADS;
-            fi[0][q] = o < nown ? m.fix[n0 + o] : 0.;
-            fi[1][q] = o < nown ? m.fiy[n0 + o] : 0.;
-        }
-        node_block_finish_lds(m, B, w, fi, sc);
-        return;
-    }
-    wg_wait(m, &sync[1], m.n_rigid);
-    for (int br = t; br < m.n_borders; br += BLOCK_THREADS) bc_eval_at(m, br, S.t_next);
-    if (t == 0) {
-        *m.time = S.t_next;
+        __hip_atomic_store(&sync[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&sync[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
 }  // namespace
 
-__global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2)))
-k_step_forces(DevModel m, BlockArrays B, FusedStep S) {
+__global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_step_forces(DevModel m, BlockArrays B, StepLaunch S) {
     __shared__ __align__(16) blockdev::Fp64::slot s_cxy[CHUNK_ENTRIES + 1];
     step_forces<blockdev::Fp64>(m, B, S, s_cxy);
 }
 
-__global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2)))
-k_step_forces32(DevModel m, BlockArrays B, FusedStep S) {
+__global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_step_forces32(DevModel m, BlockArrays B, StepLaunch S) {
     __shared__ __align__(16) blockdev::Fp32::slot s_cxy[CHUNK_ENTRIES + 1];
     step_forces<blockdev::Fp32>(m, B, S, s_cxy);
 }
 
-__global__ void __launch_bounds__(BLOCK_THREADS) k_step_finish(DevModel m, BlockArrays B, FusedStep S) {
+__global__ void __launch_bounds__(BLOCK_THREADS) k_step_finish(DevModel m, BlockArrays B) {
     extern __shared__ __align__(16) double s_fin[];
-    step_finish(m, B, S, s_fin);
-}
-
-__global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_step_fused(DevModel m, BlockArrays B, FusedStep S) {
-    __shared__ __align__(16) blockdev::Fp64::slot s_cxy[CHUNK_ENTRIES + 1];
-    step_fused<blockdev::Fp64>(m, B, S, s_cxy);
-}
-
-__global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_step_fused32(DevModel m, BlockArrays B, FusedStep S) {
-    // sized as the FP64 slots: the finish phase reuses it as scratch
-    __shared__ __align__(16) double2 s_buf[CHUNK_ENTRIES + 1];
-    step_fused<blockdev::Fp32>(m, B, S, reinterpret_cast<blockdev::Fp32::slot*>(s_buf));
+    const int blk = blockIdx.x, t = threadIdx.x;
+    double fi[2][NODES_PER_THREAD];
+    const int n0 = B.own_nodes[B.blk_own_off[blk]], nown = B.blk_nown[blk];
+#pragma unroll
+    for (int q = 0; q < NODES_PER_THREAD; q++) {
+        const int o = t + q * BLOCK_THREADS;
+        fi[0][q] = o < nown ? m.fix[n0 + o] : 0.;
+        fi[1][q] = o < nown ? m.fiy[n0 + o] : 0.;
+    }
+    node_block_finish_lds(m, B, blk, fi, s_fin);
 }
 
 }  // namespace mel

@@ -54,12 +54,12 @@ inline int blocks(long long n, int b = BLOCK) { return (int)std::max<long long>(
 
 // kernels timed individually by the optional event instrumentation
 enum KernelId {
-    KT_CONTACTS = 0, KT_GATHER, KT_DAMPING, KT_BLOCK_UPDATE, KT_FINISH, KT_PROXIMITY, KT_STEP_FUSED,
-    KT_STEP_FORCES, KT_STEP_FINISH, KT_COUNT
+    KT_CONTACTS = 0, KT_GATHER, KT_DAMPING, KT_BLOCK_UPDATE, KT_FINISH, KT_PROXIMITY, KT_STEP_FORCES,
+    KT_STEP_FINISH, KT_COUNT
 };
 static const char* kKernelNames[KT_COUNT] = {
     "contacts_bc", "gather_rigid", "damping", "block_force_update", "update_finish", "proximity_update",
-    "step_fused", "step_forces", "step_finish"};
+    "step_forces", "step_finish"};
 
 struct mel_ctx {
     Model model;
@@ -118,9 +118,9 @@ struct mel_ctx {
     ContactArrays bufA{}, bufB{};
     BlockArrays blocks_dev{};
     size_t block_lds = 0;
-    // step orchestration: 0 launch sequence on two streams, 1 one launch,
-    // 2 two launches (1 and 2 need every body in one node block)
-    int step_mode = 0;
+    // step in two launches (every deformable body in one node block), else
+    // the launch sequence on two streams
+    bool two_launch = false;
     size_t finish_lds = 0;
     int* d_kin_all = nullptr;     // every deformable node (kinematics on request)
     int n_kin_all = 0;
@@ -619,16 +619,12 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     const bool f32 = c.precision == MEL_FP32_MIXED;
     HIP_OK(hipFuncSetAttribute(f32 ? (const void*)k_block_force_update32 : (const void*)k_block_force_update,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.block_lds));
-    // one launch per step unless a body is split over several node blocks
-    // (its blocks read each other's displacement parameters)
-    // MELODY2D_FUSED_STEP: 0 launch sequence, 1 one launch, 2 two launches
+    // two launches per step unless a body is split over several node blocks
+    // (its blocks read each other's displacement parameters);
+    // MELODY2D_FUSED_STEP=0 asks for the launch sequence
     const char* fe = getenv("MELODY2D_FUSED_STEP");
-    const int want = fe ? std::atoi(fe) : 2;
-    c.step_mode = B.lds_finish_ok && (want == 1 || want == 2) ? want : 0;
-    if (c.step_mode == 1)
-        HIP_OK(hipFuncSetAttribute(f32 ? (const void*)k_step_fused32 : (const void*)k_step_fused,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.block_lds));
-    if (c.step_mode == 2) {
+    c.two_launch = B.lds_finish_ok && !(fe && fe[0] == '0');
+    if (c.two_launch) {
         HIP_OK(hipFuncSetAttribute(f32 ? (const void*)k_step_forces32 : (const void*)k_step_forces,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.block_lds));
         int max_bn = 0;
@@ -700,24 +696,16 @@ void euler_step(mel_ctx& c) {
     hipStream_t s = c.stream, a = c.aux;
     int nc = c.n_contacts;
     const double t_next = c.time + d.dt;
-    if (c.step_mode != 0) {
+    if (c.two_launch) {
         // Update_damping_forces reads the velocity parameters of the start of
         // the step, so it runs before the step kernels update them
         if (d.n_matrix > 0) TIMED(c, KT_DAMPING, s, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d)));
-        FusedStep S{nc, (nc + B.n_bc_bn + BLOCK_THREADS - 1) / BLOCK_THREADS, t_next};
-        const bool f32 = c.precision == MEL_FP32_MIXED;
-        if (c.step_mode == 1) {
-            const int grid = fused_step_grid(d, B, S);
-            TIMED(c, KT_STEP_FUSED, s,
-                  (f32 ? k_step_fused32<<<grid, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)
-                       : k_step_fused<<<grid, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)));
-        } else {
-            const int ga = S.n_cwg + B.n_blocks, gb = d.n_rigid + B.n_blocks + 1;
-            TIMED(c, KT_STEP_FORCES, s,
-                  (f32 ? k_step_forces32<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)
-                       : k_step_forces<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)));
-            TIMED(c, KT_STEP_FINISH, s, (k_step_finish<<<gb, BLOCK_THREADS, c.finish_lds, s>>>(d, B, S)));
-        }
+        StepLaunch S{nc, (nc + B.n_bc_bn + BLOCK_THREADS - 1) / BLOCK_THREADS, t_next};
+        const int ga = step_forces_grid(d, B, S);
+        TIMED(c, KT_STEP_FORCES, s,
+              (c.precision == MEL_FP32_MIXED ? k_step_forces32<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)
+                                             : k_step_forces<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)));
+        TIMED(c, KT_STEP_FINISH, s, (k_step_finish<<<B.n_blocks, BLOCK_THREADS, c.finish_lds, s>>>(d, B)));
         c.time = t_next;
         c.outputs_stale = true;
         return;
@@ -1115,6 +1103,16 @@ void build(mel_ctx& c) {
     d.br_periodicity = c.upload(br_per);
     d.br_xtype = c.upload(br_xt);
     d.br_ytype = c.upload(br_yt);
+    {
+        // Update_bc changes the values of types 2, 3 and 4 only (type 0 stays
+        // zero, type 1 drives the DOFs directly)
+        std::vector<int> timed;
+        auto moves = [](int t) { return t == 2 || t == 3 || t == 4; };
+        for (size_t br = 0; br < br_xt.size(); br++)
+            if (moves(br_xt[br]) || moves(br_yt[br])) timed.push_back((int)br);
+        d.n_bc_timed = (int)timed.size();
+        d.bc_timed = c.upload(timed);
+    }
     d.br_xparam = c.upload(br_xparam);
     d.br_yparam = c.upload(br_yparam);
     d.br_hist_ptr = c.upload(br_hist_ptr);

@@ -88,6 +88,7 @@ struct DevModel {
 
     // borders
     int *br_body, *br_slot0, *br_nslots, *br_before, *br_after, *br_periodicity, *br_xtype, *br_ytype;
+    int n_bc_timed, *bc_timed;               // borders whose BC value follows the time (type 2, 3 or 4)
     double *br_xparam, *br_yparam;           // stiffness, damping
     int *br_hist_ptr;                        // 2 per border + 1 (x then y)
     double *hist_t, *hist_v;
@@ -133,8 +134,8 @@ struct DevModel {
     int *law_first;                          // first law matching a material pair by name (Initialize_CZM)
     int *status_flags;                       // [0] capacity overflow bits, [1] non-finite state seen
     double *time;                            // device copy of the simulation time
-    unsigned *step_sync;                     // fused step: [0] contact workgroups done, [1] rigid
-                                             // workgroups done, [2] node blocks past their wait
+    unsigned *step_sync;                     // step launch A: [0] contact workgroups done, [1] rigid
+                                             // workgroups done
 };
 
 // --------------------------------------------------------------------------

@@ -90,35 +90,30 @@ def test_exact_sum_order_matches_oracle(name, golden_dir, tmp_path, monkeypatch)
 
 
 @pytest.mark.parametrize("name", VARIANTS)
-def test_step_orchestrations_bit_identical(name, golden_dir, tmp_path, monkeypatch):
-    """The three orchestrations of the step (MELODY2D_FUSED_STEP: 0 launch
-    sequence on two streams, 1 one launch with cross-workgroup counters, 2 two
-    launches) run the same device routines in the same orders: every record
-    is bit-identical after 1, 10 and 200 steps."""
+def test_two_launch_step_equals_launch_sequence(name, golden_dir, tmp_path, monkeypatch):
+    """The step in two launches (contacts, rigid bodies and BC update beside
+    the internal forces, then the node-block finish) and the launch sequence
+    (MELODY2D_FUSED_STEP=0) run the same device routines in the same
+    orders: every record is bit-identical after 1, 10 and 200 steps."""
     model, _ = golden_dir(name)
-    print("case orchestrations", name, flush=True)
+    print("case two-launch vs sequence", name, flush=True)
     refs, _ = run_oracle(model, [0], str(tmp_path / "oracle"))
     nn, nbr, sizes = layout_from_dump(refs[0])
-    ctxs = {}
-    for mode in ("0", "1", "2"):
-        monkeypatch.setenv("MELODY2D_FUSED_STEP", mode)
-        ctxs[mode] = native.Context(model)
-        ctxs[mode].set_kernel_timing(["step_fused", "step_forces"])
+    two = native.Context(model)
+    two.set_kernel_timing(["step_forces"])
+    monkeypatch.setenv("MELODY2D_FUSED_STEP", "0")
+    seq = native.Context(model)
+    seq.set_kernel_timing(["step_forces"])
     prev = 0
     for s in (1, 10, 200):
-        states = {}
-        for mode, c in ctxs.items():
-            c.euler_steps(s - prev)
-            states[mode] = device_state(c, nn, nbr, sizes)
+        two.euler_steps(s - prev)
+        seq.euler_steps(s - prev)
         prev = s
-        for mode in ("1", "2"):
-            a, b = states[mode], states["0"]
-            assert a.keys() == b.keys()
-            diff = [k for k in a if not np.array_equal(np.asarray(a[k]), np.asarray(b[k]))]
-            assert not diff, "mode %s, step %d: %s" % (mode, s, diff[:8])
-    kt = {mode: c.kernel_timing() for mode, c in ctxs.items()}
-    assert kt["0"]["step_fused"][1] == 0 and kt["0"]["step_forces"][1] == 0
-    assert kt["1"]["step_fused"][1] == 200
-    assert kt["2"]["step_forces"][1] == 200
-    for c in ctxs.values():
-        c.close()
+        a, b = device_state(two, nn, nbr, sizes), device_state(seq, nn, nbr, sizes)
+        assert a.keys() == b.keys()
+        diff = [k for k in a if not np.array_equal(np.asarray(a[k]), np.asarray(b[k]))]
+        assert not diff, "step %d: %s" % (s, diff[:8])
+    assert two.kernel_timing()["step_forces"][1] == 200
+    assert seq.kernel_timing()["step_forces"][1] == 0
+    two.close()
+    seq.close()
