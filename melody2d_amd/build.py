@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libmelody2d_hip.so")
 ORACLE = os.path.join(ROOT, "oracle")
 REFERENCE = "/root/reference"
 
-HIP_SOURCES = ["mel_context.hip", "k_step.hip", "k_block.hip", "k_fused.hip", "k_proximity.hip", "scan.hip"]
+HIP_SOURCES = ["mel_context.hip", "k_step.hip", "k_block.hip", "k_fused.hip", "k_proximity.hip"]
 CXX_SOURCES = ["model_io.cpp"]
 
 HIPCC_FLAGS = [

@@ -73,7 +73,9 @@ __device__ __forceinline__ void step_forces(DevModel& m, const BlockArrays& B, c
     const int t = threadIdx.x;
     int w = blockIdx.x;
     if (w < S.n_cwg) {
-        contact_or_bc_item(m, w * BLOCK_THREADS + t, S.nc, B.bc_bn, B.n_bc_bn);
+        const int nc = *m.c_count;
+        for (int i = w * BLOCK_THREADS + t; i < nc + B.n_bc_bn; i += S.n_cwg * BLOCK_THREADS)
+            contact_or_bc_item(m, i, nc, B.bc_bn, B.n_bc_bn);
         wg_signal(&sync[0]);
         return;
     }

@@ -6,8 +6,8 @@
 namespace mel {
 
 struct StepLaunch {
-    int nc;          // contact elements
-    int n_cwg;       // workgroups of contact elements and BC point forces
+    int n_cwg;       // workgroups of contact elements and BC point forces (they loop over the
+                     // element count on the device)
     double t_next;   // time at the end of the step
 };
 

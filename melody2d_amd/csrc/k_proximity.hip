@@ -9,96 +9,148 @@
 // looks up, per slave slot, the previous elements in their old order.
 #include "mel_device.hpp"
 #include "k_step.hpp"
-#include "scan.hpp"
 
 namespace mel {
 
 constexpr int OLD_CAP = 16;    // previous elements per slave slot
 
-// per chunk of SEG_CHUNK border slots: the box of the slot nodes inflated by
-// the detection distance (its part of the border AABB of
-// Code/Proximity.h:63-109) and the box of the chunk's segments, used to skip
-// whole chunks of master segments in the narrow phase
+// exclusive prefix of v over the workgroup (blockDim.x a multiple of 64, at
+// most 1024): wavefront scans, then the wavefront totals in order.
+// s_w: at least blockDim.x / 64 ints of LDS.  Every thread must call it.
+__device__ inline int wg_excl_scan(int v, int* s_w, int& total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_w[wv] = x;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+    for (int k = 0; k < nw; k++) {
+        int t = s_w[k];
+        if (k < wv) before += t;
+        total += t;
+    }
+    __syncthreads();
+    return before + x - v;
+}
+
+// out[0..n) = exclusive prefix of in[0..n), out[n] = total, by one workgroup
+__device__ inline void wg_scan_array(const int* in, int* out, int n, int* s_w) {
+    int base = 0;
+    for (int t0 = 0; t0 < n; t0 += blockDim.x) {
+        const int i = t0 + threadIdx.x;
+        const int v = i < n ? in[i] : 0;
+        int tot;
+        const int ex = wg_excl_scan(v, s_w, tot);
+        if (i < n) out[i] = base + ex;
+        base += tot;
+    }
+    if (threadIdx.x == 0) out[n] = base;
+}
+
+// per chunk of SEG_CHUNK border slots, one half-wavefront (lane j takes slot
+// and segment j of the chunk): the box of the slot nodes inflated by the
+// detection distance (its part of the border AABB of
+// Code/Proximity.h:63-109), the box of the chunk's segments, used to skip
+// whole chunks of master segments in the narrow phase, and the segments'
+// end-node records read by the narrow phase.  Minima and maxima do not
+// depend on the order they are taken in.
 __global__ void k_prox_chunk_boxes(DevModel m) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= m.n_seg_chunks) return;
-    int br = m.sc_border[c];
-    int b = m.br_body[br];
-    if (m.body_status[b] == 1) return;
-    int s0 = m.br_slot0[br], ns = m.br_nslots[br];
-    int first = (c - m.br_chunk0[br]) * SEG_CHUNK;
-    double d = m.body_detect[b];
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = g / SEG_CHUNK, j = g % SEG_CHUNK;
+    const bool live = c < m.n_seg_chunks;
+    const int br = live ? m.sc_border[c] : 0;
+    const int b = m.br_body[br];
+    const bool on = live && m.body_status[b] != 1;
+    const int s0 = m.br_slot0[br], ns = m.br_nslots[br];
+    const int k = (c - m.br_chunk0[br]) * SEG_CHUNK + j;
+    const double d = m.body_detect[b];
+    double nxmin = 1.e100, nxmax = -1.e100, nymin = 1.e100, nymax = -1.e100;
     double xmin = 1.e100, xmax = -1.e100, ymin = 1.e100, ymax = -1.e100;
-    for (int j = first; j < min(first + SEG_CHUNK, ns); j++) {
-        int nd = m.sl_node[s0 + j];
-        double x = m.xc[nd], y = m.yc[nd];
-        if (x - d < xmin) xmin = x - d;
-        if (x + d > xmax) xmax = x + d;
-        if (y - d < ymin) ymin = y - d;
-        if (y + d > ymax) ymax = y + d;
+    if (on && k < ns) {
+        const int nd = m.sl_node[s0 + k];
+        const double x = m.xc[nd], y = m.yc[nd];
+        nxmin = x - d; nxmax = x + d; nymin = y - d; nymax = y + d;
     }
-    m.sc_nxmin[c] = xmin; m.sc_nxmax[c] = xmax; m.sc_nymin[c] = ymin; m.sc_nymax[c] = ymax;
-    double P = m.period;
-    xmin = 1.e100; xmax = -1.e100; ymin = 1.e100; ymax = -1.e100;
-    for (int seg = first; seg < min(first + SEG_CHUNK, ns - 1); seg++) {
-        int s = s0 + seg;
-        int n1 = m.sl_n1[s], n2 = m.sl_n2[s];
-        double x1 = m.xc[n1] + P * m.sl_s1[s], x2 = m.xc[n2] + P * m.sl_s2[s];
-        double y1 = m.yc[n1], y2 = m.yc[n2];
-        xmin = fmin(xmin, fmin(x1, x2)); xmax = fmax(xmax, fmax(x1, x2));
-        ymin = fmin(ymin, fmin(y1, y2)); ymax = fmax(ymax, fmax(y1, y2));
+    if (on && k < ns - 1) {
+        const int s = s0 + k;
+        const int n1 = m.sl_n1[s], n2 = m.sl_n2[s], s1 = m.sl_s1[s], s2 = m.sl_s2[s];
+        const double xa = m.xc[n1], ya = m.yc[n1], xb = m.xc[n2], yb = m.yc[n2];
+        const double P = m.period;
+        const double x1 = xa + P * s1, x2 = xb + P * s2;
+        xmin = fmin(x1, x2); xmax = fmax(x1, x2); ymin = fmin(ya, yb); ymax = fmax(ya, yb);
+        m.seg_ns[s] = make_int4(n1, n2, s1, s2);
+        m.seg_xy[s] = make_double4(xa, ya, xb, yb);
     }
-    m.sc_xmin[c] = xmin; m.sc_xmax[c] = xmax; m.sc_ymin[c] = ymin; m.sc_ymax[c] = ymax;
+    for (int off = SEG_CHUNK / 2; off > 0; off >>= 1) {
+        nxmin = fmin(nxmin, __shfl_xor(nxmin, off, 64)); nxmax = fmax(nxmax, __shfl_xor(nxmax, off, 64));
+        nymin = fmin(nymin, __shfl_xor(nymin, off, 64)); nymax = fmax(nymax, __shfl_xor(nymax, off, 64));
+        xmin = fmin(xmin, __shfl_xor(xmin, off, 64)); xmax = fmax(xmax, __shfl_xor(xmax, off, 64));
+        ymin = fmin(ymin, __shfl_xor(ymin, off, 64)); ymax = fmax(ymax, __shfl_xor(ymax, off, 64));
+    }
+    if (on && j == 0) {
+        m.sc_nxmin[c] = nxmin; m.sc_nxmax[c] = nxmax; m.sc_nymin[c] = nymin; m.sc_nymax[c] = nymax;
+        m.sc_box[c] = make_double4(xmin, xmax, ymin, ymax);
+    }
 }
 
 // border AABBs inflated by the detection distance + periodic shift range
 // (Code/Proximity.h:63-109): min / max over the chunk boxes, which equals
-// the reference's running min / max over the border nodes
+// the reference's running min / max over the border nodes; one wavefront
+// per border
 __global__ void k_prox_aabb(DevModel m) {
-    int br = blockIdx.x * blockDim.x + threadIdx.x;
-    if (br >= m.n_borders) return;
-    int b = m.br_body[br];
-    if (m.body_status[b] == 1) return;
+    const int br = (blockIdx.x * blockDim.x + threadIdx.x) / 64, lane = threadIdx.x & 63;
+    const bool on = br < m.n_borders && m.body_status[m.br_body[br < m.n_borders ? br : 0]] != 1;
     double xmin = 1.e100, xmax = -1.e100, ymin = 1.e100, ymax = -1.e100;
-    for (int c = m.br_chunk0[br]; c < m.br_chunk0[br + 1]; c++) {
-        if (m.sc_nxmin[c] < xmin) xmin = m.sc_nxmin[c];
-        if (m.sc_nxmax[c] > xmax) xmax = m.sc_nxmax[c];
-        if (m.sc_nymin[c] < ymin) ymin = m.sc_nymin[c];
-        if (m.sc_nymax[c] > ymax) ymax = m.sc_nymax[c];
+    if (on)
+        for (int c = m.br_chunk0[br] + lane; c < m.br_chunk0[br + 1]; c += 64) {
+            xmin = fmin(xmin, m.sc_nxmin[c]); xmax = fmax(xmax, m.sc_nxmax[c]);
+            ymin = fmin(ymin, m.sc_nymin[c]); ymax = fmax(ymax, m.sc_nymax[c]);
+        }
+    for (int off = 32; off > 0; off >>= 1) {
+        xmin = fmin(xmin, __shfl_xor(xmin, off, 64)); xmax = fmax(xmax, __shfl_xor(xmax, off, 64));
+        ymin = fmin(ymin, __shfl_xor(ymin, off, 64)); ymax = fmax(ymax, __shfl_xor(ymax, off, 64));
     }
+    if (!on || lane != 0) return;
     double inv = 1. / m.period;
     m.pb_xmin[br] = xmin; m.pb_xmax[br] = xmax; m.pb_ymin[br] = ymin; m.pb_ymax[br] = ymax;
     m.pb_shmin[br] = (int)floor((xmin - m.xmin_period) * inv);
     m.pb_shmax[br] = (int)floor((xmax - m.xmin_period) * inv);
 }
 
-// neighbour lists (Code/Proximity.h:112-169).  One 64-wide block per body1;
-// lanes take consecutive body2 so the order of insertion is the reference's
-// (border1, body2, border2, shift) order with first-occurrence de-duplication.
-__global__ void __launch_bounds__(64) k_prox_neighbours(DevModel m) {
-    int b1 = blockIdx.x;
-    int lane = threadIdx.x;
+// neighbour lists (Code/Proximity.h:112-169).  One workgroup of NB_THREADS
+// per body1; lanes take consecutive body2 so the order of insertion is the
+// reference's (border1, body2, border2, shift) order with first-occurrence
+// de-duplication.
+constexpr int NB_THREADS = 1024;
+__global__ void __launch_bounds__(NB_THREADS) k_prox_neighbours(DevModel m) {
+    const int b1 = blockIdx.x;
+    const int lane = threadIdx.x;
     __shared__ int count_sh;
-    __shared__ int pref[64];
+    __shared__ int s_w[NB_THREADS / 64];
     if (lane == 0) count_sh = 0;
     __syncthreads();
     int* list = m.nb_list + (size_t)b1 * m.nb_cap * 3;
     if (m.body_status[b1] != 1) {
-        double P = m.period;
+        const double P = m.period;
         for (int r1 = 0; r1 < m.body_nborders[b1]; r1++) {
-            int g1 = m.border_off[b1] + r1;
-            double xmin1 = m.pb_xmin[g1], xmax1 = m.pb_xmax[g1], ymin1 = m.pb_ymin[g1], ymax1 = m.pb_ymax[g1];
-            int shmin1 = m.pb_shmin[g1], shmax1 = m.pb_shmax[g1];
-            int before = count_sh;     // entries from earlier border1 iterations
-            for (int base = 0; base < m.n_bodies; base += 64) {
-                int b2 = base + lane;
-                int mine = 0;
-                // pass 1: count this lane's new neighbours
+            const int g1 = m.border_off[b1] + r1;
+            const double xmin1 = m.pb_xmin[g1], xmax1 = m.pb_xmax[g1], ymin1 = m.pb_ymin[g1], ymax1 = m.pb_ymax[g1];
+            const int shmin1 = m.pb_shmin[g1], shmax1 = m.pb_shmax[g1];
+            const int before = count_sh;     // entries from earlier border1 iterations
+            for (int base = 0; base < m.n_bodies; base += NB_THREADS) {
+                const int b2 = base + lane;
+                const int start = count_sh;
+                int ex = 0;
+                // pass 0 counts this lane's new neighbours, pass 1 writes them
                 for (int pass = 0; pass < 2; pass++) {
                     int w = 0;
                     if (b2 < m.n_bodies && b2 != b1 && m.body_status[b2] != 1) {
                         for (int r2 = 0; r2 < m.body_nborders[b2]; r2++) {
-                            int g2 = m.border_off[b2] + r2;
+                            const int g2 = m.border_off[b2] + r2;
                             if ((m.pb_ymax[g2] < ymin1) || (ymax1 < m.pb_ymin[g2])) continue;
                             for (int sh = shmin1 - m.pb_shmax[g2]; sh <= shmax1 - m.pb_shmin[g2]; sh++) {
                                 if ((m.pb_xmax[g2] + sh * P > xmin1) && (m.pb_xmin[g2] + sh * P < xmax1)) {
@@ -107,7 +159,7 @@ __global__ void __launch_bounds__(64) k_prox_neighbours(DevModel m) {
                                         if (list[3 * k] == b2 && list[3 * k + 1] == r2 && list[3 * k + 2] == sh) { exists = true; break; }
                                     if (exists) continue;
                                     if (pass == 1) {
-                                        int pos = count_sh + pref[lane] + w;
+                                        const int pos = start + ex + w;
                                         if (pos < m.nb_cap) {
                                             list[3 * pos] = b2; list[3 * pos + 1] = r2; list[3 * pos + 2] = sh;
                                         }
@@ -118,23 +170,12 @@ __global__ void __launch_bounds__(64) k_prox_neighbours(DevModel m) {
                         }
                     }
                     if (pass == 0) {
-                        mine = w;
-                        pref[lane] = mine;
-                        __syncthreads();
-                        // exclusive scan over the 64 lanes
-                        for (int off = 1; off < 64; off <<= 1) {
-                            int v = lane >= off ? pref[lane - off] : 0;
-                            __syncthreads();
-                            pref[lane] += v;
-                            __syncthreads();
-                        }
-                        pref[lane] -= mine;
-                        __syncthreads();
+                        int tot;
+                        ex = wg_excl_scan(w, s_w, tot);
+                        if (lane == 0) count_sh = start + tot;
                     }
                 }
-                __syncthreads();
-                if (lane == 63) count_sh += pref[63] + mine;
-                __syncthreads();
+                __syncthreads();   // this body2 range's entries and count are in place
             }
         }
     }
@@ -153,39 +194,40 @@ __device__ inline bool self_enabled(const DevModel& m, int b) {
     return m.self_contact && !m.body_rigid_flag[b] && m.body_periodicity[b] != 2;
 }
 
-// rows per body (non-self + self)
+// rows of body b: non-self rows (borderS, neighbour k, nodeS), then self
+// rows (borderS, nodeS, borderM)
+__device__ inline int body_rows(const DevModel& m, int b) {
+    if (m.body_status[b] == 1 || m.body_nborders[b] == 0) return 0;
+    const int ns = body_slots(m, b);
+    int r = m.nb_count[b] * ns;
+    if (self_enabled(m, b)) r += ns * m.body_nborders[b];
+    return r;
+}
+
+// row chunks per body
 __global__ void k_prox_rows(DevModel m) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= m.n_bodies) return;
-    int r = 0;
-    if (m.body_status[b] != 1 && m.body_nborders[b] > 0) {
-        int ns = body_slots(m, b);
-        r = m.nb_count[b] * ns;
-        if (self_enabled(m, b)) r += ns * m.body_nborders[b];
-    }
-    m.body_rows[b] = r;
+    m.body_chunks[b] = (body_rows(m, b) + ROW_CHUNK - 1) / ROW_CHUNK;
+}
+
+// first row chunk of every body (one workgroup); clears the hit list
+__global__ void __launch_bounds__(1024) k_prox_chunk_offsets(DevModel m) {
+    __shared__ int s_w[16];
+    wg_scan_array(m.body_chunks, m.chunk_off, m.n_bodies, s_w);
+    if (threadIdx.x == 0) *m.hit_n = 0;
 }
 
 struct RowInfo {
     int body, self, borderS, k, jS, borderM;
 };
 
-__device__ inline bool decode_row(const DevModel& m, int row, RowInfo& ri) {
-    int total = m.body_row_off[m.n_bodies];
-    if (row >= total) return false;
-    // binary search for the body
-    int lo = 0, hi = m.n_bodies;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (m.body_row_off[mid] <= row) lo = mid;
-        else hi = mid;
-    }
-    int b = lo;
-    int local = row - m.body_row_off[b];
-    int nb = m.nb_count[b];
-    int g0 = m.border_off[b];
-    int s0 = m.br_slot0[g0];
-    int ns = body_slots(m, b);
+// row `local` of body b
+__device__ inline bool decode_local_row(const DevModel& m, int b, int local, RowInfo& ri) {
+    const int nb = m.nb_count[b];
+    const int g0 = m.border_off[b];
+    const int s0 = m.br_slot0[g0];
+    const int ns = body_slots(m, b);
     ri.body = b;
     if (local < nb * ns) {
         ri.self = 0;
@@ -221,9 +263,23 @@ __device__ inline bool decode_row(const DevModel& m, int row, RowInfo& ri) {
     return false;
 }
 
-// scans one row; write == false counts, write == true emits elements at base
+// body of row chunk c: the last b with chunk_off[b] <= c
+__device__ inline int chunk_body(const DevModel& m, int c) {
+    int lo = 0, hi = m.n_bodies;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (m.chunk_off[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Scans one row from master segment seg_start on.  WRITE == false counts the
+// candidates (and reports the first one's segment in *first), WRITE == true
+// emits them as elements base, base + 1, ...
+constexpr int SEG_BATCH = 4;     // segment records loaded together
 template <bool WRITE>
-__device__ int scan_row(const DevModel& m, const RowInfo& ri, int base) {
+__device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_start, int* first) {
     int bS = ri.body;
     int gS = m.border_off[bS] + ri.borderS;
     int slotS = m.br_slot0[gS] + ri.jS;
@@ -257,16 +313,29 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base) {
     const double Xq = Xs - P * shM;
     const double dd = 1.000001 * detect + 1e-10 * (1. + fabs(Xs) + fabs(Ys) + fabs(P * shM));
     const int cM0 = m.br_chunk0[gM], cM1 = m.br_chunk0[gM + 1];
-    for (int ch = cM0; ch < cM1; ch++) {
-    if (!(Xq > m.sc_xmin[ch] - dd && Xq < m.sc_xmax[ch] + dd && Ys > m.sc_ymin[ch] - dd && Ys < m.sc_ymax[ch] + dd))
+    for (int ch = cM0 + seg_start / SEG_CHUNK; ch < cM1; ch++) {
+    const double4 bx = m.sc_box[ch];
+    if (!(Xq > bx.x - dd && Xq < bx.y + dd && Ys > bx.z - dd && Ys < bx.w + dd))
         continue;
-    const int sg0 = (ch - cM0) * SEG_CHUNK, sg1 = min(sg0 + SEG_CHUNK, nM - 1);
-    for (int seg = sg0; seg < sg1; seg++) {
-        int s = s0 + seg;
-        int n1 = m.sl_n1[s], n2 = m.sl_n2[s];
+    const int sg0 = max((ch - cM0) * SEG_CHUNK, seg_start), sg1 = min((ch - cM0) * SEG_CHUNK + SEG_CHUNK, nM - 1);
+    for (int g = sg0; g < sg1; g += SEG_BATCH) {
+    int4 sn[SEG_BATCH];
+    double4 sx[SEG_BATCH];
+#pragma unroll
+    for (int u = 0; u < SEG_BATCH; u++) {
+        const int s = s0 + min(g + u, sg1 - 1);
+        sn[u] = m.seg_ns[s];
+        sx[u] = m.seg_xy[s];
+    }
+#pragma unroll
+    for (int u = 0; u < SEG_BATCH; u++) {
+        const int seg = g + u;
+        if (seg >= sg1) break;
+        const int s = s0 + seg;
+        int n1 = sn[u].x, n2 = sn[u].y;
         if (ri.self && (n1 == nodeS || n2 == nodeS)) continue;
-        double X1 = m.xc[n1] + P * (shM + m.sl_s1[s]), Y1 = m.yc[n1];
-        double X2 = m.xc[n2] + P * (shM + m.sl_s2[s]), Y2 = m.yc[n2];
+        double X1 = sx[u].x + P * (shM + sn[u].z), Y1 = sx[u].y;
+        double X2 = sx[u].z + P * (shM + sn[u].w), Y2 = sx[u].w;
         if (!detect_segment(Xs, Ys, X1, Y1, X2, Y2, detect)) continue;
         int n0 = m.sl_n0[s], n3 = m.sl_n3[s];
         double X0 = m.xc[n0] + P * (shM + m.sl_s0[s]), Y0 = m.yc[n0];
@@ -284,6 +353,7 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base) {
             accept = accept && (p.gap > -cdist);
         }
         if (!accept) continue;
+        if (!WRITE && cnt == 0) *first = seg;
         if (WRITE) {
             int e = base + cnt;
             if (e < m.c_cap) {
@@ -292,7 +362,7 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base) {
                 C.bodyM[e] = bM; C.borderM[e] = rM; C.shiftM[e] = shM; C.segM[e] = seg;
                 C.nM0[e] = n0; C.nM1[e] = n1; C.nM2[e] = n2; C.nM3[e] = n3;
                 if (ri.self) { C.sM0[e] = 0; C.sM1[e] = 0; C.sM2[e] = 0; C.sM3[e] = 0; }
-                else { C.sM0[e] = m.sl_s0[s]; C.sM1[e] = m.sl_s1[s]; C.sM2[e] = m.sl_s2[s]; C.sM3[e] = m.sl_s3[s]; }
+                else { C.sM0[e] = m.sl_s0[s]; C.sM1[e] = sn[u].z; C.sM2[e] = sn[u].w; C.sM3[e] = m.sl_s3[s]; }
                 C.N0[e] = p.N0; C.N1[e] = p.N1; C.N2[e] = p.N2; C.N3[e] = p.N3;
                 C.gapn[e] = p.gap; C.gapt[e] = 0.; C.xsi[e] = p.xsi;
                 C.xn[e] = p.xn; C.yn[e] = p.yn; C.xt[e] = p.xt; C.yt[e] = p.yt;
@@ -305,35 +375,72 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base) {
         cnt++;
     }
     }
+    }
     return cnt;
 }
 
-__global__ void k_prox_count(DevModel m, int n_rows) {
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= n_rows) return;
-    RowInfo ri;
-    if (!decode_row(m, row, ri)) { m.row_count[row] = 0; return; }
-    m.row_count[row] = scan_row<false>(m, ri, 0);
+// Counting pass over row chunks of ROW_CHUNK rows (one row per thread, the
+// grid loops over the chunks): candidates per chunk, and one hit record per
+// row with candidates: (chunk, elements of the chunk's earlier rows, first
+// candidate's segment, row of the body)
+__global__ void __launch_bounds__(ROW_CHUNK) k_prox_count(DevModel m) {
+    __shared__ int s_w[ROW_CHUNK / 64];
+    __shared__ int s_body;
+    const int n_chunks = m.chunk_off[m.n_bodies];
+    for (int c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        if (threadIdx.x == 0) s_body = chunk_body(m, c);
+        __syncthreads();
+        const int b = s_body;
+        const int local = (c - m.chunk_off[b]) * ROW_CHUNK + threadIdx.x;
+        int cnt = 0, first = 0;
+        RowInfo ri;
+        if (local < body_rows(m, b) && decode_local_row(m, b, local, ri)) cnt = scan_row<false>(m, ri, 0, 0, &first);
+        int tot;
+        const int ex = wg_excl_scan(cnt, s_w, tot);     // its barriers also retire s_body
+        if (threadIdx.x == 0) m.chunk_cnt[c] = tot;
+        if (cnt > 0) {
+            const int h = atomicAdd(m.hit_n, 1);
+            if (h < m.c_cap) m.hit_rec[h] = make_int4(c, ex, first, local);
+            else atomicOr(&m.status_flags[0], 8);
+        }
+    }
 }
 
-__global__ void k_prox_write(DevModel m, int n_rows) {
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= n_rows) return;
-    RowInfo ri;
-    if (!decode_row(m, row, ri)) return;
-    if (m.row_count[row] == 0) return;
-    scan_row<true>(m, ri, m.row_off[row]);
+// first element of every row chunk (one workgroup) and the element count
+__global__ void __launch_bounds__(1024) k_prox_elem_offsets(DevModel m) {
+    __shared__ int s_w[16];
+    const int n_chunks = m.chunk_off[m.n_bodies];
+    wg_scan_array(m.chunk_cnt, m.chunk_elem_off, n_chunks, s_w);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int total = m.chunk_elem_off[n_chunks];
+        if (total > m.c_cap) { atomicOr(&m.status_flags[0], 8); total = m.c_cap; }
+        *m.c_count = total;
+    }
+}
+
+// Writing pass: the rows with candidates, each from its first candidate on
+__global__ void k_prox_write(DevModel m) {
+    const int n = min(*m.hit_n, m.c_cap);
+    for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < n; h += gridDim.x * blockDim.x) {
+        const int4 r = m.hit_rec[h];
+        const int b = chunk_body(m, r.x);
+        RowInfo ri;
+        if (!decode_local_row(m, b, r.w, ri)) continue;
+        scan_row<true>(m, ri, m.chunk_elem_off[r.x] + r.y, r.z, nullptr);
+    }
 }
 
 // index of the previous elements by slave slot, for history transfer
 __global__ void k_prox_old_index(DevModel m) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= *m.c_count_old) return;
+    const int n = *m.c_count_old;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     const ContactArrays& O = m.Cold;
     int slot = m.br_slot0[m.border_off[O.bodyS[e]] + O.borderS[e]] + O.bnodeS[e];
     int pos = atomicAdd(&m.slot_old_cursor[slot], 1);
     if (pos < OLD_CAP) m.slot_old_list[slot * OLD_CAP + pos] = e;
     else atomicOr(&m.status_flags[0], 4);
+    }
 }
 
 __device__ inline void copy_element(const ContactArrays& D, int e, const ContactArrays& S, int o) {
@@ -351,10 +458,10 @@ __device__ inline void copy_element(const ContactArrays& D, int e, const Contact
 // history transfer: first previous element of the same slave slot with the
 // same master body and border and |segment difference| <= 1
 __global__ void k_prox_history(DevModel m) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= *m.c_count) return;
+    const int ne = *m.c_count;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
     const ContactArrays& C = m.C;
-    if (C.fresh[e] != 1) return;          // self elements never inherit (Code/Proximity.h:410-425)
+    if (C.fresh[e] != 1) continue;        // self elements never inherit (Code/Proximity.h:410-425)
     int slot = m.br_slot0[m.border_off[C.bodyS[e]] + C.borderS[e]] + C.bnodeS[e];
     int n = min(m.slot_old_cursor[slot], OLD_CAP);
     const int* lst = m.slot_old_list + slot * OLD_CAP;
@@ -370,18 +477,18 @@ __global__ void k_prox_history(DevModel m) {
         copy_element(C, e, m.Cold, best);
         C.fresh[e] = 0;
     }
+    }
 }
 
 // fresh elements carry the relative velocities of the element created just
 // before them in the same pass (the reused new_contact record of
 // Code/Proximity.h:211 and :350), zero at the start of a pass
 __global__ void k_prox_chain(DevModel m) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    int n = *m.c_count;
-    if (e >= n) return;
+    const int n = *m.c_count;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     const ContactArrays& C = m.C;
     int f = C.fresh[e];
-    if (f == 0) return;
+    if (f == 0) continue;
     int b = C.bodyS[e];
     int phase = f == 2;
     double vn = 0., vt = 0.;
@@ -394,13 +501,7 @@ __global__ void k_prox_chain(DevModel m) {
     }
     C.vgapn[e] = vn;
     C.vgapt[e] = vt;
-}
-
-__global__ void k_prox_finish(DevModel m, int n_rows) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int total = n_rows > 0 ? m.row_off[n_rows - 1] + m.row_count[n_rows - 1] : 0;
-    if (total > m.c_cap) { atomicOr(&m.status_flags[0], 8); total = m.c_cap; }
-    *m.c_count = total;
+    }
 }
 
 // runs after the host swapped the element buffers: C is empty, Cold holds
@@ -414,20 +515,21 @@ __global__ void k_prox_clear_old(DevModel m) {
 // Initialize_CZM (Code/Proximity.h:481-541) after the elements have been
 // re-projected: damage 0 below gapinit, 1 otherwise, for CZMlinear pairs.
 __global__ void k_init_czm(DevModel m) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= *m.c_count) return;
-    const ContactArrays& C = m.C;
-    int ms = m.body_material[C.bodyS[e]], mm = m.body_material[C.bodyM[e]];
-    int li = m.law_first[ms * m.n_materials + mm];
-    if (li < 0 || m.law_type[li] != 6) return;
-    C.nbint[e] = 1;
-    C.int0[e] = C.gapn[e] < m.law_p[12 * li + 5] ? 0. : 1.;
+    const int n = *m.c_count;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const ContactArrays& C = m.C;
+        int ms = m.body_material[C.bodyS[e]], mm = m.body_material[C.bodyM[e]];
+        int li = m.law_first[ms * m.n_materials + mm];
+        if (li < 0 || m.law_type[li] != 6) continue;
+        C.nbint[e] = 1;
+        C.int0[e] = C.gapn[e] < m.law_p[12 * li + 5] ? 0. : 1.;
+    }
 }
 
 __global__ void k_project_all(DevModel m) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= *m.c_count) return;
-    project_element(m, m.C, e);
+    const int n = *m.c_count;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+        project_element(m, m.C, e);
 }
 
 }  // namespace mel

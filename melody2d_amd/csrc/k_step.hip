@@ -70,8 +70,10 @@ __global__ void k_rigid_nodes(DevModel m, int with_forces) {
 //                     lengths, border-node kinematics | rigid nodes |
 //                     Update_bc(t + dt) | time = t + dt
 // ---------------------------------------------------------------------------
-__global__ void k_step_contacts(DevModel m, int nc, const int* bc_bn, int n_bc) {
-    contact_or_bc_item(m, blockIdx.x * blockDim.x + threadIdx.x, nc, bc_bn, n_bc);
+__global__ void k_step_contacts(DevModel m, const int* bc_bn, int n_bc) {
+    const int nc = *m.c_count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc + n_bc; i += gridDim.x * blockDim.x)
+        contact_or_bc_item(m, i, nc, bc_bn, n_bc);
 }
 
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_gather(DevModel m, const int* split_bn, int n_split) {

@@ -7,12 +7,13 @@ namespace mel {
 constexpr int BUCKET_CAP = 32;     // point-force records per border node per step
 constexpr int BLOCK = 256;
 constexpr int SEG_CHUNK = 32;     // border slots per proximity culling chunk
+constexpr int ROW_CHUNK = 256;    // narrow-phase rows per counting workgroup
 
 __global__ void k_border_length(DevModel m);
 __global__ void k_bc_eval(DevModel m, int plus_dt);
 __global__ void k_damping(DevModel m);
 __global__ void k_rigid_nodes(DevModel m, int with_forces);
-__global__ void k_step_contacts(DevModel m, int nc, const int* bc_bn, int n_bc);
+__global__ void k_step_contacts(DevModel m, const int* bc_bn, int n_bc);
 __global__ void k_step_gather(DevModel m, const int* split_bn, int n_split);
 
 // proximity (k_proximity.hip)
@@ -20,13 +21,14 @@ __global__ void k_prox_chunk_boxes(DevModel m);
 __global__ void k_prox_aabb(DevModel m);
 __global__ void k_prox_neighbours(DevModel m);
 __global__ void k_prox_rows(DevModel m);
-__global__ void k_prox_count(DevModel m, int n_rows);
-__global__ void k_prox_write(DevModel m, int n_rows);
+__global__ void k_prox_chunk_offsets(DevModel m);
+__global__ void k_prox_count(DevModel m);
+__global__ void k_prox_elem_offsets(DevModel m);
+__global__ void k_prox_write(DevModel m);
 __global__ void k_prox_clear_old(DevModel m);
 __global__ void k_prox_old_index(DevModel m);
 __global__ void k_prox_history(DevModel m);
 __global__ void k_prox_chain(DevModel m);
-__global__ void k_prox_finish(DevModel m, int n_rows);
 __global__ void k_init_czm(DevModel m);
 __global__ void k_project_all(DevModel m);
 

@@ -17,7 +17,6 @@
 #include "k_step.hpp"
 #include "mel_device.hpp"
 #include "model_io.hpp"
-#include "scan.hpp"
 
 using namespace mel;
 
@@ -105,12 +104,17 @@ struct mel_ctx {
     std::vector<int> h_node_off, h_border_off, h_gp_off, h_rigid_ids;
     std::vector<int> h_br_slot0, h_br_nslots;
     long long nnz_g = 0, nnz_d = 0;
-    int n_contacts = 0;          // host mirror of *d.c_count after the last proximity update
+    int n_contacts = 0;          // *d.c_count, read back when an API call needs it exactly
+    // element count and status flags of the last proximity update, copied
+    // back asynchronously into pinned memory: the count sizes the contact
+    // workgroups of later steps (their kernels read the exact count), the
+    // flags are checked once the copy has landed
+    int* h_async = nullptr;      // [0] element count, [1..2] status flags
+    hipEvent_t ev_async = nullptr;
+    bool async_pending = false;
+    int nc_hint = 0;
     int row_cap = 0;
-    int* d_row_count = nullptr;
-    int* d_row_off = nullptr;
-    int* d_scan_tmp = nullptr;
-    int scan_tmp_cap = 0;
+    int max_row_chunks = 0;
     int* d_rigid_ids = nullptr;
     double time = 0.0;
     double next_contact_update = 0.0;
@@ -154,6 +158,8 @@ struct mel_ctx {
         for (auto& e : pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
         for (auto& e : pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_async) (void)hipEventDestroy(ev_async);
+        if (h_async) (void)hipHostFree(h_async);
         if (ev_join) (void)hipEventDestroy(ev_join);
         for (void* p : allocs) (void)hipFree(p);
         if (aux) (void)hipStreamDestroy(aux);
@@ -245,33 +251,8 @@ void compute_masses(Model& m, std::vector<double>& body_mass, std::vector<double
     }
 }
 
-void ensure_rows(mel_ctx& c, int n_rows) {
-    if (n_rows + 1 > c.row_cap) {
-        int cap = std::max(n_rows + 1, c.row_cap * 2);
-        if (c.d_row_count) { (void)hipFree(c.d_row_count); (void)hipFree(c.d_row_off); }
-        c.allocs.erase(std::remove(c.allocs.begin(), c.allocs.end(), (void*)c.d_row_count), c.allocs.end());
-        c.allocs.erase(std::remove(c.allocs.begin(), c.allocs.end(), (void*)c.d_row_off), c.allocs.end());
-        c.d_row_count = c.alloc<int>(cap);
-        c.d_row_off = c.alloc<int>(cap);
-        c.row_cap = cap;
-    }
-    int need = scan_tmp_ints(std::max(n_rows, c.d.n_bodies));
-    if (need > c.scan_tmp_cap) {
-        if (c.d_scan_tmp) {
-            (void)hipFree(c.d_scan_tmp);
-            c.allocs.erase(std::remove(c.allocs.begin(), c.allocs.end(), (void*)c.d_scan_tmp), c.allocs.end());
-        }
-        c.d_scan_tmp = c.alloc<int>(need * 2);
-        c.scan_tmp_cap = need * 2;
-    }
-    c.d.row_count = c.d_row_count;
-    c.d.row_off = c.d_row_off;
-}
 
-void check_flags(mel_ctx& c) {
-    int f[2];
-    HIP_OK(hipMemcpyAsync(f, c.d.status_flags, sizeof f, hipMemcpyDeviceToHost, c.stream));
-    HIP_OK(hipStreamSynchronize(c.stream));
+void raise_flags(const int* f) {
     if (f[0]) {
         std::string what;
         if (f[0] & 1) what += " point-force bucket";
@@ -281,6 +262,39 @@ void check_flags(mel_ctx& c) {
         if (f[0] & 16) what += " cross-workgroup wait time limit (fused step)";
         throw CapacityError{"device capacity exceeded:" + what};
     }
+}
+
+void check_flags(mel_ctx& c) {
+    int f[2];
+    HIP_OK(hipMemcpyAsync(f, c.d.status_flags, sizeof f, hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    raise_flags(f);
+}
+
+// takes in the asynchronous copy of the last proximity update's count and
+// flags once it has landed (wait: block until it has)
+void poll_async(mel_ctx& c, bool wait) {
+    if (!c.async_pending) return;
+    if (wait) HIP_OK(hipEventSynchronize(c.ev_async));
+    else if (hipEventQuery(c.ev_async) != hipSuccess) return;
+    c.async_pending = false;
+    c.nc_hint = c.h_async[0];
+    raise_flags(c.h_async + 1);
+}
+
+// exact element count (waits for the queued work)
+int contact_count(mel_ctx& c) {
+    HIP_OK(hipMemcpyAsync(&c.n_contacts, c.d.c_count, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    c.nc_hint = c.n_contacts;
+    return c.n_contacts;
+}
+
+// workgroups of BLOCK_THREADS that the contact kernels start with: the
+// last known count with room to grow (the kernels loop over the exact one)
+int contact_workgroups(const mel_ctx& c) {
+    long long n = (long long)c.nc_hint + c.nc_hint / 4 + 64 + c.blocks_dev.n_bc_bn;
+    return (int)std::max<long long>(1, (n + BLOCK_THREADS - 1) / BLOCK_THREADS);
 }
 
 
@@ -645,46 +659,39 @@ void refresh_outputs(mel_ctx& c) {
 }
 
 // Update_proximity (Code/Proximity.h:48-473)
+// Update_proximity (Code/Proximity.h:48-473) without host round trips: the
+// kernels size their loops from counts on the device
 void update_proximity(mel_ctx& c) {
     DevModel& d = c.d;
     hipStream_t s = c.stream;
-    if (d.n_seg_chunks > 0) k_prox_chunk_boxes<<<blocks(d.n_seg_chunks), BLOCK, 0, s>>>(d);
-    k_prox_aabb<<<blocks(d.n_borders), BLOCK, 0, s>>>(d);
-    k_prox_neighbours<<<d.n_bodies, 64, 0, s>>>(d);
+    const int persistent = 4 * 256;     // workgroups of the grid-stride loops
+    if (d.n_seg_chunks > 0) k_prox_chunk_boxes<<<blocks((long long)d.n_seg_chunks * SEG_CHUNK), BLOCK, 0, s>>>(d);
+    k_prox_aabb<<<blocks((long long)d.n_borders * 64), BLOCK, 0, s>>>(d);
+    k_prox_neighbours<<<d.n_bodies, 1024, 0, s>>>(d);
     k_prox_rows<<<blocks(d.n_bodies), BLOCK, 0, s>>>(d);
-    ensure_rows(c, 0);
-    exclusive_scan_int(d.body_rows, d.body_row_off, d.n_bodies, c.d_scan_tmp, s);
-    int n_rows = 0;
-    HIP_OK(hipMemcpyAsync(&n_rows, d.body_row_off + d.n_bodies, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
-    ensure_rows(c, n_rows);
+    k_prox_chunk_offsets<<<1, 1024, 0, s>>>(d);
     // previous elements become the history source
     std::swap(c.bufA, c.bufB);
     d.C = c.bufA;
     d.Cold = c.bufB;
     k_prox_clear_old<<<blocks(d.n_slots), BLOCK, 0, s>>>(d);
-    int old = c.n_contacts;
-    if (old > 0) k_prox_old_index<<<blocks(old), BLOCK, 0, s>>>(d);
-    if (n_rows > 0) k_prox_count<<<blocks(n_rows), BLOCK, 0, s>>>(d, n_rows);
-    exclusive_scan_int(d.row_count, d.row_off, n_rows, c.d_scan_tmp, s);
-    k_prox_finish<<<1, 64, 0, s>>>(d, n_rows);
-    if (n_rows > 0) k_prox_write<<<blocks(n_rows), BLOCK, 0, s>>>(d, n_rows);
-    int n = 0;
-    HIP_OK(hipMemcpyAsync(&n, d.c_count, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
-    c.n_contacts = n;
-    if (n > 0) {
-        k_prox_history<<<blocks(n), BLOCK, 0, s>>>(d);
-        k_prox_chain<<<blocks(n), BLOCK, 0, s>>>(d);
-    }
-    check_flags(c);
+    k_prox_old_index<<<persistent, BLOCK, 0, s>>>(d);
+    k_prox_count<<<std::min(persistent, std::max(1, c.max_row_chunks)), ROW_CHUNK, 0, s>>>(d);
+    k_prox_elem_offsets<<<1, 1024, 0, s>>>(d);
+    k_prox_write<<<persistent, BLOCK, 0, s>>>(d);
+    k_prox_history<<<persistent, BLOCK, 0, s>>>(d);
+    k_prox_chain<<<persistent, BLOCK, 0, s>>>(d);
+    HIP_OK(hipMemcpyAsync(c.h_async, d.c_count, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(c.h_async + 1, d.status_flags, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipEventRecord(c.ev_async, s));
+    c.async_pending = true;
 }
 
 // Initialize_CZM (Code/Proximity.h:481-541); it clears the INITIALIZE_CZM
 // flag (Code/Proximity.h:486), so the second call of Code/main.cpp:124 is a no-op
 void initialize_czm(mel_ctx& c) {
     c.model.flags[F_INITIALIZE_CZM] = 0;
-    if (c.n_contacts == 0) return;
+    if (contact_count(c) == 0) return;
     k_project_all<<<blocks(c.n_contacts), BLOCK, 0, c.stream>>>(c.d);
     k_init_czm<<<blocks(c.n_contacts), BLOCK, 0, c.stream>>>(c.d);
 }
@@ -694,13 +701,14 @@ void euler_step(mel_ctx& c) {
     DevModel& d = c.d;
     const BlockArrays& B = c.blocks_dev;
     hipStream_t s = c.stream, a = c.aux;
-    int nc = c.n_contacts;
+    poll_async(c, false);
+    const int n_cwg = contact_workgroups(c);
     const double t_next = c.time + d.dt;
     if (c.two_launch) {
         // Update_damping_forces reads the velocity parameters of the start of
         // the step, so it runs before the step kernels update them
         if (d.n_matrix > 0) TIMED(c, KT_DAMPING, s, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d)));
-        StepLaunch S{nc, (nc + B.n_bc_bn + BLOCK_THREADS - 1) / BLOCK_THREADS, t_next};
+        StepLaunch S{n_cwg, t_next};
         const int ga = step_forces_grid(d, B, S);
         TIMED(c, KT_STEP_FORCES, s,
               (c.precision == MEL_FP32_MIXED ? k_step_forces32<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)
@@ -717,8 +725,7 @@ void euler_step(mel_ctx& c) {
     HIP_OK(hipStreamWaitEvent(a, c.ev_fork, 0));
     // Initialize_contact_forces, Update_contacts, Update_contact_forces
     // (records into buckets) | Update_bc_forces
-    if (nc + B.n_bc_bn > 0)
-        TIMED(c, KT_CONTACTS, a, (k_step_contacts<<<blocks(nc + B.n_bc_bn), BLOCK, 0, a>>>(d, nc, B.bc_bn, B.n_bc_bn)));
+    TIMED(c, KT_CONTACTS, a, (k_step_contacts<<<n_cwg, BLOCK_THREADS, 0, a>>>(d, B.bc_bn, B.n_bc_bn)));
     // Update_damping_forces
     if (d.n_matrix > 0) TIMED(c, KT_DAMPING, a, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, a>>>(d)));
     // rigid resultants, Newton, Euler | sends to the border nodes of split bodies
@@ -1169,14 +1176,31 @@ void build(mel_ctx& c) {
         size_t nc = (size_t)d.n_seg_chunks;
         d.sc_nxmin = c.alloc<double>(nc); d.sc_nxmax = c.alloc<double>(nc);
         d.sc_nymin = c.alloc<double>(nc); d.sc_nymax = c.alloc<double>(nc);
-        d.sc_xmin = c.alloc<double>(nc); d.sc_xmax = c.alloc<double>(nc);
-        d.sc_ymin = c.alloc<double>(nc); d.sc_ymax = c.alloc<double>(nc);
+        d.sc_box = c.alloc<double4>(nc);
     }
+    d.seg_ns = c.alloc<int4>(NS);
+    d.seg_xy = c.alloc<double4>(NS);
     d.nb_cap = 256;
     d.nb_count = c.alloc<int>(NB);
     d.nb_list = c.alloc<int>((size_t)NB * d.nb_cap * 3);
-    d.body_rows = c.alloc<int>(NB);
-    d.body_row_off = c.alloc<int>(NB + 1);
+    {
+        // narrow-phase row chunks: at most nb_cap neighbours per body
+        long long ch = 0;
+        for (int b = 0; b < NB; b++) {
+            long long ns = 0;
+            for (auto& br : m.bodies[b].borders) ns += (long long)br.nodes.size();
+            long long rows = (long long)d.nb_cap * ns + ns * (long long)m.bodies[b].borders.size();
+            ch += (rows + ROW_CHUNK - 1) / ROW_CHUNK;
+        }
+        if (ch > INT32_MAX / 2) throw CapacityError{"narrow-phase row chunks exceed the index range"};
+        c.max_row_chunks = (int)ch;
+    }
+    d.body_chunks = c.alloc<int>(NB);
+    d.chunk_off = c.alloc<int>(NB + 1);
+    d.chunk_cnt = c.alloc<int>(c.max_row_chunks);
+    d.chunk_elem_off = c.alloc<int>((size_t)c.max_row_chunks + 1);
+    d.hit_rec = c.alloc<int4>(d.c_cap);
+    d.hit_n = c.alloc<int>(1);
     d.slot_old_list = c.alloc<int>((size_t)NS * 16);
     d.slot_old_cursor = c.alloc<int>(NS);
     d.status_flags = c.alloc<int>(4);
@@ -1235,6 +1259,8 @@ void build(mel_ctx& c) {
     update_proximity(c);
     if (m.flags[F_INITIALIZE_CZM]) initialize_czm(c);
     HIP_OK(hipStreamSynchronize(c.stream));
+    poll_async(c, true);
+    contact_count(c);
 }
 
 thread_local std::string g_load_err;
@@ -1257,6 +1283,8 @@ mel_status create(mel_ctx** out, int device, int precision, const std::function<
         HIP_OK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
         HIP_OK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         HIP_OK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&c->ev_async, hipEventDisableTiming));
+        HIP_OK(hipHostMalloc((void**)&c->h_async, 8 * sizeof(int), hipHostMallocDefault));
         build(*c);
     } catch (HipError& e) {
         g_load_err = e.msg;
@@ -1343,6 +1371,7 @@ mel_status mel_synchronize(mel_ctx* c) {
     return guarded(c, [&]() {
         HIP_OK(hipStreamSynchronize(c->aux));
         HIP_OK(hipStreamSynchronize(c->stream));
+        poll_async(*c, true);
         check_flags(*c);
         return MEL_OK;
     });
@@ -1377,7 +1406,7 @@ mel_status mel_set_node_field(mel_ctx* c, int body, int field, const double* in,
 mel_status mel_get_contacts(mel_ctx* c, int body, int32_t* ids, double* state, size_t cap, size_t* count) {
     if (!c || !count || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
     return guarded(c, [&]() {
-        int n = c->n_contacts;
+        int n = contact_count(*c);
         ContactArrays& A = c->bufA;
         std::vector<int> I[18];
         int* ip[18] = {A.bodyS, A.borderS, A.bnodeS, A.nodeS, A.bodyM, A.borderM, A.shiftM, A.segM, A.nM0, A.nM1,
@@ -1476,7 +1505,7 @@ mel_status mel_get_border_field(mel_ctx* c, int body, int border, int field, dou
         }
         // Update_contact_pressures (Code/Proximity.h:1014-1069): evaluated on
         // request from the current elements, in element order
-        int nc = c->n_contacts;
+        int nc = contact_count(*c);
         ContactArrays& A = c->bufA;
         std::vector<int> bS, brS, jS, bM, brM, seg;
         std::vector<double> fx, fy, len, N1, N2;
@@ -1501,6 +1530,8 @@ mel_status mel_get_border_field(mel_ctx* c, int body, int border, int field, dou
 
 mel_status mel_get_info(mel_ctx* c, mel_info_t* info) {
     if (!c || !info) return MEL_E_ARG;
+    mel_status st = guarded(c, [&]() { contact_count(*c); return MEL_OK; });
+    if (st != MEL_OK) return st;
     info->n_bodies = c->d.n_bodies;
     info->n_nodes = c->d.n_nodes;
     info->n_gauss = c->d.n_gauss;

@@ -124,11 +124,15 @@ struct DevModel {
     int n_seg_chunks;                        // chunks of SEG_CHUNK border slots
     int *br_chunk0, *sc_border;              // first chunk per border (n_borders + 1), border per chunk
     double *sc_nxmin, *sc_nxmax, *sc_nymin, *sc_nymax;   // slot-node box inflated by detection
-    double *sc_xmin, *sc_xmax, *sc_ymin, *sc_ymax;       // segment box (slot shifts applied)
+    double4 *sc_box;                         // segment box (slot shifts applied): xmin, xmax, ymin, ymax
+    int4 *seg_ns;                            // per slot s, segment s: end nodes n1, n2 and their shifts
+    double4 *seg_xy;                         // per slot s, segment s: x, y of n1, then of n2
     int nb_cap;                              // neighbour capacity per body
     int *nb_count, *nb_list;                 // per body: count, 3 ints per neighbour
-    int *row_count, *row_off;                // narrow-phase rows
-    int *body_rows, *body_row_off;           // rows per body (non-self + self)
+    int *body_chunks, *chunk_off;            // narrow-phase row chunks per body, first chunk per body
+    int *chunk_cnt, *chunk_elem_off;         // candidates per row chunk, first element per row chunk
+    int4 *hit_rec;                           // rows with candidates: chunk, offset, first segment, row
+    int *hit_n;
     int *slot_old_list, *slot_old_cursor;
     int *c_count_old;                        // element count of Cold
     int *law_first;                          // first law matching a material pair by name (Initialize_CZM)
