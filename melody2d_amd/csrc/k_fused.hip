@@ -8,8 +8,7 @@
 //                      border nodes' buckets) and BC point forces of
 //                      deformable border nodes (Update_bc_forces); signal [0]
 //   next n_rigid       one rigid body each: wait [0]; resultants,
-//                      Sum_up_forces, Apply_Newton, Apply_Euler; signal [1];
-//                      move the body's nodes
+//                      Sum_up_forces, Apply_Newton, Apply_Euler; signal [1]
 //   next n_blocks      one node block each: internal forces
 //                      (Update_internal_forces), which depend on nothing else
 //                      of the step
@@ -19,7 +18,9 @@
 // Launch B, one workgroup per node block: bucket sums (the contact sends),
 // Sum_up_forces + Apply_Newton + Apply_Euler with works, Update_borders and
 // the border-node kinematics, with the block's point forces and new
-// parameters staged in LDS.
+// parameters staged in LDS; then one thread per rigid-body node: its new
+// position and velocity (Update_kinematics of rigid bodies), which only the
+// next step reads.
 //
 // A workgroup only ever waits for workgroups of lower index, which the
 // hardware dispatches first, so every wait ends; the internal-force blocks
@@ -85,8 +86,7 @@ __device__ __forceinline__ void step_forces(DevModel& m, const BlockArrays& B, c
         const RigidForces F = rigid_forces_block(m, w, reinterpret_cast<double*>(s_cxy));
         const int b = m.rigid_ids[w];
         if (t == 0) rigid_step_body(m, b, F);
-        wg_signal(&sync[1]);          // its barrier also shows thread 0's body state to the workgroup
-        if (m.body_status[b] != 1) rigid_body_nodes(m, b);
+        wg_signal(&sync[1]);
         return;
     }
     w -= m.n_rigid;
@@ -123,6 +123,11 @@ k_step_forces32(DevModel m, BlockArrays B, StepLaunch S) {
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_finish(DevModel m, BlockArrays B) {
     extern __shared__ __align__(16) double s_fin[];
     const int blk = blockIdx.x, t = threadIdx.x;
+    if (blk >= B.n_blocks) {
+        const int q = (blk - B.n_blocks) * BLOCK_THREADS + t;
+        if (q < m.n_rigid_nodes) rigid_node(m, m.rigid_node_ids[q], 1);
+        return;
+    }
     double fi[2][NODES_PER_THREAD];
     const int n0 = B.own_nodes[B.blk_own_off[blk]], nown = B.blk_nown[blk];
 #pragma unroll

@@ -16,6 +16,11 @@ inline int step_forces_grid(const DevModel& d, const BlockArrays& B, const StepL
     return S.n_cwg + d.n_rigid + B.n_blocks + 1;
 }
 
+// workgroups of launch B: node blocks, then the rigid-body nodes
+inline int step_finish_grid(const DevModel& d, const BlockArrays& B) {
+    return B.n_blocks + (d.n_rigid_nodes + BLOCK_THREADS - 1) / BLOCK_THREADS;
+}
+
 // dynamic LDS of k_step_finish: a node block's staged point forces and
 // parameters
 inline size_t step_finish_lds(int max_bn, int max_own) {

@@ -211,10 +211,14 @@ __global__ void k_prox_rows(DevModel m) {
     m.body_chunks[b] = (body_rows(m, b) + ROW_CHUNK - 1) / ROW_CHUNK;
 }
 
-// first row chunk of every body (one workgroup); clears the hit list
+// first row chunk of every body and the body of every row chunk (one
+// workgroup); clears the hit list
 __global__ void __launch_bounds__(1024) k_prox_chunk_offsets(DevModel m) {
     __shared__ int s_w[16];
     wg_scan_array(m.body_chunks, m.chunk_off, m.n_bodies, s_w);
+    __syncthreads();
+    for (int b = threadIdx.x; b < m.n_bodies; b += blockDim.x)
+        for (int c = m.chunk_off[b]; c < m.chunk_off[b + 1]; c++) m.chunk_body[c] = b;
     if (threadIdx.x == 0) *m.hit_n = 0;
 }
 
@@ -263,21 +267,11 @@ __device__ inline bool decode_local_row(const DevModel& m, int b, int local, Row
     return false;
 }
 
-// body of row chunk c: the last b with chunk_off[b] <= c
-__device__ inline int chunk_body(const DevModel& m, int c) {
-    int lo = 0, hi = m.n_bodies;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (m.chunk_off[mid] <= c) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // Scans one row from master segment seg_start on.  WRITE == false counts the
 // candidates (and reports the first one's segment in *first), WRITE == true
 // emits them as elements base, base + 1, ...
 constexpr int SEG_BATCH = 4;     // segment records loaded together
+constexpr int BOX_BATCH = 4;     // chunk boxes loaded together
 template <bool WRITE>
 __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_start, int* first) {
     int bS = ri.body;
@@ -313,10 +307,15 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_
     const double Xq = Xs - P * shM;
     const double dd = 1.000001 * detect + 1e-10 * (1. + fabs(Xs) + fabs(Ys) + fabs(P * shM));
     const int cM0 = m.br_chunk0[gM], cM1 = m.br_chunk0[gM + 1];
-    for (int ch = cM0 + seg_start / SEG_CHUNK; ch < cM1; ch++) {
-    const double4 bx = m.sc_box[ch];
-    if (!(Xq > bx.x - dd && Xq < bx.y + dd && Ys > bx.z - dd && Ys < bx.w + dd))
-        continue;
+    for (int cb = cM0 + seg_start / SEG_CHUNK; cb < cM1; cb += BOX_BATCH) {
+    unsigned pass = 0;        // chunks of this batch whose box is within reach
+#pragma unroll
+    for (int u = 0; u < BOX_BATCH; u++) {
+        const double4 bx = m.sc_box[min(cb + u, cM1 - 1)];
+        if (cb + u < cM1 && Xq > bx.x - dd && Xq < bx.y + dd && Ys > bx.z - dd && Ys < bx.w + dd) pass |= 1u << u;
+    }
+    for (; pass; pass &= pass - 1) {
+    const int ch = cb + __builtin_ctz(pass);
     const int sg0 = max((ch - cM0) * SEG_CHUNK, seg_start), sg1 = min((ch - cM0) * SEG_CHUNK + SEG_CHUNK, nM - 1);
     for (int g = sg0; g < sg1; g += SEG_BATCH) {
     int4 sn[SEG_BATCH];
@@ -376,6 +375,7 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_
     }
     }
     }
+    }
     return cnt;
 }
 
@@ -385,18 +385,15 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_
 // candidate's segment, row of the body)
 __global__ void __launch_bounds__(ROW_CHUNK) k_prox_count(DevModel m) {
     __shared__ int s_w[ROW_CHUNK / 64];
-    __shared__ int s_body;
     const int n_chunks = m.chunk_off[m.n_bodies];
     for (int c = blockIdx.x; c < n_chunks; c += gridDim.x) {
-        if (threadIdx.x == 0) s_body = chunk_body(m, c);
-        __syncthreads();
-        const int b = s_body;
+        const int b = m.chunk_body[c];
         const int local = (c - m.chunk_off[b]) * ROW_CHUNK + threadIdx.x;
         int cnt = 0, first = 0;
         RowInfo ri;
         if (local < body_rows(m, b) && decode_local_row(m, b, local, ri)) cnt = scan_row<false>(m, ri, 0, 0, &first);
         int tot;
-        const int ex = wg_excl_scan(cnt, s_w, tot);     // its barriers also retire s_body
+        const int ex = wg_excl_scan(cnt, s_w, tot);
         if (threadIdx.x == 0) m.chunk_cnt[c] = tot;
         if (cnt > 0) {
             const int h = atomicAdd(m.hit_n, 1);
@@ -424,7 +421,7 @@ __global__ void k_prox_write(DevModel m) {
     const int n = min(*m.hit_n, m.c_cap);
     for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < n; h += gridDim.x * blockDim.x) {
         const int4 r = m.hit_rec[h];
-        const int b = chunk_body(m, r.x);
+        const int b = m.chunk_body[r.x];
         RowInfo ri;
         if (!decode_local_row(m, b, r.w, ri)) continue;
         scan_row<true>(m, ri, m.chunk_elem_off[r.x] + r.y, r.z, nullptr);

@@ -532,35 +532,6 @@ __device__ inline void rigid_node(const DevModel& m, int i, int with_forces) {
     rigid_node_at(m, i, R, cos(dr), sin(dr), with_forces, m.X[i], m.Y[i]);
 }
 
-// every node of rigid body b (active), by one workgroup: the rotation is
-// evaluated once, node coordinates are loaded RIGID_NODE_BATCH at a time
-constexpr int RIGID_NODE_BATCH = 4;
-__device__ inline void rigid_body_nodes(const DevModel& m, int b) {
-    const double* Rg = m.body_rigid + b * RIGID_FIELDS;
-    double R[RIGID_FIELDS];
-#pragma unroll
-    for (int k = 0; k < RIGID_FIELDS; k++) R[k] = Rg[k];
-    const double dr = R[R_DISP + 2] - 0.;
-    const double c = cos(dr), s = sin(dr);
-    const int n0 = m.node_off[b], n1 = n0 + m.body_nnodes[b];
-    for (int i0 = n0 + (int)threadIdx.x; i0 < n1; i0 += RIGID_NODE_BATCH * blockDim.x) {
-        double x[RIGID_NODE_BATCH], y[RIGID_NODE_BATCH];
-#pragma unroll
-        for (int u = 0; u < RIGID_NODE_BATCH; u++) {
-            const int i = i0 + u * blockDim.x;
-            x[u] = i < n1 ? m.X[i] : 0.;
-            y[u] = i < n1 ? m.Y[i] : 0.;
-        }
-#pragma unroll
-        for (int u = 0; u < RIGID_NODE_BATCH; u++) {
-            const int i = i0 + u * blockDim.x;
-            if (i < n1) rigid_node_at(m, i, R, c, s, 1, x[u], y[u]);
-        }
-    }
-}
-
-
-
 // Initialize_contact_forces + Update_contacts + Update_contact_forces (records
 // into buckets) for element i < nc, Update_bc_forces for deformable border
 // node bc_bn[i - nc] after them

@@ -713,7 +713,7 @@ void euler_step(mel_ctx& c) {
         TIMED(c, KT_STEP_FORCES, s,
               (c.precision == MEL_FP32_MIXED ? k_step_forces32<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)
                                              : k_step_forces<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)));
-        TIMED(c, KT_STEP_FINISH, s, (k_step_finish<<<B.n_blocks, BLOCK_THREADS, c.finish_lds, s>>>(d, B)));
+        TIMED(c, KT_STEP_FINISH, s, (k_step_finish<<<step_finish_grid(d, B), BLOCK_THREADS, c.finish_lds, s>>>(d, B)));
         c.time = t_next;
         c.outputs_stale = true;
         return;
@@ -1198,6 +1198,7 @@ void build(mel_ctx& c) {
     d.body_chunks = c.alloc<int>(NB);
     d.chunk_off = c.alloc<int>(NB + 1);
     d.chunk_cnt = c.alloc<int>(c.max_row_chunks);
+    d.chunk_body = c.alloc<int>(c.max_row_chunks);
     d.chunk_elem_off = c.alloc<int>((size_t)c.max_row_chunks + 1);
     d.hit_rec = c.alloc<int4>(d.c_cap);
     d.hit_n = c.alloc<int>(1);

@@ -130,6 +130,7 @@ struct DevModel {
     int nb_cap;                              // neighbour capacity per body
     int *nb_count, *nb_list;                 // per body: count, 3 ints per neighbour
     int *body_chunks, *chunk_off;            // narrow-phase row chunks per body, first chunk per body
+    int *chunk_body;                         // body of each row chunk
     int *chunk_cnt, *chunk_elem_off;         // candidates per row chunk, first element per row chunk
     int4 *hit_rec;                           // rows with candidates: chunk, offset, first segment, row
     int *hit_n;
