@@ -63,19 +63,21 @@ def algorithmic_bytes(kernel, info, fp32=False):
 def measured_traffic(kernel, info, fp32=False):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary, used only when it was measured on this workload with the
-    current kernel source; otherwise None."""
+    current kernel sources; otherwise None."""
     import hashlib
     path = os.path.join(ROOT, "profiles", "%s%s_1000grains.json" % (kernel, "32" if fp32 else ""))
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
         rec = json.load(f)
-    with open(os.path.join(ROOT, "melody2d_amd", "csrc", "k_block.hip"), "rb") as f:
-        src = hashlib.sha256(f.read()).hexdigest()[:16]
-    if rec.get("k_block_sha256_16") != src or rec.get("nodes") != info.n_nodes or \
+    h = hashlib.sha256()
+    for src in rec.get("sources", []):
+        with open(os.path.join(ROOT, "melody2d_amd", "csrc", src), "rb") as f:
+            h.update(f.read())
+    if rec.get("src_sha256_16") != h.hexdigest()[:16] or rec.get("nodes") != info.n_nodes or \
             rec.get("support_entries") != info.nnz_gauss:
         return None, None
-    return rec["traffic_bytes"], os.path.relpath(path[:-5] + ".md", ROOT)
+    return rec["traffic_bytes"], "profiles/step_kernels_1000grains.md"
 
 
 def cpu_baseline(model_path, n_nodes, steps, threads):

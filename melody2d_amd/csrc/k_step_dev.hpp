@@ -605,8 +605,10 @@ __device__ inline void node_block_finish(const DevModel& m, const BlockArrays& B
         w[4] += dt * (hx * kx + hy * ky);
         vpx = vpx + dt * apx;
         vpy = vpy + dt * apy;
-        m.upx[j] = m.upx[j] + dt * vpx;
-        m.upy[j] = m.upy[j] + dt * vpy;
+        const double nux = m.upx[j] + dt * vpx, nuy = m.upy[j] + dt * vpy;
+        if (!isfinite(nux) || !isfinite(nuy)) atomicOr(&m.status_flags[1], 1);
+        m.upx[j] = nux;
+        m.upy[j] = nuy;
         m.vpx[j] = vpx; m.vpy[j] = vpy;
         m.apx[j] = apx; m.apy[j] = apy;
     }
@@ -712,6 +714,7 @@ __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArray
         w[4] += dt * (hx * kx[q] + hy * ky[q]);
         const double nvx = vpx[q] + dt * apx, nvy = vpy[q] + dt * apy;
         const double nux = upx[q] + dt * nvx, nuy = upy[q] + dt * nvy;
+        if (!isfinite(nux) || !isfinite(nuy)) atomicOr(&m.status_flags[1], 1);
         m.upx[j] = nux; m.upy[j] = nuy;
         m.vpx[j] = nvx; m.vpy[j] = nvy;
         m.apx[j] = apx; m.apy[j] = apy;

@@ -36,6 +36,10 @@ struct CapacityError {
     std::string msg;
 };
 
+struct NanError {
+    std::string msg;
+};
+
 inline int blocks(long long n, int b = BLOCK) { return (int)std::max<long long>(1, (n + b - 1) / b); }
 
 #define TIMED(ctx, id, st, launch)                 \
@@ -262,6 +266,7 @@ void raise_flags(const int* f) {
         if (f[0] & 16) what += " cross-workgroup wait time limit (fused step)";
         throw CapacityError{"device capacity exceeded:" + what};
     }
+    if (f[1]) throw NanError{"non-finite displacement parameters (the run is no longer meaningful)"};
 }
 
 void check_flags(mel_ctx& c) {
@@ -1293,6 +1298,9 @@ mel_status create(mel_ctx** out, int device, int precision, const std::function<
     } catch (CapacityError& e) {
         g_load_err = e.msg;
         return e.msg.find("capacity") != std::string::npos ? MEL_E_CAPACITY : MEL_E_UNSUPPORTED;
+    } catch (NanError& e) {
+        g_load_err = e.msg;
+        return MEL_E_NAN;
     }
     *out = c.release();
     return MEL_OK;
@@ -1310,6 +1318,9 @@ mel_status guarded(mel_ctx* c, F&& f) {
     } catch (CapacityError& e) {
         c->err = e.msg;
         return MEL_E_CAPACITY;
+    } catch (NanError& e) {
+        c->err = e.msg;
+        return MEL_E_NAN;
     }
 }
 
