@@ -146,6 +146,7 @@ __device__ inline float2 contribution32(float c0, f32x2 f, const float4& P) {
 // precision policies of the block kernel
 struct Fp64 {
     static constexpr bool f32 = false;
+    static constexpr bool dbuf = false;     // one chunk's entries in flight (registers)
     static constexpr int KREG = KREG_ENTRIES;
     typedef f64x2 vec;
     typedef double2 slot;
@@ -154,6 +155,7 @@ struct Fp64 {
 };
 struct Fp32 {
     static constexpr bool f32 = true;
+    static constexpr bool dbuf = true;      // two chunks' entries in flight
     static constexpr int KREG = KREG_ENTRIES_F32;
     typedef f32x2 vec;
     typedef float2 slot;
@@ -187,7 +189,7 @@ template <class PR>
 struct LaneRegs {
     typename PR::vec f[PR::KREG];
     unsigned pk[PR::KREG];
-    typename PR::real c0;
+    double c0;      // kept as loaded: a use before the chunk's barrier would wait for every load in flight
     int reg;
     int tp0[NODES_PER_THREAD], tp1[NODES_PER_THREAD];
 };
@@ -211,7 +213,7 @@ __device__ inline void load_lane(const BlockArrays& B, int4 meta, int2 po, int n
         }
     }
     if (L.valid) {
-        R.c0 = (typename PR::real)B.bgp_c0[meta.x + L.lane];
+        R.c0 = B.bgp_c0[meta.x + L.lane];
         R.reg = B.bgp_region[meta.x + L.lane];
 #pragma unroll
         for (int i = 0; i < PR::KREG; i++) {
@@ -292,12 +294,13 @@ __device__ inline void gp_lane32(const BlockArrays& B, const LaneMap& L, const L
         grad_add32<MAT>(a, b, c, d, fxy[e], uv[B.be_pk[e] & 0xFFFF]);
     }
     const float4 P = stress_terms32<MAT>(Mu, K2, a, b, c, d);
+    const float c0 = (float)R.c0;
 #pragma unroll
     for (int i = 0; i < KREG_ENTRIES_F32; i++)
-        if (i < L.K) s_cxy[R.pk[i] >> 16] = contribution32(R.c0, R.f[i], P);
+        if (i < L.K) s_cxy[R.pk[i] >> 16] = contribution32(c0, R.f[i], P);
     for (int i = KREG_ENTRIES_F32; i < L.K; i++) {
         size_t e = L.base + (size_t)i * L.stride;
-        s_cxy[B.be_pk[e] >> 16] = contribution32(R.c0, fxy[e], P);
+        s_cxy[B.be_pk[e] >> 16] = contribution32(c0, fxy[e], P);
     }
     if (!single) {
 #pragma unroll
@@ -365,7 +368,9 @@ __device__ __forceinline__ void block_force_update(DevModel& m, const BlockArray
 #pragma unroll
     for (int q = 0; q < NODES_PER_THREAD; q++) { tx[q] = ty[q] = rx[q] = ry[q] = 0.; cur[q] = -1; }
 
-    for (int c = 0; c < nch; c++) {
+    // one chunk with its entries in R; R is refilled with chunk nxt once its
+    // Gauss-point work is done
+    auto chunk = [&](int c, LaneRegs<PR>& R, int nxt) {
         __syncthreads();      // descriptors and displacements staged; previous chunk's slots read
         const int4 meta = s_meta[c];
         const LaneMap L = lane_map(meta, t);
@@ -383,8 +388,7 @@ __device__ __forceinline__ void block_force_update(DevModel& m, const BlockArray
         int tp0[NODES_PER_THREAD], tp1[NODES_PER_THREAD];
 #pragma unroll
         for (int q = 0; q < NODES_PER_THREAD; q++) { tp0[q] = R.tp0[q]; tp1[q] = R.tp1[q]; }
-        // next chunk's entries load while this chunk is summed
-        if (c + 1 < nch) load_lane<PR>(B, s_meta[c + 1], s_po[c + 1], nown, t, R);
+        if (nxt < nch) load_lane<PR>(B, s_meta[nxt], s_po[nxt], nown, t, R);
         __syncthreads();      // contributions visible
         if (single) {
 #pragma unroll
@@ -416,6 +420,18 @@ __device__ __forceinline__ void block_force_update(DevModel& m, const BlockArray
                 }
             }
         }
+    };
+    if constexpr (PR::dbuf) {
+        // chunk c + 1's entries load during the whole of chunk c
+        LaneRegs<PR> R2;
+        if (nch > 1) load_lane<PR>(B, gmeta[1], gpo[1], nown, t, R2);
+        for (int c = 0; c < nch; c += 2) {
+            chunk(c, R, c + 2);
+            if (c + 1 < nch) chunk(c + 1, R2, c + 3);
+        }
+    } else {
+        // the next chunk's entries load while this chunk is summed
+        for (int c = 0; c < nch; c++) chunk(c, R, c + 1);
     }
 
     // internal force of the owned nodes (Node::Sum_up_forces region sum)
