@@ -146,21 +146,32 @@ def main():
             name, time.time() - t0, info.n_nodes, info.n_gauss, info.nnz_gauss, info.n_contacts))
         ctx.euler_steps(args.warmup)
         ctx.synchronize()
-        # the step's dominant launch: launch A of the two-launch step, or the
-        # internal-force kernel of the launch sequence
-        ctx.set_kernel_timing(["step_forces", "block_force_update"])
-        ctx.synchronize()
+        # value window: the steps as a user runs them (runs between contact
+        # updates replayed from captured graphs)
         t0 = time.perf_counter()
         ctx.euler_steps(args.steps)
         ctx.synchronize()
         t1 = time.perf_counter()
-        kt = ctx.kernel_timing()
-        ctx.set_kernel_timing(False)
         sec = t1 - t0
         info = ctx.get_info()
         value = info.n_nodes * args.steps / sec
         log("[%s] timed %d steps in %.3fs: %.3e node-steps/s, %.3f ms/step" % (
             name, args.steps, sec, value, 1e3 * sec / args.steps))
+        # launch-duration window: the same number of steps with HIP events
+        # around every launch of the step's dominant kernel (launch A of the
+        # two-launch step, or the internal-force kernel of the launch
+        # sequence), recorded on the library's stream; per-launch events
+        # switch graph replay off, so this window runs launch by launch
+        ctx.set_kernel_timing(["step_forces", "step_finish", "block_force_update"])
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        ctx.euler_steps(args.steps)
+        ctx.synchronize()
+        t3 = time.perf_counter()
+        kt = ctx.kernel_timing()
+        ctx.set_kernel_timing(False)
+        log("[%s] event window: %d steps in %.3fs (%.3f ms/step launch by launch)" % (
+            name, args.steps, t3 - t2, 1e3 * (t3 - t2) / args.steps))
         roof = None
         dominant = next(k for k in ("step_forces", "block_force_update") if kt[k][1])
         ms, n = kt[dominant]
@@ -194,7 +205,7 @@ def main():
     # rounding residue of the reference formula), then both forms take one
     # step from the FP64 displacement parameters
     t0 = time.time()
-    done = args.warmup + args.steps + (min(args.steps, 50) if args.breakdown else 0)
+    done = args.warmup + 2 * args.steps + (min(args.steps, 50) if args.breakdown else 0)
     if args.error_step > done:
         ctx64.euler_steps(args.error_step - done)
     counts = [b.n_nodes for b in model.bodies]
@@ -257,7 +268,7 @@ def main():
                        "ms_per_step": round(1e3 * sec32 / args.steps, 4),
                        "speedup_vs_fp64": round(value32 / value, 3),
                        "rel_internal_force_err_vs_fp64": rel_err,
-                       "error_state_step": max(args.error_step, args.warmup + args.steps),
+                       "error_state_step": max(args.error_step, done),
                        "dtype": "f32 gradients/stresses/contributions, f64 state and sums",
                        "roofline": roof32},
     }

@@ -18,9 +18,9 @@
 // Launch B, one workgroup per node block: bucket sums (the contact sends),
 // Sum_up_forces + Apply_Newton + Apply_Euler with works, Update_borders and
 // the border-node kinematics, with the block's point forces and new
-// parameters staged in LDS; then one thread per rigid-body node: its new
-// position and velocity (Update_kinematics of rigid bodies), which only the
-// next step reads.
+// parameters staged in LDS; beside them, one thread per rigid-body node: its
+// new position and velocity (Update_kinematics of rigid bodies), which only
+// the next step reads.
 //
 // A workgroup only ever waits for workgroups of lower index, which the
 // hardware dispatches first, so every wait ends; the internal-force blocks
@@ -95,12 +95,16 @@ __device__ __forceinline__ void step_forces(DevModel& m, const BlockArrays& B, c
         block_force_update<PR>(m, B, w, s_cxy, fi);
         return;
     }
-    // last workgroup
+    // last workgroup: the time of the end of the step from the device's copy
+    // of the time (the same sum the host forms), so that a launch does not
+    // carry it and a captured run of steps can be replayed
     wg_wait(m, &sync[0], S.n_cwg);
     wg_wait(m, &sync[1], m.n_rigid);
-    for (int k = t; k < m.n_bc_timed; k += BLOCK_THREADS) bc_eval_at(m, m.bc_timed[k], S.t_next);
+    const double t_next = *m.time + m.dt;
+    __syncthreads();           // every thread has read the time before it is replaced
+    for (int k = t; k < m.n_bc_timed; k += BLOCK_THREADS) bc_eval_at(m, m.bc_timed[k], t_next);
     if (t == 0) {
-        *m.time = S.t_next;
+        *m.time = t_next;
         __hip_atomic_store(&sync[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&sync[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -122,7 +126,8 @@ k_step_forces32(DevModel m, BlockArrays B, StepLaunch S) {
 
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_finish(DevModel m, BlockArrays B) {
     extern __shared__ __align__(16) double s_fin[];
-    const int blk = blockIdx.x, t = threadIdx.x;
+    const int t = threadIdx.x;
+    const int blk = blockIdx.x;
     if (blk >= B.n_blocks) {
         const int q = (blk - B.n_blocks) * BLOCK_THREADS + t;
         if (q < m.n_rigid_nodes) rigid_node(m, m.rigid_node_ids[q], 1);

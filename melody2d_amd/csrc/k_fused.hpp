@@ -8,7 +8,6 @@ namespace mel {
 struct StepLaunch {
     int n_cwg;       // workgroups of contact elements and BC point forces (they loop over the
                      // element count on the device)
-    double t_next;   // time at the end of the step
 };
 
 // workgroups of launch A
@@ -16,7 +15,7 @@ inline int step_forces_grid(const DevModel& d, const BlockArrays& B, const StepL
     return S.n_cwg + d.n_rigid + B.n_blocks + 1;
 }
 
-// workgroups of launch B: node blocks, then the rigid-body nodes
+// workgroups of launch B: node blocks, then one thread per rigid-body node
 inline int step_finish_grid(const DevModel& d, const BlockArrays& B) {
     return B.n_blocks + (d.n_rigid_nodes + BLOCK_THREADS - 1) / BLOCK_THREADS;
 }

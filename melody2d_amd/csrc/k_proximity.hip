@@ -7,6 +7,7 @@
 // scanned and written, so the element index of a candidate equals its rank in
 // the reference's loop nest.  History transfer (Code/Proximity.h:271-319)
 // looks up, per slave slot, the previous elements in their old order.
+#include <climits>
 #include "mel_device.hpp"
 #include "k_step.hpp"
 
@@ -105,16 +106,22 @@ __global__ void k_prox_aabb(DevModel m) {
     const int br = (blockIdx.x * blockDim.x + threadIdx.x) / 64, lane = threadIdx.x & 63;
     const bool on = br < m.n_borders && m.body_status[m.br_body[br < m.n_borders ? br : 0]] != 1;
     double xmin = 1.e100, xmax = -1.e100, ymin = 1.e100, ymax = -1.e100;
+    double sxmin = 1.e100, sxmax = -1.e100, symin = 1.e100, symax = -1.e100;
     if (on)
         for (int c = m.br_chunk0[br] + lane; c < m.br_chunk0[br + 1]; c += 64) {
             xmin = fmin(xmin, m.sc_nxmin[c]); xmax = fmax(xmax, m.sc_nxmax[c]);
             ymin = fmin(ymin, m.sc_nymin[c]); ymax = fmax(ymax, m.sc_nymax[c]);
+            const double4 sb = m.sc_box[c];
+            sxmin = fmin(sxmin, sb.x); sxmax = fmax(sxmax, sb.y); symin = fmin(symin, sb.z); symax = fmax(symax, sb.w);
         }
     for (int off = 32; off > 0; off >>= 1) {
         xmin = fmin(xmin, __shfl_xor(xmin, off, 64)); xmax = fmax(xmax, __shfl_xor(xmax, off, 64));
         ymin = fmin(ymin, __shfl_xor(ymin, off, 64)); ymax = fmax(ymax, __shfl_xor(ymax, off, 64));
+        sxmin = fmin(sxmin, __shfl_xor(sxmin, off, 64)); sxmax = fmax(sxmax, __shfl_xor(sxmax, off, 64));
+        symin = fmin(symin, __shfl_xor(symin, off, 64)); symax = fmax(symax, __shfl_xor(symax, off, 64));
     }
     if (!on || lane != 0) return;
+    m.pb_segbox[br] = make_double4(sxmin, sxmax, symin, symax);
     double inv = 1. / m.period;
     m.pb_xmin[br] = xmin; m.pb_xmax[br] = xmax; m.pb_ymin[br] = ymin; m.pb_ymax[br] = ymax;
     m.pb_shmin[br] = (int)floor((xmin - m.xmin_period) * inv);
@@ -273,7 +280,7 @@ __device__ inline bool decode_local_row(const DevModel& m, int b, int local, Row
 constexpr int SEG_BATCH = 4;     // segment records loaded together
 constexpr int BOX_BATCH = 4;     // chunk boxes loaded together
 template <bool WRITE>
-__device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_start, int* first) {
+__device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_start, int* first, int limit) {
     int bS = ri.body;
     int gS = m.border_off[bS] + ri.borderS;
     int slotS = m.br_slot0[gS] + ri.jS;
@@ -307,17 +314,22 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_
     const double Xq = Xs - P * shM;
     const double dd = 1.000001 * detect + 1e-10 * (1. + fabs(Xs) + fabs(Ys) + fabs(P * shM));
     const int cM0 = m.br_chunk0[gM], cM1 = m.br_chunk0[gM + 1];
-    for (int cb = cM0 + seg_start / SEG_CHUNK; cb < cM1; cb += BOX_BATCH) {
+    if (!WRITE) {
+        // outside the whole border's segment box: no chunk box can pass
+        const double4 sb = m.pb_segbox[gM];
+        if (!(Xq > sb.x - dd && Xq < sb.y + dd && Ys > sb.z - dd && Ys < sb.w + dd)) return 0;
+    }
+    for (int cb = cM0 + seg_start / SEG_CHUNK; cb < cM1 && cnt < limit; cb += BOX_BATCH) {
     unsigned pass = 0;        // chunks of this batch whose box is within reach
 #pragma unroll
     for (int u = 0; u < BOX_BATCH; u++) {
         const double4 bx = m.sc_box[min(cb + u, cM1 - 1)];
         if (cb + u < cM1 && Xq > bx.x - dd && Xq < bx.y + dd && Ys > bx.z - dd && Ys < bx.w + dd) pass |= 1u << u;
     }
-    for (; pass; pass &= pass - 1) {
+    for (; pass && cnt < limit; pass &= pass - 1) {
     const int ch = cb + __builtin_ctz(pass);
     const int sg0 = max((ch - cM0) * SEG_CHUNK, seg_start), sg1 = min((ch - cM0) * SEG_CHUNK + SEG_CHUNK, nM - 1);
-    for (int g = sg0; g < sg1; g += SEG_BATCH) {
+    for (int g = sg0; g < sg1 && cnt < limit; g += SEG_BATCH) {
     int4 sn[SEG_BATCH];
     double4 sx[SEG_BATCH];
 #pragma unroll
@@ -391,14 +403,18 @@ __global__ void __launch_bounds__(ROW_CHUNK) k_prox_count(DevModel m) {
         const int local = (c - m.chunk_off[b]) * ROW_CHUNK + threadIdx.x;
         int cnt = 0, first = 0;
         RowInfo ri;
-        if (local < body_rows(m, b) && decode_local_row(m, b, local, ri)) cnt = scan_row<false>(m, ri, 0, 0, &first);
+        if (local < body_rows(m, b) && decode_local_row(m, b, local, ri)) cnt = scan_row<false>(m, ri, 0, 0, &first, INT_MAX);
         int tot;
         const int ex = wg_excl_scan(cnt, s_w, tot);
         if (threadIdx.x == 0) m.chunk_cnt[c] = tot;
         if (cnt > 0) {
             const int h = atomicAdd(m.hit_n, 1);
-            if (h < m.c_cap) m.hit_rec[h] = make_int4(c, ex, first, local);
-            else atomicOr(&m.status_flags[0], 8);
+            if (h < m.c_cap) {
+                m.hit_rec[h] = make_int4(c, ex, first, local);
+                m.hit_cnt[h] = cnt;
+            } else {
+                atomicOr(&m.status_flags[0], 8);
+            }
         }
     }
 }
@@ -424,7 +440,7 @@ __global__ void k_prox_write(DevModel m) {
         const int b = m.chunk_body[r.x];
         RowInfo ri;
         if (!decode_local_row(m, b, r.w, ri)) continue;
-        scan_row<true>(m, ri, m.chunk_elem_off[r.x] + r.y, r.z, nullptr);
+        scan_row<true>(m, ri, m.chunk_elem_off[r.x] + r.y, r.z, nullptr, m.hit_cnt[h]);
     }
 }
 

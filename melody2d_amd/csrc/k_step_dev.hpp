@@ -639,17 +639,50 @@ __device__ inline void node_block_finish(const DevModel& m, const BlockArrays& B
     for (int i = t; i < nbn; i += BLOCK_THREADS) kinematics_node(m, m.bn_node[bns[i]]);
 }
 
-// node_block_finish for a block that holds a whole body, as the one-launch
+// a slot's tributary-length inputs, loaded ahead of use (slot_length)
+struct SlotPre {
+    int j, n, simple;
+    double x0, y0, x1, y1, x2, y2;
+};
+
+__device__ inline SlotPre slot_prefetch(const DevModel& m, int s) {
+    SlotPre r;
+    const int br = m.sl_border[s];
+    r.j = m.sl_j[s];
+    r.n = m.br_nslots[br];
+    r.simple = m.br_periodicity[br] == 1;
+    const double P = m.period;
+    const int a0 = m.sl_n0[s], a1 = m.sl_n1[s], a2 = m.sl_n2[s];
+    r.x0 = m.xc[a0] + P * m.sl_s0[s]; r.y0 = m.yc[a0];
+    r.x1 = m.xc[a1] + P * m.sl_s1[s]; r.y1 = m.yc[a1];
+    r.x2 = m.xc[a2] + P * m.sl_s2[s]; r.y2 = m.yc[a2];
+    return r;
+}
+
+// slot_length from prefetched inputs (same expressions)
+__device__ inline double slot_length_pre(const SlotPre& r) {
+    double l21 = __dsqrt_rn((r.x2 - r.x1) * (r.x2 - r.x1) + (r.y2 - r.y1) * (r.y2 - r.y1));
+    double l10 = __dsqrt_rn((r.x1 - r.x0) * (r.x1 - r.x0) + (r.y1 - r.y0) * (r.y1 - r.y0));
+    if (r.j == 0 && r.simple) return l21 * 0.5;
+    if (r.j == r.n - 1 && r.simple) return l10 * 0.5;
+    return (l21 + l10) * 0.5;
+}
+
+// node_block_finish for a block that holds a whole body, as the two-launch
 // step runs it: the same operations and summation orders, with the block's
 // point forces and new nodal parameters staged in LDS so that the spreading
 // to the nodes and the border-node kinematics read LDS instead of chasing
-// global indices.
+// global indices, and every load that does not depend on an earlier phase
+// (node data, the slot stencils and old positions of the lengths, the
+// DOF_TO_DISP rows of the kinematics) issued at the start.
 //   fi     internal forces of the thread's owned nodes (o = t + q * BLOCK_THREADS)
 //   sc     LDS scratch of at least 6 * (blk_bn_n + blk_nown) doubles
 // Preconditions (checked when the blocks are built): the owned nodes are
 // own[0] .. own[0] + nown - 1, every point-force spreading entry of an owned
 // node names a border node of the block (bt_lbn is its index in the block's
 // list) and every DOF_TO_DISP entry of a border node names an owned node.
+constexpr int FIN_SLOTS = 2;     // slots of a border node with prefetched length inputs
+constexpr int FIN_ROW = 24;      // DOF_TO_DISP entries of a border node prefetched
 __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArrays& B, int blk,
                                              const double (&fi)[2][NODES_PER_THREAD], double* sc) {
     __shared__ double red[BLOCK_THREADS / 64][5];
@@ -662,19 +695,9 @@ __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArray
     const int nbn = B.blk_bn_n[blk];
     double* s_bnp = sc;                // [nbn][6]: contact, Dirichlet, Neumann point forces (x, y)
     double* s_up = sc + 6 * nbn;       // [nown][6]: new displacement, velocity, acceleration parameters
-    for (int i = t; i < nbn; i += BLOCK_THREADS) {
-        const int bn = bns[i];
-        const double pdx = m.bn_pdx[bn], pdy = m.bn_pdy[bn], pnx = m.bn_pnx[bn], pny = m.bn_pny[bn];
-        const double2 pc = bucket_sum_node(m, bn);
-        double* q = s_bnp + 6 * i;
-        q[0] = pc.x; q[1] = pc.y; q[2] = pdx; q[3] = pdy; q[4] = pnx; q[5] = pny;
-    }
-    __syncthreads();
-    double w[5] = {0., 0., 0., 0., 0.};   // internal body dirichlet neumann damping
-    double gx = m.gravity_x, gy = m.gravity_y;
-    if (m.body_status[body] == 2) { gx = 0.; gy = 0.; }
-    const double dt = m.dt;
-    // every per-node input of both owned nodes is loaded before any store
+
+    // ---- loads issued up front ----
+    // the owned nodes' inputs of the update
     double mx[NODES_PER_THREAD], my[NODES_PER_THREAD], imx[NODES_PER_THREAD], imy[NODES_PER_THREAD];
     double kx[NODES_PER_THREAD], ky[NODES_PER_THREAD], vpx[NODES_PER_THREAD], vpy[NODES_PER_THREAD];
     double upx[NODES_PER_THREAD], upy[NODES_PER_THREAD];
@@ -688,6 +711,43 @@ __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArray
         upx[q] = m.upx[j]; upy[q] = m.upy[j];
         e0[q] = m.bt_ptr[j]; e1[q] = o < nown ? m.bt_ptr[j + 1] : e0[q];
     }
+    // the thread's first border node: its lengths' inputs (old positions)
+    // and its kinematics row
+    const bool mine = t < nbn;
+    const int bn_t = mine ? bns[t] : 0;
+    int sq0 = 0, sq1 = 0, knode = 0, k0 = 0, k1 = 0;
+    SlotPre sp[FIN_SLOTS];
+    int di[FIN_ROW];
+    double dp[FIN_ROW];
+    double kX = 0., kY = 0.;
+    if (mine) {
+        sq0 = m.bn_slot_ptr[bn_t]; sq1 = m.bn_slot_ptr[bn_t + 1];
+        knode = m.bn_node[bn_t];
+        k0 = m.d_ptr[knode]; k1 = m.d_ptr[knode + 1];
+        kX = m.X[knode]; kY = m.Y[knode];
+#pragma unroll
+        for (int u = 0; u < FIN_SLOTS; u++)
+            if (sq0 + u < sq1) sp[u] = slot_prefetch(m, m.bn_slots[sq0 + u]);
+#pragma unroll
+        for (int u = 0; u < FIN_ROW; u++)
+            if (k0 + u < k1) { di[u] = m.d_idx[k0 + u]; dp[u] = m.d_phi[k0 + u]; }
+    }
+
+    // ---- sends: bucket sums, point forces staged ----
+    for (int i = t; i < nbn; i += BLOCK_THREADS) {
+        const int bn = bns[i];
+        const double pdx = m.bn_pdx[bn], pdy = m.bn_pdy[bn], pnx = m.bn_pnx[bn], pny = m.bn_pny[bn];
+        const double2 pc = bucket_sum_node(m, bn);
+        double* q = s_bnp + 6 * i;
+        q[0] = pc.x; q[1] = pc.y; q[2] = pdx; q[3] = pdy; q[4] = pnx; q[5] = pny;
+    }
+    __syncthreads();
+
+    // ---- Sum_up_forces, Apply_Newton, Apply_Euler, works ----
+    double w[5] = {0., 0., 0., 0., 0.};   // internal body dirichlet neumann damping
+    double gx = m.gravity_x, gy = m.gravity_y;
+    if (m.body_status[body] == 2) { gx = 0.; gy = 0.; }
+    const double dt = m.dt;
 #pragma unroll
     for (int q = 0; q < NODES_PER_THREAD; q++) {
         const int o = t + q * BLOCK_THREADS;
@@ -727,7 +787,16 @@ __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArray
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
         if ((t & 63) == 0) red[t >> 6][q] = v;
     }
-    __syncthreads();
+
+    // ---- Update_borders: lengths from the old positions ----
+    if (mine) {
+#pragma unroll
+        for (int u = 0; u < FIN_SLOTS; u++)
+            if (sq0 + u < sq1) m.sl_len[m.bn_slots[sq0 + u]] = slot_length_pre(sp[u]);
+        for (int q = sq0 + FIN_SLOTS; q < sq1; q++) m.sl_len[m.bn_slots[q]] = slot_length(m, m.bn_slots[q]);
+    }
+    for (int i = t + BLOCK_THREADS; i < nbn; i += BLOCK_THREADS) bn_lengths(m, bns[i]);
+    __syncthreads();       // works partials in place; lengths read the old positions before any moves
     if (t == 0) {
         const int slot[5] = {W_INTERNAL, W_BODY, W_DIRICHLET, W_NEUMANN, W_DAMPING};
         double* acc = B.blk_work + 14 * (size_t)blk;
@@ -740,17 +809,40 @@ __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArray
             acc[2 * s] = nt;
         }
     }
-    for (int i = t; i < nbn; i += BLOCK_THREADS) bn_lengths(m, bns[i]);
-    __syncthreads();       // lengths from the old positions are in place
-    for (int i = t; i < nbn; i += BLOCK_THREADS) {
+
+    // ---- Update_kinematics and positions of the border nodes ----
+    if (mine) {
+        double dx = 0., dy = 0., vx = 0., vy = 0., ax = 0., ay = 0.;
+#pragma unroll
+        for (int u = 0; u < FIN_ROW; u++) {
+            if (k0 + u < k1) {
+                const double* up = s_up + 6 * (di[u] - n0);
+                const double f = dp[u];
+                dx += f * up[0]; dy += f * up[1];
+                vx += f * up[2]; vy += f * up[3];
+                ax += f * up[4]; ay += f * up[5];
+            }
+        }
+        for (int k = k0 + FIN_ROW; k < k1; k++) {
+            const double* up = s_up + 6 * (m.d_idx[k] - n0);
+            const double f = m.d_phi[k];
+            dx += f * up[0]; dy += f * up[1];
+            vx += f * up[2]; vy += f * up[3];
+            ax += f * up[4]; ay += f * up[5];
+        }
+        m.ux[knode] = dx; m.uy[knode] = dy; m.vx[knode] = vx; m.vy[knode] = vy; m.ax[knode] = ax; m.ay[knode] = ay;
+        m.xc[knode] = kX + dx;
+        m.yc[knode] = kY + dy;
+    }
+    for (int i = t + BLOCK_THREADS; i < nbn; i += BLOCK_THREADS) {
         const int node = m.bn_node[bns[i]];
         double dx = 0., dy = 0., vx = 0., vy = 0., ax = 0., ay = 0.;
         for (int k = m.d_ptr[node]; k < m.d_ptr[node + 1]; k++) {
-            const double* u = s_up + 6 * (m.d_idx[k] - n0);
+            const double* up = s_up + 6 * (m.d_idx[k] - n0);
             const double f = m.d_phi[k];
-            dx += f * u[0]; dy += f * u[1];
-            vx += f * u[2]; vy += f * u[3];
-            ax += f * u[4]; ay += f * u[5];
+            dx += f * up[0]; dy += f * up[1];
+            vx += f * up[2]; vy += f * up[3];
+            ax += f * up[4]; ay += f * up[5];
         }
         m.ux[node] = dx; m.uy[node] = dy; m.vx[node] = vx; m.vy[node] = vy; m.ax[node] = ax; m.ay[node] = ay;
         m.xc[node] = m.X[node] + dx;

@@ -7,6 +7,7 @@
 #include <functional>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -129,6 +130,17 @@ struct mel_ctx {
     // step in two launches (every deformable body in one node block), else
     // the launch sequence on two streams
     bool two_launch = false;
+    // captured runs of steps (two-launch form): run length, contact
+    // workgroups, current contact buffer -> executable graph
+    struct GraphKey {
+        int n, n_cwg;
+        const void* cbuf;
+        bool operator<(const GraphKey& o) const {
+            return n != o.n ? n < o.n : n_cwg != o.n_cwg ? n_cwg < o.n_cwg : cbuf < o.cbuf;
+        }
+    };
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    bool use_graphs = false;
     size_t finish_lds = 0;
     int* d_kin_all = nullptr;     // every deformable node (kinematics on request)
     int n_kin_all = 0;
@@ -163,6 +175,7 @@ struct mel_ctx {
         for (auto& e : pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_async) (void)hipEventDestroy(ev_async);
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
         if (h_async) (void)hipHostFree(h_async);
         if (ev_join) (void)hipEventDestroy(ev_join);
         for (void* p : allocs) (void)hipFree(p);
@@ -643,6 +656,10 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     // MELODY2D_FUSED_STEP=0 asks for the launch sequence
     const char* fe = getenv("MELODY2D_FUSED_STEP");
     c.two_launch = B.lds_finish_ok && !(fe && fe[0] == '0');
+    // runs of steps replayed from captured graphs; MELODY2D_GRAPHS=0 launches
+    // every step
+    const char* ge = getenv("MELODY2D_GRAPHS");
+    c.use_graphs = c.two_launch && !(ge && ge[0] == '0');
     if (c.two_launch) {
         HIP_OK(hipFuncSetAttribute(f32 ? (const void*)k_step_forces32 : (const void*)k_step_forces,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.block_lds));
@@ -701,6 +718,48 @@ void initialize_czm(mel_ctx& c) {
     k_init_czm<<<blocks(c.n_contacts), BLOCK, 0, c.stream>>>(c.d);
 }
 
+// the launches of one step in two launches (the device advances its time)
+void launch_step(mel_ctx& c, int n_cwg) {
+    DevModel& d = c.d;
+    const BlockArrays& B = c.blocks_dev;
+    hipStream_t s = c.stream;
+    // Update_damping_forces reads the velocity parameters of the start of
+    // the step, so it runs before the step kernels update them
+    if (d.n_matrix > 0) TIMED(c, KT_DAMPING, s, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d)));
+    StepLaunch S{n_cwg};
+    const int ga = step_forces_grid(d, B, S);
+    TIMED(c, KT_STEP_FORCES, s,
+          (c.precision == MEL_FP32_MIXED ? k_step_forces32<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)
+                                         : k_step_forces<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)));
+    TIMED(c, KT_STEP_FINISH, s, (k_step_finish<<<step_finish_grid(d, B), BLOCK_THREADS, c.finish_lds, s>>>(d, B)));
+}
+
+// a run of steps between contact updates: replayed from a captured graph
+// (one per run length, contact workgroup count and current contact
+// buffer), or launched step by step while kernels are being timed
+void run_steps(mel_ctx& c, int n) {
+    poll_async(c, false);
+    const int n_cwg = contact_workgroups(c);
+    if (!c.use_graphs || c.timing || n < 2) {
+        for (int k = 0; k < n; k++) launch_step(c, n_cwg);
+        return;
+    }
+    const mel_ctx::GraphKey key{n, n_cwg, (const void*)c.d.C.bodyS};
+    auto it = c.graphs.find(key);
+    if (it == c.graphs.end()) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+        HIP_OK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < n; k++) launch_step(c, n_cwg);
+        HIP_OK(hipStreamEndCapture(c.stream, &g));
+        hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_OK(err);
+        it = c.graphs.emplace(key, e).first;
+    }
+    HIP_OK(hipGraphLaunch(it->second, c.stream));
+}
+
 // Euler_step (Code/Solver.h:10-88)
 void euler_step(mel_ctx& c) {
     DevModel& d = c.d;
@@ -710,15 +769,7 @@ void euler_step(mel_ctx& c) {
     const int n_cwg = contact_workgroups(c);
     const double t_next = c.time + d.dt;
     if (c.two_launch) {
-        // Update_damping_forces reads the velocity parameters of the start of
-        // the step, so it runs before the step kernels update them
-        if (d.n_matrix > 0) TIMED(c, KT_DAMPING, s, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d)));
-        StepLaunch S{n_cwg, t_next};
-        const int ga = step_forces_grid(d, B, S);
-        TIMED(c, KT_STEP_FORCES, s,
-              (c.precision == MEL_FP32_MIXED ? k_step_forces32<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)
-                                             : k_step_forces<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)));
-        TIMED(c, KT_STEP_FINISH, s, (k_step_finish<<<step_finish_grid(d, B), BLOCK_THREADS, c.finish_lds, s>>>(d, B)));
+        launch_step(c, n_cwg);
         c.time = t_next;
         c.outputs_stale = true;
         return;
@@ -1206,6 +1257,8 @@ void build(mel_ctx& c) {
     d.chunk_body = c.alloc<int>(c.max_row_chunks);
     d.chunk_elem_off = c.alloc<int>((size_t)c.max_row_chunks + 1);
     d.hit_rec = c.alloc<int4>(d.c_cap);
+    d.hit_cnt = c.alloc<int>(d.c_cap);
+    d.pb_segbox = c.alloc<double4>(NBR);
     d.hit_n = c.alloc<int>(1);
     d.slot_old_list = c.alloc<int>((size_t)NS * 16);
     d.slot_old_cursor = c.alloc<int>(NS);
@@ -1359,6 +1412,40 @@ mel_status mel_update_proximity(mel_ctx* c) {
 mel_status mel_euler_steps(mel_ctx* c, int nsteps) {
     if (nsteps < 0) return MEL_E_ARG;
     return guarded(c, [&]() {
+        if (c->two_launch) {
+            // runs of steps up to and including the one after which the
+            // contact update falls due (the host keeps the same time sums as
+            // the device)
+            int k = 0;
+            while (k < nsteps) {
+                int n = 0;
+                double t = c->time;
+                bool due = false;
+                while (k + n < nsteps) {
+                    t = t + c->d.dt;
+                    n++;
+                    if (t > c->next_contact_update - 1.e-6 * c->d.dt) { due = true; break; }
+                }
+                run_steps(*c, n);
+                c->time = t;
+                c->iteration += n;
+                c->outputs_stale = true;
+                k += n;
+                if (due) {
+                    // contact-update cadence, Code/main.cpp:191-196
+                    c->next_contact_update = c->next_contact_update + c->model.contact_update_period;
+                    if (c->timing && (c->timing_mask >> KT_PROXIMITY & 1u)) {
+                        int ev = c->ev_begin(KT_PROXIMITY, c->stream);
+                        update_proximity(*c);
+                        c->ev_end(ev, c->stream);
+                    } else {
+                        update_proximity(*c);
+                    }
+                }
+            }
+            HIP_OK(hipGetLastError());
+            return MEL_OK;
+        }
         for (int k = 0; k < nsteps; k++) {
             c->iteration++;
             euler_step(*c);

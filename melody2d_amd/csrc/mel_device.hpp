@@ -133,7 +133,9 @@ struct DevModel {
     int *chunk_body;                         // body of each row chunk
     int *chunk_cnt, *chunk_elem_off;         // candidates per row chunk, first element per row chunk
     int4 *hit_rec;                           // rows with candidates: chunk, offset, first segment, row
+    int *hit_cnt;                            // their candidate counts
     int *hit_n;
+    double4 *pb_segbox;                      // per border: box of its segments (union of its chunks' boxes)
     int *slot_old_list, *slot_old_cursor;
     int *c_count_old;                        // element count of Cold
     int *law_first;                          // first law matching a material pair by name (Initialize_CZM)

@@ -117,3 +117,26 @@ def test_two_launch_step_equals_launch_sequence(name, golden_dir, tmp_path, monk
     assert seq.kernel_timing()["step_forces"][1] == 0
     two.close()
     seq.close()
+
+
+@pytest.mark.parametrize("name", ["tiny_platen", "tiny_czm", "tiny_damped"])
+def test_graph_replays_equal_launches(name, golden_dir, tmp_path, monkeypatch):
+    """Runs of steps between contact updates replayed from captured graphs
+    give the same state as the same launches issued one by one
+    (MELODY2D_GRAPHS=0), replay after replay, with API reads and single steps
+    (eager launches) between the replays."""
+    model, _ = golden_dir(name)
+    print("case graphs", name, flush=True)
+    refs, _ = run_oracle(model, [0], str(tmp_path / "oracle"))
+    nn, nbr, sizes = layout_from_dump(refs[0])
+    gr = native.Context(model)
+    monkeypatch.setenv("MELODY2D_GRAPHS", "0")
+    ea = native.Context(model)
+    for chunk in (10, 1, 25, 10, 3, 40, 10):
+        for c in (gr, ea):
+            c.euler_steps(chunk)
+        a, b = device_state(gr, nn, nbr, sizes), device_state(ea, nn, nbr, sizes)
+        diff = [k for k in a if not np.array_equal(np.asarray(a[k]), np.asarray(b[k]))]
+        assert not diff, "after a run of %d: %s" % (chunk, diff[:8])
+    gr.close()
+    ea.close()
