@@ -1,0 +1,141 @@
+"""Size-independent checks on the full benchmark workload (1000 deformable
+grains + 4 rigid walls, ~209k nodes, ~16M support entries), through the
+C-ABI.  The oracle is too slow for exact comparison at this size, so these
+tests use properties the mechanics fixes:
+
+* the two-launch step and the launch sequence are bit-identical (across
+  contact updates);
+* a rigid translation of every grain gives no internal force (the MLS
+  derivatives sum to zero, F = I, the stress vanishes);
+* under a linear displacement field the internal forces of every grain sum
+  to zero (sum over the support of the derivatives is zero at every Gauss
+  point), while the forces themselves are of the size the strain implies;
+* the FP32-mixed internal forces are within 1e-5 relative of FP64 from the
+  same deformed state."""
+import os
+
+import numpy as np
+import pytest
+
+from melody2d_amd import native
+
+pytestmark = pytest.mark.gpu
+
+FIELDS_U = ["x_disp_param", "y_disp_param"]
+
+
+@pytest.fixture(scope="module")
+def workload(tmp_path_factory):
+    from melody2d_amd.preprocessor import c2
+    from melody2d_amd.preprocessor.binary import write_binary
+    model = c2(n_grains=1000)
+    path = str(tmp_path_factory.mktemp("c2") / "c2.bin")
+    write_binary(model, path)
+    rigid = [b.rigid for b in model.bodies]
+    counts = [b.n_nodes for b in model.bodies]
+    print("workload: %d bodies, %d nodes" % (len(counts), sum(counts)), flush=True)
+    return path, counts, rigid
+
+
+def _deformable(counts, rigid):
+    return [(b, n) for b, n in enumerate(counts) if not rigid[b]]
+
+
+def _set_field(ctx, counts, rigid, fn):
+    """fn(body, X, Y) -> (ux, uy) displacement parameters of every deformable body"""
+    for b, n in _deformable(counts, rigid):
+        X = ctx.node_field(b, "x_current", n) - ctx.node_field(b, "x_disp", n)
+        Y = ctx.node_field(b, "y_current", n) - ctx.node_field(b, "y_disp", n)
+        ux, uy = fn(b, X, Y)
+        ctx.set_node_field(b, "x_disp_param", ux)
+        ctx.set_node_field(b, "y_disp_param", uy)
+        ctx.set_node_field(b, "x_vel_param", np.zeros(n))
+        ctx.set_node_field(b, "y_vel_param", np.zeros(n))
+
+
+def _internal(ctx, counts, rigid):
+    return {b: (ctx.node_field(b, "x_internal_force", n), ctx.node_field(b, "y_internal_force", n))
+            for b, n in _deformable(counts, rigid)}
+
+
+def test_full_size_two_launch_equals_launch_sequence(workload, monkeypatch):
+    path, counts, rigid = workload
+    print("case full-size two-launch vs sequence", flush=True)
+    two = native.Context(path, binary=True)
+    monkeypatch.setenv("MELODY2D_FUSED_STEP", "0")
+    seq = native.Context(path, binary=True)
+    for c in (two, seq):
+        c.euler_steps(25)          # crosses two contact updates
+        c.synchronize()
+    assert two.get_info().n_contacts == seq.get_info().n_contacts
+    for b, n in enumerate(counts):
+        for f in ("x_current", "y_current", "x_disp_param", "y_vel_param", "x_internal_force", "y_contact_force"):
+            a, r = two.node_field(b, f, n), seq.node_field(b, f, n)
+            assert np.array_equal(a, r), (b, f, float(np.max(np.abs(a - r))))
+    two.close()
+    seq.close()
+
+
+def test_full_size_translation_gives_no_internal_force(workload):
+    path, counts, rigid = workload
+    print("case full-size translation", flush=True)
+    ctx = native.Context(path, binary=True)
+    # reference scale: 1e-4 uniaxial strain
+    _set_field(ctx, counts, rigid, lambda b, X, Y: (1e-4 * (X - X.mean()), np.zeros_like(Y)))
+    ctx.euler_steps(1)
+    ref = _internal(ctx, counts, rigid)
+    scale = max(float(np.max(np.abs(fx))) for fx, _ in ref.values())
+    assert scale > 0
+    _set_field(ctx, counts, rigid, lambda b, X, Y: (np.full_like(X, 3e-5), np.full_like(Y, -2e-5)))
+    ctx.euler_steps(1)
+    got = _internal(ctx, counts, rigid)
+    worst = max(float(max(np.max(np.abs(fx)), np.max(np.abs(fy)))) for fx, fy in got.values())
+    print("  translation: max |f_int| %.3e vs %.3e at strain 1e-4" % (worst, scale), flush=True)
+    assert worst <= 1e-7 * scale
+    ctx.close()
+
+
+def test_full_size_linear_field_forces_sum_to_zero(workload):
+    path, counts, rigid = workload
+    print("case full-size patch", flush=True)
+    ctx = native.Context(path, binary=True)
+    A = np.array([[2e-4, -5e-5], [3e-5, -1e-4]])
+    _set_field(ctx, counts, rigid, lambda b, X, Y: (A[0, 0] * X + A[0, 1] * Y, A[1, 0] * X + A[1, 1] * Y))
+    ctx.euler_steps(1)
+    got = _internal(ctx, counts, rigid)
+    worst = 0.0
+    for b, (fx, fy) in got.items():
+        mag = float(np.sum(np.abs(fx)) + np.sum(np.abs(fy)))
+        assert mag > 0, b
+        worst = max(worst, (abs(float(np.sum(fx))) + abs(float(np.sum(fy)))) / mag)
+    print("  linear field: worst |sum f| / sum |f| over grains %.3e" % worst, flush=True)
+    assert worst <= 1e-10
+    ctx.close()
+
+
+def test_full_size_fp32_mixed_forces_within_1e5(workload):
+    path, counts, rigid = workload
+    print("case full-size fp32-mixed", flush=True)
+    rng = np.random.default_rng(1234)
+    A = np.array([[2e-4, -5e-5], [3e-5, -1e-4]])
+    noise = {b: (rng.standard_normal(n) * 2e-7, rng.standard_normal(n) * 2e-7) for b, n in _deformable(counts, rigid)}
+
+    def field(b, X, Y):
+        return (A[0, 0] * X + A[0, 1] * Y + noise[b][0], A[1, 0] * X + A[1, 1] * Y + noise[b][1])
+
+    forces = []
+    for prec in (native.FP64, native.FP32_MIXED):
+        ctx = native.Context(path, binary=True, precision=prec)
+        _set_field(ctx, counts, rigid, field)
+        ctx.euler_steps(1)
+        forces.append(_internal(ctx, counts, rigid))
+        ctx.close()
+    num = den = 0.0
+    for b in forces[0]:
+        for k in range(2):
+            d = forces[1][b][k] - forces[0][b][k]
+            num += float(d @ d)
+            den += float(forces[0][b][k] @ forces[0][b][k])
+    err = (num / den) ** 0.5
+    print("  fp32-mixed vs fp64 internal force: %.3e" % err, flush=True)
+    assert err <= 1e-5
