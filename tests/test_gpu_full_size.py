@@ -12,17 +12,12 @@ tests use properties the mechanics fixes:
   point), while the forces themselves are of the size the strain implies;
 * the FP32-mixed internal forces are within 1e-5 relative of FP64 from the
   same deformed state."""
-import os
-
 import numpy as np
 import pytest
 
 from melody2d_amd import native
 
 pytestmark = pytest.mark.gpu
-
-FIELDS_U = ["x_disp_param", "y_disp_param"]
-
 
 @pytest.fixture(scope="module")
 def workload(tmp_path_factory):
