@@ -75,6 +75,7 @@ __global__ void k_prox_chunk_boxes(DevModel m) {
         const int nd = m.sl_node[s0 + k];
         const double x = m.xc[nd], y = m.yc[nd];
         nxmin = x - d; nxmax = x + d; nymin = y - d; nymax = y + d;
+        m.slot_xy[s0 + k] = make_double2(x, y);
     }
     if (on && k < ns - 1) {
         const int s = s0 + k;
@@ -169,6 +170,7 @@ __global__ void __launch_bounds__(NB_THREADS) k_prox_neighbours(DevModel m) {
                                         const int pos = start + ex + w;
                                         if (pos < m.nb_cap) {
                                             list[3 * pos] = b2; list[3 * pos + 1] = r2; list[3 * pos + 2] = sh;
+                                            m.nb_gm[(size_t)b1 * m.nb_cap + pos] = g2;
                                         }
                                     }
                                     w++;
@@ -211,11 +213,18 @@ __device__ inline int body_rows(const DevModel& m, int b) {
     return r;
 }
 
-// row chunks per body
+// row chunks per body, and the body's row layout for the counting pass:
+// (rows, slots, neighbours, self | borders << 1) and (first border, first slot)
 __global__ void k_prox_rows(DevModel m) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= m.n_bodies) return;
-    m.body_chunks[b] = (body_rows(m, b) + ROW_CHUNK - 1) / ROW_CHUNK;
+    const int rows = body_rows(m, b);
+    m.body_chunks[b] = (rows + ROW_CHUNK - 1) / ROW_CHUNK;
+    const int g0 = m.border_off[b];
+    const int nbr = m.body_nborders[b];
+    m.body_rowrec[b] = make_int4(rows, nbr > 0 ? body_slots(m, b) : 0, m.nb_count[b],
+                                 (self_enabled(m, b) ? 1 : 0) | (nbr << 1));
+    m.body_gs0[b] = make_int2(g0, nbr > 0 ? m.br_slot0[g0] : 0);
 }
 
 // first row chunk of every body and the body of every row chunk (one
@@ -231,6 +240,7 @@ __global__ void __launch_bounds__(1024) k_prox_chunk_offsets(DevModel m) {
 
 struct RowInfo {
     int body, self, borderS, k, jS, borderM;
+    int gS, slotS;     // slave border (global) and slot
 };
 
 // row `local` of body b
@@ -251,6 +261,8 @@ __device__ inline bool decode_local_row(const DevModel& m, int b, int local, Row
                 ri.k = q / n;
                 ri.jS = q - ri.k * n;
                 ri.borderM = -1;
+                ri.gS = g0 + r;
+                ri.slotS = s0 + pre + ri.jS;
                 return true;
             }
         }
@@ -268,10 +280,35 @@ __device__ inline bool decode_local_row(const DevModel& m, int b, int local, Row
             ri.borderS = r;
             ri.jS = q - pre;
             ri.k = -1;
+            ri.gS = g0 + r;
+            ri.slotS = s0 + q;
             return true;
         }
     }
     return false;
+}
+
+// row `local` of body b from its row records; bodies with one border
+// decode without further loads
+__device__ inline bool decode_row_rec(const DevModel& m, int b, int4 rr, int2 gs, int local, RowInfo& ri) {
+    if ((rr.w >> 1) != 1) return decode_local_row(m, b, local, ri);
+    const int ns = rr.y, nb = rr.z;
+    ri.body = b;
+    ri.borderS = 0;
+    ri.gS = gs.x;
+    if (local < nb * ns) {
+        ri.self = 0;
+        ri.k = local / ns;
+        ri.jS = local - ri.k * ns;
+        ri.borderM = -1;
+    } else {
+        ri.self = 1;
+        ri.k = -1;
+        ri.jS = local - nb * ns;
+        ri.borderM = 0;
+    }
+    ri.slotS = gs.y + ri.jS;
+    return true;
 }
 
 // Scans one row from master segment seg_start on.  WRITE == false counts the
@@ -282,20 +319,21 @@ constexpr int BOX_BATCH = 4;     // chunk boxes loaded together
 template <bool WRITE>
 __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_start, int* first, int limit) {
     int bS = ri.body;
-    int gS = m.border_off[bS] + ri.borderS;
-    int slotS = m.br_slot0[gS] + ri.jS;
+    int slotS = ri.slotS;
     int nodeS = m.sl_node[slotS];
-    double Xs = m.xc[nodeS], Ys = m.yc[nodeS];
+    const double2 xs = m.slot_xy[slotS];          // position of the slot's node (this update)
+    double Xs = xs.x, Ys = xs.y;
     double detect = m.body_detect[bS], cdist = m.body_contact[bS];
     double P = m.period;
-    int bM, rM, shM;
+    int bM, rM, shM, gM;
     if (!ri.self) {
         const int* nb = m.nb_list + ((size_t)bS * m.nb_cap + ri.k) * 3;
         bM = nb[0]; rM = nb[1]; shM = nb[2];
+        gM = m.nb_gm[(size_t)bS * m.nb_cap + ri.k];
     } else {
         bM = bS; rM = ri.borderM; shM = 0;
+        gM = ri.gS - ri.borderS + rM;
     }
-    int gM = m.border_off[bM] + rM;
     int nM = m.br_nslots[gM];
     int s0 = m.br_slot0[gM];
     double emass;
@@ -400,10 +438,12 @@ __global__ void __launch_bounds__(ROW_CHUNK) k_prox_count(DevModel m) {
     const int n_chunks = m.chunk_off[m.n_bodies];
     for (int c = blockIdx.x; c < n_chunks; c += gridDim.x) {
         const int b = m.chunk_body[c];
+        const int4 rr = m.body_rowrec[b];
+        const int2 gs = m.body_gs0[b];
         const int local = (c - m.chunk_off[b]) * ROW_CHUNK + threadIdx.x;
         int cnt = 0, first = 0;
         RowInfo ri;
-        if (local < body_rows(m, b) && decode_local_row(m, b, local, ri)) cnt = scan_row<false>(m, ri, 0, 0, &first, INT_MAX);
+        if (local < rr.x && decode_row_rec(m, b, rr, gs, local, ri)) cnt = scan_row<false>(m, ri, 0, 0, &first, INT_MAX);
         int tot;
         const int ex = wg_excl_scan(cnt, s_w, tot);
         if (threadIdx.x == 0) m.chunk_cnt[c] = tot;
@@ -439,7 +479,7 @@ __global__ void k_prox_write(DevModel m) {
         const int4 r = m.hit_rec[h];
         const int b = m.chunk_body[r.x];
         RowInfo ri;
-        if (!decode_local_row(m, b, r.w, ri)) continue;
+        if (!decode_row_rec(m, b, m.body_rowrec[b], m.body_gs0[b], r.w, ri)) continue;
         scan_row<true>(m, ri, m.chunk_elem_off[r.x] + r.y, r.z, nullptr, m.hit_cnt[h]);
     }
 }

@@ -1239,6 +1239,7 @@ void build(mel_ctx& c) {
     d.nb_cap = 256;
     d.nb_count = c.alloc<int>(NB);
     d.nb_list = c.alloc<int>((size_t)NB * d.nb_cap * 3);
+    d.nb_gm = c.alloc<int>((size_t)NB * d.nb_cap);
     {
         // narrow-phase row chunks: at most nb_cap neighbours per body
         long long ch = 0;
@@ -1255,6 +1256,9 @@ void build(mel_ctx& c) {
     d.chunk_off = c.alloc<int>(NB + 1);
     d.chunk_cnt = c.alloc<int>(c.max_row_chunks);
     d.chunk_body = c.alloc<int>(c.max_row_chunks);
+    d.body_rowrec = c.alloc<int4>(NB);
+    d.body_gs0 = c.alloc<int2>(NB);
+    d.slot_xy = c.alloc<double2>(NS);
     d.chunk_elem_off = c.alloc<int>((size_t)c.max_row_chunks + 1);
     d.hit_rec = c.alloc<int4>(d.c_cap);
     d.hit_cnt = c.alloc<int>(d.c_cap);

@@ -131,6 +131,10 @@ struct DevModel {
     int *nb_count, *nb_list;                 // per body: count, 3 ints per neighbour
     int *body_chunks, *chunk_off;            // narrow-phase row chunks per body, first chunk per body
     int *chunk_body;                         // body of each row chunk
+    int4 *body_rowrec;                       // per body: rows, slots, neighbours, self | borders << 1
+    int2 *body_gs0;                          // per body: first border, first slot
+    double2 *slot_xy;                        // per slot: its node's position at the last proximity update
+    int *nb_gm;                              // per neighbour entry: the neighbour border's global index
     int *chunk_cnt, *chunk_elem_off;         // candidates per row chunk, first element per row chunk
     int4 *hit_rec;                           // rows with candidates: chunk, offset, first segment, row
     int *hit_cnt;                            // their candidate counts
