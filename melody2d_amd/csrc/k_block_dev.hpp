@@ -163,24 +163,40 @@ struct Fp32 {
     typedef float real;
 };
 
+// a value that is the same in every lane of the wave, moved to a scalar
+// register: loops and branches on it (a wave's support size, its lane count)
+// are then scalar branches instead of per-entry lane masks, and entry
+// addresses are a scalar base plus the lane's 32-bit offset
+__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 // where a lane's entries live in the chunk's stream
 struct LaneMap {
     int valid;      // lane holds a Gauss point
     int lane;       // lane index within the chunk
-    int K;          // the wave's largest support: entries per lane
-    int stride;     // lanes in the wave
-    size_t base;    // entry 0 of this lane
+    int K;          // the wave's largest support: entries per lane (wave-uniform)
+    int stride;     // lanes in the wave (wave-uniform)
+    size_t wbase;   // entry 0 of the wave's lane 0 (wave-uniform)
+    unsigned lw;    // lane within the wave
 };
 
+// entry i of the lane (wbase + i * stride + lw)
+template <class T>
+__device__ __forceinline__ const T& lane_entry(const T* s, const LaneMap& L, int i) {
+    return (s + (L.wbase + (size_t)i * L.stride))[L.lw];
+}
+
 __device__ inline LaneMap lane_map(int4 meta, int t) {
-    const int w = t >> 6, lw = t & 63;
-    const int K0 = meta.w & 255, K1 = (meta.w >> 8) & 255, n0 = (meta.w >> 16) & 255, n1 = (meta.w >> 24) & 255;
+    const int w = wave_uniform(t >> 6);
+    const unsigned lw = (unsigned)t & 63u;
+    const int mw = wave_uniform(meta.w);
+    const int K0 = mw & 255, K1 = (mw >> 8) & 255, n0 = (mw >> 16) & 255, n1 = (mw >> 24) & 255;
     LaneMap L;
     L.stride = w ? n1 : n0;
     L.K = w ? K1 : K0;
-    L.valid = lw < L.stride;
-    L.lane = 64 * w + lw;
-    L.base = (size_t)meta.z + (w ? (size_t)K0 * n0 : 0) + lw;
+    L.valid = (int)lw < L.stride;
+    L.lane = 64 * w + (int)lw;
+    L.wbase = (size_t)(unsigned)wave_uniform(meta.z) + (w ? (size_t)K0 * n0 : 0);
+    L.lw = lw;
     return L;
 }
 
@@ -218,9 +234,8 @@ __device__ inline void load_lane(const BlockArrays& B, int4 meta, int2 po, int n
 #pragma unroll
         for (int i = 0; i < PR::KREG; i++) {
             if (i < L.K) {
-                size_t e = L.base + (size_t)i * L.stride;
-                R.f[i] = fxy[e];
-                R.pk[i] = B.be_pk[e];
+                R.f[i] = lane_entry(fxy, L, i);
+                R.pk[i] = lane_entry(B.be_pk, L, i);
             }
         }
     }
@@ -248,24 +263,20 @@ __device__ inline void gp_lane(const BlockArrays& B, const LaneMap& L, const Lan
         if (HALF + i < L.K) grad_add<MAT>(a, b, c, d, R.f[HALF + i], u[i]);
     // supports longer than the register window: straight from memory
     const f64x2* fxy = reinterpret_cast<const f64x2*>(B.be_fxy);
-    for (int i = KREG_ENTRIES; i < L.K; i++) {
-        size_t e = L.base + (size_t)i * L.stride;
-        grad_add<MAT>(a, b, c, d, fxy[e], uv[B.be_pk[e] & 0xFFFF]);
-    }
+    for (int i = KREG_ENTRIES; i < L.K; i++)
+        grad_add<MAT>(a, b, c, d, lane_entry(fxy, L, i), uv[lane_entry(B.be_pk, L, i) & 0xFFFF]);
     const double4 P = stress_terms<MAT>(Mu, K2, a, b, c, d);
 #pragma unroll
     for (int i = 0; i < KREG_ENTRIES; i++)
         if (i < L.K) s_cxy[R.pk[i] >> 16] = contribution(R.c0, R.f[i], P);
-    for (int i = KREG_ENTRIES; i < L.K; i++) {
-        size_t e = L.base + (size_t)i * L.stride;
-        s_cxy[B.be_pk[e] >> 16] = contribution(R.c0, fxy[e], P);
-    }
+    for (int i = KREG_ENTRIES; i < L.K; i++)
+        s_cxy[lane_entry(B.be_pk, L, i) >> 16] = contribution(R.c0, lane_entry(fxy, L, i), P);
     if (!single) {
 #pragma unroll
         for (int i = 0; i < KREG_ENTRIES; i++)
             if (i < L.K) s_creg[R.pk[i] >> 16] = (uint16_t)R.reg;
         for (int i = KREG_ENTRIES; i < L.K; i++)
-            s_creg[B.be_pk[L.base + (size_t)i * L.stride] >> 16] = (uint16_t)R.reg;
+            s_creg[lane_entry(B.be_pk, L, i) >> 16] = (uint16_t)R.reg;
     }
 }
 
@@ -289,25 +300,21 @@ __device__ inline void gp_lane32(const BlockArrays& B, const LaneMap& L, const L
     for (int i = 0; i < HALF; i++)
         if (HALF + i < L.K) grad_add32<MAT>(a, b, c, d, R.f[HALF + i], u[i]);
     const f32x2* fxy = reinterpret_cast<const f32x2*>(B.be_fxy32);
-    for (int i = KREG_ENTRIES_F32; i < L.K; i++) {
-        size_t e = L.base + (size_t)i * L.stride;
-        grad_add32<MAT>(a, b, c, d, fxy[e], uv[B.be_pk[e] & 0xFFFF]);
-    }
+    for (int i = KREG_ENTRIES_F32; i < L.K; i++)
+        grad_add32<MAT>(a, b, c, d, lane_entry(fxy, L, i), uv[lane_entry(B.be_pk, L, i) & 0xFFFF]);
     const float4 P = stress_terms32<MAT>(Mu, K2, a, b, c, d);
     const float c0 = (float)R.c0;
 #pragma unroll
     for (int i = 0; i < KREG_ENTRIES_F32; i++)
         if (i < L.K) s_cxy[R.pk[i] >> 16] = contribution32(c0, R.f[i], P);
-    for (int i = KREG_ENTRIES_F32; i < L.K; i++) {
-        size_t e = L.base + (size_t)i * L.stride;
-        s_cxy[B.be_pk[e] >> 16] = contribution32(c0, fxy[e], P);
-    }
+    for (int i = KREG_ENTRIES_F32; i < L.K; i++)
+        s_cxy[lane_entry(B.be_pk, L, i) >> 16] = contribution32(c0, lane_entry(fxy, L, i), P);
     if (!single) {
 #pragma unroll
         for (int i = 0; i < KREG_ENTRIES_F32; i++)
             if (i < L.K) s_creg[R.pk[i] >> 16] = (uint16_t)R.reg;
         for (int i = KREG_ENTRIES_F32; i < L.K; i++)
-            s_creg[B.be_pk[L.base + (size_t)i * L.stride] >> 16] = (uint16_t)R.reg;
+            s_creg[lane_entry(B.be_pk, L, i) >> 16] = (uint16_t)R.reg;
     }
 }
 

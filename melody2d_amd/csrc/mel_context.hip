@@ -447,25 +447,51 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
                     chunks.push_back(ch);
                 }
             } else {
-                // Gauss points dealt round-robin over the chunks: a node's
-                // entries spread evenly over the chunks, which balances the
-                // per-node gather (sum order: chunk by chunk, Gauss-point
-                // order inside a chunk)
+                // waves of 64 Gauss points of nearly equal support size, so
+                // that a lane pads little to its wave's largest support: the
+                // Gauss points by decreasing support size (equal sizes in
+                // round-robin order over the chunk count, which spreads a
+                // node's entries over the chunks and balances the per-node
+                // gather), cut into waves; a chunk pairs the heaviest
+                // remaining wave with the lightest one when both fit.  Sum
+                // order: chunk by chunk, Gauss-point order inside a chunk.
                 long long tot = 0;
                 for (int g : gps) tot += g_ptr[g + 1] - g_ptr[g];
-                int nch = (int)std::max<long long>({1, (tot + ECAP * 15 / 16 - 1) / (ECAP * 15 / 16),
-                                                    ((long long)gps.size() + CHCAP - 1) / CHCAP});
-                for (;; nch++) {
-                    chunks.assign(nch, {});
-                    std::vector<int> ne(nch, 0);
-                    bool ok = true;
-                    for (size_t a = 0; a < gps.size() && ok; a++) {
-                        int k = (int)(a % nch), g = gps[a];
-                        ne[k] += g_ptr[g + 1] - g_ptr[g];
-                        chunks[k].push_back(g);
-                        ok = ne[k] <= ECAP && (int)chunks[k].size() <= CHCAP;
+                const int nrr = (int)std::max<long long>({1, (tot + ECAP * 15 / 16 - 1) / (ECAP * 15 / 16),
+                                                          ((long long)gps.size() + CHCAP - 1) / CHCAP});
+                std::vector<int> ord(gps.size());
+                for (size_t a = 0; a < gps.size(); a++) ord[a] = (int)a;
+                std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+                    const int kx = g_ptr[gps[x] + 1] - g_ptr[gps[x]], ky = g_ptr[gps[y] + 1] - g_ptr[gps[y]];
+                    if (kx != ky) return kx > ky;
+                    return x % nrr < y % nrr;
+                });
+                std::vector<std::vector<int>> waves;
+                std::vector<int> wsum;
+                for (size_t a = 0; a < ord.size();) {
+                    // at most 64 lanes and ECAP entries per wave
+                    std::vector<int> wv;
+                    int s = 0;
+                    while (a < ord.size() && wv.size() < 64) {
+                        const int cnt = g_ptr[gps[ord[a]] + 1] - g_ptr[gps[ord[a]]];
+                        if (s + cnt > ECAP) break;
+                        wv.push_back(gps[ord[a++]]);
+                        s += cnt;
                     }
-                    if (ok) break;
+                    waves.push_back(wv);
+                    wsum.push_back(s);
+                }
+                int i = 0, j = (int)waves.size() - 1;
+                while (i <= j) {
+                    std::vector<int> ch = waves[i];
+                    if (i < j && wsum[i] + wsum[j] <= ECAP) {
+                        ch.insert(ch.end(), waves[j].begin(), waves[j].end());
+                        j--;
+                    }
+                    i++;
+                    // Gauss-point order inside the chunk
+                    std::sort(ch.begin(), ch.end());
+                    chunks.push_back(ch);
                 }
             }
             for (const std::vector<int>& chunk : chunks) {
