@@ -33,8 +33,22 @@ constexpr int KREG_ENTRIES = 20;                                    // support e
 constexpr int KREG_ENTRIES_F32 = 24;                                // the same for the FP32-mixed form
 constexpr int MAX_BLOCK_CHUNKS = 64;                                // chunks per node block
 
+// per node block, a header of four int4 read at once when the block starts
+// (everything the internal-force pass needs before its first loads):
+//   [0] first owned node, owned nodes, touched nodes, touched-node list offset
+//       (-1: the touched nodes are the owned ones, first owned node on)
+//   [1] chunk offset, chunks, flags (BLK_*), owned-node list offset
+//   [2] descriptor of the first chunk (ch_meta)
+//   [3] slot counts of the first chunk (ch_ptr_off), 0, 0
+constexpr int BLK_SINGLE = 1;          // every Gauss point of the block in one region
+constexpr int BLK_INACTIVE = 2;        // body status 1: no update
+constexpr int BLK_NEO_HOOKEAN = 4;     // material law (else linear)
+constexpr int BLK_OWN_CONTIGUOUS = 8;  // owned nodes are first owned node + 0, 1, ...
+
 struct BlockArrays {
     int n_blocks;
+    int4* blk_hdr;             // 4 per block, see above
+    double2* blk_par;          // per block: (Mu, K2) of its material
     int max_touch, max_own;
     int *blk_body, *blk_own_off, *blk_nown, *blk_touch_off, *blk_ntouch, *blk_chunk_off, *blk_nchunk;
     int* blk_single_region;    // 1 when every Gauss point of the block is in one region

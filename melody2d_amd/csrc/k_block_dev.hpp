@@ -334,39 +334,44 @@ __device__ __forceinline__ void block_force_update(DevModel& m, const BlockArray
     __shared__ int4 s_meta[MAX_BLOCK_CHUNKS];
     __shared__ int2 s_po[MAX_BLOCK_CHUNKS];
     const int t = threadIdx.x;
-    const int body = B.blk_body[blk];
-    if (m.body_status[body] == 1) return;   // inactive body: no update (uniform over the block)
-    const bool single = B.blk_single_region[blk] != 0;
-    const int nown = B.blk_nown[blk];
-    const int ntouch = B.blk_ntouch[blk];
-    const int* own = B.own_nodes + B.blk_own_off[blk];
-    const int* touch = B.touch_nodes + B.blk_touch_off[blk];
-    const int mat = m.body_material[body];
-    const int mtype = m.mat_type[mat];
-    const double Mu = m.mat_p[6 * mat + 3], K2 = m.mat_p[6 * mat + 4];
+    // the block's header: one round trip before the first chunk's loads
+    const int4 h0 = B.blk_hdr[4 * blk], h1 = B.blk_hdr[4 * blk + 1], meta0 = B.blk_hdr[4 * blk + 2];
+    const int4 h3 = B.blk_hdr[4 * blk + 3];
+    const double2 par = B.blk_par[blk];
+    const int flags = h1.z;
+    if (flags & BLK_INACTIVE) return;   // inactive body: no update (uniform over the block)
+    const bool single = (flags & BLK_SINGLE) != 0;
+    const int own0 = h0.x, nown = h0.y, ntouch = h0.z;
+    const bool own_contig = (flags & BLK_OWN_CONTIGUOUS) != 0;
+    const int* own = B.own_nodes + h1.w;
+    const int mtype = (flags & BLK_NEO_HOOKEAN) ? 1 : 0;
+    const double Mu = par.x, K2 = par.y;
     disp_t* s_uv = reinterpret_cast<disp_t*>(s_dyn);
     uint16_t* s_creg = reinterpret_cast<uint16_t*>(s_dyn + sizeof(disp_t) * B.max_touch);
 
-    const int nch = B.blk_nchunk[blk];
-    const int4* gmeta = B.ch_meta + B.blk_chunk_off[blk];
-    const int2* gpo = B.ch_ptr_off + B.blk_chunk_off[blk];
+    const int nch = h1.y;
+    const int4* gmeta = B.ch_meta + h1.x;
+    const int2* gpo = B.ch_ptr_off + h1.x;
     LaneRegs<PR> R;
-    if (nch > 0) load_lane<PR>(B, gmeta[0], gpo[0], nown, t, R);
+    if (nch > 0) load_lane<PR>(B, meta0, make_int2(h3.x, h3.y), nown, t, R);
     // chunk descriptors in LDS: read without waiting on the entry stream
     for (int i = t; i < nch; i += BLOCK_THREADS) {
         s_meta[i] = gmeta[i];
         s_po[i] = gpo[i];
     }
+    // displacement parameters of the touched nodes (a contiguous range when
+    // they are the owned nodes: no index load first)
+    const int* touch = h0.w >= 0 ? B.touch_nodes + h0.w : nullptr;
     if constexpr (PR::f32) {
         // FP32 displacements relative to the block's first owned node
-        const double ax = nown > 0 ? m.upx[own[0]] : 0., ay = nown > 0 ? m.upy[own[0]] : 0.;
+        const double ax = nown > 0 ? m.upx[own0] : 0., ay = nown > 0 ? m.upy[own0] : 0.;
         for (int i = t; i < ntouch; i += BLOCK_THREADS) {
-            int g = touch[i];
+            int g = touch ? touch[i] : own0 + i;
             s_uv[i] = make_float2((float)(m.upx[g] - ax), (float)(m.upy[g] - ay));
         }
     } else {
         for (int i = t; i < ntouch; i += BLOCK_THREADS) {
-            int g = touch[i];
+            int g = touch ? touch[i] : own0 + i;
             s_uv[i] = make_double2(m.upx[g], m.upy[g]);
         }
     }
@@ -446,7 +451,7 @@ __device__ __forceinline__ void block_force_update(DevModel& m, const BlockArray
     for (int q = 0; q < NODES_PER_THREAD; q++) {
         int o = t + q * BLOCK_THREADS;
         if (o >= nown) break;
-        int j = own[o];
+        int j = own_contig ? own0 + o : own[o];
         if (cur[q] >= 0) { tx[q] += rx[q]; ty[q] += ry[q]; }
         m.fix[j] = tx[q];
         m.fiy[j] = ty[q];

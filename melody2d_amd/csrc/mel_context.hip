@@ -493,6 +493,24 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
                     std::sort(ch.begin(), ch.end());
                     chunks.push_back(ch);
                 }
+                // a block's time grows with its chunk count: when the pairing
+                // needs more chunks than dealing the Gauss points round-robin
+                // over the chunks, deal them (more padding, fewer chunks)
+                for (int nch = nrr; nch < (int)chunks.size(); nch++) {
+                    std::vector<std::vector<int>> rr(nch);
+                    std::vector<int> ne(nch, 0);
+                    bool ok = true;
+                    for (size_t a = 0; a < gps.size() && ok; a++) {
+                        const int k = (int)(a % nch), g = gps[a];
+                        ne[k] += g_ptr[g + 1] - g_ptr[g];
+                        rr[k].push_back(g);
+                        ok = ne[k] <= ECAP && (int)rr[k].size() <= CHCAP;
+                    }
+                    if (ok) {
+                        chunks = rr;
+                        break;
+                    }
+                }
             }
             for (const std::vector<int>& chunk : chunks) {
                 // lanes: the chunk's Gauss points by decreasing support size
@@ -637,7 +655,31 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
             for (int i = 0; i < nbn; i++) bn_local[bl[i]] = -1;
         }
     }
+    // block headers (what a block reads before its first loads, in one go)
+    std::vector<int4> blk_hdr;
+    std::vector<double2> blk_par;
+    for (size_t k = 0; k < blk_body.size(); k++) {
+        const Body& Bd = m.bodies[blk_body[k]];
+        const int* own = own_nodes.data() + blk_own_off[k];
+        const int* touch = touch_nodes.data() + blk_touch_off[k];
+        const int nown = blk_nown[k], ntouch = blk_ntouch[k], own0 = nown > 0 ? own[0] : 0;
+        bool own_contig = true, touch_contig = true;
+        for (int o = 0; o < nown; o++) own_contig = own_contig && own[o] == own0 + o;
+        for (int i = 0; i < ntouch; i++) touch_contig = touch_contig && touch[i] == own0 + i;
+        const int flags = (blk_single[k] ? BLK_SINGLE : 0) | (Bd.status == 1 ? BLK_INACTIVE : 0) |
+                          (m.materials[Bd.material].type != 0 ? BLK_NEO_HOOKEAN : 0) |
+                          (own_contig ? BLK_OWN_CONTIGUOUS : 0);
+        blk_hdr.push_back(make_int4(own0, nown, ntouch, touch_contig ? -1 : blk_touch_off[k]));
+        blk_hdr.push_back(make_int4(blk_chunk_off[k], blk_nchunk[k], flags, blk_own_off[k]));
+        const bool any = blk_nchunk[k] > 0;
+        blk_hdr.push_back(any ? ch_meta[blk_chunk_off[k]] : make_int4(0, 0, 0, 0));
+        const int2 po = any ? ch_ptr_off[blk_chunk_off[k]] : make_int2(0, 0);
+        blk_hdr.push_back(make_int4(po.x, po.y, 0, 0));
+        blk_par.push_back(make_double2(m.materials[Bd.material].p[3], m.materials[Bd.material].p[4]));
+    }
     BlockArrays& B = c.blocks_dev;
+    B.blk_hdr = c.upload(blk_hdr);
+    B.blk_par = c.upload(blk_par);
     B.bt_lbn = c.upload(bt_lbn);
     B.lds_finish_ok = lds_ok ? 1 : 0;
     B.n_blocks = (int)blk_body.size();
