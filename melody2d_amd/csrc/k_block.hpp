@@ -32,6 +32,9 @@ constexpr int CHUNK_ENTRIES = 2048;                                 // owned-nod
 constexpr int KREG_ENTRIES = 20;                                    // support entries held in registers per lane (FP64)
 constexpr int KREG_ENTRIES_F32 = 24;                                // the same for the FP32-mixed form
 constexpr int MAX_BLOCK_CHUNKS = 64;                                // chunks per node block
+// LDS scratch (doubles) of a workgroup that steps one rigid body
+// (rigid_body_wg): reductions, the body's record, its driven DOFs
+constexpr int RIGID_WG_LDS = 6 * BLOCK_THREADS + 8 + RIGID_FIELDS + 8;
 
 // per node block, a header of four int4 read at once when the block starts
 // (everything the internal-force pass needs before its first loads):

@@ -5,34 +5,38 @@
 // Launch A, workgroups by index range, all of BLOCK_THREADS threads:
 //   [0, n_cwg)         contact elements (Update_contacts +
 //                      Update_contact_forces, point-force records into the
-//                      border nodes' buckets) and BC point forces of
-//                      deformable border nodes (Update_bc_forces); signal [0]
-//   next n_rigid       one rigid body each: wait [0]; resultants,
-//                      Sum_up_forces, Apply_Newton, Apply_Euler; signal [1]
+//                      border nodes' buckets and the rigid bodies' records)
+//                      and BC point forces of deformable border nodes
+//                      (Update_bc_forces)
 //   next n_blocks      one node block each: internal forces
 //                      (Update_internal_forces), which depend on nothing else
 //                      of the step
-//   last               wait [0] and [1] (the readers of the BC values and of
-//                      the time); Update_bc(t + dt), time = t + dt; clear the
-//                      counters for the next launch
-// Launch B, one workgroup per node block: bucket sums (the contact sends),
-// Sum_up_forces + Apply_Newton + Apply_Euler with works, Update_borders and
-// the border-node kinematics, with the block's point forces and new
-// parameters staged in LDS; beside them, one thread per rigid-body node: its
-// new position and velocity (Update_kinematics of rigid bodies), which only
-// the next step reads.
+// No workgroup of launch A waits for another.
+// Launch B:
+//   [0, n_rigid)       one rigid body each: resultants of its contact records
+//                      and BCs, Sum_up_forces, Apply_Newton, Apply_Euler;
+//                      signal [1]
+//   next n_blocks      one node block each: bucket sums (the contact sends),
+//                      Sum_up_forces + Apply_Newton + Apply_Euler with works,
+//                      Update_borders and the border-node kinematics, with
+//                      the block's point forces and new parameters staged in
+//                      LDS
+//   next               one thread per rigid-body node: wait [1]; its new
+//                      position and velocity (Update_kinematics of rigid
+//                      bodies), which only the next step reads
+//   last               wait [1] (the rigid bodies read the BC values and the
+//                      time); Update_bc(t + dt), time = t + dt
 //
 // A workgroup only ever waits for workgroups of lower index, which the
-// hardware dispatches first, so every wait ends; the internal-force blocks
-// never wait.  A wait is also bounded in time: one that runs out sets bit 16
-// of the status flags (reported to the host as an error) instead of hanging.
-// The counters (DevModel::step_sync) start at zero and are zero again when
-// launch A ends: the last workgroup clears them once the rigid workgroups,
-// the only other waiters, have signalled.  Publication: every thread fences
-// its writes at device scope, the workgroup meets at a barrier, then one
-// thread increments the counter.  A waiting workgroup polls with one thread,
-// meets at a barrier, then every thread fences (acquire, device scope)
-// before it reads what the producers wrote.
+// hardware dispatches first, so every wait ends; a wait is also bounded in
+// time: one that runs out sets bit 16 of the status flags (reported to the
+// host as an error) instead of hanging.  The counter (DevModel::step_sync[1])
+// is zero when launch B starts: the first thread of launch A clears it, when
+// every waiter of the previous launch B has passed.  Publication: every
+// thread fences its writes at device scope, the workgroup meets at a
+// barrier, then one thread increments the counter.  A waiting workgroup
+// polls with one thread, meets at a barrier, then every thread fences
+// (acquire, device scope) before it reads what the producers wrote.
 #include <hip/hip_runtime.h>
 #include "mel_device.hpp"
 #include "k_block.hpp"
@@ -70,44 +74,19 @@ __device__ inline void wg_wait(const DevModel& m, unsigned* ctr, unsigned target
 template <class PR>
 __device__ __forceinline__ void step_forces(DevModel& m, const BlockArrays& B, const StepLaunch& S,
                                             typename PR::slot* s_cxy) {
-    unsigned* sync = m.step_sync;
     const int t = threadIdx.x;
     int w = blockIdx.x;
     if (w < S.n_cwg) {
+        // launch B of the previous step has ended: its counter starts again at zero
+        if (w == 0 && t == 0) __hip_atomic_store(&m.step_sync[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int nc = *m.c_count;
         for (int i = w * BLOCK_THREADS + t; i < nc + B.n_bc_bn; i += S.n_cwg * BLOCK_THREADS)
             contact_or_bc_item(m, i, nc, B.bc_bn, B.n_bc_bn);
-        wg_signal(&sync[0]);
         return;
     }
     w -= S.n_cwg;
-    if (w < m.n_rigid) {
-        wg_wait(m, &sync[0], S.n_cwg);
-        const RigidForces F = rigid_forces_block(m, w, reinterpret_cast<double*>(s_cxy));
-        const int b = m.rigid_ids[w];
-        if (t == 0) rigid_step_body(m, b, F);
-        wg_signal(&sync[1]);
-        return;
-    }
-    w -= m.n_rigid;
-    if (w < B.n_blocks) {
-        double fi[2][NODES_PER_THREAD];
-        block_force_update<PR>(m, B, w, s_cxy, fi);
-        return;
-    }
-    // last workgroup: the time of the end of the step from the device's copy
-    // of the time (the same sum the host forms), so that a launch does not
-    // carry it and a captured run of steps can be replayed
-    wg_wait(m, &sync[0], S.n_cwg);
-    wg_wait(m, &sync[1], m.n_rigid);
-    const double t_next = *m.time + m.dt;
-    __syncthreads();           // every thread has read the time before it is replaced
-    for (int k = t; k < m.n_bc_timed; k += BLOCK_THREADS) bc_eval_at(m, m.bc_timed[k], t_next);
-    if (t == 0) {
-        *m.time = t_next;
-        __hip_atomic_store(&sync[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&sync[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    double fi[2][NODES_PER_THREAD];
+    block_force_update<PR>(m, B, w, s_cxy, fi);
 }
 
 }  // namespace
@@ -126,22 +105,46 @@ k_step_forces32(DevModel m, BlockArrays B, StepLaunch S) {
 
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_finish(DevModel m, BlockArrays B) {
     extern __shared__ __align__(16) double s_fin[];
+    unsigned* sync = m.step_sync;
     const int t = threadIdx.x;
-    const int blk = blockIdx.x;
-    if (blk >= B.n_blocks) {
-        const int q = (blk - B.n_blocks) * BLOCK_THREADS + t;
-        if (q < m.n_rigid_nodes) rigid_node(m, m.rigid_node_ids[q], 1);
+    int w = blockIdx.x;
+    if (w < m.n_rigid) {
+        rigid_body_wg(m, w, s_fin);
+        wg_signal(&sync[1]);
         return;
     }
-    double fi[2][NODES_PER_THREAD];
-    const int n0 = B.own_nodes[B.blk_own_off[blk]], nown = B.blk_nown[blk];
+    w -= m.n_rigid;
+    if (w < B.n_blocks) {
+        double fi[2][NODES_PER_THREAD];
+        const int n0 = B.own_nodes[B.blk_own_off[w]], nown = B.blk_nown[w];
 #pragma unroll
-    for (int q = 0; q < NODES_PER_THREAD; q++) {
-        const int o = t + q * BLOCK_THREADS;
-        fi[0][q] = o < nown ? m.fix[n0 + o] : 0.;
-        fi[1][q] = o < nown ? m.fiy[n0 + o] : 0.;
+        for (int q = 0; q < NODES_PER_THREAD; q++) {
+            const int o = t + q * BLOCK_THREADS;
+            fi[0][q] = o < nown ? m.fix[n0 + o] : 0.;
+            fi[1][q] = o < nown ? m.fiy[n0 + o] : 0.;
+        }
+        node_block_finish_lds(m, B, w, fi, s_fin);
+        return;
     }
-    node_block_finish_lds(m, B, blk, fi, s_fin);
+    w -= B.n_blocks;
+    const int n_rnwg = (m.n_rigid_nodes + BLOCK_THREADS - 1) / BLOCK_THREADS;
+    if (w < n_rnwg) {
+        // the node's own data is loaded before the wait for its body
+        const int q = w * BLOCK_THREADS + t;
+        RigidNodePre p{};
+        if (q < m.n_rigid_nodes) p = rigid_node_pre(m, m.rigid_node_ids[q]);
+        wg_wait(m, &sync[1], m.n_rigid);
+        rigid_node_apply(m, p, 1);
+        return;
+    }
+    // last workgroup: the time of the end of the step from the device's copy
+    // of the time (the same sum the host forms), so that a launch does not
+    // carry it and a captured run of steps can be replayed
+    wg_wait(m, &sync[1], m.n_rigid);
+    const double t_next = *m.time + m.dt;
+    __syncthreads();           // every thread has read the time before it is replaced
+    for (int k = t; k < m.n_bc_timed; k += BLOCK_THREADS) bc_eval_at(m, m.bc_timed[k], t_next);
+    if (t == 0) *m.time = t_next;
 }
 
 }  // namespace mel

@@ -10,20 +10,24 @@ struct StepLaunch {
                      // element count on the device)
 };
 
-// workgroups of launch A
+// workgroups of launch A: contact elements and BC point forces, node blocks
 inline int step_forces_grid(const DevModel& d, const BlockArrays& B, const StepLaunch& S) {
-    return S.n_cwg + d.n_rigid + B.n_blocks + 1;
+    (void)d;
+    return S.n_cwg + B.n_blocks;
 }
 
-// workgroups of launch B: node blocks, then one thread per rigid-body node
+// workgroups of launch B: rigid bodies, node blocks, one thread per
+// rigid-body node, the BC update
 inline int step_finish_grid(const DevModel& d, const BlockArrays& B) {
-    return B.n_blocks + (d.n_rigid_nodes + BLOCK_THREADS - 1) / BLOCK_THREADS;
+    return d.n_rigid + B.n_blocks + (d.n_rigid_nodes + BLOCK_THREADS - 1) / BLOCK_THREADS + 1;
 }
 
 // dynamic LDS of k_step_finish: a node block's staged point forces and
-// parameters
+// parameters, or a rigid body's reduction scratch
 inline size_t step_finish_lds(int max_bn, int max_own) {
-    return 6 * (size_t)(max_bn + max_own) * sizeof(double);
+    const size_t rigid = (size_t)RIGID_WG_LDS * sizeof(double);
+    const size_t blk = 6 * (size_t)(max_bn + max_own) * sizeof(double);
+    return blk > rigid ? blk : rigid;
 }
 
 __global__ void k_step_forces(DevModel m, BlockArrays B, StepLaunch S);

@@ -77,11 +77,10 @@ __global__ void k_step_contacts(DevModel m, const int* bc_bn, int n_bc) {
 }
 
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_gather(DevModel m, const int* split_bn, int n_split) {
-    __shared__ double sc[6 * BLOCK_THREADS + 8];
+    __shared__ double sc[RIGID_WG_LDS];
     if ((int)blockIdx.x < m.n_rigid) {
         // resultants, then Newton + Euler of the same body by thread 0
-        const RigidForces F = rigid_forces_block(m, blockIdx.x, sc);
-        if (threadIdx.x == 0) rigid_step_body(m, m.rigid_ids[blockIdx.x], F);
+        rigid_body_wg(m, blockIdx.x, sc);
         return;
     }
     // border nodes of bodies split over several node blocks: bucket sums and

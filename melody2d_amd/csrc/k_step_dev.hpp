@@ -348,9 +348,16 @@ __device__ inline RigidForces rigid_forces_block(const DevModel& m, int rigid_in
     int tid = threadIdx.x;
     double sx = 0., sy = 0., sm = 0.;
     int nc = *m.c_count;
+    // every element's records are loaded without waiting on its bodies (an
+    // element that does not name the body adds +0, which leaves the sums
+    // unchanged: they start at +0 and never become -0)
+#pragma unroll 4
     for (int e = tid; e < nc; e += blockDim.x) {
-        if (m.C.bodyS[e] == b) { sx += m.rig_rec[6 * e]; sy += m.rig_rec[6 * e + 1]; sm += m.rig_rec[6 * e + 2]; }
-        if (m.C.bodyM[e] == b) { sx += m.rig_rec[6 * e + 3]; sy += m.rig_rec[6 * e + 4]; sm += m.rig_rec[6 * e + 5]; }
+        const int bs = m.C.bodyS[e], bm = m.C.bodyM[e];
+        const double* r = m.rig_rec + 6 * e;
+        const double r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
+        sx += bs == b ? r0 : 0.; sy += bs == b ? r1 : 0.; sm += bs == b ? r2 : 0.;
+        sx += bm == b ? r3 : 0.; sy += bm == b ? r4 : 0.; sm += bm == b ? r5 : 0.;
     }
     red[3 * tid] = sx; red[3 * tid + 1] = sy; red[3 * tid + 2] = sm;
     __syncthreads();
@@ -366,11 +373,23 @@ __device__ inline RigidForces rigid_forces_block(const DevModel& m, int rigid_in
     double acc[6] = {0., 0., 0., 0., 0., 0.};
     const double xc0 = R[R_CUR], yc0 = R[R_CUR + 1];
     const int gb0 = m.border_off[b];
-    for (int k = 0; k < m.body_nborders[b]; k++) {
+    const int nbr = m.body_nborders[b];
+    // only types 2 (spring-dashpot), 3 and 4 (pressure) give forces; the
+    // others add zeros, which leave the sums unchanged.  The borders' types
+    // are read at once, one thread per border, into a mask.
+    int* s_mask = reinterpret_cast<int*>(sc + 6 * nt + 6);
+    if (tid == 0) *s_mask = 0;
+    __syncthreads();
+    for (int k = tid; k < nbr; k += nt) {
+        const int xt = m.br_xtype[gb0 + k], yt = m.br_ytype[gb0 + k];
+        if (k < 32 && (xt == 2 || xt == 3 || xt == 4 || yt == 2 || yt == 3 || yt == 4)) atomicOr(s_mask, 1 << k);
+    }
+    __syncthreads();
+    const int mask = *s_mask;
+    for (int k = 0; k < nbr; k++) {
+        if (k < 32 && !(mask >> k & 1)) continue;
         const int br = gb0 + k;
         const int xt = m.br_xtype[br], yt = m.br_ytype[br];
-        // only types 2 (spring-dashpot), 3 and 4 (pressure) give forces; the
-        // others add zeros, which leave the sums unchanged
         const bool fx_bc = xt == 2 || xt == 3 || xt == 4, fy_bc = yt == 2 || yt == 3 || yt == 4;
         if (!fx_bc && !fy_bc) continue;
         const int ns = m.br_nslots[br];
@@ -393,7 +412,9 @@ __device__ inline RigidForces rigid_forces_block(const DevModel& m, int rigid_in
         for (int q = 0; q < 6; q++) part[6 * tid + q] = p6[q];
         __syncthreads();
         if (tid < 6) {
+            // runs in slot order; the loads of a batch of runs issue together
             double a = acc[tid];
+#pragma unroll 16
             for (int r = 0; r < nt; r++) a += part[6 * r + tid];
             acc[tid] = a;
         }
@@ -440,12 +461,15 @@ __device__ inline void rigid_driven_dofs(const DevModel& m, int b, double t, dou
 // F: the body's forces from rigid_forces_block.  The body record is worked
 // on in registers and stored back once (the contact work slot, which
 // contact elements add to atomically, is left alone).
-__device__ inline void rigid_step_body(const DevModel& m, int b, const RigidForces& F) {
+// rec: the body's record as it was when the step started (RIGID_FIELDS
+// values), dvt: its driven DOFs at the end of the step (rigid_driven_dofs).
+__device__ inline void rigid_step_body(const DevModel& m, int b, const RigidForces& F, const double* rec,
+                                       const double* dvt) {
     if (m.body_status[b] == 1) return;
     double* Rg = m.body_rigid + b * RIGID_FIELDS;
     double R[RIGID_FIELDS];
 #pragma unroll
-    for (int k = 0; k < RIGID_FIELDS; k++) R[k] = Rg[k];
+    for (int k = 0; k < RIGID_FIELDS; k++) R[k] = rec[k];
     for (int q = 0; q < 3; q++) { R[R_FCON + q] = F.fcon[q]; R[R_FNEU + q] = F.fneu[q]; R[R_FDIR + q] = F.fdir[q]; }
     R[R_FBODY] = F.fbody[0];
     R[R_FBODY + 1] = F.fbody[1];
@@ -459,7 +483,8 @@ __device__ inline void rigid_step_body(const DevModel& m, int b, const RigidForc
     R[R_NODE_ACC] = nacc_x; R[R_NODE_ACC + 1] = nacc_y;   // node copies of Body::Apply_Newton (Code/Body.h:2038-2042)
     R[R_FDIR_NODE] = R[R_FDIR]; R[R_FDIR_NODE + 1] = R[R_FDIR + 1];   // copied to nodes by Code/Body.h:2005-2006
     double* dv = R + R_DRIVEN;
-    rigid_driven_dofs(m, b, *m.time + dt, dv);
+#pragma unroll
+    for (int k = 0; k < 8; k++) dv[k] = dvt[k];
     double hx = R[R_VEL] + 0.5 * dt * R[R_ACC];
     double hy = R[R_VEL + 1] + 0.5 * dt * R[R_ACC + 1];
     double hr = R[R_VEL + 2] + 0.5 * dt * R[R_ACC + 2];
@@ -497,6 +522,28 @@ __device__ inline void rigid_step_body(const DevModel& m, int b, const RigidForc
         if (k != R_WORK + W_CONTACT) Rg[k] = R[k];
 }
 
+// One rigid body by a workgroup: its record and driven DOFs (which do not
+// depend on this step's forces) are staged in LDS while the resultants are
+// formed (rigid_forces_block), then thread 0 steps the body.  sc: LDS
+// scratch of RIGID_WG_LDS doubles.
+
+__device__ inline void rigid_body_wg(const DevModel& m, int rigid_index, double* sc) {
+    const int b = m.rigid_ids[rigid_index];
+    double* s_rec = sc + 6 * BLOCK_THREADS + 8;
+    double* s_dv = s_rec + RIGID_FIELDS;
+    const double* Rg = m.body_rigid + b * RIGID_FIELDS;
+    for (int k = threadIdx.x; k < RIGID_FIELDS; k += blockDim.x) s_rec[k] = Rg[k];
+    if (threadIdx.x == blockDim.x - 1) {
+        double dv[8];
+        for (int k = 0; k < 8; k++) dv[k] = Rg[R_DRIVEN + k];
+        rigid_driven_dofs(m, b, *m.time + m.dt, dv);
+        for (int k = 0; k < 8; k++) s_dv[k] = dv[k];
+    }
+    const RigidForces F = rigid_forces_block(m, rigid_index, sc);   // its barriers publish s_rec, s_dv
+    __syncthreads();
+    if (threadIdx.x == 0) rigid_step_body(m, b, F, s_rec, s_dv);
+}
+
 // rigid node kinematics and positions (Code/Body.h:2214-2226, 2281-2301);
 // node force fields mirror the body totals as Code/Body.h:1999-2013 does.
 __device__ inline void rigid_node_at(const DevModel& m, int i, const double* R, double c, double s,
@@ -524,12 +571,33 @@ __device__ inline void rigid_node_at(const DevModel& m, int i, const double* R, 
     }
 }
 
-__device__ inline void rigid_node(const DevModel& m, int i, int with_forces) {
-    int b = m.node_body[i];
-    if (!m.body_rigid_flag[b] || m.body_status[b] == 1) return;
-    const double* R = m.body_rigid + b * RIGID_FIELDS;
+// what a rigid-body node needs besides its body's new state (loaded before
+// the body has stepped)
+struct RigidNodePre {
+    int i, b;
+    bool on;
+    double X, Y;
+};
+
+__device__ inline RigidNodePre rigid_node_pre(const DevModel& m, int i) {
+    RigidNodePre p;
+    p.i = i;
+    p.b = m.node_body[i];
+    p.on = m.body_rigid_flag[p.b] && m.body_status[p.b] != 1;
+    p.X = m.X[i];
+    p.Y = m.Y[i];
+    return p;
+}
+
+__device__ inline void rigid_node_apply(const DevModel& m, const RigidNodePre& p, int with_forces) {
+    if (!p.on) return;
+    const double* R = m.body_rigid + p.b * RIGID_FIELDS;
     double dr = R[R_DISP + 2] - 0.;
-    rigid_node_at(m, i, R, cos(dr), sin(dr), with_forces, m.X[i], m.Y[i]);
+    rigid_node_at(m, p.i, R, cos(dr), sin(dr), with_forces, p.X, p.Y);
+}
+
+__device__ inline void rigid_node(const DevModel& m, int i, int with_forces) {
+    rigid_node_apply(m, rigid_node_pre(m, i), with_forces);
 }
 
 // Initialize_contact_forces + Update_contacts + Update_contact_forces (records

@@ -145,8 +145,7 @@ struct DevModel {
     int *law_first;                          // first law matching a material pair by name (Initialize_CZM)
     int *status_flags;                       // [0] capacity overflow bits, [1] non-finite state seen
     double *time;                            // device copy of the simulation time
-    unsigned *step_sync;                     // step launch A: [0] contact workgroups done, [1] rigid
-                                             // workgroups done
+    unsigned *step_sync;                     // step launch B: [1] rigid-body workgroups done
 };
 
 // --------------------------------------------------------------------------
