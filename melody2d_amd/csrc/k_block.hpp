@@ -33,8 +33,8 @@ constexpr int KREG_ENTRIES = 20;                                    // support e
 constexpr int KREG_ENTRIES_F32 = 24;                                // the same for the FP32-mixed form
 constexpr int MAX_BLOCK_CHUNKS = 64;                                // chunks per node block
 // LDS scratch (doubles) of a workgroup that steps one rigid body
-// (rigid_body_wg): reductions, the body's record, its driven DOFs
-constexpr int RIGID_WG_LDS = 6 * BLOCK_THREADS + 8 + RIGID_FIELDS + 8;
+// (rigid_body_wg): reductions, the body's record, its driven DOFs, BC flags
+constexpr int RIGID_WG_LDS = 6 * BLOCK_THREADS + 8 + RIGID_FIELDS + 8 + 16;
 
 // per node block, a header of four int4 read at once when the block starts
 // (everything the internal-force pass needs before its first loads):
@@ -42,11 +42,13 @@ constexpr int RIGID_WG_LDS = 6 * BLOCK_THREADS + 8 + RIGID_FIELDS + 8;
 //       (-1: the touched nodes are the owned ones, first owned node on)
 //   [1] chunk offset, chunks, flags (BLK_*), owned-node list offset
 //   [2] descriptor of the first chunk (ch_meta)
-//   [3] slot counts of the first chunk (ch_ptr_off), 0, 0
+//   [3] slot counts of the first chunk (ch_ptr_off), border-node list offset,
+//       border nodes finished by the block
 constexpr int BLK_SINGLE = 1;          // every Gauss point of the block in one region
 constexpr int BLK_INACTIVE = 2;        // body status 1: no update
 constexpr int BLK_NEO_HOOKEAN = 4;     // material law (else linear)
 constexpr int BLK_OWN_CONTIGUOUS = 8;  // owned nodes are first owned node + 0, 1, ...
+constexpr int BLK_NO_GRAVITY = 16;     // body status 2: no body force
 
 struct BlockArrays {
     int n_blocks;
@@ -73,6 +75,13 @@ struct BlockArrays {
     int *bc_bn, n_bc_bn;                    // deformable border nodes on borders with a BC
     uint16_t* bt_lbn;          // per point-force spreading entry: its border node's index in the
                                // block's list (one-launch step only)
+    // per entry of bn_list, what the finish of its block reads first:
+    int4* fin_rec0;            // (border node, node, DOF_TO_DISP row start, row end)
+    int4* fin_rec1;            // (slot list start, slot list end, first slot, second slot)
+    double2* fin_recX;         // initial position of the node
+    // per slot, the inputs of its tributary length besides positions:
+    int4* sl_rec0;             // (stencil nodes 0, 1, 2, index j on the border)
+    int4* sl_rec1;             // (periodic shifts 0, 1, 2, slots of the border | 1 << 30 when simple)
     int lds_finish_ok;         // every block meets the one-launch step's finish preconditions
 };
 

@@ -7,15 +7,16 @@
 //                      Update_contact_forces, point-force records into the
 //                      border nodes' buckets and the rigid bodies' records)
 //                      and BC point forces of deformable border nodes
-//                      (Update_bc_forces)
+//                      (Update_bc_forces); the first also forms the rigid
+//                      bodies' driven DOFs at the end of the step
 //   next n_blocks      one node block each: internal forces
 //                      (Update_internal_forces), which depend on nothing else
 //                      of the step
 // No workgroup of launch A waits for another.
 // Launch B:
 //   [0, n_rigid)       one rigid body each: resultants of its contact records
-//                      and BCs, Sum_up_forces, Apply_Newton, Apply_Euler;
-//                      signal [1]
+//                      and BCs, Sum_up_forces, Apply_Newton, Apply_Euler
+//                      with the driven DOFs launch A formed; signal [1]
 //   next n_blocks      one node block each: bucket sums (the contact sends),
 //                      Sum_up_forces + Apply_Newton + Apply_Euler with works,
 //                      Update_borders and the border-node kinematics, with
@@ -79,6 +80,14 @@ __device__ __forceinline__ void step_forces(DevModel& m, const BlockArrays& B, c
     if (w < S.n_cwg) {
         // launch B of the previous step has ended: its counter starts again at zero
         if (w == 0 && t == 0) __hip_atomic_store(&m.step_sync[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the rigid bodies' driven DOFs at the end of the step (time and BC
+        // tables only), ready for launch B
+        if (w == 0)
+            for (int k = t; k < m.n_rigid; k += BLOCK_THREADS) {
+                double dv[8];
+                rigid_dv_end_of_step(m, k, dv);
+                for (int q = 0; q < 8; q++) m.rigid_dv[8 * k + q] = dv[q];
+            }
         const int nc = *m.c_count;
         for (int i = w * BLOCK_THREADS + t; i < nc + B.n_bc_bn; i += S.n_cwg * BLOCK_THREADS)
             contact_or_bc_item(m, i, nc, B.bc_bn, B.n_bc_bn);
@@ -103,20 +112,24 @@ k_step_forces32(DevModel m, BlockArrays B, StepLaunch S) {
     step_forces<blockdev::Fp32>(m, B, S, s_cxy);
 }
 
-__global__ void __launch_bounds__(BLOCK_THREADS) k_step_finish(DevModel m, BlockArrays B) {
+// two waves per SIMD (four workgroups per CU) keep a 1000-body model's
+// blocks resident at once
+__global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_step_finish(DevModel m, BlockArrays B) {
     extern __shared__ __align__(16) double s_fin[];
     unsigned* sync = m.step_sync;
     const int t = threadIdx.x;
     int w = blockIdx.x;
     if (w < m.n_rigid) {
-        rigid_body_wg(m, w, s_fin);
+        rigid_body_wg(m, w, s_fin, m.rigid_dv);
         wg_signal(&sync[1]);
         return;
     }
     w -= m.n_rigid;
     if (w < B.n_blocks) {
         double fi[2][NODES_PER_THREAD];
-        const int n0 = B.own_nodes[B.blk_own_off[w]], nown = B.blk_nown[w];
+        const int4 h0 = B.blk_hdr[4 * w];
+        const int n0 = h0.x, nown = h0.y;
 #pragma unroll
         for (int q = 0; q < NODES_PER_THREAD; q++) {
             const int o = t + q * BLOCK_THREADS;

@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_step_gather(DevModel m, const
     __shared__ double sc[RIGID_WG_LDS];
     if ((int)blockIdx.x < m.n_rigid) {
         // resultants, then Newton + Euler of the same body by thread 0
-        rigid_body_wg(m, blockIdx.x, sc);
+        rigid_body_wg(m, blockIdx.x, sc, nullptr);
         return;
     }
     // border nodes of bodies split over several node blocks: bucket sums and

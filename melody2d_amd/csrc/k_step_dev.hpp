@@ -339,26 +339,39 @@ struct RigidForces {
 
 __device__ inline RigidForces rigid_forces_block(const DevModel& m, int rigid_index, double* sc) {
     RigidForces F{};
-    int b = m.rigid_ids[rigid_index];
-    if (m.body_status[b] == 1) return F;
-    double* R = m.body_rigid + b * RIGID_FIELDS;
-    const int nt = blockDim.x;
+    const int b = m.rigid_ids[rigid_index];
+    const int nc = *m.c_count;
+    constexpr int nt = BLOCK_THREADS;   // workgroup size of every caller
+    const int tid = threadIdx.x;
     double* red = sc;              // [nt][3], then [nt][6] for the BC runs
     double* fsum = sc + 6 * nt;    // [6]
-    int tid = threadIdx.x;
+    int* s_bcf = reinterpret_cast<int*>(sc + 6 * nt + 8 + RIGID_FIELDS + 8);   // [32] border gives BC forces
+    // what depends only on the body, loaded with the contact records
+    const int status = m.body_status[b];
+    const int gb0 = m.border_off[b];
+    const int nbr = m.body_nborders[b];
+    const double* R = m.body_rigid + b * RIGID_FIELDS;
+    const double xc0 = R[R_CUR], yc0 = R[R_CUR + 1];
+    // only BC types 2 (spring-dashpot), 3 and 4 (pressure) give forces; the
+    // others add zeros, which leave the sums unchanged.  One thread per
+    // border reads its types.
+    for (int k = tid; k < nbr && k < 32; k += nt) {
+        const int xt = m.br_xtype[gb0 + k], yt = m.br_ytype[gb0 + k];
+        s_bcf[k] = xt == 2 || xt == 3 || xt == 4 || yt == 2 || yt == 3 || yt == 4;
+    }
     double sx = 0., sy = 0., sm = 0.;
-    int nc = *m.c_count;
     // every element's records are loaded without waiting on its bodies (an
     // element that does not name the body adds +0, which leaves the sums
     // unchanged: they start at +0 and never become -0)
-#pragma unroll 4
-    for (int e = tid; e < nc; e += blockDim.x) {
+#pragma unroll 8
+    for (int e = tid; e < nc; e += nt) {
         const int bs = m.C.bodyS[e], bm = m.C.bodyM[e];
         const double* r = m.rig_rec + 6 * e;
         const double r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
         sx += bs == b ? r0 : 0.; sy += bs == b ? r1 : 0.; sm += bs == b ? r2 : 0.;
         sx += bm == b ? r3 : 0.; sy += bm == b ? r4 : 0.; sm += bm == b ? r5 : 0.;
     }
+    if (status == 1) return F;     // inactive body (the same for the whole workgroup)
     red[3 * tid] = sx; red[3 * tid + 1] = sy; red[3 * tid + 2] = sm;
     __syncthreads();
     for (int w = nt / 2; w > 0; w >>= 1) {
@@ -371,23 +384,8 @@ __device__ inline RigidForces rigid_forces_block(const DevModel& m, int rigid_in
     // the reference's serial loop)
     double* part = sc;
     double acc[6] = {0., 0., 0., 0., 0., 0.};
-    const double xc0 = R[R_CUR], yc0 = R[R_CUR + 1];
-    const int gb0 = m.border_off[b];
-    const int nbr = m.body_nborders[b];
-    // only types 2 (spring-dashpot), 3 and 4 (pressure) give forces; the
-    // others add zeros, which leave the sums unchanged.  The borders' types
-    // are read at once, one thread per border, into a mask.
-    int* s_mask = reinterpret_cast<int*>(sc + 6 * nt + 6);
-    if (tid == 0) *s_mask = 0;
-    __syncthreads();
-    for (int k = tid; k < nbr; k += nt) {
-        const int xt = m.br_xtype[gb0 + k], yt = m.br_ytype[gb0 + k];
-        if (k < 32 && (xt == 2 || xt == 3 || xt == 4 || yt == 2 || yt == 3 || yt == 4)) atomicOr(s_mask, 1 << k);
-    }
-    __syncthreads();
-    const int mask = *s_mask;
     for (int k = 0; k < nbr; k++) {
-        if (k < 32 && !(mask >> k & 1)) continue;
+        if (k < 32 && !s_bcf[k]) continue;
         const int br = gb0 + k;
         const int xt = m.br_xtype[br], yt = m.br_ytype[br];
         const bool fx_bc = xt == 2 || xt == 3 || xt == 4, fy_bc = yt == 2 || yt == 3 || yt == 4;
@@ -395,18 +393,40 @@ __device__ inline RigidForces rigid_forces_block(const DevModel& m, int rigid_in
         const int ns = m.br_nslots[br];
         const int run = (ns + nt - 1) / nt;
         double p6[6] = {0., 0., 0., 0., 0., 0.};
-        for (int j = tid * run; j < min(ns, (tid + 1) * run); j++) {
-            int s = m.br_slot0[br] + j;
-            int node = m.sl_node[s];
-            double L = m.sl_len[s];
-            double fxn = 0., fyn = 0., fxd = 0., fyd = 0.;
-            if (xt == 3 || xt == 4) fxn = m.br_xval[br] * L;
-            else if (xt == 2) fxd = (m.br_xparam[2 * br] * (m.br_xval[br] - m.ux[node]) + m.br_xparam[2 * br + 1] * (m.br_xvel[br] - m.vx[node])) * L;
-            if (yt == 3 || yt == 4) fyn = m.br_yval[br] * L;
-            else if (yt == 2) fyd = (m.br_yparam[2 * br] * (m.br_yval[br] - m.uy[node]) + m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node])) * L;
-            double dx = m.xc[node] - xc0, dy = m.yc[node] - yc0;
-            p6[0] += fxn; p6[1] += fyn; p6[2] += -fxn * dy + fyn * dx;
-            p6[3] += fxd; p6[4] += fyd; p6[5] += -fxd * dy + fyd * dx;
+        const int sb = m.br_slot0[br];
+        const int j1 = min(ns, (tid + 1) * run);
+        // batches of RUN_BATCH slots: their nodes, then the nodes' state,
+        // loaded together (indices past the run repeat its last slot; their
+        // terms add +0, which leaves the sums unchanged: they start at +0
+        // and never become -0)
+        constexpr int RUN_BATCH = 8;
+        const bool spring_x = xt == 2, spring_y = yt == 2;
+        const double xval = m.br_xval[br], yval = m.br_yval[br];
+        for (int jb = tid * run; jb < j1; jb += RUN_BATCH) {
+            int nd[RUN_BATCH];
+            double Ls[RUN_BATCH], px[RUN_BATCH], py[RUN_BATCH];
+#pragma unroll
+            for (int u = 0; u < RUN_BATCH; u++) {
+                const int jj = min(jb + u, j1 - 1);
+                nd[u] = m.sl_node[sb + jj];
+                Ls[u] = m.sl_len[sb + jj];
+            }
+#pragma unroll
+            for (int u = 0; u < RUN_BATCH; u++) { px[u] = m.xc[nd[u]]; py[u] = m.yc[nd[u]]; }
+#pragma unroll
+            for (int u = 0; u < RUN_BATCH; u++) {
+                const int node = nd[u];
+                const double L = Ls[u];
+                double fxn = 0., fyn = 0., fxd = 0., fyd = 0.;
+                if (xt == 3 || xt == 4) fxn = xval * L;
+                else if (spring_x) fxd = (m.br_xparam[2 * br] * (xval - m.ux[node]) + m.br_xparam[2 * br + 1] * (m.br_xvel[br] - m.vx[node])) * L;
+                if (yt == 3 || yt == 4) fyn = yval * L;
+                else if (spring_y) fyd = (m.br_yparam[2 * br] * (yval - m.uy[node]) + m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node])) * L;
+                const double dx = px[u] - xc0, dy = py[u] - yc0;
+                const bool in = jb + u < j1;
+                p6[0] += in ? fxn : 0.; p6[1] += in ? fyn : 0.; p6[2] += in ? -fxn * dy + fyn * dx : 0.;
+                p6[3] += in ? fxd : 0.; p6[4] += in ? fyd : 0.; p6[5] += in ? -fxd * dy + fyd * dx : 0.;
+            }
         }
         __syncthreads();
         for (int q = 0; q < 6; q++) part[6 * tid + q] = p6[q];
@@ -463,9 +483,25 @@ __device__ inline void rigid_driven_dofs(const DevModel& m, int b, double t, dou
 // contact elements add to atomically, is left alone).
 // rec: the body's record as it was when the step started (RIGID_FIELDS
 // values), dvt: its driven DOFs at the end of the step (rigid_driven_dofs).
+// the body's constants the step reads (loaded before its forces are known)
+struct RigidScalars {
+    int status, periodicity;
+    double inv_mass, inv_inertia, mass;
+};
+
+__device__ inline RigidScalars rigid_scalars(const DevModel& m, int b) {
+    RigidScalars r;
+    r.status = m.body_status[b];
+    r.periodicity = m.body_periodicity[b];
+    r.inv_mass = m.body_inv_mass[b];
+    r.inv_inertia = m.body_inv_inertia[b];
+    r.mass = m.body_mass[b];
+    return r;
+}
+
 __device__ inline void rigid_step_body(const DevModel& m, int b, const RigidForces& F, const double* rec,
-                                       const double* dvt) {
-    if (m.body_status[b] == 1) return;
+                                       const double* dvt, const RigidScalars& rs) {
+    if (rs.status == 1) return;
     double* Rg = m.body_rigid + b * RIGID_FIELDS;
     double R[RIGID_FIELDS];
 #pragma unroll
@@ -476,9 +512,9 @@ __device__ inline void rigid_step_body(const DevModel& m, int b, const RigidForc
     double dt = m.dt;
     for (int q = 0; q < 3; q++)
         R[R_FTOT + q] = R[R_FCON + q] + R[R_FBODY + q] + R[R_FDIR + q] + R[R_FNEU + q] + R[R_FDAMP + q];
-    R[R_ACC] = R[R_FTOT] * m.body_inv_mass[b];
-    R[R_ACC + 1] = R[R_FTOT + 1] * m.body_inv_mass[b];
-    R[R_ACC + 2] = R[R_FTOT + 2] * m.body_inv_inertia[b];
+    R[R_ACC] = R[R_FTOT] * rs.inv_mass;
+    R[R_ACC + 1] = R[R_FTOT + 1] * rs.inv_mass;
+    R[R_ACC + 2] = R[R_FTOT + 2] * rs.inv_inertia;
     double nacc_x = R[R_ACC], nacc_y = R[R_ACC + 1];
     R[R_NODE_ACC] = nacc_x; R[R_NODE_ACC + 1] = nacc_y;   // node copies of Body::Apply_Newton (Code/Body.h:2038-2042)
     R[R_FDIR_NODE] = R[R_FDIR]; R[R_FDIR_NODE + 1] = R[R_FDIR + 1];   // copied to nodes by Code/Body.h:2005-2006
@@ -492,7 +528,7 @@ __device__ inline void rigid_step_body(const DevModel& m, int b, const RigidForc
     R[R_WORK + W_DIRICHLET] += dt * (hx * R[R_FDIR] + hy * R[R_FDIR + 1] + hr * R[R_FDIR + 2]);
     R[R_WORK + W_NEUMANN] += dt * (hx * R[R_FNEU] + hy * R[R_FNEU + 1] + hr * R[R_FNEU + 2]);
     R[R_WORK + W_DAMPING] += dt * (hx * R[R_FDAMP] + hy * R[R_FDAMP + 1] + hr * R[R_FDAMP + 2]);
-    double mass = m.body_mass[b];
+    double mass = rs.mass;
     if (dv[0] == 0.) {
         R[R_VEL] += dt * R[R_ACC];
         R[R_DISP] = R[R_DISP] + dt * R[R_VEL];
@@ -507,7 +543,7 @@ __device__ inline void rigid_step_body(const DevModel& m, int b, const RigidForc
         R[R_ACC + 1] = dv[7]; R[R_VEL + 1] = dv[6]; R[R_DISP + 1] = dv[5];
         R[R_FDIR + 1] = mass * R[R_ACC + 1] - R[R_FTOT + 1];
     }
-    if (dv[0] == 0. && dv[4] == 0. && m.body_periodicity[b] != 2) {
+    if (dv[0] == 0. && dv[4] == 0. && rs.periodicity != 2) {
         R[R_VEL + 2] += dt * R[R_ACC + 2];
         R[R_DISP + 2] = R[R_DISP + 2] + dt * R[R_VEL + 2];
     } else {
@@ -527,21 +563,34 @@ __device__ inline void rigid_step_body(const DevModel& m, int b, const RigidForc
 // formed (rigid_forces_block), then thread 0 steps the body.  sc: LDS
 // scratch of RIGID_WG_LDS doubles.
 
-__device__ inline void rigid_body_wg(const DevModel& m, int rigid_index, double* sc) {
+// The driven DOFs at the end of the step of rigid body rigid_index (its
+// record's previous values updated by rigid_driven_dofs), into dv[8].
+__device__ inline void rigid_dv_end_of_step(const DevModel& m, int rigid_index, double* dv) {
+    const int b = m.rigid_ids[rigid_index];
+    const double* Rg = m.body_rigid + b * RIGID_FIELDS;
+    for (int k = 0; k < 8; k++) dv[k] = Rg[R_DRIVEN + k];
+    rigid_driven_dofs(m, b, *m.time + m.dt, dv);
+}
+
+// dv_pre: the driven DOFs formed earlier by rigid_dv_end_of_step (8 per
+// rigid body), or nullptr to form them here
+__device__ inline void rigid_body_wg(const DevModel& m, int rigid_index, double* sc, const double* dv_pre) {
     const int b = m.rigid_ids[rigid_index];
     double* s_rec = sc + 6 * BLOCK_THREADS + 8;
     double* s_dv = s_rec + RIGID_FIELDS;
     const double* Rg = m.body_rigid + b * RIGID_FIELDS;
+    const RigidScalars rs = rigid_scalars(m, b);
     for (int k = threadIdx.x; k < RIGID_FIELDS; k += blockDim.x) s_rec[k] = Rg[k];
-    if (threadIdx.x == blockDim.x - 1) {
+    if (dv_pre) {
+        if (threadIdx.x < 8) s_dv[threadIdx.x] = dv_pre[8 * rigid_index + threadIdx.x];
+    } else if (threadIdx.x == blockDim.x - 1) {
         double dv[8];
-        for (int k = 0; k < 8; k++) dv[k] = Rg[R_DRIVEN + k];
-        rigid_driven_dofs(m, b, *m.time + m.dt, dv);
+        rigid_dv_end_of_step(m, rigid_index, dv);
         for (int k = 0; k < 8; k++) s_dv[k] = dv[k];
     }
     const RigidForces F = rigid_forces_block(m, rigid_index, sc);   // its barriers publish s_rec, s_dv
     __syncthreads();
-    if (threadIdx.x == 0) rigid_step_body(m, b, F, s_rec, s_dv);
+    if (threadIdx.x == 0) rigid_step_body(m, b, F, s_rec, s_dv, rs);
 }
 
 // rigid node kinematics and positions (Code/Body.h:2214-2226, 2281-2301);
@@ -727,6 +776,20 @@ __device__ inline SlotPre slot_prefetch(const DevModel& m, int s) {
     return r;
 }
 
+// the same from the slot's record (one load before the positions)
+__device__ inline SlotPre slot_prefetch_rec(const DevModel& m, const BlockArrays& B, int s) {
+    SlotPre r;
+    const int4 a = B.sl_rec0[s], h = B.sl_rec1[s];
+    r.j = a.w;
+    r.n = h.w & ((1 << 30) - 1);
+    r.simple = (h.w >> 30) & 1;
+    const double P = m.period;
+    r.x0 = m.xc[a.x] + P * h.x; r.y0 = m.yc[a.x];
+    r.x1 = m.xc[a.y] + P * h.y; r.y1 = m.yc[a.y];
+    r.x2 = m.xc[a.z] + P * h.z; r.y2 = m.yc[a.z];
+    return r;
+}
+
 // slot_length from prefetched inputs (same expressions)
 __device__ inline double slot_length_pre(const SlotPre& r) {
     double l21 = __dsqrt_rn((r.x2 - r.x1) * (r.x2 - r.x1) + (r.y2 - r.y1) * (r.y2 - r.y1));
@@ -755,12 +818,13 @@ __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArray
                                              const double (&fi)[2][NODES_PER_THREAD], double* sc) {
     __shared__ double red[BLOCK_THREADS / 64][5];
     const int t = threadIdx.x;
-    const int body = B.blk_body[blk];
-    if (m.body_status[body] == 1) return;
-    const int nown = B.blk_nown[blk];
-    const int n0 = B.own_nodes[B.blk_own_off[blk]];
-    const int* bns = B.bn_list + B.blk_bn_off[blk];
-    const int nbn = B.blk_bn_n[blk];
+    // the block's header (one load for everything the first loads need)
+    const int4 h0 = B.blk_hdr[4 * blk], h1 = B.blk_hdr[4 * blk + 1], h3 = B.blk_hdr[4 * blk + 3];
+    if (h1.z & BLK_INACTIVE) return;
+    const int nown = h0.y;
+    const int n0 = h0.x;
+    const int* bns = B.bn_list + h3.z;
+    const int nbn = h3.w;
     double* s_bnp = sc;                // [nbn][6]: contact, Dirichlet, Neumann point forces (x, y)
     double* s_up = sc + 6 * nbn;       // [nown][6]: new displacement, velocity, acceleration parameters
 
@@ -782,20 +846,20 @@ __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArray
     // the thread's first border node: its lengths' inputs (old positions)
     // and its kinematics row
     const bool mine = t < nbn;
-    const int bn_t = mine ? bns[t] : 0;
-    int sq0 = 0, sq1 = 0, knode = 0, k0 = 0, k1 = 0;
+    int bn_t = 0, sq0 = 0, sq1 = 0, knode = 0, k0 = 0, k1 = 0;
     SlotPre sp[FIN_SLOTS];
     int di[FIN_ROW];
     double dp[FIN_ROW];
     double kX = 0., kY = 0.;
+    static_assert(FIN_SLOTS == 2, "the border-node record names two slots");
     if (mine) {
-        sq0 = m.bn_slot_ptr[bn_t]; sq1 = m.bn_slot_ptr[bn_t + 1];
-        knode = m.bn_node[bn_t];
-        k0 = m.d_ptr[knode]; k1 = m.d_ptr[knode + 1];
-        kX = m.X[knode]; kY = m.Y[knode];
-#pragma unroll
-        for (int u = 0; u < FIN_SLOTS; u++)
-            if (sq0 + u < sq1) sp[u] = slot_prefetch(m, m.bn_slots[sq0 + u]);
+        const int4 r0 = B.fin_rec0[h3.z + t], r1 = B.fin_rec1[h3.z + t];
+        const double2 rx = B.fin_recX[h3.z + t];
+        bn_t = r0.x; knode = r0.y; k0 = r0.z; k1 = r0.w;
+        sq0 = r1.x; sq1 = r1.y;
+        kX = rx.x; kY = rx.y;
+        if (sq0 < sq1) sp[0] = slot_prefetch_rec(m, B, r1.z);
+        if (sq0 + 1 < sq1) sp[1] = slot_prefetch_rec(m, B, r1.w);
 #pragma unroll
         for (int u = 0; u < FIN_ROW; u++)
             if (k0 + u < k1) { di[u] = m.d_idx[k0 + u]; dp[u] = m.d_phi[k0 + u]; }
@@ -803,7 +867,7 @@ __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArray
 
     // ---- sends: bucket sums, point forces staged ----
     for (int i = t; i < nbn; i += BLOCK_THREADS) {
-        const int bn = bns[i];
+        const int bn = i == t ? bn_t : bns[i];
         const double pdx = m.bn_pdx[bn], pdy = m.bn_pdy[bn], pnx = m.bn_pnx[bn], pny = m.bn_pny[bn];
         const double2 pc = bucket_sum_node(m, bn);
         double* q = s_bnp + 6 * i;
@@ -814,7 +878,7 @@ __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArray
     // ---- Sum_up_forces, Apply_Newton, Apply_Euler, works ----
     double w[5] = {0., 0., 0., 0., 0.};   // internal body dirichlet neumann damping
     double gx = m.gravity_x, gy = m.gravity_y;
-    if (m.body_status[body] == 2) { gx = 0.; gy = 0.; }
+    if (h1.z & BLK_NO_GRAVITY) { gx = 0.; gy = 0.; }
     const double dt = m.dt;
 #pragma unroll
     for (int q = 0; q < NODES_PER_THREAD; q++) {

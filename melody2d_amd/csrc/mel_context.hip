@@ -146,6 +146,7 @@ struct mel_ctx {
     int n_kin_all = 0;
     bool outputs_stale = false;   // interior kinematics / force fields lag the last steps
     std::vector<int> h_blk_body;  // body of each block (work sums)
+    std::vector<int> h_bn_list;   // border nodes finished by each block (BlockArrays::bn_list)
 
     template <typename T>
     T* alloc(size_t n) {
@@ -667,6 +668,7 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
         for (int o = 0; o < nown; o++) own_contig = own_contig && own[o] == own0 + o;
         for (int i = 0; i < ntouch; i++) touch_contig = touch_contig && touch[i] == own0 + i;
         const int flags = (blk_single[k] ? BLK_SINGLE : 0) | (Bd.status == 1 ? BLK_INACTIVE : 0) |
+                          (Bd.status == 2 ? BLK_NO_GRAVITY : 0) |
                           (m.materials[Bd.material].type != 0 ? BLK_NEO_HOOKEAN : 0) |
                           (own_contig ? BLK_OWN_CONTIGUOUS : 0);
         blk_hdr.push_back(make_int4(own0, nown, ntouch, touch_contig ? -1 : blk_touch_off[k]));
@@ -674,7 +676,7 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
         const bool any = blk_nchunk[k] > 0;
         blk_hdr.push_back(any ? ch_meta[blk_chunk_off[k]] : make_int4(0, 0, 0, 0));
         const int2 po = any ? ch_ptr_off[blk_chunk_off[k]] : make_int2(0, 0);
-        blk_hdr.push_back(make_int4(po.x, po.y, 0, 0));
+        blk_hdr.push_back(make_int4(po.x, po.y, blk_bn_off[k], blk_bn_n[k]));
         blk_par.push_back(make_double2(m.materials[Bd.material].p[3], m.materials[Bd.material].p[4]));
     }
     BlockArrays& B = c.blocks_dev;
@@ -709,6 +711,7 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     }
     B.blk_work = c.upload(blk_work);
     B.blk_bn_off = c.upload(blk_bn_off); B.blk_bn_n = c.upload(blk_bn_n); B.bn_list = c.upload(bn_list);
+    c.h_bn_list = bn_list;
     B.split_bn = c.upload(split_bn); B.n_split_bn = (int)split_bn.size();
     B.split_kin = c.upload(split_kin); B.n_split_kin = (int)split_kin.size();
     c.d_kin_all = c.upload(kin_all);
@@ -1204,6 +1207,30 @@ void build(mel_ctx& c) {
     build_blocks(c, g_ptr, g_idx, g_phix, g_phiy, g_c0, t_ptr, t_gp, bn_node, node_off, gp_off, perreg, d_ptr, d_idx,
                  bt_ptr, bt_bn);
     {
+        // what the finish of a node block reads first, per listed border
+        // node and per slot (static model data gathered so that one load
+        // replaces a chain of index loads)
+        const std::vector<int>& bl = c.h_bn_list;
+        std::vector<int4> fr0(std::max<size_t>(bl.size(), 1)), fr1(std::max<size_t>(bl.size(), 1));
+        std::vector<double2> frX(std::max<size_t>(bl.size(), 1));
+        for (size_t i = 0; i < bl.size(); i++) {
+            const int q = bl[i], node = bn_node[q];
+            const int a0 = bn_slot_ptr[q], a1 = bn_slot_ptr[q + 1];
+            fr0[i] = make_int4(q, node, d_ptr[node], d_ptr[node + 1]);
+            fr1[i] = make_int4(a0, a1, a1 > a0 ? bn_slots[a0] : 0, a1 > a0 + 1 ? bn_slots[a0 + 1] : 0);
+            frX[i] = make_double2(X[node], Y[node]);
+        }
+        std::vector<int4> sr0(std::max(NS, 1)), sr1(std::max(NS, 1));
+        for (int s2 = 0; s2 < NS; s2++) {
+            const int br = sl_border[s2];
+            sr0[s2] = make_int4(sl_n0[s2], sl_n1[s2], sl_n2[s2], sl_j[s2]);
+            sr1[s2] = make_int4(sl_s0[s2], sl_s1[s2], sl_s2[s2], br_nslots[br] | (br_per[br] == 1 ? 1 << 30 : 0));
+        }
+        BlockArrays& B = c.blocks_dev;
+        B.fin_rec0 = c.upload(fr0); B.fin_rec1 = c.upload(fr1); B.fin_recX = c.upload(frX);
+        B.sl_rec0 = c.upload(sr0); B.sl_rec1 = c.upload(sr1);
+    }
+    {
         // deformable border nodes with a slot on a border that carries a BC
         std::vector<int> bc_bn;
         for (int q = 0; q < NBN; q++) {
@@ -1336,6 +1363,7 @@ void build(mel_ctx& c) {
     d.slot_old_cursor = c.alloc<int>(NS);
     d.status_flags = c.alloc<int>(4);
     d.step_sync = c.alloc<unsigned>(4);
+    d.rigid_dv = c.alloc<double>(8 * (size_t)std::max(1, d.n_rigid));
     d.time = c.alloc<double>(1);
     HIP_OK(hipMemcpyAsync(d.time, &m.time, sizeof(double), hipMemcpyHostToDevice, c.stream));
     c.d_rigid_ids = c.upload(c.h_rigid_ids);

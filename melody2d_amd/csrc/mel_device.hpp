@@ -145,6 +145,8 @@ struct DevModel {
     int *law_first;                          // first law matching a material pair by name (Initialize_CZM)
     int *status_flags;                       // [0] capacity overflow bits, [1] non-finite state seen
     double *time;                            // device copy of the simulation time
+    double *rigid_dv;                        // per rigid body: driven DOFs at the end of the current
+                                             // step (8 each, as R_DRIVEN), formed by the force launch
     unsigned *step_sync;                     // step launch B: [1] rigid-body workgroups done
 };
 
