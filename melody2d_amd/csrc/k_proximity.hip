@@ -52,7 +52,7 @@ __device__ inline void wg_scan_array(const int* in, int* out, int n, int* s_w) {
     if (threadIdx.x == 0) out[n] = base;
 }
 
-// per chunk of SEG_CHUNK border slots, one half-wavefront (lane j takes slot
+// per chunk of SEG_CHUNK border slots, SEG_CHUNK lanes (lane j takes slot
 // and segment j of the chunk): the box of the slot nodes inflated by the
 // detection distance (its part of the border AABB of
 // Code/Proximity.h:63-109), the box of the chunk's segments, used to skip
@@ -97,6 +97,13 @@ __global__ void k_prox_chunk_boxes(DevModel m) {
         m.sc_nxmin[c] = nxmin; m.sc_nxmax[c] = nxmax; m.sc_nymin[c] = nymin; m.sc_nymax[c] = nymax;
         m.sc_box[c] = make_double4(xmin, xmax, ymin, ymax);
     }
+    // the wavefront's SC_GROUP chunks: their union (chunks of neighbouring
+    // borders may share it, which only makes it larger)
+    for (int off = SEG_CHUNK; off < 64; off <<= 1) {
+        xmin = fmin(xmin, __shfl_xor(xmin, off, 64)); xmax = fmax(xmax, __shfl_xor(xmax, off, 64));
+        ymin = fmin(ymin, __shfl_xor(ymin, off, 64)); ymax = fmax(ymax, __shfl_xor(ymax, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0 && c < m.n_seg_chunks) m.sc_box2[g / 64] = make_double4(xmin, xmax, ymin, ymax);
 }
 
 // border AABBs inflated by the detection distance + periodic shift range
@@ -357,12 +364,25 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_
         const double4 sb = m.pb_segbox[gM];
         if (!(Xq > sb.x - dd && Xq < sb.y + dd && Ys > sb.z - dd && Ys < sb.w + dd)) return 0;
     }
-    for (int cb = cM0 + seg_start / SEG_CHUNK; cb < cM1 && cnt < limit; cb += BOX_BATCH) {
+    // groups of SC_GROUP chunks first, then the chunks of the groups in reach
+    const int cS = cM0 + seg_start / SEG_CHUNK;
+    for (int gq = cS / SC_GROUP; gq * SC_GROUP < cM1 && cnt < limit; gq += BOX_BATCH) {
+    unsigned gpass = 0;       // groups of this batch whose box is within reach
+#pragma unroll
+    for (int u = 0; u < BOX_BATCH; u++) {
+        const double4 bx = m.sc_box2[min(gq + u, (cM1 - 1) / SC_GROUP)];
+        if ((gq + u) * SC_GROUP < cM1 && Xq > bx.x - dd && Xq < bx.y + dd && Ys > bx.z - dd && Ys < bx.w + dd)
+            gpass |= 1u << u;
+    }
+    for (; gpass && cnt < limit; gpass &= gpass - 1) {
+    const int grp = gq + __builtin_ctz(gpass);
+    const int c1 = min(grp * SC_GROUP + SC_GROUP, cM1);
+    for (int cb = max(grp * SC_GROUP, cS); cb < c1 && cnt < limit; cb += BOX_BATCH) {
     unsigned pass = 0;        // chunks of this batch whose box is within reach
 #pragma unroll
     for (int u = 0; u < BOX_BATCH; u++) {
-        const double4 bx = m.sc_box[min(cb + u, cM1 - 1)];
-        if (cb + u < cM1 && Xq > bx.x - dd && Xq < bx.y + dd && Ys > bx.z - dd && Ys < bx.w + dd) pass |= 1u << u;
+        const double4 bx = m.sc_box[min(cb + u, c1 - 1)];
+        if (cb + u < c1 && Xq > bx.x - dd && Xq < bx.y + dd && Ys > bx.z - dd && Ys < bx.w + dd) pass |= 1u << u;
     }
     for (; pass && cnt < limit; pass &= pass - 1) {
     const int ch = cb + __builtin_ctz(pass);
@@ -426,6 +446,8 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_
     }
     }
     }
+    }
+    }
     return cnt;
 }
 
@@ -473,7 +495,8 @@ __global__ void __launch_bounds__(1024) k_prox_elem_offsets(DevModel m) {
 }
 
 // Writing pass: the rows with candidates, each from its first candidate on
-__global__ void k_prox_write(DevModel m) {
+// (up to 256 registers per lane: the emitting scan does not spill)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, 2))) k_prox_write(DevModel m) {
     const int n = min(*m.hit_n, m.c_cap);
     for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < n; h += gridDim.x * blockDim.x) {
         const int4 r = m.hit_rec[h];

@@ -6,7 +6,9 @@ namespace mel {
 
 constexpr int BUCKET_CAP = 32;     // point-force records per border node per step
 constexpr int BLOCK = 256;
-constexpr int SEG_CHUNK = 32;     // border slots per proximity culling chunk
+constexpr int SEG_CHUNK = 8;      // border slots per proximity culling chunk
+constexpr int SC_GROUP = 8;       // chunks per second-level culling box (a wavefront of the box kernel)
+static_assert(SEG_CHUNK * SC_GROUP == 64, "a chunk group is one wavefront of k_prox_chunk_boxes");
 constexpr int ROW_CHUNK = 256;    // narrow-phase rows per counting workgroup
 
 __global__ void k_border_length(DevModel m);

@@ -1328,6 +1328,7 @@ void build(mel_ctx& c) {
         d.sc_nxmin = c.alloc<double>(nc); d.sc_nxmax = c.alloc<double>(nc);
         d.sc_nymin = c.alloc<double>(nc); d.sc_nymax = c.alloc<double>(nc);
         d.sc_box = c.alloc<double4>(nc);
+        d.sc_box2 = c.alloc<double4>((nc + SC_GROUP - 1) / SC_GROUP);
     }
     d.seg_ns = c.alloc<int4>(NS);
     d.seg_xy = c.alloc<double4>(NS);

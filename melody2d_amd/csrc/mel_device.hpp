@@ -125,6 +125,8 @@ struct DevModel {
     int *br_chunk0, *sc_border;              // first chunk per border (n_borders + 1), border per chunk
     double *sc_nxmin, *sc_nxmax, *sc_nymin, *sc_nymax;   // slot-node box inflated by detection
     double4 *sc_box;                         // segment box (slot shifts applied): xmin, xmax, ymin, ymax
+    double4 *sc_box2;                        // per group of SC_GROUP consecutive chunks: the union of
+                                             // their segment boxes
     int4 *seg_ns;                            // per slot s, segment s: end nodes n1, n2 and their shifts
     double4 *seg_xy;                         // per slot s, segment s: x, y of n1, then of n2
     int nb_cap;                              // neighbour capacity per body
