@@ -71,6 +71,10 @@ __global__ void k_prox_chunk_boxes(DevModel m) {
     const double d = m.body_detect[b];
     double nxmin = 1.e100, nxmax = -1.e100, nymin = 1.e100, nymax = -1.e100;
     double xmin = 1.e100, xmax = -1.e100, ymin = 1.e100, ymax = -1.e100;
+    // the slots' old-element cursors for the history transfer start again
+    // (the count of the elements about to become the previous ones is kept)
+    if (live && k < ns) m.slot_old_cursor[s0 + k] = 0;
+    if (g == 0) *m.c_count_old = *m.c_count;
     if (on && k < ns) {
         const int nd = m.sl_node[s0 + k];
         const double x = m.xc[nd], y = m.yc[nd];
@@ -140,7 +144,6 @@ __global__ void k_prox_aabb(DevModel m) {
 // per body1; lanes take consecutive body2 so the order of insertion is the
 // reference's (border1, body2, border2, shift) order with first-occurrence
 // de-duplication.
-constexpr int NB_THREADS = 1024;
 __global__ void __launch_bounds__(NB_THREADS) k_prox_neighbours(DevModel m) {
     const int b1 = blockIdx.x;
     const int lane = threadIdx.x;
@@ -156,16 +159,29 @@ __global__ void __launch_bounds__(NB_THREADS) k_prox_neighbours(DevModel m) {
             const double xmin1 = m.pb_xmin[g1], xmax1 = m.pb_xmax[g1], ymin1 = m.pb_ymin[g1], ymax1 = m.pb_ymax[g1];
             const int shmin1 = m.pb_shmin[g1], shmax1 = m.pb_shmax[g1];
             const int before = count_sh;     // entries from earlier border1 iterations
-            for (int base = 0; base < m.n_bodies; base += NB_THREADS) {
-                const int b2 = base + lane;
+            for (int base = 0; base < m.n_bodies; base += NB_THREADS * NB_PER) {
+                // this lane's bodies: base + NB_PER * lane + u, their records
+                // loaded before any test
+                int st[NB_PER], go[NB_PER], nbr[NB_PER];
+#pragma unroll
+                for (int u = 0; u < NB_PER; u++) {
+                    const int b2 = base + NB_PER * lane + u;
+                    const bool live = b2 < m.n_bodies && b2 != b1;
+                    st[u] = live ? m.body_status[b2] : 1;
+                    go[u] = live ? m.border_off[b2] : 0;
+                    nbr[u] = live ? m.body_nborders[b2] : 0;
+                }
                 const int start = count_sh;
                 int ex = 0;
                 // pass 0 counts this lane's new neighbours, pass 1 writes them
                 for (int pass = 0; pass < 2; pass++) {
                     int w = 0;
-                    if (b2 < m.n_bodies && b2 != b1 && m.body_status[b2] != 1) {
-                        for (int r2 = 0; r2 < m.body_nborders[b2]; r2++) {
-                            const int g2 = m.border_off[b2] + r2;
+#pragma unroll
+                    for (int u = 0; u < NB_PER; u++) {
+                        const int b2 = base + NB_PER * lane + u;
+                        if (st[u] == 1) continue;
+                        for (int r2 = 0; r2 < nbr[u]; r2++) {
+                            const int g2 = go[u] + r2;
                             if ((m.pb_ymax[g2] < ymin1) || (ymax1 < m.pb_ymin[g2])) continue;
                             for (int sh = shmin1 - m.pb_shmax[g2]; sh <= shmax1 - m.pb_shmin[g2]; sh++) {
                                 if ((m.pb_xmax[g2] + sh * P > xmin1) && (m.pb_xmin[g2] + sh * P < xmax1)) {
@@ -222,9 +238,7 @@ __device__ inline int body_rows(const DevModel& m, int b) {
 
 // row chunks per body, and the body's row layout for the counting pass:
 // (rows, slots, neighbours, self | borders << 1) and (first border, first slot)
-__global__ void k_prox_rows(DevModel m) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= m.n_bodies) return;
+__device__ inline void body_row_layout(const DevModel& m, int b) {
     const int rows = body_rows(m, b);
     m.body_chunks[b] = (rows + ROW_CHUNK - 1) / ROW_CHUNK;
     const int g0 = m.border_off[b];
@@ -234,10 +248,12 @@ __global__ void k_prox_rows(DevModel m) {
     m.body_gs0[b] = make_int2(g0, nbr > 0 ? m.br_slot0[g0] : 0);
 }
 
-// first row chunk of every body and the body of every row chunk (one
-// workgroup); clears the hit list
+// One workgroup: every body's row layout (body_row_layout), the first row
+// chunk of every body and the body of every row chunk; clears the hit list
 __global__ void __launch_bounds__(1024) k_prox_chunk_offsets(DevModel m) {
     __shared__ int s_w[16];
+    for (int b = threadIdx.x; b < m.n_bodies; b += blockDim.x) body_row_layout(m, b);
+    __syncthreads();       // the bodies' chunk counts are in place
     wg_scan_array(m.body_chunks, m.chunk_off, m.n_bodies, s_w);
     __syncthreads();
     for (int b = threadIdx.x; b < m.n_bodies; b += blockDim.x)
@@ -451,14 +467,32 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_
     return cnt;
 }
 
+// index of the previous elements by slave slot, for history transfer
+__device__ inline void old_index(const DevModel& m, int wg, int n_wg) {
+    const int n = *m.c_count_old;
+    for (int e = wg * blockDim.x + threadIdx.x; e < n; e += n_wg * blockDim.x) {
+    const ContactArrays& O = m.Cold;
+    int slot = m.br_slot0[m.border_off[O.bodyS[e]] + O.borderS[e]] + O.bnodeS[e];
+    int pos = atomicAdd(&m.slot_old_cursor[slot], 1);
+    if (pos < OLD_CAP) m.slot_old_list[slot * OLD_CAP + pos] = e;
+    else atomicOr(&m.status_flags[0], 4);
+    }
+}
+
 // Counting pass over row chunks of ROW_CHUNK rows (one row per thread, the
 // grid loops over the chunks): candidates per chunk, and one hit record per
 // row with candidates: (chunk, elements of the chunk's earlier rows, first
 // candidate's segment, row of the body)
-__global__ void __launch_bounds__(ROW_CHUNK) k_prox_count(DevModel m) {
+// Workgroups from n_count on build the index of the previous elements
+// (old_index), which nothing before the history transfer reads.
+__global__ void __launch_bounds__(ROW_CHUNK) k_prox_count(DevModel m, int n_count) {
+    if ((int)blockIdx.x >= n_count) {
+        old_index(m, blockIdx.x - n_count, gridDim.x - n_count);
+        return;
+    }
     __shared__ int s_w[ROW_CHUNK / 64];
     const int n_chunks = m.chunk_off[m.n_bodies];
-    for (int c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    for (int c = blockIdx.x; c < n_chunks; c += n_count) {
         const int b = m.chunk_body[c];
         const int4 rr = m.body_rowrec[b];
         const int2 gs = m.body_gs0[b];
@@ -504,18 +538,6 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, 2
         RowInfo ri;
         if (!decode_row_rec(m, b, m.body_rowrec[b], m.body_gs0[b], r.w, ri)) continue;
         scan_row<true>(m, ri, m.chunk_elem_off[r.x] + r.y, r.z, nullptr, m.hit_cnt[h]);
-    }
-}
-
-// index of the previous elements by slave slot, for history transfer
-__global__ void k_prox_old_index(DevModel m) {
-    const int n = *m.c_count_old;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-    const ContactArrays& O = m.Cold;
-    int slot = m.br_slot0[m.border_off[O.bodyS[e]] + O.borderS[e]] + O.bnodeS[e];
-    int pos = atomicAdd(&m.slot_old_cursor[slot], 1);
-    if (pos < OLD_CAP) m.slot_old_list[slot * OLD_CAP + pos] = e;
-    else atomicOr(&m.status_flags[0], 4);
     }
 }
 
@@ -580,13 +602,6 @@ __global__ void k_prox_chain(DevModel m) {
     }
 }
 
-// runs after the host swapped the element buffers: C is empty, Cold holds
-// the previous elements whose count is still in c_count
-__global__ void k_prox_clear_old(DevModel m) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < m.n_slots) m.slot_old_cursor[s] = 0;
-    if (s == 0) *m.c_count_old = *m.c_count;
-}
 
 // Initialize_CZM (Code/Proximity.h:481-541) after the elements have been
 // re-projected: damage 0 below gapinit, 1 otherwise, for CZMlinear pairs.

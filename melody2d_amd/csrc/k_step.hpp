@@ -10,6 +10,8 @@ constexpr int SEG_CHUNK = 8;      // border slots per proximity culling chunk
 constexpr int SC_GROUP = 8;       // chunks per second-level culling box (a wavefront of the box kernel)
 static_assert(SEG_CHUNK * SC_GROUP == 64, "a chunk group is one wavefront of k_prox_chunk_boxes");
 constexpr int ROW_CHUNK = 256;    // narrow-phase rows per counting workgroup
+constexpr int NB_THREADS = 256;   // neighbour search: threads per body
+constexpr int NB_PER = 4;         // neighbour search: candidate bodies per thread and pass
 
 __global__ void k_border_length(DevModel m);
 __global__ void k_bc_eval(DevModel m, int plus_dt);
@@ -22,13 +24,10 @@ __global__ void k_step_gather(DevModel m, const int* split_bn, int n_split);
 __global__ void k_prox_chunk_boxes(DevModel m);
 __global__ void k_prox_aabb(DevModel m);
 __global__ void k_prox_neighbours(DevModel m);
-__global__ void k_prox_rows(DevModel m);
 __global__ void k_prox_chunk_offsets(DevModel m);
-__global__ void k_prox_count(DevModel m);
+__global__ void k_prox_count(DevModel m, int n_count);
 __global__ void k_prox_elem_offsets(DevModel m);
 __global__ void k_prox_write(DevModel m);
-__global__ void k_prox_clear_old(DevModel m);
-__global__ void k_prox_old_index(DevModel m);
 __global__ void k_prox_history(DevModel m);
 __global__ void k_prox_chain(DevModel m);
 __global__ void k_init_czm(DevModel m);

@@ -760,16 +760,14 @@ void update_proximity(mel_ctx& c) {
     const int persistent = 4 * 256;     // workgroups of the grid-stride loops
     if (d.n_seg_chunks > 0) k_prox_chunk_boxes<<<blocks((long long)d.n_seg_chunks * SEG_CHUNK), BLOCK, 0, s>>>(d);
     k_prox_aabb<<<blocks((long long)d.n_borders * 64), BLOCK, 0, s>>>(d);
-    k_prox_neighbours<<<d.n_bodies, 1024, 0, s>>>(d);
-    k_prox_rows<<<blocks(d.n_bodies), BLOCK, 0, s>>>(d);
+    k_prox_neighbours<<<d.n_bodies, NB_THREADS, 0, s>>>(d);
     k_prox_chunk_offsets<<<1, 1024, 0, s>>>(d);
     // previous elements become the history source
     std::swap(c.bufA, c.bufB);
     d.C = c.bufA;
     d.Cold = c.bufB;
-    k_prox_clear_old<<<blocks(d.n_slots), BLOCK, 0, s>>>(d);
-    k_prox_old_index<<<persistent, BLOCK, 0, s>>>(d);
-    k_prox_count<<<std::min(persistent, std::max(1, c.max_row_chunks)), ROW_CHUNK, 0, s>>>(d);
+    const int n_count = std::min(persistent, std::max(1, c.max_row_chunks));
+    k_prox_count<<<n_count + 64, ROW_CHUNK, 0, s>>>(d, n_count);
     k_prox_elem_offsets<<<1, 1024, 0, s>>>(d);
     k_prox_write<<<persistent, BLOCK, 0, s>>>(d);
     k_prox_history<<<persistent, BLOCK, 0, s>>>(d);
