@@ -143,10 +143,14 @@ __device__ inline float2 contribution32(float c0, f32x2 f, const float4& P) {
     return make_float2(c0 * fmaf(f.x, P.x, f.y * P.y), c0 * fmaf(f.x, P.z, f.y * P.w));
 }
 
-// precision policies of the block kernel
+// precision policies of the block kernel.  nt_f / nt_pk: the derivative and
+// index streams are read with non-temporal loads (read once per step, they
+// then do not displace the node data the rest of the step reads from the
+// caches; measured per form)
 struct Fp64 {
     static constexpr bool f32 = false;
     static constexpr bool dbuf = false;     // one chunk's entries in flight (registers)
+    static constexpr bool nt_f = true, nt_pk = false;
     static constexpr int KREG = KREG_ENTRIES;
     typedef f64x2 vec;
     typedef double2 slot;
@@ -156,6 +160,7 @@ struct Fp64 {
 struct Fp32 {
     static constexpr bool f32 = true;
     static constexpr bool dbuf = true;      // two chunks' entries in flight
+    static constexpr bool nt_f = true, nt_pk = false;
     static constexpr int KREG = KREG_ENTRIES_F32;
     typedef f32x2 vec;
     typedef float2 slot;
@@ -234,8 +239,10 @@ __device__ inline void load_lane(const BlockArrays& B, int4 meta, int2 po, int n
 #pragma unroll
         for (int i = 0; i < PR::KREG; i++) {
             if (i < L.K) {
-                R.f[i] = lane_entry(fxy, L, i);
-                R.pk[i] = lane_entry(B.be_pk, L, i);
+                if constexpr (PR::nt_f) R.f[i] = __builtin_nontemporal_load(&lane_entry(fxy, L, i));
+                else R.f[i] = lane_entry(fxy, L, i);
+                if constexpr (PR::nt_pk) R.pk[i] = __builtin_nontemporal_load(&lane_entry(B.be_pk, L, i));
+                else R.pk[i] = lane_entry(B.be_pk, L, i);
             }
         }
     }
