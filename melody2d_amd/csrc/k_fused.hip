@@ -7,22 +7,25 @@
 //                      Update_contact_forces, point-force records into the
 //                      border nodes' buckets and the rigid bodies' records)
 //                      and BC point forces of deformable border nodes
-//                      (Update_bc_forces); the first also forms the rigid
-//                      bodies' driven DOFs at the end of the step
+//                      (Update_bc_forces), then the BC terms of rigid
+//                      borders run by run (rigid_bc_item); the first also
+//                      forms the rigid bodies' driven DOFs at the end of the
+//                      step
 //   next n_blocks      one node block each: internal forces
 //                      (Update_internal_forces), which depend on nothing else
 //                      of the step
 // No workgroup of launch A waits for another.
 // Launch B:
 //   [0, n_rigid)       one rigid body each: resultants of its contact records
-//                      and BCs, Sum_up_forces, Apply_Newton, Apply_Euler
-//                      with the driven DOFs launch A formed; signal [1]
+//                      and of the BC terms launch A formed, Sum_up_forces,
+//                      Apply_Newton, Apply_Euler with the driven DOFs launch
+//                      A formed; signal [1]
 //   next n_blocks      one node block each: bucket sums (the contact sends),
 //                      Sum_up_forces + Apply_Newton + Apply_Euler with works,
 //                      Update_borders and the border-node kinematics, with
 //                      the block's point forces and new parameters staged in
 //                      LDS
-//   next               one thread per rigid-body node: wait [1]; its new
+//   next               one rigid-body node per thread: wait [1]; its new
 //                      position and velocity (Update_kinematics of rigid
 //                      bodies), which only the next step reads
 //   last               wait [1] (the rigid bodies read the BC values and the
@@ -89,8 +92,11 @@ __device__ __forceinline__ void step_forces(DevModel& m, const BlockArrays& B, c
                 for (int q = 0; q < 8; q++) m.rigid_dv[8 * k + q] = dv[q];
             }
         const int nc = *m.c_count;
-        for (int i = w * BLOCK_THREADS + t; i < nc + B.n_bc_bn; i += S.n_cwg * BLOCK_THREADS)
-            contact_or_bc_item(m, i, nc, B.bc_bn, B.n_bc_bn);
+        const int n_items = nc + B.n_bc_bn + m.n_rbc * BLOCK_THREADS;
+        for (int i = w * BLOCK_THREADS + t; i < n_items; i += S.n_cwg * BLOCK_THREADS) {
+            if (i < nc + B.n_bc_bn) contact_or_bc_item(m, i, nc, B.bc_bn, B.n_bc_bn);
+            else rigid_bc_item(m, i - nc - B.n_bc_bn);
+        }
         return;
     }
     w -= S.n_cwg;
@@ -140,14 +146,32 @@ k_step_finish(DevModel m, BlockArrays B) {
         return;
     }
     w -= B.n_blocks;
-    const int n_rnwg = (m.n_rigid_nodes + BLOCK_THREADS - 1) / BLOCK_THREADS;
-    if (w < n_rnwg) {
-        // the node's own data is loaded before the wait for its body
-        const int q = w * BLOCK_THREADS + t;
-        RigidNodePre p{};
-        if (q < m.n_rigid_nodes) p = rigid_node_pre(m, m.rigid_node_ids[q]);
+    if (w < step_rigid_node_wgs(m.n_rigid_nodes)) {
+        // RIGID_NODES_PER_THREAD nodes per thread; their own data is loaded
+        // before the wait for their bodies
+        RigidNodePre p[RIGID_NODES_PER_THREAD];
+#pragma unroll
+        for (int u = 0; u < RIGID_NODES_PER_THREAD; u++) {
+            const int q = (w * RIGID_NODES_PER_THREAD + u) * BLOCK_THREADS + t;
+            p[u] = RigidNodePre{};
+            if (q < m.n_rigid_nodes) p[u] = rigid_node_pre(m, m.rigid_node_ids[q]);
+        }
         wg_wait(m, &sync[1], m.n_rigid);
-        rigid_node_apply(m, p, 1);
+        // the rotation's cosine and sine once per body and thread
+        int cb = -1;
+        double c = 0., sn = 0.;
+#pragma unroll
+        for (int u = 0; u < RIGID_NODES_PER_THREAD; u++) {
+            if (!p[u].on) continue;
+            const double* R = m.body_rigid + p[u].b * RIGID_FIELDS;
+            if (p[u].b != cb) {
+                const double dr = R[R_DISP + 2] - 0.;
+                c = cos(dr);
+                sn = sin(dr);
+                cb = p[u].b;
+            }
+            rigid_node_at(m, p[u].i, R, c, sn, 1, p[u].X, p[u].Y);
+        }
         return;
     }
     // last workgroup: the time of the end of the step from the device's copy

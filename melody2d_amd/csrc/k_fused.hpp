@@ -16,10 +16,17 @@ inline int step_forces_grid(const DevModel& d, const BlockArrays& B, const StepL
     return S.n_cwg + B.n_blocks;
 }
 
-// workgroups of launch B: rigid bodies, node blocks, one thread per
-// rigid-body node, the BC update
+// rigid-body nodes per thread of launch B, and the workgroups they take
+constexpr int RIGID_NODES_PER_THREAD = 1;
+__host__ __device__ inline int step_rigid_node_wgs(int n_rigid_nodes) {
+    const int per = BLOCK_THREADS * RIGID_NODES_PER_THREAD;
+    return (n_rigid_nodes + per - 1) / per;
+}
+
+// workgroups of launch B: rigid bodies, node blocks, rigid-body nodes, the
+// BC update
 inline int step_finish_grid(const DevModel& d, const BlockArrays& B) {
-    return d.n_rigid + B.n_blocks + (d.n_rigid_nodes + BLOCK_THREADS - 1) / BLOCK_THREADS + 1;
+    return d.n_rigid + B.n_blocks + step_rigid_node_wgs(d.n_rigid_nodes) + 1;
 }
 
 // dynamic LDS of k_step_finish: a node block's staged point forces and

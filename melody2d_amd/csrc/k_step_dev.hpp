@@ -337,7 +337,57 @@ struct RigidForces {
     double fcon[3], fneu[3], fdir[3], fbody[2];
 };
 
-__device__ inline RigidForces rigid_forces_block(const DevModel& m, int rigid_index, double* sc) {
+// BC terms of run r (of BLOCK_THREADS runs of contiguous slots) of rigid
+// border br (Code/Body.h:816-899, rigid branch of Update_bc_forces): Neumann
+// and spring-dashpot forces of its slots and their moments about the body's
+// current centre (xc0, yc0), added in slot order; rigid_forces_block adds
+// the runs in slot order
+__device__ inline void rigid_bc_run(const DevModel& m, int br, int r, double xc0, double yc0, double (&p6)[6]) {
+    const int tid = r;
+    const int xt = m.br_xtype[br], yt = m.br_ytype[br];
+    const int ns = m.br_nslots[br];
+    const int run = (ns + BLOCK_THREADS - 1) / BLOCK_THREADS;
+    for (int q = 0; q < 6; q++) p6[q] = 0.;
+    const int sb = m.br_slot0[br];
+    const int j1 = min(ns, (tid + 1) * run);
+    // batches of RUN_BATCH slots: their nodes, then the nodes' state,
+    // loaded together (indices past the run repeat its last slot; their
+    // terms add +0, which leaves the sums unchanged: they start at +0
+    // and never become -0)
+    constexpr int RUN_BATCH = 8;
+    const bool spring_x = xt == 2, spring_y = yt == 2;
+    const double xval = m.br_xval[br], yval = m.br_yval[br];
+    for (int jb = tid * run; jb < j1; jb += RUN_BATCH) {
+        int nd[RUN_BATCH];
+        double Ls[RUN_BATCH], px[RUN_BATCH], py[RUN_BATCH];
+#pragma unroll
+        for (int u = 0; u < RUN_BATCH; u++) {
+            const int jj = min(jb + u, j1 - 1);
+            nd[u] = m.sl_node[sb + jj];
+            Ls[u] = m.sl_len[sb + jj];
+        }
+#pragma unroll
+        for (int u = 0; u < RUN_BATCH; u++) { px[u] = m.xc[nd[u]]; py[u] = m.yc[nd[u]]; }
+#pragma unroll
+        for (int u = 0; u < RUN_BATCH; u++) {
+            const int node = nd[u];
+            const double L = Ls[u];
+            double fxn = 0., fyn = 0., fxd = 0., fyd = 0.;
+            if (xt == 3 || xt == 4) fxn = xval * L;
+            else if (spring_x) fxd = (m.br_xparam[2 * br] * (xval - m.ux[node]) + m.br_xparam[2 * br + 1] * (m.br_xvel[br] - m.vx[node])) * L;
+            if (yt == 3 || yt == 4) fyn = yval * L;
+            else if (spring_y) fyd = (m.br_yparam[2 * br] * (yval - m.uy[node]) + m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node])) * L;
+            const double dx = px[u] - xc0, dy = py[u] - yc0;
+            const bool in = jb + u < j1;
+            p6[0] += in ? fxn : 0.; p6[1] += in ? fyn : 0.; p6[2] += in ? -fxn * dy + fyn * dx : 0.;
+            p6[3] += in ? fxd : 0.; p6[4] += in ? fyd : 0.; p6[5] += in ? -fxd * dy + fyd * dx : 0.;
+        }
+    }
+}
+
+// bc_pre: the BC terms of the force launch (rbc_part), or nullptr to form them here
+__device__ inline RigidForces rigid_forces_block(const DevModel& m, int rigid_index, double* sc,
+                                                 const double* bc_pre) {
     RigidForces F{};
     const int b = m.rigid_ids[rigid_index];
     const int nc = *m.c_count;
@@ -390,43 +440,12 @@ __device__ inline RigidForces rigid_forces_block(const DevModel& m, int rigid_in
         const int xt = m.br_xtype[br], yt = m.br_ytype[br];
         const bool fx_bc = xt == 2 || xt == 3 || xt == 4, fy_bc = yt == 2 || yt == 3 || yt == 4;
         if (!fx_bc && !fy_bc) continue;
-        const int ns = m.br_nslots[br];
-        const int run = (ns + nt - 1) / nt;
-        double p6[6] = {0., 0., 0., 0., 0., 0.};
-        const int sb = m.br_slot0[br];
-        const int j1 = min(ns, (tid + 1) * run);
-        // batches of RUN_BATCH slots: their nodes, then the nodes' state,
-        // loaded together (indices past the run repeat its last slot; their
-        // terms add +0, which leaves the sums unchanged: they start at +0
-        // and never become -0)
-        constexpr int RUN_BATCH = 8;
-        const bool spring_x = xt == 2, spring_y = yt == 2;
-        const double xval = m.br_xval[br], yval = m.br_yval[br];
-        for (int jb = tid * run; jb < j1; jb += RUN_BATCH) {
-            int nd[RUN_BATCH];
-            double Ls[RUN_BATCH], px[RUN_BATCH], py[RUN_BATCH];
-#pragma unroll
-            for (int u = 0; u < RUN_BATCH; u++) {
-                const int jj = min(jb + u, j1 - 1);
-                nd[u] = m.sl_node[sb + jj];
-                Ls[u] = m.sl_len[sb + jj];
-            }
-#pragma unroll
-            for (int u = 0; u < RUN_BATCH; u++) { px[u] = m.xc[nd[u]]; py[u] = m.yc[nd[u]]; }
-#pragma unroll
-            for (int u = 0; u < RUN_BATCH; u++) {
-                const int node = nd[u];
-                const double L = Ls[u];
-                double fxn = 0., fyn = 0., fxd = 0., fyd = 0.;
-                if (xt == 3 || xt == 4) fxn = xval * L;
-                else if (spring_x) fxd = (m.br_xparam[2 * br] * (xval - m.ux[node]) + m.br_xparam[2 * br + 1] * (m.br_xvel[br] - m.vx[node])) * L;
-                if (yt == 3 || yt == 4) fyn = yval * L;
-                else if (spring_y) fyd = (m.br_yparam[2 * br] * (yval - m.uy[node]) + m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node])) * L;
-                const double dx = px[u] - xc0, dy = py[u] - yc0;
-                const bool in = jb + u < j1;
-                p6[0] += in ? fxn : 0.; p6[1] += in ? fyn : 0.; p6[2] += in ? -fxn * dy + fyn * dx : 0.;
-                p6[3] += in ? fxd : 0.; p6[4] += in ? fyd : 0.; p6[5] += in ? -fxd * dy + fyd * dx : 0.;
-            }
+        double p6[6];
+        if (bc_pre && m.br_rbc[br] >= 0) {
+            const double* q6 = bc_pre + ((size_t)m.br_rbc[br] * BLOCK_THREADS + tid) * 6;
+            for (int q = 0; q < 6; q++) p6[q] = q6[q];
+        } else {
+            rigid_bc_run(m, br, tid, xc0, yc0, p6);
         }
         __syncthreads();
         for (int q = 0; q < 6; q++) part[6 * tid + q] = p6[q];
@@ -588,9 +607,21 @@ __device__ inline void rigid_body_wg(const DevModel& m, int rigid_index, double*
         rigid_dv_end_of_step(m, rigid_index, dv);
         for (int k = 0; k < 8; k++) s_dv[k] = dv[k];
     }
-    const RigidForces F = rigid_forces_block(m, rigid_index, sc);   // its barriers publish s_rec, s_dv
+    const RigidForces F = rigid_forces_block(m, rigid_index, sc, dv_pre ? m.rbc_part : nullptr);   // its barriers publish s_rec, s_dv
     __syncthreads();
     if (threadIdx.x == 0) rigid_step_body(m, b, F, s_rec, s_dv, rs);
+}
+
+// the force launch's share of a rigid border's BC terms: item j is run
+// j % BLOCK_THREADS of border rbc_border[j / BLOCK_THREADS]
+__device__ inline void rigid_bc_item(const DevModel& m, int j) {
+    const int k = j / BLOCK_THREADS, r = j - k * BLOCK_THREADS;
+    const int br = m.rbc_border[k];
+    const double* R = m.body_rigid + m.br_body[br] * RIGID_FIELDS;
+    double p6[6];
+    rigid_bc_run(m, br, r, R[R_CUR], R[R_CUR + 1], p6);
+    double* o = m.rbc_part + (size_t)j * 6;
+    for (int q = 0; q < 6; q++) o[q] = p6[q];
 }
 
 // rigid node kinematics and positions (Code/Body.h:2214-2226, 2281-2301);

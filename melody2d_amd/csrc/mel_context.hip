@@ -312,7 +312,7 @@ int contact_count(mel_ctx& c) {
 // workgroups of BLOCK_THREADS that the contact kernels start with: the
 // last known count with room to grow (the kernels loop over the exact one)
 int contact_workgroups(const mel_ctx& c) {
-    long long n = (long long)c.nc_hint + c.nc_hint / 4 + 64 + c.blocks_dev.n_bc_bn;
+    long long n = (long long)c.nc_hint + c.nc_hint / 4 + 64 + c.blocks_dev.n_bc_bn + (long long)c.d.n_rbc * BLOCK_THREADS;
     return (int)std::max<long long>(1, (n + BLOCK_THREADS - 1) / BLOCK_THREADS);
 }
 
@@ -1259,6 +1259,21 @@ void build(mel_ctx& c) {
     d.br_periodicity = c.upload(br_per);
     d.br_xtype = c.upload(br_xt);
     d.br_ytype = c.upload(br_yt);
+    {
+        // borders of rigid bodies whose BCs give forces: their terms are
+        // formed by the force launch, run by run
+        auto force = [](int t) { return t == 2 || t == 3 || t == 4; };
+        std::vector<int> rbc, br_rbc(std::max(NBR, 1), -1);
+        for (int br = 0; br < NBR; br++)
+            if (rigid[br_body[br]] && (force(br_xt[br]) || force(br_yt[br]))) {
+                br_rbc[br] = (int)rbc.size();
+                rbc.push_back(br);
+            }
+        d.n_rbc = (int)rbc.size();
+        d.rbc_border = c.upload(rbc.empty() ? std::vector<int>{0} : rbc);
+        d.br_rbc = c.upload(br_rbc);
+        d.rbc_part = c.alloc<double>((size_t)std::max(1, d.n_rbc) * BLOCK_THREADS * 6);
+    }
     {
         // Update_bc changes the values of types 2, 3 and 4 only (type 0 stays
         // zero, type 1 drives the DOFs directly)

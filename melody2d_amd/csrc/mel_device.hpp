@@ -149,6 +149,11 @@ struct DevModel {
     double *time;                            // device copy of the simulation time
     double *rigid_dv;                        // per rigid body: driven DOFs at the end of the current
                                              // step (8 each, as R_DRIVEN), formed by the force launch
+    int n_rbc;                               // borders of rigid bodies with force BCs (types 2, 3, 4)
+    int *rbc_border;                         // their global border ids
+    int *br_rbc;                             // per global border: its index in rbc_border, or -1
+    double *rbc_part;                        // per such border and run: the run's BC terms (6 each),
+                                             // formed by the force launch
     unsigned *step_sync;                     // step launch B: [1] rigid-body workgroups done
 };
 
