@@ -35,7 +35,7 @@ k_block_force_update(DevModel m, BlockArrays B) {
 
 __global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_block_force_update32(DevModel m, BlockArrays B) {
-    __shared__ __align__(16) blockdev::Fp32::slot s_cxy[CHUNK_ENTRIES + 1];
+    __shared__ __align__(16) blockdev::Fp32::slot s_cxy[2 * (CHUNK_ENTRIES + 1)];
     double fi[2][NODES_PER_THREAD];
     block_force_update<blockdev::Fp32>(m, B, blockIdx.x, s_cxy, fi);
 }

@@ -150,6 +150,7 @@ __device__ inline float2 contribution32(float c0, f32x2 f, const float4& P) {
 struct Fp64 {
     static constexpr bool f32 = false;
     static constexpr bool dbuf = false;     // one chunk's entries in flight (registers)
+    static constexpr bool pipe = false;     // one contribution buffer
     static constexpr bool nt_f = true, nt_pk = false;
     static constexpr int KREG = KREG_ENTRIES;
     typedef f64x2 vec;
@@ -160,6 +161,7 @@ struct Fp64 {
 struct Fp32 {
     static constexpr bool f32 = true;
     static constexpr bool dbuf = true;      // two chunks' entries in flight
+    static constexpr bool pipe = true;      // two contribution buffers (2 * (CHUNK_ENTRIES + 1) slots)
     static constexpr bool nt_f = true, nt_pk = false;
     static constexpr int KREG = KREG_ENTRIES_F32;
     typedef f32x2 vec;
@@ -330,7 +332,8 @@ __device__ inline void gp_lane32(const BlockArrays& B, const LaneMap& L, const L
 // Internal forces of the owned nodes of node block blk, written to fix, fiy
 // and returned in fi[0][q], fi[1][q] for the thread's owned node
 // o = threadIdx.x + q * BLOCK_THREADS.  s_cxy: the caller's LDS array of
-// CHUNK_ENTRIES + 1 contribution slots (the last one is the sink).
+// CHUNK_ENTRIES + 1 contribution slots (the last one is the sink), twice
+// that when PR::pipe.
 template <class PR>
 __device__ __forceinline__ void block_force_update(DevModel& m, const BlockArrays& B, int blk,
                                                    typename PR::slot* s_cxy, double (&fi)[2][NODES_PER_THREAD]) {
@@ -440,7 +443,57 @@ __device__ __forceinline__ void block_force_update(DevModel& m, const BlockArray
             }
         }
     };
-    if constexpr (PR::dbuf) {
+    if constexpr (PR::pipe) {
+        // two contribution buffers: chunk c's Gauss points write buffer
+        // c & 1 while the slots of chunk c - 1 are summed from the other,
+        // with one barrier per chunk; each node's slots are still summed
+        // chunk by chunk, in their order.  Single-region blocks only (the
+        // FP32-mixed form always builds them).
+        LaneRegs<PR> R2;
+        if (nch > 1) load_lane<PR>(B, gmeta[1], gpo[1], nown, t, R2);
+        int ptp0[NODES_PER_THREAD], ptp1[NODES_PER_THREAD];
+        auto sum = [&](int c) {
+            const slot_t* buf = s_cxy + (c & 1) * (CHUNK_ENTRIES + 1);
+#pragma unroll
+            for (int q = 0; q < NODES_PER_THREAD; q++) {
+                int o = t + q * BLOCK_THREADS;
+                if (o >= nown) break;
+                if (ptp1[q] > ptp0[q]) cur[q] = 0;
+#pragma unroll 4
+                for (int p = ptp0[q]; p < ptp1[q]; p++) {
+                    slot_t v = buf[p];
+                    rx[q] -= (double)v.x;
+                    ry[q] -= (double)v.y;
+                }
+            }
+        };
+        auto gp = [&](int c, LaneRegs<PR>& Rc) {
+            __syncthreads();      // chunk c - 1's contributions visible; buffer c & 1 summed
+            const int4 meta = s_meta[c];
+            const LaneMap L = lane_map(meta, t);
+            slot_t* buf = s_cxy + (c & 1) * (CHUNK_ENTRIES + 1);
+            if (L.valid) {
+                const f32x2* uv = reinterpret_cast<const f32x2*>(s_uv);
+                if (mtype == 0) gp_lane32<0>(B, L, Rc, uv, buf, s_creg, true, (float)Mu, (float)K2);
+                else gp_lane32<1>(B, L, Rc, uv, buf, s_creg, true, (float)Mu, (float)K2);
+            }
+            int tp0[NODES_PER_THREAD], tp1[NODES_PER_THREAD];
+#pragma unroll
+            for (int q = 0; q < NODES_PER_THREAD; q++) { tp0[q] = Rc.tp0[q]; tp1[q] = Rc.tp1[q]; }
+            if (c + 2 < nch) load_lane<PR>(B, s_meta[c + 2], s_po[c + 2], nown, t, Rc);
+            if (c > 0) sum(c - 1);
+#pragma unroll
+            for (int q = 0; q < NODES_PER_THREAD; q++) { ptp0[q] = tp0[q]; ptp1[q] = tp1[q]; }
+        };
+        for (int c = 0; c < nch; c += 2) {
+            gp(c, R);
+            if (c + 1 < nch) gp(c + 1, R2);
+        }
+        if (nch > 0) {
+            __syncthreads();      // the last chunk's contributions visible
+            sum(nch - 1);
+        }
+    } else if constexpr (PR::dbuf) {
         // chunk c + 1's entries load during the whole of chunk c
         LaneRegs<PR> R2;
         if (nch > 1) load_lane<PR>(B, gmeta[1], gpo[1], nown, t, R2);

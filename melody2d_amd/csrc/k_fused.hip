@@ -114,7 +114,7 @@ k_step_forces(DevModel m, BlockArrays B, StepLaunch S) {
 
 __global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_step_forces32(DevModel m, BlockArrays B, StepLaunch S) {
-    __shared__ __align__(16) blockdev::Fp32::slot s_cxy[CHUNK_ENTRIES + 1];
+    __shared__ __align__(16) blockdev::Fp32::slot s_cxy[2 * (CHUNK_ENTRIES + 1)];
     step_forces<blockdev::Fp32>(m, B, S, s_cxy);
 }
 

@@ -356,9 +356,11 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
         if (v >= 1 && v < BLOCK_MAX_OWN) own_cap = v;
     }
     // MELODY2D_EXACT_SUMS=1: internal forces summed in the reference's order
-    // (bit-identical where pow rounds alike); default: balanced order
+    // (bit-identical where pow rounds alike); default: balanced order.  The
+    // FP32-mixed form, which rounds differently anyway, always takes the
+    // balanced order (its kernels assume single-region blocks).
     const char* ex = std::getenv("MELODY2D_EXACT_SUMS");
-    const bool exact = ex && std::atoi(ex) != 0;
+    const bool exact = ex && std::atoi(ex) != 0 && c.precision != MEL_FP32_MIXED;
     for (size_t b = 0; b < m.bodies.size(); b++) {
         const Body& Bd = m.bodies[b];
         if (Bd.rigid) continue;
