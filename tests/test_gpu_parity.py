@@ -91,18 +91,30 @@ def test_exact_sum_order_matches_oracle(name, golden_dir, tmp_path, monkeypatch)
 
 @pytest.mark.parametrize("name", VARIANTS)
 def test_two_launch_step_equals_launch_sequence(name, golden_dir, tmp_path, monkeypatch):
-    """The step in two launches (contacts, rigid bodies and BC update beside
-    the internal forces, then the node-block finish) and the launch sequence
-    (MELODY2D_FUSED_STEP=0) run the same device routines in the same
+    """The step in two launches (contact elements beside the internal forces,
+    then rigid bodies, node-block finish and BC update) and the launch
+    sequence (MELODY2D_FUSED_STEP=0) run the same device routines in the same
     orders: every record is bit-identical after 1, 10 and 200 steps."""
+    _two_launch_vs_sequence(name, golden_dir, tmp_path, monkeypatch, native.FP64)
+
+
+@pytest.mark.parametrize("name", ["tiny_platen", "tiny_czm", "tiny_bc"])
+def test_two_launch_step_equals_launch_sequence_fp32_mixed(name, golden_dir, tmp_path, monkeypatch):
+    """The same in the FP32-mixed form, whose internal-force pass sums one
+    chunk while the next chunk's Gauss points write the other contribution
+    buffer, in both paths."""
+    _two_launch_vs_sequence(name, golden_dir, tmp_path, monkeypatch, native.FP32_MIXED)
+
+
+def _two_launch_vs_sequence(name, golden_dir, tmp_path, monkeypatch, precision):
     model, _ = golden_dir(name)
-    print("case two-launch vs sequence", name, flush=True)
+    print("case two-launch vs sequence", name, precision, flush=True)
     refs, _ = run_oracle(model, [0], str(tmp_path / "oracle"))
     nn, nbr, sizes = layout_from_dump(refs[0])
-    two = native.Context(model)
+    two = native.Context(model, precision=precision)
     two.set_kernel_timing(["step_forces"])
     monkeypatch.setenv("MELODY2D_FUSED_STEP", "0")
-    seq = native.Context(model)
+    seq = native.Context(model, precision=precision)
     seq.set_kernel_timing(["step_forces"])
     prev = 0
     for s in (1, 10, 200):
