@@ -95,6 +95,66 @@ def cpu_baseline(model_path, n_nodes, steps, threads):
     return n_nodes * steps / sec.value, sec.value
 
 
+def serve_leg(args, native, tmpdir):
+    """Continuous batching through the job server: Poisson arrivals of
+    independent granular jobs of log-uniform size, half with a cohesive
+    contact law, admitted under an HBM budget and run concurrently; the same
+    arrival schedule in FP64 and in the FP32-mixed form.  Latency is
+    completion minus arrival (queueing, loading and stepping)."""
+    import math
+    import numpy as np
+    from melody2d_amd.preprocessor import configs
+    from melody2d_amd.preprocessor.binary import write_binary
+    rng = np.random.default_rng(4242)
+    lo, hi = args.serve_grains
+    t0 = time.time()
+    jobs = []
+    for k in range(args.serve_jobs):
+        g = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        mu = float(rng.uniform(0.2, 0.6))
+        m = configs.c2(n_grains=g, seed=3000 + k, mu=mu)
+        if k % 2:
+            # cohesive: peak traction 1 MPa, linear softening over 10 um
+            m = configs._set_law(m, "CZMlinear", [1e13, 1e6, 1e6 / 1e13 + 1e-5, 0.0, 1.0, 2e-6])
+        p = os.path.join(tmpdir, "job%03d.bin" % k)
+        write_binary(m, p)
+        jobs.append((p, m.n_nodes))
+    gaps = rng.exponential(1.0 / args.serve_rate, args.serve_jobs)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    log("serve: generated %d jobs (%d..%d nodes) in %.1fs" % (
+        len(jobs), min(n for _, n in jobs), max(n for _, n in jobs), time.time() - t0))
+    out = {}
+    for prec, name in ((native.FP64, "fp64"), (native.FP32_MIXED, "fp32_mixed")):
+        with native.Server(hbm_budget=int(args.serve_budget_gb * 1e9), steps_per_slice=args.serve_slice) as srv:
+            ids = [srv.submit(p, args.serve_steps, precision=prec, arrival=float(a))
+                   for (p, _), a in zip(jobs, arrivals)]
+            left = srv.run(timeout=600.0)
+            infos = [srv.job(j) for j in ids]
+            failed = [(j, srv.job_error(j)) for j, inf in zip(ids, infos) if inf.state != native.JOB_DONE]
+            if left or failed:
+                raise RuntimeError("serve (%s): %d unfinished, failed %s" % (name, left, failed[:3]))
+        lat = np.array([inf.finished - inf.arrival for inf in infos])
+        load = np.array([inf.admitted - inf.arrival for inf in infos])
+        span = max(inf.finished for inf in infos) - min(inf.arrival for inf in infos)
+        work = sum(inf.n_nodes * inf.steps_done for inf in infos)
+        out[name] = {"p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+                     "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+                     "p50_wait_s": round(float(np.percentile(load, 50)), 4),
+                     "node_steps_per_s": round(work / span, 1), "makespan_s": round(span, 3)}
+        log("serve %s: p50 %.3fs p99 %.3fs, %.3e node-steps/s over %.2fs" % (
+            name, out[name]["p50_latency_s"], out[name]["p99_latency_s"], work / span, span))
+    for p, _ in jobs:
+        os.remove(p)
+    out.update({"jobs": len(jobs), "steps_per_job": args.serve_steps,
+                "nodes_min": min(n for _, n in jobs), "nodes_max": max(n for _, n in jobs),
+                "nodes_total": sum(n for _, n in jobs), "arrival_rate_per_s": args.serve_rate,
+                "hbm_budget_gb": args.serve_budget_gb, "steps_per_slice": args.serve_slice,
+                "workload": "Poisson arrivals of biaxial granular jobs, grains log-uniform in [%d, %d] "
+                            "(~210 nodes each), Coulomb mu ~ U[0.2, 0.6], every second job CZMlinear"
+                            % (lo, hi)})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +169,12 @@ def main():
                     help="FP64 step at which the FP32-mixed force error is evaluated")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (scale reference for HBM byte counters)")
+    ap.add_argument("--serve-jobs", type=int, default=16, help="jobs of the serving leg (0: skip it)")
+    ap.add_argument("--serve-steps", type=int, default=5000)
+    ap.add_argument("--serve-grains", type=int, nargs=2, default=[20, 200])
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="job arrivals per second")
+    ap.add_argument("--serve-slice", type=int, default=100, help="steps per slice of a running job")
+    ap.add_argument("--serve-budget-gb", type=float, default=250.0)
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -244,6 +310,7 @@ def main():
                "sample": "%d Euler steps of the same %d-node workload after its initialisation, "
                          "FP64 OpenMP oracle" % (args.cpu_steps, info.n_nodes)}
     os.remove(path)
+    serving = serve_leg(args, native, tmpdir) if args.serve_jobs > 0 else None
     os.rmdir(tmpdir)
 
     out = {
@@ -272,6 +339,8 @@ def main():
                        "dtype": "f32 gradients/stresses/contributions, f64 state and sums",
                        "roofline": roof32},
     }
+    if serving:
+        out["serving"] = serving
     if breakdown:
         out["kernel_us"] = breakdown
         out["fp32_mixed"]["kernel_us"] = breakdown32

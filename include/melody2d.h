@@ -132,6 +132,60 @@ void mel_destroy(mel_ctx* ctx);
 const char* mel_last_error(const mel_ctx* ctx);
 const char* mel_version(void);
 
+/* device memory the context holds, in bytes (its footprint for admission by
+ * the job server below); no reference counterpart */
+mel_status mel_get_device_bytes(const mel_ctx* ctx, size_t* bytes);
+
+/* ---- job server: many independent jobs on one GPU at once ----
+ * The reference runs one model per process (Code/main.cpp, with the
+ * Euler_step loop of Code/main.cpp:141-238 until Tend); a batch of models is
+ * a series of such processes.  The server replaces that series: each job is
+ * one binary model container (as for mel_load_binary) run for a step count,
+ * in its own context on its own stream, so the jobs' work overlaps on the
+ * device and each job's state is exactly what it would be alone.  Jobs are
+ * admitted in arrival order while their device footprint fits the budget
+ * (estimated from the container's size until the job is loaded); running
+ * jobs are kept `steps_per_slice`-step slices ahead of the device.  One
+ * thread drives a server (loading included); distinct servers are
+ * independent. */
+typedef struct mel_server mel_server;
+
+typedef enum {
+    MEL_JOB_WAITING = 0,    /* submitted, not yet arrived or not yet admitted */
+    MEL_JOB_RUNNING = 1,
+    MEL_JOB_DONE = 2,       /* all steps ran; the context is held if the job was kept */
+    MEL_JOB_FAILED = 3,     /* see status and mel_server_job_error */
+    MEL_JOB_RELEASED = 4    /* done, and its context given back by mel_server_release */
+} mel_job_state;
+
+typedef struct {
+    int state;              /* mel_job_state */
+    int status;             /* mel_status of a failed job */
+    long long steps, steps_done;
+    double arrival;         /* seconds from the first mel_server_run, as submitted */
+    double admitted;        /* when its loading began; -1 when not yet */
+    double finished;        /* when its last step was seen complete; -1 when not yet */
+    size_t device_bytes;    /* footprint once loaded */
+    int n_nodes;
+} mel_job_info_t;
+
+mel_status mel_server_create(int hip_device, size_t hbm_budget, long long steps_per_slice, mel_server** out);
+/* queue a job that arrives `arrival_s` seconds after the first
+ * mel_server_run; keep != 0 holds its context once done (read its state
+ * through mel_server_job_context, then mel_server_release) */
+mel_status mel_server_submit(mel_server* s, const char* binary_path, mel_precision precision, long long nsteps,
+                             double arrival_s, int keep, int* job);
+/* drive the loop until every submitted job is done or failed, or until
+ * `timeout_s` seconds have passed; may be called again (more submissions in
+ * between are fine) */
+mel_status mel_server_run(mel_server* s, double timeout_s, int* n_unfinished);
+mel_status mel_server_job(const mel_server* s, int job, mel_job_info_t* info);
+mel_status mel_server_job_context(mel_server* s, int job, mel_ctx** ctx);
+mel_status mel_server_release(mel_server* s, int job);
+mel_status mel_server_usage(const mel_server* s, size_t* used, size_t* budget);
+const char* mel_server_job_error(const mel_server* s, int job);
+void mel_server_destroy(mel_server* s);
+
 #ifdef __cplusplus
 }
 #endif

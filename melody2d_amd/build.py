@@ -19,7 +19,7 @@ ORACLE = os.path.join(ROOT, "oracle")
 REFERENCE = "/root/reference"
 
 HIP_SOURCES = ["mel_context.hip", "k_step.hip", "k_block.hip", "k_fused.hip", "k_proximity.hip"]
-CXX_SOURCES = ["model_io.cpp"]
+CXX_SOURCES = ["model_io.cpp", "mel_serve.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

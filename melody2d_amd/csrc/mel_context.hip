@@ -104,6 +104,7 @@ struct mel_ctx {
     int device = 0;
     int precision = MEL_FP64;
     std::vector<void*> allocs;
+    size_t dev_bytes = 0;        // device memory held by allocs
     std::string err;
     // host mirrors
     std::vector<int> h_node_off, h_border_off, h_gp_off, h_rigid_ids;
@@ -154,6 +155,7 @@ struct mel_ctx {
         size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
         if (hipMalloc(&p, bytes) != hipSuccess) throw HipError{"hipMalloc of " + std::to_string(bytes) + " bytes failed"};
         allocs.push_back(p);
+        dev_bytes += bytes;
         HIP_OK(hipMemsetAsync(p, 0, bytes, stream));
         return (T*)p;
     }
@@ -1810,6 +1812,12 @@ mel_status mel_write_dynamic(mel_ctx* c, const char* path) {
         if (!write_dynamic_text(path, m, err)) { c->err = err; return MEL_E_IO; }
         return MEL_OK;
     });
+}
+
+mel_status mel_get_device_bytes(const mel_ctx* c, size_t* bytes) {
+    if (!c || !bytes) return MEL_E_ARG;
+    *bytes = c->dev_bytes;
+    return MEL_OK;
 }
 
 mel_status mel_get_stream(mel_ctx* c, void** stream) {

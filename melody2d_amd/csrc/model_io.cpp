@@ -619,18 +619,28 @@ struct Rec {
 
 bool load_model_binary(const std::string& path, Model& m, std::string& err) {
     m = Model();
-    std::ifstream in(path, std::ios::binary);
+    std::ifstream in(path, std::ios::binary | std::ios::ate);
     if (!in) { err = "cannot open " + path; return false; }
+    // every length is checked against the bytes left before anything is
+    // allocated, so a damaged container is an error, not a huge allocation
+    const uint64_t size = (uint64_t)in.tellg();
+    in.seekg(0);
+    auto left = [&]() { return size - (uint64_t)in.tellg(); };
     std::map<std::string, Rec> recs;
     while (true) {
         uint32_t l;
         if (!in.read((char*)&l, 4)) break;
+        if ((uint64_t)l + 9 > left()) { err = "malformed record header in " + path; return false; }
         std::string name(l, '\0');
         in.read(&name[0], l);
         uint8_t kind;
         uint64_t n;
         in.read((char*)&kind, 1);
         in.read((char*)&n, 8);
+        if (!in || kind > 1 || n > left() / (kind == 0 ? 8 : 4)) {
+            err = "malformed record " + name.substr(0, 64) + " in " + path;
+            return false;
+        }
         Rec& r = recs[name];
         r.kind = kind;
         if (kind == 0) { r.d.resize(n); in.read((char*)r.d.data(), 8 * n); }

@@ -42,6 +42,15 @@ class Info(ctypes.Structure):
                 ("precision", ctypes.c_int)]
 
 
+JOB_WAITING, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_RELEASED = range(5)
+
+
+class JobInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("status", ctypes.c_int), ("steps", ctypes.c_longlong),
+                ("steps_done", ctypes.c_longlong), ("arrival", ctypes.c_double), ("admitted", ctypes.c_double),
+                ("finished", ctypes.c_double), ("device_bytes", ctypes.c_size_t), ("n_nodes", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -79,6 +88,17 @@ def library():
             "mel_destroy": ([vp], None),
             "mel_last_error": ([vp], ctypes.c_char_p),
             "mel_version": ([], ctypes.c_char_p),
+            "mel_get_device_bytes": ([vp, P(ctypes.c_size_t)], ctypes.c_int),
+            "mel_server_create": ([ctypes.c_int, ctypes.c_size_t, ctypes.c_longlong, P(vp)], ctypes.c_int),
+            "mel_server_submit": ([vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_double,
+                                   ctypes.c_int, P(ctypes.c_int)], ctypes.c_int),
+            "mel_server_run": ([vp, ctypes.c_double, P(ctypes.c_int)], ctypes.c_int),
+            "mel_server_job": ([vp, ctypes.c_int, P(JobInfo)], ctypes.c_int),
+            "mel_server_job_context": ([vp, ctypes.c_int, P(vp)], ctypes.c_int),
+            "mel_server_release": ([vp, ctypes.c_int], ctypes.c_int),
+            "mel_server_usage": ([vp, P(ctypes.c_size_t), P(ctypes.c_size_t)], ctypes.c_int),
+            "mel_server_job_error": ([vp, ctypes.c_int], ctypes.c_char_p),
+            "mel_server_destroy": ([vp], None),
         }
         for name, (args, res) in sigs.items():
             f = getattr(lib, name)
@@ -93,7 +113,10 @@ def exported_symbols():
             "mel_get_node_field", "mel_set_node_field", "mel_get_contacts", "mel_get_neighbours",
             "mel_get_body_state", "mel_get_border_field", "mel_get_info", "mel_write_dynamic", "mel_get_stream",
             "mel_set_kernel_timing", "mel_get_kernel_timing", "mel_kernel_count", "mel_kernel_name",
-            "mel_destroy", "mel_last_error", "mel_version"]
+            "mel_destroy", "mel_last_error", "mel_version", "mel_get_device_bytes",
+            "mel_server_create", "mel_server_submit", "mel_server_run", "mel_server_job",
+            "mel_server_job_context", "mel_server_release", "mel_server_usage", "mel_server_job_error",
+            "mel_server_destroy"]
 
 
 class Context:
@@ -110,7 +133,18 @@ class Context:
             raise MelError(st, lib.mel_last_error(None).decode())
         self._h = h
         self.lib = lib
+        self._owner = None
         self.info = self.get_info()
+
+    @classmethod
+    def _borrowed(cls, handle, owner):
+        """A view of a context that `owner` (a Server) holds and destroys."""
+        self = cls.__new__(cls)
+        self._h = handle
+        self.lib = library()
+        self._owner = owner
+        self.info = self.get_info()
+        return self
 
     def _check(self, st):
         if st != MEL_OK:
@@ -118,8 +152,14 @@ class Context:
 
     def close(self):
         if getattr(self, "_h", None):
-            self.lib.mel_destroy(self._h)
+            if getattr(self, "_owner", None) is None:
+                self.lib.mel_destroy(self._h)
             self._h = None
+
+    def device_bytes(self):
+        n = ctypes.c_size_t()
+        self._check(self.lib.mel_get_device_bytes(self._h, ctypes.byref(n)))
+        return n.value
 
     def __del__(self):
         try:
@@ -232,3 +272,83 @@ class Context:
 
     def write_dynamic(self, path):
         self._check(self.lib.mel_write_dynamic(self._h, os.fsencode(path)))
+
+
+class Server:
+    """Job server (mel_server_*): independent jobs, each a binary model
+    container run for a number of steps, admitted by device footprint under
+    an HBM budget and run concurrently on one device."""
+
+    def __init__(self, hbm_budget, steps_per_slice=50, device=0):
+        lib = library()
+        h = ctypes.c_void_p()
+        st = lib.mel_server_create(device, int(hbm_budget), int(steps_per_slice), ctypes.byref(h))
+        if st != MEL_OK:
+            raise MelError(st, "mel_server_create")
+        self._h = h
+        self.lib = lib
+        self._views = {}
+
+    def _check(self, st, job=-1):
+        if st != MEL_OK:
+            raise MelError(st, self.lib.mel_server_job_error(self._h, job).decode())
+
+    def submit(self, path, steps, precision=FP64, arrival=0.0, keep=False):
+        j = ctypes.c_int()
+        self._check(self.lib.mel_server_submit(self._h, os.fsencode(path), precision, int(steps), float(arrival),
+                                               int(bool(keep)), ctypes.byref(j)))
+        return j.value
+
+    def run(self, timeout=3600.0):
+        """Drive the loop until every job is done or failed; returns the
+        number still unfinished when `timeout` seconds have passed."""
+        left = ctypes.c_int()
+        self._check(self.lib.mel_server_run(self._h, float(timeout), ctypes.byref(left)))
+        return left.value
+
+    def job(self, j):
+        inf = JobInfo()
+        self._check(self.lib.mel_server_job(self._h, j, ctypes.byref(inf)))
+        return inf
+
+    def job_error(self, j):
+        return self.lib.mel_server_job_error(self._h, j).decode()
+
+    def context(self, j):
+        """The held context of a kept, done job (valid until release)."""
+        if j not in self._views:
+            h = ctypes.c_void_p()
+            self._check(self.lib.mel_server_job_context(self._h, j, ctypes.byref(h)), j)
+            self._views[j] = Context._borrowed(h, self)
+        return self._views[j]
+
+    def release(self, j):
+        v = self._views.pop(j, None)
+        if v is not None:
+            v._h = None
+        self._check(self.lib.mel_server_release(self._h, j), j)
+
+    def usage(self):
+        u, b = ctypes.c_size_t(), ctypes.c_size_t()
+        self._check(self.lib.mel_server_usage(self._h, ctypes.byref(u), ctypes.byref(b)))
+        return u.value, b.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for v in self._views.values():
+                v._h = None
+            self._views = {}
+            self.lib.mel_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -34,3 +34,42 @@ def test_missing_model_reports_io_error(tmp_path):
     with pytest.raises(native.MelError) as e:
         native.Context(str(tmp_path / "nowhere"))
     assert e.value.status == 2
+
+
+def test_damaged_container_reports_parse_error(tmp_path):
+    """The binary container reader checks every record length against the
+    bytes left in the file: random bytes and a cut-off container are parse
+    errors (before any device call), not allocation failures."""
+    import pytest
+    import numpy as np
+    from melody2d_amd.preprocessor import configs
+    from melody2d_amd.preprocessor.binary import write_binary
+    rng = np.random.default_rng(5)
+    bad = tmp_path / "random.bin"
+    bad.write_bytes(rng.integers(0, 256, 4096, dtype=np.uint8).tobytes())
+    good = tmp_path / "tiny.bin"
+    write_binary(configs.variants()["tiny"], str(good))
+    cut = tmp_path / "cut.bin"
+    data = good.read_bytes()
+    cut.write_bytes(data[: len(data) // 2])
+    for p in (bad, cut):
+        with pytest.raises(native.MelError) as e:
+            native.Context(str(p), binary=True)
+        assert e.value.status == 3, p
+    with pytest.raises(native.MelError) as e:
+        native.Context(str(tmp_path / "missing.bin"), binary=True)
+    assert e.value.status == 2
+
+
+def test_server_entry_points_check_arguments():
+    """Argument checks of the job server that need no device."""
+    lib = native.library()
+    info = native.JobInfo()
+    assert lib.mel_server_job(None, 0, ctypes.byref(info)) == 1
+    assert lib.mel_server_run(None, 1.0, None) == 1
+    assert lib.mel_server_release(None, 0) == 1
+    assert lib.mel_get_device_bytes(None, None) == 1
+    h = ctypes.c_void_p()
+    assert lib.mel_server_create(0, 0, 10, ctypes.byref(h)) == 1      # no budget
+    assert lib.mel_server_create(0, 1 << 30, 0, ctypes.byref(h)) == 1  # no slice
+    lib.mel_server_destroy(None)
