@@ -160,7 +160,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", choices=["granular", "plate"], default="granular",
+                    help="granular: biaxial compression of --grains grains (the headline); plate: the "
+                         "perforated RPIM plate of --plate-nodes nodes, internal forces only")
     ap.add_argument("--grains", type=int, default=1000)
+    ap.add_argument("--plate-nodes", type=int, default=1000000)
     ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -192,16 +196,36 @@ def main():
         sys.exit(2)
 
     from melody2d_amd import native
-    from melody2d_amd.preprocessor import c2
+    from melody2d_amd.preprocessor import c2, configs
     from melody2d_amd.preprocessor.binary import write_binary
 
     t0 = time.time()
-    model = c2(n_grains=args.grains)
+    plate = args.workload == "plate"
+    if plate:
+        model = configs.c3(n_nodes=args.plate_nodes)
+        args.serve_jobs = 0
+    else:
+        model = c2(n_grains=args.grains)
     log("generated %d bodies, %d nodes, %d Gauss points in %.1fs" % (len(model.bodies), model.n_nodes,
                                                                      model.n_gauss, time.time() - t0))
     tmpdir = tempfile.mkdtemp(prefix="mel_bench_")
     path = os.path.join(tmpdir, "c2.bin")
+    t0 = time.time()
     write_binary(model, path)
+    log("wrote %.0f MB in %.1fs" % (os.path.getsize(path) / 1e6, time.time() - t0))
+
+    def uniaxial(ctx):
+        # the plate's displacement and velocity parameters: the uniaxial
+        # field of a 10/s strain rate after 1000 steps (RPIM interpolates,
+        # so parameters are displacements)
+        b = model.bodies[0]
+        rate = 10.0
+        eps = rate * 1000 * model.dt
+        x, y = b.X[:, 0], b.X[:, 1]
+        nu = model.materials[0][2][4]
+        for f, v in (("x_disp_param", eps * x), ("y_disp_param", -nu * eps * y),
+                     ("x_vel_param", rate * x), ("y_vel_param", -nu * rate * y)):
+            ctx.set_node_field(0, f, v)
 
     def run(precision):
         name = "fp64" if precision == native.FP64 else "fp32-mixed"
@@ -210,6 +234,8 @@ def main():
         info = ctx.get_info()
         log("[%s] loaded in %.1fs: nodes %d, gauss %d, nnz %d, contacts %d" % (
             name, time.time() - t0, info.n_nodes, info.n_gauss, info.nnz_gauss, info.n_contacts))
+        if plate:
+            uniaxial(ctx)
         ctx.euler_steps(args.warmup)
         ctx.synchronize()
         # value window: the steps as a user runs them (runs between contact
@@ -302,6 +328,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        if plate:
+            # one body: only its internal forces spread over the threads
+            args.cpu_steps = min(args.cpu_steps, 10)
         t0 = time.time()
         v, sc = cpu_baseline(path, info.n_nodes, args.cpu_steps, threads)
         log("cpu baseline: %.3e node-steps/s over %d steps (%.1fs stepping, %.1fs total)" % (
@@ -326,7 +355,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded generator)",
-        "config": {"workload": "biaxial compression, %d deformable grains + 4 rigid walls (FP64 drop-in)" % args.grains,
+        "config": {"workload": ("perforated plate, %d RPIM nodes (30-node supports), %d holes, linear elastic, "
+                                "uniaxial 10/s field, no contact (FP64 drop-in)" % (info.n_nodes, len(model.holes))
+                                if plate else
+                                "biaxial compression, %d deformable grains + 4 rigid walls (FP64 drop-in)" % args.grains),
                    "nodes": info.n_nodes, "gauss_points": info.n_gauss, "support_entries": info.nnz_gauss,
                    "contacts": info.n_contacts, "precision": "fp64", "parallelism": "single GPU"},
         "roofline": roof,

@@ -40,6 +40,22 @@ k_block_force_update32(DevModel m, BlockArrays B) {
     block_force_update<blockdev::Fp32>(m, B, blockIdx.x, s_cxy, fi);
 }
 
+// the same with the wide register window (supports of more than
+// KREG_ENTRIES / KREG_ENTRIES_F32 entries)
+__global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_block_force_update_wide(DevModel m, BlockArrays B) {
+    __shared__ __align__(16) blockdev::Fp64W::slot s_cxy[CHUNK_ENTRIES + 1];
+    double fi[2][NODES_PER_THREAD];
+    block_force_update<blockdev::Fp64W>(m, B, blockIdx.x, s_cxy, fi);
+}
+
+__global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_block_force_update32_wide(DevModel m, BlockArrays B) {
+    __shared__ __align__(16) blockdev::Fp32W::slot s_cxy[2 * (CHUNK_ENTRIES + 1)];
+    double fi[2][NODES_PER_THREAD];
+    block_force_update<blockdev::Fp32W>(m, B, blockIdx.x, s_cxy, fi);
+}
+
 // DOF_TO_DISP gather (Body::Update_kinematics, Code/Body.h:2176-2212) and
 // current positions for the listed nodes
 __global__ void k_kinematics_list(DevModel m, const int* nodes, int n) {

@@ -31,6 +31,7 @@ constexpr int BLOCK_MAX_OWN = BLOCK_THREADS * NODES_PER_THREAD;
 constexpr int CHUNK_ENTRIES = 2048;                                 // owned-node entry slots per chunk
 constexpr int KREG_ENTRIES = 20;                                    // support entries held in registers per lane (FP64)
 constexpr int KREG_ENTRIES_F32 = 24;                                // the same for the FP32-mixed form
+constexpr int KREG_ENTRIES_WIDE = 30;                               // the wide window (both forms)
 constexpr int MAX_BLOCK_CHUNKS = 64;                                // chunks per node block
 // LDS scratch (doubles) of a workgroup that steps one rigid body
 // (rigid_body_wg): reductions, the body's record, its driven DOFs, BC flags
@@ -87,6 +88,8 @@ struct BlockArrays {
 
 __global__ void k_block_force_update(DevModel m, BlockArrays B);
 __global__ void k_block_force_update32(DevModel m, BlockArrays B);
+__global__ void k_block_force_update_wide(DevModel m, BlockArrays B);
+__global__ void k_block_force_update32_wide(DevModel m, BlockArrays B);
 __global__ void k_update_finish(DevModel m, BlockArrays B, double t_next);
 __global__ void k_kinematics_list(DevModel m, const int* nodes, int n);
 __global__ void k_node_outputs(DevModel m);

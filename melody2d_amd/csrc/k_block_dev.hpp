@@ -153,6 +153,7 @@ struct Fp64 {
     static constexpr bool pipe = false;     // one contribution buffer
     static constexpr bool nt_f = true, nt_pk = false;
     static constexpr int KREG = KREG_ENTRIES;
+    static constexpr int GATHER = KREG_ENTRIES / 2;   // displacement gathers in flight per batch
     typedef f64x2 vec;
     typedef double2 slot;
     typedef double2 disp;
@@ -164,10 +165,25 @@ struct Fp32 {
     static constexpr bool pipe = true;      // two contribution buffers (2 * (CHUNK_ENTRIES + 1) slots)
     static constexpr bool nt_f = true, nt_pk = false;
     static constexpr int KREG = KREG_ENTRIES_F32;
+    static constexpr int GATHER = KREG_ENTRIES_F32 / 2;
     typedef f32x2 vec;
     typedef float2 slot;
     typedef float2 disp;
     typedef float real;
+};
+// wide register windows, for models whose Gauss points have more support
+// entries than the windows above (RPIM supports of 30 nodes): entries past
+// the window are read from memory one dependent load after another, which
+// is what a window that holds them avoids
+struct Fp64W : Fp64 {
+    static constexpr int KREG = KREG_ENTRIES_WIDE;
+    static constexpr int GATHER = 8;
+};
+struct Fp32W : Fp32 {
+    static constexpr bool dbuf = false;     // one chunk's entries in flight: two wide windows do not fit
+    static constexpr bool pipe = false;
+    static constexpr int KREG = KREG_ENTRIES_WIDE;
+    static constexpr int GATHER = 8;
 };
 
 // a value that is the same in every lane of the wave, moved to a scalar
@@ -251,78 +267,73 @@ __device__ inline void load_lane(const BlockArrays& B, int4 meta, int2 po, int n
 }
 
 // the Gauss-point work of one lane: gradient, stress, contributions to slots
-template <int MAT>
-__device__ inline void gp_lane(const BlockArrays& B, const LaneMap& L, const LaneRegs<Fp64>& R, const f64x2* uv,
+// (PR::KREG entries from registers, in batches of PR::GATHER displacement
+// gathers; entries past the window from memory)
+template <int MAT, class PR>
+__device__ inline void gp_lane(const BlockArrays& B, const LaneMap& L, const LaneRegs<PR>& R, const f64x2* uv,
                                double2* s_cxy, uint16_t* s_creg, bool single, double Mu, double K2) {
-    constexpr int HALF = KREG_ENTRIES / 2;
+    constexpr int KR = PR::KREG, GB = PR::GATHER;
     double a = MAT == 0 ? 0. : 1., b = 0., c = 0., d = MAT == 0 ? 0. : 1.;
-    // displacement gathers in two batches of register entries
-    f64x2 u[HALF];
+    f64x2 u[GB];
 #pragma unroll
-    for (int i = 0; i < HALF; i++)
-        if (i < L.K) u[i] = uv[R.pk[i] & 0xFFFF];
+    for (int j0 = 0; j0 < KR; j0 += GB) {
 #pragma unroll
-    for (int i = 0; i < HALF; i++)
-        if (i < L.K) grad_add<MAT>(a, b, c, d, R.f[i], u[i]);
+        for (int i = 0; i < GB; i++)
+            if (j0 + i < KR && j0 + i < L.K) u[i] = uv[R.pk[j0 + i] & 0xFFFF];
 #pragma unroll
-    for (int i = 0; i < HALF; i++)
-        if (HALF + i < L.K) u[i] = uv[R.pk[HALF + i] & 0xFFFF];
-#pragma unroll
-    for (int i = 0; i < HALF; i++)
-        if (HALF + i < L.K) grad_add<MAT>(a, b, c, d, R.f[HALF + i], u[i]);
+        for (int i = 0; i < GB; i++)
+            if (j0 + i < KR && j0 + i < L.K) grad_add<MAT>(a, b, c, d, R.f[j0 + i], u[i]);
+    }
     // supports longer than the register window: straight from memory
     const f64x2* fxy = reinterpret_cast<const f64x2*>(B.be_fxy);
-    for (int i = KREG_ENTRIES; i < L.K; i++)
+    for (int i = KR; i < L.K; i++)
         grad_add<MAT>(a, b, c, d, lane_entry(fxy, L, i), uv[lane_entry(B.be_pk, L, i) & 0xFFFF]);
     const double4 P = stress_terms<MAT>(Mu, K2, a, b, c, d);
 #pragma unroll
-    for (int i = 0; i < KREG_ENTRIES; i++)
+    for (int i = 0; i < KR; i++)
         if (i < L.K) s_cxy[R.pk[i] >> 16] = contribution(R.c0, R.f[i], P);
-    for (int i = KREG_ENTRIES; i < L.K; i++)
+    for (int i = KR; i < L.K; i++)
         s_cxy[lane_entry(B.be_pk, L, i) >> 16] = contribution(R.c0, lane_entry(fxy, L, i), P);
     if (!single) {
 #pragma unroll
-        for (int i = 0; i < KREG_ENTRIES; i++)
+        for (int i = 0; i < KR; i++)
             if (i < L.K) s_creg[R.pk[i] >> 16] = (uint16_t)R.reg;
-        for (int i = KREG_ENTRIES; i < L.K; i++)
+        for (int i = KR; i < L.K; i++)
             s_creg[lane_entry(B.be_pk, L, i) >> 16] = (uint16_t)R.reg;
     }
 }
 
 // the same in the FP32-mixed form (H sums from anchored displacements)
-template <int MAT>
-__device__ inline void gp_lane32(const BlockArrays& B, const LaneMap& L, const LaneRegs<Fp32>& R, const f32x2* uv,
+template <int MAT, class PR>
+__device__ inline void gp_lane32(const BlockArrays& B, const LaneMap& L, const LaneRegs<PR>& R, const f32x2* uv,
                                  float2* s_cxy, uint16_t* s_creg, bool single, float Mu, float K2) {
-    constexpr int HALF = KREG_ENTRIES_F32 / 2;
+    constexpr int KR = PR::KREG, GB = PR::GATHER;
     float a = 0.f, b = 0.f, c = 0.f, d = 0.f;
-    f32x2 u[HALF];
+    f32x2 u[GB];
 #pragma unroll
-    for (int i = 0; i < HALF; i++)
-        if (i < L.K) u[i] = uv[R.pk[i] & 0xFFFF];
+    for (int j0 = 0; j0 < KR; j0 += GB) {
 #pragma unroll
-    for (int i = 0; i < HALF; i++)
-        if (i < L.K) grad_add32<MAT>(a, b, c, d, R.f[i], u[i]);
+        for (int i = 0; i < GB; i++)
+            if (j0 + i < KR && j0 + i < L.K) u[i] = uv[R.pk[j0 + i] & 0xFFFF];
 #pragma unroll
-    for (int i = 0; i < HALF; i++)
-        if (HALF + i < L.K) u[i] = uv[R.pk[HALF + i] & 0xFFFF];
-#pragma unroll
-    for (int i = 0; i < HALF; i++)
-        if (HALF + i < L.K) grad_add32<MAT>(a, b, c, d, R.f[HALF + i], u[i]);
+        for (int i = 0; i < GB; i++)
+            if (j0 + i < KR && j0 + i < L.K) grad_add32<MAT>(a, b, c, d, R.f[j0 + i], u[i]);
+    }
     const f32x2* fxy = reinterpret_cast<const f32x2*>(B.be_fxy32);
-    for (int i = KREG_ENTRIES_F32; i < L.K; i++)
+    for (int i = KR; i < L.K; i++)
         grad_add32<MAT>(a, b, c, d, lane_entry(fxy, L, i), uv[lane_entry(B.be_pk, L, i) & 0xFFFF]);
     const float4 P = stress_terms32<MAT>(Mu, K2, a, b, c, d);
     const float c0 = (float)R.c0;
 #pragma unroll
-    for (int i = 0; i < KREG_ENTRIES_F32; i++)
+    for (int i = 0; i < KR; i++)
         if (i < L.K) s_cxy[R.pk[i] >> 16] = contribution32(c0, R.f[i], P);
-    for (int i = KREG_ENTRIES_F32; i < L.K; i++)
+    for (int i = KR; i < L.K; i++)
         s_cxy[lane_entry(B.be_pk, L, i) >> 16] = contribution32(c0, lane_entry(fxy, L, i), P);
     if (!single) {
 #pragma unroll
-        for (int i = 0; i < KREG_ENTRIES_F32; i++)
+        for (int i = 0; i < KR; i++)
             if (i < L.K) s_creg[R.pk[i] >> 16] = (uint16_t)R.reg;
-        for (int i = KREG_ENTRIES_F32; i < L.K; i++)
+        for (int i = KR; i < L.K; i++)
             s_creg[lane_entry(B.be_pk, L, i) >> 16] = (uint16_t)R.reg;
     }
 }

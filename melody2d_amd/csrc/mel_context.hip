@@ -128,6 +128,9 @@ struct mel_ctx {
     ContactArrays bufA{}, bufB{};
     BlockArrays blocks_dev{};
     size_t block_lds = 0;
+    // internal-force kernel with the wide register window (some Gauss point
+    // has more support entries than the default window holds)
+    bool wide = false;
     // step in two launches (every deformable body in one node block), else
     // the launch sequence on two streams
     bool two_launch = false;
@@ -724,13 +727,29 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     if (c.block_lds + 16 * ((size_t)CHUNK_ENTRIES + 1) + 2048 > 160 * 1024)
         throw CapacityError{"node block needs more LDS than a workgroup has"};
     const bool f32 = c.precision == MEL_FP32_MIXED;
-    HIP_OK(hipFuncSetAttribute(f32 ? (const void*)k_block_force_update32 : (const void*)k_block_force_update,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.block_lds));
     // two launches per step unless a body is split over several node blocks
     // (its blocks read each other's displacement parameters);
     // MELODY2D_FUSED_STEP=0 asks for the launch sequence
     const char* fe = getenv("MELODY2D_FUSED_STEP");
     c.two_launch = B.lds_finish_ok && !(fe && fe[0] == '0');
+    // the launch sequence takes the wide-window kernel when Gauss points
+    // with more support entries than the default window are common (RPIM
+    // supports of 30 nodes; a few long MLS supports read their tail from
+    // memory at less cost than the wide window's register pressure);
+    // MELODY2D_WIDE_LANES=1 asks for the launch sequence with that kernel
+    const int window = f32 ? KREG_ENTRIES_F32 : KREG_ENTRIES;
+    long long n_gp = 0, n_long = 0;
+    for (size_t g = 0; g + 1 < g_ptr.size(); g++) {
+        n_gp++;
+        n_long += g_ptr[g + 1] - g_ptr[g] > window;
+    }
+    const char* we = getenv("MELODY2D_WIDE_LANES");
+    const bool force_wide = we && we[0] == '1';
+    if (force_wide) c.two_launch = false;
+    c.wide = force_wide || (!c.two_launch && 10 * n_long >= n_gp && n_long > 0);
+    HIP_OK(hipFuncSetAttribute(f32 ? (c.wide ? (const void*)k_block_force_update32_wide : (const void*)k_block_force_update32)
+                                   : (c.wide ? (const void*)k_block_force_update_wide : (const void*)k_block_force_update),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.block_lds));
     // runs of steps replayed from captured graphs; MELODY2D_GRAPHS=0 launches
     // every step
     const char* ge = getenv("MELODY2D_GRAPHS");
@@ -865,8 +884,10 @@ void euler_step(mel_ctx& c) {
     if (B.n_blocks > 0)
         TIMED(c, KT_BLOCK_UPDATE, s,
               (c.precision == MEL_FP32_MIXED
-                   ? k_block_force_update32<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B)
-                   : k_block_force_update<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B)));
+                   ? (c.wide ? k_block_force_update32_wide<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B)
+                             : k_block_force_update32<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B))
+                   : (c.wide ? k_block_force_update_wide<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B)
+                             : k_block_force_update<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B))));
     HIP_OK(hipStreamWaitEvent(s, c.ev_join, 0));
     // sends, Sum_up_forces, Apply_Newton, Apply_Euler, Update_borders,
     // Update_kinematics of border nodes (interior ones are rebuilt on request)

@@ -78,7 +78,8 @@ def write_binary(model: Model, path: str):
             _rec(f, p + "hist_t", np.array(ht, dtype=float))
             _rec(f, p + "hist_v", np.array(hv, dtype=float))
             _rec(f, p + "cells", np.asarray(b.tri, dtype=np.int32).reshape(-1))
-            vx, vy = b.velocity
+            vel = np.asarray(b.velocity, dtype=float)
+            vx, vy = (vel[:, 0], vel[:, 1]) if vel.ndim == 2 else (float(vel[0]), float(vel[1]))
             if not b.rigid:
                 _rec(f, p + "domain", b.domain)
                 _rec(f, p + "mx", b.mass[:, 0])

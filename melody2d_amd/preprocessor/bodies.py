@@ -59,7 +59,7 @@ class Body:
     contact: float
     periodicity: str = "Simple"
     status: str = "active"
-    velocity: tuple = (0.0, 0.0)
+    velocity: tuple = (0.0, 0.0)   # (vx, vy), or per node an (n, 2) array (deformable bodies)
     # deformable only
     domain: np.ndarray | None = None
     mass: np.ndarray | None = None          # (N,2)
@@ -209,13 +209,36 @@ def closed_border(n_border):
     return Border(nodes=nodes)
 
 
+def _rpim_chunk(job):
+    pts, X, nn, c = job
+    k = nn.shape[1]
+    ptr = np.arange(len(pts) + 1, dtype=np.int64) * k
+    return shape.rpim(pts, X, ptr, nn.reshape(-1).astype(np.int64), shape_c=c)
+
+
+def _rpim(points, X, nn, c, workers):
+    """RPIM over fixed-size supports (rows of nn), in point chunks on worker
+    processes when there are many points (the result does not depend on the
+    worker count: every point is solved on its own)."""
+    n = len(points)
+    if workers is None or workers <= 1 or n < 200000:
+        return _rpim_chunk((points, X, nn, c))
+    import concurrent.futures as cf
+    step = 50000
+    jobs = [(points[a:a + step], X, nn[a:a + step], c) for a in range(0, n, step)]
+    with cf.ProcessPoolExecutor(max_workers=workers) as ex:
+        parts = list(ex.map(_rpim_chunk, jobs))
+    return tuple(np.concatenate([p[i] for p in parts]) for i in range(3))
+
+
 def deformable_body(border_xy, interior_xy, material, h, rho, radius_factor=2.5,
                     detection=None, contact=None, nregions=1, velocity=(0.0, 0.0),
-                    keep_triangle=None, kind="MLS", rpim_c=1.0, rpim_support=30):
+                    keep_triangle=None, kind="MLS", rpim_c=1.0, rpim_support=30, workers=None):
     """Assemble a deformable meshfree body with one closed border.
 
     kind "MLS": node radius R = radius_factor * sqrt(area / N) (uniform);
-    kind "RPIM": supports are the `rpim_support` nearest nodes of each point.
+    kind "RPIM": supports are the `rpim_support` nearest nodes of each point
+    (shape functions on `workers` processes for large bodies).
     """
     X = np.concatenate([border_xy, interior_xy], axis=0)
     nb = len(border_xy)
@@ -235,16 +258,16 @@ def deformable_body(border_xy, interior_xy, material, h, rho, radius_factor=2.5,
         dphi, dphix, dphiy = shape.mls(X, X, domain, dptr, didx)
     else:
         tree = cKDTree(X)
-        _, nn = tree.query(gp, k=rpim_support)
+        _, nn = tree.query(gp, k=rpim_support, workers=-1)
         nn = np.sort(nn, axis=1)
         gptr = np.arange(len(gp) + 1, dtype=np.int64) * rpim_support
         gidx = nn.reshape(-1).astype(np.int64)
-        phi, phix, phiy = shape.rpim(gp, X, gptr, gidx, shape_c=rpim_c)
-        _, nn2 = tree.query(X, k=rpim_support)
+        phi, phix, phiy = _rpim(gp, X, nn, rpim_c, workers)
+        _, nn2 = tree.query(X, k=rpim_support, workers=-1)
         nn2 = np.sort(nn2, axis=1)
         dptr = np.arange(len(X) + 1, dtype=np.int64) * rpim_support
         didx = nn2.reshape(-1).astype(np.int64)
-        dphi, dphix, dphiy = shape.rpim(X, X, dptr, didx, shape_c=rpim_c)
+        dphi, dphix, dphiy = _rpim(X, X, nn2, rpim_c, workers)
     # lumped mass: rho * sum_g c0 * phi_j * sum_k phi_k  (row sum of consistent mass)
     c0 = w * jac
     rowsum = np.add.reduceat(phi, gptr[:-1]) if len(phi) else np.zeros(len(gp))

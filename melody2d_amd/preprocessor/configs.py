@@ -16,6 +16,9 @@ Workloads
               (mass scaling expressed as density).  1000 grains cannot fit
               a 40 x 40 mm box (their area is ~3200 mm^2), so the box side
               is chosen for a 0.45 area fraction (~84 mm for 1000 grains).
+  c3          a 0.1 x 0.1 m plate with random circular holes (porosity 0.3),
+              ~1M RPIM nodes, linear elastic, no contact: the
+              internal-force workload.
 """
 from __future__ import annotations
 
@@ -219,6 +222,70 @@ def c2(n_grains=1000, seed=1002, nodes_per_grain=200, area_fraction=0.45, mu=0.5
     return m
 
 
+def c3(n_nodes=1_000_000, seed=1003, side=0.1, porosity=0.3, rmin=1.0e-3, rmax=4.0e-3, E=7.0e10, nu=0.33,
+       rho=2700.0, nregions=64, workers=None):
+    """A side x side plate with seeded random circular holes (radius
+    U[rmin, rmax], no overlap, until the hole area reaches `porosity`),
+    ~n_nodes RPIM nodes (30-node supports, Gaussian RBF exp(-(3 d / D)^2)
+    with D the support size, as Preprocessor/Gauss_RBF.m), three Gauss
+    points per triangle, linear elastic.  The internal-force workload: one
+    body, no contact (NO_MONITORING, no contact update within a run).
+
+    Node cloud: the outer rectangle's border nodes plus a jittered grid at
+    the spacing that gives n_nodes over the solid area; triangles whose
+    centroid falls in a hole are dropped, so holes are free surfaces without
+    border nodes.  Masses are lumped uniformly (rho x solid area / N per
+    node) and kept as written (no UPDATE_MASS_MATRIX): the row sums of RPIM
+    shape functions are not all positive."""
+    from .bodies import rectangle_border
+    if workers is None:
+        workers = min(16, os.cpu_count() or 1)
+    rng_holes, rng_nodes = _streams(seed, 2)
+    holes = []
+    target = porosity * side * side
+    area = 0.0
+    tries = 0
+    while area < target and tries < 200000:
+        tries += 1
+        r = rng_holes.uniform(rmin, rmax)
+        x, y = rng_holes.uniform(r + 0.5e-3, side - r - 0.5e-3, 2)
+        if any((x - a) ** 2 + (y - b) ** 2 < (r + c + 0.2e-3) ** 2 for a, b, c in holes):
+            continue
+        holes.append((x, y, r))
+        area += math.pi * r * r
+    H = np.array(holes)
+    solid = side * side - area
+    h = math.sqrt(solid / n_nodes)
+
+    def in_holes(p, grow=0.0):
+        inside = np.zeros(len(p), dtype=bool)
+        for x, y, r in H:
+            inside |= (p[:, 0] - x) ** 2 + (p[:, 1] - y) ** 2 < (r + grow) ** 2
+        return inside
+
+    border = rectangle_border(0.0, 0.0, side, side, h)
+    g = np.arange(h, side - 0.5 * h, h)
+    gx, gy = np.meshgrid(g, g, indexing="ij")
+    pts = np.stack([gx.ravel(), gy.ravel()], axis=1)
+    pts += rng_nodes.uniform(-0.25 * h, 0.25 * h, pts.shape)
+    interior = pts[~in_holes(pts, 0.3 * h)]
+    body = deformable_body(border, interior, "Plate", h, rho, kind="RPIM", rpim_c=3.0, rpim_support=30,
+                           keep_triangle=lambda c: ~in_holes(c), nregions=nregions, workers=workers,
+                           detection=0.5 * h, contact=0.5 * h)
+    body.mass = np.full((body.n_nodes, 2), rho * solid / body.n_nodes)
+    m = Model(name="c3_plate_%d" % body.n_nodes, materials=[("Plate", "ElasticLinear", [rho, 0.0, 0.0, E, nu, 1.0])],
+              contact_laws=[("Plate", "Plate", "Frictionless", "Evolutive", [1e13])], bodies=[body],
+              periodic=(-1.0, 1.0))
+    m.flags = [f for f in m.flags if f != "UPDATE_MASS_MATRIX"] + ["NO_MONITORING"]
+    # RPIM stiffness is higher than a linear element's at the same spacing:
+    # its stable step is about 0.2 h / c, so half of that is used
+    m.dt = 0.1 * h / math.sqrt(E / rho)
+    m.contact_update_period = 1.0e6 * m.dt
+    m.tend = 2000 * m.dt
+    m.holes = H
+    return m
+
+
 def _set_law(m, typ, params, evolution="Evolutive"):
     m.contact_laws = [(a, b, typ, evolution, list(params)) for (a, b, _, _, _) in m.contact_laws]
     return m
@@ -266,4 +333,4 @@ def variants():
     return v
 
 
-WORKLOADS = {"tiny": tiny, "tiny_platen": lambda: tiny(with_platens=True), "c1": c1, "c2": c2}
+WORKLOADS = {"tiny": tiny, "tiny_platen": lambda: tiny(with_platens=True), "c1": c1, "c2": c2, "c3": c3}

@@ -191,16 +191,19 @@ def write_dynamic(model: Model, d: str, number_save: int = 0):
     out.append(" ")
     for i, b in enumerate(model.bodies):
         _body_banner(out, i, b.rigid)
-        vx, vy = b.velocity
+        vel = np.asarray(b.velocity, dtype=float)
         if not b.rigid:
+            V = vel if vel.ndim == 2 else np.tile(vel, (b.n_nodes, 1))
             out += ["DEFORMABLE", b.status, "%d" % i, " ", "KINEMATICS", "%d" % b.n_nodes]
             for j in range(b.n_nodes):
+                vx, vy = V[j]
                 out.append("%d %s %s 0 0 %s %s 0 0 0 0 %s %s 0 0" % (j, g(b.X[j, 0]), g(b.X[j, 1]), g(vx), g(vy), g(vx), g(vy)))
             out.append(" ")
             out.append("FORCES")
             out += ["%d 0 0 0 0 0 0 0 0 0 0 0 0 0 0 0 0" % j for j in range(b.n_nodes)]
             out.append(" ")
         else:
+            vx, vy = vel
             out += ["RIGID", b.status, "%d" % i, " ", "KINEMATICS",
                     "%s %s 0" % (g(b.centre[0]), g(b.centre[1])), "0 0 0", "%s %s 0" % (g(vx), g(vy)), " "]
             out += ["FORCES"] + ["0 0 0"] * 6 + [" "]

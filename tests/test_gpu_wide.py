@@ -1,0 +1,87 @@
+"""The internal-force kernel with the wide register window (Gauss points with
+more support entries than the default window, as RPIM supports of 30
+nodes): bit-identical to the default kernel where both apply, and on a
+perforated RPIM plate, where it is the kernel the library selects, equal to
+the oracle within the FP64 drop-in bar."""
+import numpy as np
+import pytest
+
+from mel_compare import compare, device_state, layout_from_dump, run_oracle
+from melody2d_amd import native
+from melody2d_amd.preprocessor import configs
+from melody2d_amd.preprocessor.binary import write_binary
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["x_disp_param", "y_disp_param", "x_vel_param", "y_vel_param", "x_internal_force", "y_internal_force",
+          "x_contact_force", "y_contact_force", "x_current", "y_current"]
+
+
+def _fields(ctx, nodes):
+    return {"b%d.%s" % (b, f): ctx.node_field(b, f, n) for b, n in enumerate(nodes) for f in FIELDS}
+
+
+@pytest.mark.parametrize("precision", [native.FP64, native.FP32_MIXED])
+@pytest.mark.parametrize("name", ["tiny", "tiny_czm", "tiny_platen"])
+def test_wide_window_equals_default(name, precision, tmp_path, monkeypatch):
+    print("case wide vs default", name, precision, flush=True)
+    m = configs.variants()[name]
+    path = str(tmp_path / "m.bin")
+    write_binary(m, path)
+    nodes = [b.n_nodes for b in m.bodies]
+    monkeypatch.setenv("MELODY2D_FUSED_STEP", "0")
+    ref = native.Context(path, binary=True, precision=precision)
+    monkeypatch.setenv("MELODY2D_WIDE_LANES", "1")
+    wide = native.Context(path, binary=True, precision=precision)
+    prev = 0
+    for s in (1, 10, 60):
+        for c in (ref, wide):
+            c.euler_steps(s - prev)
+        prev = s
+        a, b = _fields(ref, nodes), _fields(wide, nodes)
+        diff = [k for k in a if not np.array_equal(a[k], b[k])]
+        assert not diff, "step %d: %s" % (s, diff[:6])
+    ref.close()
+    wide.close()
+
+
+def plate(n_nodes=2500):
+    m = configs.c3(n_nodes=n_nodes, workers=1)
+    b = m.bodies[0]
+    # start from the uniaxial 10/s velocity field, so the plate strains
+    v = np.stack([10.0 * b.X[:, 0], -0.33 * 10.0 * b.X[:, 1]], axis=1)
+    b.velocity = v
+    return m
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_rpim_plate_matches_oracle(exact, tmp_path, monkeypatch):
+    """Balanced sums: within 1e-12 through step 10 (RPIM node sums cancel
+    more than MLS ones, so rounding-order differences grow along the
+    trajectory: 2e-11 by step 50).  The reference's summation order
+    (MELODY2D_EXACT_SUMS=1): within 1e-12 through step 50."""
+    print("case rpim plate exact=%d" % exact, flush=True)
+    if exact:
+        monkeypatch.setenv("MELODY2D_EXACT_SUMS", "1")
+    m = plate()
+    path = str(tmp_path / "plate.bin")
+    write_binary(m, path)
+    assert max(np.diff(m.bodies[0].gp_ptr)) == 30
+    steps = [0, 1, 10, 50] if exact else [0, 1, 10]
+    refs, _ = run_oracle(path, steps, str(tmp_path / "oracle"), save_index=-1)
+    ctx = native.Context(path, binary=True)
+    ctx.set_kernel_timing(["block_force_update", "step_forces"])
+    prev = 0
+    for s in steps:
+        ctx.euler_steps(s - prev)
+        prev = s
+        nn, nbr, sizes = layout_from_dump(refs[s])
+        res = compare(device_state(ctx, nn, nbr, sizes), refs[s])
+        bad = {k: v[0] for k, v in res.items() if not v[0] <= 1e-12}
+        assert not bad, "step %d: %s" % (s, sorted(bad.items(), key=lambda kv: -kv[1])[:8])
+    f = np.abs(ctx.node_field(0, "x_internal_force", nn[0])).max()
+    assert f > 0
+    kt = ctx.kernel_timing()
+    # the launch sequence (wide window), not the two-launch step
+    assert kt["block_force_update"][1] == steps[-1] and kt["step_forces"][1] == 0
+    ctx.close()
