@@ -463,8 +463,16 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
                 // gather), cut into waves; a chunk pairs the heaviest
                 // remaining wave with the lightest one when both fit.  Sum
                 // order: chunk by chunk, Gauss-point order inside a chunk.
+                // a chunk's capacity is in contribution slots, which only
+                // the entries of owned nodes take (the others go to the sink):
+                // a block of a split body streams the whole support of its
+                // Gauss points but fills slots with about half of it
+                std::vector<int> own_k(gps.size(), 0);   // per position in gps
                 long long tot = 0;
-                for (int g : gps) tot += g_ptr[g + 1] - g_ptr[g];
+                for (size_t a = 0; a < gps.size(); a++) {
+                    for (int k = g_ptr[gps[a]]; k < g_ptr[gps[a] + 1]; k++) own_k[a] += local[g_idx[k]] < nown;
+                    tot += own_k[a];
+                }
                 const int nrr = (int)std::max<long long>({1, (tot + ECAP * 15 / 16 - 1) / (ECAP * 15 / 16),
                                                           ((long long)gps.size() + CHCAP - 1) / CHCAP});
                 std::vector<int> ord(gps.size());
@@ -481,7 +489,7 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
                     std::vector<int> wv;
                     int s = 0;
                     while (a < ord.size() && wv.size() < 64) {
-                        const int cnt = g_ptr[gps[ord[a]] + 1] - g_ptr[gps[ord[a]]];
+                        const int cnt = own_k[ord[a]];
                         if (s + cnt > ECAP) break;
                         wv.push_back(gps[ord[a++]]);
                         s += cnt;
@@ -510,7 +518,7 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
                     bool ok = true;
                     for (size_t a = 0; a < gps.size() && ok; a++) {
                         const int k = (int)(a % nch), g = gps[a];
-                        ne[k] += g_ptr[g + 1] - g_ptr[g];
+                        ne[k] += own_k[a];
                         rr[k].push_back(g);
                         ok = ne[k] <= ECAP && (int)rr[k].size() <= CHCAP;
                     }
@@ -548,6 +556,7 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
                 tptr.push_back(0);
                 std::vector<int> next(nown);
                 for (int i = 0; i < nown; i++) { next[i] = acc; acc += cnt[i]; tptr.push_back(acc); }
+                if (acc > CHUNK_ENTRIES) throw CapacityError{"chunk has more contribution slots than CHUNK_ENTRIES"};
                 for (int i = 0; i < ngp; i++)
                     for (int k = g_ptr[chunk[i]]; k < g_ptr[chunk[i] + 1]; k++) {
                         int li = local[g_idx[k]];
