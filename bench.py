@@ -155,6 +155,27 @@ def serve_leg(args, native, tmpdir):
     return out
 
 
+def run_serve_child(args):
+    """The serving leg in a child process of its own (its contexts, streams
+    and graphs come and go by the dozen), so that whatever happens there
+    cannot take the headline figures with it."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--serve-only", "--serve-jobs", str(args.serve_jobs),
+           "--serve-steps", str(args.serve_steps), "--serve-grains", str(args.serve_grains[0]),
+           str(args.serve_grains[1]), "--serve-rate", str(args.serve_rate), "--serve-slice", str(args.serve_slice),
+           "--serve-budget-gb", str(args.serve_budget_gb)]
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, timeout=900)
+    except subprocess.TimeoutExpired:
+        log("serve: child timed out")
+        return {"error": "timed out"}
+    lines = r.stdout.decode().strip().splitlines()
+    if r.returncode != 0 or not lines:
+        log("serve: child exited with %d" % r.returncode)
+        return {"error": "exit status %d" % r.returncode}
+    return json.loads(lines[-1])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,7 +200,15 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=8.0, help="job arrivals per second")
     ap.add_argument("--serve-slice", type=int, default=100, help="steps per slice of a running job")
     ap.add_argument("--serve-budget-gb", type=float, default=250.0)
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving leg and print its JSON object (the child process of a full run)")
     args = ap.parse_args()
+    if args.serve_only:
+        from melody2d_amd import native
+        d = tempfile.mkdtemp(prefix="mel_serve_")
+        print(json.dumps(serve_leg(args, native, d)), flush=True)
+        os.rmdir(d)
+        return
     if args.copy_calibration:
         import torch
         n = args.copy_calibration * (1 << 20) // 8
@@ -339,8 +368,8 @@ def main():
                "sample": "%d Euler steps of the same %d-node workload after its initialisation, "
                          "FP64 OpenMP oracle" % (args.cpu_steps, info.n_nodes)}
     os.remove(path)
-    serving = serve_leg(args, native, tmpdir) if args.serve_jobs > 0 else None
     os.rmdir(tmpdir)
+    serving = run_serve_child(args) if args.serve_jobs > 0 else None
 
     out = {
         "metric": "node-steps/s, FP32-mixed vs FP64 at \u22641e-5 rel force err; p50/p99 job latency",
