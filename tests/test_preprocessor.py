@@ -76,3 +76,40 @@ def test_border_is_counter_clockwise_and_closed(body):
     P = body.X[br.nodes[:-1]]
     area2 = np.sum(P[:, 0] * np.roll(P[:, 1], -1) - np.roll(P[:, 0], -1) * P[:, 1])
     assert area2 > 0
+
+
+def test_plate_generator_rpim_consistency():
+    """The perforated RPIM plate: shape functions reproduce constants and
+    linear fields (values and derivatives) at every Gauss point and node, no
+    Gauss point lies in a hole, masses are positive and sum to rho x the
+    solid area, and every Gauss point has the 30-node support."""
+    import numpy as np
+    from melody2d_amd.preprocessor import configs
+    m = configs.c3(n_nodes=4000, side=0.02, workers=1)
+    b = m.bodies[0]
+    X = b.X
+    ptr, idx = b.gp_ptr, b.gp_idx
+    assert set(np.diff(ptr)) == {30}
+    seg = ptr[:-1]
+    one = np.add.reduceat(b.gp_phi, seg)
+    assert np.abs(one - 1).max() < 1e-9
+    for d, phid in ((0, b.gp_phix), (1, b.gp_phiy)):
+        assert np.abs(np.add.reduceat(phid, seg)).max() * b.nodal_distance < 1e-8
+        for e in (0, 1):
+            g = np.add.reduceat(phid * X[idx, e], seg)
+            assert np.abs(g - (1.0 if d == e else 0.0)).max() < 1e-8
+        xg = np.add.reduceat(b.gp_phi * X[idx, d], seg)
+        assert np.abs(xg - b.gp_xy[:, d]).max() < 1e-9 * 0.1
+    dseg = b.d2d_ptr[:-1]
+    assert np.abs(np.add.reduceat(b.d2d_phi, dseg) - 1).max() < 1e-9
+    h = b.nodal_distance
+    for x, y, r in m.holes:
+        # triangles are kept by their centroid: a Gauss point can sit inside
+        # a hole's edge by less than a spacing
+        assert np.min((b.gp_xy[:, 0] - x) ** 2 + (b.gp_xy[:, 1] - y) ** 2) > (r - h) ** 2
+    rho = m.materials[0][2][0]
+    solid = 0.02 ** 2 - sum(np.pi * r * r for _, _, r in m.holes)
+    assert b.mass.min() > 0 and abs(b.mass[:, 0].sum() - rho * solid) < 1e-9 * rho * solid
+    # triangles tile the solid, within half a spacing along the hole edges
+    # (kept whole, the holes would add 40 %)
+    assert abs(np.sum(b.gp_w * b.gp_jac) - solid) < 0.05 * solid
