@@ -125,7 +125,8 @@ def serve_leg(args, native, tmpdir):
         len(jobs), min(n for _, n in jobs), max(n for _, n in jobs), time.time() - t0))
     out = {}
     for prec, name in ((native.FP64, "fp64"), (native.FP32_MIXED, "fp32_mixed")):
-        with native.Server(hbm_budget=int(args.serve_budget_gb * 1e9), steps_per_slice=args.serve_slice) as srv:
+        with native.Server(hbm_budget=int(args.serve_budget_gb * 1e9), steps_per_slice=args.serve_slice,
+                           graph_replay=bool(args.serve_graphs)) as srv:
             ids = [srv.submit(p, args.serve_steps, precision=prec, arrival=float(a))
                    for (p, _), a in zip(jobs, arrivals)]
             left = srv.run(timeout=600.0)
@@ -149,6 +150,8 @@ def serve_leg(args, native, tmpdir):
                 "nodes_min": min(n for _, n in jobs), "nodes_max": max(n for _, n in jobs),
                 "nodes_total": sum(n for _, n in jobs), "arrival_rate_per_s": args.serve_rate,
                 "hbm_budget_gb": args.serve_budget_gb, "steps_per_slice": args.serve_slice,
+                "graph_replay": bool(args.serve_graphs),
+                "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                 "workload": "Poisson arrivals of biaxial granular jobs, grains log-uniform in [%d, %d] "
                             "(~210 nodes each), Coulomb mu ~ U[0.2, 0.6], every second job CZMlinear"
                             % (lo, hi)})
@@ -163,9 +166,11 @@ def run_serve_child(args):
     cmd = [sys.executable, os.path.abspath(__file__), "--serve-only", "--serve-jobs", str(args.serve_jobs),
            "--serve-steps", str(args.serve_steps), "--serve-grains", str(args.serve_grains[0]),
            str(args.serve_grains[1]), "--serve-rate", str(args.serve_rate), "--serve-slice", str(args.serve_slice),
-           "--serve-budget-gb", str(args.serve_budget_gb)]
+           "--serve-budget-gb", str(args.serve_budget_gb), "--serve-graphs", str(args.serve_graphs)]
+    # more hardware queues let more of the jobs' streams run at once
+    env = dict(os.environ, GPU_MAX_HW_QUEUES=str(max(1, min(32, args.serve_hw_queues))))
     try:
-        r = subprocess.run(cmd, stdout=subprocess.PIPE, timeout=900)
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, timeout=900, env=env)
     except subprocess.TimeoutExpired:
         log("serve: child timed out")
         return {"error": "timed out"}
@@ -200,6 +205,10 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=8.0, help="job arrivals per second")
     ap.add_argument("--serve-slice", type=int, default=100, help="steps per slice of a running job")
     ap.add_argument("--serve-budget-gb", type=float, default=250.0)
+    ap.add_argument("--serve-graphs", type=int, default=0,
+                    help="1: the jobs replay captured graphs (their launches then overlap less)")
+    ap.add_argument("--serve-hw-queues", type=int, default=16,
+                    help="hardware queues of the serving child process (GPU_MAX_HW_QUEUES, at most 32)")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving leg and print its JSON object (the child process of a full run)")
     args = ap.parse_args()

@@ -136,6 +136,12 @@ const char* mel_version(void);
  * the job server below); no reference counterpart */
 mel_status mel_get_device_bytes(const mel_ctx* ctx, size_t* bytes);
 
+/* runs of steps replayed from captured HIP graphs (on = 1, the default of
+ * the two-launch step) or launched step by step (on = 0: the replays of
+ * several contexts overlap less on the device than their launches do); no
+ * reference counterpart */
+mel_status mel_set_graph_replay(mel_ctx* ctx, int on);
+
 /* ---- job server: many independent jobs on one GPU at once ----
  * The reference runs one model per process (Code/main.cpp, with the
  * Euler_step loop of Code/main.cpp:141-238 until Tend); a batch of models is
@@ -183,6 +189,9 @@ mel_status mel_server_job(const mel_server* s, int job, mel_job_info_t* info);
 mel_status mel_server_job_context(mel_server* s, int job, mel_ctx** ctx);
 mel_status mel_server_release(mel_server* s, int job);
 mel_status mel_server_usage(const mel_server* s, size_t* used, size_t* budget);
+/* graph replay (mel_set_graph_replay) of the jobs loaded from now on;
+ * default on */
+mel_status mel_server_set_graph_replay(mel_server* s, int on);
 const char* mel_server_job_error(const mel_server* s, int job);
 void mel_server_destroy(mel_server* s);
 

@@ -1844,6 +1844,12 @@ mel_status mel_write_dynamic(mel_ctx* c, const char* path) {
     });
 }
 
+mel_status mel_set_graph_replay(mel_ctx* c, int on) {
+    if (!c) return MEL_E_ARG;
+    c->use_graphs = on != 0 && c->two_launch;
+    return MEL_OK;
+}
+
 mel_status mel_get_device_bytes(const mel_ctx* c, size_t* bytes) {
     if (!c || !bytes) return MEL_E_ARG;
     *bytes = c->dev_bytes;

@@ -69,6 +69,7 @@ struct mel_server {
     size_t budget = 0, used = 0;
     long long slice = 0;
     double ratio = DEFAULT_RATIO;
+    int graph_replay = 1;               // of the jobs loaded from now on
     std::vector<Job> jobs;
     std::vector<size_t> order;          // job indices by arrival (stable)
     size_t next_admit = 0;              // first entry of `order` not yet admitted or failed
@@ -172,6 +173,7 @@ bool admit(mel_server& s) {
         j.admitted = t_admit;
         if (st != MEL_OK) { s.fail(j, st, ctx_error(nullptr)); continue; }
         j.ctx = c;
+        mel_set_graph_replay(c, s.graph_replay);
         size_t bytes = 0;
         mel_get_device_bytes(c, &bytes);
         mel_info_t info;
@@ -316,6 +318,12 @@ mel_status mel_server_release(mel_server* s, int job) {
     if (j.state != MEL_JOB_DONE) return MEL_E_ARG;
     s->release(j);
     j.state = MEL_JOB_RELEASED;
+    return MEL_OK;
+}
+
+mel_status mel_server_set_graph_replay(mel_server* s, int on) {
+    if (!s) return MEL_E_ARG;
+    s->graph_replay = on != 0;
     return MEL_OK;
 }
 

@@ -89,6 +89,8 @@ def library():
             "mel_last_error": ([vp], ctypes.c_char_p),
             "mel_version": ([], ctypes.c_char_p),
             "mel_get_device_bytes": ([vp, P(ctypes.c_size_t)], ctypes.c_int),
+            "mel_set_graph_replay": ([vp, ctypes.c_int], ctypes.c_int),
+            "mel_server_set_graph_replay": ([vp, ctypes.c_int], ctypes.c_int),
             "mel_server_create": ([ctypes.c_int, ctypes.c_size_t, ctypes.c_longlong, P(vp)], ctypes.c_int),
             "mel_server_submit": ([vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_double,
                                    ctypes.c_int, P(ctypes.c_int)], ctypes.c_int),
@@ -113,7 +115,8 @@ def exported_symbols():
             "mel_get_node_field", "mel_set_node_field", "mel_get_contacts", "mel_get_neighbours",
             "mel_get_body_state", "mel_get_border_field", "mel_get_info", "mel_write_dynamic", "mel_get_stream",
             "mel_set_kernel_timing", "mel_get_kernel_timing", "mel_kernel_count", "mel_kernel_name",
-            "mel_destroy", "mel_last_error", "mel_version", "mel_get_device_bytes",
+            "mel_destroy", "mel_last_error", "mel_version", "mel_get_device_bytes", "mel_set_graph_replay",
+            "mel_server_set_graph_replay",
             "mel_server_create", "mel_server_submit", "mel_server_run", "mel_server_job",
             "mel_server_job_context", "mel_server_release", "mel_server_usage", "mel_server_job_error",
             "mel_server_destroy"]
@@ -155,6 +158,9 @@ class Context:
             if getattr(self, "_owner", None) is None:
                 self.lib.mel_destroy(self._h)
             self._h = None
+
+    def set_graph_replay(self, on):
+        self._check(self.lib.mel_set_graph_replay(self._h, int(bool(on))))
 
     def device_bytes(self):
         n = ctypes.c_size_t()
@@ -279,7 +285,7 @@ class Server:
     container run for a number of steps, admitted by device footprint under
     an HBM budget and run concurrently on one device."""
 
-    def __init__(self, hbm_budget, steps_per_slice=50, device=0):
+    def __init__(self, hbm_budget, steps_per_slice=50, device=0, graph_replay=True):
         lib = library()
         h = ctypes.c_void_p()
         st = lib.mel_server_create(device, int(hbm_budget), int(steps_per_slice), ctypes.byref(h))
@@ -288,6 +294,7 @@ class Server:
         self._h = h
         self.lib = lib
         self._views = {}
+        self._check(lib.mel_server_set_graph_replay(h, int(bool(graph_replay))))
 
     def _check(self, st, job=-1):
         if st != MEL_OK:

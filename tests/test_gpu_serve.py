@@ -56,12 +56,12 @@ def _same(a, b):
     assert not diff, diff[:8]
 
 
-@pytest.mark.parametrize("precision", [native.FP64, native.FP32_MIXED])
-def test_jobs_in_the_mix_equal_jobs_alone(job_files, precision):
+@pytest.mark.parametrize("precision,graphs", [(native.FP64, True), (native.FP32_MIXED, True), (native.FP64, False)])
+def test_jobs_in_the_mix_equal_jobs_alone(job_files, precision, graphs):
     steps = {name: 60 + 17 * i for i, name in enumerate(JOBS)}
-    print("case serve mix", precision, flush=True)
+    print("case serve mix", precision, graphs, flush=True)
     alone = {n: _alone(job_files[n][0], job_files[n][1], steps[n], precision)[0] for n in JOBS}
-    with native.Server(hbm_budget=8 << 30, steps_per_slice=7) as srv:
+    with native.Server(hbm_budget=8 << 30, steps_per_slice=7, graph_replay=graphs) as srv:
         ids = {}
         for i, n in enumerate(JOBS):
             # two arrive later, while the others run
