@@ -204,9 +204,45 @@ def _triangulate(X, keep=None):
     return tri[good]
 
 
-def closed_border(n_border):
-    nodes = np.concatenate([np.arange(n_border), [0]]).astype(np.int64)
-    return Border(nodes=nodes)
+def closed_border(n_border, first=0, index=0):
+    """Closed border through nodes first .. first+n_border-1, listing its
+    first node again at the end, its own border before and after."""
+    nodes = np.concatenate([np.arange(first, first + n_border), [first]]).astype(np.int64)
+    return Border(nodes=nodes, before=index, after=index)
+
+
+def simple_borders(corner_ids, n_total):
+    """A closed contour through nodes 0 .. n_total-1 cut into 'Simple'
+    borders at the given corner node ids (increasing, the first one 0).
+    Border k runs from corner k to corner k+1 inclusive, so consecutive
+    borders share their end node and the last one ends on node 0; each
+    border's before/after are its neighbours on the contour
+    (SetUnstructuredNodesForSimpleContours.m)."""
+    nc = len(corner_ids)
+    out = []
+    for k in range(nc):
+        a = corner_ids[k]
+        b = corner_ids[k + 1] if k + 1 < nc else n_total
+        nodes = np.arange(a, b + 1, dtype=np.int64)
+        if k + 1 == nc:
+            nodes[-1] = 0
+        out.append(Border(nodes=nodes, periodicity="Simple", before=(k - 1) % nc, after=(k + 1) % nc))
+    return out
+
+
+def annulus_nodes(rng, xc, yc, R, r, n_outer, n_inner, h):
+    """Outer border counter-clockwise, hole border clockwise (so both
+    right-hand normals point out of the material), Poisson-disc interior
+    between them."""
+    th = 2.0 * np.pi * np.arange(n_outer) / n_outer - 0.5 * np.pi
+    outer = np.stack([xc + R * np.cos(th), yc + R * np.sin(th)], axis=1)
+    th = -2.0 * np.pi * np.arange(n_inner) / n_inner - 0.5 * np.pi
+    inner = np.stack([xc + r * np.cos(th), yc + r * np.sin(th)], axis=1)
+    margin = 0.5 * h
+    inside = lambda p: (r + margin) ** 2 < (p[0] - xc) ** 2 + (p[1] - yc) ** 2 < (R - margin) ** 2
+    border = np.concatenate([outer, inner])
+    interior = poisson_disc(rng, inside, ((xc - R, yc - R), (xc + R, yc + R)), h, exclude=border)
+    return outer, inner, interior
 
 
 def _rpim_chunk(job):
@@ -233,9 +269,13 @@ def _rpim(points, X, nn, c, workers):
 
 def deformable_body(border_xy, interior_xy, material, h, rho, radius_factor=2.5,
                     detection=None, contact=None, nregions=1, velocity=(0.0, 0.0),
-                    keep_triangle=None, kind="MLS", rpim_c=1.0, rpim_support=30, workers=None):
-    """Assemble a deformable meshfree body with one closed border.
+                    keep_triangle=None, kind="MLS", rpim_c=1.0, rpim_support=30, workers=None,
+                    borders=None, periodicity="Simple"):
+    """Assemble a deformable meshfree body.
 
+    `borders` lists the body's Border records over the node numbering in
+    which the `border_xy` nodes come first; by default the body has one
+    closed border through all of them.
     kind "MLS": node radius R = radius_factor * sqrt(area / N) (uniform);
     kind "RPIM": supports are the `rpim_support` nearest nodes of each point
     (shape functions on `workers` processes for large bodies).
@@ -274,7 +314,9 @@ def deformable_body(border_xy, interior_xy, material, h, rho, radius_factor=2.5,
     m = np.zeros(len(X))
     np.add.at(m, gidx, rho * np.repeat(c0 * rowsum, np.diff(gptr)) * phi)
     mass = np.stack([m, m], axis=1)
-    b = Body(rigid=False, material=material, X=X, borders=[closed_border(nb)], tri=tri,
+    b = Body(rigid=False, material=material, X=X,
+             borders=borders if borders is not None else [closed_border(nb)], tri=tri,
+             periodicity=periodicity,
              nodal_distance=h, detection=detection if detection is not None else 0.2 * h,
              contact=contact if contact is not None else 0.2 * h, velocity=velocity,
              domain=domain, mass=mass, gp_xy=gp, gp_w=w, gp_jac=jac, gp_ptr=gptr,
@@ -283,11 +325,20 @@ def deformable_body(border_xy, interior_xy, material, h, rho, radius_factor=2.5,
     return b
 
 
+def rectangle_corners(x0, y0, x1, y1, h):
+    """Node ids of the four corners in rectangle_border's numbering."""
+    ids = [0]
+    for a, b in (((x0, y0), (x1, y0)), ((x1, y0), (x1, y1)), ((x1, y1), (x0, y1))):
+        ids.append(ids[-1] + max(int(np.ceil(np.hypot(b[0] - a[0], b[1] - a[1]) / h)), 1))
+    return ids
+
+
 def rigid_rectangle(x0, y0, x1, y1, material, h, rho, detection=None, contact=None,
-                    xbc=None, ybc=None, periodicity="Simple"):
+                    xbc=None, ybc=None, periodicity="Simple", simple_sides=False):
     """Rigid rectangle: border nodes on the perimeter, triangulated for mass
     and inertia (AnalyseRigidBody.m; Code/Body.h:2877-2905 recomputes both
-    from the triangulation at start-up)."""
+    from the triangulation at start-up).  One closed border, or with
+    `simple_sides` four 'Simple' borders, one per side (BCs on the first)."""
     X = rectangle_border(x0, y0, x1, y1, h)
     tri = _triangulate(X)
     P = X[tri]
@@ -295,12 +346,47 @@ def rigid_rectangle(x0, y0, x1, y1, material, h, rho, detection=None, contact=No
     xc, yc = 0.5 * (x0 + x1), 0.5 * (y0 + y1)
     mass = rho * (x1 - x0) * (y1 - y0)
     inertia = mass * ((x1 - x0) ** 2 + (y1 - y0) ** 2) / 12.0
-    brd = closed_border(len(X))
+    brds = simple_borders(rectangle_corners(x0, y0, x1, y1, h), len(X)) if simple_sides else [closed_border(len(X))]
+    brd = brds[0]
     if xbc is not None:
         brd.xbc = xbc
     if ybc is not None:
         brd.ybc = ybc
-    return Body(rigid=True, material=material, X=X, borders=[brd], tri=tri,
+    return Body(rigid=True, material=material, X=X, borders=brds, tri=tri,
                 nodal_distance=h, detection=detection if detection is not None else 0.2 * h,
                 contact=contact if contact is not None else 0.2 * h, periodicity=periodicity,
                 centre=(xc, yc), rigid_mass=mass, rigid_inertia=inertia)
+
+
+def rigid_periodic_wall(xmin, xmax, y0, y1, material, h, rho, detection=None, contact=None,
+                        xbc=None, ybc=None, bc_border=1):
+    """Rigid wall spanning the periodic cell [xmin, xmax) between heights
+    y0 < y1, with the two 'Periodic' borders Body::Initialize_segments
+    expects (Code/Body.h:403-605): border 0 is the lower face, run left to
+    right from the node at (xmin, y0), whose repeated last entry stands for
+    that node one period to the right (shift +1); border 1 is the upper
+    face, run right to left from the node at (xmax, y1), whose repeated last
+    entry stands for that node one period to the left (shift -1).  Both
+    right-hand normals point out of the wall.  The BCs are set on border
+    `bc_border` (a rigid body sums its borders' BC resultants)."""
+    P = xmax - xmin
+    n = max(int(np.ceil(P / h)), 3)
+    xs = xmin + P * np.arange(n) / n
+    lower = np.stack([xs, np.full(n, y0)], axis=1)
+    upper = np.stack([xmax - (xs - xmin), np.full(n, y1)], axis=1)
+    X = np.concatenate([lower, upper])
+    tri = _triangulate(X)
+    brd = [Border(nodes=np.concatenate([np.arange(n), [0]]).astype(np.int64), periodicity="Periodic",
+                  before=0, after=0),
+           Border(nodes=np.concatenate([np.arange(n, 2 * n), [n]]).astype(np.int64), periodicity="Periodic",
+                  before=1, after=1)]
+    if xbc is not None:
+        brd[bc_border].xbc = xbc
+    if ybc is not None:
+        brd[bc_border].ybc = ybc
+    mass = rho * P * (y1 - y0)
+    inertia = mass * (P ** 2 + (y1 - y0) ** 2) / 12.0
+    return Body(rigid=True, material=material, X=X, borders=brd, tri=tri, nodal_distance=h,
+                detection=detection if detection is not None else 0.2 * h,
+                contact=contact if contact is not None else 0.2 * h, periodicity="Periodic",
+                centre=(0.5 * (xmin + xmax), 0.5 * (y0 + y1)), rigid_mass=mass, rigid_inertia=inertia)

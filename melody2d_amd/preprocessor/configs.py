@@ -27,7 +27,8 @@ import os
 
 import numpy as np
 
-from .bodies import BC, deformable_body, disc_nodes, rigid_rectangle
+from .bodies import (BC, annulus_nodes, closed_border, deformable_body, disc_nodes, poisson_disc,
+                     rectangle_border, rectangle_corners, rigid_periodic_wall, rigid_rectangle, simple_borders)
 from .export import Model
 
 E_GRAIN = 1.0e7
@@ -286,6 +287,122 @@ def c3(n_nodes=1_000_000, seed=1003, side=0.1, porosity=0.3, rmin=1.0e-3, rmax=4
     return m
 
 
+def tiny_periodic(seed=1010, h=0.45e-3, R=1.0e-3, period=5.0e-3, shear=20.0, overlap=2e-6, t=0.3e-3):
+    """A miniature periodic shear cell: two discs between two rigid
+    'Periodic' walls spanning the cell [-period/2, period/2).  Disc A
+    straddles the right end of the cell; disc B touches A across the
+    periodic boundary (one period to the left of its contact partner).  The
+    top wall slides at `shear` m/s (frictionless against the grains), so
+    contact points walk along its segments and across its periodic seam;
+    the bottom wall is fixed (Mohr-Coulomb).  Exercises the periodic shift
+    ranges of Update_proximity (Code/Proximity.h:100-101, 145-147), the
+    shifted segment stencils (Code/Body.h:403-605) and the walk across them
+    (Code/Body.h:1402-1599)."""
+    rngs = _streams(seed, 2)
+    xmin, xmax = -0.5 * period, 0.5 * period
+    nb = 2 * max(int(round(math.pi * R / h)), 4)
+    xa = xmax - 0.5 * R
+    xb = xa + 2 * R - overlap - period
+    bodies = []
+    for i, xc in enumerate([xa, xb]):
+        bxy, ixy = disc_nodes(rngs[i], xc, R - overlap, R, nb, h)
+        bodies.append(deformable_body(bxy, ixy, "Grain", h, RHO_GRAIN, detection=0.5 * h, contact=0.5 * h,
+                                      nregions=2))
+    fixed = lambda: BC("Driven", [(0.0, 0.0), (1e3, 0.0)])
+    bodies.append(rigid_periodic_wall(xmin, xmax, -t, 0.0, "SteelB", h, 7800.0, detection=0.5 * h,
+                                      contact=0.5 * h, xbc=fixed(), ybc=fixed()))
+    ytop = 2 * R - 2 * overlap
+    bodies.append(rigid_periodic_wall(xmin, xmax, ytop, ytop + t, "SteelT", h, 7800.0, detection=0.5 * h,
+                                      contact=0.5 * h, xbc=BC("Driven", [(0.0, shear), (1e3, shear)]),
+                                      ybc=fixed(), bc_border=0))
+    mats = [("Grain", "NeoHookean", [RHO_GRAIN, 0.0, 0.0, E_GRAIN, NU_GRAIN, 1.0]),
+            ("SteelB", "ElasticLinear", [7800.0, 0.0, 0.0, 2e11, 0.3, 1.0]),
+            ("SteelT", "ElasticLinear", [7800.0, 0.0, 0.0, 2e11, 0.3, 1.0])]
+    laws = [("Grain", "Grain", "MohrCoulomb", "Evolutive", [1e13, 1e13, 0.3, 0.0, 0.0]),
+            ("Grain", "SteelB", "MohrCoulomb", "Evolutive", [1e13, 1e13, 0.3, 0.0, 0.0]),
+            ("Grain", "SteelT", "Frictionless", "Evolutive", [1e13])]
+    m = Model(name="tiny_periodic", materials=mats, contact_laws=laws, bodies=bodies, periodic=(xmin, xmax))
+    m.dt = _critical_dt(m, 1e13)
+    m.contact_update_period = 10 * m.dt
+    m.tend = 1.0
+    return m
+
+
+def tiny_simple(seed=1011, h=0.4e-3, W=3.0e-3, H=1.0e-3, R=0.8e-3, overlap=2e-6, t=0.9e-3):
+    """A deformable block whose contour is cut into four 'Simple' borders
+    (one per side, sharing their corner nodes), on a rigid platen that also
+    has four 'Simple' borders, with a disc pressed on its top face.  BCs on
+    the block: soft Dirichlet on the left side, oriented Neumann on the
+    right, following Neumann on the top.  Exercises the Simple branches of
+    the segment stencils and tributary lengths (Code/Body.h:403-661) and the
+    BC spreading over borders that share end nodes (Code/Body.h:669-901)."""
+    rngs = _streams(seed, 2)
+    x0, y0, x1, y1 = -0.5 * W, 0.0, 0.5 * W, H
+    bxy = rectangle_border(x0, y0, x1, y1, h)
+    corners = rectangle_corners(x0, y0, x1, y1, h)
+    margin = 0.5 * h
+    inside = lambda p: (x0 + margin < p[0] < x1 - margin) and (y0 + margin < p[1] < y1 - margin)
+    ixy = poisson_disc(rngs[0], inside, ((x0, y0), (x1, y1)), h, exclude=bxy)
+    brds = simple_borders(corners, len(bxy))
+    brds[3].xbc = BC("Soft", [(0.0, 0.0), (1e3, 0.0)], (2e5, 1.0))
+    brds[1].xbc = BC("Oriented", [(0.0, -1e4), (1e3, -2e4)])
+    brds[2].ybc = BC("Following", [(0.0, -5e3), (1e3, -5e3)])
+    block = deformable_body(bxy, ixy, "Grain", h, RHO_GRAIN, detection=0.5 * h, contact=0.5 * h,
+                            nregions=2, borders=brds)
+    nbd = 2 * max(int(round(math.pi * R / h)), 4)
+    dxy, dixy = disc_nodes(rngs[1], 0.3e-3, y1 + R - overlap, R, nbd, h)
+    disc = deformable_body(dxy, dixy, "Grain", h, RHO_GRAIN, detection=0.5 * h, contact=0.5 * h,
+                           velocity=(0.0, -0.05))
+    platen = rigid_rectangle(x0 - 0.5e-3, -t + overlap, x1 + 0.5e-3, overlap, "Steel", h, 7800.0,
+                             detection=0.5 * h, contact=0.5 * h, simple_sides=True,
+                             xbc=BC("Driven", [(0.0, 0.0), (1e3, 0.0)]), ybc=BC("Driven", [(0.0, 0.0), (1e3, 0.0)]))
+    mats = [("Grain", "NeoHookean", [RHO_GRAIN, 0.0, 0.0, E_GRAIN, NU_GRAIN, 1.0]),
+            ("Steel", "ElasticLinear", [7800.0, 0.0, 0.0, 2e11, 0.3, 1.0])]
+    laws = [("Grain", "Grain", "MohrCoulomb", "Evolutive", [1e13, 1e13, 0.3, 0.0, 0.0]),
+            ("Grain", "Steel", "MohrCoulomb", "Evolutive", [1e13, 1e13, 0.3, 0.0, 0.0])]
+    m = Model(name="tiny_simple", materials=mats, contact_laws=laws, bodies=[block, disc, platen],
+              periodic=(-1.0, 1.0))
+    m.dt = _critical_dt(m, 1e13)
+    m.contact_update_period = 10 * m.dt
+    m.tend = 1.0
+    return m
+
+
+def tiny_holed(seed=1012, h=0.3e-3, R=1.5e-3, r=0.7e-3, rd=0.5e-3, hd=0.2e-3, overlap=2e-6):
+    """A deformable ring (outer border counter-clockwise, hole border
+    clockwise: two closed borders) with a small disc inside the hole pressed
+    against the hole's lower face, and a disc resting on the ring's top.
+    Exercises bodies with several borders: per-border stencils and
+    before/after indices, neighbour triples naming border 1, contact
+    elements with borderS / borderM = 1 (Code/Proximity.h:112-330)."""
+    rngs = _streams(seed, 3)
+    no = 2 * max(int(round(math.pi * R / h)), 4)
+    ni = 2 * max(int(round(math.pi * r / h)), 4)
+    outer, inner, ixy = annulus_nodes(rngs[0], 0.0, 0.0, R, r, no, ni, h)
+    brds = [closed_border(no, 0, 0), closed_border(ni, no, 1)]
+    ring = deformable_body(np.concatenate([outer, inner]), ixy, "Grain", h, RHO_GRAIN, detection=0.5 * h,
+                           contact=0.5 * h, nregions=2, borders=brds,
+                           keep_triangle=lambda c: c[:, 0] ** 2 + c[:, 1] ** 2 > r * r)
+    nbd = 2 * max(int(round(math.pi * rd / hd)), 4)
+    dxy, dixy = disc_nodes(rngs[1], 0.0, -(r - rd) + overlap, rd, nbd, hd)
+    inner_disc = deformable_body(dxy, dixy, "Grain", hd, RHO_GRAIN, detection=0.5 * hd, contact=0.5 * hd,
+                                 velocity=(0.02, -0.05))
+    Rt = 1.0e-3
+    nbt = 2 * max(int(round(math.pi * Rt / h)), 4)
+    txy, tixy = disc_nodes(rngs[2], 0.2e-3, R + Rt - overlap, Rt, nbt, h)
+    top = deformable_body(txy, tixy, "Grain", h, RHO_GRAIN, detection=0.5 * h, contact=0.5 * h,
+                          velocity=(0.0, -0.05))
+    mats = [("Grain", "NeoHookean", [RHO_GRAIN, 0.0, 0.0, E_GRAIN, NU_GRAIN, 1.0])]
+    laws = [("Grain", "Grain", "MohrCoulomb", "Evolutive", [1e13, 1e13, 0.3, 0.0, 0.0])]
+    m = Model(name="tiny_holed", materials=mats, contact_laws=laws, bodies=[ring, inner_disc, top],
+              periodic=(-1.0, 1.0))
+    m.gravity = (0.0, -9.81)
+    m.dt = _critical_dt(m, 1e13)
+    m.contact_update_period = 10 * m.dt
+    m.tend = 1.0
+    return m
+
+
 def _set_law(m, typ, params, evolution="Evolutive"):
     m.contact_laws = [(a, b, typ, evolution, list(params)) for (a, b, _, _, _) in m.contact_laws]
     return m
@@ -330,6 +447,10 @@ def variants():
     m = tiny(h=0.45e-3)
     m.flags = m.flags + ["NO_MONITORING"]
     v["tiny_nomonitor"] = m
+    # periodic shear cell, Simple borders, a body with two borders
+    v["tiny_periodic"] = tiny_periodic()
+    v["tiny_simple"] = tiny_simple()
+    v["tiny_holed"] = tiny_holed()
     return v
 
 

@@ -65,6 +65,7 @@ struct State {
     double works[7] = {0, 0, 0, 0, 0, 0, 0};
     // per border
     std::vector<std::vector<double>> length, xcp, ycp;
+    std::vector<std::vector<double>> xbp, ybp;    // per-slot BC pressure (Border::x_bc_pressure)
     std::vector<double> xbv, ybv, xbvel, ybvel;    // current bc value / velocity
     std::vector<double> pxmin, pxmax, pymin, pymax;
     std::vector<int> shmin, shmax;
@@ -291,6 +292,8 @@ void update_bc_forces(Sim& S, int bi) {
         for (size_t i = 0; i < b.borders.size(); i++) {
             const Border& B = b.borders[i];
             int xt = B.xbc.type, yt = B.ybc.type;
+            std::fill(s.xbp[i].begin(), s.xbp[i].end(), 0.);
+            std::fill(s.ybp[i].begin(), s.ybp[i].end(), 0.);
             if (xt == BC_NONE && yt == BC_NONE) continue;
             int nn = (int)B.nodes.size();
             for (int n = 0; n < nn - 1; n++) {
@@ -324,6 +327,8 @@ void update_bc_forces(Sim& S, int bi) {
                 } else if (yt == BC_SOFT) {
                     py = B.ybc.stiffness * (s.ybv[i] - s.uy[node]) - B.ybc.damping * (s.ybvel[i] - s.vy[node]);
                 }
+                s.xbp[i][n] = px;
+                s.ybp[i][n] = py;
                 for (int k = b.dptr[node]; k < b.dptr[node + 1]; k++) {
                     double sf = b.dphi[k];
                     double fxn = 0., fyn = 0., fxd = 0., fyd = 0.;
@@ -343,18 +348,27 @@ void update_bc_forces(Sim& S, int bi) {
         for (size_t i = 0; i < b.borders.size(); i++) {
             const Border& B = b.borders[i];
             int xt = B.xbc.type, yt = B.ybc.type;
+            std::fill(s.xbp[i].begin(), s.xbp[i].end(), 0.);
+            std::fill(s.ybp[i].begin(), s.ybp[i].end(), 0.);
             if (xt == BC_NONE && yt == BC_NONE) continue;
             int nn = (int)B.nodes.size();
             for (int n = 0; n < nn; n++) {
                 int node = B.nodes[n];
                 double L = s.length[i][n];
                 double fxn = 0., fyn = 0., fxd = 0., fyd = 0.;
-                if (xt == BC_ORIENTED || xt == BC_FOLLOWING) fxn = s.xbv[i] * L;
-                else if (xt == BC_SOFT)
-                    fxd = (B.xbc.stiffness * (s.xbv[i] - s.ux[node]) + B.xbc.damping * (s.xbvel[i] - s.vx[node])) * L;
-                if (yt == BC_ORIENTED || yt == BC_FOLLOWING) fyn = s.ybv[i] * L;
-                else if (yt == BC_SOFT)
-                    fyd = (B.ybc.stiffness * (s.ybv[i] - s.uy[node]) + B.ybc.damping * (s.ybvel[i] - s.vy[node])) * L;
+                double px = 0., py = 0.;
+                if (xt == BC_ORIENTED || xt == BC_FOLLOWING) { px = s.xbv[i]; fxn = px * L; }
+                else if (xt == BC_SOFT) {
+                    px = B.xbc.stiffness * (s.xbv[i] - s.ux[node]) + B.xbc.damping * (s.xbvel[i] - s.vx[node]);
+                    fxd = px * L;
+                }
+                if (yt == BC_ORIENTED || yt == BC_FOLLOWING) { py = s.ybv[i]; fyn = py * L; }
+                else if (yt == BC_SOFT) {
+                    py = B.ybc.stiffness * (s.ybv[i] - s.uy[node]) + B.ybc.damping * (s.ybvel[i] - s.vy[node]);
+                    fyd = py * L;
+                }
+                s.xbp[i][n] = px;
+                s.ybp[i][n] = py;
                 double dx = s.xc[node] - s.cur[0], dy = s.yc[node] - s.cur[1];
                 s.rfn[0] += fxn; s.rfn[1] += fyn; s.rfn[2] += -fxn * dy + fyn * dx;
                 s.rfd[0] += fxd; s.rfd[1] += fyd; s.rfd[2] += -fxd * dy + fyd * dx;
@@ -1215,11 +1229,13 @@ void setup(Sim& S) {
         s.regy.assign((size_t)std::max(b.n_regions, 1) * n, 0.);
         for (int q = 0; q < 7; q++) s.works[q] = b.works[q];
         size_t nbr = b.borders.size();
-        s.length.resize(nbr); s.xcp.resize(nbr); s.ycp.resize(nbr);
+        s.length.resize(nbr); s.xcp.resize(nbr); s.ycp.resize(nbr); s.xbp.resize(nbr); s.ybp.resize(nbr);
         for (size_t k = 0; k < nbr; k++) {
             s.length[k] = b.borders[k].initial_length;
             s.xcp[k].assign(b.borders[k].nodes.size(), 0.);
             s.ycp[k].assign(b.borders[k].nodes.size(), 0.);
+            s.xbp[k].assign(b.borders[k].nodes.size(), 0.);
+            s.ybp[k].assign(b.borders[k].nodes.size(), 0.);
         }
         s.xbv.assign(nbr, 0.); s.ybv.assign(nbr, 0.); s.xbvel.assign(nbr, 0.); s.ybvel.assign(nbr, 0.);
         s.pxmin.assign(nbr, 0.); s.pxmax.assign(nbr, 0.); s.pymin.assign(nbr, 0.); s.pymax.assign(nbr, 0.);
@@ -1299,6 +1315,39 @@ void euler_step(Sim& S) {
     update_contact_pressures(S);
 }
 
+// Per-body energy scalars {kinetic energy from physical velocities and
+// nodal masses, kinetic energy from velocity parameters (rigid bodies: from
+// the body's velocity, mass and inertia), boundary work of the last step}:
+// the expressions of the MONITOR_ENERGY block, Code/Monitoring.h:508-537,
+// and of monitor type 9, Code/Monitoring.h:395-412, in their summation order.
+std::vector<double> body_energy(const Sim& S, int bi) {
+    const Body& b = S.m.bodies[bi];
+    const State& s = S.s[bi];
+    double ke = 0., kp = 0., dw = 0.;
+    for (size_t j = 0; j < b.borders.size(); j++) {
+        const Border& B = b.borders[j];
+        for (size_t k = 0; k < B.nodes.size(); k++) {
+            int node = B.nodes[k];
+            dw += (s.xbp[j][k] * s.vx[node] + s.ybp[j][k] * s.vy[node]) * s.length[j][k] * S.m.dt;
+        }
+    }
+    for (int j = 0; j < b.n_nodes; j++) {
+        ke += 0.5 * s.vx[j] * s.vx[j] * s.mx[j];
+        ke += 0.5 * s.vy[j] * s.vy[j] * s.my[j];
+    }
+    if (!b.rigid) {
+        for (int j = 0; j < b.n_nodes; j++) {
+            kp += 0.5 * s.vpx[j] * s.vpx[j] * s.mx[j];
+            kp += 0.5 * s.vpy[j] * s.vpy[j] * s.my[j];
+        }
+    } else {
+        kp += 0.5 * s.vel[0] * s.vel[0] * s.mass;
+        kp += 0.5 * s.vel[1] * s.vel[1] * s.mass;
+        kp += 0.5 * s.vel[2] * s.vel[2] * s.inertia;
+    }
+    return {ke, kp, dw};
+}
+
 void dump(Sim& S, const std::string& path) {
     DumpWriter w(path);
     w.scalar("time", S.time);
@@ -1332,6 +1381,7 @@ void dump(Sim& S, const std::string& path) {
                             s.rfn[0], s.rfn[1], s.rfn[2]});
         w.f64(p + "works", std::vector<double>(s.works, s.works + 7));
         w.scalar(p + "mass", s.mass);
+        w.f64(p + "energy", body_energy(S, (int)bi));
         w.i32(p + "neighbours", s.neighbours);
         std::vector<int> ci;
         std::vector<double> cd;

@@ -38,8 +38,63 @@ using namespace std;
 
 #include "dump_format.hpp"
 
-static void dump_state(const string& path, vector<Body>& Bodies, double Time, int iteration) {
-    DumpWriter w(path);
+// Per-body energy scalars with the expressions of the reference's monitors:
+// kinetic energy from physical velocities and nodal masses and the boundary
+// work increment of the MONITOR_ENERGY block (Code/Monitoring.h:508-537), and
+// kinetic energy from velocity parameters (monitor type 9,
+// Code/Monitoring.h:395-412; rigid bodies from their own velocity, mass and
+// inertia).
+static vector<double> body_energy(Body& B, double Deltat) {
+    double ke = 0., kp = 0., dw = 0.;
+    for (int j = 0; j < B.nb_borders; j++)
+        for (int k = 0; k < B.borders[j].number_border_nodes; k++) {
+            double px = B.borders[j].x_bc_pressure[k];
+            double py = B.borders[j].y_bc_pressure[k];
+            double length = B.borders[j].length[k];
+            double vx = B.nodes[B.borders[j].border_nodes[k]].x_velocity;
+            double vy = B.nodes[B.borders[j].border_nodes[k]].y_velocity;
+            dw += (px * vx + py * vy) * length * Deltat;
+        }
+    for (int j = 0; j < B.nb_nodes; j++) {
+        ke += 0.5 * B.nodes[j].x_velocity * B.nodes[j].x_velocity * B.nodes[j].x_mass;
+        ke += 0.5 * B.nodes[j].y_velocity * B.nodes[j].y_velocity * B.nodes[j].y_mass;
+    }
+    if (B.type == "deformable") {
+        for (int j = 0; j < B.nb_nodes; j++) {
+            kp += 0.5 * B.nodes[j].x_velocity_parameter * B.nodes[j].x_velocity_parameter * B.nodes[j].x_mass;
+            kp += 0.5 * B.nodes[j].y_velocity_parameter * B.nodes[j].y_velocity_parameter * B.nodes[j].y_mass;
+        }
+    } else if (B.type == "rigid") {
+        kp += 0.5 * B.x_velocity * B.x_velocity * B.mass;
+        kp += 0.5 * B.y_velocity * B.y_velocity * B.mass;
+        kp += 0.5 * B.r_velocity * B.r_velocity * B.inertia;
+    }
+    return {ke, kp, dw};
+}
+
+// Records to write: all, or (MEL_DUMP_FIELDS set) only the scalar records
+// and the per-body records whose names are listed, comma-separated.
+static bool keep_record(const string& name) {
+    static const char* env = getenv("MEL_DUMP_FIELDS");
+    if (!env || !*env) return true;
+    size_t dot = name.find('.');
+    string key = dot == string::npos ? name : name.substr(dot + 1);
+    if (key.compare(0, 6, "border") == 0 && key.find('.') != string::npos) key = key.substr(key.find('.') + 1);
+    string list = string(",") + env + ",";
+    return dot == string::npos || list.find("," + key + ",") != string::npos;
+}
+
+struct FilteredWriter {
+    DumpWriter w;
+    explicit FilteredWriter(const string& path) : w(path) {}
+    bool ok() const { return w.ok(); }
+    void f64(const string& n, const vector<double>& v) { if (keep_record(n)) w.f64(n, v); }
+    void i32(const string& n, const vector<int>& v) { if (keep_record(n)) w.i32(n, v); }
+    void scalar(const string& n, double x) { if (keep_record(n)) w.scalar(n, x); }
+};
+
+static void dump_state(const string& path, vector<Body>& Bodies, double Time, int iteration, double Deltat) {
+    FilteredWriter w(path);
     if (!w.ok()) { cerr << "cannot write " << path << endl; exit(2); }
     w.scalar("time", Time);
     w.scalar("iteration", (double)iteration);
@@ -90,6 +145,7 @@ static void dump_state(const string& path, vector<Body>& Bodies, double Time, in
         w.f64(p + "works", {B.internal_work, B.contact_work, B.body_work, B.dirichlet_work,
                            B.neumann_work, B.damping_work, B.alid_work});
         w.scalar(p + "mass", B.mass);
+        w.f64(p + "energy", body_energy(B, Deltat));
         vector<int> nb;
         for (int k = 0; k < B.nb_neighbours; k++)
             for (int c = 0; c < 3; c++) nb.push_back(B.neighbours[k][c]);
@@ -200,7 +256,7 @@ int main(int argc, char** argv) {
     char name[64];
     if (want(0)) {
         snprintf(name, sizeof name, "/step_%06d.bin", 0);
-        dump_state(out_dir + name, Bodies, Time, Number_iteration);
+        dump_state(out_dir + name, Bodies, Time, Number_iteration, Deltat);
     }
     double t0 = omp_get_wtime();
     for (int s = 1; s <= nsteps; s++) {
@@ -216,7 +272,7 @@ int main(int argc, char** argv) {
         }
         if (want(s)) {
             snprintf(name, sizeof name, "/step_%06d.bin", s);
-            dump_state(out_dir + name, Bodies, Time, Number_iteration);
+            dump_state(out_dir + name, Bodies, Time, Number_iteration, Deltat);
         }
     }
     double t1 = omp_get_wtime();

@@ -10,7 +10,9 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, HERE)
 
 GOLDEN = os.path.join(HERE, "golden")
-VARIANTS = sorted(f[:-7] for f in os.listdir(GOLDEN) if f.endswith(".tar.gz"))
+# golden sets that carry their model files; c1.tar.gz carries the
+# reference's dumps and the hash of a model the generator rebuilds
+VARIANTS = sorted(f[:-7] for f in os.listdir(GOLDEN) if f.endswith(".tar.gz") and f != "c1.tar.gz")
 
 
 def pytest_configure(config):
@@ -18,13 +20,30 @@ def pytest_configure(config):
 
 
 def extract_golden(name, dest):
-    """Model files under dest/model, reference dumps under dest/ref."""
+    """Model files under dest/model, reference dumps under dest/ref.  For
+    c1 the model is rebuilt by the seeded generator and must hash to the
+    value stored beside the dumps."""
     with tarfile.open(os.path.join(GOLDEN, name + ".tar.gz"), "r:gz") as tf:
         for m in tf.getmembers():
             if not m.isfile() or m.name.startswith("/") or ".." in m.name:
                 raise ValueError("unexpected member %s" % m.name)
         tf.extractall(dest)
-    return os.path.join(dest, "model"), os.path.join(dest, "ref")
+    model = os.path.join(dest, "model")
+    if name == "c1":
+        from golden.make_golden import model_hash
+        from melody2d_amd.preprocessor import write_model
+        from melody2d_amd.preprocessor.configs import c1
+        write_model(c1(), model)
+        with open(os.path.join(dest, "model.sha256")) as f:
+            want = f.read().strip()
+        if model_hash(model) != want:
+            pytest.skip("the generator did not rebuild the c1 model bit for bit on this host")
+    return model, os.path.join(dest, "ref")
+
+
+def golden_steps(ref_dir):
+    """Steps of the reference dumps in a golden set."""
+    return sorted(int(f[5:11]) for f in os.listdir(ref_dir) if f.startswith("step_"))
 
 
 @pytest.fixture(scope="session")

@@ -3,14 +3,21 @@
 For every small model of melody2d_amd.preprocessor.configs.variants() this
 script writes the three model files, runs the reference MELODY 2D solver
 compiled from its own sources (oracle/_ref/melody_ref, built by
-`make -C oracle ref` from /root/reference/Code) for 200 steps, and stores the
-model files together with the reference's state dumps at steps 0, 1, 2, 10,
-50 and 200 in <variant>.tar.gz.  Only data is stored: model files and FP64
-state dumps (format of oracle/dump_format.hpp).
+`make -C oracle ref` from /root/reference/Code), and stores the model files
+together with the reference's state dumps (format of oracle/dump_format.hpp)
+in <variant>.tar.gz: steps 0, 1, 2, 10, 50 and 200, and also 1000 for the
+periodic shear cell, whose top wall needs that long to carry contact points
+across its periodic seam.
 
-Run from the repository root:  python tests/golden/make_golden.py
+The two-disc platen workload (configs.c1, ~800 nodes) is too large to store
+as model files: c1.tar.gz holds only the reference's dumps at steps 0, 1,
+10, 100 and 1000 (a subset of the records) and the SHA-256 of the model
+files the generator writes, which the tests regenerate and check before
+use.  Only data is stored.
+
+Run from the repository root:  python tests/golden/make_golden.py [name,...]
 """
-import io
+import hashlib
 import os
 import subprocess
 import sys
@@ -22,31 +29,62 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from melody2d_amd.preprocessor import write_model  # noqa: E402
-from melody2d_amd.preprocessor.configs import variants  # noqa: E402
+from melody2d_amd.preprocessor.configs import c1, variants  # noqa: E402
 
 STEPS = [0, 1, 2, 10, 50, 200]
+LONG = {"tiny_periodic": STEPS + [1000]}
+C1_STEPS = [0, 1, 10, 100, 1000]
+C1_FIELDS = ("x_current,y_current,x_disp_param,y_disp_param,x_vel_param,y_vel_param,x_internal_force,"
+             "y_internal_force,x_contact_force,y_contact_force,x_force,y_force,rigid,works,energy,neighbours,"
+             "contacts_i,contacts_d,length")
 REF = os.path.join(ROOT, "oracle", "_ref", "melody_ref")
 
 
-def main():
+def model_hash(mdir):
+    h = hashlib.sha256()
+    for f in sorted(os.listdir(mdir)):
+        h.update(f.encode())
+        with open(os.path.join(mdir, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def run_ref(mdir, odir, steps, env=None):
+    subprocess.run([REF, mdir, str(max(steps)), odir, ",".join(map(str, steps))], check=True,
+                   stdout=subprocess.DEVNULL, env=env)
+
+
+def main(names=None):
     if not os.path.exists(REF):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
-    for name, model in variants().items():
+    todo = dict(variants())
+    todo["c1"] = None
+    for name in (names or list(todo)):
         with tempfile.TemporaryDirectory() as tmp:
             mdir = os.path.join(tmp, "model")
             odir = os.path.join(tmp, "ref")
             os.makedirs(odir)
-            write_model(model, mdir)
-            subprocess.run([REF, mdir, str(max(STEPS)), odir, ",".join(map(str, STEPS))], check=True,
-                           stdout=subprocess.DEVNULL)
             out = os.path.join(HERE, name + ".tar.gz")
-            with tarfile.open(out, "w:gz", compresslevel=9) as tf:
-                for f in sorted(os.listdir(mdir)):
-                    tf.add(os.path.join(mdir, f), arcname="model/" + f)
-                for f in sorted(os.listdir(odir)):
-                    tf.add(os.path.join(odir, f), arcname="ref/" + f)
+            if name == "c1":
+                write_model(c1(), mdir)
+                env = dict(os.environ, MEL_DUMP_FIELDS=C1_FIELDS)
+                run_ref(mdir, odir, C1_STEPS, env)
+                with open(os.path.join(tmp, "model.sha256"), "w") as f:
+                    f.write(model_hash(mdir) + "\n")
+                with tarfile.open(out, "w:gz", compresslevel=9) as tf:
+                    tf.add(os.path.join(tmp, "model.sha256"), arcname="model.sha256")
+                    for f in sorted(os.listdir(odir)):
+                        tf.add(os.path.join(odir, f), arcname="ref/" + f)
+            else:
+                write_model(todo[name], mdir)
+                run_ref(mdir, odir, LONG.get(name, STEPS))
+                with tarfile.open(out, "w:gz", compresslevel=9) as tf:
+                    for f in sorted(os.listdir(mdir)):
+                        tf.add(os.path.join(mdir, f), arcname="model/" + f)
+                    for f in sorted(os.listdir(odir)):
+                        tf.add(os.path.join(odir, f), arcname="ref/" + f)
             print(name, os.path.getsize(out))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1].split(",") if len(sys.argv) > 1 else None)

@@ -8,18 +8,19 @@ import os
 import numpy as np
 import pytest
 
-from conftest import VARIANTS
+from conftest import VARIANTS, golden_steps
 from mel_compare import run_oracle
 from mel_dump import read_dump
 
 STEPS = [0, 1, 2, 10, 50, 200]
 
 
-@pytest.mark.parametrize("name", VARIANTS)
+@pytest.mark.parametrize("name", VARIANTS + ["c1"])
 def test_oracle_matches_reference_bitwise(name, golden_dir, tmp_path):
     model, ref = golden_dir(name)
-    got, _ = run_oracle(model, STEPS, str(tmp_path / "oracle"))
-    for s in STEPS:
+    steps = golden_steps(ref)
+    got, _ = run_oracle(model, steps, str(tmp_path / "oracle"))
+    for s in steps:
         R = read_dump(os.path.join(ref, "step_%06d.bin" % s))
         G = got[s]
         assert set(R) <= set(G), set(R) - set(G)
@@ -39,11 +40,12 @@ def test_goldens_exercise_the_step():
     import tempfile
     from conftest import extract_golden
     seen = {"contact": False, "neumann": False, "dirichlet": False, "damping": False, "czm": False,
-            "rigid_rotation": False}
+            "rigid_rotation": False, "periodic_neighbour": False, "periodic_stencil": False,
+            "slave_border_1": False, "master_border_3": False, "boundary_work": False}
     for name in VARIANTS:
         with tempfile.TemporaryDirectory() as d:
             _, ref = extract_golden(name, d)
-            for s in STEPS:
+            for s in golden_steps(ref):
                 D = read_dump(os.path.join(ref, "step_%06d.bin" % s))
                 nb = int(D["n_bodies"][0])
                 for b in range(nb):
@@ -53,6 +55,15 @@ def test_goldens_exercise_the_step():
                     seen["dirichlet"] |= bool(np.any(D[p + "x_dirichlet_force"] != 0))
                     seen["damping"] |= bool(np.any(D[p + "x_damping_force"] != 0))
                     cd = D[p + "contacts_d"].reshape(-1, 18)
+                    ci = D[p + "contacts_i"].reshape(-1, 16)
                     seen["czm"] |= bool(np.any((cd[:, 17] > 0) & (cd[:, 17] < 1)))
                     seen["rigid_rotation"] |= bool(D[p + "rigid"][8] != 0)
+                    # periodic cells: neighbours one period away, and
+                    # contact stencils that cross the master's periodic seam
+                    seen["periodic_neighbour"] |= bool(np.any(D[p + "neighbours"].reshape(-1, 3)[:, 2] != 0))
+                    seen["periodic_stencil"] |= bool(np.any(ci[:, 11:15] != 0))
+                    # bodies with several borders
+                    seen["slave_border_1"] |= bool(np.any(ci[:, 0] == 1))
+                    seen["master_border_3"] |= bool(np.any(ci[:, 4] == 3))
+                    seen["boundary_work"] |= bool(D[p + "energy"][2] != 0)
     assert all(seen.values()), seen
