@@ -102,6 +102,39 @@ mel_status mel_get_neighbours(mel_ctx* ctx, int body, int32_t* out, size_t cap, 
  * (Code/Body.h:112-118, accumulated as in Code/Body.h:2058-2081) */
 mel_status mel_get_body_state(mel_ctx* ctx, int body, double* rigid26, double* works7);
 
+/* energy scalars of one body, energy3 = {kinetic energy from the physical
+ * velocities and nodal masses, and boundary work of the last step, sum over
+ * border slots of (px vx + py vy) * length * dt: the MONITOR_ENERGY block of
+ * Code/Monitoring.h:508-537; kinetic energy from the velocity parameters
+ * (rigid bodies: from the body's velocity, mass and inertia), monitor type 9
+ * of Code/Monitoring.h:395-412}, in the order {kinetic, kinetic from
+ * parameters, boundary work} */
+mel_status mel_get_body_energy(mel_ctx* ctx, int body, double* energy3);
+
+/* whole-model scalars for the energy balance (Code/Monitoring.h:508-537):
+ * the time, the 7 works summed over bodies (as mel_get_body_state) and the 3
+ * energy scalars summed over bodies (as mel_get_body_energy); any pointer
+ * may be NULL.  Elastic energy is not returned: the reference never
+ * computes it (Code/Model.h:28, Code/Model.h:72). */
+mel_status mel_get_scalars(mel_ctx* ctx, double* time, double* works7, double* energy3);
+
+/* what mel_eval_forces evaluates */
+typedef enum {
+    MEL_EVAL_INTERNAL = 1,  /* Body::Update_internal_forces (Code/Body.h:937-1049) */
+    MEL_EVAL_CONTACT = 2,   /* Update_contacts + Update_contact_forces + sends (Code/Body.h:1341-1967) */
+    MEL_EVAL_EXTERNAL = 4,  /* Update_bc_forces, Update_body_forces, Update_damping_forces (Code/Body.h:669-1154) */
+    MEL_EVAL_ALL = 7
+} mel_eval_what;
+
+/* One force evaluation at the current state without integration: the force
+ * stage of Euler_step (Code/Solver.h:32-67) and Sum_up_forces
+ * (Code/Node.h:212-225), for the parts named in `what` (the others read 0).
+ * The node force fields (MEL_X_INTERNAL_FORCE ... MEL_Y_FORCE) of deformable
+ * bodies then hold the result; contact elements are evaluated on a copy of
+ * their state, so the step's state is left as it was.  Rigid bodies'
+ * resultants are not re-formed. */
+mel_status mel_eval_forces(mel_ctx* ctx, unsigned what);
+
 /* per border-node slot: 0 tributary length (Code/Body.h:614-661),
  * 1 / 2 x / y contact pressure (Update_contact_pressures, Code/Proximity.h:1014-1069) */
 mel_status mel_get_border_field(mel_ctx* ctx, int body, int border, int field, double* out, size_t n);

@@ -65,8 +65,9 @@ __global__ void k_kinematics_list(DevModel m, const int* nodes, int n) {
 }
 
 // force fields of deformable nodes rebuilt from the last step's components
-// (same expression as the update), for output
-__global__ void k_node_outputs(DevModel m) {
+// (same expression as the update), for output; body forces m g
+// (Body::Update_body_forces) unless with_body_forces is 0
+__global__ void k_node_outputs(DevModel m, int with_body_forces) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m.n_nodes) return;
     int b = m.node_body[j];
@@ -80,7 +81,7 @@ __global__ void k_node_outputs(DevModel m) {
         nx += m.bn_pnx[bn] * f; ny += m.bn_pny[bn] * f;
     }
     double gx = m.gravity_x, gy = m.gravity_y;
-    if (m.body_status[b] == 2) { gx = 0.; gy = 0.; }
+    if (m.body_status[b] == 2 || !with_body_forces) { gx = 0.; gy = 0.; }
     double bx = m.mx[j] * gx, by = m.my[j] * gy;
     m.fcx[j] = cx; m.fcy[j] = cy; m.fdx[j] = dx; m.fdy[j] = dy; m.fnx[j] = nx; m.fny[j] = ny;
     m.fbx[j] = bx; m.fby[j] = by;

@@ -92,6 +92,6 @@ __global__ void k_block_force_update_wide(DevModel m, BlockArrays B);
 __global__ void k_block_force_update32_wide(DevModel m, BlockArrays B);
 __global__ void k_update_finish(DevModel m, BlockArrays B, double t_next);
 __global__ void k_kinematics_list(DevModel m, const int* nodes, int n);
-__global__ void k_node_outputs(DevModel m);
+__global__ void k_node_outputs(DevModel m, int with_body_forces);
 
 }  // namespace mel

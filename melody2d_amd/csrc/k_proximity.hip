@@ -453,7 +453,7 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_
                 C.xn[e] = p.xn; C.yn[e] = p.yn; C.xt[e] = p.xt; C.yt[e] = p.yt;
                 C.emass[e] = emass; C.len[e] = m.sl_len[slotS]; C.fx[e] = 0.; C.fy[e] = 0.;
                 C.nbint[e] = 0; C.int0[e] = 0.;
-                C.vgapn[e] = 0.; C.vgapt[e] = 0.;
+                C.vgapn[e] = 0.; C.vgapt[e] = 0.; C.wacc[e] = 0.;
                 C.fresh[e] = ri.self ? 2 : 1;
             }
         }

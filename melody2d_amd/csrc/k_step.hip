@@ -109,4 +109,122 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_update_finish(DevModel m, Blo
     if (i < m.n_borders) bc_eval_at(m, i, t_next);
 }
 
+// ---------------------------------------------------------------------------
+// Contact work (Code/Body.h:1847-1848): every element holds the half of its
+// work accumulated since the last flush; a flush adds, per body, the halves
+// of the elements naming it as slave and as master (a self-contact element
+// twice), in a fixed order, then clears them.  One workgroup per body.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLOCK) k_flush_contact_work(DevModel m) {
+    __shared__ double red[BLOCK];
+    const int b = blockIdx.x;
+    const int nc = *m.c_count;
+    double v = 0.;
+    for (int e = threadIdx.x; e < nc; e += BLOCK) {
+        const double w = m.C.wacc[e];
+        v += m.C.bodyS[e] == b ? w : 0.;
+        v += m.C.bodyM[e] == b ? w : 0.;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = BLOCK / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0] != 0.) m.body_rigid[b * RIGID_FIELDS + R_WORK + W_CONTACT] += red[0];
+}
+
+__global__ void k_clear_contact_work(DevModel m) {
+    const int nc = *m.c_count;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nc; e += gridDim.x * blockDim.x) m.C.wacc[e] = 0.;
+}
+
+// ---------------------------------------------------------------------------
+// Per-body energy scalars, out[3 * b + k]: k = 0 kinetic energy from the
+// physical velocities and nodal masses, k = 1 kinetic energy from the
+// velocity parameters (rigid bodies: from the body's velocity, mass and
+// inertia), k = 2 boundary work of the last step, sum over the border slots
+// of (px vx + py vy) * length * dt: the MONITOR_ENERGY block
+// (Code/Monitoring.h:508-537) and monitor type 9 (Code/Monitoring.h:395-412).
+// One workgroup per body; each sum is per-thread partials in index order,
+// then a fixed tree.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLOCK) k_body_energy(DevModel m, double* out) {
+    __shared__ double red[3][BLOCK];
+    const int b = blockIdx.x;
+    const int t = threadIdx.x;
+    const int n0 = m.node_off[b], nn = m.body_nnodes[b];
+    const bool rigid = m.body_rigid_flag[b] != 0;
+    double ke = 0., kp = 0., dw = 0.;
+    for (int i = t; i < nn; i += BLOCK) {
+        const int j = n0 + i;
+        ke += 0.5 * m.vx[j] * m.vx[j] * m.mx[j];
+        ke += 0.5 * m.vy[j] * m.vy[j] * m.my[j];
+        if (!rigid) {
+            kp += 0.5 * m.vpx[j] * m.vpx[j] * m.mx[j];
+            kp += 0.5 * m.vpy[j] * m.vpy[j] * m.my[j];
+        }
+    }
+    const int br0 = m.border_off[b], nbr = m.body_nborders[b];
+    for (int k = 0; k < nbr; k++) {
+        const int s0 = m.br_slot0[br0 + k], ns = m.br_nslots[br0 + k];
+        for (int q = t; q < ns; q += BLOCK) {
+            const int s = s0 + q, node = m.sl_node[s];
+            dw += (m.sl_xbp[s] * m.vx[node] + m.sl_ybp[s] * m.vy[node]) * m.sl_len[s] * m.dt;
+        }
+    }
+    red[0][t] = ke; red[1][t] = kp; red[2][t] = dw;
+    __syncthreads();
+    for (int h = BLOCK / 2; h > 0; h >>= 1) {
+        if (t < h)
+            for (int k = 0; k < 3; k++) red[k][t] += red[k][t + h];
+        __syncthreads();
+    }
+    if (t == 0) {
+        double p = red[1][0];
+        if (rigid) {
+            const double* R = m.body_rigid + b * RIGID_FIELDS;
+            const double mass = m.body_mass[b], inertia = m.body_inertia[b];
+            p = 0.;
+            p += 0.5 * R[R_VEL] * R[R_VEL] * mass;
+            p += 0.5 * R[R_VEL + 1] * R[R_VEL + 1] * mass;
+            p += 0.5 * R[R_VEL + 2] * R[R_VEL + 2] * inertia;
+        }
+        out[3 * b] = red[0][0]; out[3 * b + 1] = p; out[3 * b + 2] = red[2][0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One force evaluation without integration: items i < nc are contact
+// elements (Update_contacts + Update_contact_forces, when do_contacts),
+// the next n_bc are deformable border nodes with a BC (Update_bc_forces,
+// when do_bc); then k_bucket_sums gathers the contact point forces of every
+// border node.
+// ---------------------------------------------------------------------------
+__global__ void k_eval_items(DevModel m, const int* bc_bn, int n_bc, int do_contacts, int do_bc) {
+    const int nc = do_contacts ? *m.c_count : 0;
+    const int nb = do_bc ? n_bc : 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc + nb; i += gridDim.x * blockDim.x)
+        contact_or_bc_item(m, i, nc, bc_bn, nb);
+}
+
+__global__ void k_bucket_sums(DevModel m) {
+    int bn = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bn < m.n_bnodes) bucket_sum_node(m, bn);
+}
+
+__global__ void k_clear_point_forces(DevModel m, int contacts, int bcs) {
+    int bn = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bn >= m.n_bnodes) return;
+    if (contacts) { m.bn_pcx[bn] = 0.; m.bn_pcy[bn] = 0.; }
+    if (bcs) { m.bn_pdx[bn] = 0.; m.bn_pdy[bn] = 0.; m.bn_pnx[bn] = 0.; m.bn_pny[bn] = 0.; }
+}
+
+__global__ void k_clear_node_forces(DevModel m, int internal, int damping) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m.n_nodes) return;
+    if (internal) { m.fix[j] = 0.; m.fiy[j] = 0.; }
+    if (damping) { m.fkx[j] = 0.; m.fky[j] = 0.; }
+}
+
 }  // namespace mel

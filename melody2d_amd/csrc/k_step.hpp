@@ -19,6 +19,13 @@ __global__ void k_damping(DevModel m);
 __global__ void k_rigid_nodes(DevModel m, int with_forces);
 __global__ void k_step_contacts(DevModel m, const int* bc_bn, int n_bc);
 __global__ void k_step_gather(DevModel m, const int* split_bn, int n_split);
+__global__ void k_flush_contact_work(DevModel m);
+__global__ void k_clear_contact_work(DevModel m);
+__global__ void k_body_energy(DevModel m, double* out);
+__global__ void k_eval_items(DevModel m, const int* bc_bn, int n_bc, int do_contacts, int do_bc);
+__global__ void k_bucket_sums(DevModel m);
+__global__ void k_clear_point_forces(DevModel m, int contacts, int bcs);
+__global__ void k_clear_node_forces(DevModel m, int internal, int damping);
 
 // proximity (k_proximity.hip)
 __global__ void k_prox_chunk_boxes(DevModel m);

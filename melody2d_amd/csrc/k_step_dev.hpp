@@ -173,10 +173,9 @@ __device__ inline void contact_law_element(const DevModel& m, int e) {
     double Fx = Px * L, Fy = Py * L;
     W *= L * m.dt;
     C.gapn[e] = gapn; C.gapt[e] = gapt; C.len[e] = length; C.fx[e] = Fx; C.fy[e] = Fy;
-    if (W != 0.) {
-        atomicAdd(&m.body_rigid[bS * RIGID_FIELDS + R_WORK + W_CONTACT], 0.5 * W);
-        atomicAdd(&m.body_rigid[bM * RIGID_FIELDS + R_WORK + W_CONTACT], 0.5 * W);
-    }
+    // Code/Body.h:1847-1848 add 0.5 W to the slave's and the master's
+    // contact work; the element keeps its half until the works are flushed
+    if (W != 0.) C.wacc[e] += 0.5 * W;
     int nodeS = C.nodeS[e];
     if (!m.body_rigid_flag[bS]) {
         int bn = m.sl_bn[slot];
@@ -312,6 +311,7 @@ __device__ inline void bc_point_force_node(const DevModel& m, int bn) {
             if (yt == 3) py = m.br_yval[br];
             else if (yt == 4) { px += m.br_yval[br] * (ty * tpx - tx * tpy); py += m.br_yval[br] * (ty * tpy + tx * tpx); }
             else if (yt == 2) py = m.br_yparam[2 * br] * (m.br_yval[br] - m.uy[node]) - m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node]);
+            m.sl_xbp[s] = px; m.sl_ybp[s] = py;
             double a_nx = 0., a_ny = 0., a_dx = 0., a_dy = 0.;
             if (xt == 3) a_nx = px * L;
             else if (xt == 4) { a_nx = px * L; a_ny = py * L; }
@@ -372,13 +372,20 @@ __device__ inline void rigid_bc_run(const DevModel& m, int br, int r, double xc0
         for (int u = 0; u < RUN_BATCH; u++) {
             const int node = nd[u];
             const double L = Ls[u];
-            double fxn = 0., fyn = 0., fxd = 0., fyd = 0.;
-            if (xt == 3 || xt == 4) fxn = xval * L;
-            else if (spring_x) fxd = (m.br_xparam[2 * br] * (xval - m.ux[node]) + m.br_xparam[2 * br + 1] * (m.br_xvel[br] - m.vx[node])) * L;
-            if (yt == 3 || yt == 4) fyn = yval * L;
-            else if (spring_y) fyd = (m.br_yparam[2 * br] * (yval - m.uy[node]) + m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node])) * L;
+            double fxn = 0., fyn = 0., fxd = 0., fyd = 0., bpx = 0., bpy = 0.;
+            if (xt == 3 || xt == 4) { bpx = xval; fxn = bpx * L; }
+            else if (spring_x) {
+                bpx = m.br_xparam[2 * br] * (xval - m.ux[node]) + m.br_xparam[2 * br + 1] * (m.br_xvel[br] - m.vx[node]);
+                fxd = bpx * L;
+            }
+            if (yt == 3 || yt == 4) { bpy = yval; fyn = bpy * L; }
+            else if (spring_y) {
+                bpy = m.br_yparam[2 * br] * (yval - m.uy[node]) + m.br_yparam[2 * br + 1] * (m.br_yvel[br] - m.vy[node]);
+                fyd = bpy * L;
+            }
             const double dx = px[u] - xc0, dy = py[u] - yc0;
             const bool in = jb + u < j1;
+            if (in) { m.sl_xbp[sb + jb + u] = bpx; m.sl_ybp[sb + jb + u] = bpy; }
             p6[0] += in ? fxn : 0.; p6[1] += in ? fyn : 0.; p6[2] += in ? -fxn * dy + fyn * dx : 0.;
             p6[3] += in ? fxd : 0.; p6[4] += in ? fyd : 0.; p6[5] += in ? -fxd * dy + fyd * dx : 0.;
         }

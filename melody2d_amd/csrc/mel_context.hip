@@ -150,6 +150,10 @@ struct mel_ctx {
     int n_kin_all = 0;
     bool outputs_stale = false;   // interior kinematics / force fields lag the last steps
     std::vector<int> h_blk_body;  // body of each block (work sums)
+    bool work_law = false;        // some contact law does work (DampedMohrCoulomb)
+    double* d_energy = nullptr;   // per body: kinetic, kinetic from parameters, boundary work
+    ContactArrays eval_save{};    // contact elements saved around mel_eval_forces
+    int eval_save_cap = 0;
     std::vector<int> h_bn_list;   // border nodes finished by each block (BlockArrays::bn_list)
 
     template <typename T>
@@ -197,7 +201,7 @@ void alloc_contacts(mel_ctx& c, ContactArrays& A, int cap) {
                     &A.fresh, &A.nM0, &A.nM1, &A.nM2, &A.nM3, &A.sM0, &A.sM1, &A.sM2, &A.sM3};
     for (int** p : ints) *p = c.alloc<int>(cap);
     double** dbl[] = {&A.N0, &A.N1, &A.N2, &A.N3, &A.gapn, &A.vgapn, &A.gapt, &A.vgapt, &A.xsi, &A.xn, &A.yn,
-                      &A.xt, &A.yt, &A.emass, &A.len, &A.fx, &A.fy, &A.int0};
+                      &A.xt, &A.yt, &A.emass, &A.len, &A.fx, &A.fy, &A.int0, &A.wacc};
     for (double** p : dbl) *p = c.alloc<double>(cap);
 }
 
@@ -778,17 +782,27 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
 void refresh_outputs(mel_ctx& c) {
     if (!c.outputs_stale) return;
     if (c.n_kin_all > 0) k_kinematics_list<<<blocks(c.n_kin_all), BLOCK, 0, c.stream>>>(c.d, c.d_kin_all, c.n_kin_all);
-    k_node_outputs<<<blocks(c.d.n_nodes), BLOCK, 0, c.stream>>>(c.d);
+    k_node_outputs<<<blocks(c.d.n_nodes), BLOCK, 0, c.stream>>>(c.d, 1);
     HIP_OK(hipGetLastError());
     c.outputs_stale = false;
 }
 
-// Update_proximity (Code/Proximity.h:48-473)
+// contact work held by the elements (Code/Body.h:1847-1848) added to their
+// bodies' works, in a fixed order; before each proximity update (which
+// replaces the elements) and before works are read
+void flush_contact_work(mel_ctx& c) {
+    if (!c.work_law) return;
+    k_flush_contact_work<<<c.d.n_bodies, BLOCK, 0, c.stream>>>(c.d);
+    k_clear_contact_work<<<blocks(std::max(1, c.d.c_cap)), BLOCK, 0, c.stream>>>(c.d);
+    HIP_OK(hipGetLastError());
+}
+
 // Update_proximity (Code/Proximity.h:48-473) without host round trips: the
 // kernels size their loops from counts on the device
 void update_proximity(mel_ctx& c) {
     DevModel& d = c.d;
     hipStream_t s = c.stream;
+    flush_contact_work(c);
     const int persistent = 4 * 256;     // workgroups of the grid-stride loops
     if (d.n_seg_chunks > 0) k_prox_chunk_boxes<<<blocks((long long)d.n_seg_chunks * SEG_CHUNK), BLOCK, 0, s>>>(d);
     k_prox_aabb<<<blocks((long long)d.n_borders * 64), BLOCK, 0, s>>>(d);
@@ -861,6 +875,20 @@ void run_steps(mel_ctx& c, int n) {
     HIP_OK(hipGraphLaunch(it->second, c.stream));
 }
 
+// Update_internal_forces (Code/Body.h:937-1049) over every node block
+void launch_internal_forces(mel_ctx& c, hipStream_t s) {
+    const DevModel& d = c.d;
+    const BlockArrays& B = c.blocks_dev;
+    if (B.n_blocks == 0) return;
+    if (c.precision == MEL_FP32_MIXED) {
+        if (c.wide) k_block_force_update32_wide<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B);
+        else k_block_force_update32<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B);
+    } else {
+        if (c.wide) k_block_force_update_wide<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B);
+        else k_block_force_update<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B);
+    }
+}
+
 // Euler_step (Code/Solver.h:10-88)
 void euler_step(mel_ctx& c) {
     DevModel& d = c.d;
@@ -890,13 +918,7 @@ void euler_step(mel_ctx& c) {
         TIMED(c, KT_GATHER, a, (k_step_gather<<<d.n_rigid + blocks(B.n_split_bn, BLOCK_THREADS), BLOCK_THREADS, 0, a>>>(d, B.split_bn, B.n_split_bn)));
     HIP_OK(hipEventRecord(c.ev_join, a));
     // Update_internal_forces
-    if (B.n_blocks > 0)
-        TIMED(c, KT_BLOCK_UPDATE, s,
-              (c.precision == MEL_FP32_MIXED
-                   ? (c.wide ? k_block_force_update32_wide<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B)
-                             : k_block_force_update32<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B))
-                   : (c.wide ? k_block_force_update_wide<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B)
-                             : k_block_force_update<<<B.n_blocks, BLOCK_THREADS, c.block_lds, s>>>(d, B))));
+    if (B.n_blocks > 0) TIMED(c, KT_BLOCK_UPDATE, s, launch_internal_forces(c, s));
     HIP_OK(hipStreamWaitEvent(s, c.ev_join, 0));
     // sends, Sum_up_forces, Apply_Newton, Apply_Euler, Update_borders,
     // Update_kinematics of border nodes (interior ones are rebuilt on request)
@@ -1155,6 +1177,7 @@ void build(mel_ctx& c) {
     }
     for (int i = 0; i < d.n_laws; i++) {
         law_type[i] = m.laws[i].type;
+        if (m.laws[i].type == LAW_DAMPED_MOHR_COULOMB) c.work_law = true;
         law_fixed[i] = m.laws[i].fixed_length;
         for (size_t k = 0; k < m.laws[i].p.size() && k < 12; k++) law_p[12 * i + k] = m.laws[i].p[k];
     }
@@ -1333,6 +1356,9 @@ void build(mel_ctx& c) {
     d.sl_len0 = c.upload(sl_len0);
     d.sl_xcp = c.alloc<double>(NS);
     d.sl_ycp = c.alloc<double>(NS);
+    d.sl_xbp = c.alloc<double>(NS);
+    d.sl_ybp = c.alloc<double>(NS);
+    c.d_energy = c.alloc<double>(3 * (size_t)std::max(1, NB));
     d.bn_node = c.upload(bn_node);
     d.bn_slot_ptr = c.upload(bn_slot_ptr);
     d.bn_slots = c.upload(bn_slots);
@@ -1525,6 +1551,106 @@ mel_status guarded(mel_ctx* c, F&& f) {
     }
 }
 
+// the 7 works of every body (Code/Body.h:112-118): rigid bodies and the
+// contact work from the body records, the other works of deformable bodies
+// as compensated (Neumaier) sums of their node blocks' accumulators
+std::vector<double> all_works(mel_ctx& c) {
+    const int NB = c.d.n_bodies;
+    flush_contact_work(c);
+    std::vector<double> R, w, out(7 * (size_t)NB);
+    c.download(R, c.d.body_rigid, (size_t)NB * RIGID_FIELDS);
+    if (c.blocks_dev.n_blocks > 0) c.download(w, c.blocks_dev.blk_work, 14 * (size_t)c.blocks_dev.n_blocks);
+    for (int b = 0; b < NB; b++)
+        for (int q = 0; q < 7; q++) out[7 * b + q] = R[(size_t)b * RIGID_FIELDS + R_WORK + q];
+    std::vector<double> sum(7 * (size_t)NB, 0.), comp(7 * (size_t)NB, 0.);
+    std::vector<char> has(NB, 0);
+    for (size_t k = 0; k < c.h_blk_body.size(); k++) {
+        const int b = c.h_blk_body[k];
+        has[b] = 1;
+        for (int q = 0; q < 7; q++) {
+            if (q == W_CONTACT) continue;
+            for (int h = 0; h < 2; h++) {
+                double& S = sum[7 * b + q];
+                double x = w[14 * k + 2 * q + h], t = S + x;
+                comp[7 * b + q] += std::fabs(S) >= std::fabs(x) ? (S - t) + x : (x - t) + S;
+                S = t;
+            }
+        }
+    }
+    for (int b = 0; b < NB; b++)
+        if (has[b] && !c.model.bodies[b].rigid)
+            for (int q = 0; q < 7; q++)
+                if (q != W_CONTACT) out[7 * b + q] = sum[7 * b + q] + comp[7 * b + q];
+    return out;
+}
+
+// per-body energy scalars (k_body_energy), 3 per body
+std::vector<double> all_energy(mel_ctx& c) {
+    refresh_outputs(c);
+    k_body_energy<<<c.d.n_bodies, BLOCK, 0, c.stream>>>(c.d, c.d_energy);
+    HIP_OK(hipGetLastError());
+    std::vector<double> e;
+    c.download(e, c.d_energy, 3 * (size_t)c.d.n_bodies);
+    return e;
+}
+
+// the current contact elements copied to `dst` (the first nc of each array)
+void copy_contacts(mel_ctx& c, const ContactArrays& dst, const ContactArrays& src, int nc) {
+    if (nc <= 0) return;
+    int* const* di[] = {&dst.bodyS, &dst.borderS, &dst.bnodeS, &dst.nodeS, &dst.bodyM, &dst.borderM, &dst.shiftM,
+                        &dst.segM, &dst.nbint, &dst.fresh, &dst.nM0, &dst.nM1, &dst.nM2, &dst.nM3, &dst.sM0,
+                        &dst.sM1, &dst.sM2, &dst.sM3};
+    int* const* si[] = {&src.bodyS, &src.borderS, &src.bnodeS, &src.nodeS, &src.bodyM, &src.borderM, &src.shiftM,
+                        &src.segM, &src.nbint, &src.fresh, &src.nM0, &src.nM1, &src.nM2, &src.nM3, &src.sM0,
+                        &src.sM1, &src.sM2, &src.sM3};
+    for (size_t k = 0; k < sizeof(di) / sizeof(di[0]); k++)
+        HIP_OK(hipMemcpyAsync(*di[k], *si[k], nc * sizeof(int), hipMemcpyDeviceToDevice, c.stream));
+    double* const* dd[] = {&dst.N0, &dst.N1, &dst.N2, &dst.N3, &dst.gapn, &dst.vgapn, &dst.gapt, &dst.vgapt,
+                           &dst.xsi, &dst.xn, &dst.yn, &dst.xt, &dst.yt, &dst.emass, &dst.len, &dst.fx, &dst.fy,
+                           &dst.int0, &dst.wacc};
+    double* const* sd[] = {&src.N0, &src.N1, &src.N2, &src.N3, &src.gapn, &src.vgapn, &src.gapt, &src.vgapt,
+                           &src.xsi, &src.xn, &src.yn, &src.xt, &src.yt, &src.emass, &src.len, &src.fx, &src.fy,
+                           &src.int0, &src.wacc};
+    for (size_t k = 0; k < sizeof(dd) / sizeof(dd[0]); k++)
+        HIP_OK(hipMemcpyAsync(*dd[k], *sd[k], nc * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+}
+
+// One force evaluation at the current state, without integration
+// (mel_eval_forces).  The contact elements are re-projected and their laws
+// applied on a copy of their state, which is put back afterwards, so the
+// evaluation leaves the step's state as it found it.
+void eval_forces(mel_ctx& c, unsigned what) {
+    DevModel& d = c.d;
+    hipStream_t s = c.stream;
+    const bool internal = what & MEL_EVAL_INTERNAL, contact = what & MEL_EVAL_CONTACT,
+               external = what & MEL_EVAL_EXTERNAL;
+    refresh_outputs(c);
+    const int nc = contact ? contact_count(c) : 0;
+    if (nc > 0) {
+        if (c.eval_save_cap < d.c_cap) {
+            alloc_contacts(c, c.eval_save, d.c_cap);
+            c.eval_save_cap = d.c_cap;
+        }
+        copy_contacts(c, c.eval_save, d.C, nc);
+    }
+    const BlockArrays& B = c.blocks_dev;
+    const int n_items = nc + (external ? B.n_bc_bn : 0);
+    if (n_items > 0) k_eval_items<<<blocks(n_items), BLOCK, 0, s>>>(d, B.bc_bn, B.n_bc_bn, contact, external);
+    if (d.n_bnodes > 0) {
+        k_bucket_sums<<<blocks(d.n_bnodes), BLOCK, 0, s>>>(d);
+        if (!external) k_clear_point_forces<<<blocks(d.n_bnodes), BLOCK, 0, s>>>(d, 0, 1);
+    }
+    if (external && d.n_matrix > 0) k_damping<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d);
+    if (!external || d.n_matrix == 0) k_clear_node_forces<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d, 0, 1);
+    if (internal) launch_internal_forces(c, s);
+    else k_clear_node_forces<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d, 1, 0);
+    k_node_outputs<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d, external ? 1 : 0);
+    HIP_OK(hipGetLastError());
+    if (nc > 0) copy_contacts(c, d.C, c.eval_save, nc);
+    HIP_OK(hipStreamSynchronize(s));
+    c.outputs_stale = false;
+}
+
 double* node_field_ptr(mel_ctx& c, int field) {
     DevModel& d = c.d;
     double* f[] = {d.xc, d.yc, d.ux, d.uy, d.vx, d.vy, d.ax, d.ay, d.upx, d.upy, d.vpx, d.vpy, d.apx, d.apy,
@@ -1713,27 +1839,51 @@ mel_status mel_get_body_state(mel_ctx* c, int body, double* rigid26, double* wor
             std::memcpy(rigid26, v, sizeof v);
         }
         if (works7) {
-            for (int q = 0; q < 7; q++) works7[q] = R[R_WORK + q];
-            const Body& B = c->model.bodies[body];
-            if (!B.rigid && c->blocks_dev.n_blocks > 0) {
-                std::vector<double> w;
-                c->download(w, c->blocks_dev.blk_work, 14 * (size_t)c->blocks_dev.n_blocks);
-                // compensated (Neumaier) sum over the body's blocks
-                for (int q = 0; q < 7; q++) {
-                    if (q == W_CONTACT) continue;
-                    double s = 0., comp = 0.;
-                    for (size_t k = 0; k < c->h_blk_body.size(); k++) {
-                        if (c->h_blk_body[k] != body) continue;
-                        for (int h = 0; h < 2; h++) {
-                            double x = w[14 * k + 2 * q + h], t = s + x;
-                            comp += std::fabs(s) >= std::fabs(x) ? (s - t) + x : (x - t) + s;
-                            s = t;
-                        }
-                    }
-                    works7[q] = s + comp;
-                }
+            const std::vector<double> w = all_works(*c);
+            for (int q = 0; q < 7; q++) works7[q] = w[7 * (size_t)body + q];
+        }
+        return MEL_OK;
+    });
+}
+
+mel_status mel_get_body_energy(mel_ctx* c, int body, double* energy3) {
+    if (!c || !energy3 || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        const std::vector<double> e = all_energy(*c);
+        for (int k = 0; k < 3; k++) energy3[k] = e[3 * (size_t)body + k];
+        return MEL_OK;
+    });
+}
+
+mel_status mel_get_scalars(mel_ctx* c, double* time, double* works7, double* energy3) {
+    if (!c) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        if (time) *time = c->time;
+        if (works7) {
+            const std::vector<double> w = all_works(*c);
+            for (int q = 0; q < 7; q++) {
+                works7[q] = 0.;
+                for (int b = 0; b < c->d.n_bodies; b++) works7[q] += w[7 * (size_t)b + q];
             }
         }
+        if (energy3) {
+            const std::vector<double> e = all_energy(*c);
+            for (int k = 0; k < 3; k++) {
+                energy3[k] = 0.;
+                for (int b = 0; b < c->d.n_bodies; b++) energy3[k] += e[3 * (size_t)b + k];
+            }
+        }
+        return MEL_OK;
+    });
+}
+
+mel_status mel_eval_forces(mel_ctx* c, unsigned what) {
+    if (!c || (what & ~(unsigned)MEL_EVAL_ALL)) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        HIP_OK(hipStreamSynchronize(c->stream));
+        poll_async(*c, true);
+        check_flags(*c);
+        eval_forces(*c, what);
         return MEL_OK;
     });
 }

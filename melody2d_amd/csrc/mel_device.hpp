@@ -35,6 +35,10 @@ struct ContactArrays {
     int *nM0, *nM1, *nM2, *nM3, *sM0, *sM1, *sM2, *sM3;
     double *N0, *N1, *N2, *N3, *gapn, *vgapn, *gapt, *vgapt, *xsi, *xn, *yn, *xt, *yt, *emass, *len, *fx, *fy,
         *int0;
+    // half of the element's contact work accumulated since the last flush
+    // into its bodies' work (one thread per element adds, so the sums do not
+    // depend on scheduling)
+    double* wacc;
 };
 
 struct DevModel {
@@ -97,6 +101,7 @@ struct DevModel {
     // slots
     int *sl_border, *sl_j, *sl_node, *sl_n0, *sl_n1, *sl_n2, *sl_n3, *sl_s0, *sl_s1, *sl_s2, *sl_s3, *sl_bn;
     double *sl_len, *sl_len0, *sl_xcp, *sl_ycp;
+    double *sl_xbp, *sl_ybp;                 // BC pressure of the last step (Border::x_bc_pressure)
 
     // distinct border nodes and point forces
     int *bn_node;

@@ -18,6 +18,7 @@ MEL_OK = 0
 STATUS_NAMES = {0: "MEL_OK", 1: "MEL_E_ARG", 2: "MEL_E_IO", 3: "MEL_E_PARSE", 4: "MEL_E_OOM", 5: "MEL_E_HIP",
                 6: "MEL_E_NAN", 7: "MEL_E_CAPACITY", 8: "MEL_E_UNSUPPORTED"}
 FP64, FP32_MIXED = 0, 1
+EVAL_INTERNAL, EVAL_CONTACT, EVAL_EXTERNAL, EVAL_ALL = 1, 2, 4, 7
 
 NODE_FIELDS = ["x_current", "y_current", "x_disp", "y_disp", "x_vel", "y_vel", "x_acc", "y_acc",
                "x_disp_param", "y_disp_param", "x_vel_param", "y_vel_param", "x_acc_param", "y_acc_param",
@@ -78,6 +79,9 @@ def library():
             "mel_get_body_state": ([vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)], ctypes.c_int),
             "mel_get_border_field": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_double),
                                       ctypes.c_size_t], ctypes.c_int),
+            "mel_get_body_energy": ([vp, ctypes.c_int, P(ctypes.c_double)], ctypes.c_int),
+            "mel_get_scalars": ([vp, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double)], ctypes.c_int),
+            "mel_eval_forces": ([vp, ctypes.c_uint], ctypes.c_int),
             "mel_get_info": ([vp, P(Info)], ctypes.c_int),
             "mel_write_dynamic": ([vp, ctypes.c_char_p], ctypes.c_int),
             "mel_get_stream": ([vp, P(vp)], ctypes.c_int),
@@ -113,7 +117,8 @@ def library():
 def exported_symbols():
     return ["mel_load", "mel_load_binary", "mel_update_proximity", "mel_euler_steps", "mel_synchronize",
             "mel_get_node_field", "mel_set_node_field", "mel_get_contacts", "mel_get_neighbours",
-            "mel_get_body_state", "mel_get_border_field", "mel_get_info", "mel_write_dynamic", "mel_get_stream",
+            "mel_get_body_state", "mel_get_body_energy", "mel_get_scalars", "mel_eval_forces",
+            "mel_get_border_field", "mel_get_info", "mel_write_dynamic", "mel_get_stream",
             "mel_set_kernel_timing", "mel_get_kernel_timing", "mel_kernel_count", "mel_kernel_name",
             "mel_destroy", "mel_last_error", "mel_version", "mel_get_device_bytes", "mel_set_graph_replay",
             "mel_server_set_graph_replay",
@@ -243,6 +248,27 @@ class Context:
         self._check(self.lib.mel_get_body_state(self._h, body, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                                 w.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return r, w
+
+    def body_energy(self, body):
+        """{kinetic, kinetic from velocity parameters, boundary work of the
+        last step} of one body (mel_get_body_energy)."""
+        e = np.zeros(3)
+        self._check(self.lib.mel_get_body_energy(self._h, body, e.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return e
+
+    def scalars(self):
+        """(time, works summed over bodies [7], energy scalars summed over
+        bodies [3]) (mel_get_scalars)."""
+        t = ctypes.c_double()
+        w = np.zeros(7)
+        e = np.zeros(3)
+        self._check(self.lib.mel_get_scalars(self._h, ctypes.byref(t), w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                             e.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return t.value, w, e
+
+    def eval_forces(self, what=EVAL_ALL):
+        """One force evaluation at the current state (mel_eval_forces)."""
+        self._check(self.lib.mel_eval_forces(self._h, int(what)))
 
     def border_field(self, body, border, field, n):
         out = np.zeros(n)

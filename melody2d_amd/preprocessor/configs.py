@@ -152,6 +152,98 @@ def _rsa(rng, n, rmin, rmax, W, H, gap, max_tries=20000000):
     return out
 
 
+def _deposit(rng, n, rmin, rmax, W, max_rolls=400):
+    """Sequential deposition under gravity (Visscher-Bolsterli): disc k,
+    radius ~ U[rmin, rmax], is dropped at a random abscissa in [r, W - r]
+    onto the pile, then rolls down over the disc it lands on until it rests
+    on two discs (its centre between theirs), on the floor y = 0 or against
+    a side wall x = 0 or x = W.  Every disc rests on the pile, so the
+    packing comes out with every grain touching two others or a wall; discs
+    never overlap (touching to rounding).  Returns [(x, y, r)]."""
+    cell = 2.0 * rmax
+    grid = {}
+    xs, ys, rs = [], [], []
+    cols = {}
+
+    def near(x, y, reach):
+        i0, i1 = int((x - reach) // cell), int((x + reach) // cell)
+        j0, j1 = int((y - reach) // cell), int((y + reach) // cell)
+        for i in range(i0, i1 + 1):
+            for j in range(j0, j1 + 1):
+                for k in grid.get((i, j), ()):
+                    yield k
+
+    def drop(x, r, ylim):
+        # highest resting height not above ylim when falling at abscissa x
+        best, sup = r, -1
+        for i in range(int((x - r - rmax) // cell), int((x + r + rmax) // cell) + 1):
+            for k in cols.get(i, ()):
+                dx = x - xs[k]
+                R = r + rs[k]
+                if abs(dx) < R:
+                    yy = ys[k] + math.sqrt(R * R - dx * dx)
+                    if best < yy <= ylim:
+                        best, sup = yy, k
+        return best, sup
+
+    for _ in range(n):
+        r = rmin + (rmax - rmin) * rng.random()
+        x = r + (W - 2.0 * r) * rng.random()
+        y, s1 = drop(x, r, math.inf)
+        rolls = 0
+        while s1 >= 0 and rolls < max_rolls:
+            rolls += 1
+            x1, y1, R1 = xs[s1], ys[s1], r + rs[s1]
+            th = math.atan2(x - x1, y - y1)     # from the vertical, positive towards +x
+            sgn = 1.0 if th > 0 else (-1.0 if th < 0 else (1.0 if rng.random() < 0.5 else -1.0))
+            a0 = abs(th)
+            best_a, best_k, kind = 0.5 * math.pi, -1, "fall"
+            if y1 - r < R1:                     # floor: y1 + R1 cos a = r
+                c = (r - y1) / R1
+                if -1.0 <= c <= 1.0:
+                    a = math.acos(c)
+                    if a0 - 1e-12 < a < best_a:
+                        best_a, kind = a, "floor"
+            wx = (W - r) if sgn > 0 else r      # side wall: x1 + sgn R1 sin a = wx
+            sv = (wx - x1) * sgn / R1
+            if 0.0 <= sv <= 1.0:
+                a = math.asin(sv)
+                if a0 - 1e-12 < a < best_a:
+                    best_a, kind = a, "wall"
+            for k in near(x1, y1, R1 + r + rmax):
+                if k == s1:
+                    continue
+                Rk = r + rs[k]
+                dxk, dyk = xs[k] - x1, ys[k] - y1
+                D = math.hypot(dxk, dyk)
+                if D == 0.0 or D > R1 + Rk or D < abs(R1 - Rk):
+                    continue
+                aa = (R1 * R1 - Rk * Rk + D * D) / (2.0 * D)
+                hh = math.sqrt(max(R1 * R1 - aa * aa, 0.0))
+                px, py = aa * dxk / D, aa * dyk / D
+                for q in (1.0, -1.0):
+                    ix, iy = px - q * hh * dyk / D, py + q * hh * dxk / D
+                    a = math.atan2(ix, iy) * sgn
+                    if a0 + 1e-12 < a < best_a:
+                        best_a, best_k, kind = a, k, "disc"
+            x, y = x1 + sgn * R1 * math.sin(best_a), y1 + R1 * math.cos(best_a)
+            if kind in ("floor", "wall"):
+                break
+            if kind == "fall":
+                y, s1 = drop(x, r, y)
+                continue
+            if min(x1, xs[best_k]) <= x <= max(x1, xs[best_k]):
+                break
+            s1 = best_k
+        xs.append(x)
+        ys.append(y)
+        rs.append(r)
+        k = len(xs) - 1
+        grid.setdefault((int(x // cell), int(y // cell)), []).append(k)
+        cols.setdefault(int(x // cell), []).append(k)
+    return list(zip(xs, ys, rs))
+
+
 def _grain(job):
     """One seeded polydisperse grain: border + Poisson-disc interior, MLS."""
     ss, x, y, r, nodes_per_grain, rho, nregions = job
@@ -176,13 +268,28 @@ def _map(fn, jobs, workers):
 
 
 def c2(n_grains=1000, seed=1002, nodes_per_grain=200, area_fraction=0.45, mu=0.5,
-       mass_scaling=100.0, confinement=1e5, wall_velocity=0.05, nregions=1, workers=None):
-    """Biaxial compression of polydisperse deformable grains with rigid walls."""
+       mass_scaling=100.0, confinement=1e5, wall_velocity=0.05, nregions=1, workers=None, packing="deposit"):
+    """Biaxial compression of polydisperse deformable grains with rigid walls.
+
+    packing "deposit" (default): the grains are deposited under gravity
+    (_deposit) into a box whose width makes the pile about square (1000
+    grains of radius U[0.8, 1.2] mm cover ~3200 mm^2, more than a
+    40 x 40 mm box holds); every grain rests on two others or a wall, so
+    contacts act from the start of the compression.  The top wall is put on
+    the highest grain.  packing "rsa": random sequential addition at
+    `area_fraction` in a square box, grains apart (a loose start the walls
+    take thousands of steps to reach)."""
     rng_place = np.random.Generator(np.random.PCG64(np.random.SeedSequence(seed).spawn(1)[0]))
     rmin, rmax = 0.8e-3, 1.2e-3
     mean_area = math.pi * (rmin * rmin + rmin * rmax + rmax * rmax) / 3.0
-    side = math.sqrt(n_grains * mean_area / area_fraction)
-    grains = _rsa(rng_place, n_grains, rmin, rmax, side, side, gap=0.02e-3)
+    if packing == "rsa":
+        side = math.sqrt(n_grains * mean_area / area_fraction)
+        grains = _rsa(rng_place, n_grains, rmin, rmax, side, side, gap=0.02e-3)
+        width = height = side
+    else:
+        width = math.sqrt(n_grains * mean_area / 0.74)
+        grains = _deposit(rng_place, n_grains, rmin, rmax, width)
+        height = max(y + r for _, y, r in grains)
     seeds = np.random.SeedSequence(seed + 7).spawn(n_grains)
     rho = RHO_GRAIN * mass_scaling
     jobs = [(s, x, y, r, nodes_per_grain, rho, nregions) for s, (x, y, r) in zip(seeds, grains)]
@@ -193,21 +300,21 @@ def c2(n_grains=1000, seed=1002, nodes_per_grain=200, area_fraction=0.45, mu=0.5
     # bottom and left walls fixed; top wall driven down; right wall under confinement
     fixed = lambda: BC("Driven", [(0.0, 0.0), (1e3, 0.0)])
     walls = [
-        rigid_rectangle(-t, -t, side + t, 0.0, "Steel", hw, 7800.0, detection=0.5 * hw, contact=0.5 * hw,
+        rigid_rectangle(-t, -t, width + t, 0.0, "Steel", hw, 7800.0, detection=0.5 * hw, contact=0.5 * hw,
                         xbc=fixed(), ybc=fixed()),
-        rigid_rectangle(-t, 0.0, 0.0, side, "Steel", hw, 7800.0, detection=0.5 * hw, contact=0.5 * hw,
+        rigid_rectangle(-t, 0.0, 0.0, height, "Steel", hw, 7800.0, detection=0.5 * hw, contact=0.5 * hw,
                         xbc=fixed(), ybc=fixed()),
-        rigid_rectangle(-t, side, side + t, side + t, "Steel", hw, 7800.0, detection=0.5 * hw,
+        rigid_rectangle(-t, height, width + t, height + t, "Steel", hw, 7800.0, detection=0.5 * hw,
                         contact=0.5 * hw, xbc=fixed(),
                         ybc=BC("Driven", [(0.0, -wall_velocity), (1e3, -wall_velocity)])),
     ]
     # a rigid border takes the Neumann value at every border node times its
     # tributary length (Code/Body.h:854-858): scale so the net push equals
     # confinement x the wall's face length
-    rw = rigid_rectangle(side, 0.0, side + t, side, "Steel", hw, 7800.0, detection=0.5 * hw,
+    rw = rigid_rectangle(width, 0.0, width + t, height, "Steel", hw, 7800.0, detection=0.5 * hw,
                          contact=0.5 * hw, ybc=fixed())
-    perim = 2 * (t + side)
-    p = -confinement * side / perim
+    perim = 2 * (t + height)
+    p = -confinement * height / perim
     rw.borders[0].xbc = BC("Oriented", [(0.0, p), (1e3, p)])
     walls.append(rw)
     bodies += walls
@@ -400,6 +507,28 @@ def tiny_holed(seed=1012, h=0.3e-3, R=1.5e-3, r=0.7e-3, rd=0.5e-3, hd=0.2e-3, ov
     m.dt = _critical_dt(m, 1e13)
     m.contact_update_period = 10 * m.dt
     m.tend = 1.0
+    return m
+
+
+def tiny_free(h=0.45e-3):
+    """Two discs that do not touch within a few hundred steps, loaded by
+    every force the works track except contact: a soft Dirichlet spring and
+    a following pressure on one, an oriented pressure on the other, gravity,
+    Rayleigh damping entries.  With no contact force, the kinetic energy
+    from velocity parameters changes by exactly the sum of the works
+    (Code/Body.h:2058-2066 accumulate dt (v + dt a / 2) . f for each force
+    kind): the energy balance test case."""
+    m = tiny(h=h, overlap=-3e-4)
+    m.name = "tiny_free"
+    m.materials[0] = ("Grain", "ElasticLinear", [RHO_GRAIN, 2e-6, 50.0, E_GRAIN, NU_GRAIN, 1.0])
+    m.gravity = (0.3, -9.81)
+    b0 = m.bodies[0]
+    b0.borders[0].xbc = BC("Soft", [(0.0, 0.01), (1e3, 0.01)], (1e5, 2.0))
+    b0.borders[0].ybc = BC("Following", [(0.0, -2e4), (1e3, -2e4)])
+    b1 = m.bodies[1]
+    b1.borders[0].xbc = BC("Oriented", [(0.0, 1e4), (1e3, 3e4)])
+    n = b1.n_nodes
+    b1.matrix = [(2 * i + d, 2 * i + d, 1e3 * (1 + i), 0.0) for i in range(n) for d in (0, 1)]
     return m
 
 

@@ -56,6 +56,7 @@ def device_state(ctx, n_nodes_per_body, n_borders_per_body, border_sizes):
         r, w = ctx.body_state(b)
         D[p + "rigid"] = r
         D[p + "works"] = w
+        D[p + "energy"] = ctx.body_energy(b)
         for k in range(n_borders_per_body[b]):
             q = p + "border%d." % k
             ns = border_sizes[b][k]
@@ -105,6 +106,10 @@ def compare(dev, ref, skip=()):
             continue
         if r.dtype.kind == "i":
             out[k] = (0.0 if np.array_equal(d, r) else float("inf"), bool(np.array_equal(d, r)))
+        elif k.endswith(".energy"):
+            # kinetic energies and the boundary work differ in scale: each
+            # component against its own magnitude
+            out[k] = (max(rel_err(d[i:i + 1], r[i:i + 1]) for i in range(len(r))), bool(np.array_equal(d, r)))
         else:
             out[k] = (rel_err(d, r), bool(np.array_equal(d, r)))
     return out

@@ -1,7 +1,12 @@
-"""Size-independent checks on the full benchmark workload (1000 deformable
-grains + 4 rigid walls, ~209k nodes, ~16M support entries), through the
-C-ABI.  The oracle is too slow for exact comparison at this size, so these
-tests use properties the mechanics fixes:
+"""The full benchmark workload (1000 deformable grains deposited into a
+pile, + 4 rigid walls, ~208k nodes, ~16M support entries, ~1.8e4 contact
+elements), through the C-ABI.
+
+Against the oracle, from the same binary container: contact candidate and
+active-contact sets and neighbour lists exactly, every floating-point record
+of every body within 1e-12 relative, at steps 0, 1, 10 and 25 (two contact
+updates); on the deposited pile (every grain touching others) and on a
+loose random-addition packing.  Then properties the mechanics fixes:
 
 * the two-launch step and the launch sequence are bit-identical (across
   contact updates);
@@ -19,17 +24,48 @@ from melody2d_amd import native
 
 pytestmark = pytest.mark.gpu
 
-@pytest.fixture(scope="module")
-def workload(tmp_path_factory):
+def _make(tmp_path_factory, packing):
     from melody2d_amd.preprocessor import c2
     from melody2d_amd.preprocessor.binary import write_binary
-    model = c2(n_grains=1000)
-    path = str(tmp_path_factory.mktemp("c2") / "c2.bin")
+    model = c2(n_grains=1000, packing=packing)
+    path = str(tmp_path_factory.mktemp("c2_" + packing) / "c2.bin")
     write_binary(model, path)
     rigid = [b.rigid for b in model.bodies]
     counts = [b.n_nodes for b in model.bodies]
-    print("workload: %d bodies, %d nodes" % (len(counts), sum(counts)), flush=True)
+    print("workload %s: %d bodies, %d nodes" % (packing, len(counts), sum(counts)), flush=True)
     return path, counts, rigid
+
+
+@pytest.fixture(scope="module")
+def workload(tmp_path_factory):
+    return _make(tmp_path_factory, "deposit")
+
+
+@pytest.mark.parametrize("packing", ["deposit", "rsa"])
+def test_full_size_matches_oracle(packing, workload, tmp_path_factory, tmp_path):
+    import sys, os
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from mel_compare import compare, device_state, layout_from_dump, run_oracle
+    path, counts, rigid = workload if packing == "deposit" else _make(tmp_path_factory, packing)
+    print("case full-size vs oracle", packing, flush=True)
+    steps = [0, 1, 10, 25]
+    refs, _ = run_oracle(path, steps, str(tmp_path / "oracle"), save_index=-1)
+    ctx = native.Context(path, binary=True)
+    prev = 0
+    n_contacts = 0
+    for s in steps:
+        ctx.euler_steps(s - prev)
+        prev = s
+        nn, nbr, sizes = layout_from_dump(refs[s])
+        dev = device_state(ctx, nn, nbr, sizes)
+        res = compare(dev, refs[s])
+        bad = sorted(((k, v[0]) for k, v in res.items() if not v[0] <= 1e-12), key=lambda kv: -kv[1])
+        assert not bad, "step %d: %d records off, worst %s" % (s, len(bad), bad[:6])
+        n_contacts = sum(len(refs[s]["b%d.contacts_i" % b]) // 16 for b in range(len(nn)))
+        print("  step %d: %d records, %d contact elements" % (s, len(res), n_contacts), flush=True)
+    if packing == "deposit":
+        assert n_contacts >= 10000
+    ctx.close()
 
 
 def _deformable(counts, rigid):

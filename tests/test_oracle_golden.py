@@ -67,3 +67,26 @@ def test_goldens_exercise_the_step():
                     seen["master_border_3"] |= bool(np.any(ci[:, 4] == 3))
                     seen["boundary_work"] |= bool(D[p + "energy"][2] != 0)
     assert all(seen.values()), seen
+
+
+def test_oracle_energy_balance(tmp_path):
+    """Pins the oracle's work bookkeeping on the mathematics: with no contact
+    force, v' = v + dt a and a = f / m give 1/2 m (v'^2 - v^2) =
+    dt (v + dt a / 2) . f for every node, so the kinetic energy from velocity
+    parameters changes by exactly the sum of the works (internal, body,
+    Dirichlet, Neumann, damping) up to rounding."""
+    from melody2d_amd.preprocessor import write_model
+    from melody2d_amd.preprocessor.configs import tiny_free
+    write_model(tiny_free(), str(tmp_path / "model"))
+    steps = [0, 1, 10, 100, 400]
+    got, _ = run_oracle(str(tmp_path / "model"), steps, str(tmp_path / "oracle"))
+    nb = int(got[0]["n_bodies"][0])
+    ke = {s: sum(got[s]["b%d.energy" % b][1] for b in range(nb)) for s in steps}
+    w = {s: sum(got[s]["b%d.works" % b] for b in range(nb)) for s in steps}
+    for s in steps:
+        assert all(len(got[s]["b%d.contacts_i" % b]) == 0 for b in range(nb))
+    for s in steps[1:]:
+        dk = ke[s] - ke[0]
+        dw = float(np.sum(w[s] - w[0]))
+        assert np.count_nonzero(w[s]) >= 4
+        assert abs(dk - dw) <= 1e-12 * abs(dk), (s, dk, dw)
