@@ -135,6 +135,16 @@ typedef enum {
  * resultants are not re-formed. */
 mel_status mel_eval_forces(mel_ctx* ctx, unsigned what);
 
+/* mel_eval_forces without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The contact
+ * elements' state is saved in a device scratch array of the library (at
+ * most 65536 elements, 14 MiB, held by every GPU the library runs on until
+ * the process ends) instead of a buffer the context allocates on first
+ * use; larger element counts return MEL_E_CAPACITY.  All contexts on one
+ * GPU share that scratch: calls that overlap on one GPU race. */
+mel_status mel_eval_forces_noalloc(mel_ctx* ctx, unsigned what);
+
 /* per border-node slot: 0 tributary length (Code/Body.h:614-661),
  * 1 / 2 x / y contact pressure (Update_contact_pressures, Code/Proximity.h:1014-1069) */
 mel_status mel_get_border_field(mel_ctx* ctx, int body, int border, int field, double* out, size_t n);

@@ -9,6 +9,8 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -53,6 +55,76 @@ inline int blocks(long long n, int b = BLOCK) { return (int)std::max<long long>(
             launch;                                \
         }                                          \
     } while (0)
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Device scratch of the no-allocation force evaluation
+// (mel_eval_forces_noalloc): the contact elements' state saved around the
+// evaluation, at most SCRATCH_MAX_CONTACTS elements, in a file-scope device
+// array that every GPU the library runs on holds until the process ends.
+// Layout (fixed capacity, so offsets do not depend on the count): the 19
+// double fields of ContactArrays, SCRATCH_MAX_CONTACTS each, then its 18 int
+// fields.  The array's host-side name is not a device address: host code
+// uses the per-device address from hipGetSymbolAddress (scratch_address).
+// ---------------------------------------------------------------------------
+constexpr int SCRATCH_MAX_CONTACTS = 65536;
+constexpr int SCRATCH_DBL_FIELDS = 19, SCRATCH_INT_FIELDS = 18;
+constexpr size_t SCRATCH_BYTES =
+    (size_t)SCRATCH_MAX_CONTACTS * (SCRATCH_DBL_FIELDS * sizeof(double) + SCRATCH_INT_FIELDS * sizeof(int));
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+__device__ inline void contact_fields(const ContactArrays& A, double** dp, int** ip) {
+    double* d[SCRATCH_DBL_FIELDS] = {A.N0, A.N1, A.N2, A.N3, A.gapn, A.vgapn, A.gapt, A.vgapt, A.xsi, A.xn,
+                                     A.yn, A.xt, A.yt, A.emass, A.len, A.fx, A.fy, A.int0, A.wacc};
+    int* i[SCRATCH_INT_FIELDS] = {A.bodyS, A.borderS, A.bnodeS, A.nodeS, A.bodyM, A.borderM, A.shiftM, A.segM,
+                                  A.nbint, A.fresh, A.nM0, A.nM1, A.nM2, A.nM3, A.sM0, A.sM1, A.sM2, A.sM3};
+    for (int k = 0; k < SCRATCH_DBL_FIELDS; k++) dp[k] = d[k];
+    for (int k = 0; k < SCRATCH_INT_FIELDS; k++) ip[k] = i[k];
+}
+
+// element e of A to (to_buf) or from the scratch buffer `buf`, for e < nc
+__global__ void k_contacts_scratch(ContactArrays A, unsigned char* buf, int nc, int to_buf) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nc) return;
+    double* dp[SCRATCH_DBL_FIELDS];
+    int* ip[SCRATCH_INT_FIELDS];
+    contact_fields(A, dp, ip);
+    double* bd = reinterpret_cast<double*>(buf);
+    int* bi = reinterpret_cast<int*>(buf + (size_t)SCRATCH_DBL_FIELDS * SCRATCH_MAX_CONTACTS * sizeof(double));
+    for (int k = 0; k < SCRATCH_DBL_FIELDS; k++) {
+        double* q = bd + (size_t)k * SCRATCH_MAX_CONTACTS + e;
+        if (to_buf) *q = dp[k][e]; else dp[k][e] = *q;
+    }
+    for (int k = 0; k < SCRATCH_INT_FIELDS; k++) {
+        int* q = bi + (size_t)k * SCRATCH_MAX_CONTACTS + e;
+        if (to_buf) *q = ip[k][e]; else ip[k][e] = *q;
+    }
+}
+
+// the device address of `scratch` on `device` (current), taken once per
+// device when the first context is created on it
+void scratch_setup(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return;
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+}
+
+void* scratch_address(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device >= 0 && device < SCRATCH_MAX_DEVICES ? g_scratch_ptr[device] : nullptr;
+}
 
 }  // namespace
 
@@ -1512,6 +1584,11 @@ mel_status create(mel_ctx** out, int device, int precision, const std::function<
     }
     try {
         HIP_OK(hipSetDevice(device));
+        try {
+            scratch_setup(device);
+        } catch (std::runtime_error& e) {
+            throw HipError{e.what()};
+        }
         HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIP_OK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
         HIP_OK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
@@ -1619,14 +1696,21 @@ void copy_contacts(mel_ctx& c, const ContactArrays& dst, const ContactArrays& sr
 // (mel_eval_forces).  The contact elements are re-projected and their laws
 // applied on a copy of their state, which is put back afterwards, so the
 // evaluation leaves the step's state as it found it.
-void eval_forces(mel_ctx& c, unsigned what) {
+// save_buf: the device scratch (no allocation), or nullptr to save the
+// contact elements into a buffer the context allocates on first use
+void eval_forces(mel_ctx& c, unsigned what, unsigned char* save_buf = nullptr) {
     DevModel& d = c.d;
     hipStream_t s = c.stream;
     const bool internal = what & MEL_EVAL_INTERNAL, contact = what & MEL_EVAL_CONTACT,
                external = what & MEL_EVAL_EXTERNAL;
     refresh_outputs(c);
     const int nc = contact ? contact_count(c) : 0;
-    if (nc > 0) {
+    if (nc > 0 && save_buf) {
+        if (nc > SCRATCH_MAX_CONTACTS)
+            throw CapacityError{"mel_eval_forces_noalloc: " + std::to_string(nc) + " contact elements exceed the " +
+                                std::to_string(SCRATCH_MAX_CONTACTS) + " its device scratch holds"};
+        k_contacts_scratch<<<blocks(nc), BLOCK, 0, s>>>(d.C, save_buf, nc, 1);
+    } else if (nc > 0) {
         if (c.eval_save_cap < d.c_cap) {
             alloc_contacts(c, c.eval_save, d.c_cap);
             c.eval_save_cap = d.c_cap;
@@ -1646,7 +1730,9 @@ void eval_forces(mel_ctx& c, unsigned what) {
     else k_clear_node_forces<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d, 1, 0);
     k_node_outputs<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d, external ? 1 : 0);
     HIP_OK(hipGetLastError());
-    if (nc > 0) copy_contacts(c, d.C, c.eval_save, nc);
+    if (nc > 0 && save_buf) k_contacts_scratch<<<blocks(nc), BLOCK, 0, s>>>(d.C, save_buf, nc, 0);
+    else if (nc > 0) copy_contacts(c, d.C, c.eval_save, nc);
+    HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(s));
     c.outputs_stale = false;
 }
@@ -1884,6 +1970,22 @@ mel_status mel_eval_forces(mel_ctx* c, unsigned what) {
         poll_async(*c, true);
         check_flags(*c);
         eval_forces(*c, what);
+        return MEL_OK;
+    });
+}
+
+mel_status mel_eval_forces_noalloc(mel_ctx* c, unsigned what) {
+    if (!c || (what & ~(unsigned)MEL_EVAL_ALL)) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        unsigned char* buf = static_cast<unsigned char*>(scratch_address(c->device));
+        if (!buf) {
+            c->err = "mel_eval_forces_noalloc: no device scratch on device " + std::to_string(c->device);
+            return MEL_E_HIP;
+        }
+        HIP_OK(hipStreamSynchronize(c->stream));
+        poll_async(*c, true);
+        check_flags(*c);
+        eval_forces(*c, what, buf);
         return MEL_OK;
     });
 }

@@ -82,6 +82,7 @@ def library():
             "mel_get_body_energy": ([vp, ctypes.c_int, P(ctypes.c_double)], ctypes.c_int),
             "mel_get_scalars": ([vp, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double)], ctypes.c_int),
             "mel_eval_forces": ([vp, ctypes.c_uint], ctypes.c_int),
+            "mel_eval_forces_noalloc": ([vp, ctypes.c_uint], ctypes.c_int),
             "mel_get_info": ([vp, P(Info)], ctypes.c_int),
             "mel_write_dynamic": ([vp, ctypes.c_char_p], ctypes.c_int),
             "mel_get_stream": ([vp, P(vp)], ctypes.c_int),
@@ -118,6 +119,7 @@ def exported_symbols():
     return ["mel_load", "mel_load_binary", "mel_update_proximity", "mel_euler_steps", "mel_synchronize",
             "mel_get_node_field", "mel_set_node_field", "mel_get_contacts", "mel_get_neighbours",
             "mel_get_body_state", "mel_get_body_energy", "mel_get_scalars", "mel_eval_forces",
+            "mel_eval_forces_noalloc",
             "mel_get_border_field", "mel_get_info", "mel_write_dynamic", "mel_get_stream",
             "mel_set_kernel_timing", "mel_get_kernel_timing", "mel_kernel_count", "mel_kernel_name",
             "mel_destroy", "mel_last_error", "mel_version", "mel_get_device_bytes", "mel_set_graph_replay",
@@ -269,6 +271,10 @@ class Context:
     def eval_forces(self, what=EVAL_ALL):
         """One force evaluation at the current state (mel_eval_forces)."""
         self._check(self.lib.mel_eval_forces(self._h, int(what)))
+
+    def eval_forces_noalloc(self, what=EVAL_ALL):
+        """mel_eval_forces without device allocation (mel_eval_forces_noalloc)."""
+        self._check(self.lib.mel_eval_forces_noalloc(self._h, int(what)))
 
     def border_field(self, body, border, field, n):
         out = np.zeros(n)
