@@ -118,6 +118,19 @@ mel_status mel_get_body_energy(mel_ctx* ctx, int body, double* energy3);
  * computes it (Code/Model.h:28, Code/Model.h:72). */
 mel_status mel_get_scalars(mel_ctx* ctx, double* time, double* works7, double* energy3);
 
+/* In-memory checkpoint of the dynamic state at full precision: node
+ * kinematics, parameters and force fields, rigid-body records and works,
+ * BC values, border lengths, neighbour lists, contact elements, time and
+ * the contact-update schedule (what Write_dynamic / Load_dynamic,
+ * Code/IO.h:1871-2636, carry between runs, without the ASCII rounding).
+ * mel_load_state accepts a state saved from a context of the same model,
+ * in either precision; it is checked before anything is written
+ * (MEL_E_PARSE when it does not fit the model, MEL_E_CAPACITY when its
+ * contact elements exceed this context's capacity). */
+mel_status mel_get_state_size(mel_ctx* ctx, size_t* bytes);
+mel_status mel_save_state(mel_ctx* ctx, void* buf, size_t cap, size_t* written);
+mel_status mel_load_state(mel_ctx* ctx, const void* buf, size_t size);
+
 /* what mel_eval_forces evaluates */
 typedef enum {
     MEL_EVAL_INTERNAL = 1,  /* Body::Update_internal_forces (Code/Body.h:937-1049) */

@@ -633,11 +633,15 @@ __device__ inline void rigid_bc_item(const DevModel& m, int j) {
 
 // rigid node kinematics and positions (Code/Body.h:2214-2226, 2281-2301);
 // node force fields mirror the body totals as Code/Body.h:1999-2013 does.
+// with_forces < 0: positions and velocities only, as the state a model is
+// loaded with (Load_dynamic reads no node kinematics of rigid bodies)
 __device__ inline void rigid_node_at(const DevModel& m, int i, const double* R, double c, double s,
                                      int with_forces, double Xi, double Yi) {
     double dx = Xi - R[R_XC], dy = Yi - R[R_YC];
-    m.ux[i] = R[R_DISP] + dx * c - dy * s - dx;
-    m.uy[i] = R[R_DISP + 1] + dx * s + dy * c - dy;
+    if (with_forces >= 0) {
+        m.ux[i] = R[R_DISP] + dx * c - dy * s - dx;
+        m.uy[i] = R[R_DISP + 1] + dx * s + dy * c - dy;
+    }
     double xx = R[R_CUR] + dx * c - dy * s;
     double yy = R[R_CUR + 1] + dx * s + dy * c;
     m.xc[i] = xx;
@@ -645,7 +649,7 @@ __device__ inline void rigid_node_at(const DevModel& m, int i, const double* R, 
     double ex = xx - R[R_CUR], ey = yy - R[R_CUR + 1];
     m.vx[i] = -ey * R[R_VEL + 2] + R[R_VEL];
     m.vy[i] = ex * R[R_VEL + 2] + R[R_VEL + 1];
-    if (with_forces) {
+    if (with_forces > 0) {
         m.ax[i] = R[R_NODE_ACC]; m.ay[i] = R[R_NODE_ACC + 1];
         m.fcx[i] = R[R_FCON]; m.fcy[i] = R[R_FCON + 1];
         m.fbx[i] = R[R_FBODY]; m.fby[i] = R[R_FBODY + 1];

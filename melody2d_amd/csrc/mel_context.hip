@@ -223,6 +223,10 @@ struct mel_ctx {
     bool outputs_stale = false;   // interior kinematics / force fields lag the last steps
     std::vector<int> h_blk_body;  // body of each block (work sums)
     bool work_law = false;        // some contact law does work (DampedMohrCoulomb)
+    // a proximity update ran after the last step: the elements of that step
+    // (whose forces the step's contact pressures come from) are the previous
+    // buffer
+    bool prox_after_step = false;
     double* d_energy = nullptr;   // per body: kinetic, kinetic from parameters, boundary work
     ContactArrays eval_save{};    // contact elements saved around mel_eval_forces
     int eval_save_cap = 0;
@@ -876,6 +880,7 @@ void update_proximity(mel_ctx& c) {
     hipStream_t s = c.stream;
     flush_contact_work(c);
     const int persistent = 4 * 256;     // workgroups of the grid-stride loops
+    c.prox_after_step = true;
     if (d.n_seg_chunks > 0) k_prox_chunk_boxes<<<blocks((long long)d.n_seg_chunks * SEG_CHUNK), BLOCK, 0, s>>>(d);
     k_prox_aabb<<<blocks((long long)d.n_borders * 64), BLOCK, 0, s>>>(d);
     k_prox_neighbours<<<d.n_bodies, NB_THREADS, 0, s>>>(d);
@@ -927,6 +932,7 @@ void launch_step(mel_ctx& c, int n_cwg) {
 void run_steps(mel_ctx& c, int n) {
     poll_async(c, false);
     const int n_cwg = contact_workgroups(c);
+    if (n > 0) c.prox_after_step = false;
     if (!c.use_graphs || c.timing || n < 2) {
         for (int k = 0; k < n; k++) launch_step(c, n_cwg);
         return;
@@ -967,6 +973,7 @@ void euler_step(mel_ctx& c) {
     const BlockArrays& B = c.blocks_dev;
     hipStream_t s = c.stream, a = c.aux;
     poll_async(c, false);
+    c.prox_after_step = false;
     const int n_cwg = contact_workgroups(c);
     const double t_next = c.time + d.dt;
     if (c.two_launch) {
@@ -1002,6 +1009,47 @@ void euler_step(mel_ctx& c) {
         k_kinematics_list<<<blocks(B.n_split_kin), BLOCK, 0, s>>>(d, B.split_kin, B.n_split_kin);
     c.time = t_next;
     c.outputs_stale = true;
+}
+
+// The contact elements read from the DYNAMIC file (Load_dynamic,
+// Code/IO.h:2222-2287) as the current elements, so that the first
+// Update_proximity of the initialisation block transfers their history
+// (Code/Proximity.h:271-319), as the reference does on a restart.
+void upload_loaded_contacts(mel_ctx& c) {
+    const Model& m = c.model;
+    const DevModel& d = c.d;
+    std::vector<int> iv[18];
+    std::vector<double> dv[18];
+    for (size_t b = 0; b < m.bodies.size(); b++)
+        for (const ContactElement& E : m.bodies[b].contacts) {
+            // node ids are local to their body in the file, global on the device
+            const int oS = c.h_node_off[b], oM = c.h_node_off.at(E.bodyM);
+            const int ints[18] = {(int)b, E.borderS, E.border_nodeS, E.nodeS + oS, E.bodyM, E.borderM, E.shiftM,
+                                  E.segmentM, E.nb_internal, E.bodyM == (int)b ? 2 : 1, E.nodeM[0] + oM,
+                                  E.nodeM[1] + oM, E.nodeM[2] + oM, E.nodeM[3] + oM, E.shiftMk[0], E.shiftMk[1],
+                                  E.shiftMk[2], E.shiftMk[3]};
+            const double dbl[18] = {E.shapeM[0], E.shapeM[1], E.shapeM[2], E.shapeM[3], E.gapn, E.vgapn, E.gapt,
+                                    E.vgapt, E.xsi, E.xnorm, E.ynorm, E.xtan, E.ytan, E.effective_mass, E.length,
+                                    E.fx, E.fy, E.internal.empty() ? 0. : E.internal[0]};
+            for (int k = 0; k < 18; k++) { iv[k].push_back(ints[k]); dv[k].push_back(dbl[k]); }
+        }
+    const int n = (int)iv[0].size();
+    if (n > d.c_cap)
+        throw CapacityError{"the loaded state holds " + std::to_string(n) + " contact elements, above the capacity " +
+                            std::to_string(d.c_cap)};
+    if (n > 0) {
+        const ContactArrays& A = c.bufA;
+        int* ip[18] = {A.bodyS, A.borderS, A.bnodeS, A.nodeS, A.bodyM, A.borderM, A.shiftM, A.segM, A.nbint,
+                       A.fresh, A.nM0, A.nM1, A.nM2, A.nM3, A.sM0, A.sM1, A.sM2, A.sM3};
+        double* dp[18] = {A.N0, A.N1, A.N2, A.N3, A.gapn, A.vgapn, A.gapt, A.vgapt, A.xsi, A.xn, A.yn, A.xt, A.yt,
+                          A.emass, A.len, A.fx, A.fy, A.int0};
+        for (int k = 0; k < 18; k++) {
+            HIP_OK(hipMemcpyAsync(ip[k], iv[k].data(), n * sizeof(int), hipMemcpyHostToDevice, c.stream));
+            HIP_OK(hipMemcpyAsync(dp[k], dv[k].data(), n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        }
+    }
+    HIP_OK(hipMemcpyAsync(d.c_count, &n, sizeof(int), hipMemcpyHostToDevice, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
 }
 
 // flatten the parsed model into device arrays
@@ -1451,6 +1499,7 @@ void build(mel_ctx& c) {
     alloc_contacts(c, c.bufB, d.c_cap);
     d.C = c.bufA;
     d.Cold = c.bufB;
+    upload_loaded_contacts(c);
     d.rec_bn = c.alloc<int>(3 * (size_t)d.c_cap);
     d.rec_fx = c.alloc<double>(3 * (size_t)d.c_cap);
     d.rec_fy = c.alloc<double>(3 * (size_t)d.c_cap);
@@ -1528,7 +1577,7 @@ void build(mel_ctx& c) {
     c.next_contact_update = m.next_contact_update;
 
     // rigid node positions, Update_current_positions at load (Code/IO.h:2050)
-    if (d.n_rigid_nodes > 0) k_rigid_nodes<<<blocks(d.n_rigid_nodes), BLOCK, 0, c.stream>>>(d, 0);
+    if (d.n_rigid_nodes > 0) k_rigid_nodes<<<blocks(d.n_rigid_nodes), BLOCK, 0, c.stream>>>(d, -1);
     // ---- initialisation block, Code/main.cpp:103-126 ----
     k_border_length<<<blocks(NS), BLOCK, 0, c.stream>>>(d);          // Update_borders
     k_bc_eval<<<blocks(NBR), BLOCK, 0, c.stream>>>(d, 0);            // Update_bc(Time)
@@ -1735,6 +1784,64 @@ void eval_forces(mel_ctx& c, unsigned what, unsigned char* save_buf = nullptr) {
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(s));
     c.outputs_stale = false;
+}
+
+// ---------------------------------------------------------------------------
+// In-memory checkpoint of the dynamic state (mel_save_state / mel_load_state)
+// ---------------------------------------------------------------------------
+constexpr uint64_t STATE_MAGIC = 0x3145544154534c4dull;   // "MLSTATE1"
+
+struct StateArray {
+    void* p;
+    size_t bytes;
+};
+
+// the device arrays that carry the state between steps, in blob order;
+// contact element arrays hold the first nc elements
+std::vector<StateArray> state_arrays(mel_ctx& c, int nc) {
+    DevModel& d = c.d;
+    std::vector<StateArray> v;
+    const size_t N = d.n_nodes, NB = d.n_bodies, NS = d.n_slots, NBR = d.n_borders, NBN = d.n_bnodes;
+    double* nf[] = {d.xc, d.yc, d.ux, d.uy, d.vx, d.vy, d.ax, d.ay, d.upx, d.upy, d.vpx, d.vpy, d.apx, d.apy,
+                    d.fix, d.fiy, d.fcx, d.fcy, d.fbx, d.fby, d.fdx, d.fdy, d.fnx, d.fny, d.fkx, d.fky,
+                    d.fx, d.fy, d.mx, d.my, d.imx, d.imy};
+    for (double* p : nf) v.push_back({p, N * sizeof(double)});
+    v.push_back({d.body_rigid, NB * RIGID_FIELDS * sizeof(double)});
+    v.push_back({d.body_mass, NB * sizeof(double)});
+    v.push_back({d.body_inv_mass, NB * sizeof(double)});
+    double* bf[] = {d.br_xval, d.br_yval, d.br_xvel, d.br_yvel};
+    for (double* p : bf) v.push_back({p, NBR * sizeof(double)});
+    double* sf[] = {d.sl_len, d.sl_xbp, d.sl_ybp};
+    for (double* p : sf) v.push_back({p, NS * sizeof(double)});
+    v.push_back({d.slot_xy, NS * sizeof(double2)});
+    double* bnf[] = {d.bn_pcx, d.bn_pcy, d.bn_pdx, d.bn_pdy, d.bn_pnx, d.bn_pny};
+    for (double* p : bnf) v.push_back({p, NBN * sizeof(double)});
+    v.push_back({d.nb_count, NB * sizeof(int)});
+    v.push_back({d.nb_list, NB * (size_t)d.nb_cap * 3 * sizeof(int)});
+    if (d.n_rigid > 0) v.push_back({d.rigid_dv, (size_t)d.n_rigid * 8 * sizeof(double)});
+    if (c.blocks_dev.n_blocks > 0) v.push_back({c.blocks_dev.blk_work, 14 * (size_t)c.blocks_dev.n_blocks * sizeof(double)});
+    v.push_back({d.time, sizeof(double)});
+    const ContactArrays& A = d.C;
+    int* ci[] = {A.bodyS, A.borderS, A.bnodeS, A.nodeS, A.bodyM, A.borderM, A.shiftM, A.segM, A.nbint, A.fresh,
+                 A.nM0, A.nM1, A.nM2, A.nM3, A.sM0, A.sM1, A.sM2, A.sM3};
+    for (int* p : ci) v.push_back({p, (size_t)nc * sizeof(int)});
+    double* cd[] = {A.N0, A.N1, A.N2, A.N3, A.gapn, A.vgapn, A.gapt, A.vgapt, A.xsi, A.xn, A.yn, A.xt, A.yt,
+                    A.emass, A.len, A.fx, A.fy, A.int0, A.wacc};
+    for (double* p : cd) v.push_back({p, (size_t)nc * sizeof(double)});
+    return v;
+}
+
+struct StateHeader {
+    uint64_t magic;
+    uint64_t n_arrays;
+    int64_t n_nodes, n_bodies, n_slots, n_contacts, iteration, reserved;
+    double time, next_contact_update;
+};
+
+size_t state_size(mel_ctx& c, int nc) {
+    size_t n = sizeof(StateHeader);
+    for (const StateArray& a : state_arrays(c, nc)) n += sizeof(uint64_t) + a.bytes;
+    return n;
 }
 
 double* node_field_ptr(mel_ctx& c, int field) {
@@ -1974,6 +2081,101 @@ mel_status mel_eval_forces(mel_ctx* c, unsigned what) {
     });
 }
 
+mel_status mel_get_state_size(mel_ctx* c, size_t* bytes) {
+    if (!c || !bytes) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        HIP_OK(hipStreamSynchronize(c->stream));
+        poll_async(*c, true);
+        *bytes = state_size(*c, contact_count(*c));
+        return MEL_OK;
+    });
+}
+
+mel_status mel_save_state(mel_ctx* c, void* buf, size_t cap, size_t* written) {
+    if (!c || !buf) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        HIP_OK(hipStreamSynchronize(c->stream));
+        poll_async(*c, true);
+        check_flags(*c);
+        flush_contact_work(*c);
+        refresh_outputs(*c);
+        const int nc = contact_count(*c);
+        const size_t need = state_size(*c, nc);
+        if (cap < need) {
+            c->err = "mel_save_state: buffer of " + std::to_string(cap) + " bytes, the state needs " + std::to_string(need);
+            return MEL_E_ARG;
+        }
+        const std::vector<StateArray> arrs = state_arrays(*c, nc);
+        StateHeader h{STATE_MAGIC, arrs.size(), c->d.n_nodes, c->d.n_bodies, c->d.n_slots, nc, c->iteration, 0,
+                      c->time, c->next_contact_update};
+        unsigned char* o = static_cast<unsigned char*>(buf);
+        std::memcpy(o, &h, sizeof h);
+        size_t pos = sizeof h;
+        for (const StateArray& a : arrs) {
+            uint64_t nb = a.bytes;
+            std::memcpy(o + pos, &nb, sizeof nb);
+            pos += sizeof nb;
+            if (a.bytes) HIP_OK(hipMemcpyAsync(o + pos, a.p, a.bytes, hipMemcpyDeviceToHost, c->stream));
+            pos += a.bytes;
+        }
+        HIP_OK(hipStreamSynchronize(c->stream));
+        if (written) *written = pos;
+        return MEL_OK;
+    });
+}
+
+mel_status mel_load_state(mel_ctx* c, const void* buf, size_t size) {
+    if (!c || !buf) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        StateHeader h;
+        if (size < sizeof h) { c->err = "mel_load_state: truncated state"; return MEL_E_PARSE; }
+        std::memcpy(&h, buf, sizeof h);
+        if (h.magic != STATE_MAGIC || h.n_nodes != c->d.n_nodes || h.n_bodies != c->d.n_bodies ||
+            h.n_slots != c->d.n_slots) {
+            c->err = "mel_load_state: the state is not one of this model";
+            return MEL_E_PARSE;
+        }
+        if (h.n_contacts < 0 || h.n_contacts > c->d.c_cap) {
+            c->err = "mel_load_state: " + std::to_string(h.n_contacts) + " contact elements exceed the capacity " +
+                     std::to_string(c->d.c_cap);
+            return MEL_E_CAPACITY;
+        }
+        const int nc = (int)h.n_contacts;
+        HIP_OK(hipStreamSynchronize(c->stream));
+        poll_async(*c, true);
+        const std::vector<StateArray> arrs = state_arrays(*c, nc);
+        if (h.n_arrays != arrs.size()) { c->err = "mel_load_state: the state is not one of this model"; return MEL_E_PARSE; }
+        // check every record before writing any
+        const unsigned char* in = static_cast<const unsigned char*>(buf);
+        size_t pos = sizeof h;
+        for (const StateArray& a : arrs) {
+            uint64_t nb;
+            if (pos + sizeof nb > size) { c->err = "mel_load_state: truncated state"; return MEL_E_PARSE; }
+            std::memcpy(&nb, in + pos, sizeof nb);
+            if (nb != a.bytes || pos + sizeof nb + nb > size) {
+                c->err = "mel_load_state: a record does not match this model";
+                return MEL_E_PARSE;
+            }
+            pos += sizeof nb + nb;
+        }
+        pos = sizeof h;
+        for (const StateArray& a : arrs) {
+            pos += sizeof(uint64_t);
+            if (a.bytes) HIP_OK(hipMemcpyAsync(a.p, in + pos, a.bytes, hipMemcpyHostToDevice, c->stream));
+            pos += a.bytes;
+        }
+        HIP_OK(hipMemcpyAsync(c->d.c_count, &nc, sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        c->n_contacts = nc;
+        c->nc_hint = nc;
+        c->time = h.time;
+        c->next_contact_update = h.next_contact_update;
+        c->iteration = (int)h.iteration;
+        c->outputs_stale = false;
+        return MEL_OK;
+    });
+}
+
 mel_status mel_eval_forces_noalloc(mel_ctx* c, unsigned what) {
     if (!c || (what & ~(unsigned)MEL_EVAL_ALL)) return MEL_E_ARG;
     return guarded(c, [&]() {
@@ -2003,9 +2205,16 @@ mel_status mel_get_border_field(mel_ctx* c, int body, int border, int field, dou
             return MEL_OK;
         }
         // Update_contact_pressures (Code/Proximity.h:1014-1069): evaluated on
-        // request from the current elements, in element order
+        // request, in element order, from the elements of the last step: the
+        // current ones, or the previous buffer when a proximity update has
+        // replaced them since (the reference forms the pressures at the end
+        // of Euler_step, before the contact update, Code/Solver.h:86-87)
         int nc = contact_count(*c);
-        ContactArrays& A = c->bufA;
+        ContactArrays& A = c->prox_after_step ? c->bufB : c->bufA;
+        if (c->prox_after_step) {
+            HIP_OK(hipMemcpyAsync(&nc, c->d.c_count_old, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIP_OK(hipStreamSynchronize(c->stream));
+        }
         std::vector<int> bS, brS, jS, bM, brM, seg;
         std::vector<double> fx, fy, len, N1, N2;
         c->download(bS, A.bodyS, nc); c->download(brS, A.borderS, nc); c->download(jS, A.bnodeS, nc);
@@ -2065,6 +2274,38 @@ mel_status mel_write_dynamic(mel_ctx* c, const char* path) {
             for (int k = 0; k < 12; k++) c->download(*f[k], node_field_ptr(*c, fld[k]) + c->h_node_off[b], n);
             double r[26], w[7];
             mel_get_body_state(c, b, r, w);
+            if (!B.rigid) {
+                // internal, contact, self-contact (part of contact here), body,
+                // Dirichlet, Neumann, damping, total, x then y each
+                const int ff[16] = {MEL_X_INTERNAL_FORCE, MEL_Y_INTERNAL_FORCE, MEL_X_CONTACT_FORCE, MEL_Y_CONTACT_FORCE,
+                                    -1, -1, MEL_X_BODY_FORCE, MEL_Y_BODY_FORCE, MEL_X_DIRICHLET_FORCE,
+                                    MEL_Y_DIRICHLET_FORCE, MEL_X_NEUMANN_FORCE, MEL_Y_NEUMANN_FORCE,
+                                    MEL_X_DAMPING_FORCE, MEL_Y_DAMPING_FORCE, MEL_X_FORCE, MEL_Y_FORCE};
+                B.forces_out.assign(16 * n, 0.);
+                std::vector<double> col;
+                for (int k = 0; k < 16; k++) {
+                    if (ff[k] < 0) continue;
+                    c->download(col, node_field_ptr(*c, ff[k]) + c->h_node_off[b], n);
+                    for (size_t i = 0; i < n; i++) B.forces_out[16 * i + k] = col[i];
+                }
+            } else {
+                std::vector<double> R;
+                c->download(R, c->d.body_rigid + (size_t)b * RIGID_FIELDS, RIGID_FIELDS);
+                const int slot[6] = {R_FCON, R_FBODY, R_FDIR, R_FNEU, R_FDAMP, R_FTOT};
+                for (int q = 0; q < 6; q++)
+                    for (int k = 0; k < 3; k++) B.rigid_forces_out[3 * q + k] = R[slot[q] + k];
+            }
+            for (size_t j = 0; j < B.borders.size(); j++) {
+                Border& Br = B.borders[j];
+                const size_t ns = Br.nodes.size();
+                const int gb = c->h_border_off[b] + (int)j;
+                Br.length_out.resize(ns); Br.xcp_out.resize(ns); Br.ycp_out.resize(ns);
+                mel_get_border_field(c, b, (int)j, 0, Br.length_out.data(), ns);
+                mel_get_border_field(c, b, (int)j, 1, Br.xcp_out.data(), ns);
+                mel_get_border_field(c, b, (int)j, 2, Br.ycp_out.data(), ns);
+                c->download(Br.xbp_out, c->d.sl_xbp + c->h_br_slot0[gb], ns);
+                c->download(Br.ybp_out, c->d.sl_ybp + c->h_br_slot0[gb], ns);
+            }
             for (int q = 0; q < 3; q++) { B.rigid_cur[q] = r[q]; B.rigid_disp[q] = r[3 + q]; B.rigid_vel[q] = r[6 + q]; }
             for (int q = 0; q < 7; q++) B.works[q] = w[q];
             size_t cnt = 0;

@@ -813,11 +813,38 @@ bool write_dynamic_text(const std::string& path, const Model& m, std::string& er
             for (int q = 0; q < 3; q++) g(b.rigid_vel[q]);
             std::fprintf(f, "\n");
         }
+        // FORCES (Code/IO.h:2455-2500)
+        std::fprintf(f, " \nFORCES\n");
+        if (!b.rigid) {
+            for (int i = 0; i < b.n_nodes; i++) {
+                std::fprintf(f, "%d ", i);
+                for (int k = 0; k < 16; k++)
+                    g(b.forces_out.size() == (size_t)16 * b.n_nodes ? b.forces_out[16 * (size_t)i + k] : 0.);
+                std::fprintf(f, "\n");
+            }
+        } else {
+            for (int r = 0; r < 6; r++) {
+                for (int q = 0; q < 3; q++) g(b.rigid_forces_out[3 * r + q]);
+                std::fprintf(f, "\n");
+            }
+        }
         std::fprintf(f, " \nWORKS\n");
         for (int k = 0; k < 7; k++) g(b.works[k]);
         std::fprintf(f, "\n \nNEIGHBOURS\n%zu\n", b.neighbours.size() / 3);
         for (size_t k = 0; k < b.neighbours.size() / 3; k++)
             std::fprintf(f, "%d %d %d\n", b.neighbours[3 * k], b.neighbours[3 * k + 1], b.neighbours[3 * k + 2]);
+        // BORDERS: lengths and segment stencils (Code/IO.h:2530-2552)
+        std::fprintf(f, " \nBORDERS\n \n");
+        for (size_t j = 0; j < b.borders.size(); j++) {
+            const Border& B = b.borders[j];
+            std::fprintf(f, "%zu\n%zu\n", j, B.nodes.size());
+            for (size_t k = 0; k < B.nodes.size(); k++) {
+                g(B.length_out.size() == B.nodes.size() ? B.length_out[k] : B.initial_length[k]);
+                std::fprintf(f, "%d %d %d %d %d %d %d %d\n", B.node0[k], B.shift0[k], B.node1[k], B.shift1[k],
+                             B.node2[k], B.shift2[k], B.node3[k], B.shift3[k]);
+            }
+            std::fprintf(f, " \n");
+        }
         std::fprintf(f, " \nCONTACTS\n%zu\n", b.contacts.size());
         for (const ContactElement& c : b.contacts) {
             std::fprintf(f, "%d %d %d %d %d %d %d ", c.borderS, c.border_nodeS, c.nodeS, c.bodyM, c.borderM, c.shiftM,
@@ -829,6 +856,23 @@ bool write_dynamic_text(const std::string& path, const Model& m, std::string& er
             std::fprintf(f, "%d ", c.nb_internal);
             for (int k = 0; k < c.nb_internal; k++) g(c.internal[k]);
             std::fprintf(f, "\n");
+        }
+        // CONTACT_PRESSURES and BC_PRESSURES per slot (Code/IO.h:2600-2630)
+        const char* sec[2] = {"CONTACT_PRESSURES", "BC_PRESSURES"};
+        for (int q = 0; q < 2; q++) {
+            std::fprintf(f, " \n%s\n \n", sec[q]);
+            for (size_t j = 0; j < b.borders.size(); j++) {
+                const Border& B = b.borders[j];
+                const std::vector<double>& px = q == 0 ? B.xcp_out : B.xbp_out;
+                const std::vector<double>& py = q == 0 ? B.ycp_out : B.ybp_out;
+                std::fprintf(f, "%zu\n%zu\n", j, B.nodes.size());
+                for (size_t k = 0; k < B.nodes.size(); k++) {
+                    g(px.size() == B.nodes.size() ? px[k] : 0.);
+                    g(py.size() == B.nodes.size() ? py[k] : 0.);
+                    std::fprintf(f, "\n");
+                }
+                std::fprintf(f, " \n");
+            }
         }
         std::fprintf(f, " \n");
     }

@@ -49,6 +49,9 @@ struct Border {
     // four-node stencils (Body::Initialize_segments)
     std::vector<int> node0, node1, node2, node3, shift0, shift1, shift2, shift3;
     std::vector<double> initial_length;
+    // state written by Write_dynamic (Code/IO.h:2530-2630): current tributary
+    // lengths, contact and BC pressures per slot
+    std::vector<double> length_out, xcp_out, ycp_out, xbp_out, ybp_out;
 };
 
 struct Material {
@@ -101,6 +104,11 @@ struct Body {
     std::vector<double> ux, uy, vx, vy, ax, ay, upx, upy, vpx, vpy, apx, apy;
     double rigid_cur[3] = {0, 0, 0}, rigid_disp[3] = {0, 0, 0}, rigid_vel[3] = {0, 0, 0};
     double works[7] = {0, 0, 0, 0, 0, 0, 0};
+    // force fields written by Write_dynamic (Code/IO.h:2455-2500): deformable,
+    // 16 per node in the file's column order; rigid, the contact, body,
+    // Dirichlet, Neumann, damping and total resultants (x, y, moment each)
+    std::vector<double> forces_out;
+    double rigid_forces_out[18] = {0};
     std::vector<int> neighbours;             // 3 per neighbour
     std::vector<ContactElement> contacts;
 };

@@ -82,6 +82,9 @@ def library():
             "mel_get_body_energy": ([vp, ctypes.c_int, P(ctypes.c_double)], ctypes.c_int),
             "mel_get_scalars": ([vp, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double)], ctypes.c_int),
             "mel_eval_forces": ([vp, ctypes.c_uint], ctypes.c_int),
+            "mel_get_state_size": ([vp, P(ctypes.c_size_t)], ctypes.c_int),
+            "mel_save_state": ([vp, vp, ctypes.c_size_t, P(ctypes.c_size_t)], ctypes.c_int),
+            "mel_load_state": ([vp, vp, ctypes.c_size_t], ctypes.c_int),
             "mel_eval_forces_noalloc": ([vp, ctypes.c_uint], ctypes.c_int),
             "mel_get_info": ([vp, P(Info)], ctypes.c_int),
             "mel_write_dynamic": ([vp, ctypes.c_char_p], ctypes.c_int),
@@ -119,7 +122,7 @@ def exported_symbols():
     return ["mel_load", "mel_load_binary", "mel_update_proximity", "mel_euler_steps", "mel_synchronize",
             "mel_get_node_field", "mel_set_node_field", "mel_get_contacts", "mel_get_neighbours",
             "mel_get_body_state", "mel_get_body_energy", "mel_get_scalars", "mel_eval_forces",
-            "mel_eval_forces_noalloc",
+            "mel_eval_forces_noalloc", "mel_get_state_size", "mel_save_state", "mel_load_state",
             "mel_get_border_field", "mel_get_info", "mel_write_dynamic", "mel_get_stream",
             "mel_set_kernel_timing", "mel_get_kernel_timing", "mel_kernel_count", "mel_kernel_name",
             "mel_destroy", "mel_last_error", "mel_version", "mel_get_device_bytes", "mel_set_graph_replay",
@@ -271,6 +274,20 @@ class Context:
     def eval_forces(self, what=EVAL_ALL):
         """One force evaluation at the current state (mel_eval_forces)."""
         self._check(self.lib.mel_eval_forces(self._h, int(what)))
+
+    def save_state(self):
+        """The dynamic state as bytes (mel_save_state)."""
+        n = ctypes.c_size_t()
+        self._check(self.lib.mel_get_state_size(self._h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        w = ctypes.c_size_t()
+        self._check(self.lib.mel_save_state(self._h, buf, n.value, ctypes.byref(w)))
+        return buf.raw[:w.value]
+
+    def load_state(self, blob):
+        """Restore a state saved from a context of the same model (mel_load_state)."""
+        buf = ctypes.create_string_buffer(bytes(blob), len(blob))
+        self._check(self.lib.mel_load_state(self._h, buf, len(blob)))
 
     def eval_forces_noalloc(self, what=EVAL_ALL):
         """mel_eval_forces without device allocation (mel_eval_forces_noalloc)."""

@@ -1435,6 +1435,45 @@ int run(Sim& S, int nsteps, const char* out_dir, const int* dump_steps, int n_du
     return 0;
 }
 
+// the oracle's state written in the DYNAMIC_nnnnn layout by the library's
+// writer (Write_dynamic, Code/IO.h:2351-2636)
+bool write_state(Sim& S, const std::string& path, std::string& err) {
+    Model m = S.m;
+    m.time = S.time;
+    m.number_iteration = S.iteration;
+    m.next_contact_update = S.next_contact_update;
+    for (size_t bi = 0; bi < m.bodies.size(); bi++) {
+        Body& b = m.bodies[bi];
+        const State& s = S.s[bi];
+        b.ux = s.ux; b.uy = s.uy; b.vx = s.vx; b.vy = s.vy; b.ax = s.ax; b.ay = s.ay;
+        b.upx = s.upx; b.upy = s.upy; b.vpx = s.vpx; b.vpy = s.vpy; b.apx = s.apx; b.apy = s.apy;
+        for (int q = 0; q < 3; q++) { b.rigid_cur[q] = s.cur[q]; b.rigid_disp[q] = s.disp[q]; b.rigid_vel[q] = s.vel[q]; }
+        for (int q = 0; q < 7; q++) b.works[q] = s.works[q];
+        if (!b.rigid) {
+            b.forces_out.assign(16 * (size_t)b.n_nodes, 0.);
+            for (int i = 0; i < b.n_nodes; i++) {
+                const double v[16] = {s.fix[i], s.fiy[i], s.fcx[i], s.fcy[i], 0., 0., s.fbx[i], s.fby[i],
+                                      s.fdx[i], s.fdy[i], s.fnx[i], s.fny[i], s.fkx[i], s.fky[i], s.fx[i], s.fy[i]};
+                for (int k = 0; k < 16; k++) b.forces_out[16 * (size_t)i + k] = v[k];
+            }
+        } else {
+            const double* r[6] = {s.rfc, s.rfb, s.rfd, s.rfn, s.rfk, s.rf};
+            for (int q = 0; q < 6; q++)
+                for (int k = 0; k < 3; k++) b.rigid_forces_out[3 * q + k] = r[q][k];
+        }
+        b.neighbours = s.neighbours;
+        b.contacts = s.contacts;
+        for (size_t j = 0; j < b.borders.size(); j++) {
+            b.borders[j].length_out = s.length[j];
+            b.borders[j].xcp_out = s.xcp[j];
+            b.borders[j].ycp_out = s.ycp[j];
+            b.borders[j].xbp_out = s.xbp[j];
+            b.borders[j].ybp_out = s.ybp[j];
+        }
+    }
+    return write_dynamic_text(path, m, err);
+}
+
 thread_local std::string g_err;
 
 }  // namespace
@@ -1462,6 +1501,20 @@ int oracle_run(const char* path, int save_index, int nsteps, const char* out_dir
         if (S.m.flags[F_UPDATE_STIFFNESS] && !b.mat_dof.empty()) { g_err = "UPDATE_STIFFNESS_MATRIX is not supported"; return 1; }
     setup(S);
     return run(S, nsteps, out_dir, dump_steps, n_dump, seconds);
+}
+
+// Load and initialise as oracle_run, run `nsteps` steps, then write the state
+// to `dynamic_path` in the DYNAMIC_nnnnn layout.  Returns 0 on success.
+int oracle_write_after(const char* path, int save_index, int nsteps, const char* dynamic_path, int nthreads) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    Sim S;
+    std::string err;
+    bool ok = save_index >= 0 ? load_model_files(path, save_index, S.m, err) : load_model_binary(path, S.m, err);
+    if (!ok) { g_err = err; return 1; }
+    setup(S);
+    run(S, nsteps, nullptr, nullptr, 0, nullptr);
+    if (!write_state(S, dynamic_path, err)) { g_err = err; return 1; }
+    return 0;
 }
 
 }  // extern "C"

@@ -9,7 +9,8 @@
 // Code/main.cpp:191-196.  No graphics, monitoring or saves are run; they do
 // not feed back into the step.
 //
-// usage: melody_ref <model_dir> <nsteps> <out_dir> <dump_steps,comma,list>
+// usage: melody_ref <model_dir> <nsteps> <out_dir> <dump_steps,comma,list> [save_index]
+// (save_index: the DYNAMIC_<save_index>.asc file to start from, default 0)
 #include <iostream>
 #include <fstream>
 #include <iomanip>
@@ -208,7 +209,7 @@ int main(int argc, char** argv) {
     double Max_mass_scaling, Control_parameter_mass_scaling, Error_factor_mass_scaling, Decrease_factor_mass_scaling;
     double Save_period, Print_period, Contact_update_period;
     double Next_save, Next_print, Next_contact_update;
-    int Number_save(0), Number_print, Number_iteration;
+    int Number_save(argc > 5 ? atoi(argv[5]) : 0), Number_print, Number_iteration;
     double Xmin_period, Xmax_period, Penalty = 0;
     double Xgravity, Ygravity;
     int Activate_plot;

@@ -12,7 +12,14 @@ sys.path.insert(0, HERE)
 GOLDEN = os.path.join(HERE, "golden")
 # golden sets that carry their model files; c1.tar.gz carries the
 # reference's dumps and the hash of a model the generator rebuilds
-VARIANTS = sorted(f[:-7] for f in os.listdir(GOLDEN) if f.endswith(".tar.gz") and f != "c1.tar.gz")
+VARIANTS = sorted(f[:-7] for f in os.listdir(GOLDEN)
+                  if f.endswith(".tar.gz") and f != "c1.tar.gz" and not f.endswith("_restart.tar.gz"))
+RESTARTS = sorted(f[:-15] for f in os.listdir(GOLDEN) if f.endswith("_restart.tar.gz"))
+# records a restart does not carry over at its step 0: the force fields and
+# pressures Load_dynamic reads back (Code/IO.h:2055-2130, 2292-2340; the
+# internal force is then zeroed by Node::Sum_up_forces, Code/Node.h:212-225)
+# only feed outputs, never the next step
+RESTART_STEP0_SKIP = ("_force", "contact_pressure", ".energy", ".rigid")
 
 
 def pytest_configure(config):
@@ -39,6 +46,21 @@ def extract_golden(name, dest):
         if model_hash(model) != want:
             pytest.skip("the generator did not rebuild the c1 model bit for bit on this host")
     return model, os.path.join(dest, "ref")
+
+
+def extract_restart(name, dest):
+    """The model files of golden `name` with the restart file DYNAMIC_00001.asc
+    the oracle wrote after 30 steps under dest/model, and the reference's
+    dumps of the restart under dest/ref."""
+    model, _ = extract_golden(name, os.path.join(dest, "base"))
+    with tarfile.open(os.path.join(GOLDEN, name + "_restart.tar.gz"), "r:gz") as tf:
+        for m in tf.getmembers():
+            if not m.isfile() or m.name.startswith("/") or ".." in m.name:
+                raise ValueError("unexpected member %s" % m.name)
+        tf.extractall(os.path.join(dest, "r"))
+    import shutil
+    shutil.copy(os.path.join(dest, "r", "DYNAMIC_00001.asc"), os.path.join(model, "DYNAMIC_00001.asc"))
+    return model, os.path.join(dest, "r", "ref")
 
 
 def golden_steps(ref_dir):

@@ -15,6 +15,12 @@ as model files: c1.tar.gz holds only the reference's dumps at steps 0, 1,
 files the generator writes, which the tests regenerate and check before
 use.  Only data is stored.
 
+Restart sets <variant>_restart.tar.gz: the oracle runs the variant for 30
+steps and writes its state with the library's DYNAMIC writer
+(DYNAMIC_00001.asc); the reference restarts from that file (Load_dynamic
+and the initialisation block) and its dumps at steps 0, 1, 10 and 50 are
+stored beside the file.
+
 Run from the repository root:  python tests/golden/make_golden.py [name,...]
 """
 import hashlib
@@ -37,6 +43,8 @@ C1_STEPS = [0, 1, 10, 100, 1000]
 C1_FIELDS = ("x_current,y_current,x_disp_param,y_disp_param,x_vel_param,y_vel_param,x_internal_force,"
              "y_internal_force,x_contact_force,y_contact_force,x_force,y_force,rigid,works,energy,neighbours,"
              "contacts_i,contacts_d,length")
+RESTART = ["tiny_czm", "tiny_bc", "tiny_holed", "tiny_periodic"]
+RESTART_STEPS = [0, 1, 10, 50]
 REF = os.path.join(ROOT, "oracle", "_ref", "melody_ref")
 
 
@@ -49,9 +57,31 @@ def model_hash(mdir):
     return h.hexdigest()
 
 
-def run_ref(mdir, odir, steps, env=None):
-    subprocess.run([REF, mdir, str(max(steps)), odir, ",".join(map(str, steps))], check=True,
+def run_ref(mdir, odir, steps, env=None, save_index=0):
+    subprocess.run([REF, mdir, str(max(steps)), odir, ",".join(map(str, steps)), str(save_index)], check=True,
                    stdout=subprocess.DEVNULL, env=env)
+
+
+def make_restart(name, tmp):
+    import ctypes
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from mel_compare import oracle
+    mdir = os.path.join(tmp, "model")
+    odir = os.path.join(tmp, "ref")
+    os.makedirs(odir)
+    write_model(variants()[name], mdir)
+    lib = oracle()
+    lib.oracle_write_after.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    dyn = os.path.join(mdir, "DYNAMIC_00001.asc")
+    if lib.oracle_write_after(os.fsencode(mdir), 0, 30, os.fsencode(dyn), 0) != 0:
+        raise RuntimeError(lib.oracle_last_error().decode())
+    run_ref(mdir, odir, RESTART_STEPS, save_index=1)
+    out = os.path.join(HERE, name + "_restart.tar.gz")
+    with tarfile.open(out, "w:gz", compresslevel=9) as tf:
+        tf.add(dyn, arcname="DYNAMIC_00001.asc")
+        for f in sorted(os.listdir(odir)):
+            tf.add(os.path.join(odir, f), arcname="ref/" + f)
+    print(name + "_restart", os.path.getsize(out))
 
 
 def main(names=None):
@@ -59,8 +89,11 @@ def main(names=None):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     todo = dict(variants())
     todo["c1"] = None
-    for name in (names or list(todo)):
+    for name in (names or list(todo) + [r + "_restart" for r in RESTART]):
         with tempfile.TemporaryDirectory() as tmp:
+            if name.endswith("_restart"):
+                make_restart(name[:-len("_restart")], tmp)
+                continue
             mdir = os.path.join(tmp, "model")
             odir = os.path.join(tmp, "ref")
             os.makedirs(odir)
