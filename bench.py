@@ -1,20 +1,29 @@
 """Benchmark of the MELODY 2D explicit step on one MI355X.
 
-Workload: biaxial compression of 1000 polydisperse deformable grains
-(~200 MLS nodes each, ~209k nodes, ~1.04M Gauss points, neo-Hookean,
-penalty Mohr-Coulomb mu = 0.5, rigid walls, density x100), generated with
-fixed seeds on the host and handed to the library as a binary model
-container.  A "step" is one Euler_step (internal forces, contact
-re-projection and laws, BCs, integration, kinematics) plus the contact-update
-cadence (a proximity update every 10 steps), all on the device.
+Headline workload: biaxial compression of 1000 polydisperse
+deformable grains (~200 MLS nodes each, ~208k nodes, ~1.04M Gauss points,
+neo-Hookean, penalty Mohr-Coulomb mu = 0.5, rigid walls, density x100)
+deposited into a pile, so ~1.8e4 contact elements act from the start;
+generated with fixed seeds on the host and handed to the library as a
+binary model container.  A "step" is one Euler_step (internal forces,
+contact re-projection and laws, BCs, integration, kinematics) plus the
+contact-update cadence (a proximity update every 10 steps), all on the
+device, with the state resident in HBM.
 
-Printed: one JSON line with whole-job node-steps/s, the roofline of the
-dominant kernel (algorithmic bytes over its HIP-event time in the timed
-region), and the CPU baseline (the FP64 OpenMP oracle on a bounded sample
-of the same workload, on this host's cores).
+Printed: one JSON line with whole-job node-steps/s (FP64 drop-in), the
+roofline of the dominant kernel (algorithmic bytes over its HIP-event time
+in the timed region) and the CPU baseline (the FP64 OpenMP oracle on a
+bounded sample of the same workload, on this host's cores), and beside them:
+  fp32_mixed  the same workload in the FP32-mixed form, with its force,
+              displacement and energy errors against FP64 from one state;
+  plate       the perforated 1M-node RPIM plate: internal forces
+              of one large body, FP64 and FP32-mixed, roofline, CPU baseline;
+  serving     Poisson arrivals of plate jobs through the job
+              server, p50/p99 latency and aggregate node-steps/s.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import tempfile
@@ -24,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FORCE_BAR = 1e-5
 
 
 def log(*a):
@@ -60,12 +70,12 @@ def algorithmic_bytes(kernel, info, fp32=False):
     return None
 
 
-def measured_traffic(kernel, info, fp32=False):
+def measured_traffic(kernel, info, tag, fp32=False):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary, used only when it was measured on this workload with the
-    current kernel sources; otherwise None."""
+    summary of workload `tag`, used only when it was measured on this
+    workload with the current kernel sources; otherwise None."""
     import hashlib
-    path = os.path.join(ROOT, "profiles", "%s%s_1000grains.json" % (kernel, "32" if fp32 else ""))
+    path = os.path.join(ROOT, "profiles", "%s%s_%s.json" % (kernel, "32" if fp32 else "", tag))
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
@@ -77,11 +87,10 @@ def measured_traffic(kernel, info, fp32=False):
     if rec.get("src_sha256_16") != h.hexdigest()[:16] or rec.get("nodes") != info.n_nodes or \
             rec.get("support_entries") != info.nnz_gauss:
         return None, None
-    return rec["traffic_bytes"], "profiles/step_kernels_1000grains.md"
+    return rec["traffic_bytes"], rec.get("summary", "profiles/")
 
 
 def cpu_baseline(model_path, n_nodes, steps, threads):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
     import ctypes
     lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
     lib.oracle_run.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
@@ -95,30 +104,269 @@ def cpu_baseline(model_path, n_nodes, steps, threads):
     return n_nodes * steps / sec.value, sec.value
 
 
-def serve_leg(args, native, tmpdir):
-    """Continuous batching through the job server: Poisson arrivals of
-    independent granular jobs of log-uniform size, half with a cohesive
-    contact law, admitted under an HBM budget and run concurrently; the same
-    arrival schedule in FP64 and in the FP32-mixed form.  Latency is
-    completion minus arrival (queueing, loading and stepping)."""
-    import math
+def timed_run(native, path, precision, args, tag, prepare=None):
+    """Load, warm up, then two windows of args.steps steps: the value window
+    (steps as a user runs them) and the launch-duration window (HIP events
+    around the dominant kernels on the library's stream)."""
+    name = "fp64" if precision == native.FP64 else "fp32-mixed"
+    t0 = time.time()
+    ctx = native.Context(path, binary=True, precision=precision)
+    info = ctx.get_info()
+    log("[%s %s] loaded in %.1fs: nodes %d, gauss %d, nnz %d, contacts %d" % (
+        tag, name, time.time() - t0, info.n_nodes, info.n_gauss, info.nnz_gauss, info.n_contacts))
+    if prepare:
+        prepare(ctx)
+    ctx.euler_steps(args.warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.euler_steps(args.steps)
+    ctx.synchronize()
+    sec = time.perf_counter() - t0
+    info = ctx.get_info()
+    value = info.n_nodes * args.steps / sec
+    log("[%s %s] timed %d steps in %.3fs: %.3e node-steps/s, %.4f ms/step" % (
+        tag, name, args.steps, sec, value, 1e3 * sec / args.steps))
+    ctx.set_kernel_timing(["step_forces", "step_finish", "block_force_update"])
+    ctx.synchronize()
+    ctx.euler_steps(args.steps)
+    ctx.synchronize()
+    kt = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    roof = None
+    dominant = next((k for k in ("step_forces", "block_force_update") if kt[k][1]), None)
+    if dominant:
+        ms, n = kt[dominant]
+        avg_s = ms / n * 1e-3
+        fp32 = precision != native.FP64
+        b = algorithmic_bytes(dominant, info, fp32=fp32)
+        achieved = b / avg_s / 1e9
+        traffic, src = measured_traffic(dominant, info, tag, fp32=fp32)
+        roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes": b}
+        if src:
+            roof["traffic_source"] = src
+        log("[%s %s] kernel %s %.2f us/launch, %.1f GB/s (%.3f of peak)" % (
+            tag, name, dominant, avg_s * 1e6, achieved, achieved / HBM_PEAK_GBS))
+    breakdown = None
+    if args.breakdown:
+        ctx.set_kernel_timing(True)
+        ctx.euler_steps(min(args.steps, 50))
+        kt2 = ctx.kernel_timing()
+        ctx.set_kernel_timing(False)
+        breakdown = {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in kt2.items() if v[1]}
+        log("[%s %s] per-launch us: %s" % (tag, name, json.dumps(breakdown)))
+    return ctx, info, value, sec, roof, breakdown
+
+
+def _norm_err(native, a, b, counts, rigid, fields):
+    """||f_b - f_a|| / ||f_a|| over the deformable bodies' nodes."""
+    num = den = 0.0
+    for bi, n in enumerate(counts):
+        if rigid[bi]:
+            continue
+        for f in fields:
+            x = a.node_field(bi, f, n)
+            d = b.node_field(bi, f, n) - x
+            num += float(d @ d)
+            den += float(x @ x)
+    return (num / den) ** 0.5 if den > 0 else None
+
+
+def fp32_errors(native, ctx64, ctx32, counts, rigid, steps_after):
+    """Errors of the FP32-mixed form against FP64 from one FP64 state (the
+    FP64 context's, loaded into the FP32 context): internal and contact
+    forces of one evaluation (mel_eval_forces), then displacements (the
+    increment of the displacement parameters) and kinetic energy after
+    `steps_after` steps of both forms."""
+    ctx32.load_state(ctx64.save_state())
+    out = {}
+    ctx64.eval_forces(native.EVAL_INTERNAL)
+    ctx32.eval_forces(native.EVAL_INTERNAL)
+    out["rel_internal_force_err"] = _norm_err(native, ctx64, ctx32, counts, rigid,
+                                              ("x_internal_force", "y_internal_force"))
+    ctx64.eval_forces(native.EVAL_CONTACT)
+    ctx32.eval_forces(native.EVAL_CONTACT)
+    out["rel_contact_force_err"] = _norm_err(native, ctx64, ctx32, counts, rigid,
+                                             ("x_contact_force", "y_contact_force"))
+    u0 = {(bi, f): ctx64.node_field(bi, f, n) for bi, n in enumerate(counts) if not rigid[bi]
+          for f in ("x_disp_param", "y_disp_param")}
+    _, w0, e0 = ctx64.scalars()
+    ctx64.euler_steps(steps_after)
+    ctx32.euler_steps(steps_after)
+    num = den = 0.0
+    for (bi, f), u in u0.items():
+        d64 = ctx64.node_field(bi, f, counts[bi]) - u
+        d32 = ctx32.node_field(bi, f, counts[bi]) - u
+        num += float((d32 - d64) @ (d32 - d64))
+        den += float(d64 @ d64)
+    out["rel_displacement_err"] = (num / den) ** 0.5 if den > 0 else None
+    _, w64, e64 = ctx64.scalars()
+    _, w32, e32 = ctx32.scalars()
+    out["rel_kinetic_energy_err"] = abs(e32[1] - e64[1]) / e64[1] if e64[1] > 0 else None
+    dw64 = float(sum(w64 - w0))
+    out["rel_work_err"] = abs(float(sum(w32 - w0)) - dw64) / abs(dw64) if dw64 else None
+    out["steps_after"] = steps_after
+    out["meets_force_bar"] = bool(out["rel_internal_force_err"] is not None and
+                                  out["rel_internal_force_err"] <= FORCE_BAR and
+                                  (out["rel_contact_force_err"] is None or out["rel_contact_force_err"] <= FORCE_BAR))
+    return out
+
+
+def granular_leg(args, native, tmpdir):
+    from melody2d_amd.preprocessor import c2
+    from melody2d_amd.preprocessor.binary import write_binary
+    t0 = time.time()
+    model = c2(n_grains=args.grains)
+    log("[granular] generated %d bodies, %d nodes, %d Gauss points in %.1fs" % (
+        len(model.bodies), model.n_nodes, model.n_gauss, time.time() - t0))
+    path = os.path.join(tmpdir, "c2.bin")
+    write_binary(model, path)
+    tag = "%dgrains" % args.grains
+    ctx64, info, value, sec, roof, bd = timed_run(native, path, native.FP64, args, tag)
+    ctx32, info32, value32, sec32, roof32, bd32 = timed_run(native, path, native.FP32_MIXED, args, tag)
+    counts = [b.n_nodes for b in model.bodies]
+    rigid = [b.rigid for b in model.bodies]
+    done = args.warmup + 2 * args.steps + (min(args.steps, 50) if args.breakdown else 0)
+    if args.error_step > done:
+        ctx64.euler_steps(args.error_step - done)
+    state_info = ctx64.get_info()
+    t0 = time.time()
+    errs = fp32_errors(native, ctx64, ctx32, counts, rigid, args.error_steps_after)
+    errs["state_step"] = max(args.error_step, done)
+    errs["state_contacts"] = state_info.n_contacts
+    log("[granular] fp32-mixed errors vs fp64 from the step-%d state (%d contacts): %s (%.1fs)" % (
+        errs["state_step"], state_info.n_contacts, json.dumps(errs), time.time() - t0))
+    ctx64.close()
+    ctx32.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        t0 = time.time()
+        v, sc = cpu_baseline(path, info.n_nodes, args.cpu_steps, threads)
+        log("[granular] cpu baseline: %.3e node-steps/s over %d steps (%.1fs stepping, %.1fs total)" % (
+            v, args.cpu_steps, sc, time.time() - t0))
+        cpu = {"value": round(v, 1), "unit": "node-steps/s", "cores": threads, "kind": "port",
+               "sample": "%d Euler steps of the same %d-node workload after its initialisation, "
+                         "FP64 OpenMP oracle" % (args.cpu_steps, info.n_nodes)}
+    os.remove(path)
+    return dict(info=info, value=value, sec=sec, roof=roof, bd=bd, value32=value32, sec32=sec32, roof32=roof32,
+                bd32=bd32, errs=errs, cpu=cpu)
+
+
+def make_models(kind, args, out_dir):
+    """Model containers built in a process of their own: their RPIM systems
+    are solved by torch on the GPU, and torch brings a HIP runtime of its
+    own, which must not share a process with the library's."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--make-models", kind, "--out-dir", out_dir,
+           "--plate-nodes", str(args.plate_nodes), "--serve-jobs", str(args.serve_jobs),
+           "--serve-nodes", str(args.serve_nodes[0]), str(args.serve_nodes[1])]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError("model generation (%s) exited with %d" % (kind, r.returncode))
+    return json.loads(r.stdout.decode().strip().splitlines()[-1])
+
+
+def make_models_main(args):
     import numpy as np
     from melody2d_amd.preprocessor import configs
     from melody2d_amd.preprocessor.binary import write_binary
-    rng = np.random.default_rng(4242)
-    lo, hi = args.serve_grains
+    out = []
     t0 = time.time()
-    jobs = []
-    for k in range(args.serve_jobs):
-        g = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
-        mu = float(rng.uniform(0.2, 0.6))
-        m = configs.c2(n_grains=g, seed=3000 + k, mu=mu)
-        if k % 2:
-            # cohesive: peak traction 1 MPa, linear softening over 10 um
-            m = configs._set_law(m, "CZMlinear", [1e13, 1e6, 1e6 / 1e13 + 1e-5, 0.0, 1.0, 2e-6])
-        p = os.path.join(tmpdir, "job%03d.bin" % k)
+    if args.make_models == "plate":
+        m = configs.c3(n_nodes=args.plate_nodes, device="cuda")
+        p = os.path.join(args.out_dir, "c3.bin")
         write_binary(m, p)
-        jobs.append((p, m.n_nodes))
+        b = m.bodies[0]
+        np.save(os.path.join(args.out_dir, "c3_X.npy"), b.X)
+        out.append({"path": p, "nodes": m.n_nodes, "gauss": m.n_gauss, "holes": len(m.holes), "dt": m.dt,
+                    "nu": m.materials[0][2][4], "X": os.path.join(args.out_dir, "c3_X.npy")})
+    else:
+        rng = np.random.default_rng(4242)
+        lo, hi = args.serve_nodes
+        for k in range(args.serve_jobs):
+            n = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+            m = configs.c5_job(n, seed=3000 + k, cohesive=bool(k % 2), device="cuda")
+            p = os.path.join(args.out_dir, "job%03d.bin" % k)
+            write_binary(m, p)
+            out.append({"path": p, "nodes": m.n_nodes, "bodies": len(m.bodies), "holes": len(m.holes)})
+    log("generated %d %s model(s) in %.1fs" % (len(out), args.make_models, time.time() - t0))
+    print(json.dumps(out), flush=True)
+
+
+def plate_leg(args, native, tmpdir):
+    """the perforated RPIM plate, internal forces of one large
+    body (the whole step is timed: the internal-force launch and a small
+    update), from the uniaxial 10/s field."""
+    import numpy as np
+    t0 = time.time()
+    rec = make_models("plate", args, tmpdir)[0]
+    log("[plate] generated %d nodes, %d Gauss points, %d holes in %.1fs" % (
+        rec["nodes"], rec["gauss"], rec["holes"], time.time() - t0))
+    path = rec["path"]
+    X = np.load(rec["X"])
+    os.remove(rec["X"])
+    rate = 10.0
+    eps = rate * 1000 * rec["dt"]
+    nu = rec["nu"]
+    x, y = X[:, 0], X[:, 1]
+
+    def uniaxial(ctx):
+        for f, v in (("x_disp_param", eps * x), ("y_disp_param", -nu * eps * y),
+                     ("x_vel_param", rate * x), ("y_vel_param", -nu * rate * y)):
+            ctx.set_node_field(0, f, np.ascontiguousarray(v))
+
+    tag = "plate%dk" % (args.plate_nodes // 1000)
+    a = argparse.Namespace(**vars(args))
+    a.steps, a.warmup = args.plate_steps, 5
+    ctx64, info, value, sec, roof, bd = timed_run(native, path, native.FP64, a, tag, uniaxial)
+    ctx32, _, value32, sec32, roof32, bd32 = timed_run(native, path, native.FP32_MIXED, a, tag, uniaxial)
+    uniaxial(ctx64)
+    errs = fp32_errors(native, ctx64, ctx32, [rec["nodes"]], [False], 200)
+    errs.pop("rel_contact_force_err")
+    errs["meets_force_bar"] = bool(errs["rel_internal_force_err"] <= FORCE_BAR)
+    errs["state"] = "uniaxial 10/s field after 1000 steps' strain"
+    log("[plate] fp32-mixed errors vs fp64: %s" % json.dumps(errs))
+    ctx64.close()
+    ctx32.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        t0 = time.time()
+        v, sc = cpu_baseline(path, info.n_nodes, args.plate_cpu_steps, threads)
+        log("[plate] cpu baseline: %.3e node-steps/s over %d steps (%.1fs stepping, %.1fs total)" % (
+            v, args.plate_cpu_steps, sc, time.time() - t0))
+        cpu = {"value": round(v, 1), "unit": "node-steps/s", "cores": threads, "kind": "port",
+               "sample": "%d Euler steps of the same plate after its initialisation, FP64 OpenMP oracle "
+                         "(one body: only its internal forces spread over the threads)" % args.plate_cpu_steps}
+    os.remove(path)
+    out = {"workload": "perforated plate, %d RPIM nodes (30-node supports), %d holes with their own borders, "
+                       "linear elastic E = 70 GPa, uniaxial 10/s field, no contact" % (info.n_nodes, rec["holes"]),
+           "nodes": info.n_nodes, "gauss_points": info.n_gauss, "support_entries": info.nnz_gauss,
+           "value": round(value, 1), "unit": "node-steps/s", "ms_per_step": round(1e3 * sec / a.steps, 4),
+           "dtype": "f64", "roofline": roof, "cpu_baseline": cpu,
+           "fp32_mixed": {"value": round(value32, 1), "ms_per_step": round(1e3 * sec32 / a.steps, 4),
+                          "speedup_vs_fp64": round(value32 / value, 3), "errors_vs_fp64": errs,
+                          "roofline": roof32}}
+    if args.breakdown:
+        out["kernel_us"] = bd
+        out["fp32_mixed"]["kernel_us"] = bd32
+    return out
+
+
+def serve_leg(args, native, tmpdir):
+    """Continuous batching through the job server: Poisson
+    arrivals of perforated-plate jobs of log-uniform size, every second one
+    two plates bonded by a cohesive law, admitted under an HBM budget and
+    run concurrently; the same arrival schedule in FP64 and in the
+    FP32-mixed form.  Latency is completion minus arrival (queueing,
+    loading and stepping)."""
+    import numpy as np
+    lo, hi = args.serve_nodes
+    t0 = time.time()
+    jobs = [(r["path"], r["nodes"]) for r in make_models("serve", args, tmpdir)]
+    rng = np.random.default_rng(4243)
     gaps = rng.exponential(1.0 / args.serve_rate, args.serve_jobs)
     arrivals = np.cumsum(gaps) - gaps[0]
     log("serve: generated %d jobs (%d..%d nodes) in %.1fs" % (
@@ -129,18 +377,18 @@ def serve_leg(args, native, tmpdir):
                            graph_replay=bool(args.serve_graphs)) as srv:
             ids = [srv.submit(p, args.serve_steps, precision=prec, arrival=float(a))
                    for (p, _), a in zip(jobs, arrivals)]
-            left = srv.run(timeout=600.0)
+            left = srv.run(timeout=900.0)
             infos = [srv.job(j) for j in ids]
             failed = [(j, srv.job_error(j)) for j, inf in zip(ids, infos) if inf.state != native.JOB_DONE]
             if left or failed:
                 raise RuntimeError("serve (%s): %d unfinished, failed %s" % (name, left, failed[:3]))
         lat = np.array([inf.finished - inf.arrival for inf in infos])
-        load = np.array([inf.admitted - inf.arrival for inf in infos])
+        wait = np.array([inf.admitted - inf.arrival for inf in infos])
         span = max(inf.finished for inf in infos) - min(inf.arrival for inf in infos)
         work = sum(inf.n_nodes * inf.steps_done for inf in infos)
         out[name] = {"p50_latency_s": round(float(np.percentile(lat, 50)), 4),
                      "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
-                     "p50_wait_s": round(float(np.percentile(load, 50)), 4),
+                     "p50_wait_s": round(float(np.percentile(wait, 50)), 4),
                      "node_steps_per_s": round(work / span, 1), "makespan_s": round(span, 3)}
         log("serve %s: p50 %.3fs p99 %.3fs, %.3e node-steps/s over %.2fs" % (
             name, out[name]["p50_latency_s"], out[name]["p99_latency_s"], work / span, span))
@@ -152,9 +400,9 @@ def serve_leg(args, native, tmpdir):
                 "hbm_budget_gb": args.serve_budget_gb, "steps_per_slice": args.serve_slice,
                 "graph_replay": bool(args.serve_graphs),
                 "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
-                "workload": "Poisson arrivals of biaxial granular jobs, grains log-uniform in [%d, %d] "
-                            "(~210 nodes each), Coulomb mu ~ U[0.2, 0.6], every second job CZMlinear"
-                            % (lo, hi)})
+                "workload": "Poisson arrivals of perforated RPIM plates in uniaxial tension at 10/s, nodes "
+                            "log-uniform in [%d, %d], every second job two plates bonded by CZMlinear "
+                            "(peak 1 MPa, softening over 10 um)" % (lo, hi)})
     return out
 
 
@@ -164,8 +412,8 @@ def run_serve_child(args):
     cannot take the headline figures with it."""
     import subprocess
     cmd = [sys.executable, os.path.abspath(__file__), "--serve-only", "--serve-jobs", str(args.serve_jobs),
-           "--serve-steps", str(args.serve_steps), "--serve-grains", str(args.serve_grains[0]),
-           str(args.serve_grains[1]), "--serve-rate", str(args.serve_rate), "--serve-slice", str(args.serve_slice),
+           "--serve-steps", str(args.serve_steps), "--serve-nodes", str(args.serve_nodes[0]),
+           str(args.serve_nodes[1]), "--serve-rate", str(args.serve_rate), "--serve-slice", str(args.serve_slice),
            "--serve-budget-gb", str(args.serve_budget_gb), "--serve-graphs", str(args.serve_graphs)]
     # more hardware queues let more of the jobs' streams run at once
     env = dict(os.environ, GPU_MAX_HW_QUEUES=str(max(1, min(32, args.serve_hw_queues))))
@@ -186,24 +434,26 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=["granular", "plate"], default="granular",
-                    help="granular: biaxial compression of --grains grains (the headline); plate: the "
-                         "perforated RPIM plate of --plate-nodes nodes, internal forces only")
     ap.add_argument("--grains", type=int, default=1000)
     ap.add_argument("--plate-nodes", type=int, default=1000000)
-    ap.add_argument("--cpu-steps", type=int, default=2000)
+    ap.add_argument("--plate-steps", type=int, default=50)
+    ap.add_argument("--plate-cpu-steps", type=int, default=5)
+    ap.add_argument("--no-plate", action="store_true", help="skip the plate leg")
+    ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="also time every kernel in a separate window")
     ap.add_argument("--error-step", type=int, default=3000,
-                    help="FP64 step at which the FP32-mixed force error is evaluated")
+                    help="FP64 step whose state the FP32-mixed errors are evaluated from")
+    ap.add_argument("--error-steps-after", type=int, default=500,
+                    help="steps both forms run from that state for the displacement and energy errors")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (scale reference for HBM byte counters)")
-    ap.add_argument("--serve-jobs", type=int, default=16, help="jobs of the serving leg (0: skip it)")
-    ap.add_argument("--serve-steps", type=int, default=5000)
-    ap.add_argument("--serve-grains", type=int, nargs=2, default=[20, 200])
-    ap.add_argument("--serve-rate", type=float, default=8.0, help="job arrivals per second")
-    ap.add_argument("--serve-slice", type=int, default=100, help="steps per slice of a running job")
+    ap.add_argument("--serve-jobs", type=int, default=12, help="jobs of the serving leg (0: skip it)")
+    ap.add_argument("--serve-steps", type=int, default=20000)
+    ap.add_argument("--serve-nodes", type=int, nargs=2, default=[5000, 500000])
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrivals per second")
+    ap.add_argument("--serve-slice", type=int, default=200, help="steps per slice of a running job")
     ap.add_argument("--serve-budget-gb", type=float, default=250.0)
     ap.add_argument("--serve-graphs", type=int, default=0,
                     help="1: the jobs replay captured graphs (their launches then overlap less)")
@@ -211,7 +461,13 @@ def main():
                     help="hardware queues of the serving child process (GPU_MAX_HW_QUEUES, at most 32)")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving leg and print its JSON object (the child process of a full run)")
+    ap.add_argument("--make-models", choices=["plate", "serve"],
+                    help="only build the plate or the serving jobs into --out-dir (a child process of a full run)")
+    ap.add_argument("--out-dir")
     args = ap.parse_args()
+    if args.make_models:
+        make_models_main(args)
+        return
     if args.serve_only:
         from melody2d_amd import native
         d = tempfile.mkdtemp(prefix="mel_serve_")
@@ -234,186 +490,55 @@ def main():
         sys.exit(2)
 
     from melody2d_amd import native
-    from melody2d_amd.preprocessor import c2, configs
-    from melody2d_amd.preprocessor.binary import write_binary
-
-    t0 = time.time()
-    plate = args.workload == "plate"
-    if plate:
-        model = configs.c3(n_nodes=args.plate_nodes)
-        args.serve_jobs = 0
-    else:
-        model = c2(n_grains=args.grains)
-    log("generated %d bodies, %d nodes, %d Gauss points in %.1fs" % (len(model.bodies), model.n_nodes,
-                                                                     model.n_gauss, time.time() - t0))
     tmpdir = tempfile.mkdtemp(prefix="mel_bench_")
-    path = os.path.join(tmpdir, "c2.bin")
-    t0 = time.time()
-    write_binary(model, path)
-    log("wrote %.0f MB in %.1fs" % (os.path.getsize(path) / 1e6, time.time() - t0))
-
-    def uniaxial(ctx):
-        # the plate's displacement and velocity parameters: the uniaxial
-        # field of a 10/s strain rate after 1000 steps (RPIM interpolates,
-        # so parameters are displacements)
-        b = model.bodies[0]
-        rate = 10.0
-        eps = rate * 1000 * model.dt
-        x, y = b.X[:, 0], b.X[:, 1]
-        nu = model.materials[0][2][4]
-        for f, v in (("x_disp_param", eps * x), ("y_disp_param", -nu * eps * y),
-                     ("x_vel_param", rate * x), ("y_vel_param", -nu * rate * y)):
-            ctx.set_node_field(0, f, v)
-
-    def run(precision):
-        name = "fp64" if precision == native.FP64 else "fp32-mixed"
-        t0 = time.time()
-        ctx = native.Context(path, binary=True, precision=precision)
-        info = ctx.get_info()
-        log("[%s] loaded in %.1fs: nodes %d, gauss %d, nnz %d, contacts %d" % (
-            name, time.time() - t0, info.n_nodes, info.n_gauss, info.nnz_gauss, info.n_contacts))
-        if plate:
-            uniaxial(ctx)
-        ctx.euler_steps(args.warmup)
-        ctx.synchronize()
-        # value window: the steps as a user runs them (runs between contact
-        # updates replayed from captured graphs)
-        t0 = time.perf_counter()
-        ctx.euler_steps(args.steps)
-        ctx.synchronize()
-        t1 = time.perf_counter()
-        sec = t1 - t0
-        info = ctx.get_info()
-        value = info.n_nodes * args.steps / sec
-        log("[%s] timed %d steps in %.3fs: %.3e node-steps/s, %.3f ms/step" % (
-            name, args.steps, sec, value, 1e3 * sec / args.steps))
-        # launch-duration window: the same number of steps with HIP events
-        # around every launch of the step's dominant kernel (launch A of the
-        # two-launch step, or the internal-force kernel of the launch
-        # sequence), recorded on the library's stream; per-launch events
-        # switch graph replay off, so this window runs launch by launch
-        ctx.set_kernel_timing(["step_forces", "step_finish", "block_force_update"])
-        ctx.synchronize()
-        t2 = time.perf_counter()
-        ctx.euler_steps(args.steps)
-        ctx.synchronize()
-        t3 = time.perf_counter()
-        kt = ctx.kernel_timing()
-        ctx.set_kernel_timing(False)
-        log("[%s] event window: %d steps in %.3fs (%.3f ms/step launch by launch)" % (
-            name, args.steps, t3 - t2, 1e3 * (t3 - t2) / args.steps))
-        roof = None
-        dominant = next(k for k in ("step_forces", "block_force_update") if kt[k][1])
-        ms, n = kt[dominant]
-        if n:
-            avg_s = ms / n * 1e-3
-            b = algorithmic_bytes(dominant, info, fp32=precision != native.FP64)
-            achieved = b / avg_s / 1e9
-            traffic, src = measured_traffic(dominant, info, fp32=precision != native.FP64)
-            roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes": b}
-            if src:
-                roof["traffic_source"] = src
-            log("[%s] kernel %s %.2f us/launch, %.1f GB/s" % (name, dominant, avg_s * 1e6, achieved))
-        breakdown = None
-        if args.breakdown:
-            ctx.set_kernel_timing(True)
-            ctx.euler_steps(min(args.steps, 50))
-            kt2 = ctx.kernel_timing()
-            ctx.set_kernel_timing(False)
-            breakdown = {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in kt2.items() if v[1]}
-            log("[%s] per-launch us: %s" % (name, json.dumps(breakdown)))
-        return ctx, info, value, sec, roof, breakdown
-
-    ctx64, info, value, sec, roof, breakdown = run(native.FP64)
-    ctx32, _, value32, sec32, roof32, breakdown32 = run(native.FP32_MIXED)
-
-    # relative internal-force error of the FP32-mixed form, from the FP64
-    # state: the FP64 run continues to step --error-step (until then the
-    # walls have not reached the grains, whose internal forces are pure
-    # rounding residue of the reference formula), then both forms take one
-    # step from the FP64 displacement parameters
-    t0 = time.time()
-    done = args.warmup + 2 * args.steps + (min(args.steps, 50) if args.breakdown else 0)
-    if args.error_step > done:
-        ctx64.euler_steps(args.error_step - done)
-    counts = [b.n_nodes for b in model.bodies]
-    rigid = [b.rigid for b in model.bodies]
-    fields = ["x_disp_param", "y_disp_param", "x_vel_param", "y_vel_param"]
-    for bi, n in enumerate(counts):
-        if rigid[bi]:
-            continue
-        for f in fields:
-            ctx32.set_node_field(bi, f, ctx64.node_field(bi, f, n))
-    ctx64.euler_steps(1)
-    ctx32.euler_steps(1)
-    num = den = 0.0
-    for bi, n in enumerate(counts):
-        if rigid[bi]:
-            continue
-        for f in ("x_internal_force", "y_internal_force"):
-            a = ctx64.node_field(bi, f, n)
-            d = ctx32.node_field(bi, f, n) - a
-            num += float(d @ d)
-            den += float(a @ a)
-    rel_err = (num / den) ** 0.5 if den > 0 else None
-    log("fp32-mixed internal-force error vs fp64 from the same state at step %d: %.3e (%.1fs)" % (
-        max(args.error_step, done), rel_err, time.time() - t0))
-    ctx64.close()
-    ctx32.close()
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        if plate:
-            # one body: only its internal forces spread over the threads
-            args.cpu_steps = min(args.cpu_steps, 10)
-        t0 = time.time()
-        v, sc = cpu_baseline(path, info.n_nodes, args.cpu_steps, threads)
-        log("cpu baseline: %.3e node-steps/s over %d steps (%.1fs stepping, %.1fs total)" % (
-            v, args.cpu_steps, sc, time.time() - t0))
-        cpu = {"value": round(v, 1), "unit": "node-steps/s", "cores": threads, "kind": "port",
-               "sample": "%d Euler steps of the same %d-node workload after its initialisation, "
-                         "FP64 OpenMP oracle" % (args.cpu_steps, info.n_nodes)}
-    os.remove(path)
+    g = granular_leg(args, native, tmpdir)
+    plate = None
+    if not args.no_plate:
+        try:
+            plate = plate_leg(args, native, tmpdir)
+        except Exception as e:   # the headline line is printed whatever happens here
+            log("plate leg failed: %r" % (e,))
+            plate = {"error": repr(e)}
     os.rmdir(tmpdir)
     serving = run_serve_child(args) if args.serve_jobs > 0 else None
 
+    info = g["info"]
+    errs = g["errs"]
     out = {
-        "metric": "node-steps/s, FP32-mixed vs FP64 at \u22641e-5 rel force err; p50/p99 job latency",
-        "value": round(value, 1),
+        "metric": "node-steps/s, FP32-mixed vs FP64 at ≤1e-5 rel force err; p50/p99 job latency",
+        "value": round(g["value"], 1),
         "unit": "node-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * sec / args.steps, 4),
+        "ms_per_step": round(1e3 * g["sec"] / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded generator)",
-        "config": {"workload": ("perforated plate, %d RPIM nodes (30-node supports), %d holes, linear elastic, "
-                                "uniaxial 10/s field, no contact (FP64 drop-in)" % (info.n_nodes, len(model.holes))
-                                if plate else
-                                "biaxial compression, %d deformable grains + 4 rigid walls (FP64 drop-in)" % args.grains),
+        "config": {"workload": "biaxial compression, %d deformable grains deposited into a pile + 4 rigid walls "
+                               "(FP64 drop-in)" % args.grains,
                    "nodes": info.n_nodes, "gauss_points": info.n_gauss, "support_entries": info.nnz_gauss,
                    "contacts": info.n_contacts, "precision": "fp64", "parallelism": "single GPU"},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "fp32_mixed": {"value": round(value32, 1), "unit": "node-steps/s",
-                       "ms_per_step": round(1e3 * sec32 / args.steps, 4),
-                       "speedup_vs_fp64": round(value32 / value, 3),
-                       "rel_internal_force_err_vs_fp64": rel_err,
-                       "error_state_step": max(args.error_step, done),
-                       "dtype": "f32 gradients/stresses/contributions, f64 state and sums",
-                       "roofline": roof32},
+        "roofline": g["roof"],
+        "cpu_baseline": g["cpu"],
+        "fp32_mixed": {"value": round(g["value32"], 1), "unit": "node-steps/s",
+                       "ms_per_step": round(1e3 * g["sec32"] / args.steps, 4),
+                       "speedup_vs_fp64": round(g["value32"] / g["value"], 3),
+                       "meets_force_bar": errs["meets_force_bar"],
+                       "rel_internal_force_err_vs_fp64": errs["rel_internal_force_err"],
+                       "errors_vs_fp64": errs,
+                       "dtype": "f32 gradients/stresses/contributions, f64 state, contacts and sums",
+                       "roofline": g["roof32"]},
     }
+    if plate:
+        out["plate"] = plate
     if serving:
         out["serving"] = serving
-    if breakdown:
-        out["kernel_us"] = breakdown
-        out["fp32_mixed"]["kernel_us"] = breakdown32
+    if g["bd"]:
+        out["kernel_us"] = g["bd"]
+        out["fp32_mixed"]["kernel_us"] = g["bd32"]
     print(json.dumps(out), flush=True)
 
 

@@ -252,10 +252,13 @@ def _rpim_chunk(job):
     return shape.rpim(pts, X, ptr, nn.reshape(-1).astype(np.int64), shape_c=c)
 
 
-def _rpim(points, X, nn, c, workers):
+def _rpim(points, X, nn, c, workers, device=None):
     """RPIM over fixed-size supports (rows of nn), in point chunks on worker
     processes when there are many points (the result does not depend on the
-    worker count: every point is solved on its own)."""
+    worker count: every point is solved on its own), or batched on a torch
+    device when one is named."""
+    if device:
+        return shape.rpim_fixed_torch(points, X, nn, shape_c=c, device=device)
     n = len(points)
     if workers is None or workers <= 1 or n < 200000:
         return _rpim_chunk((points, X, nn, c))
@@ -270,7 +273,7 @@ def _rpim(points, X, nn, c, workers):
 def deformable_body(border_xy, interior_xy, material, h, rho, radius_factor=2.5,
                     detection=None, contact=None, nregions=1, velocity=(0.0, 0.0),
                     keep_triangle=None, kind="MLS", rpim_c=1.0, rpim_support=30, workers=None,
-                    borders=None, periodicity="Simple"):
+                    borders=None, periodicity="Simple", device=None):
     """Assemble a deformable meshfree body.
 
     `borders` lists the body's Border records over the node numbering in
@@ -302,12 +305,12 @@ def deformable_body(border_xy, interior_xy, material, h, rho, radius_factor=2.5,
         nn = np.sort(nn, axis=1)
         gptr = np.arange(len(gp) + 1, dtype=np.int64) * rpim_support
         gidx = nn.reshape(-1).astype(np.int64)
-        phi, phix, phiy = _rpim(gp, X, nn, rpim_c, workers)
+        phi, phix, phiy = _rpim(gp, X, nn, rpim_c, workers, device)
         _, nn2 = tree.query(X, k=rpim_support, workers=-1)
         nn2 = np.sort(nn2, axis=1)
         dptr = np.arange(len(X) + 1, dtype=np.int64) * rpim_support
         didx = nn2.reshape(-1).astype(np.int64)
-        dphi, dphix, dphiy = _rpim(X, X, nn2, rpim_c, workers)
+        dphi, dphix, dphiy = _rpim(X, X, nn2, rpim_c, workers, device)
     # lumped mass: rho * sum_g c0 * phi_j * sum_k phi_k  (row sum of consistent mass)
     c0 = w * jac
     rowsum = np.add.reduceat(phi, gptr[:-1]) if len(phi) else np.zeros(len(gp))

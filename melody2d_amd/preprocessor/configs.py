@@ -330,40 +330,47 @@ def c2(n_grains=1000, seed=1002, nodes_per_grain=200, area_fraction=0.45, mu=0.5
     return m
 
 
-def c3(n_nodes=1_000_000, seed=1003, side=0.1, porosity=0.3, rmin=1.0e-3, rmax=4.0e-3, E=7.0e10, nu=0.33,
-       rho=2700.0, nregions=64, workers=None):
-    """A side x side plate with seeded random circular holes (radius
-    U[rmin, rmax], no overlap, until the hole area reaches `porosity`),
-    ~n_nodes RPIM nodes (30-node supports, Gaussian RBF exp(-(3 d / D)^2)
-    with D the support size, as Preprocessor/Gauss_RBF.m), three Gauss
-    points per triangle, linear elastic.  The internal-force workload: one
-    body, no contact (NO_MONITORING, no contact update within a run).
-
-    Node cloud: the outer rectangle's border nodes plus a jittered grid at
-    the spacing that gives n_nodes over the solid area; triangles whose
-    centroid falls in a hole are dropped, so holes are free surfaces without
-    border nodes.  Masses are lumped uniformly (rho x solid area / N per
-    node) and kept as written (no UPDATE_MASS_MATRIX): the row sums of RPIM
-    shape functions are not all positive."""
-    from .bodies import rectangle_border
-    if workers is None:
-        workers = min(16, os.cpu_count() or 1)
-    rng_holes, rng_nodes = _streams(seed, 2)
+def _holes(rng, x0, y0, x1, y1, porosity, rmin, rmax, margin, clearance, avoid_x=None, max_tries=200000):
+    """Seeded circular holes, radius U[rmin, rmax], at least `margin` from the
+    box edges and `clearance` from each other (and from the line x = avoid_x
+    when given), added until their area reaches `porosity` of the box."""
     holes = []
-    target = porosity * side * side
+    target = porosity * (x1 - x0) * (y1 - y0)
     area = 0.0
     tries = 0
-    while area < target and tries < 200000:
+    while area < target and tries < max_tries:
         tries += 1
-        r = rng_holes.uniform(rmin, rmax)
-        x, y = rng_holes.uniform(r + 0.5e-3, side - r - 0.5e-3, 2)
-        if any((x - a) ** 2 + (y - b) ** 2 < (r + c + 0.2e-3) ** 2 for a, b, c in holes):
+        r = rng.uniform(rmin, rmax)
+        if x1 - x0 <= 2 * (r + margin) or y1 - y0 <= 2 * (r + margin):
+            continue
+        x = rng.uniform(x0 + r + margin, x1 - r - margin)
+        y = rng.uniform(y0 + r + margin, y1 - r - margin)
+        if avoid_x is not None and abs(x - avoid_x) < r + margin:
+            continue
+        if any((x - a) ** 2 + (y - b) ** 2 < (r + c + clearance) ** 2 for a, b, c in holes):
             continue
         holes.append((x, y, r))
         area += math.pi * r * r
-    H = np.array(holes)
-    solid = side * side - area
-    h = math.sqrt(solid / n_nodes)
+    return holes
+
+
+def _plate_body(rng, x0, y0, x1, y1, holes, h, material, rho, nregions=1, workers=None, device=None,
+                hole_borders=True, simple_sides=False, left_offset=0.0):
+    """One perforated rectangular RPIM body (30-node supports, Gaussian RBF
+    exp(-(3 d / D)^2), D the support size, as Preprocessor/Gauss_RBF.m;
+    three Gauss points per triangle).  Borders: the outer rectangle
+    (counter-clockwise; one closed border, or four 'Simple' borders, one per
+    side, with `simple_sides`), then one closed border per hole run
+    clockwise, as Preprocessor/SetUnstructuredNodesForHoledContour.m lays out
+    a holed contour (without `hole_borders` the holes are free surfaces
+    without border nodes).  Interior nodes on a jittered grid of spacing h;
+    triangles whose centroid falls in a hole are dropped.  `left_offset`
+    moves the nodes of the left side (corners excepted) along y by that
+    fraction of their spacing, so that two plates bonded side to side do not
+    put nodes on each other's nodes.  Masses are lumped uniformly (rho x
+    solid area / N): RPIM row sums are not all positive."""
+    from .bodies import Border
+    H = np.array(holes).reshape(-1, 3)
 
     def in_holes(p, grow=0.0):
         inside = np.zeros(len(p), dtype=bool)
@@ -371,16 +378,58 @@ def c3(n_nodes=1_000_000, seed=1003, side=0.1, porosity=0.3, rmin=1.0e-3, rmax=4
             inside |= (p[:, 0] - x) ** 2 + (p[:, 1] - y) ** 2 < (r + grow) ** 2
         return inside
 
-    border = rectangle_border(0.0, 0.0, side, side, h)
-    g = np.arange(h, side - 0.5 * h, h)
-    gx, gy = np.meshgrid(g, g, indexing="ij")
+    outer = rectangle_border(x0, y0, x1, y1, h)
+    corners = rectangle_corners(x0, y0, x1, y1, h)
+    if left_offset:
+        n3 = len(outer) - corners[3]
+        dy = (y1 - y0) / n3
+        outer[corners[3] + 1:, 1] -= left_offset * dy
+    parts = [outer]
+    if hole_borders:
+        for x, y, r in H:
+            n = max(8, int(math.ceil(2 * math.pi * r / h)))
+            th = -2.0 * np.pi * np.arange(n) / n - 0.5 * np.pi
+            parts.append(np.stack([x + r * np.cos(th), y + r * np.sin(th)], axis=1))
+    border = np.concatenate(parts)
+    gxs = np.arange(x0 + h, x1 - 0.5 * h, h)
+    gys = np.arange(y0 + h, y1 - 0.5 * h, h)
+    gx, gy = np.meshgrid(gxs, gys, indexing="ij")
     pts = np.stack([gx.ravel(), gy.ravel()], axis=1)
-    pts += rng_nodes.uniform(-0.25 * h, 0.25 * h, pts.shape)
-    interior = pts[~in_holes(pts, 0.3 * h)]
-    body = deformable_body(border, interior, "Plate", h, rho, kind="RPIM", rpim_c=3.0, rpim_support=30,
+    pts += rng.uniform(-0.25 * h, 0.25 * h, pts.shape)
+    interior = pts[~in_holes(pts, (0.5 if hole_borders else 0.3) * h)]
+    brds = simple_borders(corners, len(outer)) if simple_sides else [closed_border(len(outer))]
+    off = len(outer)
+    for part in parts[1:]:
+        brds.append(closed_border(len(part), off, len(brds)))
+        off += len(part)
+    body = deformable_body(border, interior, material, h, rho, kind="RPIM", rpim_c=3.0, rpim_support=30,
                            keep_triangle=lambda c: ~in_holes(c), nregions=nregions, workers=workers,
-                           detection=0.5 * h, contact=0.5 * h)
+                           detection=0.5 * h, contact=0.5 * h, borders=brds, device=device)
+    solid = (x1 - x0) * (y1 - y0) - sum(math.pi * r * r for _, _, r in H)
     body.mass = np.full((body.n_nodes, 2), rho * solid / body.n_nodes)
+    return body
+
+
+def c3(n_nodes=1_000_000, seed=1003, side=0.1, porosity=0.3, rmin=1.0e-3, rmax=4.0e-3, E=7.0e10, nu=0.33,
+       rho=2700.0, nregions=64, workers=None, device=None, hole_borders=True):
+    """A side x side plate with seeded random circular holes (radius
+    U[rmin, rmax], no overlap, until the hole area reaches `porosity`),
+    ~n_nodes RPIM nodes, linear elastic: one body (_plate_body) whose holes
+    are closed borders of their own.  The internal-force workload: no
+    contact (NO_MONITORING, no contact update within a run).  Masses are
+    kept as written (no UPDATE_MASS_MATRIX).  `device` names a torch device
+    on which the RPIM systems are solved (the 1M-node plate has ~7M)."""
+    if workers is None:
+        workers = min(16, os.cpu_count() or 1)
+    rng_holes, rng_nodes = _streams(seed, 2)
+    holes = _holes(rng_holes, 0.0, 0.0, side, side, porosity, rmin, rmax, 0.5e-3 * side / 0.1,
+                   0.2e-3 * side / 0.1)
+    H = np.array(holes)
+    area = sum(math.pi * r * r for _, _, r in holes)
+    solid = side * side - area
+    h = math.sqrt(solid / n_nodes)
+    body = _plate_body(rng_nodes, 0.0, 0.0, side, side, holes, h, "Plate", rho, nregions=nregions, workers=workers,
+                       device=device, hole_borders=hole_borders)
     m = Model(name="c3_plate_%d" % body.n_nodes, materials=[("Plate", "ElasticLinear", [rho, 0.0, 0.0, E, nu, 1.0])],
               contact_laws=[("Plate", "Plate", "Frictionless", "Evolutive", [1e13])], bodies=[body],
               periodic=(-1.0, 1.0))
@@ -391,6 +440,63 @@ def c3(n_nodes=1_000_000, seed=1003, side=0.1, porosity=0.3, rmin=1.0e-3, rmax=4
     m.contact_update_period = 1.0e6 * m.dt
     m.tend = 2000 * m.dt
     m.holes = H
+    return m
+
+
+def c5_job(n_nodes, seed, cohesive, porosity=0.3, E=7.0e10, nu=0.33, rho=2700.0, strain_rate=10.0,
+           peak=1.0e6, softening=1.0e-5, workers=1, device=None):
+    """One serving job built like the perforated plate: ~n_nodes RPIM nodes
+    at the plate's spacing, holes of radius U[1, 4] mm up to porosity 0.3
+    (the side follows the size, so small plates hold few holes), pulled in uniaxial tension at `strain_rate` by
+    soft Dirichlet springs on its left and right sides (the outer contour is
+    four 'Simple' borders).  A cohesive job is two plates bonded along
+    x = side / 2 by a CZMlinear law (peak traction `peak`, linear softening
+    over `softening`, contacts closer than 2 um start intact under
+    INITIALIZE_CZM: Code/Proximity.h:481-541); holes keep clear of the
+    bond line.  Otherwise one plate, with a frictionless self-law it never
+    uses (NO_MONITORING)."""
+    rng_holes, rng_a, rng_b = _streams(seed, 3)
+    h = 8.4e-5
+    side = math.sqrt(n_nodes * h * h / (1.0 - porosity))
+    holes = _holes(rng_holes, 0.0, 0.0, side, side, porosity, 1.0e-3, 4.0e-3, 0.5e-3, 0.2e-3,
+                   avoid_x=0.5 * side if cohesive else None, max_tries=20000)
+    # spring stiffness per unit length of border: a tenth of E / h
+    k = 0.1 * E / h
+    v = strain_rate * side
+    hold = lambda: BC("Soft", [(0.0, 0.0), (1e3, 0.0)], (k, 0.0))
+    pull = lambda: BC("Soft", [(0.0, v), (1e3, v)], (k, 0.0))
+    if cohesive:
+        mid = 0.5 * side
+        ha = [c for c in holes if c[0] < mid]
+        hb = [c for c in holes if c[0] >= mid]
+        a = _plate_body(rng_a, 0.0, 0.0, mid, side, ha, h, "Plate", rho, workers=workers, device=device,
+                        simple_sides=True)
+        b = _plate_body(rng_b, mid, 0.0, side, side, hb, h, "Plate", rho, workers=workers, device=device,
+                        simple_sides=True, left_offset=0.5)
+        a.borders[3].xbc = hold()
+        b.borders[1].xbc = pull()
+        bodies = [a, b]
+        gapnlim = peak / 1e13 + softening
+        laws = [("Plate", "Plate", "CZMlinear", "Evolutive", [1e13, peak, gapnlim, 0.0, 1.0, 2e-6])]
+    else:
+        a = _plate_body(rng_a, 0.0, 0.0, side, side, holes, h, "Plate", rho, workers=workers, device=device,
+                        simple_sides=True)
+        a.borders[3].xbc = hold()
+        a.borders[1].xbc = pull()
+        bodies = [a]
+        laws = [("Plate", "Plate", "Frictionless", "Evolutive", [1e13])]
+    m = Model(name="c5_plate_%d%s" % (sum(b.n_nodes for b in bodies), "_czm" if cohesive else ""),
+              materials=[("Plate", "ElasticLinear", [rho, 0.0, 0.0, E, nu, 1.0])], contact_laws=laws, bodies=bodies,
+              periodic=(-1.0, 1.0))
+    m.flags = [f for f in m.flags if f != "UPDATE_MASS_MATRIX"]
+    if cohesive:
+        m.flags = m.flags + ["INITIALIZE_CZM", "NO_MONITORING"]
+    else:
+        m.flags = m.flags + ["NO_MONITORING"]
+    m.dt = 0.1 * h / math.sqrt(E / rho)
+    m.contact_update_period = 10 * m.dt
+    m.tend = 20000 * m.dt
+    m.holes = np.array(holes).reshape(-1, 3)
     return m
 
 

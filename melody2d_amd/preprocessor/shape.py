@@ -148,3 +148,48 @@ def rpim(points, nodes, ptr, idx, shape_c=1.0, dscale=None):
         out_x[sel, :kk] = sol[:, :kk, 1]
         out_y[sel, :kk] = sol[:, :kk, 2]
     return out_phi[mask], out_x[mask], out_y[mask]
+
+
+def rpim_fixed_torch(points, nodes, nn, shape_c=1.0, device="cuda", chunk=200000):
+    """rpim() for supports of one fixed size (rows of nn), with the systems
+    built and solved in batches by torch on `device` (the 1M-node plate has
+    ~7M of them).  Same formulas as rpim(); the solver's rounding differs
+    from LAPACK's, so the two agree to rounding, not bit for bit."""
+    import torch
+    n, k = nn.shape
+    out = [np.zeros((n, k)) for _ in range(3)]
+    Xn = torch.as_tensor(nodes, dtype=torch.float64, device=device)
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        idx = torch.as_tensor(nn[a:b], dtype=torch.int64, device=device)
+        x = torch.as_tensor(points[a:b], dtype=torch.float64, device=device)
+        P = Xn[idx]                                              # (m,k,2)
+        m = b - a
+        D = torch.sqrt(torch.max(torch.sum((P - x[:, None, :]) ** 2, dim=2), dim=1).values)
+        al = (shape_c / D) ** 2
+        dij = P[:, :, None, :] - P[:, None, :, :]
+        r2 = torch.sum(dij * dij, dim=3)
+        del dij
+        G = torch.zeros((m, k + 3, k + 3), dtype=torch.float64, device=device)
+        G[:, :k, :k] = torch.exp(-al[:, None, None] * r2)
+        del r2
+        h = D[:, None]
+        Pm = torch.stack([torch.ones((m, k), dtype=torch.float64, device=device), (P[..., 0] - x[:, 0:1]) / h,
+                          (P[..., 1] - x[:, 1:2]) / h], dim=2)
+        G[:, :k, k:] = Pm
+        G[:, k:, :k] = Pm.transpose(1, 2)
+        dx = x[:, None, :] - P
+        rx2 = torch.sum(dx * dx, dim=2)
+        rb = torch.exp(-al[:, None] * rx2)
+        rhs = torch.zeros((m, k + 3, 3), dtype=torch.float64, device=device)
+        rhs[:, :k, 0] = rb
+        rhs[:, :k, 1] = rb * (-2.0 * al[:, None] * dx[..., 0])
+        rhs[:, :k, 2] = rb * (-2.0 * al[:, None] * dx[..., 1])
+        rhs[:, k, 0] = 1.0
+        rhs[:, k + 1, 1] = 1.0 / D
+        rhs[:, k + 2, 2] = 1.0 / D
+        sol = torch.linalg.solve(G, rhs)
+        for q in range(3):
+            out[q][a:b] = sol[:, :k, q].cpu().numpy()
+        del G, rhs, sol
+    return out[0].reshape(-1), out[1].reshape(-1), out[2].reshape(-1)
