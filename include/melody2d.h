@@ -31,7 +31,8 @@ typedef enum {
     MEL_E_HIP = 5,          /* HIP runtime error */
     MEL_E_NAN = 6,          /* non-finite kinematic state (Code/main.cpp:227-237 stops the run) */
     MEL_E_CAPACITY = 7,     /* a fixed-capacity device buffer overflowed */
-    MEL_E_UNSUPPORTED = 8   /* model uses a feature this library does not implement */
+    MEL_E_UNSUPPORTED = 8,  /* model uses a feature this library does not implement */
+    MEL_E_TIMEOUT = 9       /* a bounded cross-workgroup wait of the step ran out of time */
 } mel_status;
 
 typedef enum {

@@ -1,0 +1,15 @@
+// Per-rigid-body contact element lists (k_rigid_lists.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "mel_device.hpp"
+
+namespace mel {
+
+// elements one rigid body can be the master of (status bit 32 beyond)
+constexpr int RIGID_LIST_CAP = 4096;
+
+// after a contact update (or a restored state): the lists of DevModel
+// rm_cnt / rm_list / rs_range from the current elements, queued on stream s
+hipError_t build_rigid_lists(const DevModel& d, hipStream_t s);
+
+}  // namespace mel

@@ -5,6 +5,7 @@
 #include "mel_device.hpp"
 #include "k_block.hpp"
 #include "k_step.hpp"
+#include "k_rigid_lists.hpp"
 #include <climits>
 
 namespace mel {
@@ -417,16 +418,22 @@ __device__ inline RigidForces rigid_forces_block(const DevModel& m, int rigid_in
         s_bcf[k] = xt == 2 || xt == 3 || xt == 4 || yt == 2 || yt == 3 || yt == 4;
     }
     double sx = 0., sy = 0., sm = 0.;
-    // every element's records are loaded without waiting on its bodies (an
-    // element that does not name the body adds +0, which leaves the sums
-    // unchanged: they start at +0 and never become -0)
-#pragma unroll 8
-    for (int e = tid; e < nc; e += nt) {
-        const int bs = m.C.bodyS[e], bm = m.C.bodyM[e];
+    // the body's own elements (k_rigid_lists.hip): those it is the slave of,
+    // then those it is the master of, in element order, strided over the
+    // threads
+    (void)nc;
+    const int s0 = m.rs_range[2 * rigid_index], s1 = m.rs_range[2 * rigid_index + 1];
+    const int q1 = min(m.rm_cnt[rigid_index], RIGID_LIST_CAP);
+    const int* mlist = m.rm_list + (size_t)rigid_index * RIGID_LIST_CAP;
+#pragma unroll 4
+    for (int e = s0 + tid; e < s1; e += nt) {
         const double* r = m.rig_rec + 6 * e;
-        const double r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
-        sx += bs == b ? r0 : 0.; sy += bs == b ? r1 : 0.; sm += bs == b ? r2 : 0.;
-        sx += bm == b ? r3 : 0.; sy += bm == b ? r4 : 0.; sm += bm == b ? r5 : 0.;
+        sx += r[0]; sy += r[1]; sm += r[2];
+    }
+#pragma unroll 4
+    for (int q = tid; q < q1; q += nt) {
+        const double* r = m.rig_rec + 6 * mlist[q];
+        sx += r[3]; sy += r[4]; sm += r[5];
     }
     if (status == 1) return F;     // inactive body (the same for the whole workgroup)
     red[3 * tid] = sx; red[3 * tid + 1] = sy; red[3 * tid + 2] = sm;

@@ -17,6 +17,7 @@
 #include "../../include/melody2d.h"
 #include "k_block.hpp"
 #include "k_fused.hpp"
+#include "k_rigid_lists.hpp"
 #include "k_step.hpp"
 #include "mel_device.hpp"
 #include "model_io.hpp"
@@ -40,6 +41,10 @@ struct CapacityError {
 };
 
 struct NanError {
+    std::string msg;
+};
+
+struct TimeoutError {
     std::string msg;
 };
 
@@ -356,13 +361,14 @@ void compute_masses(Model& m, std::vector<double>& body_mass, std::vector<double
 
 
 void raise_flags(const int* f) {
+    if (f[0] & 16) throw TimeoutError{"a cross-workgroup wait of the step ran out of its time limit"};
     if (f[0]) {
         std::string what;
         if (f[0] & 1) what += " point-force bucket";
         if (f[0] & 2) what += " neighbour list";
         if (f[0] & 4) what += " history index";
         if (f[0] & 8) what += " contact elements";
-        if (f[0] & 16) what += " cross-workgroup wait time limit (fused step)";
+        if (f[0] & 32) what += " rigid-body element list";
         throw CapacityError{"device capacity exceeded:" + what};
     }
     if (f[1]) throw NanError{"non-finite displacement parameters (the run is no longer meaningful)"};
@@ -895,6 +901,7 @@ void update_proximity(mel_ctx& c) {
     k_prox_write<<<persistent, BLOCK, 0, s>>>(d);
     k_prox_history<<<persistent, BLOCK, 0, s>>>(d);
     k_prox_chain<<<persistent, BLOCK, 0, s>>>(d);
+    HIP_OK(build_rigid_lists(d, s));
     HIP_OK(hipMemcpyAsync(c.h_async, d.c_count, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(c.h_async + 1, d.status_flags, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipEventRecord(c.ev_async, s));
@@ -1564,6 +1571,14 @@ void build(mel_ctx& c) {
     c.d_rigid_ids = c.upload(c.h_rigid_ids);
     d.rigid_ids = c.d_rigid_ids;
     {
+        std::vector<int> rindex(NB, -1);
+        for (size_t r = 0; r < c.h_rigid_ids.size(); r++) rindex[c.h_rigid_ids[r]] = (int)r;
+        d.body_rindex = c.upload(rindex);
+        d.rm_cnt = c.alloc<int>(std::max(1, d.n_rigid));
+        d.rm_list = c.alloc<int>((size_t)std::max(1, d.n_rigid) * RIGID_LIST_CAP);
+        d.rs_range = c.alloc<int>(2 * (size_t)std::max(1, d.n_rigid));
+    }
+    {
         std::vector<int> rn;
         for (int b : c.h_rigid_ids)
             for (int i = 0; i < m.bodies[b].n_nodes; i++) rn.push_back(node_off[b] + i);
@@ -1654,6 +1669,9 @@ mel_status create(mel_ctx** out, int device, int precision, const std::function<
     } catch (NanError& e) {
         g_load_err = e.msg;
         return MEL_E_NAN;
+    } catch (TimeoutError& e) {
+        g_load_err = e.msg;
+        return MEL_E_TIMEOUT;
     }
     *out = c.release();
     return MEL_OK;
@@ -1674,6 +1692,9 @@ mel_status guarded(mel_ctx* c, F&& f) {
     } catch (NanError& e) {
         c->err = e.msg;
         return MEL_E_NAN;
+    } catch (TimeoutError& e) {
+        c->err = e.msg;
+        return MEL_E_TIMEOUT;
     }
 }
 
@@ -2165,6 +2186,7 @@ mel_status mel_load_state(mel_ctx* c, const void* buf, size_t size) {
             pos += a.bytes;
         }
         HIP_OK(hipMemcpyAsync(c->d.c_count, &nc, sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(build_rigid_lists(c->d, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
         c->n_contacts = nc;
         c->nc_hint = nc;

@@ -160,6 +160,11 @@ struct DevModel {
     double *rbc_part;                        // per such border and run: the run's BC terms (6 each),
                                              // formed by the force launch
     unsigned *step_sync;                     // step launch B: [1] rigid-body workgroups done
+    // contact elements of each rigid body (k_rigid_lists.hip)
+    int *body_rindex;                        // per body: its rigid index, or -1
+    int *rm_cnt;                             // per rigid body: elements it is the master of
+    int *rm_list;                            // RIGID_LIST_CAP per rigid body: those elements, ascending
+    int *rs_range;                           // 2 per rigid body: its slave elements [begin, end)
 };
 
 // --------------------------------------------------------------------------

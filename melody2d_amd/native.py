@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "libmelody2d_hip.so")
 
 MEL_OK = 0
 STATUS_NAMES = {0: "MEL_OK", 1: "MEL_E_ARG", 2: "MEL_E_IO", 3: "MEL_E_PARSE", 4: "MEL_E_OOM", 5: "MEL_E_HIP",
-                6: "MEL_E_NAN", 7: "MEL_E_CAPACITY", 8: "MEL_E_UNSUPPORTED"}
+                6: "MEL_E_NAN", 7: "MEL_E_CAPACITY", 8: "MEL_E_UNSUPPORTED",
+                9: "MEL_E_TIMEOUT"}
 FP64, FP32_MIXED = 0, 1
 EVAL_INTERNAL, EVAL_CONTACT, EVAL_EXTERNAL, EVAL_ALL = 1, 2, 4, 7
 
