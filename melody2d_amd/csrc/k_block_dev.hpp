@@ -88,8 +88,13 @@ __device__ inline double2 contribution(double c0, f64x2 f, const double4& P) {
 }
 
 // ---- FP32-mixed form --------------------------------------------------
-// The gradient is summed as H = F - I from anchored displacements (the
-// block's first owned node subtracted in FP64 before rounding to FP32), and
+// The gradient is summed as H = F - I from anchored displacements: the
+// displacement of the Gauss point's first support node is subtracted from
+// every support node's in FP64 before rounding to FP32, so the FP32 terms are
+// of the size of the displacement differences across one support (an anchor
+// further away, such as a node block's first node, makes them larger than
+// their sum by the ratio of the distances and the gradient loses as many
+// digits); and
 // the stress terms are rewritten in H so that no step takes a small
 // difference of O(1) numbers: J - 1 = H11 + H22 + H11 H22 - H12 H21,
 // F11^2 + F12^2 - 1 = 2 H11 + H11^2 + H12^2, ... .  The reference's literal
@@ -168,7 +173,7 @@ struct Fp32 {
     static constexpr int GATHER = KREG_ENTRIES_F32 / 2;
     typedef f32x2 vec;
     typedef float2 slot;
-    typedef float2 disp;
+    typedef double2 disp;     // staged in FP64: each Gauss point anchors its own differences
     typedef float real;
 };
 // wide register windows, for models whose Gauss points have more support
@@ -303,13 +308,24 @@ __device__ inline void gp_lane(const BlockArrays& B, const LaneMap& L, const Lan
     }
 }
 
-// the same in the FP32-mixed form (H sums from anchored displacements)
+// displacement difference to the Gauss point's anchor, formed in FP64 and
+// rounded once
+__device__ __forceinline__ f32x2 anchored(f64x2 u, f64x2 a) {
+    f32x2 r;
+    r.x = (float)(u.x - a.x);
+    r.y = (float)(u.y - a.y);
+    return r;
+}
+
+// the same in the FP32-mixed form (H sums from displacements anchored at the
+// Gauss point's first support node)
 template <int MAT, class PR>
-__device__ inline void gp_lane32(const BlockArrays& B, const LaneMap& L, const LaneRegs<PR>& R, const f32x2* uv,
+__device__ inline void gp_lane32(const BlockArrays& B, const LaneMap& L, const LaneRegs<PR>& R, const f64x2* uv,
                                  float2* s_cxy, uint16_t* s_creg, bool single, float Mu, float K2) {
     constexpr int KR = PR::KREG, GB = PR::GATHER;
     float a = 0.f, b = 0.f, c = 0.f, d = 0.f;
-    f32x2 u[GB];
+    f64x2 u[GB];
+    const f64x2 u0 = uv[R.pk[0] & 0xFFFF];
 #pragma unroll
     for (int j0 = 0; j0 < KR; j0 += GB) {
 #pragma unroll
@@ -317,11 +333,11 @@ __device__ inline void gp_lane32(const BlockArrays& B, const LaneMap& L, const L
             if (j0 + i < KR && j0 + i < L.K) u[i] = uv[R.pk[j0 + i] & 0xFFFF];
 #pragma unroll
         for (int i = 0; i < GB; i++)
-            if (j0 + i < KR && j0 + i < L.K) grad_add32<MAT>(a, b, c, d, R.f[j0 + i], u[i]);
+            if (j0 + i < KR && j0 + i < L.K) grad_add32<MAT>(a, b, c, d, R.f[j0 + i], anchored(u[i], u0));
     }
     const f32x2* fxy = reinterpret_cast<const f32x2*>(B.be_fxy32);
     for (int i = KR; i < L.K; i++)
-        grad_add32<MAT>(a, b, c, d, lane_entry(fxy, L, i), uv[lane_entry(B.be_pk, L, i) & 0xFFFF]);
+        grad_add32<MAT>(a, b, c, d, lane_entry(fxy, L, i), anchored(uv[lane_entry(B.be_pk, L, i) & 0xFFFF], u0));
     const float4 P = stress_terms32<MAT>(Mu, K2, a, b, c, d);
     const float c0 = (float)R.c0;
 #pragma unroll
@@ -383,18 +399,9 @@ __device__ __forceinline__ void block_force_update(DevModel& m, const BlockArray
     // displacement parameters of the touched nodes (a contiguous range when
     // they are the owned nodes: no index load first)
     const int* touch = h0.w >= 0 ? B.touch_nodes + h0.w : nullptr;
-    if constexpr (PR::f32) {
-        // FP32 displacements relative to the block's first owned node
-        const double ax = nown > 0 ? m.upx[own0] : 0., ay = nown > 0 ? m.upy[own0] : 0.;
-        for (int i = t; i < ntouch; i += BLOCK_THREADS) {
-            int g = touch ? touch[i] : own0 + i;
-            s_uv[i] = make_float2((float)(m.upx[g] - ax), (float)(m.upy[g] - ay));
-        }
-    } else {
-        for (int i = t; i < ntouch; i += BLOCK_THREADS) {
-            int g = touch ? touch[i] : own0 + i;
-            s_uv[i] = make_double2(m.upx[g], m.upy[g]);
-        }
+    for (int i = t; i < ntouch; i += BLOCK_THREADS) {
+        int g = touch ? touch[i] : own0 + i;
+        s_uv[i] = make_double2(m.upx[g], m.upy[g]);
     }
     double tx[NODES_PER_THREAD], ty[NODES_PER_THREAD], rx[NODES_PER_THREAD], ry[NODES_PER_THREAD];
     int cur[NODES_PER_THREAD];
@@ -409,7 +416,7 @@ __device__ __forceinline__ void block_force_update(DevModel& m, const BlockArray
         const LaneMap L = lane_map(meta, t);
         if (L.valid) {
             if constexpr (PR::f32) {
-                const f32x2* uv = reinterpret_cast<const f32x2*>(s_uv);
+                const f64x2* uv = reinterpret_cast<const f64x2*>(s_uv);
                 if (mtype == 0) gp_lane32<0>(B, L, R, uv, s_cxy, s_creg, single, (float)Mu, (float)K2);
                 else gp_lane32<1>(B, L, R, uv, s_cxy, s_creg, single, (float)Mu, (float)K2);
             } else {
@@ -484,7 +491,7 @@ __device__ __forceinline__ void block_force_update(DevModel& m, const BlockArray
             const LaneMap L = lane_map(meta, t);
             slot_t* buf = s_cxy + (c & 1) * (CHUNK_ENTRIES + 1);
             if (L.valid) {
-                const f32x2* uv = reinterpret_cast<const f32x2*>(s_uv);
+                const f64x2* uv = reinterpret_cast<const f64x2*>(s_uv);
                 if (mtype == 0) gp_lane32<0>(B, L, Rc, uv, buf, s_creg, true, (float)Mu, (float)K2);
                 else gp_lane32<1>(B, L, Rc, uv, buf, s_creg, true, (float)Mu, (float)K2);
             }

@@ -409,10 +409,11 @@ int contact_workgroups(const mel_ctx& c) {
 
 
 // dynamic LDS of the block kernel: displacement parameters of the touched
-// nodes (FP64 pairs, or FP32 pairs in the FP32-mixed form), then the region
-// of every contribution slot (reference order only)
+// nodes (FP64 pairs in both forms), then the region of every contribution
+// slot (reference order only)
 size_t block_lds_size(const BlockArrays& B, bool f32) {
-    size_t u = (f32 ? 8 : 16) * (size_t)B.max_touch;
+    (void)f32;
+    size_t u = 16 * (size_t)B.max_touch;
     u = (u + 15) & ~size_t(15);
     return u + (B.any_multi_region ? 2 * ((size_t)CHUNK_ENTRIES + 8) : 0);
 }

@@ -87,3 +87,37 @@ def test_rpim_plate_matches_oracle(exact, tmp_path, monkeypatch):
     # the launch sequence (wide window), not the two-launch step
     assert kt["block_force_update"][1] == steps[-1] and kt["step_forces"][1] == 0
     ctx.close()
+
+
+def test_plate_fp32_mixed_force_error_within_bar(tmp_path):
+    """The FP32-mixed internal forces of the perforated RPIM plate in the
+    uniaxial 10/s field (strain ~1.6e-5, forces small differences of large
+    stress terms) are within 1e-5 of FP64's from the same state; the
+    displacement increments of 100 further steps too."""
+    print("case plate fp32-mixed error", flush=True)
+    m = configs.c3(n_nodes=10000, side=0.1 * np.sqrt(0.01), workers=4)
+    b = m.bodies[0]
+    path = str(tmp_path / "plate.bin")
+    write_binary(m, path)
+    eps = 10.0 * 1000 * m.dt
+    x, y = b.X[:, 0], b.X[:, 1]
+    ctx = {p: native.Context(path, binary=True, precision=p) for p in (native.FP64, native.FP32_MIXED)}
+    for c in ctx.values():
+        for f, v in (("x_disp_param", eps * x), ("y_disp_param", -0.33 * eps * y),
+                     ("x_vel_param", 10.0 * x), ("y_vel_param", -0.33 * 10.0 * y)):
+            c.set_node_field(0, f, np.ascontiguousarray(v))
+        c.eval_forces(native.EVAL_INTERNAL)
+    n = b.n_nodes
+    f64 = np.concatenate([ctx[native.FP64].node_field(0, f, n) for f in ("x_internal_force", "y_internal_force")])
+    f32 = np.concatenate([ctx[native.FP32_MIXED].node_field(0, f, n) for f in ("x_internal_force", "y_internal_force")])
+    err = np.linalg.norm(f32 - f64) / np.linalg.norm(f64)
+    print("  internal-force error %.3e" % err, flush=True)
+    assert err <= 1e-5, err
+    u0 = ctx[native.FP64].node_field(0, "x_disp_param", n)
+    for c in ctx.values():
+        c.euler_steps(100)
+    d64 = ctx[native.FP64].node_field(0, "x_disp_param", n) - u0
+    d32 = ctx[native.FP32_MIXED].node_field(0, "x_disp_param", n) - u0
+    assert np.linalg.norm(d32 - d64) <= 1e-5 * np.linalg.norm(d64)
+    for c in ctx.values():
+        c.close()
