@@ -140,80 +140,204 @@ __global__ void k_prox_aabb(DevModel m) {
     m.pb_shmax[br] = (int)floor((xmax - m.xmin_period) * inv);
 }
 
-// neighbour lists (Code/Proximity.h:112-169).  One workgroup of NB_THREADS
-// per body1; lanes take consecutive body2 so the order of insertion is the
-// reference's (border1, body2, border2, shift) order with first-occurrence
-// de-duplication.
+// ---------------------------------------------------------------------------
+// Broad phase: a cell list over the border AABBs (Code/Proximity.h:63-109
+// computes them; the reference then tests every border pair,
+// Code/Proximity.h:112-169).  Borders are binned by the centre of their
+// AABB on a uniform grid (x taken modulo the period, so cells wrap across
+// the periodic boundary); a border larger than a cell is "big" and kept in
+// a list of its own.  The candidates of a border are the borders binned in
+// the cells its AABB covers grown by one cell, and every big border: two
+// AABBs of at most a cell each that overlap have centres at most a cell
+// apart, so nothing the reference finds is missed.  Every candidate gets
+// the reference's exact tests, and the body's neighbour list is put in the
+// reference's order: sorted by (border1, body2, border2, shift), keeping the
+// first of equal (body2, border2, shift) (Code/Proximity.h:149-159).
+// ---------------------------------------------------------------------------
+__global__ void k_cells_clear(DevModel m) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m.cell_n; c += gridDim.x * blockDim.x) {
+        m.cell_count[c] = 0;
+        m.cell_fill[c] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *m.big_count = 0;
+}
+
+// column of a (wrapped) cell coordinate q, which may be any finite double
+__device__ inline int wrap_col(const DevModel& m, double q) {
+    double r = fmod(q, (double)m.cell_nx);
+    if (r < 0.) r += m.cell_nx;
+    const int c = (int)r;
+    return min(max(c, 0), m.cell_nx - 1);
+}
+
+__device__ inline int cell_col(const DevModel& m, double x) {
+    return wrap_col(m, floor((x - m.xmin_period) / m.cell_dx));
+}
+
+__device__ inline int cell_row(const DevModel& m, double y) {
+    const double r = fmin(fmax(floor((y - m.cell_y0) / m.cell_dy), 0.), (double)(m.cell_ny - 1));
+    return (int)r;
+}
+
+// bin one border: its cell, or the big list (-2); inactive bodies: -1
+__global__ void k_cells_assign(DevModel m) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= m.n_borders) return;
+    if (m.body_status[m.br_body[g]] == 1) { m.border_cell[g] = -1; return; }
+    const double xmin = m.pb_xmin[g], xmax = m.pb_xmax[g], ymin = m.pb_ymin[g], ymax = m.pb_ymax[g];
+    // a border whose box is not finite (no slot, or a state gone bad, which
+    // the update reports by itself) is checked against everything
+    const bool finite = isfinite(xmin) && isfinite(xmax) && isfinite(ymin) && isfinite(ymax) &&
+                        fabs(xmin) < 1e30 && fabs(xmax) < 1e30 && fabs(ymin) < 1e30 && fabs(ymax) < 1e30;
+    if (!finite || xmax - xmin > m.cell_dx || ymax - ymin > m.cell_dy) {
+        const int k = atomicAdd(m.big_count, 1);
+        if (k < m.big_cap) m.big_list[k] = g;
+        else atomicOr(&m.status_flags[0], 2);
+        m.border_cell[g] = -2;
+        return;
+    }
+    const int c = cell_row(m, 0.5 * (ymin + ymax)) * m.cell_nx + cell_col(m, 0.5 * (xmin + xmax));
+    m.border_cell[g] = c;
+    atomicAdd(&m.cell_count[c], 1);
+}
+
+// exclusive scan of the cell counts: per 1024-cell block, then the block
+// totals, then the block offsets added
+__global__ void __launch_bounds__(1024) k_cells_scan_blocks(DevModel m) {
+    __shared__ int s_w[16];
+    const int c = blockIdx.x * 1024 + threadIdx.x;
+    const int v = c < m.cell_n ? m.cell_count[c] : 0;
+    int tot;
+    const int ex = wg_excl_scan(v, s_w, tot);
+    if (c < m.cell_n) m.cell_start[c] = ex;
+    if (threadIdx.x == 0) m.cell_bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_cells_scan_top(DevModel m, int nblocks) {
+    __shared__ int s_w[16];
+    wg_scan_array(m.cell_bsum, m.cell_boff, nblocks, s_w);
+}
+
+__global__ void k_cells_scatter(DevModel m) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= m.n_borders) return;
+    const int c = m.border_cell[g];
+    if (c < 0) return;
+    const int pos = m.cell_boff[c >> 10] + m.cell_start[c] + atomicAdd(&m.cell_fill[c], 1);
+    m.cell_items[pos] = g;
+}
+
+__device__ inline int cell_first(const DevModel& m, int c) { return m.cell_boff[c >> 10] + m.cell_start[c]; }
+
+constexpr int NB_CAND = 2048;   // candidate neighbours per body before ordering
+
+// neighbour lists (Code/Proximity.h:112-169), one workgroup of NB_THREADS
+// per body1 over the candidates of the cell list
 __global__ void __launch_bounds__(NB_THREADS) k_prox_neighbours(DevModel m) {
     const int b1 = blockIdx.x;
-    const int lane = threadIdx.x;
-    __shared__ int count_sh;
+    const int t = threadIdx.x;
+    __shared__ unsigned long long cand[NB_CAND];
+    __shared__ int n_cand;
     __shared__ int s_w[NB_THREADS / 64];
-    if (lane == 0) count_sh = 0;
+    if (t == 0) n_cand = 0;
     __syncthreads();
-    int* list = m.nb_list + (size_t)b1 * m.nb_cap * 3;
-    if (m.body_status[b1] != 1) {
-        const double P = m.period;
-        for (int r1 = 0; r1 < m.body_nborders[b1]; r1++) {
-            const int g1 = m.border_off[b1] + r1;
-            const double xmin1 = m.pb_xmin[g1], xmax1 = m.pb_xmax[g1], ymin1 = m.pb_ymin[g1], ymax1 = m.pb_ymax[g1];
-            const int shmin1 = m.pb_shmin[g1], shmax1 = m.pb_shmax[g1];
-            const int before = count_sh;     // entries from earlier border1 iterations
-            for (int base = 0; base < m.n_bodies; base += NB_THREADS * NB_PER) {
-                // this lane's bodies: base + NB_PER * lane + u, their records
-                // loaded before any test
-                int st[NB_PER], go[NB_PER], nbr[NB_PER];
-#pragma unroll
-                for (int u = 0; u < NB_PER; u++) {
-                    const int b2 = base + NB_PER * lane + u;
-                    const bool live = b2 < m.n_bodies && b2 != b1;
-                    st[u] = live ? m.body_status[b2] : 1;
-                    go[u] = live ? m.border_off[b2] : 0;
-                    nbr[u] = live ? m.body_nborders[b2] : 0;
-                }
-                const int start = count_sh;
-                int ex = 0;
-                // pass 0 counts this lane's new neighbours, pass 1 writes them
-                for (int pass = 0; pass < 2; pass++) {
-                    int w = 0;
-#pragma unroll
-                    for (int u = 0; u < NB_PER; u++) {
-                        const int b2 = base + NB_PER * lane + u;
-                        if (st[u] == 1) continue;
-                        for (int r2 = 0; r2 < nbr[u]; r2++) {
-                            const int g2 = go[u] + r2;
-                            if ((m.pb_ymax[g2] < ymin1) || (ymax1 < m.pb_ymin[g2])) continue;
-                            for (int sh = shmin1 - m.pb_shmax[g2]; sh <= shmax1 - m.pb_shmin[g2]; sh++) {
-                                if ((m.pb_xmax[g2] + sh * P > xmin1) && (m.pb_xmin[g2] + sh * P < xmax1)) {
-                                    bool exists = false;
-                                    for (int k = 0; k < before && k < m.nb_cap; k++)
-                                        if (list[3 * k] == b2 && list[3 * k + 1] == r2 && list[3 * k + 2] == sh) { exists = true; break; }
-                                    if (exists) continue;
-                                    if (pass == 1) {
-                                        const int pos = start + ex + w;
-                                        if (pos < m.nb_cap) {
-                                            list[3 * pos] = b2; list[3 * pos + 1] = r2; list[3 * pos + 2] = sh;
-                                            m.nb_gm[(size_t)b1 * m.nb_cap + pos] = g2;
-                                        }
-                                    }
-                                    w++;
-                                }
-                            }
-                        }
-                    }
-                    if (pass == 0) {
-                        int tot;
-                        ex = wg_excl_scan(w, s_w, tot);
-                        if (lane == 0) count_sh = start + tot;
-                    }
-                }
-                __syncthreads();   // this body2 range's entries and count are in place
+    const double P = m.period;
+    const int nbig = min(*m.big_count, m.big_cap);
+    auto test = [&](int r1, int g1, int g2) {
+        const int b2 = m.br_body[g2];
+        if (b2 == b1 || m.body_status[b2] == 1) return;
+        if ((m.pb_ymax[g2] < m.pb_ymin[g1]) || (m.pb_ymax[g1] < m.pb_ymin[g2])) return;
+        const int r2 = g2 - m.border_off[b2];
+        for (int sh = m.pb_shmin[g1] - m.pb_shmax[g2]; sh <= m.pb_shmax[g1] - m.pb_shmin[g2]; sh++) {
+            if ((m.pb_xmax[g2] + sh * P > m.pb_xmin[g1]) && (m.pb_xmin[g2] + sh * P < m.pb_xmax[g1])) {
+                const int k = atomicAdd(&n_cand, 1);
+                if (k < NB_CAND)
+                    cand[k] = ((unsigned long long)r1 << 56) | ((unsigned long long)b2 << 32) |
+                              ((unsigned long long)(r2 & 0xFFFF) << 16) | (unsigned long long)((sh + 32768) & 0xFFFF);
             }
         }
+    };
+    if (m.body_status[b1] != 1) {
+        for (int r1 = 0; r1 < m.body_nborders[b1]; r1++) {
+            const int g1 = m.border_off[b1] + r1;
+            // the cells the AABB covers, grown by one cell; all columns when
+            // they wrap onto themselves
+            const double xa = m.pb_xmin[g1], xb = m.pb_xmax[g1], ya = m.pb_ymin[g1], yb = m.pb_ymax[g1];
+            const bool finite = isfinite(xa) && isfinite(xb) && isfinite(ya) && isfinite(yb) &&
+                                fabs(xa) < 1e30 && fabs(xb) < 1e30 && fabs(ya) < 1e30 && fabs(yb) < 1e30;
+            const double dc0 = floor((xa - m.xmin_period) / m.cell_dx) - 1.;
+            const double dc1 = floor((xb - m.xmin_period) / m.cell_dx) + 1.;
+            const bool all_cols = !finite || dc1 - dc0 + 1. >= (double)m.cell_nx;
+            const int ncol = all_cols ? m.cell_nx : (int)(dc1 - dc0) + 1;
+            const int c0 = all_cols ? 0 : wrap_col(m, dc0);
+            const int ry0 = finite ? max(cell_row(m, ya) - 1, 0) : 0;
+            const int ry1 = finite ? min(cell_row(m, yb) + 1, m.cell_ny - 1) : m.cell_ny - 1;
+            const int nrow = ry1 - ry0 + 1;
+            // one thread per cell of the range, then its borders
+            for (int q = t; q < ncol * nrow; q += NB_THREADS) {
+                const int row = ry0 + q / ncol;
+                const int col = (c0 + q % ncol) % m.cell_nx;
+                const int c = row * m.cell_nx + col;
+                const int e0 = cell_first(m, c), e1 = e0 + m.cell_count[c];
+                for (int e = e0; e < e1; e++) test(r1, g1, m.cell_items[e]);
+            }
+            for (int q = t; q < nbig; q += NB_THREADS) test(r1, g1, m.big_list[q]);
+        }
     }
-    if (lane == 0) {
-        if (count_sh > m.nb_cap) atomicOr(&m.status_flags[0], 2);
-        m.nb_count[b1] = min(count_sh, m.nb_cap);
+    __syncthreads();
+    int n = n_cand;
+    if (n > NB_CAND) {
+        if (t == 0) atomicOr(&m.status_flags[0], 2);
+        n = NB_CAND;
+    }
+    // bitonic sort of the candidates (padded to a power of two)
+    int p = 1;
+    while (p < n) p <<= 1;
+    for (int i = n + t; i < p; i += NB_THREADS) cand[i] = ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= p; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = t; i < p; i += NB_THREADS) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const bool up = (i & k) == 0;
+                    const unsigned long long a = cand[i], b = cand[l];
+                    if ((a > b) == up) { cand[i] = b; cand[l] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // the first of equal (body2, border2, shift) in (border1, ...) order
+    const unsigned long long LOW = (1ull << 56) - 1;
+    int* list = m.nb_list + (size_t)b1 * m.nb_cap * 3;
+    int base = 0;
+    for (int i0 = 0; i0 < n; i0 += NB_THREADS) {
+        const int i = i0 + t;
+        int keep = 0;
+        if (i < n) {
+            keep = 1;
+            const unsigned long long x = cand[i] & LOW;
+            for (int j = 0; j < i; j++)
+                if ((cand[j] & LOW) == x) { keep = 0; break; }
+        }
+        int tot;
+        const int ex = wg_excl_scan(keep, s_w, tot);
+        if (keep) {
+            const int pos = base + ex;
+            if (pos < m.nb_cap) {
+                const unsigned long long x = cand[i];
+                const int b2 = (int)((x >> 32) & 0xFFFFFF), r2 = (int)((x >> 16) & 0xFFFF);
+                const int sh = (int)(x & 0xFFFF) - 32768;
+                list[3 * pos] = b2; list[3 * pos + 1] = r2; list[3 * pos + 2] = sh;
+                m.nb_gm[(size_t)b1 * m.nb_cap + pos] = m.border_off[b2] + r2;
+            }
+        }
+        base += tot;
+    }
+    if (t == 0) {
+        if (base > m.nb_cap) atomicOr(&m.status_flags[0], 2);
+        m.nb_count[b1] = min(base, m.nb_cap);
     }
 }
 

@@ -30,6 +30,11 @@ __global__ void k_clear_node_forces(DevModel m, int internal, int damping);
 // proximity (k_proximity.hip)
 __global__ void k_prox_chunk_boxes(DevModel m);
 __global__ void k_prox_aabb(DevModel m);
+__global__ void k_cells_clear(DevModel m);
+__global__ void k_cells_assign(DevModel m);
+__global__ void k_cells_scan_blocks(DevModel m);
+__global__ void k_cells_scan_top(DevModel m, int nblocks);
+__global__ void k_cells_scatter(DevModel m);
 __global__ void k_prox_neighbours(DevModel m);
 __global__ void k_prox_chunk_offsets(DevModel m);
 __global__ void k_prox_count(DevModel m, int n_count);

@@ -890,6 +890,13 @@ void update_proximity(mel_ctx& c) {
     c.prox_after_step = true;
     if (d.n_seg_chunks > 0) k_prox_chunk_boxes<<<blocks((long long)d.n_seg_chunks * SEG_CHUNK), BLOCK, 0, s>>>(d);
     k_prox_aabb<<<blocks((long long)d.n_borders * 64), BLOCK, 0, s>>>(d);
+    // broad phase: the cell list, then the neighbour lists from it
+    const int cblocks = (d.cell_n + 1023) / 1024;
+    k_cells_clear<<<std::min(blocks(d.cell_n), 1024), BLOCK, 0, s>>>(d);
+    k_cells_assign<<<blocks(d.n_borders), BLOCK, 0, s>>>(d);
+    k_cells_scan_blocks<<<cblocks, 1024, 0, s>>>(d);
+    k_cells_scan_top<<<1, 1024, 0, s>>>(d, cblocks);
+    k_cells_scatter<<<blocks(d.n_borders), BLOCK, 0, s>>>(d);
     k_prox_neighbours<<<d.n_bodies, NB_THREADS, 0, s>>>(d);
     k_prox_chunk_offsets<<<1, 1024, 0, s>>>(d);
     // previous elements become the history source
@@ -1534,6 +1541,68 @@ void build(mel_ctx& c) {
     }
     d.seg_ns = c.alloc<int4>(NS);
     d.seg_xy = c.alloc<double4>(NS);
+    // cell list of the broad phase: cells 1.25 x the largest border AABB
+    // (grown by the detection distance) among the borders up to 8 x the
+    // median size; larger borders (walls) are listed apart
+    {
+        std::vector<double> ext;
+        double ylo = 1e300, yhi = -1e300;
+        for (const Body& B : m.bodies) {
+            for (const Border& R : B.borders) {
+                double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+                for (int n : R.nodes) {
+                    double x = B.X[n], y = B.Y[n];
+                    if (!B.rigid && !B.ux.empty()) { x += B.ux[n]; y += B.uy[n]; }
+                    if (B.rigid) { x += B.rigid_disp[0]; y += B.rigid_disp[1]; }
+                    x0 = std::min(x0, x); x1 = std::max(x1, x); y0 = std::min(y0, y); y1 = std::max(y1, y);
+                }
+                if (R.nodes.empty()) continue;
+                const double dd = B.detection_distance;
+                ext.push_back(std::max(x1 - x0, y1 - y0) + 2 * dd);
+                ylo = std::min(ylo, y0 - dd);
+                yhi = std::max(yhi, y1 + dd);
+            }
+        }
+        for (const Body& B : m.bodies)
+            if (B.borders.size() > 255)
+                throw CapacityError{"a body with more than 255 borders is not supported"};
+        double cell = 1.0;
+        if (!ext.empty()) {
+            std::vector<double> e = ext;
+            std::nth_element(e.begin(), e.begin() + e.size() / 2, e.end());
+            const double med = e[e.size() / 2];
+            cell = 0.;
+            for (double x : ext)
+                if (x <= 8 * med) cell = std::max(cell, x);
+            cell = cell > 0. ? 1.25 * cell : 1.0;
+        } else {
+            ylo = 0.; yhi = 1.;
+        }
+        const double P = d.period;
+        d.cell_nx = (int)std::max(1.0, std::min(4096.0, std::floor(P / cell)));
+        d.cell_dx = P / d.cell_nx;
+        d.cell_dy = cell;
+        const double margin = 8 * cell;
+        d.cell_y0 = ylo - margin;
+        double rows = std::ceil((yhi - ylo + 2 * margin) / d.cell_dy);
+        const double max_cells = 1 << 22;
+        if (rows * d.cell_nx > max_cells) {
+            d.cell_dy *= rows * d.cell_nx / max_cells;
+            rows = std::ceil((yhi - ylo + 2 * margin) / d.cell_dy);
+        }
+        d.cell_ny = (int)std::max(1.0, rows);
+        d.cell_n = d.cell_nx * d.cell_ny;
+        d.cell_count = c.alloc<int>(d.cell_n);
+        d.cell_fill = c.alloc<int>(d.cell_n);
+        d.cell_start = c.alloc<int>(d.cell_n);
+        d.cell_bsum = c.alloc<int>((d.cell_n + 1023) / 1024 + 1);
+        d.cell_boff = c.alloc<int>((d.cell_n + 1023) / 1024 + 1);
+        d.cell_items = c.alloc<int>(NBR);
+        d.border_cell = c.alloc<int>(NBR);
+        d.big_cap = std::max(64, NBR);
+        d.big_count = c.alloc<int>(1);
+        d.big_list = c.alloc<int>(d.big_cap);
+    }
     d.nb_cap = 256;
     d.nb_count = c.alloc<int>(NB);
     d.nb_list = c.alloc<int>((size_t)NB * d.nb_cap * 3);

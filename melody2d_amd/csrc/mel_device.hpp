@@ -134,6 +134,12 @@ struct DevModel {
                                              // their segment boxes
     int4 *seg_ns;                            // per slot s, segment s: end nodes n1, n2 and their shifts
     double4 *seg_xy;                         // per slot s, segment s: x, y of n1, then of n2
+    // cell list of the broad phase (k_proximity.hip)
+    int cell_nx, cell_ny, cell_n;            // grid columns (wrapping over the period), rows, cells
+    double cell_dx, cell_dy, cell_y0;        // cell size; first row's lower edge
+    int *cell_count, *cell_fill, *cell_start, *cell_bsum, *cell_boff, *cell_items;
+    int *border_cell;                        // per border: its cell, -2 big, -1 inactive
+    int big_cap, *big_count, *big_list;      // borders larger than a cell
     int nb_cap;                              // neighbour capacity per body
     int *nb_count, *nb_list;                 // per body: count, 3 ints per neighbour
     int *body_chunks, *chunk_off;            // narrow-phase row chunks per body, first chunk per body
