@@ -72,6 +72,36 @@ mel_status mel_load(const char* dir, int save_index, mel_precision precision, in
  * melody2d_amd.preprocessor (for models too large for the ASCII files). */
 mel_status mel_load_binary(const char* path, mel_precision precision, int hip_device, mel_ctx** out);
 
+/* Load-time choices of the step's launch configuration (no reference
+ * counterpart: they change how the step is laid out on the GPU, never its
+ * results beyond rounding order).  -1 leaves a field to the library's default
+ * (or to the MELODY2D_* environment switch of the same meaning). */
+typedef struct {
+    int block_nodes;    /* owned nodes per node block of a body split over blocks, 1..256 (default 256) */
+    int fused_step;     /* 1: the step in two launches where every body fits one block; 0: the launch sequence */
+    int graph_replay;   /* 1: runs of steps between contact updates replayed from captured HIP graphs */
+    int wide_window;    /* 1: the wide register window (30 entries per lane); 0: never; -1: by support sizes */
+} mel_options;
+
+void mel_default_options(mel_options* opts);
+/* mel_load / mel_load_binary with options (NULL: defaults) */
+mel_status mel_load_ex(const char* dir, int save_index, mel_precision precision, int hip_device,
+                       const mel_options* opts, mel_ctx** out);
+mel_status mel_load_binary_ex(const char* path, mel_precision precision, int hip_device, const mel_options* opts,
+                              mel_ctx** out);
+/* the configuration a context runs with (fields resolved, no -1 left except
+ * block_nodes when it was not set) */
+mel_status mel_get_options(const mel_ctx* ctx, mel_options* opts);
+
+/* Autotuner: loads the model of `binary_path` once per candidate launch
+ * configuration (node-block size, two-launch step or launch sequence, graph
+ * replay, register window), times `steps` steps of each after a warm-up and
+ * returns the fastest in *best (and its time per step, and how many
+ * configurations were timed).  No reference counterpart (the reference has
+ * no launch configuration). */
+mel_status mel_autotune(const char* binary_path, mel_precision precision, int hip_device, int steps,
+                        mel_options* best, double* best_ms_per_step, int* n_tried);
+
 /* Update_proximity (Code/Proximity.h:48-473): neighbour lists, contact
  * candidates with history transfer, self-contact unless NO_MONITORING. */
 mel_status mel_update_proximity(mel_ctx* ctx);

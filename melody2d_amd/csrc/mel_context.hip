@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <functional>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -144,6 +145,7 @@ static const char* kKernelNames[KT_COUNT] = {
 
 struct mel_ctx {
     Model model;
+    mel_options opts{-1, -1, -1, -1};   // load options (-1: library default, or the environment's)
     // event instrumentation (off by default)
     bool timing = false;
     unsigned timing_mask = 0;     // bit k set: kernel k is instrumented
@@ -447,6 +449,7 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
         int v = std::atoi(e);
         if (v >= 1 && v < BLOCK_MAX_OWN) own_cap = v;
     }
+    if (c.opts.block_nodes >= 1) own_cap = std::min(c.opts.block_nodes, BLOCK_MAX_OWN);
     // MELODY2D_EXACT_SUMS=1: internal forces summed in the reference's order
     // (bit-identical where pow rounds alike); default: balanced order.  The
     // FP32-mixed form, which rounds differently anyway, always takes the
@@ -827,7 +830,8 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     // (its blocks read each other's displacement parameters);
     // MELODY2D_FUSED_STEP=0 asks for the launch sequence
     const char* fe = getenv("MELODY2D_FUSED_STEP");
-    c.two_launch = B.lds_finish_ok && !(fe && fe[0] == '0');
+    const bool fused_off = c.opts.fused_step >= 0 ? c.opts.fused_step == 0 : (fe && fe[0] == '0');
+    c.two_launch = B.lds_finish_ok && !fused_off;
     // the launch sequence takes the wide-window kernel when Gauss points
     // with more support entries than the default window are common (RPIM
     // supports of 30 nodes; a few long MLS supports read their tail from
@@ -840,16 +844,17 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
         n_long += g_ptr[g + 1] - g_ptr[g] > window;
     }
     const char* we = getenv("MELODY2D_WIDE_LANES");
-    const bool force_wide = we && we[0] == '1';
+    const bool force_wide = c.opts.wide_window >= 0 ? c.opts.wide_window == 1 : (we && we[0] == '1');
     if (force_wide) c.two_launch = false;
-    c.wide = force_wide || (!c.two_launch && 10 * n_long >= n_gp && n_long > 0);
+    c.wide = force_wide || (c.opts.wide_window != 0 && !c.two_launch && 10 * n_long >= n_gp && n_long > 0);
     HIP_OK(hipFuncSetAttribute(f32 ? (c.wide ? (const void*)k_block_force_update32_wide : (const void*)k_block_force_update32)
                                    : (c.wide ? (const void*)k_block_force_update_wide : (const void*)k_block_force_update),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.block_lds));
     // runs of steps replayed from captured graphs; MELODY2D_GRAPHS=0 launches
     // every step
     const char* ge = getenv("MELODY2D_GRAPHS");
-    c.use_graphs = c.two_launch && !(ge && ge[0] == '0');
+    const bool graphs_off = c.opts.graph_replay >= 0 ? c.opts.graph_replay == 0 : (ge && ge[0] == '0');
+    c.use_graphs = c.two_launch && !graphs_off;
     if (c.two_launch) {
         HIP_OK(hipFuncSetAttribute(f32 ? (const void*)k_step_forces32 : (const void*)k_step_forces,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.block_lds));
@@ -1316,14 +1321,28 @@ void build(mel_ctx& c) {
         law_fixed[i] = m.laws[i].fixed_length;
         for (size_t k = 0; k < m.laws[i].p.size() && k < 12; k++) law_p[12 * i + k] = m.laws[i].p[k];
     }
+    // first law naming a material pair, in either order (materials indexed
+    // by name once: the batched workloads carry a thousand materials)
     std::vector<int> law_first(d.n_materials * d.n_materials, -1);
-    for (int a = 0; a < d.n_materials; a++)
-        for (int bm = 0; bm < d.n_materials; bm++)
-            for (int i = 0; i < d.n_laws; i++) {
-                const ContactLaw& L = m.laws[i];
-                const std::string &ma = m.materials[a].name, &mb = m.materials[bm].name;
-                if ((L.m1 == ma && L.m2 == mb) || (L.m2 == ma && L.m1 == mb)) { law_first[a * d.n_materials + bm] = i; break; }
-            }
+    {
+        std::map<std::string, std::vector<int>> by_name;
+        for (int a = 0; a < d.n_materials; a++) by_name[m.materials[a].name].push_back(a);
+        static const std::vector<int> none;
+        auto named = [&](const std::string& n) -> const std::vector<int>& {
+            auto it = by_name.find(n);
+            return it == by_name.end() ? none : it->second;
+        };
+        for (int i = 0; i < d.n_laws; i++) {
+            const ContactLaw& L = m.laws[i];
+            for (int a : named(L.m1))
+                for (int bm : named(L.m2)) {
+                    int& x = law_first[a * d.n_materials + bm];
+                    if (x < 0) x = i;
+                    int& y = law_first[bm * d.n_materials + a];
+                    if (y < 0) y = i;
+                }
+        }
+    }
     // rigid / per-body state
     std::vector<double> body_rigid((size_t)NB * RIGID_FIELDS, 0.);
     for (int b = 0; b < NB; b++) {
@@ -1704,11 +1723,17 @@ void build(mel_ctx& c) {
 
 thread_local std::string g_load_err;
 
-mel_status create(mel_ctx** out, int device, int precision, const std::function<bool(Model&, std::string&)>& loader) {
+mel_status create(mel_ctx** out, int device, int precision, const std::function<bool(Model&, std::string&)>& loader,
+                  const mel_options* opts = nullptr) {
     if (!out) return MEL_E_ARG;
     *out = nullptr;
     if (precision != MEL_FP64 && precision != MEL_FP32_MIXED) { g_load_err = "unknown precision"; return MEL_E_ARG; }
+    if (opts && (opts->block_nodes > BLOCK_MAX_OWN || opts->block_nodes == 0)) {
+        g_load_err = "block_nodes must be 1.." + std::to_string(BLOCK_MAX_OWN) + " or -1";
+        return MEL_E_ARG;
+    }
     std::unique_ptr<mel_ctx> c(new mel_ctx());
+    if (opts) c->opts = *opts;
     c->device = device;
     c->precision = precision;
     std::string err;
@@ -1950,17 +1975,112 @@ extern "C" {
 const char* mel_version(void) { return "melody2d-mi355x 0.1"; }
 
 mel_status mel_load(const char* dir, int save_index, mel_precision precision, int hip_device, mel_ctx** out) {
-    if (!dir) return MEL_E_ARG;
-    std::string d(dir);
-    return create(out, hip_device, precision,
-                  [&](Model& m, std::string& err) { return load_model_files(d, save_index, m, err); });
+    return mel_load_ex(dir, save_index, precision, hip_device, nullptr, out);
 }
 
 mel_status mel_load_binary(const char* path, mel_precision precision, int hip_device, mel_ctx** out) {
+    return mel_load_binary_ex(path, precision, hip_device, nullptr, out);
+}
+
+void mel_default_options(mel_options* o) {
+    if (o) *o = mel_options{-1, -1, -1, -1};
+}
+
+mel_status mel_load_ex(const char* dir, int save_index, mel_precision precision, int hip_device,
+                       const mel_options* opts, mel_ctx** out) {
+    if (!dir) return MEL_E_ARG;
+    std::string d(dir);
+    return create(out, hip_device, precision,
+                  [&](Model& m, std::string& err) { return load_model_files(d, save_index, m, err); }, opts);
+}
+
+mel_status mel_load_binary_ex(const char* path, mel_precision precision, int hip_device, const mel_options* opts,
+                              mel_ctx** out) {
     if (!path) return MEL_E_ARG;
     std::string p(path);
     return create(out, hip_device, precision,
-                  [&](Model& m, std::string& err) { return load_model_binary(p, m, err); });
+                  [&](Model& m, std::string& err) { return load_model_binary(p, m, err); }, opts);
+}
+
+mel_status mel_get_options(const mel_ctx* c, mel_options* o) {
+    if (!c || !o) return MEL_E_ARG;
+    o->block_nodes = c->opts.block_nodes;
+    o->fused_step = c->two_launch ? 1 : 0;
+    o->graph_replay = c->use_graphs ? 1 : 0;
+    o->wide_window = c->wide ? 1 : 0;
+    return MEL_OK;
+}
+
+mel_status mel_autotune(const char* path, mel_precision precision, int hip_device, int steps, mel_options* best,
+                        double* best_ms, int* n_tried) {
+    if (!path || !best || steps < 1) return MEL_E_ARG;
+    Model base;
+    std::string err;
+    if (!load_model_binary(path, base, err)) {
+        g_load_err = err;
+        return err.rfind("cannot open", 0) == 0 ? MEL_E_IO : MEL_E_PARSE;
+    }
+    int max_nodes = 0;
+    for (auto& b : base.bodies)
+        if (!b.rigid) max_nodes = std::max(max_nodes, b.n_nodes);
+    // node-block sizes only matter for bodies split over blocks; a body that
+    // fits one block is never split on purpose (that forces the launch
+    // sequence)
+    std::vector<int> blocks = {-1};
+    if (max_nodes > BLOCK_MAX_OWN) blocks = {BLOCK_MAX_OWN, 192, 128};
+    std::vector<mel_options> cand;
+    for (int bn : blocks) {
+        cand.push_back(mel_options{bn, 1, 1, -1});
+        cand.push_back(mel_options{bn, 1, 0, -1});
+        cand.push_back(mel_options{bn, 0, 0, 0});
+        cand.push_back(mel_options{bn, 0, 0, 1});
+    }
+    std::vector<mel_options> seen;
+    double t_best = 1e300;
+    mel_status first_err = MEL_OK;
+    int tried = 0;
+    for (const mel_options& o : cand) {
+        mel_ctx* c = nullptr;
+        mel_status st = create(&c, hip_device, precision,
+                               [&](Model& m, std::string&) { m = base; return true; }, &o);
+        if (st != MEL_OK) {
+            if (first_err == MEL_OK) first_err = st;
+            continue;
+        }
+        mel_options r;
+        mel_get_options(c, &r);
+        bool dup = false;
+        for (auto& q : seen)
+            dup |= q.block_nodes == r.block_nodes && q.fused_step == r.fused_step &&
+                   q.graph_replay == r.graph_replay && q.wide_window == r.wide_window;
+        if (dup) { mel_destroy(c); continue; }
+        seen.push_back(r);
+        // warm-up: first graph captures and first contact update out of the timing
+        st = mel_euler_steps(c, std::min(steps, 20));
+        if (st == MEL_OK) st = mel_synchronize(c);
+        auto t0 = std::chrono::steady_clock::now();
+        if (st == MEL_OK) st = mel_euler_steps(c, steps);
+        if (st == MEL_OK) st = mel_synchronize(c);
+        double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+        if (st != MEL_OK) g_load_err = c->err;
+        mel_destroy(c);
+        if (st != MEL_OK) {
+            if (first_err == MEL_OK) first_err = st;
+            continue;
+        }
+        tried++;
+        if (ms < t_best) {
+            t_best = ms;
+            *best = o;
+            best->fused_step = r.fused_step;
+            best->graph_replay = r.graph_replay;
+            best->wide_window = r.wide_window;
+        }
+    }
+    if (n_tried) *n_tried = tried;
+    if (!tried) return first_err == MEL_OK ? MEL_E_UNSUPPORTED : first_err;
+    if (best_ms) *best_ms = t_best;
+    return MEL_OK;
 }
 
 mel_status mel_update_proximity(mel_ctx* c) {

@@ -53,6 +53,19 @@ class JobInfo(ctypes.Structure):
                 ("finished", ctypes.c_double), ("device_bytes", ctypes.c_size_t), ("n_nodes", ctypes.c_int)]
 
 
+class Options(ctypes.Structure):
+    """mel_options: load-time launch configuration (-1 = library default)."""
+    _fields_ = [("block_nodes", ctypes.c_int), ("fused_step", ctypes.c_int),
+                ("graph_replay", ctypes.c_int), ("wide_window", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def make_options(block_nodes=-1, fused_step=-1, graph_replay=-1, wide_window=-1):
+    return Options(block_nodes, fused_step, graph_replay, wide_window)
+
+
 _lib = None
 
 
@@ -68,6 +81,13 @@ def library():
         sigs = {
             "mel_load": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(vp)], ctypes.c_int),
             "mel_load_binary": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, P(vp)], ctypes.c_int),
+            "mel_load_ex": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(Options), P(vp)],
+                            ctypes.c_int),
+            "mel_load_binary_ex": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, P(Options), P(vp)], ctypes.c_int),
+            "mel_default_options": ([P(Options)], None),
+            "mel_get_options": ([vp, P(Options)], ctypes.c_int),
+            "mel_autotune": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(Options),
+                              P(ctypes.c_double), P(ctypes.c_int)], ctypes.c_int),
             "mel_update_proximity": ([vp], ctypes.c_int),
             "mel_euler_steps": ([vp, ctypes.c_int], ctypes.c_int),
             "mel_synchronize": ([vp], ctypes.c_int),
@@ -120,7 +140,8 @@ def library():
 
 
 def exported_symbols():
-    return ["mel_load", "mel_load_binary", "mel_update_proximity", "mel_euler_steps", "mel_synchronize",
+    return ["mel_load", "mel_load_binary", "mel_load_ex", "mel_load_binary_ex", "mel_default_options",
+            "mel_get_options", "mel_autotune", "mel_update_proximity", "mel_euler_steps", "mel_synchronize",
             "mel_get_node_field", "mel_set_node_field", "mel_get_contacts", "mel_get_neighbours",
             "mel_get_body_state", "mel_get_body_energy", "mel_get_scalars", "mel_eval_forces",
             "mel_eval_forces_noalloc", "mel_get_state_size", "mel_save_state", "mel_load_state",
@@ -136,19 +157,28 @@ def exported_symbols():
 class Context:
     """One loaded model on one HIP device (owns the mel_ctx)."""
 
-    def __init__(self, path, save_index=0, precision=FP64, device=0, binary=False):
+    def __init__(self, path, save_index=0, precision=FP64, device=0, binary=False, options=None):
         lib = library()
         h = ctypes.c_void_p()
+        if isinstance(options, dict):
+            options = make_options(**options)
+        opt = ctypes.byref(options) if options is not None else None
         if binary:
-            st = lib.mel_load_binary(os.fsencode(path), precision, device, ctypes.byref(h))
+            st = lib.mel_load_binary_ex(os.fsencode(path), precision, device, opt, ctypes.byref(h))
         else:
-            st = lib.mel_load(os.fsencode(path), save_index, precision, device, ctypes.byref(h))
+            st = lib.mel_load_ex(os.fsencode(path), save_index, precision, device, opt, ctypes.byref(h))
         if st != MEL_OK:
             raise MelError(st, lib.mel_last_error(None).decode())
         self._h = h
         self.lib = lib
         self._owner = None
         self.info = self.get_info()
+
+    def options(self):
+        """The launch configuration this context runs with (mel_get_options)."""
+        o = Options()
+        self._check(self.lib.mel_get_options(self._h, ctypes.byref(o)))
+        return o.as_dict()
 
     @classmethod
     def _borrowed(cls, handle, owner):
@@ -409,3 +439,18 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+def autotune(binary_path, precision=FP64, device=0, steps=100):
+    """Time every candidate launch configuration of a binary model (mel_autotune).
+
+    Returns (options dict, ms per step, configurations timed)."""
+    lib = library()
+    best = Options()
+    ms = ctypes.c_double()
+    n = ctypes.c_int()
+    st = lib.mel_autotune(os.fsencode(binary_path), precision, device, steps, ctypes.byref(best),
+                          ctypes.byref(ms), ctypes.byref(n))
+    if st != MEL_OK:
+        raise MelError(st, lib.mel_last_error(None).decode())
+    return best.as_dict(), ms.value, n.value

@@ -638,6 +638,82 @@ def tiny_free(h=0.45e-3):
     return m
 
 
+def _template_grain(job):
+    """A grain of radius R centred at the origin with about n nodes (MLS,
+    seeded by n): translated copies stand for every grain of that node count
+    (shape functions depend on relative positions only)."""
+    n, R, rho = job
+    rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence(7000 + n)))
+    h = R * 0.108 * math.sqrt(200.0 / n)
+    nb = int(round(2 * math.pi * R / h))
+    bxy, ixy = disc_nodes(rng, 0.0, 0.0, R, nb, h)
+    return deformable_body(bxy, ixy, "Grain", h, rho, detection=0.5 * h, contact=0.5 * h)
+
+
+def _translated(body, dx, dy, material):
+    import copy
+    b = copy.copy(body)
+    b.X = body.X + np.array([dx, dy])
+    b.gp_xy = body.gp_xy + np.array([dx, dy])
+    b.material = material
+    return b
+
+
+def c4_batch(n_samples=512, seed=1004, period=20.0e-3, R=1.0e-3, grains=(10, 200), nodes=(100, 300),
+             mu=(0.2, 0.6), shear=0.05, gap=2.0e-3, t=0.5e-3, only=None, dt=None, workers=None):
+    """Independent periodic shear cells in one model: sample s has
+    U{grains} grains of radius R (C1's material and contact) deposited into a
+    cell of width `period` (_deposit), a rigid 'Periodic' wall under them
+    (fixed) and one on top of the highest grain, driven along x at `shear`
+    m/s; its Coulomb friction is U[mu] (a grain and a wall material of its
+    own carry the sample's laws).  Samples are stacked along y, `gap` apart,
+    so no two samples' borders ever come within detection distance: in one
+    context they run as a batch, each exactly as it would alone.  Grains
+    with the same node count (U{nodes}) are translated copies of one
+    template grain.  `only` = s builds sample s alone at the same place
+    (with `dt` the batch's step, for comparing a sample alone and in the
+    batch)."""
+    ss = np.random.SeedSequence(seed).spawn(n_samples)
+    plan = []
+    y0 = 0.0
+    for s in range(n_samples):
+        rng = np.random.Generator(np.random.PCG64(ss[s]))
+        ng = int(rng.integers(grains[0], grains[1] + 1))
+        nn = rng.integers(nodes[0], nodes[1] + 1, ng)
+        mus = float(rng.uniform(mu[0], mu[1]))
+        pile = _deposit(rng, ng, R, R, period)
+        H = max(y + r for _, y, r in pile)
+        plan.append((y0 + t, pile, nn, mus, H))
+        y0 += H + 2 * t + gap
+    need = sorted({int(n) for (_, _, nn, _, _) in plan for n in nn})
+    tmpl = dict(zip(need, _map(_template_grain, [(n, R, RHO_GRAIN) for n in need], workers)))
+    bodies, mats, laws = [], [], []
+    for s, (base, pile, nn, mus, H) in enumerate(plan):
+        if only is not None and s != only:
+            continue
+        g, w = "G%d" % s, "W%d" % s
+        mats += [(g, "NeoHookean", [RHO_GRAIN, 0.0, 0.0, E_GRAIN, NU_GRAIN, 1.0]),
+                 (w, "ElasticLinear", [7800.0, 0.0, 0.0, 2e11, 0.3, 1.0])]
+        laws += [(g, g, "MohrCoulomb", "Evolutive", [1e13, 1e13, mus, 0.0, 0.0]),
+                 (g, w, "MohrCoulomb", "Evolutive", [1e13, 1e13, mus, 0.0, 0.0])]
+        for (x, y, _), n in zip(pile, nn):
+            bodies.append(_translated(tmpl[int(n)], x, base + y, g))
+        hw = min(tmpl[int(n)].nodal_distance for n in nn)
+        fixed = lambda: BC("Driven", [(0.0, 0.0), (1e3, 0.0)])
+        bodies.append(rigid_periodic_wall(0.0, period, base - t, base, w, hw, 7800.0, detection=0.5 * hw,
+                                          contact=0.5 * hw, xbc=fixed(), ybc=fixed()))
+        bodies.append(rigid_periodic_wall(0.0, period, base + H, base + H + t, w, hw, 7800.0, detection=0.5 * hw,
+                                          contact=0.5 * hw, xbc=BC("Driven", [(0.0, shear), (1e3, shear)]),
+                                          ybc=fixed(), bc_border=0))
+    m = Model(name="c4_shear_%d%s" % (n_samples, "" if only is None else "_s%d" % only), materials=mats,
+              contact_laws=laws, bodies=bodies, periodic=(0.0, period))
+    m.dt = dt if dt is not None else _critical_dt(m, 1e13)
+    m.contact_update_period = 10 * m.dt
+    m.tend = 10000 * m.dt
+    m.samples = [(int(len(p[1])), int(np.sum(p[2])), p[3]) for p in plan]
+    return m
+
+
 def _set_law(m, typ, params, evolution="Evolutive"):
     m.contact_laws = [(a, b, typ, evolution, list(params)) for (a, b, _, _, _) in m.contact_laws]
     return m

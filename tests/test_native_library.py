@@ -73,3 +73,17 @@ def test_server_entry_points_check_arguments():
     assert lib.mel_server_create(0, 0, 10, ctypes.byref(h)) == 1      # no budget
     assert lib.mel_server_create(0, 1 << 30, 0, ctypes.byref(h)) == 1  # no slice
     lib.mel_server_destroy(None)
+
+
+def test_load_options_are_checked_before_the_device():
+    """A node-block size outside 1..256 is an argument error, reported before
+    the model is read or any device call is made."""
+    import pytest
+    for bn in (0, 257, 100000):
+        with pytest.raises(native.MelError) as e:
+            native.Context("unused.bin", binary=True, options={"block_nodes": bn})
+        assert e.value.status == 1, bn
+    o = native.Options()
+    native.library().mel_default_options(ctypes.byref(o))
+    assert o.as_dict() == {"block_nodes": -1, "fused_step": -1, "graph_replay": -1, "wide_window": -1}
+    assert native.library().mel_autotune(None, 0, 0, 10, None, None, None) == 1
