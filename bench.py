@@ -25,6 +25,7 @@ import argparse
 import json
 import math
 import os
+import shutil
 import sys
 import tempfile
 import time
@@ -104,13 +105,13 @@ def cpu_baseline(model_path, n_nodes, steps, threads):
     return n_nodes * steps / sec.value, sec.value
 
 
-def timed_run(native, path, precision, args, tag, prepare=None):
+def timed_run(native, path, precision, args, tag, prepare=None, options=None):
     """Load, warm up, then two windows of args.steps steps: the value window
     (steps as a user runs them) and the launch-duration window (HIP events
     around the dominant kernels on the library's stream)."""
     name = "fp64" if precision == native.FP64 else "fp32-mixed"
     t0 = time.time()
-    ctx = native.Context(path, binary=True, precision=precision)
+    ctx = native.Context(path, binary=True, precision=precision, options=options)
     info = ctx.get_info()
     log("[%s %s] loaded in %.1fs: nodes %d, gauss %d, nnz %d, contacts %d" % (
         tag, name, time.time() - t0, info.n_nodes, info.n_gauss, info.nnz_gauss, info.n_contacts))
@@ -406,6 +407,49 @@ def serve_leg(args, native, tmpdir):
     return out
 
 
+def batch_leg(args, native, tmpdir):
+    """independent periodic shear cells of uneven sizes in one
+    context (every step of every sample in the same launches), FP32-mixed,
+    launch configuration chosen by the autotuner on the batch's first
+    samples; the default configuration timed beside it."""
+    from melody2d_amd.preprocessor import configs
+    from melody2d_amd.preprocessor.binary import write_binary
+    t0 = time.time()
+    sub = configs.c4_batch(n_samples=args.batch_tune_samples)
+    sub_path = os.path.join(tmpdir, "c4_tune.bin")
+    write_binary(sub, sub_path)
+    best, tune_ms, tried = native.autotune(sub_path, precision=native.FP32_MIXED, steps=100)
+    os.remove(sub_path)
+    log("[batch] autotuned on %d samples (%d nodes): %s, %.4f ms/step, %d configurations (%.1fs)" % (
+        args.batch_tune_samples, sub.n_nodes, json.dumps(best), tune_ms, tried, time.time() - t0))
+    t0 = time.time()
+    m = configs.c4_batch(n_samples=args.batch_samples)
+    path = os.path.join(tmpdir, "c4.bin")
+    write_binary(m, path)
+    log("[batch] generated %d samples, %d bodies, %d nodes, %d Gauss points in %.1fs" % (
+        args.batch_samples, len(m.bodies), m.n_nodes, m.n_gauss, time.time() - t0))
+    tag = "c4_%d" % args.batch_samples
+    ctx, info, value, sec, roof, bd = timed_run(native, path, native.FP32_MIXED, args, tag, options=best)
+    used = ctx.options()
+    ctx.close()
+    ctx, _, value_def, sec_def, _, _ = timed_run(native, path, native.FP32_MIXED, args, tag + "_default")
+    ctx.close()
+    os.remove(path)
+    grains = [g for g, _, _ in m.samples]
+    out = {"workload": "%d independent periodic shear cells in one context, U{10..200} grains of U{100..300} "
+                       "MLS nodes, Coulomb mu U[0.2, 0.6] per sample, top wall sheared at 0.05 m/s" % args.batch_samples,
+           "samples": args.batch_samples, "grains": int(sum(grains)), "nodes": info.n_nodes,
+           "gauss_points": info.n_gauss, "support_entries": info.nnz_gauss, "contacts": info.n_contacts,
+           "value": round(value, 1), "unit": "node-steps/s", "ms_per_step": round(1e3 * sec / args.steps, 4),
+           "dtype": "f32 gradients/stresses/contributions, f64 state, contacts and sums",
+           "tuned_options": used, "tuned_on_samples": args.batch_tune_samples,
+           "default_options_value": round(value_def, 1),
+           "tuned_speedup_vs_default": round(value / value_def, 3), "roofline": roof}
+    if args.breakdown:
+        out["kernel_us"] = bd
+    return out
+
+
 def run_serve_child(args):
     """The serving leg in a child process of its own (its contexts, streams
     and graphs come and go by the dozen), so that whatever happens there
@@ -449,6 +493,10 @@ def main():
                     help="steps both forms run from that state for the displacement and energy errors")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (scale reference for HBM byte counters)")
+    ap.add_argument("--batch-samples", type=int, default=64,
+                    help="shear cells of the batched leg (0: skip it; the configuration's full batch is 512)")
+    ap.add_argument("--batch-tune-samples", type=int, default=16,
+                    help="samples the autotuner times candidate configurations on")
     ap.add_argument("--serve-jobs", type=int, default=12, help="jobs of the serving leg (0: skip it)")
     ap.add_argument("--serve-steps", type=int, default=20000)
     ap.add_argument("--serve-nodes", type=int, nargs=2, default=[5000, 500000])
@@ -499,7 +547,14 @@ def main():
         except Exception as e:   # the headline line is printed whatever happens here
             log("plate leg failed: %r" % (e,))
             plate = {"error": repr(e)}
-    os.rmdir(tmpdir)
+    batch = None
+    if args.batch_samples > 0:
+        try:
+            batch = batch_leg(args, native, tmpdir)
+        except Exception as e:
+            log("batch leg failed: %r" % (e,))
+            batch = {"error": repr(e)}
+    shutil.rmtree(tmpdir, ignore_errors=True)
     serving = run_serve_child(args) if args.serve_jobs > 0 else None
 
     info = g["info"]
@@ -534,6 +589,8 @@ def main():
     }
     if plate:
         out["plate"] = plate
+    if batch:
+        out["batch"] = batch
     if serving:
         out["serving"] = serving
     if g["bd"]:

@@ -80,7 +80,8 @@ __device__ __forceinline__ void step_forces(DevModel& m, const BlockArrays& B, c
                                             typename PR::slot* s_cxy) {
     const int t = threadIdx.x;
     int w = blockIdx.x;
-    if (w < S.n_cwg) {
+    if (w >= B.n_blocks) {
+        w -= B.n_blocks;
         // launch B of the previous step has ended: its counter starts again at zero
         if (w == 0 && t == 0) __hip_atomic_store(&m.step_sync[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         // the rigid bodies' driven DOFs at the end of the step (time and BC
@@ -99,7 +100,6 @@ __device__ __forceinline__ void step_forces(DevModel& m, const BlockArrays& B, c
         }
         return;
     }
-    w -= S.n_cwg;
     double fi[2][NODES_PER_THREAD];
     block_force_update<PR>(m, B, w, s_cxy, fi);
 }

@@ -579,6 +579,13 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_
                 C.nbint[e] = 0; C.int0[e] = 0.;
                 C.vgapn[e] = 0.; C.vgapt[e] = 0.; C.wacc[e] = 0.;
                 C.fresh[e] = ri.self ? 2 : 1;
+                int4 inf, sinf;
+                element_info_from(m, bS, slotS, bM, s0, inf, sinf);
+                // the stencil written above is the slot's (load_stencil)
+                // except the shifts of a self element
+                if (!ri.self) inf.z |= CI_STENCIL;
+                C.info[e] = inf;
+                C.sinfo[e] = sinf;
             }
         }
         cnt++;
@@ -675,6 +682,7 @@ __device__ inline void copy_element(const ContactArrays& D, int e, const Contact
     D.gapn[e] = S.gapn[o]; D.vgapn[e] = S.vgapn[o]; D.gapt[e] = S.gapt[o]; D.vgapt[e] = S.vgapt[o];
     D.xsi[e] = S.xsi[o]; D.xn[e] = S.xn[o]; D.yn[e] = S.yn[o]; D.xt[e] = S.xt[o]; D.yt[e] = S.yt[o];
     D.emass[e] = S.emass[o]; D.len[e] = S.len[o]; D.fx[e] = S.fx[o]; D.fy[e] = S.fy[o]; D.int0[e] = S.int0[o];
+    D.info[e] = S.info[o]; D.sinfo[e] = S.sinfo[o];
 }
 
 // history transfer: first previous element of the same slave slot with the

@@ -70,7 +70,9 @@ __global__ void k_rigid_nodes(DevModel m, int with_forces) {
 //                     lengths, border-node kinematics | rigid nodes |
 //                     Update_bc(t + dt) | time = t + dt
 // ---------------------------------------------------------------------------
-__global__ void k_step_contacts(DevModel m, const int* bc_bn, int n_bc) {
+// (the element path needs ~230 registers: two waves per SIMD keep it out of scratch)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_step_contacts(DevModel m, const int* bc_bn, int n_bc) {
     const int nc = *m.c_count;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc + n_bc; i += gridDim.x * blockDim.x)
         contact_or_bc_item(m, i, nc, bc_bn, n_bc);
@@ -201,7 +203,8 @@ __global__ void __launch_bounds__(BLOCK) k_body_energy(DevModel m, double* out) 
 // when do_bc); then k_bucket_sums gathers the contact point forces of every
 // border node.
 // ---------------------------------------------------------------------------
-__global__ void k_eval_items(DevModel m, const int* bc_bn, int n_bc, int do_contacts, int do_bc) {
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_eval_items(DevModel m, const int* bc_bn, int n_bc, int do_contacts, int do_bc) {
     const int nc = do_contacts ? *m.c_count : 0;
     const int nb = do_bc ? n_bc : 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc + nb; i += gridDim.x * blockDim.x)

@@ -96,9 +96,9 @@ __device__ inline void law_czm_linear(double kini, double Pnlim, double gapnlim,
     Pt = -kmax * gapt;
 }
 
-// a point-force record (id r, force) appended to border node bn's bucket
-__device__ inline void bucket_append(const DevModel& m, int bn, int r, double fx, double fy) {
-    int pos = atomicAdd(&m.bk_cursor[bn], 1);
+// a point-force record (id r, force) stored at position pos of border node
+// bn's bucket, and appended to it
+__device__ inline void bucket_store(const DevModel& m, int bn, int pos, int r, double fx, double fy) {
     if (pos < BUCKET_CAP) {
         m.bk_rec[bn * BUCKET_CAP + pos] = r;
         m.bk_val[bn * BUCKET_CAP + pos] = make_double2(fx, fy);
@@ -107,25 +107,58 @@ __device__ inline void bucket_append(const DevModel& m, int bn, int r, double fx
     }
 }
 
-// Body::Update_contact_forces (Code/Body.h:1642-1941) minus the scatter, for
-// one element after its re-projection (Body::Update_contacts,
-// Code/Body.h:1341-1634): the element's force and its point-force records
-__device__ inline void contact_law_element(const DevModel& m, int e) {
+__device__ inline void bucket_append(const DevModel& m, int bn, int r, double fx, double fy) {
+    bucket_store(m, bn, atomicAdd(&m.bk_cursor[bn], 1), r, fx, fy);
+}
+
+// a deformable slot's current tributary length from its stencil record
+// (slot_length's expressions)
+__device__ __forceinline__ double slot_length_stencil(const DevModel& m, int4 st) {
+    const double P = m.period;
+    const int w = st.w;
+    const double x0 = m.xc[st.x] + P * ((w & 3) - 1), y0 = m.yc[st.x];
+    const double x1 = m.xc[st.y] + P * (((w >> 2) & 3) - 1), y1 = m.yc[st.y];
+    const double x2 = m.xc[st.z] + P * (((w >> 4) & 3) - 1), y2 = m.yc[st.z];
+    const double l21 = __dsqrt_rn((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
+    const double l10 = __dsqrt_rn((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
+    const int end = (w >> 6) & 3;
+    if (end == 1) return l21 * 0.5;
+    if (end == 2) return l10 * 0.5;
+    return (l21 + l10) * 0.5;
+}
+
+// Body::Update_contacts (project_core) then Body::Update_contact_forces
+// (Code/Body.h:1642-1941) minus the scatter, for one element: its force and
+// its point-force records.  Everything the element reads is one load away
+// from its own arrays (element_info), the slave's bucket slot is reserved
+// before the projection, and the projected state stays in registers for the
+// law.
+__device__ __forceinline__ void contact_item(const DevModel& m, int e) {
     const ContactArrays& C = m.C;
+    int4 inf, sinf;
+    element_info_ld(m, C, e, inf, sinf);
     m.rec_bn[3 * e] = -1; m.rec_bn[3 * e + 1] = -1; m.rec_bn[3 * e + 2] = -1;
     for (int q = 0; q < 6; q++) m.rig_rec[6 * e + q] = 0.;
-    int bS = C.bodyS[e];
-    if (m.body_status[bS] == 1) return;
-    int bM = C.bodyM[e];
-    int li = m.law_table[m.body_material[bS] * m.n_materials + m.body_material[bM]];
+    const int fl = inf.z;
+    if (fl & CI_INACTIVE_S) return;
+    const int li = (fl & CI_LAW_MASK) - 1;
+    // loads that depend only on the element's info
+    int posS = 0;
+    if (li >= 0 && !(fl & CI_RIGID_S)) posS = atomicAdd(&m.bk_cursor[inf.y], 1);
+    const double emass = C.emass[e], int0 = C.int0[e];
+    const int nbint = C.nbint[e];
+    const int bS = C.bodyS[e], bM = C.bodyM[e];
+    ElemProj r;
+    project_core(m, C, e, inf, r);
+    project_store(C, e, inf, r);
     if (li < 0) return;
-    int slot = slot_of(m, bS, C.borderS[e], C.bnodeS[e]);
-    double length = slot_len_now(m, slot);
-    double length0 = m.sl_len0[slot];
-    double gapn = C.gapn[e], gapt = C.gapt[e];
+    const int slot = inf.x;
+    const double length = (fl & CI_RIGID_S) ? m.sl_len[slot] : slot_length_stencil(m, sinf);
+    const double length0 = m.sl_len0[slot];
+    double gapn = r.p.gap, gapt = r.gapt;
     const double* p = m.law_p + 12 * li;
     double Pn = 0., Pt = 0., W = 0.;
-    int lt = m.law_type[li];
+    const int lt = m.law_type[li];
     if (lt == 0) {                      // Frictionless
         Pn = gapn > 0. ? 0. : -p[0] * gapn;
         Pt = 0.;
@@ -140,7 +173,7 @@ __device__ inline void contact_law_element(const DevModel& m, int e) {
         law_mohr_coulomb(p[0], p[1], p[2], p[3] * 0.5, p[4] * 0.5, gapn, gapt, Pn, Pt);
     } else if (lt == 3) {               // DampedMohrCoulomb
         double kn = p[0], kt = p[1], fric = p[2], coh = p[3] * 0.5, tens = p[4] * 0.5, damp = p[5];
-        double mass = C.emass[e], L0 = length0, vgn = C.vgapn[e], vgt = C.vgapt[e];
+        double mass = emass, L0 = length0, vgn = r.vgapn, vgt = r.vgapt;
         Pn = 0.;
         if (gapn < 0.) { Pn = -damp * vgn * 2. * __dsqrt_rn(mass * kn / L0); W += Pn * vgn; }
         Pn -= kn * gapn;
@@ -156,49 +189,48 @@ __device__ inline void contact_law_element(const DevModel& m, int e) {
         }
     } else if (lt == 4) {               // TwoSlopesMohrCoulomb
         double kn = p[0], kt = p[1], fric = p[2], coh = p[3] * 0.5, tens = p[4] * 0.5, rati = p[5];
-        Pn = C.vgapn[e] <= 0. ? -kn * gapn : -kn * gapn * rati;
+        Pn = r.vgapn <= 0. ? -kn * gapn : -kn * gapn * rati;
         if (Pn < -tens) { Pn = 0.; Pt = 0.; gapt = 0.; }
         else {
             Pt = -kt * gapt;
             if (fabs(Pt) > coh + fric * Pn) { Pt = gapt > 0. ? -coh - fric * Pn : coh + fric * Pn; gapt = -Pt / kt; }
         }
     } else if (lt == 6) {               // CZMlinear
-        double D = C.int0[e];
-        if (C.nbint[e] == 0) { D = 1.; C.nbint[e] = 1; }
+        double D = int0;
+        if (nbint == 0) { D = 1.; C.nbint[e] = 1; }
         law_czm_linear(p[0], p[1] * 0.5, p[2], p[3] * 0.5, p[4], gapn, gapt, D, Pn, Pt);
         C.int0[e] = D;
     }
-    double Px = C.xn[e] * Pn + C.xt[e] * Pt;
-    double Py = C.yn[e] * Pn + C.yt[e] * Pt;
-    double L = m.law_fixed[li] ? length0 : length;
-    double Fx = Px * L, Fy = Py * L;
+    const double Px = r.p.xn * Pn + r.p.xt * Pt;
+    const double Py = r.p.yn * Pn + r.p.yt * Pt;
+    const double L = m.law_fixed[li] ? length0 : length;
+    const double Fx = Px * L, Fy = Py * L;
     W *= L * m.dt;
     C.gapn[e] = gapn; C.gapt[e] = gapt; C.len[e] = length; C.fx[e] = Fx; C.fy[e] = Fy;
     // Code/Body.h:1847-1848 add 0.5 W to the slave's and the master's
     // contact work; the element keeps its half until the works are flushed
     if (W != 0.) C.wacc[e] += 0.5 * W;
-    int nodeS = C.nodeS[e];
-    if (!m.body_rigid_flag[bS]) {
-        int bn = m.sl_bn[slot];
+    if (!(fl & CI_RIGID_S)) {
+        const int bn = inf.y;
         m.rec_bn[3 * e] = bn; m.rec_fx[3 * e] = Fx; m.rec_fy[3 * e] = Fy;
-        bucket_append(m, bn, 3 * e, Fx, Fy);
+        bucket_store(m, bn, posS, 3 * e, Fx, Fy);
     } else {
         const double* R = m.body_rigid + bS * RIGID_FIELDS;
-        double dx = m.xc[nodeS] - R[R_CUR], dy = m.yc[nodeS] - R[R_CUR + 1];
+        double dx = r.xs - R[R_CUR], dy = r.ys - R[R_CUR + 1];
         m.rig_rec[6 * e] = Fx; m.rig_rec[6 * e + 1] = Fy; m.rig_rec[6 * e + 2] = -Fx * dy + Fy * dx;
     }
-    if (!m.body_rigid_flag[bM]) {
-        int sl = slot_of(m, bM, C.borderM[e], C.segM[e]);
-        int bn1 = m.sl_bn[sl], bn2 = m.sl_bn[sl + 1];
-        const double f1x = -Fx * C.N1[e], f1y = -Fy * C.N1[e], f2x = -Fx * C.N2[e], f2y = -Fy * C.N2[e];
+    if (!(fl & CI_RIGID_M)) {
+        const int sl = inf.w + r.segM;
+        const int bn1 = m.sl_bn[sl], bn2 = m.sl_bn[sl + 1];
+        const double f1x = -Fx * r.p.N1, f1y = -Fy * r.p.N1, f2x = -Fx * r.p.N2, f2y = -Fy * r.p.N2;
         m.rec_bn[3 * e + 1] = bn1; m.rec_fx[3 * e + 1] = f1x; m.rec_fy[3 * e + 1] = f1y;
         m.rec_bn[3 * e + 2] = bn2; m.rec_fx[3 * e + 2] = f2x; m.rec_fy[3 * e + 2] = f2y;
         bucket_append(m, bn1, 3 * e + 1, f1x, f1y);
         bucket_append(m, bn2, 3 * e + 2, f2x, f2y);
     } else {
         const double* R = m.body_rigid + bM * RIGID_FIELDS;
-        double dx = m.xc[nodeS] - R[R_CUR] - C.shiftM[e] * m.period;
-        double dy = m.yc[nodeS] - R[R_CUR + 1];
+        double dx = r.xs - R[R_CUR] - r.shiftM * m.period;
+        double dy = r.ys - R[R_CUR + 1];
         m.rig_rec[6 * e + 3] = -Fx; m.rig_rec[6 * e + 4] = -Fy; m.rig_rec[6 * e + 5] = Fx * dy - Fy * dx;
     }
 }
@@ -703,8 +735,7 @@ __device__ inline void rigid_node(const DevModel& m, int i, int with_forces) {
 // node bc_bn[i - nc] after them
 __device__ inline void contact_or_bc_item(const DevModel& m, int i, int nc, const int* bc_bn, int n_bc) {
     if (i < nc) {
-        if (m.body_status[m.C.bodyS[i]] != 1) project_element(m, m.C, i);
-        contact_law_element(m, i);
+        contact_item(m, i);
         return;
     }
     i -= nc;

@@ -112,6 +112,8 @@ __global__ void k_contacts_scratch(ContactArrays A, unsigned char* buf, int nc, 
         int* q = bi + (size_t)k * SCRATCH_MAX_CONTACTS + e;
         if (to_buf) *q = ip[k][e]; else ip[k][e] = *q;
     }
+    // the restored element's info is recomputed from its restored indices
+    if (!to_buf) A.info[e] = make_int4(0, 0, 0, 0);
 }
 
 // the device address of `scratch` on `device` (current), taken once per
@@ -286,6 +288,8 @@ void alloc_contacts(mel_ctx& c, ContactArrays& A, int cap) {
     double** dbl[] = {&A.N0, &A.N1, &A.N2, &A.N3, &A.gapn, &A.vgapn, &A.gapt, &A.vgapt, &A.xsi, &A.xn, &A.yn,
                       &A.xt, &A.yt, &A.emass, &A.len, &A.fx, &A.fy, &A.int0, &A.wacc};
     for (double** p : dbl) *p = c.alloc<double>(cap);
+    A.info = c.alloc<int4>(cap);
+    A.sinfo = c.alloc<int4>(cap);
 }
 
 // Body::Update_material masses (Code/Body.h:2751-2783) and rigid mass and
@@ -1067,6 +1071,8 @@ void upload_loaded_contacts(mel_ctx& c) {
             HIP_OK(hipMemcpyAsync(ip[k], iv[k].data(), n * sizeof(int), hipMemcpyHostToDevice, c.stream));
             HIP_OK(hipMemcpyAsync(dp[k], dv[k].data(), n * sizeof(double), hipMemcpyHostToDevice, c.stream));
         }
+        // computed from the indices by the first step that reads them
+        HIP_OK(hipMemsetAsync(A.info, 0, n * sizeof(int4), c.stream));
     }
     HIP_OK(hipMemcpyAsync(d.c_count, &n, sizeof(int), hipMemcpyHostToDevice, c.stream));
     HIP_OK(hipStreamSynchronize(c.stream));
@@ -1855,6 +1861,8 @@ void copy_contacts(mel_ctx& c, const ContactArrays& dst, const ContactArrays& sr
                            &src.int0, &src.wacc};
     for (size_t k = 0; k < sizeof(dd) / sizeof(dd[0]); k++)
         HIP_OK(hipMemcpyAsync(*dd[k], *sd[k], nc * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    HIP_OK(hipMemcpyAsync(dst.info, src.info, nc * sizeof(int4), hipMemcpyDeviceToDevice, c.stream));
+    HIP_OK(hipMemcpyAsync(dst.sinfo, src.sinfo, nc * sizeof(int4), hipMemcpyDeviceToDevice, c.stream));
 }
 
 // One force evaluation at the current state, without integration
@@ -1944,6 +1952,8 @@ std::vector<StateArray> state_arrays(mel_ctx& c, int nc) {
     double* cd[] = {A.N0, A.N1, A.N2, A.N3, A.gapn, A.vgapn, A.gapt, A.vgapt, A.xsi, A.xn, A.yn, A.xt, A.yt,
                     A.emass, A.len, A.fx, A.fy, A.int0, A.wacc};
     for (double* p : cd) v.push_back({p, (size_t)nc * sizeof(double)});
+    v.push_back({A.info, (size_t)nc * sizeof(int4)});
+    v.push_back({A.sinfo, (size_t)nc * sizeof(int4)});
     return v;
 }
 

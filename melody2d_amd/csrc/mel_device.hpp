@@ -39,7 +39,22 @@ struct ContactArrays {
     // into its bodies' work (one thread per element adds, so the sums do not
     // depend on scheduling)
     double* wacc;
+    // what the element's step reads first (element_info), so that its
+    // dependent loads are one level deep:
+    //   info   slave slot, slave border node (-1: rigid slave), law index + 1
+    //          (0: none) | CI_* flags, first slot of the master border
+    //   sinfo  the slave slot's length stencil: nodes 0, 1, 2; shifts + 1
+    //          (2 bits each) | end class << 6 (1: first slot of a Simple
+    //          border, 2: last one)
+    int4 *info, *sinfo;
 };
+
+constexpr int CI_LAW_MASK = (1 << 24) - 1;
+constexpr int CI_VALID = 1 << 24;       // info and sinfo hold the element's values
+constexpr int CI_STENCIL = 1 << 25;     // nM0-3 / sM0-3 are the stencil of (borderM, segM)
+constexpr int CI_RIGID_S = 1 << 26;
+constexpr int CI_RIGID_M = 1 << 27;
+constexpr int CI_INACTIVE_S = 1 << 28;  // slave body of status 1
 
 struct DevModel {
     // sizes
@@ -309,22 +324,70 @@ __device__ inline void load_stencil(const DevModel& m, int body, int border, int
     sh[0] = m.sl_s0[s]; sh[1] = m.sl_s1[s]; sh[2] = m.sl_s2[s]; sh[3] = m.sl_s3[s];
 }
 
-__device__ inline void project_element(const DevModel& m, const ContactArrays& C, int e) {
-    int bS = C.bodyS[e];
-    int bodyM = C.bodyM[e], borderM = C.borderM[e], shiftM = C.shiftM[e], segM = C.segM[e];
-    int nodeS = C.nodeS[e];
-    int gbM = m.border_off[bodyM] + borderM;
-    int per = m.br_periodicity[gbM];
+// info / sinfo of an element from its slave body and slot, master body and
+// the first slot of its master border
+__device__ inline void element_info_from(const DevModel& m, int bS, int slot, int bM, int slot0M, int4& inf,
+                                         int4& sinf) {
+    const int li = m.law_table[m.body_material[bS] * m.n_materials + m.body_material[bM]];
+    const int rS = m.body_rigid_flag[bS];
+    int fl = CI_VALID | (li + 1);
+    if (rS) fl |= CI_RIGID_S;
+    if (m.body_rigid_flag[bM]) fl |= CI_RIGID_M;
+    if (m.body_status[bS] == 1) fl |= CI_INACTIVE_S;
+    inf = make_int4(slot, rS ? -1 : m.sl_bn[slot], fl, slot0M);
+    const int br = m.sl_border[slot], j = m.sl_j[slot], n = m.br_nslots[br];
+    const bool simple = m.br_periodicity[br] == 1;
+    const int end = (j == 0 && simple) ? 1 : ((j == n - 1 && simple) ? 2 : 0);
+    sinf = make_int4(m.sl_n0[slot], m.sl_n1[slot], m.sl_n2[slot],
+                     (m.sl_s0[slot] + 1) | (m.sl_s1[slot] + 1) << 2 | (m.sl_s2[slot] + 1) << 4 | end << 6);
+}
+
+// the same from the element's indices (the loads they save)
+__device__ inline void element_info(const DevModel& m, const ContactArrays& C, int e, int4& inf, int4& sinf) {
+    const int bS = C.bodyS[e], bM = C.bodyM[e];
+    element_info_from(m, bS, slot_of(m, bS, C.borderS[e], C.bnodeS[e]), bM, slot_of(m, bM, C.borderM[e], 0), inf,
+                      sinf);
+}
+
+// an element's info, computed and stored when it is not there yet
+__device__ __forceinline__ void element_info_ld(const DevModel& m, const ContactArrays& C, int e, int4& inf, int4& sinf) {
+    inf = C.info[e];
+    sinf = C.sinfo[e];
+    if (!(inf.z & CI_VALID)) {
+        element_info(m, C, e, inf, sinf);
+        C.info[e] = inf;
+        C.sinfo[e] = sinf;
+    }
+}
+
+// the re-projected element (project_core), held in registers for its law
+struct ElemProj {
+    int borderM, shiftM, segM, nodeS;
+    int nd0, nd1, nd2, nd3, sh0, sh1, sh2, sh3;   // stencil of the final segment
+    double xs, ys;            // slave node position
+    double gapt, vgapn, vgapt;
+    Proj p;
+};
+
+// Body::Update_contacts for one element, from its info (whose master-border
+// slot and stencil flag it updates)
+__device__ __forceinline__ void project_core(const DevModel& m, const ContactArrays& C, int e, int4& inf, ElemProj& r) {
+    const int bS = C.bodyS[e];
+    const int bodyM = C.bodyM[e];
+    int borderM = C.borderM[e], shiftM = C.shiftM[e], segM = C.segM[e];
+    const int nodeS = C.nodeS[e];
     Proj p;
     p.gap = C.gapn[e]; p.xsi = C.xsi[e]; p.xn = C.xn[e]; p.yn = C.yn[e]; p.xt = C.xt[e]; p.yt = C.yt[e];
     p.N0 = C.N0[e]; p.N1 = C.N1[e]; p.N2 = C.N2[e]; p.N3 = C.N3[e];
     p.flag = 0;
     double gapt = C.gapt[e];
-    double xsi_prev = p.xsi;
+    const double xsi_prev = p.xsi;
     int nd[4], sh[4];
-    load_stencil(m, bodyM, borderM, segM, nd, sh);
-    double xs = m.xc[nodeS], ys = m.yc[nodeS];
-    double P = m.period;
+    nd[0] = C.nM0[e]; nd[1] = C.nM1[e]; nd[2] = C.nM2[e]; nd[3] = C.nM3[e];
+    sh[0] = C.sM0[e]; sh[1] = C.sM1[e]; sh[2] = C.sM2[e]; sh[3] = C.sM3[e];
+    if (!(inf.z & CI_STENCIL)) load_stencil(m, bodyM, borderM, segM, nd, sh);
+    const double xs = m.xc[nodeS], ys = m.yc[nodeS];
+    const double P = m.period;
     double X[4], Y[4];
     for (int k = 0; k < 4; k++) { X[k] = m.xc[nd[k]] + P * (shiftM + sh[k]); Y[k] = m.yc[nd[k]]; }
     auto project = [&]() {
@@ -335,12 +398,22 @@ __device__ inline void project_element(const DevModel& m, const ContactArrays& C
     };
     project();
     auto seglen = [&]() { return __dsqrt_rn((X[2] - X[1]) * (X[2] - X[1]) + (Y[2] - Y[1]) * (Y[2] - Y[1])); };
+    // the next stencil after a walk (slot = first slot of the master border + segment)
+    auto restencil = [&]() {
+        const int s = inf.w + segM;
+        nd[0] = m.sl_n0[s]; nd[1] = m.sl_n1[s]; nd[2] = m.sl_n2[s]; nd[3] = m.sl_n3[s];
+        sh[0] = m.sl_s0[s]; sh[1] = m.sl_s1[s]; sh[2] = m.sl_s2[s]; sh[3] = m.sl_s3[s];
+        for (int k = 0; k < 4; k++) { X[k] = m.xc[nd[k]] + P * (shiftM + sh[k]); Y[k] = m.yc[nd[k]]; }
+    };
     if (p.flag == -2 || p.flag == -3) {
+        const int gbM = m.border_off[bodyM] + borderM;
+        const int per = m.br_periodicity[gbM];
         gapt = gapt + 0.5 * (-1. - xsi_prev) * seglen();
         if (per == 1) {
             if (segM == 0) {
                 borderM = m.br_before[gbM];
                 segM = m.br_nslots[m.border_off[bodyM] + borderM] - 2;
+                inf.w = slot_of(m, bodyM, borderM, 0);
             } else segM--;
         } else if (per == 2) {
             if (segM == 0) {
@@ -352,17 +425,21 @@ __device__ inline void project_element(const DevModel& m, const ContactArrays& C
             if (segM == 0) segM = m.br_nslots[gbM] - 2;
             else segM--;
         }
-        load_stencil(m, bodyM, borderM, segM, nd, sh);
-        for (int k = 0; k < 4; k++) { X[k] = m.xc[nd[k]] + P * (shiftM + sh[k]); Y[k] = m.yc[nd[k]]; }
+        restencil();
         p.xsi = 1.;
         project();
         gapt = gapt + 0.5 * (-1. + p.xsi) * seglen();
     } else if (p.flag == 2 || p.flag == 3) {
+        const int gbM = m.border_off[bodyM] + borderM;
+        const int per = m.br_periodicity[gbM];
         gapt = gapt + 0.5 * (-1. + xsi_prev) * seglen();
-        int last = m.br_nslots[gbM] - 2;
+        const int last = m.br_nslots[gbM] - 2;
         if (per == 1) {
-            if (segM == last) { borderM = m.br_after[gbM]; segM = 0; }
-            else segM++;
+            if (segM == last) {
+                borderM = m.br_after[gbM];
+                segM = 0;
+                inf.w = slot_of(m, bodyM, borderM, 0);
+            } else segM++;
         } else if (per == 2) {
             if (segM == last) {
                 if (borderM == 0) shiftM++;
@@ -373,25 +450,49 @@ __device__ inline void project_element(const DevModel& m, const ContactArrays& C
             if (segM == last) segM = 0;
             else segM++;
         }
-        load_stencil(m, bodyM, borderM, segM, nd, sh);
-        for (int k = 0; k < 4; k++) { X[k] = m.xc[nd[k]] + P * (shiftM + sh[k]); Y[k] = m.yc[nd[k]]; }
+        restencil();
         p.xsi = -1.;
         project();
         gapt = gapt + 0.5 * (1. + p.xsi) * seglen();
     } else {
         gapt = gapt + 0.5 * (p.xsi - xsi_prev) * seglen();
     }
-    double vgx = m.vx[nodeS] - (p.N0 * m.vx[nd[0]] + p.N1 * m.vx[nd[1]] + p.N2 * m.vx[nd[2]] + p.N3 * m.vx[nd[3]]);
-    double vgy = m.vy[nodeS] - (p.N0 * m.vy[nd[0]] + p.N1 * m.vy[nd[1]] + p.N2 * m.vy[nd[2]] + p.N3 * m.vy[nd[3]]);
-    C.borderM[e] = borderM; C.shiftM[e] = shiftM; C.segM[e] = segM;
+    inf.z |= CI_STENCIL;
+    const double vgx = m.vx[nodeS] - (p.N0 * m.vx[nd[0]] + p.N1 * m.vx[nd[1]] + p.N2 * m.vx[nd[2]] + p.N3 * m.vx[nd[3]]);
+    const double vgy = m.vy[nodeS] - (p.N0 * m.vy[nd[0]] + p.N1 * m.vy[nd[1]] + p.N2 * m.vy[nd[2]] + p.N3 * m.vy[nd[3]]);
+    r.nd0 = nd[0]; r.nd1 = nd[1]; r.nd2 = nd[2]; r.nd3 = nd[3];
+    r.sh0 = sh[0]; r.sh1 = sh[1]; r.sh2 = sh[2]; r.sh3 = sh[3];
+    r.p = p;
+    r.borderM = borderM; r.shiftM = shiftM; r.segM = segM; r.nodeS = nodeS;
+    r.xs = xs; r.ys = ys;
+    r.gapt = gapt;
+    r.vgapn = vgx * p.xn + vgy * p.yn;
+    r.vgapt = vgx * p.xt + vgy * p.yt;
+}
+
+__device__ __forceinline__ void project_store(const ContactArrays& C, int e, const int4& inf, const ElemProj& r) {
+    const Proj& p = r.p;
+    C.borderM[e] = r.borderM; C.shiftM[e] = r.shiftM; C.segM[e] = r.segM;
     C.gapn[e] = p.gap;
-    C.vgapn[e] = vgx * p.xn + vgy * p.yn;
-    C.gapt[e] = gapt;
-    C.vgapt[e] = vgx * p.xt + vgy * p.yt;
+    C.vgapn[e] = r.vgapn;
+    C.gapt[e] = r.gapt;
+    C.vgapt[e] = r.vgapt;
     C.xsi[e] = p.xsi; C.xn[e] = p.xn; C.yn[e] = p.yn; C.xt[e] = p.xt; C.yt[e] = p.yt;
-    C.nM0[e] = nd[0]; C.nM1[e] = nd[1]; C.nM2[e] = nd[2]; C.nM3[e] = nd[3];
-    C.sM0[e] = sh[0]; C.sM1[e] = sh[1]; C.sM2[e] = sh[2]; C.sM3[e] = sh[3];
+    C.nM0[e] = r.nd0; C.nM1[e] = r.nd1; C.nM2[e] = r.nd2; C.nM3[e] = r.nd3;
+    C.sM0[e] = r.sh0; C.sM1[e] = r.sh1; C.sM2[e] = r.sh2; C.sM3[e] = r.sh3;
     C.N0[e] = p.N0; C.N1[e] = p.N1; C.N2[e] = p.N2; C.N3[e] = p.N3;
+    C.info[e] = inf;
+}
+
+// re-project the slave node, walk to the previous / next master segment on
+// flags +-2/+-3, accumulate the tangential gap, relative velocities.  When
+// both projections return +-3 the element keeps its own stored shape values.
+__device__ inline void project_element(const DevModel& m, const ContactArrays& C, int e) {
+    int4 inf, sinf;
+    element_info_ld(m, C, e, inf, sinf);
+    ElemProj r;
+    project_core(m, C, e, inf, r);
+    project_store(C, e, inf, r);
 }
 
 // DOF_TO_DISP gather of one node (Body::Update_kinematics, Code/Body.h:2176-2212)

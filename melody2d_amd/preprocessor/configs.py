@@ -186,11 +186,14 @@ def _deposit(rng, n, rmin, rmax, W, max_rolls=400):
                         best, sup = yy, k
         return best, sup
 
-    for _ in range(n):
+    def overlaps(x, y, r):
+        return any(math.hypot(x - xs[k], y - ys[k]) < (r + rs[k]) * (1.0 - 1e-9) for k in near(x, y, r + rmax))
+
+    def place():
         r = rmin + (rmax - rmin) * rng.random()
         x = r + (W - 2.0 * r) * rng.random()
         y, s1 = drop(x, r, math.inf)
-        rolls = 0
+        rolls, prev = 0, -1
         while s1 >= 0 and rolls < max_rolls:
             rolls += 1
             x1, y1, R1 = xs[s1], ys[s1], r + rs[s1]
@@ -210,8 +213,10 @@ def _deposit(rng, n, rmin, rmax, W, max_rolls=400):
                 a = math.asin(sv)
                 if a0 - 1e-12 < a < best_a:
                     best_a, kind = a, "wall"
+            # a disc touched where the roll starts stops it at once (the
+            # disc rests in the pocket), except the one it just rolled off
             for k in near(x1, y1, R1 + r + rmax):
-                if k == s1:
+                if k == s1 or k == prev:
                     continue
                 Rk = r + rs[k]
                 dxk, dyk = xs[k] - x1, ys[k] - y1
@@ -224,17 +229,30 @@ def _deposit(rng, n, rmin, rmax, W, max_rolls=400):
                 for q in (1.0, -1.0):
                     ix, iy = px - q * hh * dyk / D, py + q * hh * dxk / D
                     a = math.atan2(ix, iy) * sgn
-                    if a0 + 1e-12 < a < best_a:
+                    if a0 - 1e-9 < a < best_a:
                         best_a, best_k, kind = a, k, "disc"
             x, y = x1 + sgn * R1 * math.sin(best_a), y1 + R1 * math.cos(best_a)
             if kind in ("floor", "wall"):
                 break
             if kind == "fall":
                 y, s1 = drop(x, r, y)
+                prev = -1
                 continue
             if min(x1, xs[best_k]) <= x <= max(x1, xs[best_k]):
                 break
-            s1 = best_k
+            prev, s1 = s1, best_k
+        return x, y, r
+
+    for _ in range(n):
+        # with equal radii a disc can roll into a pocket another disc
+        # already fills; such a drop is drawn again
+        x, y, r = place()
+        tries = 1
+        while overlaps(x, y, r):
+            if tries == 1000:
+                raise RuntimeError("deposition found no free resting place in 1000 drops")
+            x, y, r = place()
+            tries += 1
         xs.append(x)
         ys.append(y)
         rs.append(r)

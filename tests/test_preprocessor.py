@@ -113,3 +113,21 @@ def test_plate_generator_rpim_consistency():
     # triangles tile the solid, within half a spacing along the hole edges
     # (kept whole, the holes would add 40 %)
     assert abs(np.sum(b.gp_w * b.gp_jac) - solid) < 0.05 * solid
+
+
+def test_deposit_piles_never_overlap():
+    """Sequential deposition: every disc rests inside the box on the pile and
+    no two discs overlap, with equal radii too (where a rolling disc can reach
+    a pocket another disc already fills, and the drop is drawn again)."""
+    import math
+    from melody2d_amd.preprocessor import configs
+    for seed, n, rmin, rmax, W in ((1, 119, 1e-3, 1e-3, 20e-3), (2, 71, 1e-3, 1e-3, 20e-3),
+                                   (3, 150, 0.8e-3, 1.2e-3, 15e-3)):
+        rng = np.random.default_rng(seed)
+        P = np.array(configs._deposit(rng, n, rmin, rmax, W))
+        assert len(P) == n
+        assert np.all(P[:, 0] - P[:, 2] >= -1e-12) and np.all(P[:, 0] + P[:, 2] <= W + 1e-12)
+        assert np.all(P[:, 1] - P[:, 2] >= -1e-12)
+        d = np.hypot(P[:, None, 0] - P[None, :, 0], P[:, None, 1] - P[None, :, 1])
+        gap = d - (P[:, None, 2] + P[None, :, 2]) + np.eye(n)
+        assert gap.min() > -1e-9 * rmax, (seed, gap.min())
