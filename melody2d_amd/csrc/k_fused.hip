@@ -121,7 +121,7 @@ k_step_forces32(DevModel m, BlockArrays B, StepLaunch S) {
 // two waves per SIMD (four workgroups per CU) keep a 1000-body model's
 // blocks resident at once
 __global__ void __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_step_finish(DevModel m, BlockArrays B) {
+k_step_finish(DevModel m, BlockArrays B, int n_rwg) {
     extern __shared__ __align__(16) double s_fin[];
     unsigned* sync = m.step_sync;
     const int t = threadIdx.x;
@@ -146,13 +146,15 @@ k_step_finish(DevModel m, BlockArrays B) {
         return;
     }
     w -= B.n_blocks;
-    if (w < step_rigid_node_wgs(m.n_rigid_nodes)) {
-        // RIGID_NODES_PER_THREAD nodes per thread; their own data is loaded
-        // before the wait for their bodies
-        RigidNodePre p[RIGID_NODES_PER_THREAD];
+    if (w < n_rwg) {
+        // the nodes q = w * BLOCK_THREADS + t + k * n_rwg * BLOCK_THREADS;
+        // the first RIGID_NODES_PREFETCH of them are loaded before the wait
+        // for their bodies
+        const int stride = n_rwg * BLOCK_THREADS;
+        RigidNodePre p[RIGID_NODES_PREFETCH];
 #pragma unroll
-        for (int u = 0; u < RIGID_NODES_PER_THREAD; u++) {
-            const int q = (w * RIGID_NODES_PER_THREAD + u) * BLOCK_THREADS + t;
+        for (int u = 0; u < RIGID_NODES_PREFETCH; u++) {
+            const int q = w * BLOCK_THREADS + t + u * stride;
             p[u] = RigidNodePre{};
             if (q < m.n_rigid_nodes) p[u] = rigid_node_pre(m, m.rigid_node_ids[q]);
         }
@@ -160,18 +162,21 @@ k_step_finish(DevModel m, BlockArrays B) {
         // the rotation's cosine and sine once per body and thread
         int cb = -1;
         double c = 0., sn = 0.;
-#pragma unroll
-        for (int u = 0; u < RIGID_NODES_PER_THREAD; u++) {
-            if (!p[u].on) continue;
-            const double* R = m.body_rigid + p[u].b * RIGID_FIELDS;
-            if (p[u].b != cb) {
+        auto place = [&](const RigidNodePre& r) {
+            if (!r.on) return;
+            const double* R = m.body_rigid + r.b * RIGID_FIELDS;
+            if (r.b != cb) {
                 const double dr = R[R_DISP + 2] - 0.;
                 c = cos(dr);
                 sn = sin(dr);
-                cb = p[u].b;
+                cb = r.b;
             }
-            rigid_node_at(m, p[u].i, R, c, sn, 1, p[u].X, p[u].Y);
-        }
+            rigid_node_at(m, r.i, R, c, sn, 1, r.X, r.Y);
+        };
+#pragma unroll
+        for (int u = 0; u < RIGID_NODES_PREFETCH; u++) place(p[u]);
+        for (int q = w * BLOCK_THREADS + t + RIGID_NODES_PREFETCH * stride; q < m.n_rigid_nodes; q += stride)
+            place(rigid_node_pre(m, m.rigid_node_ids[q]));
         return;
     }
     // last workgroup: the time of the end of the step from the device's copy

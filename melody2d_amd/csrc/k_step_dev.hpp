@@ -893,7 +893,8 @@ __device__ inline double slot_length_pre(const SlotPre& r) {
 // node names a border node of the block (bt_lbn is its index in the block's
 // list) and every DOF_TO_DISP entry of a border node names an owned node.
 constexpr int FIN_SLOTS = 2;     // slots of a border node with prefetched length inputs
-constexpr int FIN_ROW = 24;      // DOF_TO_DISP entries of a border node prefetched
+constexpr int FIN_ROW = 12;      // DOF_TO_DISP entries of a border node prefetched
+constexpr int FIN_BT = 6;        // point-force spreading entries of an owned node prefetched
 __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArrays& B, int blk,
                                              const double (&fi)[2][NODES_PER_THREAD], double* sc) {
     __shared__ double red[BLOCK_THREADS / 64][5];
@@ -923,10 +924,18 @@ __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArray
         upx[q] = m.upx[j]; upy[q] = m.upy[j];
         e0[q] = m.bt_ptr[j]; e1[q] = o < nown ? m.bt_ptr[j + 1] : e0[q];
     }
+    // the owned nodes' first point-force spreading entries
+    int bl[NODES_PER_THREAD][FIN_BT];
+    double bp[NODES_PER_THREAD][FIN_BT];
+#pragma unroll
+    for (int q = 0; q < NODES_PER_THREAD; q++)
+#pragma unroll
+        for (int u = 0; u < FIN_BT; u++)
+            if (e0[q] + u < e1[q]) { bl[q][u] = B.bt_lbn[e0[q] + u]; bp[q][u] = m.bt_phi[e0[q] + u]; }
     // the thread's first border node: its lengths' inputs (old positions)
     // and its kinematics row
     const bool mine = t < nbn;
-    int bn_t = 0, sq0 = 0, sq1 = 0, knode = 0, k0 = 0, k1 = 0;
+    int bn_t = 0, sq0 = 0, sq1 = 0, knode = 0, k0 = 0, k1 = 0, slot_a = 0, slot_b = 0;
     SlotPre sp[FIN_SLOTS];
     int di[FIN_ROW];
     double dp[FIN_ROW];
@@ -937,6 +946,7 @@ __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArray
         const double2 rx = B.fin_recX[h3.z + t];
         bn_t = r0.x; knode = r0.y; k0 = r0.z; k1 = r0.w;
         sq0 = r1.x; sq1 = r1.y;
+        slot_a = r1.z; slot_b = r1.w;
         kX = rx.x; kY = rx.y;
         if (sq0 < sq1) sp[0] = slot_prefetch_rec(m, B, r1.z);
         if (sq0 + 1 < sq1) sp[1] = slot_prefetch_rec(m, B, r1.w);
@@ -967,7 +977,17 @@ __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArray
         const int j = n0 + o;
         const double ix = fi[0][q], iy = fi[1][q];
         double cx = 0., cy = 0., dx = 0., dy = 0., nx = 0., ny = 0.;
-        for (int e = e0[q]; e < e1[q]; e++) {
+#pragma unroll
+        for (int u = 0; u < FIN_BT; u++) {
+            if (e0[q] + u < e1[q]) {
+                const double* p = s_bnp + 6 * bl[q][u];
+                const double f = bp[q][u];
+                cx += p[0] * f; cy += p[1] * f;
+                dx += p[2] * f; dy += p[3] * f;
+                nx += p[4] * f; ny += p[5] * f;
+            }
+        }
+        for (int e = e0[q] + FIN_BT; e < e1[q]; e++) {
             const double* p = s_bnp + 6 * (int)B.bt_lbn[e];
             const double f = m.bt_phi[e];
             cx += p[0] * f; cy += p[1] * f;
@@ -1002,9 +1022,8 @@ __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArray
 
     // ---- Update_borders: lengths from the old positions ----
     if (mine) {
-#pragma unroll
-        for (int u = 0; u < FIN_SLOTS; u++)
-            if (sq0 + u < sq1) m.sl_len[m.bn_slots[sq0 + u]] = slot_length_pre(sp[u]);
+        if (sq0 < sq1) m.sl_len[slot_a] = slot_length_pre(sp[0]);
+        if (sq0 + 1 < sq1) m.sl_len[slot_b] = slot_length_pre(sp[1]);
         for (int q = sq0 + FIN_SLOTS; q < sq1; q++) m.sl_len[m.bn_slots[q]] = slot_length(m, m.bn_slots[q]);
     }
     for (int i = t + BLOCK_THREADS; i < nbn; i += BLOCK_THREADS) bn_lengths(m, bns[i]);
@@ -1012,6 +1031,7 @@ __device__ inline void node_block_finish_lds(const DevModel& m, const BlockArray
     if (t == 0) {
         const int slot[5] = {W_INTERNAL, W_BODY, W_DIRICHLET, W_NEUMANN, W_DAMPING};
         double* acc = B.blk_work + 14 * (size_t)blk;
+#pragma unroll
         for (int q = 0; q < 5; q++) {
             double v = 0.;
             for (int wv = 0; wv < BLOCK_THREADS / 64; wv++) v += red[wv][q];

@@ -227,6 +227,7 @@ struct mel_ctx {
     std::map<GraphKey, hipGraphExec_t> graphs;
     bool use_graphs = false;
     size_t finish_lds = 0;
+    int finish_slots = 0;      // workgroups of launch B resident at once on the device
     int* d_kin_all = nullptr;     // every deformable node (kinematics on request)
     int n_kin_all = 0;
     bool outputs_stale = false;   // interior kinematics / force fields lag the last steps
@@ -867,6 +868,11 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
         c.finish_lds = step_finish_lds(max_bn, max_own);
         HIP_OK(hipFuncSetAttribute((const void*)k_step_finish, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)c.finish_lds));
+        int per_cu = 0, cus = 0;
+        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_step_finish, BLOCK_THREADS,
+                                                            c.finish_lds));
+        HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+        c.finish_slots = per_cu * cus;
     }
 }
 
@@ -947,7 +953,9 @@ void launch_step(mel_ctx& c, int n_cwg) {
     TIMED(c, KT_STEP_FORCES, s,
           (c.precision == MEL_FP32_MIXED ? k_step_forces32<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)
                                          : k_step_forces<<<ga, BLOCK_THREADS, c.block_lds, s>>>(d, B, S)));
-    TIMED(c, KT_STEP_FINISH, s, (k_step_finish<<<step_finish_grid(d, B), BLOCK_THREADS, c.finish_lds, s>>>(d, B)));
+    const int n_rwg = step_rigid_node_wgs(d.n_rigid_nodes, c.finish_slots - d.n_rigid - B.n_blocks - 1);
+    TIMED(c, KT_STEP_FINISH, s,
+          (k_step_finish<<<step_finish_grid(d, B, n_rwg), BLOCK_THREADS, c.finish_lds, s>>>(d, B, n_rwg)));
 }
 
 // a run of steps between contact updates: replayed from a captured graph
