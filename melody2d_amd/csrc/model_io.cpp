@@ -128,6 +128,111 @@ void finish_body_state(Model& m, Body& b) {
     (void)m;
 }
 
+// a body's borders can be given their stencils: every border has at least
+// three entries naming nodes of the body, and before / after name borders of
+// the body (initialize_segments indexes with all of them)
+bool check_borders(const Body& b, int bi, std::string& err) {
+    const int nbr = (int)b.borders.size();
+    for (int k = 0; k < nbr; k++) {
+        const Border& B = b.borders[k];
+        const std::string where = "body " + std::to_string(bi) + " border " + std::to_string(k);
+        if (B.nodes.size() < 3) { err = where + ": fewer than three nodes"; return false; }
+        if (B.before < 0 || B.before >= nbr || B.after < 0 || B.after >= nbr) {
+            err = where + ": before/after border out of range";
+            return false;
+        }
+        if (B.periodicity < PER_CLOSED || B.periodicity > PER_PERIODIC) { err = where + ": periodicity"; return false; }
+        for (int v : B.nodes)
+            if (v < 0 || v >= b.n_nodes) { err = where + ": node index out of range"; return false; }
+    }
+    return true;
+}
+
+// CSR pointer array of `rows` rows over `n` entries: rows + 1 values, from
+// 0 up to n, never decreasing
+bool csr_ok(const std::vector<int>& ptr, size_t rows, size_t n) {
+    if (ptr.size() != rows + 1 || ptr[0] != 0 || (size_t)ptr[rows] != n) return false;
+    for (size_t i = 0; i < rows; i++)
+        if (ptr[i + 1] < ptr[i]) return false;
+    return true;
+}
+
+bool indices_ok(const std::vector<int>& idx, int n) {
+    for (int v : idx)
+        if (v < 0 || v >= n) return false;
+    return true;
+}
+
+// every index the library uses on the device, checked once the model is read
+bool validate_model(const Model& m, std::string& err) {
+    const int nm = (int)m.materials.size();
+    if ((int)m.contacts_table.size() != nm * nm) { err = "material table size"; return false; }
+    for (size_t bi = 0; bi < m.bodies.size(); bi++) {
+        const Body& b = m.bodies[bi];
+        const std::string where = "body " + std::to_string(bi) + ": ";
+        const size_t n = (size_t)b.n_nodes;
+        if (b.n_nodes < 0 || b.n_gauss < 0) { err = where + "negative size"; return false; }
+        if (b.material < 0 || b.material >= nm) { err = where + "material index out of range"; return false; }
+        if (b.status < ST_ACTIVE || b.status > ST_NOGRAVITY) { err = where + "status"; return false; }
+        if (b.X.size() != n || b.Y.size() != n || b.mx.size() != n || b.my.size() != n || b.imx.size() != n ||
+            b.imy.size() != n || b.domain.size() != n) {
+            err = where + "node array sizes";
+            return false;
+        }
+        if (b.borders.empty()) { err = where + "no border"; return false; }
+        if (!check_borders(b, (int)bi, err)) return false;
+        if (b.cells.size() % 3 != 0 || !indices_ok(b.cells, b.n_nodes)) { err = where + "cell indices"; return false; }
+        if (b.rigid) continue;
+        const size_t G = (size_t)b.n_gauss;
+        if (b.n_regions < 1) { err = where + "region count"; return false; }
+        if (b.gx.size() != G || b.gy.size() != G || b.gw.size() != G || b.gjac.size() != G) {
+            err = where + "Gauss point array sizes";
+            return false;
+        }
+        if (!csr_ok(b.gptr, G, b.gidx.size()) || b.gphi.size() != b.gidx.size() ||
+            b.gphix.size() != b.gidx.size() || b.gphiy.size() != b.gidx.size() || !indices_ok(b.gidx, b.n_nodes)) {
+            err = where + "Gauss point shape functions (pointers or node indices)";
+            return false;
+        }
+        if (!csr_ok(b.dptr, n, b.didx.size()) || b.dphi.size() != b.didx.size() ||
+            b.dphix.size() != b.didx.size() || b.dphiy.size() != b.didx.size() || !indices_ok(b.didx, b.n_nodes)) {
+            err = where + "DOF_TO_DISP rows (pointers or node indices)";
+            return false;
+        }
+        if (b.mat_dof.size() != 2 * b.mat_stiffness.size() || !indices_ok(b.mat_dof, 2 * b.n_nodes)) {
+            err = where + "matrix entries";
+            return false;
+        }
+    }
+    const int NB = (int)m.bodies.size();
+    for (size_t bi = 0; bi < m.bodies.size(); bi++) {
+        const Body& b = m.bodies[bi];
+        for (size_t k = 0; k + 2 < b.neighbours.size(); k += 3) {
+            const int nb = b.neighbours[k], nbr = b.neighbours[k + 1];
+            if (nb < 0 || nb >= NB || nbr < 0 || nbr >= (int)m.bodies[nb].borders.size()) {
+                err = "body " + std::to_string(bi) + ": neighbour out of range";
+                return false;
+            }
+        }
+        for (const ContactElement& c : b.contacts) {
+            const std::string where = "body " + std::to_string(bi) + ": contact element ";
+            if (c.borderS < 0 || c.borderS >= (int)b.borders.size() || c.border_nodeS < 0 ||
+                c.border_nodeS >= (int)b.borders[c.borderS].nodes.size() || c.nodeS < 0 || c.nodeS >= b.n_nodes) {
+                err = where + "slave out of range";
+                return false;
+            }
+            if (c.bodyM < 0 || c.bodyM >= NB) { err = where + "master body out of range"; return false; }
+            const Body& M = m.bodies[c.bodyM];
+            if (c.borderM < 0 || c.borderM >= (int)M.borders.size() || c.segmentM < 0 ||
+                c.segmentM + 1 >= (int)M.borders[c.borderM].nodes.size()) {
+                err = where + "master segment out of range";
+                return false;
+            }
+        }
+    }
+    return true;
+}
+
 }  // namespace
 
 void initialize_segments(Model& m, Body& body) {
@@ -186,7 +291,8 @@ void initialize_segments(Model& m, Body& body) {
     }
 }
 
-bool load_model_files(const std::string& dir, int save_index, Model& m, std::string& err) {
+namespace {
+bool load_model_files_impl(const std::string& dir, int save_index, Model& m, std::string& err) {
     m = Model();
     std::string line, token;
     int index_body = 0;
@@ -360,6 +466,7 @@ bool load_model_files(const std::string& dir, int save_index, Model& m, std::str
                 in >> n;
                 std::getline(in, token);
                 std::getline(in, token);
+                if (!in || n < 0 || n > (1 << 28)) { err = "node count"; return false; }
                 b.n_nodes = n;
                 b.X.resize(n); b.Y.resize(n); b.domain.assign(n, 0); b.mx.assign(n, 0); b.my.assign(n, 0);
                 b.imx.assign(n, 0); b.imy.assign(n, 0);
@@ -383,6 +490,7 @@ bool load_model_files(const std::string& dir, int save_index, Model& m, std::str
                     in >> nn >> before >> after;
                     std::getline(in, token);
                     Border& B = b.borders[i];
+                    if (!in || nn < 0 || nn > (1 << 28)) { err = "border node count"; return false; }
                     B.nodes.resize(nn);
                     for (int j = 0; j < nn; j++) {
                         in >> B.nodes[j];
@@ -392,6 +500,7 @@ bool load_model_files(const std::string& dir, int save_index, Model& m, std::str
                     B.after = after;
                     std::getline(in, token);
                 }
+                if (!check_borders(b, index_body, err)) return false;
                 initialize_segments(m, b);
             }
             if (starts(line, "INTEGRATION")) {
@@ -591,6 +700,7 @@ bool load_model_files(const std::string& dir, int save_index, Model& m, std::str
                     // Code/IO.h:2270-2281 stores all four master nodes into slot 0;
                     // Update_contacts rebuilds them from segmentM before use.
                     c.nodeM[0] = nd[3]; c.shapeM[0] = N[3]; c.shiftMk[0] = sh[3];
+                    if (!in || ni < 0 || ni > 64) { err = "contact element internal variables"; return false; }
                     c.nb_internal = ni;
                     if (ni == 0) c.internal.assign(1, 0.0);
                     for (int k = 0; k < ni; k++) {
@@ -603,6 +713,17 @@ bool load_model_files(const std::string& dir, int save_index, Model& m, std::str
         }
     }
     return true;
+}
+}  // namespace
+
+bool load_model_files(const std::string& dir, int save_index, Model& m, std::string& err) {
+    try {
+        if (!load_model_files_impl(dir, save_index, m, err)) return false;
+    } catch (std::exception& e) {
+        err = std::string("malformed model files: ") + e.what();
+        return false;
+    }
+    return validate_model(m, err);
 }
 
 // ---------------------------------------------------------------------------
@@ -657,18 +778,25 @@ bool load_model_binary(const std::string& path, Model& m, std::string& err) {
         if (it == recs.end()) throw std::runtime_error("missing record " + k);
         return it->second.i;
     };
+    auto need = [&](bool ok, const std::string& what) {
+        if (!ok) throw std::runtime_error("malformed container " + path + ": " + what);
+    };
     try {
         auto& h = D("model.header");
+        need(h.size() >= 17, "model.header");
         m.tini = h[0]; m.dt = h[1]; m.tend = h[2]; m.save_period = h[3]; m.print_period = h[4];
         m.contact_update_period = h[5]; m.xmin_period = h[6]; m.xmax_period = h[7];
         m.gravity_x = h[8]; m.gravity_y = h[9]; m.time = h[10]; m.number_save = (int)h[11];
         m.number_iteration = (int)h[12]; m.number_print = (int)h[13]; m.next_save = h[14];
         m.next_print = h[15]; m.next_contact_update = h[16];
         auto& fl = I("model.flags");
+        need(fl.size() >= (size_t)F_COUNT, "model.flags");
         for (int f = 0; f < F_COUNT; f++) m.flags[f] = fl[f];
         auto& mt = I("model.material_type");
         auto& mr = D("model.material_raw");
+        need(mr.size() == 6 * mt.size(), "model.material_raw size");
         for (size_t k = 0; k < mt.size(); k++) {
+            need(mt[k] == MAT_ELASTIC_LINEAR || mt[k] == MAT_NEOHOOKEAN, "material type");
             Material mat;
             mat.name = "Material" + std::to_string(k);
             mat.type = mt[k];
@@ -681,7 +809,10 @@ bool load_model_binary(const std::string& path, Model& m, std::string& err) {
         auto& lf = I("model.law_fixed");
         auto& lp = I("model.law_ptr");
         auto& lv = D("model.law_params");
+        need(lm.size() == 2 * lt.size() && lf.size() == lt.size(), "law records sizes");
+        need(csr_ok(lp, lt.size(), lv.size()), "model.law_ptr");
         for (size_t k = 0; k < lt.size(); k++) {
+            need(lt[k] >= LAW_FRICTIONLESS && lt[k] <= LAW_CZM_FATIGUE, "law type");
             ContactLaw law;
             law.m1 = m.materials.at(lm[2 * k]).name;
             law.m2 = m.materials.at(lm[2 * k + 1]).name;
@@ -692,12 +823,17 @@ bool load_model_binary(const std::string& path, Model& m, std::string& err) {
             m.laws.push_back(law);
         }
         build_table(m);
-        int nb = I("model.n_bodies")[0];
+        auto& nbr_rec = I("model.n_bodies");
+        need(nbr_rec.size() == 1 && nbr_rec[0] >= 0 && (size_t)nbr_rec[0] <= recs.size(), "model.n_bodies");
+        int nb = nbr_rec[0];
         m.bodies.resize(nb);
         for (int bi = 0; bi < nb; bi++) {
             std::string p = "b" + std::to_string(bi) + ".";
             Body& b = m.bodies[bi];
             auto& info = I(p + "info");
+            need(info.size() >= 7, p + "info");
+            need(info[1] >= 0 && info[1] < (int)m.materials.size(), p + "info material");
+            need(info[4] >= 0 && info[5] >= 0 && info[6] >= 1, p + "info sizes");
             b.rigid = info[0] != 0;
             b.material = info[1];
             b.periodicity = info[2];
@@ -707,6 +843,7 @@ bool load_model_binary(const std::string& path, Model& m, std::string& err) {
             b.n_regions = info[6];
             b.density = m.materials.at(b.material).p[0];
             auto& dist = D(p + "dist");
+            need(dist.size() >= 3, p + "dist");
             b.nodal_distance = dist[0]; b.detection_distance = dist[1]; b.contact_distance = dist[2];
             b.X = D(p + "X");
             b.Y = D(p + "Y");
@@ -717,7 +854,16 @@ bool load_model_binary(const std::string& path, Model& m, std::string& err) {
             auto& hptr = I(p + "border_hist_ptr");
             auto& ht = D(p + "hist_t");
             auto& hv = D(p + "hist_v");
-            int nbr = (int)bptr.size() - 1;
+            const size_t nbr_z = bptr.empty() ? 0 : bptr.size() - 1;
+            need(nbr_z >= 1 && csr_ok(bptr, nbr_z, bnod.size()), p + "border_ptr");
+            need(meta.size() == 5 * nbr_z && bcp.size() == 4 * nbr_z, p + "border_meta / border_bcparams size");
+            need(hptr.size() == 2 * nbr_z + 1 && ht.size() == hv.size() && csr_ok(hptr, 2 * nbr_z, ht.size()),
+                 p + "border_hist_ptr");
+            for (size_t k = 0; k < nbr_z; k++)
+                need(meta[5 * k + 3] >= BC_NONE && meta[5 * k + 3] <= BC_FOLLOWING && meta[5 * k + 4] >= BC_NONE &&
+                         meta[5 * k + 4] <= BC_FOLLOWING, p + "border BC type");
+            need(b.X.size() == (size_t)b.n_nodes && b.Y.size() == (size_t)b.n_nodes, p + "X / Y size");
+            int nbr = (int)nbr_z;
             b.borders.resize(nbr);
             for (int k = 0; k < nbr; k++) {
                 Border& B = b.borders[k];
@@ -734,14 +880,20 @@ bool load_model_binary(const std::string& path, Model& m, std::string& err) {
                 B.ybc.hist.t.assign(ht.begin() + hptr[2 * k + 1], ht.begin() + hptr[2 * k + 2]);
                 B.ybc.hist.v.assign(hv.begin() + hptr[2 * k + 1], hv.begin() + hptr[2 * k + 2]);
             }
+            {
+                std::string berr;
+                need(check_borders(b, bi, berr), berr);
+            }
             initialize_segments(m, b);
             b.cells = I(p + "cells");
             if (!b.rigid) {
                 b.domain = D(p + "domain");
                 b.mx = D(p + "mx"); b.my = D(p + "my");
+                need(b.mx.size() == (size_t)b.n_nodes && b.my.size() == (size_t)b.n_nodes, p + "mx / my size");
                 b.imx.resize(b.n_nodes); b.imy.resize(b.n_nodes);
                 for (int i = 0; i < b.n_nodes; i++) { b.imx[i] = 1.0 / b.mx[i]; b.imy[i] = 1.0 / b.my[i]; }
                 auto& gq = D(p + "gauss");
+                need(gq.size() == 4 * (size_t)b.n_gauss, p + "gauss size");
                 b.gx.resize(b.n_gauss); b.gy.resize(b.n_gauss); b.gw.resize(b.n_gauss); b.gjac.resize(b.n_gauss);
                 for (int g = 0; g < b.n_gauss; g++) {
                     b.gx[g] = gq[4 * g]; b.gy[g] = gq[4 * g + 1]; b.gw[g] = gq[4 * g + 2]; b.gjac[g] = gq[4 * g + 3];
@@ -752,14 +904,17 @@ bool load_model_binary(const std::string& path, Model& m, std::string& err) {
                 b.dphi = D(p + "dphi"); b.dphix = D(p + "dphix"); b.dphiy = D(p + "dphiy");
                 finish_body_state(m, b);
                 auto& kin = D(p + "kin");
+                need(kin.size() == 12 * (size_t)b.n_nodes, p + "kin size");
                 std::vector<double>* f[12] = {&b.ux, &b.uy, &b.vx, &b.vy, &b.ax, &b.ay,
                                               &b.upx, &b.upy, &b.vpx, &b.vpy, &b.apx, &b.apy};
                 for (int q = 0; q < 12; q++)
                     for (int i = 0; i < b.n_nodes; i++) (*f[q])[i] = kin[(size_t)q * b.n_nodes + i];
             } else {
                 auto& r = D(p + "rigid");
+                need(r.size() >= 6, p + "rigid");
                 b.xc = r[0]; b.yc = r[1]; b.mass = r[2]; b.inertia = r[3]; b.inverse_mass = r[4]; b.inverse_inertia = r[5];
                 auto& rk = D(p + "rkin");
+                need(rk.size() >= 9, p + "rkin");
                 for (int q = 0; q < 3; q++) { b.rigid_cur[q] = rk[q]; b.rigid_disp[q] = rk[3 + q]; b.rigid_vel[q] = rk[6 + q]; }
                 b.domain.assign(b.n_nodes, 0.0);
                 b.mx.assign(b.n_nodes, 0.0); b.my.assign(b.n_nodes, 0.0);
@@ -771,7 +926,7 @@ bool load_model_binary(const std::string& path, Model& m, std::string& err) {
         err = e.what();
         return false;
     }
-    return true;
+    return validate_model(m, err);
 }
 
 // ---------------------------------------------------------------------------

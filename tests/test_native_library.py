@@ -87,3 +87,50 @@ def test_load_options_are_checked_before_the_device():
     native.library().mel_default_options(ctypes.byref(o))
     assert o.as_dict() == {"block_nodes": -1, "fused_step": -1, "graph_replay": -1, "wide_window": -1}
     assert native.library().mel_autotune(None, 0, 0, 10, None, None, None) == 1
+
+
+def _records(data):
+    """(name, kind, count, payload offset) of every record of a container."""
+    import struct
+    out, pos = [], 0
+    while pos < len(data):
+        (ln,) = struct.unpack_from("<I", data, pos)
+        name = data[pos + 4:pos + 4 + ln].decode()
+        kind, n = struct.unpack_from("<BQ", data, pos + 4 + ln)
+        off = pos + 4 + ln + 9
+        out.append((name, kind, n, off))
+        pos = off + n * (8 if kind == 0 else 4)
+    return out
+
+
+def test_patched_index_records_are_parse_errors(tmp_path):
+    """Index records that contradict each other (pointer arrays out of range
+    or out of order, node, border, material and shape-function indices out
+    of range) are parse errors reported before any device call, never a
+    crash of the process."""
+    import struct
+    import pytest
+    from melody2d_amd.preprocessor import configs
+    from melody2d_amd.preprocessor.binary import write_binary
+    good = tmp_path / "tiny.bin"
+    write_binary(configs.variants()["tiny_platen"], str(good))
+    data = good.read_bytes()
+    recs = {r[0]: r for r in _records(data)}
+    cases = [("b0.border_meta", 1, 10 ** 8), ("b0.border_meta", 2, -3), ("b0.border_ptr", 1, 10 ** 8),
+             ("b0.border_ptr", 1, 0), ("b0.border_nodes", 3, 10 ** 8), ("b0.gidx", 5, 10 ** 8),
+             ("b0.gidx", 7, -1), ("b0.gptr", 3, 10 ** 8), ("b0.didx", 0, 10 ** 8), ("b0.dptr", 2, -5),
+             ("b0.info", 1, 99), ("b0.info", 4, 10 ** 8), ("b0.cells", 0, 10 ** 8), ("b0.border_hist_ptr", 1, 10 ** 8),
+             ("model.law_materials", 0, 50), ("model.law_ptr", 1, 10 ** 8), ("model.n_bodies", 0, 10 ** 8),
+             ("b2.border_meta", 1, 7), ("b2.info", 1, -1)]
+    for name, k, v in cases:
+        _, kind, n, off = recs[name]
+        assert kind == 1 and k < n, name
+        bad = bytearray(data)
+        struct.pack_into("<i", bad, off + 4 * k, v)
+        p = tmp_path / "bad.bin"
+        p.write_bytes(bytes(bad))
+        with pytest.raises(native.MelError) as e:
+            native.Context(str(p), binary=True)
+        assert e.value.status == 3, (name, k, v, e.value)
+    # the unpatched container still reads (the first device call is where it stops without a GPU)
+    assert recs["b0.border_meta"][2] >= 5
