@@ -69,7 +69,7 @@ struct mel_server {
     size_t budget = 0, used = 0;
     long long slice = 0;
     double ratio = DEFAULT_RATIO;
-    int graph_replay = 1;               // of the jobs loaded from now on
+    int graph_replay = -1;              // of the jobs loaded from now on (-1: the context's own choice)
     std::vector<Job> jobs;
     std::vector<size_t> order;          // job indices by arrival (stable)
     size_t next_admit = 0;              // first entry of `order` not yet admitted or failed
@@ -90,7 +90,11 @@ struct mel_server {
         j.status = st;
         j.err = msg;
         j.finished = now();
-        for (Slice& s : j.inflight) events.push_back(s.ev);
+        // the job's queued work ends before its stream goes; its slice
+        // events are destroyed, not reused (their last record was on that
+        // stream)
+        if (j.ctx) (void)mel_synchronize(j.ctx);
+        for (Slice& s : j.inflight) (void)hipEventDestroy(s.ev);
         j.inflight.clear();
         release(j);
     }
@@ -173,11 +177,22 @@ bool admit(mel_server& s) {
         j.admitted = t_admit;
         if (st != MEL_OK) { s.fail(j, st, ctx_error(nullptr)); continue; }
         j.ctx = c;
-        mel_set_graph_replay(c, s.graph_replay);
+        if (s.graph_replay >= 0) mel_set_graph_replay(c, s.graph_replay);
         size_t bytes = 0;
         mel_get_device_bytes(c, &bytes);
         mel_info_t info;
         st = mel_get_info(c, &info);
+        if (j.file_bytes) s.ratio = std::max(s.ratio, (double)bytes / (double)j.file_bytes);
+        if (st == MEL_OK && bytes <= s.budget && s.used > 0 && s.used + bytes > s.budget) {
+            // the estimate was low and other jobs hold the memory: the job
+            // goes back to the head of the queue until they have finished
+            // (its next estimate is at least its real footprint)
+            mel_destroy(c);
+            j.ctx = nullptr;
+            j.admitted = -1.;
+            s.next_admit--;
+            break;
+        }
         j.device_bytes = bytes;
         s.used += bytes;
         if (st != MEL_OK) { s.fail(j, st, ctx_error(c)); continue; }
@@ -186,7 +201,6 @@ bool admit(mel_server& s) {
             continue;
         }
         j.n_nodes = info.n_nodes;
-        if (j.file_bytes) s.ratio = std::max(s.ratio, (double)bytes / (double)j.file_bytes);
         j.state = MEL_JOB_RUNNING;
     }
     return moved;

@@ -365,7 +365,7 @@ class Server:
     container run for a number of steps, admitted by device footprint under
     an HBM budget and run concurrently on one device."""
 
-    def __init__(self, hbm_budget, steps_per_slice=50, device=0, graph_replay=True):
+    def __init__(self, hbm_budget, steps_per_slice=50, device=0, graph_replay=None):
         lib = library()
         h = ctypes.c_void_p()
         st = lib.mel_server_create(device, int(hbm_budget), int(steps_per_slice), ctypes.byref(h))
@@ -374,7 +374,9 @@ class Server:
         self._h = h
         self.lib = lib
         self._views = {}
-        self._check(lib.mel_server_set_graph_replay(h, int(bool(graph_replay))))
+        # None: each job keeps its context's own choice (MELODY2D_GRAPHS included)
+        if graph_replay is not None:
+            self._check(lib.mel_server_set_graph_replay(h, int(bool(graph_replay))))
 
     def _check(self, st, job=-1):
         if st != MEL_OK:

@@ -56,7 +56,8 @@ def _same(a, b):
     assert not diff, diff[:8]
 
 
-@pytest.mark.parametrize("precision,graphs", [(native.FP64, True), (native.FP32_MIXED, True), (native.FP64, False)])
+@pytest.mark.parametrize("precision,graphs", [(native.FP64, True), (native.FP32_MIXED, True), (native.FP64, False),
+                                              (native.FP32_MIXED, False)])
 def test_jobs_in_the_mix_equal_jobs_alone(job_files, precision, graphs):
     steps = {name: 60 + 17 * i for i, name in enumerate(JOBS)}
     print("case serve mix", precision, graphs, flush=True)
@@ -129,3 +130,26 @@ def test_failed_job_does_not_stop_the_others(job_files, tmp_path):
         assert srv.job(j).state == native.JOB_FAILED
         assert srv.job(j).status == 7
         assert srv.usage()[0] == 0
+
+
+def test_low_estimate_waits_for_memory(job_files):
+    """A job whose device footprint per container byte is above every ratio
+    seen so far is estimated low; when its real footprint does not fit beside
+    the running job it goes back to the queue and starts once that job is
+    done, so the jobs on the device never hold more than the budget."""
+    print("case serve low estimate", flush=True)
+    fp = {n: _alone(job_files[n][0], job_files[n][1], 1, native.FP64)[1] for n in JOBS}
+    ratio = {n: fp[n] / os.path.getsize(job_files[n][0]) for n in JOBS}
+    pairs = [(a, b) for a in JOBS for b in JOBS if a != b and ratio[b] > max(4.0, ratio[a])]
+    if not pairs:
+        pytest.skip("no job here has a footprint ratio above the server's first estimate (%s)" % ratio)
+    a, b = pairs[0]
+    budget = fp[a] + fp[b] - 1
+    with native.Server(hbm_budget=budget, steps_per_slice=10) as srv:
+        ja = srv.submit(job_files[a][0], 600, arrival=0.0)
+        jb = srv.submit(job_files[b][0], 30, arrival=0.001)
+        assert srv.run(timeout=120.0) == 0
+        ia, ib = srv.job(ja), srv.job(jb)
+        assert ia.state == native.JOB_DONE and ib.state == native.JOB_DONE, srv.job_error(jb)
+        assert ib.device_bytes == fp[b]
+        assert ib.admitted >= ia.finished
