@@ -125,6 +125,10 @@ mel_status mel_set_node_field(mel_ctx* ctx, int body, int field, const double* i
  * node ids local to their body (Contact_element, Code/Contact_Element.h:11-44) */
 mel_status mel_get_contacts(mel_ctx* ctx, int body, int32_t* ids, double* state, size_t cap, size_t* count);
 
+/* sizes of a body: nodes, Gauss points, borders, and whether it is rigid
+ * (Body::nb_nodes, nb_gauss, nb_borders, type; Code/Body.h:24-60) */
+mel_status mel_get_body_sizes(mel_ctx* ctx, int body, int* n_nodes, int* n_gauss, int* n_borders, int* rigid);
+
 /* neighbour triples (body2 border2 shift) of body (Code/Proximity.h:160) */
 mel_status mel_get_neighbours(mel_ctx* ctx, int body, int32_t* out, size_t cap, size_t* count);
 

@@ -3,6 +3,8 @@
 * melody2d_amd/libmelody2d_hip.so - the HIP library for gfx950 behind
   include/melody2d.h (hipcc, FP64 path compiled with -ffp-contract=off so the
   expression order of the reference is kept)
+* melody2d_amd/examples/main_hip, main_facade - the reference's driver over
+  the C-ABI and over the C++ façade (g++, linked to the library)
 * oracle/liboracle.so              - the CPU checker (test infrastructure)
 * oracle/_ref/melody_ref           - the reference solver compiled from its own
   sources, only where /root/reference is present (test infrastructure)
@@ -47,6 +49,20 @@ def build_library(verbose=False):
     return LIB
 
 
+EXAMPLES = ["main_hip", "main_facade"]
+
+
+def build_examples(verbose=False):
+    ex = os.path.join(HERE, "examples")
+    for name in EXAMPLES:
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
+               os.path.join(ex, name + ".cpp"), "-L", HERE, "-lmelody2d_hip", "-Wl,-rpath,$ORIGIN/..",
+               "-o", os.path.join(ex, name)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+
+
 def build_oracle(verbose=False):
     subprocess.run(["make", "-s", "-C", ORACLE, "oracle"], check=True)
     if os.path.isdir(os.path.join(REFERENCE, "Code")):
@@ -55,6 +71,7 @@ def build_oracle(verbose=False):
 
 def build_all(verbose=False):
     build_library(verbose)
+    build_examples(verbose)
     build_oracle(verbose)
 
 

@@ -2172,6 +2172,16 @@ mel_status mel_synchronize(mel_ctx* c) {
     });
 }
 
+mel_status mel_get_body_sizes(mel_ctx* c, int body, int* n_nodes, int* n_gauss, int* n_borders, int* rigid) {
+    if (!c || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
+    const Body& b = c->model.bodies[body];
+    if (n_nodes) *n_nodes = b.n_nodes;
+    if (n_gauss) *n_gauss = b.n_gauss;
+    if (n_borders) *n_borders = (int)b.borders.size();
+    if (rigid) *rigid = b.rigid ? 1 : 0;
+    return MEL_OK;
+}
+
 mel_status mel_get_node_field(mel_ctx* c, int body, int field, double* out, size_t n) {
     if (!c || !out || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
     double* p = node_field_ptr(*c, field);

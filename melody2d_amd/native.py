@@ -98,6 +98,8 @@ def library():
             "mel_get_neighbours": ([vp, ctypes.c_int, P(ctypes.c_int32), ctypes.c_size_t, P(ctypes.c_size_t)],
                                    ctypes.c_int),
             "mel_get_body_state": ([vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)], ctypes.c_int),
+            "mel_get_body_sizes": ([vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
+                                    P(ctypes.c_int)], ctypes.c_int),
             "mel_get_border_field": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_double),
                                       ctypes.c_size_t], ctypes.c_int),
             "mel_get_body_energy": ([vp, ctypes.c_int, P(ctypes.c_double)], ctypes.c_int),
@@ -142,7 +144,7 @@ def library():
 def exported_symbols():
     return ["mel_load", "mel_load_binary", "mel_load_ex", "mel_load_binary_ex", "mel_default_options",
             "mel_get_options", "mel_autotune", "mel_update_proximity", "mel_euler_steps", "mel_synchronize",
-            "mel_get_node_field", "mel_set_node_field", "mel_get_contacts", "mel_get_neighbours",
+            "mel_get_node_field", "mel_set_node_field", "mel_get_body_sizes", "mel_get_contacts", "mel_get_neighbours",
             "mel_get_body_state", "mel_get_body_energy", "mel_get_scalars", "mel_eval_forces",
             "mel_eval_forces_noalloc", "mel_get_state_size", "mel_save_state", "mel_load_state",
             "mel_get_border_field", "mel_get_info", "mel_write_dynamic", "mel_get_stream",
@@ -277,6 +279,12 @@ class Context:
         self._check(self.lib.mel_get_neighbours(self._h, body, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                                 cnt.value, ctypes.byref(cnt)))
         return out[:3 * cnt.value]
+
+    def body_sizes(self, body):
+        """(nodes, Gauss points, borders, rigid) of a body (mel_get_body_sizes)."""
+        v = [ctypes.c_int() for _ in range(4)]
+        self._check(self.lib.mel_get_body_sizes(self._h, body, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def body_state(self, body):
         r = np.zeros(26)
