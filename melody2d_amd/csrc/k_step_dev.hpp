@@ -44,56 +44,140 @@ __device__ inline double slot_len_now(const DevModel& m, int s) {
 // Contact laws, Code/Model.h:81-335 (0.5 factors applied by the caller as in
 // Code/Body.h:1703-1789)
 // ---------------------------------------------------------------------------
-__device__ inline void law_mohr_coulomb(double kn, double kt, double fric, double coh, double tens, double gapn,
-                                        double& gapt, double& Pn, double& Pt) {
+__device__ inline double sqrt_r(double x) { return __dsqrt_rn(x); }
+__device__ inline float sqrt_r(float x) { return __fsqrt_rn(x); }
+
+// Written over the arithmetic type R: double in the FP64 form, float in the
+// FP32-mixed form (gaps arrive rounded from FP64; a law that does not
+// return-map the tangential gap or change the damage leaves the stored FP64
+// values as they were: *_set flags).
+template <class R>
+__device__ inline void law_mohr_coulomb(R kn, R kt, R fric, R coh, R tens, R gapn, R& gapt, bool& gapt_set, R& Pn,
+                                        R& Pt) {
     Pn = -kn * gapn;
     Pt = -kt * gapt;
-    if (Pn < -tens) { Pn = 0.; Pt = 0.; gapt = 0.; }
+    if (Pn < -tens) { Pn = R(0); Pt = R(0); gapt = R(0); gapt_set = true; }
     else if (fabs(Pt) > coh + fric * Pn) {
-        if (gapt > 0.) Pt = -coh - fric * Pn;
+        if (gapt > R(0)) Pt = -coh - fric * Pn;
         else Pt = coh + fric * Pn;
         gapt = -Pt / kt;
+        gapt_set = true;
     }
 }
 
-__device__ inline void law_czm_linear(double kini, double Pnlim, double gapnlim, double Pnres, double NT, double gapn,
-                                      double& gapt, double& D, double& Pn, double& Pt) {
-    if (D >= 1.) {
+template <class R>
+__device__ inline void law_czm_linear(R kini, R Pnlim, R gapnlim, R Pnres, R NT, R gapn, R& gapt, bool& gapt_set, R& D,
+                                      bool& D_set, R& Pn, R& Pt) {
+    if (D >= R(1)) {
         Pn = -kini * gapn;
         Pt = -kini * gapt;
-        if (Pn < -Pnres) { Pn = 0.; Pt = 0.; gapt = 0.; }
+        if (Pn < -Pnres) { Pn = R(0); Pt = R(0); gapt = R(0); gapt_set = true; }
         else if (fabs(Pt) > Pnres * NT) {
-            Pt = gapt > 0. ? -Pnres * NT : Pnres * NT;
+            Pt = gapt > R(0) ? -Pnres * NT : Pnres * NT;
             gapt = -Pt / kini;
+            gapt_set = true;
         }
         return;
     }
-    double gnmax = Pnlim / kini * (1. - D) + gapnlim * D;
-    double gtmax = gnmax * NT;
-    double kmax;
+    R gnmax = Pnlim / kini * (R(1) - D) + gapnlim * D;
+    R gtmax = gnmax * NT;
+    R kmax;
     bool n_in = gapn < gnmax, n_out = gapn > gnmax;
     bool t_in = fabs(gapt) < gtmax, t_out = fabs(gapt) > gtmax;
     if (n_in && t_in) {
-        kmax = (Pnlim * (1. - D) + Pnres * D) / gnmax;
+        kmax = (Pnlim * (R(1) - D) + Pnres * D) / gnmax;
     } else if (n_out && t_in) {
         gnmax = gapn; gtmax = gnmax * NT;
         D = (gnmax - Pnlim / kini) / (gapnlim - Pnlim / kini);
-        kmax = (Pnlim * (1. - D) + Pnres * D) / gnmax;
+        D_set = true;
+        kmax = (Pnlim * (R(1) - D) + Pnres * D) / gnmax;
     } else if (n_in && t_out) {
         gtmax = fabs(gapt); gnmax = gtmax / NT;
         D = (gnmax - Pnlim / kini) / (gapnlim - Pnlim / kini);
-        kmax = (Pnlim * (1. - D) + Pnres * D) / gnmax;
+        D_set = true;
+        kmax = (Pnlim * (R(1) - D) + Pnres * D) / gnmax;
     } else if (n_out && t_out) {
         if (gapn / gnmax > fabs(gapt) / gtmax) { gnmax = gapn; gtmax = gnmax * NT; }
         else { gtmax = fabs(gapt); gnmax = gtmax / NT; }
         D = (gnmax - Pnlim / kini) / (gapnlim - Pnlim / kini);
-        kmax = (Pnlim * (1. - D) + Pnres * D) / gnmax;
+        D_set = true;
+        kmax = (Pnlim * (R(1) - D) + Pnres * D) / gnmax;
     } else {
         return;
     }
-    if (gapn < 0.) Pn = -kini * gapn;
+    if (gapn < R(0)) Pn = -kini * gapn;
     else Pn = -kmax * gapn;
     Pt = -kmax * gapt;
+}
+
+// Update_contact_forces' law step for one element (Code/Body.h:1694-1840
+// with the laws of Code/Model.h:81-335): tractions Pn, Pt and the damped
+// law's work rate W from the element's gaps, in the arithmetic type R.
+// gapt (return mapping) and D (CZM damage, nbint its first-use flag) are
+// updated in place when the law changes them.
+template <class R>
+__device__ __forceinline__ void contact_law(const DevModel& m, int lt, const double* p, double gapn_d, double& gapt_d,
+                                            double vgn_d, double vgt_d, double emass, double length0, double& D_d,
+                                            int& nbint, double& Pn_o, double& Pt_o, double& W_o) {
+    const R gapn = (R)gapn_d;
+    R gapt = (R)gapt_d;
+    bool gapt_set = false;
+    R Pn = R(0), Pt = R(0), W = R(0);
+    if (lt == 0) {                      // Frictionless
+        Pn = gapn > R(0) ? R(0) : -(R)p[0] * gapn;
+        Pt = R(0);
+        gapt = R(0);
+        gapt_set = true;
+    } else if (lt == 1) {               // Cohesion
+        R gtang = (R)p[2] * R(0.5), gtens = (R)p[3] * R(0.5);
+        Pn = -(R)p[0] * gapn;
+        Pt = -(R)p[1] * gapt;
+        if (Pn < -gtens) { Pn = R(0); Pt = R(0); gapt = R(0); gapt_set = true; }
+        else if (fabs(Pt) > gtang) { Pt = gapt > R(0) ? -gtang : gtang; gapt = -Pt / (R)p[1]; gapt_set = true; }
+    } else if (lt == 2) {               // MohrCoulomb
+        law_mohr_coulomb<R>((R)p[0], (R)p[1], (R)p[2], (R)p[3] * R(0.5), (R)p[4] * R(0.5), gapn, gapt, gapt_set, Pn,
+                            Pt);
+    } else if (lt == 3) {               // DampedMohrCoulomb
+        R kn = (R)p[0], kt = (R)p[1], fric = (R)p[2], coh = (R)p[3] * R(0.5), tens = (R)p[4] * R(0.5), damp = (R)p[5];
+        R mass = (R)emass, L0 = (R)length0, vgn = (R)vgn_d, vgt = (R)vgt_d;
+        Pn = R(0);
+        if (gapn < R(0)) { Pn = -damp * vgn * R(2) * sqrt_r(mass * kn / L0); W += Pn * vgn; }
+        Pn -= kn * gapn;
+        if (Pn < -tens) { Pn = R(0); Pt = R(0); gapt = R(0); gapt_set = true; }
+        else {
+            R Ptd = -damp * vgt * R(2) * sqrt_r(mass * kt / L0);
+            Pt = -kt * gapt + Ptd;
+            if (fabs(Pt) > coh + fric * Pn) {
+                Pt = gapt > R(0) ? -coh - fric * Pn : coh + fric * Pn;
+                gapt = -Pt / kt;
+                gapt_set = true;
+                W += Pt * vgt;
+            } else W += Ptd * vgt;
+        }
+    } else if (lt == 4) {               // TwoSlopesMohrCoulomb
+        R kn = (R)p[0], kt = (R)p[1], fric = (R)p[2], coh = (R)p[3] * R(0.5), tens = (R)p[4] * R(0.5), rati = (R)p[5];
+        Pn = (R)vgn_d <= R(0) ? -kn * gapn : -kn * gapn * rati;
+        if (Pn < -tens) { Pn = R(0); Pt = R(0); gapt = R(0); gapt_set = true; }
+        else {
+            Pt = -kt * gapt;
+            if (fabs(Pt) > coh + fric * Pn) {
+                Pt = gapt > R(0) ? -coh - fric * Pn : coh + fric * Pn;
+                gapt = -Pt / kt;
+                gapt_set = true;
+            }
+        }
+    } else if (lt == 6) {               // CZMlinear
+        R D = (R)D_d;
+        bool D_set = false;
+        if (nbint == 0) { D = R(1); D_set = true; nbint = 1; }
+        law_czm_linear<R>((R)p[0], (R)p[1] * R(0.5), (R)p[2], (R)p[3] * R(0.5), (R)p[4], gapn, gapt, gapt_set, D, D_set,
+                          Pn, Pt);
+        if (D_set) D_d = (double)D;
+    }
+    if (gapt_set) gapt_d = (double)gapt;
+    Pn_o = (double)Pn;
+    Pt_o = (double)Pt;
+    W_o = (double)W;
 }
 
 // a point-force record (id r, force) stored at position pos of border node
@@ -159,46 +243,12 @@ __device__ __forceinline__ void contact_item(const DevModel& m, int e) {
     const double* p = m.law_p + 12 * li;
     double Pn = 0., Pt = 0., W = 0.;
     const int lt = m.law_type[li];
-    if (lt == 0) {                      // Frictionless
-        Pn = gapn > 0. ? 0. : -p[0] * gapn;
-        Pt = 0.;
-        gapt = 0.;
-    } else if (lt == 1) {               // Cohesion
-        double gtang = p[2] * 0.5, gtens = p[3] * 0.5;
-        Pn = -p[0] * gapn;
-        Pt = -p[1] * gapt;
-        if (Pn < -gtens) { Pn = 0.; Pt = 0.; gapt = 0.; }
-        else if (fabs(Pt) > gtang) { Pt = gapt > 0. ? -gtang : gtang; gapt = -Pt / p[1]; }
-    } else if (lt == 2) {               // MohrCoulomb
-        law_mohr_coulomb(p[0], p[1], p[2], p[3] * 0.5, p[4] * 0.5, gapn, gapt, Pn, Pt);
-    } else if (lt == 3) {               // DampedMohrCoulomb
-        double kn = p[0], kt = p[1], fric = p[2], coh = p[3] * 0.5, tens = p[4] * 0.5, damp = p[5];
-        double mass = emass, L0 = length0, vgn = r.vgapn, vgt = r.vgapt;
-        Pn = 0.;
-        if (gapn < 0.) { Pn = -damp * vgn * 2. * __dsqrt_rn(mass * kn / L0); W += Pn * vgn; }
-        Pn -= kn * gapn;
-        if (Pn < -tens) { Pn = 0.; Pt = 0.; gapt = 0.; }
-        else {
-            double Ptd = -damp * vgt * 2. * __dsqrt_rn(mass * kt / L0);
-            Pt = -kt * gapt + Ptd;
-            if (fabs(Pt) > coh + fric * Pn) {
-                Pt = gapt > 0. ? -coh - fric * Pn : coh + fric * Pn;
-                gapt = -Pt / kt;
-                W += Pt * vgt;
-            } else W += Ptd * vgt;
-        }
-    } else if (lt == 4) {               // TwoSlopesMohrCoulomb
-        double kn = p[0], kt = p[1], fric = p[2], coh = p[3] * 0.5, tens = p[4] * 0.5, rati = p[5];
-        Pn = r.vgapn <= 0. ? -kn * gapn : -kn * gapn * rati;
-        if (Pn < -tens) { Pn = 0.; Pt = 0.; gapt = 0.; }
-        else {
-            Pt = -kt * gapt;
-            if (fabs(Pt) > coh + fric * Pn) { Pt = gapt > 0. ? -coh - fric * Pn : coh + fric * Pn; gapt = -Pt / kt; }
-        }
-    } else if (lt == 6) {               // CZMlinear
-        double D = int0;
-        if (nbint == 0) { D = 1.; C.nbint[e] = 1; }
-        law_czm_linear(p[0], p[1] * 0.5, p[2], p[3] * 0.5, p[4], gapn, gapt, D, Pn, Pt);
+    double D = int0;
+    int nb = nbint;
+    if (m.law_f32) contact_law<float>(m, lt, p, gapn, gapt, r.vgapn, r.vgapt, emass, length0, D, nb, Pn, Pt, W);
+    else contact_law<double>(m, lt, p, gapn, gapt, r.vgapn, r.vgapt, emass, length0, D, nb, Pn, Pt, W);
+    if (lt == 6) {
+        if (nb != nbint) C.nbint[e] = nb;
         C.int0[e] = D;
     }
     const double Px = r.p.xn * Pn + r.p.xt * Pt;

@@ -1100,6 +1100,7 @@ void build(mel_ctx& c) {
     d.gravity_y = m.gravity_y;
     d.dt = m.dt;
     d.self_contact = m.flags[F_NO_MONITORING] ? 0 : 1;
+    d.law_f32 = c.precision == MEL_FP32_MIXED ? 1 : 0;
     for (auto& b : m.bodies) {
         if (b.status == ST_INACTIVE) throw CapacityError{"inactive bodies are not supported"};
         if (b.borders.empty()) throw CapacityError{"every body needs at least one border"};

@@ -62,6 +62,7 @@ struct DevModel {
     int n_materials, n_laws;
     double period, xmin_period, gravity_x, gravity_y, dt;
     int self_contact;          // 1 unless NO_MONITORING (Code/Proximity.h:332)
+    int law_f32;               // contact-law tractions in FP32 (the FP32-mixed form)
 
     // bodies
     int *body_rigid_flag, *body_material, *body_periodicity, *body_status;

@@ -74,3 +74,41 @@ def test_fp32_runs_every_golden_model(golden_dir, tmp_path):
             assert _rel(x32, x64) <= TOL, (name, b, _rel(x32, x64))
         c64.close()
         c32.close()
+
+
+@pytest.mark.parametrize("name", ["tiny_platen", "tiny_frictionless", "tiny_cohesion", "tiny_damped",
+                                  "tiny_twoslopes", "tiny_czm"])
+def test_fp32_contact_law_forces_within_1e5(name, golden_dir, tmp_path):
+    """The FP32-mixed form applies the contact laws in FP32 (gaps and
+    projections stay FP64): from one FP64 state, its contact forces are within
+    1e-5 of the FP64 form's, and not identical to them."""
+    model, _ = golden_dir(name)
+    print("case fp32 laws", name, flush=True)
+    refs, _ = run_oracle(model, [0], str(tmp_path / "oracle"))
+    nn, _, _ = layout_from_dump(refs[0])
+    c64 = native.Context(model)
+    c32 = native.Context(model, precision=native.FP32_MIXED)
+
+    def forces(c):
+        return np.concatenate([np.concatenate([c.node_field(b, "x_contact_force", n),
+                                               c.node_field(b, "y_contact_force", n)]) for b, n in enumerate(nn)])
+
+    # the first of these states at which contacts carry a force
+    done = 0
+    for k in (0, 1, 2, 5, 10, 20, 50, 100, 200):
+        c64.euler_steps(k - done)
+        done = k
+        c64.eval_forces(native.EVAL_CONTACT)
+        f64 = forces(c64)
+        if np.linalg.norm(f64) > 0:
+            break
+    assert np.linalg.norm(f64) > 0
+    c32.load_state(c64.save_state())
+    c32.eval_forces(native.EVAL_CONTACT)
+    f32 = forces(c32)
+    print("  state after %d steps" % done, flush=True)
+    err = _rel(f32, f64)
+    print("  contact-force rel err %.3e" % err, flush=True)
+    assert 0 < err <= TOL
+    c64.close()
+    c32.close()
