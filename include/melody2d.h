@@ -125,6 +125,29 @@ mel_status mel_set_node_field(mel_ctx* ctx, int body, int field, const double* i
  * node ids local to their body (Contact_element, Code/Contact_Element.h:11-44) */
 mel_status mel_get_contacts(mel_ctx* ctx, int body, int32_t* ids, double* state, size_t cap, size_t* count);
 
+/* Body::Compute_nodal_stresses (Code/Body.h:2310-2519), the stress output
+ * of Write_graphic (Code/Graphic.h:117-122): stresses, strains and principal
+ * and equivalent stresses at every node from the derivatives of its
+ * DOF_TO_DISP row and the current displacement parameters (rigid bodies:
+ * zeros, jacobian 1).  mel_compute_nodal_stresses evaluates them for the
+ * current state; mel_get_nodal_stress reads one field of one body. */
+typedef enum {
+    MEL_SIGMA_I = 0, MEL_SIGMA_II, MEL_SIGMA_III, MEL_JACOBIAN, MEL_SXX, MEL_SYY, MEL_SXY, MEL_SZZ,
+    MEL_SIGMA_XX, MEL_SIGMA_YY, MEL_SIGMA_XY, MEL_SIGMA_ZZ, MEL_SIGMA_TRESCA, MEL_SIGMA_VM, MEL_SIGMA_SPH,
+    MEL_EXX, MEL_EYY, MEL_EXY, MEL_NORM_E,
+    MEL_STRESS_FIELD_COUNT
+} mel_stress_field;
+
+mel_status mel_compute_nodal_stresses(mel_ctx* ctx);
+mel_status mel_get_nodal_stress(mel_ctx* ctx, int body, int field, double* out, size_t n);
+
+/* Write_graphic (Code/Graphic.h:21-946): a VTK legacy POLYDATA file of the
+ * current state with the point fields whose flag is set in to_plot[0..40]
+ * (the reference's To_Plot slots 00_Body_index .. 40-Epsilon_Mass_Scaling;
+ * NULL: slots 00-12 and 14-28); nodal stresses are evaluated for it.
+ * number_print labels the file as the reference's print counter does. */
+mel_status mel_write_graphic(mel_ctx* ctx, const char* path, const int* to_plot, int number_print);
+
 /* sizes of a body: nodes, Gauss points, borders, and whether it is rigid
  * (Body::nb_nodes, nb_gauss, nb_borders, type; Code/Body.h:24-60) */
 mel_status mel_get_body_sizes(mel_ctx* ctx, int body, int* n_nodes, int* n_gauss, int* n_borders, int* rigid);

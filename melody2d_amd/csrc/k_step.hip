@@ -230,4 +230,91 @@ __global__ void k_clear_node_forces(DevModel m, int internal, int damping) {
     if (damping) { m.fkx[j] = 0.; m.fky[j] = 0.; }
 }
 
+// Body::Compute_nodal_stresses (Code/Body.h:2310-2519) for every node, from
+// the derivatives of its DOF_TO_DISP row (dphix, dphiy, aligned with d_idx)
+// and its body's material (bpar: type 0 linear, 1 neo-Hookean, -1 rigid;
+// Mu; Lambda or Kappa), written field-major: MEL_STRESS_FIELD_COUNT fields
+// of n_nodes values (SigmaI, SigmaII, SigmaIII, jacobian, Sxx, Syy, Sxy,
+// Szz, Sigmaxx, Sigmayy, Sigmaxy, Sigmazz, SigmaTresca, SigmaVM, SigmaSph,
+// Exx, Eyy, Exy, NormE)
+__global__ void k_nodal_stresses(DevModel m, const double* dphix, const double* dphiy, const double* bpar,
+                                 double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.n_nodes) return;
+    const int N = m.n_nodes;
+    const int b = m.node_body[i];
+    const double type = bpar[3 * b];
+    double v[19];
+    if (type < 0.) {
+        for (int f = 0; f < 19; f++) v[f] = 0.;
+        v[3] = 1.;
+    } else {
+        double J = 0., S11 = 0., S12 = 0., S22 = 0., S33 = 0.;
+        double Sxx = 0., Syy = 0., Sxy = 0., Szz = 0., Exx = 0., Eyy = 0., Exy = 0.;
+        if (type == 0.) {
+            double E11 = 0., E12 = 0., E22 = 0.;
+            for (int k = m.d_ptr[i]; k < m.d_ptr[i + 1]; k++) {
+                const int j = m.d_idx[k];
+                const double ux = m.upx[j], uy = m.upy[j];
+                E11 += dphix[k] * ux;
+                E12 += dphix[k] * uy + dphiy[k] * ux;
+                E22 += dphiy[k] * uy;
+            }
+            E12 = 0.5 * E12;
+            const double Mu2 = 2 * bpar[3 * b + 1], Lambda = bpar[3 * b + 2];
+            J = 1. + E11 + E22;
+            S33 = Lambda * (J - 1.);
+            S11 = S33 + Mu2 * E11;
+            S22 = S33 + Mu2 * E22;
+            S12 = Mu2 * E12;
+            if (J < 0.) { S11 = 0.; S12 = 0.; S22 = 0.; S33 = 0.; }
+            Sxx = S11; Syy = S22; Sxy = S12; Szz = S33;
+            Exx = E11; Eyy = E22; Exy = E12;
+        } else {
+            double F11 = 1., F12 = 0., F21 = 0., F22 = 1.;
+            for (int k = m.d_ptr[i]; k < m.d_ptr[i + 1]; k++) {
+                const int j = m.d_idx[k];
+                const double ux = m.upx[j], uy = m.upy[j];
+                F11 += dphix[k] * ux;
+                F12 += dphiy[k] * ux;
+                F21 += dphix[k] * uy;
+                F22 += dphiy[k] * uy;
+            }
+            const double Mu = bpar[3 * b + 1], Kappa = bpar[3 * b + 2];
+            const double F11F12 = F11 * F11 + F12 * F12, F21F22 = F21 * F21 + F22 * F22;
+            J = F11 * F22 - F12 * F21;
+            const double invJ = 1. / J, K = Kappa * (J - 1.);
+            const double MuJ53 = Mu * pow(J, -1.666666666667);
+            const double trB3 = 0.333333333333 * (F11F12 + F21F22 + 1.);
+            const double s11 = K + MuJ53 * (F11F12 - trB3), s22 = K + MuJ53 * (F21F22 - trB3);
+            const double s12 = MuJ53 * (F11 * F21 + F12 * F22);
+            S11 = (F12 * (F12 * s22 - F22 * s12) - F22 * (F12 * s12 - F22 * s11)) * invJ;
+            S22 = (F11 * (F11 * s22 - F21 * s12) - F21 * (F11 * s12 - F21 * s11)) * invJ;
+            S12 = (F21 * (F12 * s12 - F22 * s11) - F11 * (F12 * s22 - F22 * s12)) * invJ;
+            S33 = (K + MuJ53 * (1. - trB3)) * J;
+            if (J < 0.) { F11 = 0.; F12 = 0.; F21 = 0.; F22 = 0.; S11 = 0.; S12 = 0.; S22 = 0.; S33 = 0.; }
+            Sxx = F11 * (S11 * F11 + S12 * F12) + F12 * (S12 * F11 + S22 * F12);
+            Syy = F21 * (S11 * F21 + S12 * F22) + F22 * (S12 * F21 + S22 * F22);
+            Sxy = F21 * (S11 * F11 + S12 * F12) + F22 * (S12 * F11 + S22 * F12);
+            Szz = S33;
+            Exx = 0.5 * (F11 * F11 + F21 * F21 - 1.);
+            Eyy = 0.5 * (F12 * F12 + F22 * F22 - 1.);
+            Exy = 0.5 * (F12 * F11 + F22 * F21);
+        }
+        const double NormE = pow(Exx * Exx + Eyy * Eyy + 2. * Exy * Exy, 0.5);
+        const double root = __dsqrt_rn((Sxx + Syy) * (Sxx + Syy) - 4. * (Sxx * Syy - Sxy * Sxy));
+        const double A = 0.5 * (Sxx + Syy + root), Bv = 0.5 * (Sxx + Syy - root);
+        double s1, s2, s3;
+        if (A > Bv && A > Szz) { s1 = A; if (Bv > Szz) { s2 = Bv; s3 = Szz; } else { s2 = Szz; s3 = Bv; } }
+        else if (Bv > A && Bv > Szz) { s1 = Bv; if (A > Szz) { s2 = A; s3 = Szz; } else { s2 = Szz; s3 = A; } }
+        else { s1 = Szz; if (Bv > A) { s2 = Bv; s3 = A; } else { s2 = A; s3 = Bv; } }
+        const double vm = __dsqrt_rn((Sxx - Syy) * (Sxx - Syy) + (Sxx - Szz) * (Sxx - Szz) + (Szz - Syy) * (Szz - Syy) +
+                                     6. * Sxy * Sxy) * 0.70710678118655;
+        v[0] = s1; v[1] = s2; v[2] = s3; v[3] = J; v[4] = S11; v[5] = S22; v[6] = S12; v[7] = S33;
+        v[8] = Sxx; v[9] = Syy; v[10] = Sxy; v[11] = Szz; v[12] = s1 - s3; v[13] = vm;
+        v[14] = 0.333333333 * (Sxx + Syy + Szz); v[15] = Exx; v[16] = Eyy; v[17] = Exy; v[18] = NormE;
+    }
+    for (int f = 0; f < 19; f++) out[(size_t)f * N + i] = v[f];
+}
+
 }  // namespace mel

@@ -44,5 +44,7 @@ __global__ void k_prox_history(DevModel m);
 __global__ void k_prox_chain(DevModel m);
 __global__ void k_init_czm(DevModel m);
 __global__ void k_project_all(DevModel m);
+__global__ void k_nodal_stresses(DevModel m, const double* dphix, const double* dphiy, const double* bpar,
+                                 double* out);
 
 }  // namespace mel

@@ -239,6 +239,10 @@ struct mel_ctx {
     bool prox_after_step = false;
     double* d_energy = nullptr;   // per body: kinetic, kinetic from parameters, boundary work
     ContactArrays eval_save{};    // contact elements saved around mel_eval_forces
+    // nodal stresses (allocated by the first mel_compute_nodal_stresses):
+    // DOF_TO_DISP derivatives aligned with d_idx, per-body material
+    // parameters, and the MEL_STRESS_FIELD_COUNT output fields
+    double *st_dphix = nullptr, *st_dphiy = nullptr, *st_bpar = nullptr, *st_out = nullptr;
     int eval_save_cap = 0;
     std::vector<int> h_bn_list;   // border nodes finished by each block (BlockArrays::bn_list)
 
@@ -2171,6 +2175,98 @@ mel_status mel_synchronize(mel_ctx* c) {
         check_flags(*c);
         return MEL_OK;
     });
+}
+
+mel_status mel_compute_nodal_stresses(mel_ctx* c) {
+    return guarded(c, [&]() {
+        const Model& m = c->model;
+        if (!c->st_out) {
+            std::vector<double> fx, fy, par;
+            for (const Body& b : m.bodies) {
+                const Material& mat = m.materials[b.material];
+                par.push_back(b.rigid ? -1. : (mat.type == MAT_ELASTIC_LINEAR ? 0. : 1.));
+                par.push_back(mat.p[3]);
+                par.push_back(mat.p[4]);
+                // the same entry order as the device's DOF_TO_DISP rows
+                if (b.rigid) continue;
+                for (int i = 0; i < b.n_nodes; i++)
+                    for (int k = b.dptr[i]; k < b.dptr[i + 1]; k++) {
+                        fx.push_back(b.dphix[k]);
+                        fy.push_back(b.dphiy[k]);
+                    }
+            }
+            if ((long long)fx.size() != c->nnz_d)
+                throw CapacityError{"nodal stresses: DOF_TO_DISP rows differ from the device's"};
+            c->st_dphix = c->upload(fx);
+            c->st_dphiy = c->upload(fy);
+            c->st_bpar = c->upload(par);
+            c->st_out = c->alloc<double>((size_t)MEL_STRESS_FIELD_COUNT * c->d.n_nodes);
+        }
+        k_nodal_stresses<<<blocks(c->d.n_nodes), BLOCK, 0, c->stream>>>(c->d, c->st_dphix, c->st_dphiy, c->st_bpar,
+                                                                      c->st_out);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(c->stream));
+        return MEL_OK;
+    });
+}
+
+mel_status mel_get_nodal_stress(mel_ctx* c, int body, int field, double* out, size_t n) {
+    if (!c || !out || body < 0 || body >= c->d.n_bodies || field < 0 || field >= MEL_STRESS_FIELD_COUNT)
+        return MEL_E_ARG;
+    const size_t nn = c->model.bodies[body].n_nodes;
+    if (n < nn || !c->st_out) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        HIP_OK(hipMemcpyAsync(out, c->st_out + (size_t)field * c->d.n_nodes + c->h_node_off[body], nn * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        return MEL_OK;
+    });
+}
+
+mel_status mel_write_graphic(mel_ctx* c, const char* path, const int* to_plot, int number_print) {
+    if (!c || !path) return MEL_E_ARG;
+    mel_status st = mel_compute_nodal_stresses(c);
+    if (st != MEL_OK) return st;
+    int plot[GRAPHIC_FIELDS];
+    for (int k = 0; k < GRAPHIC_FIELDS; k++) plot[k] = to_plot ? to_plot[k] : ((k <= 12) || (k >= 14 && k <= 28));
+    const Model& m = c->model;
+    std::vector<GraphicBody> g(m.bodies.size());
+    static const int vec_fields[10][2] = {
+        {MEL_X_DISP, MEL_Y_DISP}, {MEL_X_VEL, MEL_Y_VEL}, {MEL_X_ACC, MEL_Y_ACC}, {MEL_X_FORCE, MEL_Y_FORCE},
+        {MEL_X_INTERNAL_FORCE, MEL_Y_INTERNAL_FORCE}, {MEL_X_CONTACT_FORCE, MEL_Y_CONTACT_FORCE},
+        {MEL_X_BODY_FORCE, MEL_Y_BODY_FORCE}, {MEL_X_DIRICHLET_FORCE, MEL_Y_DIRICHLET_FORCE},
+        {MEL_X_NEUMANN_FORCE, MEL_Y_NEUMANN_FORCE}, {MEL_X_DAMPING_FORCE, MEL_Y_DAMPING_FORCE}};
+    for (size_t bi = 0; bi < m.bodies.size(); bi++) {
+        const Body& b = m.bodies[bi];
+        GraphicBody& G = g[bi];
+        const size_t n = b.n_nodes;
+        G.active = b.status != ST_INACTIVE;
+        G.x0 = b.X;
+        G.y0 = b.Y;
+        G.x.resize(n);
+        G.y.resize(n);
+        auto get = [&](int field, double* out) { return mel_get_node_field(c, (int)bi, field, out, n); };
+        if ((st = get(MEL_X_CURRENT, G.x.data())) != MEL_OK || (st = get(MEL_Y_CURRENT, G.y.data())) != MEL_OK)
+            return st;
+        G.vec.assign(10, std::vector<double>(2 * n));
+        for (int v = 0; v < 10; v++)
+            if ((st = get(vec_fields[v][0], G.vec[v].data())) != MEL_OK ||
+                (st = get(vec_fields[v][1], G.vec[v].data() + n)) != MEL_OK)
+                return st;
+        G.stress.resize((size_t)MEL_STRESS_FIELD_COUNT * n);
+        for (int f = 0; f < MEL_STRESS_FIELD_COUNT; f++)
+            if ((st = mel_get_nodal_stress(c, (int)bi, f, G.stress.data() + (size_t)f * n, n)) != MEL_OK) return st;
+        double rigid26[26];
+        if ((st = mel_get_body_state(c, (int)bi, rigid26, G.works)) != MEL_OK) return st;
+    }
+    mel_info_t info;
+    if ((st = mel_get_info(c, &info)) != MEL_OK) return st;
+    std::string err;
+    if (!write_graphic_text(path, m, g, info.iteration, m.number_save, number_print, info.time, plot, err)) {
+        c->err = err;
+        return MEL_E_IO;
+    }
+    return MEL_OK;
 }
 
 mel_status mel_get_body_sizes(mel_ctx* c, int body, int* n_nodes, int* n_gauss, int* n_borders, int* rigid) {

@@ -1,6 +1,7 @@
 // Model-file parsing (ASCII and binary) into mel::Model.  See model_io.hpp.
 #include "model_io.hpp"
 
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1030,6 +1031,103 @@ bool write_dynamic_text(const std::string& path, const Model& m, std::string& er
             }
         }
         std::fprintf(f, " \n");
+    }
+    std::fclose(f);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Graphic writer (layout of Write_graphic, Code/Graphic.h:21-946)
+// ---------------------------------------------------------------------------
+bool write_graphic_text(const std::string& path, const Model& m, const std::vector<GraphicBody>& g, int iteration,
+                        int number_save, int number_print, double time, const int* to_plot, std::string& err) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) { err = "cannot write " + path; return false; }
+    const double xmin = m.xmin_period, xmax = m.xmax_period, period = xmax - xmin;
+    // points (body, node, x, y) and triangles of every active body
+    struct Pt { int b, i; double x, y; };
+    std::vector<Pt> pts;
+    std::vector<std::array<int, 3>> tris;
+    for (size_t bi = 0; bi < m.bodies.size(); bi++) {
+        const Body& b = m.bodies[bi];
+        const GraphicBody& G = g[bi];
+        if (!G.active) continue;
+        const int base = (int)pts.size();
+        double ymin = 1.e6, ymax = -1.e6;
+        for (int i = 0; i < b.n_nodes; i++) {
+            const double x = G.x[i] - period * std::floor((G.x[i] - xmin) / period);
+            pts.push_back({(int)bi, i, x, G.y[i]});
+            ymin = std::min(ymin, G.y[i]);
+            ymax = std::max(ymax, G.y[i]);
+        }
+        // a cell with vertices within `dist` of both ends of the period
+        // crosses the seam: it is drawn at each end, its vertices from the
+        // other end copied across by one period
+        const double dist = b.n_nodes > 0 ? 6 * std::sqrt((period * (ymax - ymin)) / b.n_nodes) : 0.;
+        for (size_t c = 0; c + 2 < b.cells.size(); c += 3) {
+            const int v[3] = {b.cells[c], b.cells[c + 1], b.cells[c + 2]};
+            bool lo = false, hi = false;
+            for (int k = 0; k < 3; k++) {
+                lo |= pts[base + v[k]].x <= xmin + dist;
+                hi |= pts[base + v[k]].x >= xmax - dist;
+            }
+            if (!(lo && hi)) {
+                tris.push_back({base + v[0], base + v[1], base + v[2]});
+                continue;
+            }
+            for (int side = 0; side < 2; side++) {
+                std::array<int, 3> t;
+                for (int k = 0; k < 3; k++) {
+                    const Pt& p = pts[base + v[k]];
+                    const bool far = side == 0 ? p.x >= xmax - dist : p.x <= xmin + dist;
+                    if (!far) { t[k] = base + v[k]; continue; }
+                    t[k] = (int)pts.size();
+                    pts.push_back({p.b, p.i, p.x + (side == 0 ? -period : period), p.y});
+                }
+                tris.push_back(t);
+            }
+        }
+    }
+    auto clamp = [](double v) { return std::fabs(v) < 1.e-16 ? 0. : v; };
+    std::fprintf(f, "# vtk DataFile Version 2.0\nMELODY 2D graphic file ; Iteration %d ; Save %d ; Print %d ; Time %.12g\n"
+                    "ASCII\n\nDATASET POLYDATA\n\n", iteration, number_save, number_print, time);
+    std::fprintf(f, "POINTS %zu float\n", pts.size());
+    for (const Pt& p : pts) std::fprintf(f, "%.12g %.12g 0\n", clamp(p.x), clamp(p.y));
+    std::fprintf(f, "\nPOLYGONS %zu %zu\n", tris.size(), 4 * tris.size());
+    for (const auto& t : tris) std::fprintf(f, "3 %d %d %d\n", t[0], t[1], t[2]);
+    std::fprintf(f, "\nPOINT_DATA %zu\n", pts.size());
+    static const char* names[GRAPHIC_FIELDS] = {
+        "00_Body_index", "01_Initial_position", "02_Current_position", "03_Displacement", "04_Velocity",
+        "05_Acceleration", "06_Force", "07_Internal_force", "08_Contact_force", "09_Body_force", "10_Dirichlet_force",
+        "11_Neumann_force", "12_Damping_force", "13_Alid_force", "14_Jacobian", "15_Cauchy_XX_stress",
+        "16_Cauchy_YY_stress", "17_Cauchy_XY_stress", "18_Cauchy_ZZ_stress", "19_Tresca_stress",
+        "20_Von_Mises_stress", "21_Major_principal_stress", "22_Intermediate_principal_stress",
+        "23_Minor_principal_stress", "24_Spherical_stress", "25_Green-Lagrange_XX_Strain",
+        "26_Green-Lagrange_YY_Strain", "27_Green-Lagrange_XY_Strain", "28_Green-Lagrange_Norm", "29_Body_Damage",
+        "30_Normalized_displacement_error", "31_Displacement_error", "32_Internal_work", "33_Contact_work",
+        "34_Body_work", "35_Dirichlet_work", "36_Neumann_work", "37_Damping_work", "38_Alid_work", "39_Temperature",
+        "40-Epsilon_Mass_Scaling"};
+    // fields of the 19 stress fields behind slots 14-28 (mel_stress_field order)
+    static const int stress_of[15] = {3, 8, 9, 10, 11, 12, 13, 0, 1, 2, 14, 15, 16, 17, 18};
+    for (int s = 0; s < GRAPHIC_FIELDS; s++) {
+        if (!to_plot[s]) continue;
+        const bool two = (s >= 1 && s <= 13) || s == 31;
+        std::fprintf(f, "SCALARS %s float %d\nLOOKUP_TABLE default\n", names[s], two ? 2 : 1);
+        for (const Pt& p : pts) {
+            const GraphicBody& G = g[p.b];
+            const size_t n = G.x.size();
+            double a = 0., b = 0.;
+            if (s == 0) { std::fprintf(f, "%d\n", p.b); continue; }
+            if (s == 1) { a = G.x0[p.i]; b = G.y0[p.i]; }
+            else if (s == 2) { a = G.x[p.i]; b = G.y[p.i]; }
+            else if (s >= 3 && s <= 12) { const std::vector<double>& v = G.vec[s - 3]; a = v[p.i]; b = v[n + p.i]; }
+            else if (s >= 14 && s <= 28) { a = G.stress.empty() ? (s == 14 ? 1. : 0.) : G.stress[(size_t)stress_of[s - 14] * n + p.i]; }
+            else if (s >= 32 && s <= 37) a = G.works[s - 32];   // internal contact body dirichlet neumann damping
+            if (two) std::fprintf(f, "%.6g %.6g\n", clamp(a), clamp(b));
+            else if (s == 14 || s >= 32) std::fprintf(f, "%.6g\n", a);
+            else std::fprintf(f, "%.6g\n", clamp(a));
+        }
+        std::fprintf(f, "\n");
     }
     std::fclose(f);
     return true;

@@ -151,4 +151,26 @@ void initialize_segments(Model& m, Body& b);
 // 17 significant digits).
 bool write_dynamic_text(const std::string& path, const Model& m, std::string& err);
 
+// What Write_graphic draws of one body (Code/Graphic.h:21-946): positions,
+// node fields (x then y per vector field), the 19 nodal stress fields of
+// Body::Compute_nodal_stresses, the body's works.
+struct GraphicBody {
+    bool active = true;
+    std::vector<double> x0, y0, x, y;
+    std::vector<std::vector<double>> vec;    // displacement, velocity, acceleration, force, internal, contact,
+                                             // body, dirichlet, neumann, damping: 2 * n values each
+    std::vector<double> stress;              // 19 fields of n values
+    double works[7] = {0, 0, 0, 0, 0, 0, 0};
+};
+
+constexpr int GRAPHIC_FIELDS = 41;           // To_Plot slots 00-40 of Code/Graphic.h
+
+// VTK legacy POLYDATA of every active body in the layout of Write_graphic
+// (Code/Graphic.h:21-946): positions wrapped into the period, cells that
+// cross the periodic seam drawn on both sides, the point fields whose
+// To_Plot flag is set (fields this library does not model - ALID, damage,
+// error estimates, temperature, mass-scaling epsilon - written as zeros).
+bool write_graphic_text(const std::string& path, const Model& m, const std::vector<GraphicBody>& g, int iteration,
+                        int number_save, int number_print, double time, const int* to_plot, std::string& err);
+
 }  // namespace mel

@@ -53,6 +53,11 @@ class JobInfo(ctypes.Structure):
                 ("finished", ctypes.c_double), ("device_bytes", ctypes.c_size_t), ("n_nodes", ctypes.c_int)]
 
 
+# mel_stress_field order
+STRESS_FIELDS = ["sigma_I", "sigma_II", "sigma_III", "jacobian", "Sxx", "Syy", "Sxy", "Szz", "sigma_xx", "sigma_yy",
+                 "sigma_xy", "sigma_zz", "sigma_tresca", "sigma_vm", "sigma_sph", "Exx", "Eyy", "Exy", "norm_E"]
+
+
 class Options(ctypes.Structure):
     """mel_options: load-time launch configuration (-1 = library default)."""
     _fields_ = [("block_nodes", ctypes.c_int), ("fused_step", ctypes.c_int),
@@ -100,6 +105,10 @@ def library():
             "mel_get_body_state": ([vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)], ctypes.c_int),
             "mel_get_body_sizes": ([vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
                                     P(ctypes.c_int)], ctypes.c_int),
+            "mel_compute_nodal_stresses": ([vp], ctypes.c_int),
+            "mel_write_graphic": ([vp, ctypes.c_char_p, P(ctypes.c_int), ctypes.c_int], ctypes.c_int),
+            "mel_get_nodal_stress": ([vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_double), ctypes.c_size_t],
+                                     ctypes.c_int),
             "mel_get_border_field": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_double),
                                       ctypes.c_size_t], ctypes.c_int),
             "mel_get_body_energy": ([vp, ctypes.c_int, P(ctypes.c_double)], ctypes.c_int),
@@ -144,7 +153,8 @@ def library():
 def exported_symbols():
     return ["mel_load", "mel_load_binary", "mel_load_ex", "mel_load_binary_ex", "mel_default_options",
             "mel_get_options", "mel_autotune", "mel_update_proximity", "mel_euler_steps", "mel_synchronize",
-            "mel_get_node_field", "mel_set_node_field", "mel_get_body_sizes", "mel_get_contacts", "mel_get_neighbours",
+            "mel_get_node_field", "mel_set_node_field", "mel_get_body_sizes", "mel_compute_nodal_stresses", "mel_write_graphic",
+            "mel_get_nodal_stress", "mel_get_contacts", "mel_get_neighbours",
             "mel_get_body_state", "mel_get_body_energy", "mel_get_scalars", "mel_eval_forces",
             "mel_eval_forces_noalloc", "mel_get_state_size", "mel_save_state", "mel_load_state",
             "mel_get_border_field", "mel_get_info", "mel_write_dynamic", "mel_get_stream",
@@ -279,6 +289,29 @@ class Context:
         self._check(self.lib.mel_get_neighbours(self._h, body, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                                 cnt.value, ctypes.byref(cnt)))
         return out[:3 * cnt.value]
+
+    def nodal_stresses(self):
+        """Body::Compute_nodal_stresses at the current state: one array of
+        shape (len(STRESS_FIELDS), nodes) per body (mel_compute_nodal_stresses,
+        mel_get_nodal_stress)."""
+        self._check(self.lib.mel_compute_nodal_stresses(self._h))
+        out = []
+        for b in range(self.get_info().n_bodies):
+            n = self.body_sizes(b)[0]
+            a = np.zeros((len(STRESS_FIELDS), n))
+            for f in range(len(STRESS_FIELDS)):
+                self._check(self.lib.mel_get_nodal_stress(self._h, b, f, a[f].ctypes.data_as(
+                    ctypes.POINTER(ctypes.c_double)), n))
+            out.append(a)
+        return out
+
+    def write_graphic(self, path, to_plot=None, number_print=0):
+        """Write_graphic's VTK file of the current state (mel_write_graphic);
+        to_plot: 41 flags of the reference's To_Plot slots, or None."""
+        arr = None
+        if to_plot is not None:
+            arr = (ctypes.c_int * 41)(*[int(bool(x)) for x in to_plot])
+        self._check(self.lib.mel_write_graphic(self._h, os.fsencode(path), arr, int(number_print)))
 
     def body_sizes(self, body):
         """(nodes, Gauss points, borders, rigid) of a body (mel_get_body_sizes)."""

@@ -38,6 +38,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1348,6 +1349,91 @@ std::vector<double> body_energy(const Sim& S, int bi) {
     return {ke, kp, dw};
 }
 
+// Body::Compute_nodal_stresses (Code/Body.h:2310-2519): stresses, strains,
+// principal and equivalent stresses at every node of body bi from the
+// derivatives of its DOF_TO_DISP row (its own nodes' displacement
+// parameters), with the materials of Code/Model.h:19-73; 19 node fields one
+// after another (SigmaI, SigmaII, SigmaIII, jacobian, Sxx, Syy, Sxy, Szz,
+// Sigmaxx, Sigmayy, Sigmaxy, Sigmazz, SigmaTresca, SigmaVM, SigmaSph, Exx,
+// Eyy, Exy, NormE).  Rigid bodies: zeros, jacobian 1.
+std::vector<double> nodal_stresses(const Sim& S, int bi) {
+    const Body& b = S.m.bodies[bi];
+    const State& s = S.s[bi];
+    const int n = b.n_nodes;
+    std::vector<double> out(19 * (size_t)n, 0.);
+    auto put = [&](int f, int i, double v) { out[(size_t)f * n + i] = v; };
+    if (b.rigid) {
+        for (int i = 0; i < n; i++) put(3, i, 1.);
+        return out;
+    }
+    const Material& mat = S.m.materials[b.material];
+    for (int i = 0; i < n; i++) {
+        double J = 0, S11 = 0, S12 = 0, S22 = 0, S33 = 0;
+        double Sxx = 0, Syy = 0, Sxy = 0, Szz = 0, Exx = 0, Eyy = 0, Exy = 0, NormE = 0;
+        if (mat.type == MAT_ELASTIC_LINEAR) {
+            double E11 = 0, E12 = 0, E22 = 0;
+            for (int k = b.dptr[i]; k < b.dptr[i + 1]; k++) {
+                const int j = b.didx[k];
+                E11 += b.dphix[k] * s.upx[j];
+                E12 += b.dphix[k] * s.upy[j] + b.dphiy[k] * s.upx[j];
+                E22 += b.dphiy[k] * s.upy[j];
+            }
+            E12 = 0.5 * E12;
+            const double Mu2 = 2 * mat.p[3], Lambda = mat.p[4];
+            J = 1. + E11 + E22;
+            S33 = Lambda * (J - 1.);
+            S11 = S33 + Mu2 * E11;
+            S22 = S33 + Mu2 * E22;
+            S12 = Mu2 * E12;
+            if (J < 0.) { S11 = 0.; S12 = 0.; S22 = 0.; S33 = 0.; }
+            Sxx = S11; Syy = S22; Sxy = S12; Szz = S33;
+            Exx = E11; Eyy = E22; Exy = E12;
+        } else {
+            double F11 = 1., F12 = 0., F21 = 0., F22 = 1.;
+            for (int k = b.dptr[i]; k < b.dptr[i + 1]; k++) {
+                const int j = b.didx[k];
+                F11 += b.dphix[k] * s.upx[j];
+                F12 += b.dphiy[k] * s.upx[j];
+                F21 += b.dphix[k] * s.upy[j];
+                F22 += b.dphiy[k] * s.upy[j];
+            }
+            const double Mu = mat.p[3], Kappa = mat.p[4];
+            const double F11F12 = F11 * F11 + F12 * F12, F21F22 = F21 * F21 + F22 * F22;
+            J = F11 * F22 - F12 * F21;
+            const double invJ = 1. / J, K = Kappa * (J - 1.);
+            const double MuJ53 = Mu * std::pow(J, -1.666666666667);
+            const double trB3 = 0.333333333333 * (F11F12 + F21F22 + 1.);
+            const double s11 = K + MuJ53 * (F11F12 - trB3), s22 = K + MuJ53 * (F21F22 - trB3);
+            const double s12 = MuJ53 * (F11 * F21 + F12 * F22);
+            S11 = (F12 * (F12 * s22 - F22 * s12) - F22 * (F12 * s12 - F22 * s11)) * invJ;
+            S22 = (F11 * (F11 * s22 - F21 * s12) - F21 * (F11 * s12 - F21 * s11)) * invJ;
+            S12 = (F21 * (F12 * s12 - F22 * s11) - F11 * (F12 * s22 - F22 * s12)) * invJ;
+            S33 = (K + MuJ53 * (1. - trB3)) * J;
+            if (J < 0.) { F11 = 0.; F12 = 0.; F21 = 0.; F22 = 0.; S11 = 0.; S12 = 0.; S22 = 0.; S33 = 0.; }
+            Sxx = F11 * (S11 * F11 + S12 * F12) + F12 * (S12 * F11 + S22 * F12);
+            Syy = F21 * (S11 * F21 + S12 * F22) + F22 * (S12 * F21 + S22 * F22);
+            Sxy = F21 * (S11 * F11 + S12 * F12) + F22 * (S12 * F11 + S22 * F12);
+            Szz = S33;
+            Exx = 0.5 * (F11 * F11 + F21 * F21 - 1.);
+            Eyy = 0.5 * (F12 * F12 + F22 * F22 - 1.);
+            Exy = 0.5 * (F12 * F11 + F22 * F21);
+        }
+        NormE = std::pow(Exx * Exx + Eyy * Eyy + 2. * Exy * Exy, 0.5);
+        const double root = std::sqrt((Sxx + Syy) * (Sxx + Syy) - 4. * (Sxx * Syy - Sxy * Sxy));
+        const double A = 0.5 * (Sxx + Syy + root), Bv = 0.5 * (Sxx + Syy - root);
+        double s1, s2, s3;
+        if (A > Bv && A > Szz) { s1 = A; if (Bv > Szz) { s2 = Bv; s3 = Szz; } else { s2 = Szz; s3 = Bv; } }
+        else if (Bv > A && Bv > Szz) { s1 = Bv; if (A > Szz) { s2 = A; s3 = Szz; } else { s2 = Szz; s3 = A; } }
+        else { s1 = Szz; if (Bv > A) { s2 = Bv; s3 = A; } else { s2 = A; s3 = Bv; } }
+        const double vm = std::sqrt((Sxx - Syy) * (Sxx - Syy) + (Sxx - Szz) * (Sxx - Szz) + (Szz - Syy) * (Szz - Syy) +
+                                    6. * Sxy * Sxy) * 0.70710678118655;
+        const double v[19] = {s1, s2, s3, J, S11, S22, S12, S33, Sxx, Syy, Sxy, Szz, s1 - s3, vm,
+                              0.333333333 * (Sxx + Syy + Szz), Exx, Eyy, Exy, NormE};
+        for (int f = 0; f < 19; f++) put(f, i, v[f]);
+    }
+    return out;
+}
+
 void dump(Sim& S, const std::string& path) {
     DumpWriter w(path);
     w.scalar("time", S.time);
@@ -1397,6 +1483,7 @@ void dump(Sim& S, const std::string& path) {
         }
         w.i32(p + "contacts_i", ci);
         w.f64(p + "contacts_d", cd);
+        if (std::getenv("MEL_DUMP_STRESS")) w.f64(p + "stress", nodal_stresses(S, (int)bi));
         for (size_t k = 0; k < b.borders.size(); k++) {
             std::string q = p + "border" + std::to_string(k) + ".";
             w.f64(q + "length", s.length[k]);

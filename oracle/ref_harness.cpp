@@ -94,6 +94,9 @@ struct FilteredWriter {
     void scalar(const string& n, double x) { if (keep_record(n)) w.scalar(n, x); }
 };
 
+// materials of the run, for the nodal stresses dumped when MEL_DUMP_STRESS is set
+static vector<Material>* g_materials = nullptr;
+
 static void dump_state(const string& path, vector<Body>& Bodies, double Time, int iteration, double Deltat) {
     FilteredWriter w(path);
     if (!w.ok()) { cerr << "cannot write " << path << endl; exit(2); }
@@ -166,6 +169,21 @@ static void dump_state(const string& path, vector<Body>& Bodies, double Time, in
         }
         w.i32(p + "contacts_i", ci);
         w.f64(p + "contacts_d", cd);
+        if (getenv("MEL_DUMP_STRESS") && g_materials) {
+            // Body::Compute_nodal_stresses (Code/Body.h:2310-2519), its 19
+            // node fields one after another
+            B.Compute_nodal_stresses((int)g_materials->size(), *g_materials);
+            vector<double> st;
+            st.reserve(19 * (size_t)n);
+#define STRESS_FIELD(EXPR) for (int i = 0; i < n; i++) { Node& N = B.nodes[i]; st.push_back(EXPR); }
+            STRESS_FIELD(N.SigmaI) STRESS_FIELD(N.SigmaII) STRESS_FIELD(N.SigmaIII) STRESS_FIELD(N.jacobian)
+            STRESS_FIELD(N.Sxx) STRESS_FIELD(N.Syy) STRESS_FIELD(N.Sxy) STRESS_FIELD(N.Szz)
+            STRESS_FIELD(N.Sigmaxx) STRESS_FIELD(N.Sigmayy) STRESS_FIELD(N.Sigmaxy) STRESS_FIELD(N.Sigmazz)
+            STRESS_FIELD(N.SigmaTresca) STRESS_FIELD(N.SigmaVM) STRESS_FIELD(N.SigmaSph)
+            STRESS_FIELD(N.Exx) STRESS_FIELD(N.Eyy) STRESS_FIELD(N.Exy) STRESS_FIELD(N.NormE)
+#undef STRESS_FIELD
+            w.f64(p + "stress", st);
+        }
         for (int k = 0; k < B.nb_borders; k++) {
             string q = p + "border" + to_string(k) + ".";
             w.f64(q + "length", B.borders[k].length);
@@ -228,6 +246,7 @@ int main(int argc, char** argv) {
     vector<int> To_Plot(41);
     vector<vector<int>> Contacts_Table;
 
+    g_materials = &Materials;
     Load_static(Simulation_name, Nb_materials, Materials, Nb_contact_laws, Contact_laws, Contacts_Table,
                 Solver, Tini, Deltat, Tend, Target_error, Inv_Target_error, Control_parameter, Accepted_ratio,
                 Max_mass_scaling, Control_parameter_mass_scaling, Error_factor_mass_scaling,

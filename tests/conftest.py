@@ -13,8 +13,11 @@ GOLDEN = os.path.join(HERE, "golden")
 # golden sets that carry their model files; c1.tar.gz carries the
 # reference's dumps and the hash of a model the generator rebuilds
 VARIANTS = sorted(f[:-7] for f in os.listdir(GOLDEN)
-                  if f.endswith(".tar.gz") and f != "c1.tar.gz" and not f.endswith("_restart.tar.gz"))
+                  if f.endswith(".tar.gz") and f != "c1.tar.gz" and not f.endswith("_restart.tar.gz")
+                  and not f.endswith("_stress.tar.gz"))
 RESTARTS = sorted(f[:-15] for f in os.listdir(GOLDEN) if f.endswith("_restart.tar.gz"))
+# nodal stress sets: model files and the reference's stress fields
+STRESSES = sorted(f[:-14] for f in os.listdir(GOLDEN) if f.endswith("_stress.tar.gz"))
 # records a restart does not carry over at its step 0: the force fields and
 # pressures Load_dynamic reads back (Code/IO.h:2055-2130, 2292-2340; the
 # internal force is then zeroed by Node::Sum_up_forces, Code/Node.h:212-225)

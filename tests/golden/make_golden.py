@@ -21,6 +21,9 @@ steps and writes its state with the library's DYNAMIC writer
 and the initialisation block) and its dumps at steps 0, 1, 10 and 50 are
 stored beside the file.
 
+Stress sets <variant>_stress.tar.gz: the model files and the reference's
+nodal stress fields (Body::Compute_nodal_stresses) at steps 0, 10 and 50.
+
 Run from the repository root:  python tests/golden/make_golden.py [name,...]
 """
 import hashlib
@@ -45,6 +48,13 @@ C1_FIELDS = ("x_current,y_current,x_disp_param,y_disp_param,x_vel_param,y_vel_pa
              "contacts_i,contacts_d,length")
 RESTART = ["tiny_czm", "tiny_bc", "tiny_holed", "tiny_periodic"]
 RESTART_STEPS = [0, 1, 10, 50]
+# nodal stress sets <name>_stress.tar.gz: the model files and the
+# reference's Body::Compute_nodal_stresses fields (the harness's "stress"
+# record, MEL_DUMP_STRESS) with the displacement parameters they come from;
+# tiny_free has linear-elastic grains, the others neo-Hookean ones and rigid
+# platens
+STRESS = ["tiny_platen", "tiny_bc", "tiny_free"]
+STRESS_STEPS = [0, 10, 50]
 REF = os.path.join(ROOT, "oracle", "_ref", "melody_ref")
 
 
@@ -84,15 +94,35 @@ def make_restart(name, tmp):
     print(name + "_restart", os.path.getsize(out))
 
 
+def make_stress(name, tmp):
+    from melody2d_amd.preprocessor.configs import tiny_free
+    mdir = os.path.join(tmp, "model")
+    odir = os.path.join(tmp, "ref")
+    os.makedirs(odir)
+    write_model(tiny_free() if name == "tiny_free" else variants()[name], mdir)
+    env = dict(os.environ, MEL_DUMP_STRESS="1", MEL_DUMP_FIELDS="stress,x_disp_param,y_disp_param")
+    run_ref(mdir, odir, STRESS_STEPS, env)
+    out = os.path.join(HERE, name + "_stress.tar.gz")
+    with tarfile.open(out, "w:gz", compresslevel=9) as tf:
+        for f in sorted(os.listdir(mdir)):
+            tf.add(os.path.join(mdir, f), arcname="model/" + f)
+        for f in sorted(os.listdir(odir)):
+            tf.add(os.path.join(odir, f), arcname="ref/" + f)
+    print(name + "_stress", os.path.getsize(out))
+
+
 def main(names=None):
     if not os.path.exists(REF):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     todo = dict(variants())
     todo["c1"] = None
-    for name in (names or list(todo) + [r + "_restart" for r in RESTART]):
+    for name in (names or list(todo) + [r + "_restart" for r in RESTART] + [r + "_stress" for r in STRESS]):
         with tempfile.TemporaryDirectory() as tmp:
             if name.endswith("_restart"):
                 make_restart(name[:-len("_restart")], tmp)
+                continue
+            if name.endswith("_stress"):
+                make_stress(name[:-len("_stress")], tmp)
                 continue
             mdir = os.path.join(tmp, "model")
             odir = os.path.join(tmp, "ref")

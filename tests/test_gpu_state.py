@@ -136,3 +136,92 @@ def test_restart_golden_matches_oracle(name, tmp_path):
         bad = sorted(((k, v[0]) for k, v in res.items() if not v[0] <= 1e-12), key=lambda kv: -kv[1])
         assert not bad, (s, bad[:6])
     b.close()
+
+
+@pytest.mark.parametrize("name", __import__("conftest").STRESSES)
+def test_nodal_stresses_match_reference(name, tmp_path):
+    """mel_compute_nodal_stresses (Body::Compute_nodal_stresses,
+    Code/Body.h:2310-2519) against the reference's stress fields at steps 0,
+    10 and 50 of the same run: within 1e-11 of each field's largest value
+    (the trajectory agrees to ~1e-12 and pow rounds on the device as on the
+    host to within an ulp)."""
+    from conftest import extract_golden, golden_steps
+    from mel_dump import read_dump
+    print("case nodal stresses", name, flush=True)
+    model, ref = extract_golden(name + "_stress", str(tmp_path / "g"))
+    ctx = native.Context(model)
+    done = 0
+    for s in golden_steps(ref):
+        ctx.euler_steps(s - done)
+        done = s
+        R = read_dump(os.path.join(ref, "step_%06d.bin" % s))
+        got = ctx.nodal_stresses()
+        for b, a in enumerate(got):
+            want = R["b%d.stress" % b].reshape(19, -1)
+            assert a.shape == want.shape
+            for f in range(19):
+                scale = max(np.abs(want[f]).max(), 1e-300)
+                err = np.abs(a[f] - want[f]).max() / scale
+                assert err <= 1e-11, (s, b, native.STRESS_FIELDS[f], err)
+    ctx.close()
+
+
+def _read_vtk(path):
+    lines = open(path).read().split("\n")
+    out = {"fields": {}}
+    i = 0
+    while i < len(lines):
+        ln = lines[i]
+        if ln.startswith("POINTS"):
+            n = int(ln.split()[1])
+            out["points"] = np.array([[float(v) for v in lines[i + 1 + k].split()] for k in range(n)])
+            i += n
+        elif ln.startswith("POLYGONS"):
+            n = int(ln.split()[1])
+            out["cells"] = np.array([[int(v) for v in lines[i + 1 + k].split()] for k in range(n)])
+            i += n
+        elif ln.startswith("SCALARS"):
+            name, comps = ln.split()[1], int(ln.split()[3])
+            npts = len(out["points"])
+            out["fields"][name] = np.array([[float(v) for v in lines[i + 2 + k].split()] for k in range(npts)])
+            assert out["fields"][name].shape == (npts, comps)
+            i += 1 + npts
+        i += 1
+    return out
+
+
+@pytest.mark.parametrize("name", ["tiny_platen", "tiny_periodic"])
+def test_graphic_file_holds_the_state(name, golden_dir, tmp_path):
+    """mel_write_graphic (Write_graphic, Code/Graphic.h:21-946): a VTK
+    POLYDATA file whose points are every active node (periodic positions
+    wrapped into the period, seam cells drawn on both sides), whose triangles
+    reference existing points, and whose fields are the state's (6
+    significant digits, as the reference prints them)."""
+    print("case graphic", name, flush=True)
+    model, _ = golden_dir(name)
+    ctx = native.Context(model)
+    ctx.euler_steps(30)
+    path = str(tmp_path / "GRAPHIC_00001.vtk")
+    ctx.write_graphic(path, number_print=1)
+    G = _read_vtk(path)
+    info = ctx.get_info()
+    sizes = [ctx.body_sizes(b)[0] for b in range(info.n_bodies)]
+    assert len(G["points"]) >= sum(sizes)
+    assert G["cells"][:, 0].max() == 3 and G["cells"][:, 1:].max() < len(G["points"])
+    stress = ctx.nodal_stresses()
+    F = G["fields"]
+    assert "03_Displacement" in F and "20_Von_Mises_stress" in F and "32_Internal_work" not in F
+    for b, n in enumerate(sizes):
+        # a body's nodes, then the copies of its seam cells' vertices
+        first = int(np.argmax(F["00_Body_index"][:, 0] == b))
+        body = F["00_Body_index"][first:first + n, 0]
+        assert np.all(body == b)
+        for name_f, want in (("03_Displacement", np.stack([ctx.node_field(b, "x_disp", n), ctx.node_field(b, "y_disp", n)], 1)),
+                             ("08_Contact_force", np.stack([ctx.node_field(b, "x_contact_force", n),
+                                                            ctx.node_field(b, "y_contact_force", n)], 1)),
+                             ("20_Von_Mises_stress", stress[b][13][:, None]),
+                             ("21_Major_principal_stress", stress[b][0][:, None])):
+            got = F[name_f][first:first + n]
+            w = np.where(np.abs(want) < 1e-16, 0., want)
+            assert np.allclose(got, w, rtol=1e-5, atol=1e-300), (b, name_f)
+    ctx.close()
