@@ -152,6 +152,12 @@ mel_status mel_write_graphic(mel_ctx* ctx, const char* path, const int* to_plot,
  * (Body::nb_nodes, nb_gauss, nb_borders, type; Code/Body.h:24-60) */
 mel_status mel_get_body_sizes(mel_ctx* ctx, int body, int* n_nodes, int* n_gauss, int* n_borders, int* rigid);
 
+/* the 3 internal variables of each contact element of body (in the order of
+ * mel_get_contacts; Contact_element::internal, Code/Contact_Element.h:44:
+ * damage, and for BondedMohrCoulomb and CZMfatigue the last normal and
+ * tangential tractions; zeros where the law keeps fewer) */
+mel_status mel_get_contact_internals(mel_ctx* ctx, int body, double* out3, size_t cap, size_t* count);
+
 /* neighbour triples (body2 border2 shift) of body (Code/Proximity.h:160) */
 mel_status mel_get_neighbours(mel_ctx* ctx, int body, int32_t* out, size_t cap, size_t* count);
 

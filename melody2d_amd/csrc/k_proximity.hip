@@ -576,7 +576,7 @@ __device__ int scan_row(const DevModel& m, const RowInfo& ri, int base, int seg_
                 C.gapn[e] = p.gap; C.gapt[e] = 0.; C.xsi[e] = p.xsi;
                 C.xn[e] = p.xn; C.yn[e] = p.yn; C.xt[e] = p.xt; C.yt[e] = p.yt;
                 C.emass[e] = emass; C.len[e] = m.sl_len[slotS]; C.fx[e] = 0.; C.fy[e] = 0.;
-                C.nbint[e] = 0; C.int0[e] = 0.;
+                C.nbint[e] = 0; C.int0[e] = 0.; C.int1[e] = 0.; C.int2[e] = 0.;
                 C.vgapn[e] = 0.; C.vgapt[e] = 0.; C.wacc[e] = 0.;
                 C.fresh[e] = ri.self ? 2 : 1;
                 int4 inf, sinf;
@@ -682,6 +682,7 @@ __device__ inline void copy_element(const ContactArrays& D, int e, const Contact
     D.gapn[e] = S.gapn[o]; D.vgapn[e] = S.vgapn[o]; D.gapt[e] = S.gapt[o]; D.vgapt[e] = S.vgapt[o];
     D.xsi[e] = S.xsi[o]; D.xn[e] = S.xn[o]; D.yn[e] = S.yn[o]; D.xt[e] = S.xt[o]; D.yt[e] = S.yt[o];
     D.emass[e] = S.emass[o]; D.len[e] = S.len[o]; D.fx[e] = S.fx[o]; D.fy[e] = S.fy[o]; D.int0[e] = S.int0[o];
+    D.int1[e] = S.int1[o]; D.int2[e] = S.int2[o];
     D.info[e] = S.info[o]; D.sinfo[e] = S.sinfo[o];
 }
 
@@ -743,9 +744,15 @@ __global__ void k_init_czm(DevModel m) {
         const ContactArrays& C = m.C;
         int ms = m.body_material[C.bodyS[e]], mm = m.body_material[C.bodyM[e]];
         int li = m.law_first[ms * m.n_materials + mm];
-        if (li < 0 || m.law_type[li] != 6) continue;
-        C.nbint[e] = 1;
-        C.int0[e] = C.gapn[e] < m.law_p[12 * li + 5] ? 0. : 1.;
+        if (li < 0) continue;
+        // gapinit: parameter 5 of CZMlinear, 8 of CZMfatigue, 10 of BondedMohrCoulomb
+        const int lt = m.law_type[li];
+        const int gp = lt == 6 ? 5 : (lt == 7 ? 8 : (lt == 5 ? 10 : -1));
+        if (gp < 0) continue;
+        C.nbint[e] = lt == 6 ? 1 : 3;
+        C.int0[e] = C.gapn[e] < m.law_p[12 * li + gp] ? 0. : 1.;
+        C.int1[e] = 0.;
+        C.int2[e] = 0.;
     }
 }
 

@@ -110,6 +110,94 @@ __device__ inline void law_czm_linear(R kini, R Pnlim, R gapnlim, R Pnres, R NT,
     Pt = -kmax * gapt;
 }
 
+// Apply_CZM_fatigue (Code/Model.h:343-446): the linear cohesive law whose
+// damage also grows with the traction increments above the fatigue
+// threshold Pnfat; Pn, Pt enter as the element's previous tractions
+template <class R>
+__device__ inline void law_czm_fatigue(R kini, R Pnlim, R gapnres, R Pnres, R NT, R Pnfat, R Raten, R Ratet, R gapn,
+                                       R& gapt, bool& gapt_set, R& D, R& Pn, R& Pt) {
+    if (D < R(1)) {
+        R gnmax = Pnlim / kini * (R(1) - D) + gapnres * D;
+        R gtmax = gnmax * NT;
+        R kmax;
+        if (gapn < gnmax && fabs(gapt) < gtmax) {
+            const R Pnm = Pn, Ptm = Pt;
+            kmax = (Pnlim * (R(1) - D) + Pnres * D) / gnmax;
+            Pn = gapn < R(0) ? -kini * gapn : -kmax * gapn;
+            Pt = -kmax * gapt;
+            const R Pnmax = Pnlim * (R(1) - D) + Pnres * D;
+            if (fabs(Pn) > Pnfat && fabs(Pn) > fabs(Pnm))
+                D = D + (fabs(Pn) - Pnfat) / (Pnmax - Pnfat) * Raten * (fabs(Pn) - fabs(Pnm));
+            if (fabs(Pt) > NT * Pnfat && fabs(Pt) > fabs(Ptm))
+                D = D + (fabs(Pt) - Pnfat * NT) / (NT * (Pnmax - Pnfat)) * Ratet * (fabs(Pt) - fabs(Ptm));
+        } else {
+            bool grew = true;
+            if (gapn > gnmax && fabs(gapt) < gtmax) {
+                gnmax = gapn;
+                gtmax = gnmax * NT;
+            } else if (gapn < gnmax && fabs(gapt) > gtmax) {
+                gtmax = fabs(gapt);
+                gnmax = gtmax / NT;
+            } else if (gapn > gnmax && fabs(gapt) > gtmax) {
+                if (gapn / gnmax > fabs(gapt) / gtmax) { gnmax = gapn; gtmax = gnmax * NT; }
+                else { gtmax = fabs(gapt); gnmax = gtmax / NT; }
+            } else {
+                grew = false;     // on the elastic domain's boundary: tractions and damage unchanged
+            }
+            if (grew) {
+                D = (gnmax - Pnlim / kini) / (gapnres - Pnlim / kini);
+                kmax = (Pnlim * (R(1) - D) + Pnres * D) / gnmax;
+                Pn = gapn < R(0) ? -kini * gapn : -kmax * gapn;
+                Pt = -kmax * gapt;
+            }
+        }
+    }
+    if (D >= R(1)) {
+        D = R(1);
+        Pn = -kini * gapn;
+        Pt = -kini * gapt;
+        if (Pn < -Pnres) { Pn = R(0); Pt = R(0); gapt = R(0); gapt_set = true; }
+        else if (fabs(Pt) > Pnres * NT) {
+            Pt = gapt > R(0) ? -Pnres * NT : Pnres * NT;
+            gapt = -Pt / kini;
+            gapt_set = true;
+        }
+    }
+}
+
+// Apply_Bonded_Mohr_Coulomb (Code/Model.h:453-506): damped Mohr-Coulomb with
+// a bond (stiffness kbond in tension, its own friction, cohesion and tensile
+// strength) that breaks for good (damage 1) when exceeded
+template <class R>
+__device__ inline void law_bonded_mc(R kn, R kt, R kbond, R fricbond, R cohbond, R tensbond, R fricfree, R cohfree,
+                                     R tensfree, R damp, R mass, R length, R gapn, R vgapn, R& gapt, bool& gapt_set,
+                                     R vgapt, R& D, R& Pn, R& Pt) {
+    if (D >= R(1)) {
+        Pn = -kn * gapn - damp * vgapn * R(2) * sqrt_r(mass * kn / length);
+        if (Pn < -tensfree) { Pn = R(0); Pt = R(0); gapt = R(0); gapt_set = true; }
+        else Pt = -kt * gapt - damp * vgapt * R(2) * sqrt_r(mass * kt / length);
+        if (fabs(Pt) > cohfree + fricfree * Pn) {
+            Pt = gapt > R(0) ? -cohfree - fricfree * Pn : cohfree + fricfree * Pn;
+            gapt = -Pt / kt;
+            gapt_set = true;
+        }
+    } else {
+        if (gapn < R(0)) Pn = -kn * gapn - damp * vgapn * R(2) * sqrt_r(mass * kn / length);
+        else Pn = -kbond * gapn - damp * vgapn * R(2) * sqrt_r(mass * kbond / length);
+        Pt = -kt * gapt - damp * vgapt * R(2) * sqrt_r(mass * kt / length);
+        if (Pn < -tensbond) {
+            D = R(1);
+            if (Pn < -tensfree) { Pn = R(0); Pt = R(0); gapt = R(0); gapt_set = true; }
+        }
+        if (fabs(Pt) > cohbond + fricbond * Pn) {
+            D = R(1);
+            Pt = gapt > R(0) ? -cohfree - fricfree * Pn : cohfree + fricfree * Pn;
+            gapt = -Pt / kt;
+            gapt_set = true;
+        }
+    }
+}
+
 // Update_contact_forces' law step for one element (Code/Body.h:1694-1840
 // with the laws of Code/Model.h:81-335): tractions Pn, Pt and the damped
 // law's work rate W from the element's gaps, in the arithmetic type R.
@@ -118,7 +206,8 @@ __device__ inline void law_czm_linear(R kini, R Pnlim, R gapnlim, R Pnres, R NT,
 template <class R>
 __device__ __forceinline__ void contact_law(const DevModel& m, int lt, const double* p, double gapn_d, double& gapt_d,
                                             double vgn_d, double vgt_d, double emass, double length0, double& D_d,
-                                            int& nbint, double& Pn_o, double& Pt_o, double& W_o) {
+                                            int& nbint, double& P1_d, double& P2_d, double& Pn_o, double& Pt_o,
+                                            double& W_o) {
     const R gapn = (R)gapn_d;
     R gapt = (R)gapt_d;
     bool gapt_set = false;
@@ -173,6 +262,21 @@ __device__ __forceinline__ void contact_law(const DevModel& m, int lt, const dou
         law_czm_linear<R>((R)p[0], (R)p[1] * R(0.5), (R)p[2], (R)p[3] * R(0.5), (R)p[4], gapn, gapt, gapt_set, D, D_set,
                           Pn, Pt);
         if (D_set) D_d = (double)D;
+    } else if (lt == 5 || lt == 7) {    // BondedMohrCoulomb, CZMfatigue: damage and the previous tractions
+        R D = (R)D_d;
+        Pn = (R)P1_d;
+        Pt = (R)P2_d;
+        if (nbint == 0) { D = R(1); nbint = 3; }
+        if (lt == 5)
+            law_bonded_mc<R>((R)p[0], (R)p[1], (R)p[2], (R)p[3], (R)p[4] * R(0.5), (R)p[5] * R(0.5), (R)p[6],
+                             (R)p[7] * R(0.5), (R)p[8] * R(0.5), (R)p[9], (R)emass, (R)length0, gapn, (R)vgn_d, gapt,
+                             gapt_set, (R)vgt_d, D, Pn, Pt);
+        else
+            law_czm_fatigue<R>((R)p[0], (R)p[1] * R(0.5), (R)p[2], (R)p[3] * R(0.5), (R)p[4], (R)p[5], (R)p[6],
+                               (R)p[7], gapn, gapt, gapt_set, D, Pn, Pt);
+        D_d = (double)D;
+        P1_d = (double)Pn;
+        P2_d = (double)Pt;
     }
     if (gapt_set) gapt_d = (double)gapt;
     Pn_o = (double)Pn;
@@ -245,11 +349,16 @@ __device__ __forceinline__ void contact_item(const DevModel& m, int e) {
     const int lt = m.law_type[li];
     double D = int0;
     int nb = nbint;
-    if (m.law_f32) contact_law<float>(m, lt, p, gapn, gapt, r.vgapn, r.vgapt, emass, length0, D, nb, Pn, Pt, W);
-    else contact_law<double>(m, lt, p, gapn, gapt, r.vgapn, r.vgapt, emass, length0, D, nb, Pn, Pt, W);
-    if (lt == 6) {
+    // the three-variable laws' previous tractions (zero for a fresh element,
+    // whose law starts from a broken bond and does not read them)
+    double P1 = 0., P2 = 0.;
+    if ((lt == 5 || lt == 7) && nbint == 3) { P1 = C.int1[e]; P2 = C.int2[e]; }
+    if (m.law_f32) contact_law<float>(m, lt, p, gapn, gapt, r.vgapn, r.vgapt, emass, length0, D, nb, P1, P2, Pn, Pt, W);
+    else contact_law<double>(m, lt, p, gapn, gapt, r.vgapn, r.vgapt, emass, length0, D, nb, P1, P2, Pn, Pt, W);
+    if (lt == 6 || lt == 5 || lt == 7) {
         if (nb != nbint) C.nbint[e] = nb;
         C.int0[e] = D;
+        if (lt != 6) { C.int1[e] = P1; C.int2[e] = P2; }
     }
     const double Px = r.p.xn * Pn + r.p.xt * Pt;
     const double Py = r.p.yn * Pn + r.p.yt * Pt;

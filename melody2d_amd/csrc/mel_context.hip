@@ -75,7 +75,7 @@ inline int blocks(long long n, int b = BLOCK) { return (int)std::max<long long>(
 // uses the per-device address from hipGetSymbolAddress (scratch_address).
 // ---------------------------------------------------------------------------
 constexpr int SCRATCH_MAX_CONTACTS = 65536;
-constexpr int SCRATCH_DBL_FIELDS = 19, SCRATCH_INT_FIELDS = 18;
+constexpr int SCRATCH_DBL_FIELDS = 21, SCRATCH_INT_FIELDS = 18;
 constexpr size_t SCRATCH_BYTES =
     (size_t)SCRATCH_MAX_CONTACTS * (SCRATCH_DBL_FIELDS * sizeof(double) + SCRATCH_INT_FIELDS * sizeof(int));
 static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
@@ -88,7 +88,7 @@ void* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
 
 __device__ inline void contact_fields(const ContactArrays& A, double** dp, int** ip) {
     double* d[SCRATCH_DBL_FIELDS] = {A.N0, A.N1, A.N2, A.N3, A.gapn, A.vgapn, A.gapt, A.vgapt, A.xsi, A.xn,
-                                     A.yn, A.xt, A.yt, A.emass, A.len, A.fx, A.fy, A.int0, A.wacc};
+                                     A.yn, A.xt, A.yt, A.emass, A.len, A.fx, A.fy, A.int0, A.wacc, A.int1, A.int2};
     int* i[SCRATCH_INT_FIELDS] = {A.bodyS, A.borderS, A.bnodeS, A.nodeS, A.bodyM, A.borderM, A.shiftM, A.segM,
                                   A.nbint, A.fresh, A.nM0, A.nM1, A.nM2, A.nM3, A.sM0, A.sM1, A.sM2, A.sM3};
     for (int k = 0; k < SCRATCH_DBL_FIELDS; k++) dp[k] = d[k];
@@ -291,7 +291,7 @@ void alloc_contacts(mel_ctx& c, ContactArrays& A, int cap) {
                     &A.fresh, &A.nM0, &A.nM1, &A.nM2, &A.nM3, &A.sM0, &A.sM1, &A.sM2, &A.sM3};
     for (int** p : ints) *p = c.alloc<int>(cap);
     double** dbl[] = {&A.N0, &A.N1, &A.N2, &A.N3, &A.gapn, &A.vgapn, &A.gapt, &A.vgapt, &A.xsi, &A.xn, &A.yn,
-                      &A.xt, &A.yt, &A.emass, &A.len, &A.fx, &A.fy, &A.int0, &A.wacc};
+                      &A.xt, &A.yt, &A.emass, &A.len, &A.fx, &A.fy, &A.int0, &A.wacc, &A.int1, &A.int2};
     for (double** p : dbl) *p = c.alloc<double>(cap);
     A.info = c.alloc<int4>(cap);
     A.sinfo = c.alloc<int4>(cap);
@@ -1055,7 +1055,7 @@ void upload_loaded_contacts(mel_ctx& c) {
     const Model& m = c.model;
     const DevModel& d = c.d;
     std::vector<int> iv[18];
-    std::vector<double> dv[18];
+    std::vector<double> dv[20];
     for (size_t b = 0; b < m.bodies.size(); b++)
         for (const ContactElement& E : m.bodies[b].contacts) {
             // node ids are local to their body in the file, global on the device
@@ -1064,10 +1064,12 @@ void upload_loaded_contacts(mel_ctx& c) {
                                   E.segmentM, E.nb_internal, E.bodyM == (int)b ? 2 : 1, E.nodeM[0] + oM,
                                   E.nodeM[1] + oM, E.nodeM[2] + oM, E.nodeM[3] + oM, E.shiftMk[0], E.shiftMk[1],
                                   E.shiftMk[2], E.shiftMk[3]};
-            const double dbl[18] = {E.shapeM[0], E.shapeM[1], E.shapeM[2], E.shapeM[3], E.gapn, E.vgapn, E.gapt,
+            auto internal = [&](size_t k) { return k < E.internal.size() ? E.internal[k] : 0.; };
+            const double dbl[20] = {E.shapeM[0], E.shapeM[1], E.shapeM[2], E.shapeM[3], E.gapn, E.vgapn, E.gapt,
                                     E.vgapt, E.xsi, E.xnorm, E.ynorm, E.xtan, E.ytan, E.effective_mass, E.length,
-                                    E.fx, E.fy, E.internal.empty() ? 0. : E.internal[0]};
-            for (int k = 0; k < 18; k++) { iv[k].push_back(ints[k]); dv[k].push_back(dbl[k]); }
+                                    E.fx, E.fy, internal(0), internal(1), internal(2)};
+            for (int k = 0; k < 18; k++) iv[k].push_back(ints[k]);
+            for (int k = 0; k < 20; k++) dv[k].push_back(dbl[k]);
         }
     const int n = (int)iv[0].size();
     if (n > d.c_cap)
@@ -1077,12 +1079,12 @@ void upload_loaded_contacts(mel_ctx& c) {
         const ContactArrays& A = c.bufA;
         int* ip[18] = {A.bodyS, A.borderS, A.bnodeS, A.nodeS, A.bodyM, A.borderM, A.shiftM, A.segM, A.nbint,
                        A.fresh, A.nM0, A.nM1, A.nM2, A.nM3, A.sM0, A.sM1, A.sM2, A.sM3};
-        double* dp[18] = {A.N0, A.N1, A.N2, A.N3, A.gapn, A.vgapn, A.gapt, A.vgapt, A.xsi, A.xn, A.yn, A.xt, A.yt,
-                          A.emass, A.len, A.fx, A.fy, A.int0};
-        for (int k = 0; k < 18; k++) {
+        double* dp[20] = {A.N0, A.N1, A.N2, A.N3, A.gapn, A.vgapn, A.gapt, A.vgapt, A.xsi, A.xn, A.yn, A.xt, A.yt,
+                          A.emass, A.len, A.fx, A.fy, A.int0, A.int1, A.int2};
+        for (int k = 0; k < 18; k++)
             HIP_OK(hipMemcpyAsync(ip[k], iv[k].data(), n * sizeof(int), hipMemcpyHostToDevice, c.stream));
+        for (int k = 0; k < 20; k++)
             HIP_OK(hipMemcpyAsync(dp[k], dv[k].data(), n * sizeof(double), hipMemcpyHostToDevice, c.stream));
-        }
         // computed from the indices by the first step that reads them
         HIP_OK(hipMemsetAsync(A.info, 0, n * sizeof(int4), c.stream));
     }
@@ -1109,9 +1111,6 @@ void build(mel_ctx& c) {
         if (b.status == ST_INACTIVE) throw CapacityError{"inactive bodies are not supported"};
         if (b.borders.empty()) throw CapacityError{"every body needs at least one border"};
     }
-    for (auto& l : m.laws)
-        if (l.type == LAW_BONDED_MOHR_COULOMB || l.type == LAW_CZM_FATIGUE)
-            throw CapacityError{"BondedMohrCoulomb and CZMfatigue contact laws are not supported"};
     for (auto& b : m.bodies)
         if (m.flags[F_UPDATE_STIFFNESS] && !b.mat_dof.empty())
             throw CapacityError{"UPDATE_STIFFNESS_MATRIX is not supported"};
@@ -1868,10 +1867,10 @@ void copy_contacts(mel_ctx& c, const ContactArrays& dst, const ContactArrays& sr
         HIP_OK(hipMemcpyAsync(*di[k], *si[k], nc * sizeof(int), hipMemcpyDeviceToDevice, c.stream));
     double* const* dd[] = {&dst.N0, &dst.N1, &dst.N2, &dst.N3, &dst.gapn, &dst.vgapn, &dst.gapt, &dst.vgapt,
                            &dst.xsi, &dst.xn, &dst.yn, &dst.xt, &dst.yt, &dst.emass, &dst.len, &dst.fx, &dst.fy,
-                           &dst.int0, &dst.wacc};
+                           &dst.int0, &dst.wacc, &dst.int1, &dst.int2};
     double* const* sd[] = {&src.N0, &src.N1, &src.N2, &src.N3, &src.gapn, &src.vgapn, &src.gapt, &src.vgapt,
                            &src.xsi, &src.xn, &src.yn, &src.xt, &src.yt, &src.emass, &src.len, &src.fx, &src.fy,
-                           &src.int0, &src.wacc};
+                           &src.int0, &src.wacc, &src.int1, &src.int2};
     for (size_t k = 0; k < sizeof(dd) / sizeof(dd[0]); k++)
         HIP_OK(hipMemcpyAsync(*dd[k], *sd[k], nc * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     HIP_OK(hipMemcpyAsync(dst.info, src.info, nc * sizeof(int4), hipMemcpyDeviceToDevice, c.stream));
@@ -1963,7 +1962,7 @@ std::vector<StateArray> state_arrays(mel_ctx& c, int nc) {
                  A.nM0, A.nM1, A.nM2, A.nM3, A.sM0, A.sM1, A.sM2, A.sM3};
     for (int* p : ci) v.push_back({p, (size_t)nc * sizeof(int)});
     double* cd[] = {A.N0, A.N1, A.N2, A.N3, A.gapn, A.vgapn, A.gapt, A.vgapt, A.xsi, A.xn, A.yn, A.xt, A.yt,
-                    A.emass, A.len, A.fx, A.fy, A.int0, A.wacc};
+                    A.emass, A.len, A.fx, A.fy, A.int0, A.wacc, A.int1, A.int2};
     for (double* p : cd) v.push_back({p, (size_t)nc * sizeof(double)});
     v.push_back({A.info, (size_t)nc * sizeof(int4)});
     v.push_back({A.sinfo, (size_t)nc * sizeof(int4)});
@@ -2337,6 +2336,29 @@ mel_status mel_get_contacts(mel_ctx* c, int body, int32_t* ids, double* state, s
     });
 }
 
+mel_status mel_get_contact_internals(mel_ctx* c, int body, double* out, size_t cap, size_t* count) {
+    if (!c || !count || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        int n = contact_count(*c);
+        const ContactArrays& A = c->bufA;
+        std::vector<int> bs;
+        std::vector<double> v[3];
+        c->download(bs, A.bodyS, n);
+        c->download(v[0], A.int0, n);
+        c->download(v[1], A.int1, n);
+        c->download(v[2], A.int2, n);
+        size_t w = 0;
+        for (int e = 0; e < n; e++) {
+            if (bs[e] != body) continue;
+            if (out && w < cap)
+                for (int k = 0; k < 3; k++) out[3 * w + k] = v[k][e];
+            w++;
+        }
+        *count = w;
+        return w > cap && out ? MEL_E_ARG : MEL_OK;
+    });
+}
+
 mel_status mel_get_neighbours(mel_ctx* c, int body, int32_t* out, size_t cap, size_t* count) {
     if (!c || !count || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
     return guarded(c, [&]() {
@@ -2653,6 +2675,8 @@ mel_status mel_write_dynamic(mel_ctx* c, const char* path) {
             std::vector<int32_t> ids(16 * cnt);
             std::vector<double> st(18 * cnt);
             mel_get_contacts(c, b, ids.data(), st.data(), cnt, &cnt);
+            std::vector<double> iv(3 * cnt);
+            mel_get_contact_internals(c, b, iv.data(), cnt, &cnt);
             B.contacts.assign(cnt, ContactElement());
             for (size_t e = 0; e < cnt; e++) {
                 ContactElement& E = B.contacts[e];
@@ -2665,7 +2689,8 @@ mel_status mel_write_dynamic(mel_ctx* c, const char* path) {
                 E.gapn = s[4]; E.vgapn = s[5]; E.gapt = s[6]; E.vgapt = s[7]; E.xsi = s[8]; E.xnorm = s[9];
                 E.ynorm = s[10]; E.xtan = s[11]; E.ytan = s[12]; E.effective_mass = s[13]; E.length = s[14];
                 E.fx = s[15]; E.fy = s[16];
-                E.internal.assign(std::max(1, E.nb_internal), s[17]);
+                E.internal.assign(std::max(1, E.nb_internal), 0.);
+                for (int k = 0; k < std::min(3, (int)E.internal.size()); k++) E.internal[k] = iv[3 * e + k];
             }
         }
         std::string err;

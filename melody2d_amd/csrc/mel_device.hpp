@@ -34,7 +34,7 @@ struct ContactArrays {
     int *bodyS, *borderS, *bnodeS, *nodeS, *bodyM, *borderM, *shiftM, *segM, *nbint, *fresh;
     int *nM0, *nM1, *nM2, *nM3, *sM0, *sM1, *sM2, *sM3;
     double *N0, *N1, *N2, *N3, *gapn, *vgapn, *gapt, *vgapt, *xsi, *xn, *yn, *xt, *yt, *emass, *len, *fx, *fy,
-        *int0;
+        *int0, *int1, *int2;     // internal variables (damage; the three-variable laws' tractions)
     // half of the element's contact work accumulated since the last flush
     // into its bodies' work (one thread per element adds, so the sums do not
     // depend on scheduling)

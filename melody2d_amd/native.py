@@ -106,6 +106,8 @@ def library():
             "mel_get_body_sizes": ([vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
                                     P(ctypes.c_int)], ctypes.c_int),
             "mel_compute_nodal_stresses": ([vp], ctypes.c_int),
+            "mel_get_contact_internals": ([vp, ctypes.c_int, P(ctypes.c_double), ctypes.c_size_t, P(ctypes.c_size_t)],
+                                          ctypes.c_int),
             "mel_write_graphic": ([vp, ctypes.c_char_p, P(ctypes.c_int), ctypes.c_int], ctypes.c_int),
             "mel_get_nodal_stress": ([vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_double), ctypes.c_size_t],
                                      ctypes.c_int),
@@ -154,6 +156,7 @@ def exported_symbols():
     return ["mel_load", "mel_load_binary", "mel_load_ex", "mel_load_binary_ex", "mel_default_options",
             "mel_get_options", "mel_autotune", "mel_update_proximity", "mel_euler_steps", "mel_synchronize",
             "mel_get_node_field", "mel_set_node_field", "mel_get_body_sizes", "mel_compute_nodal_stresses", "mel_write_graphic",
+            "mel_get_contact_internals",
             "mel_get_nodal_stress", "mel_get_contacts", "mel_get_neighbours",
             "mel_get_body_state", "mel_get_body_energy", "mel_get_scalars", "mel_eval_forces",
             "mel_eval_forces_noalloc", "mel_get_state_size", "mel_save_state", "mel_load_state",
@@ -303,6 +306,15 @@ class Context:
                 self._check(self.lib.mel_get_nodal_stress(self._h, b, f, a[f].ctypes.data_as(
                     ctypes.POINTER(ctypes.c_double)), n))
             out.append(a)
+        return out
+
+    def contact_internals(self, body):
+        """(elements, 3) internal variables of body's contact elements (mel_get_contact_internals)."""
+        n = ctypes.c_size_t()
+        self._check(self.lib.mel_get_contact_internals(self._h, body, None, 0, ctypes.byref(n)))
+        out = np.zeros(3 * n.value)
+        self._check(self.lib.mel_get_contact_internals(self._h, body, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                        n.value, ctypes.byref(n)))
         return out
 
     def write_graphic(self, path, to_plot=None, number_print=0):

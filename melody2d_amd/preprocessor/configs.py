@@ -751,6 +751,15 @@ def variants():
     m = _set_law(tiny(h=0.45e-3, overlap=-1e-6), "CZMlinear", [1e13, 2e6, 5e-6, 1e5, 1.0, 2e-6])
     m.flags = m.flags + ["INITIALIZE_CZM"]
     v["tiny_czm"] = m
+    # the three-variable laws (damage and the last tractions per element)
+    m = _set_law(tiny(h=0.45e-3, overlap=-1e-6), "CZMfatigue",
+                 [1e13, 4e7, 5e-6, 1e5, 1.0, 2e6, 1e-8, 1e-8, 2e-6])
+    m.flags = m.flags + ["INITIALIZE_CZM"]
+    v["tiny_czmfatigue"] = m
+    m = _set_law(tiny(h=0.45e-3, overlap=-1e-6), "BondedMohrCoulomb",
+                 [1e13, 1e13, 1e12, 0.5, 4e6, 3e6, 0.3, 0.0, 0.0, 0.05, 2e-6])
+    m.flags = m.flags + ["INITIALIZE_CZM"]
+    v["tiny_bonded"] = m
     # linear elastic material, gravity, Neumann/Dirichlet on deformable
     # borders, Rayleigh damping entries, a free rigid body
     m = tiny(h=0.45e-3, with_platens=True)

@@ -695,6 +695,85 @@ void law_czm_linear(double kini, double Pnlim, double gapnlim, double Pnres, dou
     Pt = -kmax * gapt;
 }
 
+// Apply_CZM_fatigue, Code/Model.h:343-446 (Pn, Pt enter as the previous tractions)
+void law_czm_fatigue(double kini, double Pnlim, double gapnres, double Pnres, double NT, double Pnfat, double Raten,
+                     double Ratet, double gapn, double& gapt, double& D, double& Pn, double& Pt) {
+    if (D < 1.) {
+        double gnmax = Pnlim / kini * (1. - D) + gapnres * D;
+        double gtmax = gnmax * NT;
+        double kmax;
+        if (gapn < gnmax && std::abs(gapt) < gtmax) {
+            double Pnm = Pn, Ptm = Pt;
+            kmax = (Pnlim * (1. - D) + Pnres * D) / gnmax;
+            if (gapn < 0.) Pn = -kini * gapn;
+            else Pn = -kmax * gapn;
+            Pt = -kmax * gapt;
+            double Pnmax = Pnlim * (1. - D) + Pnres * D;
+            if (std::abs(Pn) > Pnfat && std::abs(Pn) > std::abs(Pnm)) {
+                double ratio = (std::abs(Pn) - Pnfat) / (Pnmax - Pnfat);
+                D = D + ratio * Raten * (std::abs(Pn) - std::abs(Pnm));
+            }
+            if (std::abs(Pt) > NT * Pnfat && std::abs(Pt) > std::abs(Ptm)) {
+                double ratio = (std::abs(Pt) - Pnfat * NT) / (NT * (Pnmax - Pnfat));
+                D = D + ratio * Ratet * (std::abs(Pt) - std::abs(Ptm));
+            }
+        } else {
+            bool grew = true;
+            if (gapn > gnmax && std::abs(gapt) < gtmax) { gnmax = gapn; gtmax = gnmax * NT; }
+            else if (gapn < gnmax && std::abs(gapt) > gtmax) { gtmax = std::abs(gapt); gnmax = gtmax / NT; }
+            else if (gapn > gnmax && std::abs(gapt) > gtmax) {
+                if (gapn / gnmax > std::abs(gapt) / gtmax) { gnmax = gapn; gtmax = gnmax * NT; }
+                else { gtmax = std::abs(gapt); gnmax = gtmax / NT; }
+            } else grew = false;
+            if (grew) {
+                D = (gnmax - Pnlim / kini) / (gapnres - Pnlim / kini);
+                kmax = (Pnlim * (1. - D) + Pnres * D) / gnmax;
+                if (gapn < 0.) Pn = -kini * gapn;
+                else Pn = -kmax * gapn;
+                Pt = -kmax * gapt;
+            }
+        }
+    }
+    if (D >= 1.) {
+        D = 1.;
+        Pn = -kini * gapn;
+        Pt = -kini * gapt;
+        if (Pn < -Pnres) { Pn = 0.; Pt = 0.; gapt = 0.; }
+        else if (std::abs(Pt) > Pnres * NT) {
+            Pt = gapt > 0. ? -Pnres * NT : Pnres * NT;
+            gapt = -Pt / kini;
+        }
+    }
+}
+
+// Apply_Bonded_Mohr_Coulomb, Code/Model.h:453-506
+void law_bonded_mc(double kn, double kt, double kbond, double fricbond, double cohbond, double tensbond,
+                   double fricfree, double cohfree, double tensfree, double damp, double mass, double length,
+                   double gapn, double vgapn, double& gapt, double vgapt, double& D, double& Pn, double& Pt) {
+    if (D >= 1.) {
+        Pn = -kn * gapn - damp * vgapn * 2. * std::sqrt(mass * kn / length);
+        if (Pn < -tensfree) { Pn = 0.; Pt = 0.; gapt = 0.; }
+        else Pt = -kt * gapt - damp * vgapt * 2. * std::sqrt(mass * kt / length);
+        if (std::abs(Pt) > cohfree + fricfree * Pn) {
+            Pt = gapt > 0. ? -cohfree - fricfree * Pn : cohfree + fricfree * Pn;
+            gapt = -Pt / kt;
+        }
+    } else {
+        if (gapn < 0.) Pn = -kn * gapn - damp * vgapn * 2. * std::sqrt(mass * kn / length);
+        else Pn = -kbond * gapn - damp * vgapn * 2. * std::sqrt(mass * kbond / length);
+        Pt = -kt * gapt - damp * vgapt * 2. * std::sqrt(mass * kt / length);
+        if (Pn < -tensbond) {
+            D = 1.;
+            if (Pn < -tensfree) { Pn = 0.; Pt = 0.; gapt = 0.; }
+        }
+        if (std::abs(Pt) > cohbond + fricbond * Pn) {
+            D = 1.;
+            Pt = gapt > 0. ? -cohfree - fricfree * Pn : cohfree + fricfree * Pn;
+            gapt = -Pt / kt;
+        }
+    }
+}
+
 void update_contact_forces(Sim& S, int bi) {
     Body& b = S.m.bodies[bi];
     State& s = S.s[bi];
@@ -727,6 +806,25 @@ void update_contact_forces(Sim& S, int bi) {
                 double D = c.internal[0];
                 law_czm_linear(p[0], p[1] * 0.5, p[2], p[3] * 0.5, p[4], gapn, gapt, D, Pn, Pt);
                 c.internal[0] = D;
+                break;
+            }
+            case LAW_BONDED_MOHR_COULOMB:
+            case LAW_CZM_FATIGUE: {
+                // Code/Body.h:1745-1810: damage and the previous tractions from
+                // the element (a fresh element starts from a broken bond, and its
+                // law then does not read the tractions)
+                double D = c.internal.empty() ? 0. : c.internal[0];
+                if (c.internal.size() >= 3) { Pn = c.internal[1]; Pt = c.internal[2]; }
+                if (c.nb_internal == 0) { D = 1.; c.nb_internal = 3; c.internal = {1., 0., 0.}; }
+                if (c.internal.size() < 3) c.internal.resize(3, 0.);
+                if (law.type == LAW_BONDED_MOHR_COULOMB)
+                    law_bonded_mc(p[0], p[1], p[2], p[3], p[4] * 0.5, p[5] * 0.5, p[6], p[7] * 0.5, p[8] * 0.5, p[9],
+                                  c.effective_mass, initial_length, gapn, c.vgapn, gapt, c.vgapt, D, Pn, Pt);
+                else
+                    law_czm_fatigue(p[0], p[1] * 0.5, p[2], p[3] * 0.5, p[4], p[5], p[6], p[7], gapn, gapt, D, Pn, Pt);
+                c.internal[0] = D;
+                c.internal[1] = Pn;
+                c.internal[2] = Pt;
                 break;
             }
             default: break;
@@ -1136,6 +1234,10 @@ void initialize_czm(Sim& S) {
             if (law && law->type == LAW_CZM_LINEAR) {
                 c.nb_internal = 1;
                 c.internal.assign(1, c.gapn < law->p[5] ? 0. : 1.);
+            } else if (law && (law->type == LAW_CZM_FATIGUE || law->type == LAW_BONDED_MOHR_COULOMB)) {
+                const double gapinit = law->p[law->type == LAW_CZM_FATIGUE ? 8 : 10];
+                c.nb_internal = 3;
+                c.internal = {c.gapn < gapinit ? 0. : 1., 0., 0.};
             }
         }
     }
@@ -1483,6 +1585,10 @@ void dump(Sim& S, const std::string& path) {
         }
         w.i32(p + "contacts_i", ci);
         w.f64(p + "contacts_d", cd);
+        std::vector<double> cint;
+        for (const ContactElement& c : s.contacts)
+            for (size_t k = 0; k < 3; k++) cint.push_back(k < c.internal.size() ? c.internal[k] : 0.);
+        w.f64(p + "contacts_int", cint);
         if (std::getenv("MEL_DUMP_STRESS")) w.f64(p + "stress", nodal_stresses(S, (int)bi));
         for (size_t k = 0; k < b.borders.size(); k++) {
             std::string q = p + "border" + std::to_string(k) + ".";
@@ -1582,8 +1688,6 @@ int oracle_run(const char* path, int save_index, int nsteps, const char* out_dir
     if (!ok) { g_err = err; return 1; }
     for (auto& b : S.m.bodies)
         if (b.status == ST_INACTIVE) { g_err = "inactive bodies are not supported"; return 1; }
-    for (auto& l : S.m.laws)
-        if (l.type == LAW_BONDED_MOHR_COULOMB || l.type == LAW_CZM_FATIGUE) { g_err = "contact law not supported"; return 1; }
     for (auto& b : S.m.bodies)
         if (S.m.flags[F_UPDATE_STIFFNESS] && !b.mat_dof.empty()) { g_err = "UPDATE_STIFFNESS_MATRIX is not supported"; return 1; }
     setup(S);

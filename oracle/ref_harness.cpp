@@ -169,6 +169,14 @@ static void dump_state(const string& path, vector<Body>& Bodies, double Time, in
         }
         w.i32(p + "contacts_i", ci);
         w.f64(p + "contacts_d", cd);
+        vector<double> cint;
+        for (int k = 0; k < B.nb_contact_elements; k++) {
+            Contact_element& C = B.contact_elements[k];
+            // a fresh element's internal vector holds one value; the
+            // three-variable laws make it three
+            for (int q = 0; q < 3; q++) cint.push_back(q < (int)C.internal.size() ? C.internal[q] : 0.0);
+        }
+        w.f64(p + "contacts_int", cint);
         if (getenv("MEL_DUMP_STRESS") && g_materials) {
             // Body::Compute_nodal_stresses (Code/Body.h:2310-2519), its 19
             // node fields one after another

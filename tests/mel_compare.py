@@ -52,6 +52,7 @@ def device_state(ctx, n_nodes_per_body, n_borders_per_body, border_sizes):
         ids, st = ctx.contacts(b)
         D[p + "contacts_i"] = ids
         D[p + "contacts_d"] = st
+        D[p + "contacts_int"] = ctx.contact_internals(b)
         D[p + "neighbours"] = ctx.neighbours(b)
         r, w = ctx.body_state(b)
         D[p + "rigid"] = r

@@ -41,7 +41,8 @@ def test_goldens_exercise_the_step():
     from conftest import extract_golden
     seen = {"contact": False, "neumann": False, "dirichlet": False, "damping": False, "czm": False,
             "rigid_rotation": False, "periodic_neighbour": False, "periodic_stencil": False,
-            "slave_border_1": False, "master_border_3": False, "boundary_work": False}
+            "slave_border_1": False, "master_border_3": False, "boundary_work": False,
+            "fatigue_damage": False, "bond_broken_later": False}
     for name in VARIANTS:
         with tempfile.TemporaryDirectory() as d:
             _, ref = extract_golden(name, d)
@@ -57,6 +58,12 @@ def test_goldens_exercise_the_step():
                     cd = D[p + "contacts_d"].reshape(-1, 18)
                     ci = D[p + "contacts_i"].reshape(-1, 16)
                     seen["czm"] |= bool(np.any((cd[:, 17] > 0) & (cd[:, 17] < 1)))
+                    if p + "contacts_int" in D:
+                        ci3 = D[p + "contacts_int"].reshape(-1, 3)
+                        # damage between 0 and 1 with a last traction kept (CZMfatigue)
+                        seen["fatigue_damage"] |= bool(np.any((ci3[:, 0] > 0) & (ci3[:, 0] < 1) & (ci3[:, 1] != 0)))
+                        # an intact bond (damage 0, tractions kept) after step 10 (BondedMohrCoulomb)
+                        seen["bond_broken_later"] |= bool(s > 10 and np.any((ci3[:, 0] == 0) & (ci3[:, 1] != 0)))
                     seen["rigid_rotation"] |= bool(D[p + "rigid"][8] != 0)
                     # periodic cells: neighbours one period away, and
                     # contact stencils that cross the master's periodic seam
