@@ -622,6 +622,7 @@ __global__ void __launch_bounds__(ROW_CHUNK) k_prox_count(DevModel m, int n_coun
         return;
     }
     __shared__ int s_w[ROW_CHUNK / 64];
+    __shared__ int s_hbase;
     const int n_chunks = m.chunk_off[m.n_bodies];
     for (int c = blockIdx.x; c < n_chunks; c += n_count) {
         const int b = m.chunk_body[c];
@@ -634,8 +635,15 @@ __global__ void __launch_bounds__(ROW_CHUNK) k_prox_count(DevModel m, int n_coun
         int tot;
         const int ex = wg_excl_scan(cnt, s_w, tot);
         if (threadIdx.x == 0) m.chunk_cnt[c] = tot;
+        // the workgroup's rows with candidates take consecutive hit records:
+        // one atomic per chunk (the records' order is free, k_prox_write
+        // places each row's elements by its chunk and offset)
+        int nhit;
+        const int hex = wg_excl_scan(cnt > 0 ? 1 : 0, s_w, nhit);
+        if (threadIdx.x == 0 && nhit > 0) s_hbase = atomicAdd(m.hit_n, nhit);
+        __syncthreads();
         if (cnt > 0) {
-            const int h = atomicAdd(m.hit_n, 1);
+            const int h = s_hbase + hex;
             if (h < m.c_cap) {
                 m.hit_rec[h] = make_int4(c, ex, first, local);
                 m.hit_cnt[h] = cnt;
@@ -643,6 +651,7 @@ __global__ void __launch_bounds__(ROW_CHUNK) k_prox_count(DevModel m, int n_coun
                 atomicOr(&m.status_flags[0], 8);
             }
         }
+        __syncthreads();       // s_hbase is read before the next chunk's atomic replaces it
     }
 }
 
