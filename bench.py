@@ -229,15 +229,18 @@ def granular_leg(args, native, tmpdir):
     counts = [b.n_nodes for b in model.bodies]
     rigid = [b.rigid for b in model.bodies]
     done = args.warmup + 2 * args.steps + (min(args.steps, 50) if args.breakdown else 0)
-    if args.error_step > done:
-        ctx64.euler_steps(args.error_step - done)
-    state_info = ctx64.get_info()
-    t0 = time.time()
-    errs = fp32_errors(native, ctx64, ctx32, counts, rigid, args.error_steps_after)
-    errs["state_step"] = max(args.error_step, done)
-    errs["state_contacts"] = state_info.n_contacts
-    log("[granular] fp32-mixed errors vs fp64 from the step-%d state (%d contacts): %s (%.1fs)" % (
-        errs["state_step"], state_info.n_contacts, json.dumps(errs), time.time() - t0))
+    if args.no_fp32_errors:
+        errs = {"skipped": "--no-fp32-errors", "meets_force_bar": None, "rel_internal_force_err": None}
+    else:
+        if args.error_step > done:
+            ctx64.euler_steps(args.error_step - done)
+        state_info = ctx64.get_info()
+        t0 = time.time()
+        errs = fp32_errors(native, ctx64, ctx32, counts, rigid, args.error_steps_after)
+        errs["state_step"] = max(args.error_step, done)
+        errs["state_contacts"] = state_info.n_contacts
+        log("[granular] fp32-mixed errors vs fp64 from the step-%d state (%d contacts): %s (%.1fs)" % (
+            errs["state_step"], state_info.n_contacts, json.dumps(errs), time.time() - t0))
     ctx64.close()
     ctx32.close()
     cpu = None
@@ -323,12 +326,15 @@ def plate_leg(args, native, tmpdir):
     a.steps, a.warmup = args.plate_steps, 5
     ctx64, info, value, sec, roof, bd = timed_run(native, path, native.FP64, a, tag, uniaxial)
     ctx32, _, value32, sec32, roof32, bd32 = timed_run(native, path, native.FP32_MIXED, a, tag, uniaxial)
-    uniaxial(ctx64)
-    errs = fp32_errors(native, ctx64, ctx32, [rec["nodes"]], [False], 200)
-    errs.pop("rel_contact_force_err")
-    errs["meets_force_bar"] = bool(errs["rel_internal_force_err"] <= FORCE_BAR)
-    errs["state"] = "uniaxial 10/s field after 1000 steps' strain"
-    log("[plate] fp32-mixed errors vs fp64: %s" % json.dumps(errs))
+    if args.no_fp32_errors:
+        errs = {"skipped": "--no-fp32-errors", "meets_force_bar": None}
+    else:
+        uniaxial(ctx64)
+        errs = fp32_errors(native, ctx64, ctx32, [rec["nodes"]], [False], 200)
+        errs.pop("rel_contact_force_err")
+        errs["meets_force_bar"] = bool(errs["rel_internal_force_err"] <= FORCE_BAR)
+        errs["state"] = "uniaxial 10/s field after 1000 steps' strain"
+        log("[plate] fp32-mixed errors vs fp64: %s" % json.dumps(errs))
     ctx64.close()
     ctx32.close()
     cpu = None
@@ -483,12 +489,16 @@ def main():
     ap.add_argument("--plate-steps", type=int, default=50)
     ap.add_argument("--plate-cpu-steps", type=int, default=5)
     ap.add_argument("--no-plate", action="store_true", help="skip the plate leg")
+    ap.add_argument("--plate-only", action="store_true",
+                    help="run only the plate leg and print its JSON object (for profiling its kernels)")
     ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="also time every kernel in a separate window")
     ap.add_argument("--error-step", type=int, default=3000,
                     help="FP64 step whose state the FP32-mixed errors are evaluated from")
+    ap.add_argument("--no-fp32-errors", action="store_true",
+                    help="skip the FP32-mixed error evaluation (profiling passes time only the kernels)")
     ap.add_argument("--error-steps-after", type=int, default=500,
                     help="steps both forms run from that state for the displacement and energy errors")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -539,6 +549,10 @@ def main():
 
     from melody2d_amd import native
     tmpdir = tempfile.mkdtemp(prefix="mel_bench_")
+    if args.plate_only:
+        print(json.dumps(plate_leg(args, native, tmpdir)), flush=True)
+        shutil.rmtree(tmpdir, ignore_errors=True)
+        return
     g = granular_leg(args, native, tmpdir)
     plate = None
     if not args.no_plate:
