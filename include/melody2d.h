@@ -114,6 +114,44 @@ mel_status mel_euler_steps(mel_ctx* ctx, int nsteps);
 /* wait for queued device work; reports deferred errors */
 mel_status mel_synchronize(mel_ctx* ctx);
 
+/* The model's time-integration scheme (SOLVER section, Code/IO.h:821-890;
+ * the DYNAMIC file names the one that runs, Code/IO.h:1936-1940). */
+typedef enum {
+    MEL_SCHEME_EULER = 0, MEL_SCHEME_ADAPTIVE_EULER = 1, MEL_SCHEME_MASS_SCALING = 2,
+    MEL_SCHEME_ADAPTIVE_MASS_SCALING = 3
+} mel_scheme;
+
+/* nsteps iterations of the stepping loop of Code/main.cpp:141-238 with the
+ * model's scheme: Euler_step (as mel_euler_steps), or Solver_step
+ * (Code/Solver.h:98-486: two half steps per attempt, the error estimate,
+ * rejected attempts restored and retried with a smaller time step,
+ * mass-scaling factors), each followed by the contact-update cadence of
+ * Code/main.cpp:191-196 with the updated time step.  Every attempt of an
+ * adaptive scheme waits for its error scalars (a few doubles) on the host.
+ * MEL_E_NAN when a body's total error is NaN (Code/main.cpp:227-237), after
+ * that step.  The "Euler_2" scheme is refused at load (MEL_E_PARSE): the
+ * reference reads no target error for it, yet divides by one. */
+mel_status mel_solver_steps(mel_ctx* ctx, int nsteps);
+
+typedef struct {
+    int scheme;             /* mel_scheme */
+    double dt;              /* current time step (Deltat) */
+    int neval;              /* attempts of the last accepted step */
+    long long attempts;     /* attempts since load */
+    double max_error, mean_error;   /* Solver_step's max_error, mean_error of the last attempt */
+    double total_mass, max_mass;    /* mass-scaling schemes: total scaled mass, largest scaled/initial ratio */
+} mel_solver_info_t;
+mel_status mel_get_solver_info(mel_ctx* ctx, mel_solver_info_t* info);
+
+/* Body::total_error, max_error, node_for_max_error, mass_mass_scaling,
+ * factor_mass_scaling (Code/Body.h:2524-2585) after the last attempt;
+ * MEL_E_ARG for an Euler model. */
+mel_status mel_get_body_error(mel_ctx* ctx, int body, double* out5);
+
+/* Node::x_factor_mass_scaling, y_factor_mass_scaling of a body's n nodes
+ * (Code/Node.h:274-320; all 1 unless a mass-scaling scheme changed them). */
+mel_status mel_get_mass_scaling(mel_ctx* ctx, int body, double* fx, double* fy, size_t n);
+
 /* node field of one body (Node members, Code/Node.h:10-99) */
 mel_status mel_get_node_field(mel_ctx* ctx, int body, int field, double* out, size_t n);
 mel_status mel_set_node_field(mel_ctx* ctx, int body, int field, const double* in, size_t n);

@@ -138,6 +138,16 @@ public:
     // one Euler_step (Code/Solver.h:10-88), then Update_proximity when the
     // contact-update period has elapsed (Code/main.cpp:191-196)
     void Euler_step(Model& m) { check(mel_euler_steps(m.handle(), 1), m.handle()); }
+    // one iteration of the stepping loop with the model's scheme: Euler_step,
+    // or Solver_step (Code/Solver.h:98-486) for Adaptive_Euler, Mass_Scaling
+    // and Adaptive_Mass_Scaling (Code/main.cpp:148-166), then the cadence
+    void Solver_step(Model& m) { check(mel_solver_steps(m.handle(), 1), m.handle()); }
+    void Solver_steps(Model& m, int n) { check(mel_solver_steps(m.handle(), n), m.handle()); }
+    mel_solver_info_t info(Model& m) {
+        mel_solver_info_t i;
+        check(mel_get_solver_info(m.handle(), &i), m.handle());
+        return i;
+    }
     // n of them, queued on the device at once (a run replays captured graphs)
     void Euler_steps(Model& m, int n) { check(mel_euler_steps(m.handle(), n), m.handle()); }
     // Update_proximity (Code/Proximity.h:48-473) now

@@ -20,7 +20,8 @@ LIB = os.path.join(HERE, "libmelody2d_hip.so")
 ORACLE = os.path.join(ROOT, "oracle")
 REFERENCE = "/root/reference"
 
-HIP_SOURCES = ["mel_context.hip", "k_step.hip", "k_block.hip", "k_fused.hip", "k_proximity.hip", "k_rigid_lists.hip"]
+HIP_SOURCES = ["mel_context.hip", "k_step.hip", "k_block.hip", "k_fused.hip", "k_proximity.hip", "k_rigid_lists.hip",
+               "k_adaptive.hip"]
 CXX_SOURCES = ["model_io.cpp", "mel_serve.cpp"]
 
 HIPCC_FLAGS = [

@@ -18,6 +18,7 @@
 #include "../../include/melody2d.h"
 #include "k_block.hpp"
 #include "k_fused.hpp"
+#include "k_adaptive.hpp"
 #include "k_rigid_lists.hpp"
 #include "k_step.hpp"
 #include "mel_device.hpp"
@@ -245,6 +246,19 @@ struct mel_ctx {
     double *st_dphix = nullptr, *st_dphiy = nullptr, *st_bpar = nullptr, *st_out = nullptr;
     int eval_save_cap = 0;
     std::vector<int> h_bn_list;   // border nodes finished by each block (BlockArrays::bn_list)
+    // adaptive schemes (Solver_step): per-node and per-body arrays, the
+    // Store snapshot and its copy tables (one per contact buffer that can be
+    // current, stores then restores), the attempt's scalars in pinned memory
+    AdArrays ad{};
+    CopyDesc* d_copy[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    const void* copy_key[2] = {nullptr, nullptr};
+    int n_copy = 0;
+    double* snap_rigid = nullptr;
+    double* h_ad = nullptr;
+    hipEvent_t ev_ad = nullptr;
+    int neval = 0;                // attempts of the last accepted step
+    long long attempts = 0;       // attempts since load
+    bool nan_stop = false;        // some body's total error was NaN
 
     template <typename T>
     T* alloc(size_t n) {
@@ -277,6 +291,8 @@ struct mel_ctx {
         if (ev_async) (void)hipEventDestroy(ev_async);
         for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
         if (h_async) (void)hipHostFree(h_async);
+        if (h_ad) (void)hipHostFree(h_ad);
+        if (ev_ad) (void)hipEventDestroy(ev_ad);
         if (ev_join) (void)hipEventDestroy(ev_join);
         for (void* p : allocs) (void)hipFree(p);
         if (aux) (void)hipStreamDestroy(aux);
@@ -840,7 +856,8 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     // MELODY2D_FUSED_STEP=0 asks for the launch sequence
     const char* fe = getenv("MELODY2D_FUSED_STEP");
     const bool fused_off = c.opts.fused_step >= 0 ? c.opts.fused_step == 0 : (fe && fe[0] == '0');
-    c.two_launch = B.lds_finish_ok && !fused_off;
+    // (the adaptive schemes run their half steps as the launch sequence)
+    c.two_launch = B.lds_finish_ok && !fused_off && c.model.scheme == SCH_EULER;
     // the launch sequence takes the wide-window kernel when Gauss points
     // with more support entries than the default window are common (RPIM
     // supports of 30 nodes; a few long MLS supports read their tail from
@@ -1045,6 +1062,116 @@ void euler_step(mel_ctx& c) {
         k_kinematics_list<<<blocks(B.n_split_kin), BLOCK, 0, s>>>(d, B.split_kin, B.n_split_kin);
     c.time = t_next;
     c.outputs_stale = true;
+}
+
+// ---------------------------------------------------------------------------
+// Solver_step (Code/Solver.h:98-486): set-up of the adaptive schemes
+// ---------------------------------------------------------------------------
+// Per-node inverse distance estimators (Load_static, Code/IO.h:1620, 1819),
+// unit mass-scaling factors (Code/Node.h:195-199), the deformable bodies'
+// driven directions, and the tables of the Store / Restore copies: what
+// Body::Store keeps (Code/Body.h:2593-2617) and the step changes - node
+// kinematics, parameters and mass-scaling factors, border lengths, BC
+// values and BC pressures, contact elements, works - into a snapshot of the
+// same layout.
+void setup_adaptive(mel_ctx& c) {
+    const Model& m = c.model;
+    DevModel& d = c.d;
+    AdArrays& A = c.ad;
+    const size_t N = d.n_nodes, NB = d.n_bodies;
+    A.uptx = c.alloc<double>(N); A.upty = c.alloc<double>(N);
+    A.xerr = c.alloc<double>(N); A.yerr = c.alloc<double>(N);
+    A.enorm = c.alloc<double>(N); A.msm = c.alloc<double>(N);
+    std::vector<double> ide(N, 0.), one(N, 1.), nd(NB, 0.), body(NB * AD_BODY_FIELDS, 0.);
+    std::vector<int> drv(NB, 0);
+    for (size_t b = 0; b < NB; b++) {
+        const Body& B = m.bodies[b];
+        nd[b] = B.nodal_distance;
+        body[b * AD_BODY_FIELDS + AB_FACTOR] = 1.;
+        for (const Border& br : B.borders) {
+            if (br.xbc.type == BC_DRIVEN) drv[b] |= 1;
+            if (br.ybc.type == BC_DRIVEN) drv[b] |= 2;
+        }
+        for (int i = 0; i < B.n_nodes; i++) {
+            const size_t g = c.h_node_off[b] + i;
+            if (B.rigid) ide[g] = 1. / B.nodal_distance;
+            else ide[g] = 1. / (B.domain[i] * std::sqrt(3.1415927 / 4. / (B.dptr[i + 1] - B.dptr[i])));
+        }
+    }
+    A.ide = c.upload(ide);
+    A.fsx = c.upload(one); A.fsy = c.upload(one); A.fsn = c.upload(one);
+    A.body = c.upload(body);
+    A.body_nd = c.upload(nd);
+    A.body_drv = c.upload(drv);
+    A.out = c.alloc<double>(AD_OUT_FIELDS);
+    A.kin_nodes = c.d_kin_all;
+    A.n_kin = c.n_kin_all;
+    A.mass_scaling = m.scheme == SCH_MASS_SCALING || m.scheme == SCH_ADAPTIVE_MASS_SCALING;
+    A.target_error = m.target_error;
+    A.max_ms = m.max_mass_scaling;
+    A.ctrl_ms = m.control_parameter_mass_scaling;
+    A.err_factor_ms = m.error_factor_mass_scaling;
+    A.decrease_ms = m.decrease_factor_mass_scaling;
+    // the rigid bodies' driven DOFs of the initialisation block's
+    // Update_bc(Time) (Code/main.cpp:109-110): the first half of the first
+    // attempt moves driven bodies with them (the Euler step forms them
+    // afresh at t + dt every step and never reads these)
+    k_ad_bc<<<blocks(d.n_borders + d.n_rigid), BLOCK, 0, c.stream>>>(d, c.time);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipHostMalloc((void**)&c.h_ad, AD_OUT_FIELDS * sizeof(double), hipHostMallocDefault));
+    HIP_OK(hipEventCreateWithFlags(&c.ev_ad, hipEventDisableTiming));
+    // the snapshot: fixed arrays, then the contact elements at capacity
+    struct Arr { void* p; unsigned elem; size_t n; bool per_contact; };
+    std::vector<Arr> fixed;
+    for (double* p : {d.xc, d.yc, d.ux, d.uy, d.vx, d.vy, d.ax, d.ay, d.upx, d.upy, d.vpx, d.vpy, d.apx, d.apy, A.fsx,
+                      A.fsy})
+        fixed.push_back({p, 8, N, false});
+    for (double* p : {d.sl_len, d.sl_xbp, d.sl_ybp}) fixed.push_back({p, 8, (size_t)d.n_slots, false});
+    for (double* p : {d.br_xval, d.br_yval, d.br_xvel, d.br_yvel}) fixed.push_back({p, 8, (size_t)d.n_borders, false});
+    if (c.blocks_dev.n_blocks > 0) fixed.push_back({c.blocks_dev.blk_work, 8, 14 * (size_t)c.blocks_dev.n_blocks, false});
+    auto contact_arrays = [&](const ContactArrays& C) {
+        std::vector<Arr> v;
+        for (int* p : {C.bodyS, C.borderS, C.bnodeS, C.nodeS, C.bodyM, C.borderM, C.shiftM, C.segM, C.nbint, C.fresh,
+                       C.nM0, C.nM1, C.nM2, C.nM3, C.sM0, C.sM1, C.sM2, C.sM3})
+            v.push_back({p, 4, 0, true});
+        for (double* p : {C.N0, C.N1, C.N2, C.N3, C.gapn, C.vgapn, C.gapt, C.vgapt, C.xsi, C.xn, C.yn, C.xt, C.yt,
+                          C.emass, C.len, C.fx, C.fy, C.int0, C.wacc, C.int1, C.int2})
+            v.push_back({p, 8, 0, true});
+        v.push_back({C.info, 16, 0, true});
+        v.push_back({C.sinfo, 16, 0, true});
+        return v;
+    };
+    size_t snap_bytes = 0;
+    std::vector<size_t> off;
+    auto layout = [&](const std::vector<Arr>& v) {
+        for (const Arr& a : v) {
+            off.push_back(snap_bytes);
+            const size_t n = a.per_contact ? (size_t)d.c_cap : a.n;
+            snap_bytes += (n * a.elem + 255) / 256 * 256;
+        }
+    };
+    const std::vector<Arr> ca = contact_arrays(c.bufA), cb = contact_arrays(c.bufB);
+    layout(fixed);
+    layout(ca);
+    unsigned char* snap = c.alloc<unsigned char>(snap_bytes);
+    c.snap_rigid = c.alloc<double>(NB * RIGID_FIELDS);
+    const ContactArrays* bufs[2] = {&c.bufA, &c.bufB};
+    for (int k = 0; k < 2; k++) {
+        std::vector<Arr> all = fixed;
+        const std::vector<Arr>& cc = k == 0 ? ca : cb;
+        all.insert(all.end(), cc.begin(), cc.end());
+        std::vector<CopyDesc> st(all.size()), rs(all.size());
+        for (size_t q = 0; q < all.size(); q++) {
+            const unsigned long long live = (unsigned long long)all[q].p, copy = (unsigned long long)(snap + off[q]);
+            st[q] = {live, copy, all[q].n, all[q].elem, all[q].per_contact ? 1u : 0u};
+            rs[q] = {copy, live, all[q].n, all[q].elem, all[q].per_contact ? 1u : 0u};
+        }
+        c.d_copy[k][0] = c.upload(st);
+        c.d_copy[k][1] = c.upload(rs);
+        c.copy_key[k] = bufs[k]->bodyS;
+        c.n_copy = (int)all.size();
+    }
+    HIP_OK(hipStreamSynchronize(c.stream));
 }
 
 // The contact elements read from the DYNAMIC file (Load_dynamic,
@@ -1737,6 +1864,7 @@ void build(mel_ctx& c) {
     HIP_OK(hipStreamSynchronize(c.stream));
     poll_async(c, true);
     contact_count(c);
+    if (m.scheme != SCH_EULER) setup_adaptive(c);
 }
 
 thread_local std::string g_load_err;
@@ -1923,9 +2051,109 @@ void eval_forces(mel_ctx& c, unsigned what, unsigned char* save_buf = nullptr) {
 }
 
 // ---------------------------------------------------------------------------
+// Solver_step (Code/Solver.h:98-486)
+// ---------------------------------------------------------------------------
+// Body::Store (dir 0) or Body::Restore (dir 1) of every body: the copy table
+// of the current contact buffer, then the rigid records' restored fields
+void store_restore(mel_ctx& c, int dir) {
+    const int k = c.d.C.bodyS == c.copy_key[0] ? 0 : 1;
+    hipStream_t s = c.stream;
+    k_ad_copy<<<dim3(64, c.n_copy), BLOCK, 0, s>>>(c.d_copy[k][dir], c.n_copy, c.d.c_count);
+    const int nr = 16 * c.d.n_bodies;
+    if (dir == 0) k_ad_rigid_copy<<<blocks(nr), BLOCK, 0, s>>>(c.d.body_rigid, c.snap_rigid, c.d.n_bodies);
+    else k_ad_rigid_copy<<<blocks(nr), BLOCK, 0, s>>>(c.snap_rigid, c.d.body_rigid, c.d.n_bodies);
+    HIP_OK(hipGetLastError());
+}
+
+// one half of an attempt of step dt from time t (Code/Solver.h:155-214,
+// 218-279): Update_bc(t + dt/2) first in the second half; contacts, BC
+// forces and damping with the laws' time step dt/2 and the rigid bodies'
+// half on the second stream beside the internal forces; then the half's end
+void solver_half(mel_ctx& c, int half, double dt, double t) {
+    DevModel& d = c.d;
+    const BlockArrays& B = c.blocks_dev;
+    hipStream_t s = c.stream, a = c.aux;
+    const int n_cwg = contact_workgroups(c);
+    DevModel dh = d;
+    dh.dt = dt * 0.5;
+    if (half == 2) k_ad_bc<<<blocks(d.n_borders + d.n_rigid), BLOCK, 0, s>>>(d, t + dt * 0.5);
+    HIP_OK(hipEventRecord(c.ev_fork, s));
+    HIP_OK(hipStreamWaitEvent(a, c.ev_fork, 0));
+    TIMED(c, KT_CONTACTS, a, (k_step_contacts<<<n_cwg, BLOCK_THREADS, 0, a>>>(dh, B.bc_bn, B.n_bc_bn)));
+    if (d.n_matrix > 0) TIMED(c, KT_DAMPING, a, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, a>>>(d)));
+    if (d.n_rigid + B.n_split_bn > 0)
+        TIMED(c, KT_GATHER, a, (k_ad_gather<<<d.n_rigid + blocks(B.n_split_bn, BLOCK_THREADS), BLOCK_THREADS, 0, a>>>(
+                                   d, c.ad, B.split_bn, B.n_split_bn, half, dt, t + dt)));
+    HIP_OK(hipEventRecord(c.ev_join, a));
+    if (B.n_blocks > 0) TIMED(c, KT_BLOCK_UPDATE, s, launch_internal_forces(c, s));
+    HIP_OK(hipStreamWaitEvent(s, c.ev_join, 0));
+    const int extra = std::max(1, (d.n_rigid_nodes + d.n_borders + BLOCK_THREADS - 1) / BLOCK_THREADS);
+    TIMED(c, KT_FINISH, s, (k_ad_finish<<<B.n_blocks + extra, BLOCK_THREADS, 0, s>>>(d, B, c.ad, half, dt, t + dt)));
+    if (B.n_split_kin > 0)
+        k_kinematics_list<<<blocks(B.n_split_kin), BLOCK, 0, s>>>(d, B.split_kin, B.n_split_kin);
+    HIP_OK(hipGetLastError());
+}
+
+// One Solver_step: Store, then attempts of the current time step until one
+// is accepted; after every attempt the device reduces the error estimate
+// (and the scaled masses) and the host decides and updates the time step
+// as Code/Solver.h:328-484 does.
+void solver_step(mel_ctx& c) {
+    const Model& m = c.model;
+    DevModel& d = c.d;
+    poll_async(c, false);
+    c.prox_after_step = false;
+    store_restore(c, 0);
+    c.neval = 0;
+    for (;;) {
+        c.neval++;
+        c.attempts++;
+        const double dt = d.dt;
+        solver_half(c, 1, dt, c.time);
+        solver_half(c, 2, dt, c.time);
+        if (c.ad.n_kin > 0) k_ad_error<<<blocks(c.ad.n_kin), BLOCK, 0, c.stream>>>(d, c.ad);
+        k_ad_body_reduce<<<d.n_bodies, BLOCK, 0, c.stream>>>(d, c.ad);
+        k_ad_global<<<1, 64, 0, c.stream>>>(d, c.ad);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(c.h_ad, c.ad.out, AD_OUT_FIELDS * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        HIP_OK(hipEventRecord(c.ev_ad, c.stream));
+        HIP_OK(hipEventSynchronize(c.ev_ad));
+        const double max_error = c.h_ad[AO_MAX_ERROR], max_mass = c.h_ad[AO_MAX_MASS];
+        bool fail;
+        double correction1;
+        if (m.scheme == SCH_ADAPTIVE_EULER) {
+            fail = max_error > m.target_error * m.accepted_ratio;
+            correction1 = max_error / m.target_error;
+        } else if (m.scheme == SCH_MASS_SCALING) {
+            fail = max_error > m.target_error * m.accepted_ratio;
+            correction1 = max_error * m.inv_target_error;
+        } else {
+            fail = max_mass > m.accepted_ratio + 0.05;
+            correction1 = max_mass;
+        }
+        const double correction2 = 1. / m.accepted_ratio;
+        if (fail) store_restore(c, 1);
+        else c.time += dt;
+        double ndt;
+        if (m.scheme == SCH_ADAPTIVE_MASS_SCALING && !fail) ndt = dt / std::pow(correction2, m.control_parameter);
+        else if (correction1 > correction2) ndt = dt / std::pow(correction1, m.control_parameter);
+        else ndt = dt / std::pow(correction2, m.control_parameter);
+        if (ndt > m.contact_update_period) ndt = m.contact_update_period;
+        d.dt = ndt;
+        if (!fail) {
+            c.nan_stop = c.h_ad[AO_NAN] != 0.;
+            break;
+        }
+    }
+    k_set_time<<<1, 64, 0, c.stream>>>(d.time, c.time);
+    HIP_OK(hipGetLastError());
+    c.outputs_stale = true;
+}
+
+// ---------------------------------------------------------------------------
 // In-memory checkpoint of the dynamic state (mel_save_state / mel_load_state)
 // ---------------------------------------------------------------------------
-constexpr uint64_t STATE_MAGIC = 0x3145544154534c4dull;   // "MLSTATE1"
+constexpr uint64_t STATE_MAGIC = 0x3245544154534c4dull;   // "MLSTATE2"
 
 struct StateArray {
     void* p;
@@ -1957,6 +2185,12 @@ std::vector<StateArray> state_arrays(mel_ctx& c, int nc) {
     if (d.n_rigid > 0) v.push_back({d.rigid_dv, (size_t)d.n_rigid * 8 * sizeof(double)});
     if (c.blocks_dev.n_blocks > 0) v.push_back({c.blocks_dev.blk_work, 14 * (size_t)c.blocks_dev.n_blocks * sizeof(double)});
     v.push_back({d.time, sizeof(double)});
+    if (c.model.scheme != SCH_EULER) {
+        // mass-scaling factors and the bodies' adaptive scalars
+        v.push_back({c.ad.fsx, N * sizeof(double)});
+        v.push_back({c.ad.fsy, N * sizeof(double)});
+        v.push_back({c.ad.body, NB * AD_BODY_FIELDS * sizeof(double)});
+    }
     const ContactArrays& A = d.C;
     int* ci[] = {A.bodyS, A.borderS, A.bnodeS, A.nodeS, A.bodyM, A.borderM, A.shiftM, A.segM, A.nbint, A.fresh,
                  A.nM0, A.nM1, A.nM2, A.nM3, A.sM0, A.sM1, A.sM2, A.sM3};
@@ -1973,7 +2207,7 @@ struct StateHeader {
     uint64_t magic;
     uint64_t n_arrays;
     int64_t n_nodes, n_bodies, n_slots, n_contacts, iteration, reserved;
-    double time, next_contact_update;
+    double time, next_contact_update, dt;
 };
 
 size_t state_size(mel_ctx& c, int nc) {
@@ -2162,6 +2396,79 @@ mel_status mel_euler_steps(mel_ctx* c, int nsteps) {
             }
         }
         HIP_OK(hipGetLastError());
+        return MEL_OK;
+    });
+}
+
+mel_status mel_solver_steps(mel_ctx* c, int nsteps) {
+    if (nsteps < 0) return MEL_E_ARG;
+    if (c && c->model.scheme == SCH_EULER) return mel_euler_steps(c, nsteps);
+    return guarded(c, [&]() {
+        for (int k = 0; k < nsteps; k++) {
+            c->iteration++;
+            solver_step(*c);
+            // contact-update cadence with the new time step, Code/main.cpp:191-196
+            if (c->time > c->next_contact_update - 1.e-6 * c->d.dt) {
+                c->next_contact_update = c->next_contact_update + c->model.contact_update_period;
+                update_proximity(*c);
+            }
+            if (c->nan_stop) {
+                c->err = "a body's total error is NaN (the reference stops the run, Code/main.cpp:227-237)";
+                return MEL_E_NAN;
+            }
+        }
+        HIP_OK(hipGetLastError());
+        return MEL_OK;
+    });
+}
+
+mel_status mel_get_solver_info(mel_ctx* c, mel_solver_info_t* info) {
+    if (!c || !info) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        const Model& m = c->model;
+        info->scheme = m.scheme;
+        info->dt = c->d.dt;
+        info->neval = c->neval;
+        info->attempts = c->attempts;
+        info->max_error = info->mean_error = info->total_mass = info->max_mass = 0.;
+        if (m.scheme != SCH_EULER && c->h_ad) {
+            HIP_OK(hipStreamSynchronize(c->stream));
+            info->max_error = c->h_ad[AO_MAX_ERROR];
+            info->mean_error = c->h_ad[AO_MEAN_ERROR];
+            info->total_mass = c->h_ad[AO_TOTAL_MASS];
+            info->max_mass = c->h_ad[AO_MAX_MASS];
+        }
+        return MEL_OK;
+    });
+}
+
+mel_status mel_get_body_error(mel_ctx* c, int body, double* out5) {
+    if (!c || !out5 || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
+    if (c->model.scheme == SCH_EULER) {
+        c->err = "mel_get_body_error: the model's scheme is Euler (no error estimate)";
+        return MEL_E_ARG;
+    }
+    return guarded(c, [&]() {
+        std::vector<double> v;
+        c->download(v, c->ad.body + (size_t)body * AD_BODY_FIELDS, AD_BODY_FIELDS);
+        out5[0] = v[AB_TOTAL]; out5[1] = v[AB_MAX]; out5[2] = v[AB_NODE];
+        out5[3] = v[AB_MASS_MS]; out5[4] = v[AB_FACTOR];
+        return MEL_OK;
+    });
+}
+
+mel_status mel_get_mass_scaling(mel_ctx* c, int body, double* fx, double* fy, size_t n) {
+    if (!c || !fx || !fy || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
+    if ((size_t)c->model.bodies[body].n_nodes != n) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        const size_t o = c->h_node_off[body];
+        if (c->model.scheme == SCH_EULER) {
+            for (size_t i = 0; i < n; i++) fx[i] = fy[i] = 1.;
+            return MEL_OK;
+        }
+        HIP_OK(hipMemcpyAsync(fx, c->ad.fsx + o, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(fy, c->ad.fsy + o, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
         return MEL_OK;
     });
 }
@@ -2465,7 +2772,7 @@ mel_status mel_save_state(mel_ctx* c, void* buf, size_t cap, size_t* written) {
         }
         const std::vector<StateArray> arrs = state_arrays(*c, nc);
         StateHeader h{STATE_MAGIC, arrs.size(), c->d.n_nodes, c->d.n_bodies, c->d.n_slots, nc, c->iteration, 0,
-                      c->time, c->next_contact_update};
+                      c->time, c->next_contact_update, c->d.dt};
         unsigned char* o = static_cast<unsigned char*>(buf);
         std::memcpy(o, &h, sizeof h);
         size_t pos = sizeof h;
@@ -2530,6 +2837,7 @@ mel_status mel_load_state(mel_ctx* c, const void* buf, size_t size) {
         c->time = h.time;
         c->next_contact_update = h.next_contact_update;
         c->iteration = (int)h.iteration;
+        c->d.dt = h.dt;
         c->outputs_stale = false;
         return MEL_OK;
     });
@@ -2624,6 +2932,7 @@ mel_status mel_write_dynamic(mel_ctx* c, const char* path) {
         m.time = c->time;
         m.number_iteration = c->iteration;
         m.next_contact_update = c->next_contact_update;
+        m.dt = c->d.dt;     // the adaptive schemes' current time step (Write_dynamic, Code/IO.h:2396)
         for (int b = 0; b < c->d.n_bodies; b++) {
             Body& B = m.bodies[b];
             size_t n = B.n_nodes;

@@ -350,14 +350,28 @@ bool load_model_files_impl(const std::string& dir, int save_index, Model& m, std
                 }
             }
             if (starts(line, "SOLVER")) {
-                double target, ctrl, ratio;
+                // Code/IO.h:821-890: every scheme reads its times and error
+                // control; the mass-scaling schemes one more line
                 in >> m.solver;
                 std::getline(in, token);
-                if (m.solver != "Euler") { err = "only the Euler solver is supported, found " + m.solver; return false; }
+                m.scheme = scheme_of(m.solver);
+                if (m.scheme < 0) {
+                    err = "unsupported solver " + m.solver +
+                          (m.solver == "Euler_2" ? " (the reference reads no target error for it, Code/IO.h:839-847, "
+                                                   "yet its time-step update divides by one)"
+                                                 : "");
+                    return false;
+                }
                 in >> m.tini >> m.dt >> m.tend;
                 std::getline(in, token);
-                in >> target >> ctrl >> ratio;
+                in >> m.target_error >> m.control_parameter >> m.accepted_ratio;
                 std::getline(in, token);
+                m.inv_target_error = 1 / m.target_error;
+                if (m.scheme == SCH_MASS_SCALING || m.scheme == SCH_ADAPTIVE_MASS_SCALING) {
+                    in >> m.max_mass_scaling >> m.control_parameter_mass_scaling >> m.error_factor_mass_scaling >>
+                        m.decrease_factor_mass_scaling;
+                    std::getline(in, token);
+                }
                 in >> m.save_period >> m.print_period >> m.contact_update_period;
                 std::getline(in, token);
             }
@@ -609,9 +623,21 @@ bool load_model_files_impl(const std::string& dir, int save_index, Model& m, std
             for (int f = 0; f < F_COUNT; f++)
                 if (starts(line, flag_names[f])) m.flags[f] = 1;
             if (starts(line, "SOLVER")) {
+                // the DYNAMIC file names the scheme that runs (Load_dynamic,
+                // Code/IO.h:1936-1940); its parameters come from the static file
                 std::string solver;
                 in >> solver;
                 std::getline(in, token);
+                const int sch = scheme_of(solver);
+                if (sch < 0) { err = "unsupported solver " + solver + " in " + name; return false; }
+                if (sch != m.scheme && (sch == SCH_MASS_SCALING || sch == SCH_ADAPTIVE_MASS_SCALING) &&
+                    m.scheme != SCH_MASS_SCALING && m.scheme != SCH_ADAPTIVE_MASS_SCALING) {
+                    err = "the DYNAMIC file asks for " + solver + ", whose mass-scaling parameters the static file "
+                          "(solver " + m.solver + ") does not give";
+                    return false;
+                }
+                m.solver = solver;
+                m.scheme = sch;
                 in >> m.number_save >> m.number_iteration >> m.time >> m.number_print;
                 std::getline(in, token);
                 in >> m.dt;
@@ -793,6 +819,20 @@ bool load_model_binary(const std::string& path, Model& m, std::string& err) {
         auto& fl = I("model.flags");
         need(fl.size() >= (size_t)F_COUNT, "model.flags");
         for (int f = 0; f < F_COUNT; f++) m.flags[f] = fl[f];
+        // optional: the scheme and its error control (absent: Euler)
+        if (recs.count("model.solver")) {
+            auto& sv = D("model.solver");
+            need(sv.size() == 8, "model.solver size");
+            const int sch = (int)sv[0];
+            need(sch >= SCH_EULER && sch <= SCH_ADAPTIVE_MASS_SCALING && sv[0] == (double)sch, "model.solver scheme");
+            static const char* names[] = {"Euler", "Adaptive_Euler", "Mass_Scaling", "Adaptive_Mass_Scaling"};
+            m.scheme = sch;
+            m.solver = names[sch];
+            m.target_error = sv[1]; m.control_parameter = sv[2]; m.accepted_ratio = sv[3];
+            m.inv_target_error = 1 / m.target_error;
+            m.max_mass_scaling = sv[4]; m.control_parameter_mass_scaling = sv[5];
+            m.error_factor_mass_scaling = sv[6]; m.decrease_factor_mass_scaling = sv[7];
+        }
         auto& mt = I("model.material_type");
         auto& mr = D("model.material_raw");
         need(mr.size() == 6 * mt.size(), "model.material_raw size");
@@ -1131,6 +1171,14 @@ bool write_graphic_text(const std::string& path, const Model& m, const std::vect
     }
     std::fclose(f);
     return true;
+}
+
+int scheme_of(const std::string& name) {
+    if (name == "Euler") return SCH_EULER;
+    if (name == "Adaptive_Euler") return SCH_ADAPTIVE_EULER;
+    if (name == "Mass_Scaling") return SCH_MASS_SCALING;
+    if (name == "Adaptive_Mass_Scaling") return SCH_ADAPTIVE_MASS_SCALING;
+    return -1;
 }
 
 }  // namespace mel

@@ -22,6 +22,9 @@ enum LawType {
     LAW_TWO_SLOPES = 4, LAW_BONDED_MOHR_COULOMB = 5, LAW_CZM_LINEAR = 6, LAW_CZM_FATIGUE = 7
 };
 enum Status { ST_ACTIVE = 0, ST_INACTIVE = 1, ST_NOGRAVITY = 2 };
+// time-integration schemes of the SOLVER section (Code/IO.h:821-890)
+enum Scheme { SCH_EULER = 0, SCH_ADAPTIVE_EULER = 1, SCH_MASS_SCALING = 2, SCH_ADAPTIVE_MASS_SCALING = 3 };
+int scheme_of(const std::string& name);     // -1: not a scheme this library runs
 
 // flag slots of the DYNAMIC file, Code/IO.h:1912-1933
 enum Flag {
@@ -119,7 +122,14 @@ struct Model {
     std::vector<ContactLaw> laws;
     std::vector<int> contacts_table;          // n_materials^2, -1 = no law
     std::string solver = "Euler";
+    int scheme = SCH_EULER;
     double tini = 0, dt = 0, tend = 0;
+    // error control of the adaptive schemes (Code/Solver.h:98-486): target
+    // error, control exponent, accepted ratio; mass scaling: maximum factor,
+    // control exponent, error factor, decrease factor
+    double target_error = 0, inv_target_error = 0, control_parameter = 0, accepted_ratio = 0;
+    double max_mass_scaling = 0, control_parameter_mass_scaling = 0, error_factor_mass_scaling = 0,
+           decrease_factor_mass_scaling = 0;
     double save_period = 0, print_period = 0, contact_update_period = 0;
     double xmin_period = 0, xmax_period = 0, gravity_x = 0, gravity_y = 0;
     int flags[F_COUNT] = {0};

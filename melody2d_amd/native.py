@@ -58,6 +58,13 @@ STRESS_FIELDS = ["sigma_I", "sigma_II", "sigma_III", "jacobian", "Sxx", "Syy", "
                  "sigma_xy", "sigma_zz", "sigma_tresca", "sigma_vm", "sigma_sph", "Exx", "Eyy", "Exy", "norm_E"]
 
 
+class SolverInfo(ctypes.Structure):
+    """mel_solver_info_t"""
+    _fields_ = [("scheme", ctypes.c_int), ("dt", ctypes.c_double), ("neval", ctypes.c_int),
+                ("attempts", ctypes.c_longlong), ("max_error", ctypes.c_double), ("mean_error", ctypes.c_double),
+                ("total_mass", ctypes.c_double), ("max_mass", ctypes.c_double)]
+
+
 class Options(ctypes.Structure):
     """mel_options: load-time launch configuration (-1 = library default)."""
     _fields_ = [("block_nodes", ctypes.c_int), ("fused_step", ctypes.c_int),
@@ -96,6 +103,11 @@ def library():
             "mel_update_proximity": ([vp], ctypes.c_int),
             "mel_euler_steps": ([vp, ctypes.c_int], ctypes.c_int),
             "mel_synchronize": ([vp], ctypes.c_int),
+            "mel_solver_steps": ([vp, ctypes.c_int], ctypes.c_int),
+            "mel_get_solver_info": ([vp, P(SolverInfo)], ctypes.c_int),
+            "mel_get_body_error": ([vp, ctypes.c_int, P(ctypes.c_double)], ctypes.c_int),
+            "mel_get_mass_scaling": ([vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double), ctypes.c_size_t],
+                                     ctypes.c_int),
             "mel_get_node_field": ([vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_double), ctypes.c_size_t], ctypes.c_int),
             "mel_set_node_field": ([vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_double), ctypes.c_size_t], ctypes.c_int),
             "mel_get_contacts": ([vp, ctypes.c_int, P(ctypes.c_int32), P(ctypes.c_double), ctypes.c_size_t,
@@ -155,6 +167,7 @@ def library():
 def exported_symbols():
     return ["mel_load", "mel_load_binary", "mel_load_ex", "mel_load_binary_ex", "mel_default_options",
             "mel_get_options", "mel_autotune", "mel_update_proximity", "mel_euler_steps", "mel_synchronize",
+            "mel_solver_steps", "mel_get_solver_info", "mel_get_body_error", "mel_get_mass_scaling",
             "mel_get_node_field", "mel_set_node_field", "mel_get_body_sizes", "mel_compute_nodal_stresses", "mel_write_graphic",
             "mel_get_contact_internals",
             "mel_get_nodal_stress", "mel_get_contacts", "mel_get_neighbours",
@@ -242,6 +255,29 @@ class Context:
 
     def euler_steps(self, n):
         self._check(self.lib.mel_euler_steps(self._h, n))
+
+    def solver_steps(self, n):
+        """n iterations with the model's scheme (mel_solver_steps)."""
+        self._check(self.lib.mel_solver_steps(self._h, n))
+
+    def solver_info(self):
+        inf = SolverInfo()
+        self._check(self.lib.mel_get_solver_info(self._h, ctypes.byref(inf)))
+        return inf
+
+    def body_error(self, body):
+        """(total_error, max_error, node_for_max_error, mass_mass_scaling,
+        factor_mass_scaling) of the last attempt (mel_get_body_error)."""
+        out = np.zeros(5)
+        self._check(self.lib.mel_get_body_error(self._h, body, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def mass_scaling(self, body, n):
+        """Node mass-scaling factors (x, y) of a body (mel_get_mass_scaling)."""
+        fx, fy = np.zeros(n), np.zeros(n)
+        P = ctypes.POINTER(ctypes.c_double)
+        self._check(self.lib.mel_get_mass_scaling(self._h, body, fx.ctypes.data_as(P), fy.ctypes.data_as(P), n))
+        return fx, fy
 
     def update_proximity(self):
         self._check(self.lib.mel_update_proximity(self._h))

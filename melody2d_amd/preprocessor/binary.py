@@ -32,6 +32,9 @@ def _rec(f, name, arr):
     f.write(a.tobytes())
 
 
+_SCHEME = {"Euler": 0, "Adaptive_Euler": 1, "Mass_Scaling": 2, "Adaptive_Mass_Scaling": 3}
+
+
 def write_binary(model: Model, path: str):
     mats = [m[0] for m in model.materials]
     with open(path, "wb") as f:
@@ -41,6 +44,9 @@ def write_binary(model: Model, path: str):
                                           m.gravity[1], m.tini, 0, 0, 0, m.tini + m.save_period,
                                           m.tini + m.print_period, m.tini + m.contact_update_period], dtype=float))
         _rec(f, "model.flags", np.array([1 if fl in m.flags else 0 for fl in _FLAGS], dtype=np.int32))
+        if m.scheme != "Euler":
+            _rec(f, "model.solver", np.array([_SCHEME[m.scheme], m.target_error, m.control_parameter,
+                                              m.accepted_ratio] + list(m.mass_scaling), dtype=float))
         _rec(f, "model.material_type", np.array([_MAT[t] for _, t, _ in m.materials], dtype=np.int32))
         _rec(f, "model.material_raw", np.array([p for _, _, p in m.materials], dtype=float).reshape(-1))
         _rec(f, "model.law_materials", np.array([[mats.index(a), mats.index(b)] for a, b, *_ in m.contact_laws],

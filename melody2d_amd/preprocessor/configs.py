@@ -789,6 +789,16 @@ def variants():
     v["tiny_periodic"] = tiny_periodic()
     v["tiny_simple"] = tiny_simple()
     v["tiny_holed"] = tiny_holed()
+    # the adaptive schemes (Code/Solver.h:98-486) on the two discs between
+    # platens: error-controlled time steps with rejected attempts, and the
+    # two mass-scaling forms (the last one rejects on the mass ratio)
+    for name, scheme, ratio in (("tiny_adaptive", "Adaptive_Euler", 2.0), ("tiny_mass_scaling", "Mass_Scaling", 2.0),
+                                ("tiny_adaptive_ms", "Adaptive_Mass_Scaling", 1.1)):
+        m = tiny(h=0.45e-3, with_platens=True)
+        m.scheme = scheme
+        m.target_error, m.control_parameter, m.accepted_ratio = 1e-4, 0.2, ratio
+        m.mass_scaling = (4.0, 0.5, 0.5, 0.99)
+        v[name] = m
     return v
 
 

@@ -35,6 +35,9 @@ class Model:
     target_error: float = 1e-4
     control_parameter: float = 0.2
     accepted_ratio: float = 2.0
+    # Mass_Scaling / Adaptive_Mass_Scaling: maximum factor, control
+    # exponent, error factor, decrease factor (Code/IO.h:866)
+    mass_scaling: tuple = (10.0, 0.5, 0.5, 0.99)
     save_period: float = 1e9
     print_period: float = 1e9
     contact_update_period: float = 1e-6
@@ -152,8 +155,10 @@ def write_static(model: Model, d: str):
         out += [m1, m2, typ, lev, " ".join(g(v) for v in p) + "  ", " "]
     out.append(" ")
     out += ["SOLVER", model.scheme, "%s %s %s" % (g(model.tini), g(model.dt), g(model.tend)),
-            "%s %s %s" % (g(model.target_error), g(model.control_parameter), g(model.accepted_ratio)),
-            "%s %s %s" % (g(model.save_period), g(model.print_period), g(model.contact_update_period)), " ", " "]
+            "%s %s %s" % (g(model.target_error), g(model.control_parameter), g(model.accepted_ratio))]
+    if model.scheme in ("Mass_Scaling", "Adaptive_Mass_Scaling"):
+        out.append(" ".join(g(v) for v in model.mass_scaling))
+    out += ["%s %s %s" % (g(model.save_period), g(model.print_period), g(model.contact_update_period)), " ", " "]
     out += ["PERIODIC_BOUNDARIES", "%s %s" % (g(model.periodic[0]), g(model.periodic[1])), " ", " "]
     out += ["GRAVITY", "%s %s" % (g(model.gravity[0]), g(model.gravity[1])), " ", " "]
     out += ["NUMBER_BODIES", "%d" % len(model.bodies), " ", " "]

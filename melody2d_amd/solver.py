@@ -109,6 +109,12 @@ class Solver:
         (Code/main.cpp:191-196)."""
         model.context.euler_steps(n)
 
+    def solver_step(self, model, n=1):
+        """n iterations with the model's scheme: Euler_step, or Solver_step
+        (Code/Solver.h:98-486) for the adaptive schemes, each followed by the
+        contact-update cadence (Code/main.cpp:148-196)."""
+        model.context.solver_steps(n)
+
     def update_proximity(self, model):
         """Update_proximity (Code/Proximity.h:48-473) now."""
         model.context.update_proximity()

@@ -6,6 +6,12 @@
 //
 // It restates, over flat per-body arrays, the reference functions:
 //   Euler_step                 Code/Solver.h:10-88
+//   Solver_step (Adaptive_Euler, Mass_Scaling, Adaptive_Mass_Scaling)
+//                              Code/Solver.h:98-486, with Body::Store /
+//                              Restore, Apply_Euler_temporary,
+//                              Update_kinematics_temporary, Compute_error,
+//                              Compute_mass_scaling (Code/Body.h:2126-2264,
+//                              2524-2659) and Code/Node.h:247-320
 //   init sequence              Code/main.cpp:103-126, cadence :191-196
 //   Body::Update_bc            Code/Body.h:335-395, Border::Update_bc Code/Border.h:120-321
 //   Body::Update_borders       Code/Body.h:614-661
@@ -75,6 +81,15 @@ struct State {
     std::vector<ContactElement> contacts;
     std::vector<std::array<double, 3>> send_f;
     std::vector<std::array<int, 2>> send_to;
+    // adaptive schemes: per node the temporary displacement parameters and
+    // displacements, errors, inverse distance estimator, mass-scaling
+    // factors and their increments, scaled masses (Code/Node.h:72-99);
+    // per body the rigid temporaries, error and mass-scaling scalars
+    std::vector<double> uptx, upty, utx, uty, xerr, yerr, enorm, ide, fsx, fsy, dfsx, dfsy, fsn, msm;
+    double disp_t[3] = {0, 0, 0}, vel_t[3] = {0, 0, 0};
+    double total_error = 0, max_error = 0;
+    int node_for_max = 0;
+    double mass_ms = 0, factor_ms = 1, delta_factor_ms = 0, max_factor_ms = 0;
 };
 
 struct Sim {
@@ -774,10 +789,9 @@ void law_bonded_mc(double kn, double kt, double kbond, double fricbond, double c
     }
 }
 
-void update_contact_forces(Sim& S, int bi) {
+void update_contact_forces(Sim& S, int bi, double dt) {
     Body& b = S.m.bodies[bi];
     State& s = S.s[bi];
-    double dt = S.m.dt;
     double period = S.m.xmax_period - S.m.xmin_period;
     double Pn = 0., Pt = 0.;   // carried between elements as in Code/Body.h:1647
     for (ContactElement& c : s.contacts) {
@@ -879,7 +893,10 @@ void send_contact_forces(Sim& S, int bi) {
 // --------------------------------------------------------------------------
 // integration
 // --------------------------------------------------------------------------
-void sum_newton_euler_kinematics(Sim& S, int bi, double t, double dt) {
+// Sum_up_forces + Apply_Newton (Code/Node.h:212-231, Code/Body.h:1975-2043):
+// accelerations divided by the mass-scaling factors (1 unless a
+// mass-scaling scheme changed them)
+void sum_newton(Sim& S, int bi) {
     Body& b = S.m.bodies[bi];
     State& s = S.s[bi];
     int n = b.n_nodes;
@@ -897,14 +914,14 @@ void sum_newton_euler_kinematics(Sim& S, int bi, double t, double dt) {
             s.fx[i] = ix + s.fcx[i] + 0. + s.fbx[i] + s.fdx[i] + s.fnx[i] + s.fkx[i] + 0.;
             s.fy[i] = iy + s.fcy[i] + 0. + s.fby[i] + s.fdy[i] + s.fny[i] + s.fky[i] + 0.;
         }
-        // Apply_Newton (Code/Node.h:227-231); mass-scaling factors are 1
-        for (int i = 0; i < n; i++) { s.apx[i] = s.fx[i] * s.imx[i] / 1.; s.apy[i] = s.fy[i] * s.imy[i] / 1.; }
+        // Apply_Newton (Code/Node.h:227-231)
+        for (int i = 0; i < n; i++) { s.apx[i] = s.fx[i] * s.imx[i] / s.fsx[i]; s.apy[i] = s.fy[i] * s.imy[i] / s.fsy[i]; }
         for (int i = 0; i < n; i++) { s.ax[i] = 0.; s.ay[i] = 0.; }
     } else {
         for (int q = 0; q < 3; q++) s.rf[q] = s.rfc[q] + s.rfb[q] + s.rfd[q] + s.rfn[q] + s.rfk[q];
-        s.acc[0] = s.rf[0] * s.inverse_mass / 1.;
-        s.acc[1] = s.rf[1] * s.inverse_mass / 1.;
-        s.acc[2] = s.rf[2] * s.inverse_inertia / 1.;
+        s.acc[0] = s.rf[0] * s.inverse_mass / s.factor_ms;
+        s.acc[1] = s.rf[1] * s.inverse_mass / s.factor_ms;
+        s.acc[2] = s.rf[2] * s.inverse_inertia / s.factor_ms;
         for (int i = 0; i < n; i++) {
             s.fcx[i] = s.rfc[0]; s.fcy[i] = s.rfc[1];
             s.fbx[i] = s.rfb[0]; s.fby[i] = s.rfb[1];
@@ -915,8 +932,14 @@ void sum_newton_euler_kinematics(Sim& S, int bi, double t, double dt) {
             s.ax[i] = s.acc[0]; s.ay[i] = s.acc[1];
         }
     }
-    update_bc(S, bi, t + dt);
-    // Apply_Euler (Code/Body.h:2051-2118)
+}
+
+// Apply_Euler (Code/Body.h:2051-2118) + Update_kinematics +
+// Update_current_positions (Code/Body.h:2176-2302)
+void euler_kinematics(Sim& S, int bi, double dt) {
+    Body& b = S.m.bodies[bi];
+    State& s = S.s[bi];
+    int n = b.n_nodes;
     if (!b.rigid) {
         for (int i = 0; i < n; i++) {
             double vx = s.vpx[i] + 0.5 * dt * s.apx[i];
@@ -997,6 +1020,12 @@ void sum_newton_euler_kinematics(Sim& S, int bi, double t, double dt) {
             s.vy[i] = ex * s.vel[2] + s.vel[1];
         }
     }
+}
+
+void sum_newton_euler_kinematics(Sim& S, int bi, double t, double dt) {
+    sum_newton(S, bi);
+    update_bc(S, bi, t + dt);
+    euler_kinematics(S, bi, dt);
 }
 
 void rigid_positions(Sim& S, int bi) {
@@ -1343,6 +1372,17 @@ void setup(Sim& S) {
         s.xbv.assign(nbr, 0.); s.ybv.assign(nbr, 0.); s.xbvel.assign(nbr, 0.); s.ybvel.assign(nbr, 0.);
         s.pxmin.assign(nbr, 0.); s.pxmax.assign(nbr, 0.); s.pymin.assign(nbr, 0.); s.pymax.assign(nbr, 0.);
         s.shmin.assign(nbr, 0); s.shmax.assign(nbr, 0);
+        for (auto* v : {&s.uptx, &s.upty, &s.utx, &s.uty, &s.xerr, &s.yerr, &s.enorm, &s.msm}) z(*v);
+        // Node constructor values (Code/Node.h:186-200) and the inverse
+        // distance estimator of Load_static (Code/IO.h:1620, 1819)
+        s.fsx.assign(n, 1.); s.fsy.assign(n, 1.); s.fsn.assign(n, 1.);
+        s.dfsx.assign(n, 1.); s.dfsy.assign(n, 1.);
+        s.ide.assign(n, 0.);
+        for (int i = 0; i < n; i++) {
+            if (b.rigid) s.ide[i] = 1. / b.nodal_distance;
+            else if (!b.dptr.empty())
+                s.ide[i] = 1. / (b.domain[i] * std::sqrt(3.1415927 / 4. / (b.dptr[i + 1] - b.dptr[i])));
+        }
         s.contacts = b.contacts;
         s.neighbours = b.neighbours;
         s.nb_neighbours = (int)b.neighbours.size() / 3;
@@ -1395,7 +1435,7 @@ void euler_step(Sim& S) {
     // damped-law work update of another body is a data race there, so the
     // restatement runs this loop in body order.
     for (int i = 0; i < NB; i++)
-        if (m.bodies[i].status != ST_INACTIVE) update_contact_forces(S, i);
+        if (m.bodies[i].status != ST_INACTIVE) update_contact_forces(S, i, dt);
     for (int i = 0; i < NB; i++) send_contact_forces(S, i);
     std::vector<std::pair<int, int>> regions;
     for (int i = 0; i < NB; i++)
@@ -1416,6 +1456,271 @@ void euler_step(Sim& S) {
     }
     S.time += dt;
     update_contact_pressures(S);
+}
+
+// --------------------------------------------------------------------------
+// adaptive schemes (Code/Solver.h:98-486)
+// --------------------------------------------------------------------------
+
+// Body::Apply_Euler_temporary + Update_kinematics_temporary
+// (Code/Body.h:2126-2264, Code/Node.h:247-251)
+void euler_temporary(Sim& S, int bi, double dt) {
+    Body& b = S.m.bodies[bi];
+    State& s = S.s[bi];
+    const int n = b.n_nodes;
+    if (!b.rigid) {
+        for (int i = 0; i < n; i++) {
+            s.uptx[i] = s.upx[i] + dt * (s.vpx[i] + dt * s.apx[i]);
+            s.upty[i] = s.upy[i] + dt * (s.vpy[i] + dt * s.apy[i]);
+        }
+        for (int i = 0; i < n; i++) {
+            double dx = 0., dy = 0.;
+            for (int k = b.dptr[i]; k < b.dptr[i + 1]; k++) {
+                dx += b.dphi[k] * s.uptx[b.didx[k]];
+                dy += b.dphi[k] * s.upty[b.didx[k]];
+            }
+            s.utx[i] = dx; s.uty[i] = dy;
+        }
+        return;
+    }
+    if (s.driven[0] == 0.) {
+        s.vel_t[0] = s.vel[0] + dt * s.acc[0];
+        s.disp_t[0] = s.disp[0] + dt * s.vel_t[0];
+    } else if (s.driven[0] == 1.) {
+        s.vel_t[0] = s.driven[2];
+        s.disp_t[0] = s.driven[1];
+    }
+    if (s.driven[4] == 0.) {
+        s.vel_t[1] = s.vel[1] + dt * s.acc[1];
+        s.disp_t[1] = s.disp[1] + dt * s.vel_t[1];
+    } else if (s.driven[4] == 1.) {
+        s.vel_t[1] = s.driven[6];
+        s.disp_t[1] = s.driven[5];
+    }
+    if (s.driven[0] == 0. && s.driven[4] == 0. && b.periodicity != PER_PERIODIC) {
+        s.vel_t[2] = s.vel[2] + dt * s.acc[2];
+        s.disp_t[2] = s.disp[2] + dt * s.vel_t[2];
+    } else {
+        s.vel_t[2] = 0.;
+        s.disp_t[2] = 0.;
+    }
+    double dr = s.disp_t[2] - 0.;
+    for (int i = 0; i < n; i++) {
+        double dx = b.X[i] - b.xc, dy = b.Y[i] - b.yc;
+        s.utx[i] = s.disp_t[0] + dx * std::cos(dr) - dy * std::sin(dr) - dx;
+        s.uty[i] = s.disp_t[1] + dx * std::sin(dr) + dy * std::cos(dr) - dy;
+    }
+}
+
+// Body::Compute_error (Code/Body.h:2524-2551, Code/Node.h:253-272)
+void compute_error(Sim& S, int bi) {
+    Body& b = S.m.bodies[bi];
+    State& s = S.s[bi];
+    s.total_error = 0.;
+    if (!b.rigid) {
+        s.max_error = 0.;
+        s.node_for_max = 0;
+        for (int i = 0; i < b.n_nodes; i++) {
+            s.xerr[i] = s.driven[0] == 1. ? 0. : s.ux[i] - s.utx[i];
+            s.yerr[i] = s.driven[4] == 1. ? 0. : s.uy[i] - s.uty[i];
+            s.enorm[i] = std::sqrt(s.xerr[i] * s.xerr[i] + s.yerr[i] * s.yerr[i]) * s.ide[i];
+            s.total_error += s.enorm[i];
+            if (s.enorm[i] == 0.) s.enorm[i] = 1.e-16;
+            if (s.enorm[i] > s.max_error) { s.max_error = s.enorm[i]; s.node_for_max = i; }
+        }
+        return;
+    }
+    if (s.driven[0] != 1.) s.total_error += (s.disp[0] - s.disp_t[0]) * (s.disp[0] - s.disp_t[0]);
+    if (s.driven[4] != 1.) s.total_error += (s.disp[1] - s.disp_t[1]) * (s.disp[1] - s.disp_t[1]);
+    s.total_error = 0.1 * std::sqrt(s.total_error) / b.nodal_distance;
+    if (s.total_error == 0.) s.total_error = 1.e-16;
+    s.max_error = s.total_error;
+    s.node_for_max = -1;
+    for (int i = 0; i < b.n_nodes; i++) s.enorm[i] = s.total_error;
+}
+
+// Body::Compute_mass_scaling (Code/Body.h:2557-2585, Code/Node.h:274-320)
+void compute_mass_scaling(Sim& S, int bi) {
+    const Model& m = S.m;
+    Body& b = S.m.bodies[bi];
+    State& s = S.s[bi];
+    const double thr = m.target_error * m.error_factor_mass_scaling;
+    s.mass_ms = 0.;
+    if (!b.rigid) {
+        s.max_factor_ms = 0.;
+        auto one = [&](double err, double ide, double& f, double& df) {
+            if (std::fabs(err * ide) >= thr) {
+                df = std::pow((std::fabs(err * ide) - thr), m.control_parameter_mass_scaling);
+                f += df;
+                if (f > m.max_mass_scaling) f = m.max_mass_scaling;
+            } else {
+                if (f != 1) f *= m.decrease_factor_mass_scaling;
+                if (f < 1) f = 1;
+            }
+        };
+        for (int i = 0; i < b.n_nodes; i++) {
+            one(s.xerr[i], s.ide[i], s.fsx[i], s.dfsx[i]);
+            one(s.yerr[i], s.ide[i], s.fsy[i], s.dfsy[i]);
+            s.fsn[i] = (s.fsx[i] + s.fsy[i]) * 0.5;
+            if (s.fsn[i] > s.max_factor_ms) s.max_factor_ms = s.fsn[i];
+            s.msm[i] = std::sqrt(s.mx[i] * s.mx[i] * s.fsx[i] * s.fsx[i] + s.my[i] * s.my[i] * s.fsy[i] * s.fsy[i]);
+            s.mass_ms += s.msm[i];
+        }
+        return;
+    }
+    if (s.total_error >= thr) {
+        s.delta_factor_ms = std::pow((s.total_error - thr), m.control_parameter_mass_scaling);
+        s.factor_ms += s.delta_factor_ms;
+        if (s.factor_ms >= m.max_mass_scaling) s.factor_ms = m.max_mass_scaling;
+    } else {
+        if (s.factor_ms != 1) s.factor_ms *= m.decrease_factor_mass_scaling;
+        if (s.factor_ms < 1) s.factor_ms = 1;
+    }
+    s.mass_ms = s.mass * s.factor_ms;
+}
+
+// Body::Restore (Code/Body.h:2625-2652): the stored nodes, borders, contact
+// elements, works, mass-scaling increment and rigid current / displacement /
+// velocity come back; the rest of the body (driven DOFs, accelerations,
+// resultants, its mass-scaling factor, error scalars) stays as it is
+void restore(State& live, const State& st) {
+    State keep = live;
+    live = st;
+    for (int q = 0; q < 3; q++) {
+        live.acc[q] = keep.acc[q]; live.rfc[q] = keep.rfc[q]; live.rfb[q] = keep.rfb[q];
+        live.rfd[q] = keep.rfd[q]; live.rfn[q] = keep.rfn[q]; live.rfk[q] = keep.rfk[q]; live.rf[q] = keep.rf[q];
+        live.disp_t[q] = keep.disp_t[q]; live.vel_t[q] = keep.vel_t[q];
+    }
+    for (int q = 0; q < 8; q++) live.driven[q] = keep.driven[q];
+    live.total_error = keep.total_error; live.max_error = keep.max_error; live.node_for_max = keep.node_for_max;
+    live.mass_ms = keep.mass_ms; live.factor_ms = keep.factor_ms; live.max_factor_ms = keep.max_factor_ms;
+    live.neighbours = keep.neighbours; live.nb_neighbours = keep.nb_neighbours;
+}
+
+struct SolverScalars {
+    bool stepped = false;
+    int neval = 0;
+    double max_error = 0, mean_error = 0, total_mass = 0, max_mass = 0;
+};
+SolverScalars g_solver;
+
+// one half of an attempt: contacts and forces with the half step's law
+// time step, then the half's end (Code/Solver.h:155-214 and 218-279)
+void solver_half(Sim& S, int half, double dt) {
+    Model& m = S.m;
+    const int NB = (int)m.bodies.size();
+    const double t = S.time;
+#pragma omp parallel
+    {
+#pragma omp for schedule(dynamic)
+        for (int i = 0; i < NB; i++) {
+            if (m.bodies[i].status == ST_INACTIVE) continue;
+            initialize_contact_forces(S, i);
+            if (half == 2) update_bc(S, i, t + dt * 0.5);
+            update_borders(S, i);
+            update_contacts(S, i);
+        }
+#pragma omp for schedule(dynamic)
+        for (int i = 0; i < NB; i++) {
+            if (m.bodies[i].status == ST_INACTIVE) continue;
+            update_bc_forces(S, i);
+            update_body_forces(S, i);
+            update_damping_forces(S, i);
+        }
+    }
+    for (int i = 0; i < NB; i++)
+        if (m.bodies[i].status != ST_INACTIVE) update_contact_forces(S, i, dt * 0.5);
+    for (int i = 0; i < NB; i++) send_contact_forces(S, i);
+    std::vector<std::pair<int, int>> regions;
+    for (int i = 0; i < NB; i++)
+        if (!m.bodies[i].rigid)
+            for (int r = 0; r < m.bodies[i].n_regions; r++) regions.push_back({i, r});
+    const bool ms = m.scheme == SCH_MASS_SCALING || m.scheme == SCH_ADAPTIVE_MASS_SCALING;
+#pragma omp parallel
+    {
+#pragma omp for schedule(dynamic)
+        for (int k = 0; k < (int)regions.size(); k++) {
+            if (m.bodies[regions[k].first].status == ST_INACTIVE) continue;
+            update_internal_forces(S, regions[k].first, regions[k].second);
+        }
+#pragma omp for schedule(dynamic)
+        for (int i = 0; i < NB; i++) {
+            if (m.bodies[i].status == ST_INACTIVE) continue;
+            sum_newton(S, i);
+            if (half == 1) {
+                euler_temporary(S, i, dt);
+                euler_kinematics(S, i, dt * 0.5);
+            } else {
+                update_bc(S, i, t + dt);
+                euler_kinematics(S, i, dt * 0.5);
+                compute_error(S, i);
+                if (ms) compute_mass_scaling(S, i);
+            }
+        }
+    }
+}
+
+// Solver_step: attempts from the stored state until one is accepted; the
+// time step S.m.dt is updated after every attempt
+void solver_step(Sim& S) {
+    Model& m = S.m;
+    const int NB = (int)m.bodies.size();
+    std::vector<State> stored = S.s;     // Body::Store (Code/Body.h:2593-2617)
+    g_solver.neval = 0;
+    g_solver.stepped = true;
+    for (;;) {
+        g_solver.neval++;
+        double& dt = m.dt;
+        solver_half(S, 1, dt);
+        solver_half(S, 2, dt);
+        double mean_error = 0., max_error = 0.;
+        int nnodes = 0;
+        for (int i = 0; i < NB; i++) {
+            if (m.bodies[i].status == ST_INACTIVE) continue;
+            mean_error += S.s[i].total_error;
+            nnodes += m.bodies[i].n_nodes;
+            if (S.s[i].max_error > max_error) max_error = S.s[i].max_error;
+        }
+        mean_error /= nnodes;
+        g_solver.max_error = max_error;
+        g_solver.mean_error = mean_error;
+        if (m.scheme == SCH_MASS_SCALING || m.scheme == SCH_ADAPTIVE_MASS_SCALING) {
+            double total_mass = 0., max_mass = 0.;
+            for (int i = 0; i < NB; i++) {
+                if (m.bodies[i].status == ST_INACTIVE) continue;
+                total_mass += S.s[i].mass_ms;
+                if (S.s[i].mass_ms * S.s[i].inverse_mass > max_mass) max_mass = S.s[i].mass_ms * S.s[i].inverse_mass;
+            }
+            g_solver.total_mass = total_mass;
+            g_solver.max_mass = max_mass;
+        }
+        const double cup = m.contact_update_period;
+        bool fail;
+        double correction1;
+        if (m.scheme == SCH_ADAPTIVE_EULER) {
+            fail = max_error > m.target_error * m.accepted_ratio;
+            correction1 = max_error / m.target_error;
+        } else if (m.scheme == SCH_MASS_SCALING) {
+            fail = max_error > m.target_error * m.accepted_ratio;
+            correction1 = max_error * m.inv_target_error;
+        } else {
+            fail = g_solver.max_mass > m.accepted_ratio + 0.05;
+            correction1 = fail ? g_solver.max_mass : 0.;
+        }
+        const double correction2 = 1. / m.accepted_ratio;
+        if (fail)
+            for (int i = 0; i < NB; i++) restore(S.s[i], stored[i]);
+        else
+            S.time += dt;
+        if (m.scheme == SCH_ADAPTIVE_MASS_SCALING && !fail) dt = dt / std::pow(correction2, m.control_parameter);
+        else if (correction1 > correction2) dt = dt / std::pow(correction1, m.control_parameter);
+        else dt = dt / std::pow(correction2, m.control_parameter);
+        if (dt > cup) dt = cup;
+        if (!fail) {
+            update_contact_pressures(S);
+            return;
+        }
+    }
 }
 
 // Per-body energy scalars {kinetic energy from physical velocities and
@@ -1541,6 +1846,18 @@ void dump(Sim& S, const std::string& path) {
     w.scalar("time", S.time);
     w.scalar("iteration", (double)S.iteration);
     w.scalar("n_bodies", (double)S.s.size());
+    const bool adaptive = S.m.scheme != SCH_EULER;
+    const bool ms = S.m.scheme == SCH_MASS_SCALING || S.m.scheme == SCH_ADAPTIVE_MASS_SCALING;
+    if (adaptive) w.scalar("dt", S.m.dt);
+    if (adaptive && g_solver.stepped) {
+        w.scalar("neval", (double)g_solver.neval);
+        w.scalar("max_error", g_solver.max_error);
+        w.scalar("mean_error", g_solver.mean_error);
+        if (ms) {
+            w.scalar("total_mass", g_solver.total_mass);
+            w.scalar("max_mass", g_solver.max_mass);
+        }
+    }
     for (size_t bi = 0; bi < S.s.size(); bi++) {
         const State& s = S.s[bi];
         const Body& b = S.m.bodies[bi];
@@ -1570,6 +1887,12 @@ void dump(Sim& S, const std::string& path) {
         w.f64(p + "works", std::vector<double>(s.works, s.works + 7));
         w.scalar(p + "mass", s.mass);
         w.f64(p + "energy", body_energy(S, (int)bi));
+        if (adaptive && g_solver.stepped) {
+            std::vector<double> er = {s.total_error, s.max_error, (double)s.node_for_max};
+            if (ms) { er.push_back(s.mass_ms); er.push_back(s.factor_ms); }
+            w.f64(p + "error", er);
+            if (ms) { w.f64(p + "x_msf", s.fsx); w.f64(p + "y_msf", s.fsy); }
+        }
         w.i32(p + "neighbours", s.neighbours);
         std::vector<int> ci;
         std::vector<double> cd;
@@ -1613,7 +1936,8 @@ int run(Sim& S, int nsteps, const char* out_dir, const int* dump_steps, int n_du
     auto t0 = std::chrono::steady_clock::now();
     for (int st = 1; st <= nsteps; st++) {
         S.iteration++;
-        euler_step(S);
+        if (S.m.scheme == SCH_EULER) euler_step(S);
+        else solver_step(S);
         if (S.time > S.next_contact_update - 1.e-6 * S.m.dt) {
             S.next_contact_update = S.next_contact_update + S.m.contact_update_period;
             update_proximity(S);
@@ -1691,6 +2015,7 @@ int oracle_run(const char* path, int save_index, int nsteps, const char* out_dir
     for (auto& b : S.m.bodies)
         if (S.m.flags[F_UPDATE_STIFFNESS] && !b.mat_dof.empty()) { g_err = "UPDATE_STIFFNESS_MATRIX is not supported"; return 1; }
     setup(S);
+    g_solver = SolverScalars();
     return run(S, nsteps, out_dir, dump_steps, n_dump, seconds);
 }
 
@@ -1703,6 +2028,7 @@ int oracle_write_after(const char* path, int save_index, int nsteps, const char*
     bool ok = save_index >= 0 ? load_model_files(path, save_index, S.m, err) : load_model_binary(path, S.m, err);
     if (!ok) { g_err = err; return 1; }
     setup(S);
+    g_solver = SolverScalars();
     run(S, nsteps, nullptr, nullptr, 0, nullptr);
     if (!write_state(S, dynamic_path, err)) { g_err = err; return 1; }
     return 0;

@@ -97,12 +97,31 @@ struct FilteredWriter {
 // materials of the run, for the nodal stresses dumped when MEL_DUMP_STRESS is set
 static vector<Material>* g_materials = nullptr;
 
+// the adaptive schemes' step control (Code/Solver.h:98-486), dumped when
+// the scheme is not Euler
+struct SolverScalars {
+    bool adaptive = false, mass_scaling = false, stepped = false;
+    int neval = 0;
+    double max_error = 0, mean_error = 0, total_mass = 0, max_mass = 0;
+};
+static SolverScalars g_solver;
+
 static void dump_state(const string& path, vector<Body>& Bodies, double Time, int iteration, double Deltat) {
     FilteredWriter w(path);
     if (!w.ok()) { cerr << "cannot write " << path << endl; exit(2); }
     w.scalar("time", Time);
     w.scalar("iteration", (double)iteration);
     w.scalar("n_bodies", (double)Bodies.size());
+    if (g_solver.adaptive) w.scalar("dt", Deltat);
+    if (g_solver.adaptive && g_solver.stepped) {
+        w.scalar("neval", (double)g_solver.neval);
+        w.scalar("max_error", g_solver.max_error);
+        w.scalar("mean_error", g_solver.mean_error);
+        if (g_solver.mass_scaling) {
+            w.scalar("total_mass", g_solver.total_mass);
+            w.scalar("max_mass", g_solver.max_mass);
+        }
+    }
     for (size_t b = 0; b < Bodies.size(); b++) {
         Body& B = Bodies[b];
         string p = "b" + to_string(b) + ".";
@@ -139,7 +158,6 @@ static void dump_state(const string& path, vector<Body>& Bodies, double Time, in
         NODE_FIELD("y_force", N.y_force)
         NODE_FIELD("x_mass", N.x_mass)
         NODE_FIELD("y_mass", N.y_mass)
-#undef NODE_FIELD
         w.f64(p + "rigid", {B.x_current, B.y_current, B.r_current, B.x_displacement, B.y_displacement,
                            B.r_displacement, B.x_velocity, B.y_velocity, B.r_velocity, B.x_acceleration,
                            B.y_acceleration, B.r_acceleration, B.x_contact_force, B.y_contact_force,
@@ -150,9 +168,21 @@ static void dump_state(const string& path, vector<Body>& Bodies, double Time, in
                            B.neumann_work, B.damping_work, B.alid_work});
         w.scalar(p + "mass", B.mass);
         w.f64(p + "energy", body_energy(B, Deltat));
+        if (g_solver.adaptive && g_solver.stepped) {
+            // Body::Compute_error / Compute_mass_scaling results (Code/Body.h:2524-2585),
+            // once a step has set them (the body's mass_mass_scaling has no initial value)
+            vector<double> er = {B.total_error, B.max_error, (double)B.node_for_max_error};
+            if (g_solver.mass_scaling) { er.push_back(B.mass_mass_scaling); er.push_back(B.factor_mass_scaling); }
+            w.f64(p + "error", er);
+            if (g_solver.mass_scaling) {
+                NODE_FIELD("x_msf", N.x_factor_mass_scaling)
+                NODE_FIELD("y_msf", N.y_factor_mass_scaling)
+            }
+        }
         vector<int> nb;
         for (int k = 0; k < B.nb_neighbours; k++)
             for (int c = 0; c < 3; c++) nb.push_back(B.neighbours[k][c]);
+#undef NODE_FIELD
         w.i32(p + "neighbours", nb);
         vector<int> ci;
         vector<double> cd;
@@ -280,6 +310,8 @@ int main(int argc, char** argv) {
     if (flags[3] == 1)
         Initialize_CZM(Nb_bodies, Bodies, Nb_contact_laws, Contact_laws, flags, Xmin_period, Xmax_period);
 
+    g_solver.adaptive = Solver != "Euler";
+    g_solver.mass_scaling = Solver == "Mass_Scaling" || Solver == "Adaptive_Mass_Scaling";
     auto want = [&](int s) { return find(dump_steps.begin(), dump_steps.end(), s) != dump_steps.end(); };
     char name[64];
     if (want(0)) {
@@ -289,9 +321,21 @@ int main(int argc, char** argv) {
     double t0 = omp_get_wtime();
     for (int s = 1; s <= nsteps; s++) {
         Number_iteration++;
-        Euler_step(Nb_bodies, Bodies, Nb_regions, Regions, Nb_materials, Materials, Nb_contact_laws,
-                   Contact_laws, Contacts_Table, Tend, Xmin_period, Xmax_period, Penalty, Xgravity, Ygravity,
-                   Time, Deltat, Number_iteration, flags);
+        // Code/main.cpp:148-166
+        if (Solver == "Euler") {
+            Euler_step(Nb_bodies, Bodies, Nb_regions, Regions, Nb_materials, Materials, Nb_contact_laws,
+                       Contact_laws, Contacts_Table, Tend, Xmin_period, Xmax_period, Penalty, Xgravity, Ygravity,
+                       Time, Deltat, Number_iteration, flags);
+        } else {
+            Solver_step(Nb_bodies, Bodies, Nb_regions, Regions, Nb_materials, Materials, Nb_contact_laws,
+                        Contact_laws, Contacts_Table, Tend, Xmin_period, Xmax_period, Penalty, Xgravity, Ygravity,
+                        Time, Deltat, Number_iteration, Next_contact_update, Contact_update_period, Target_error,
+                        Inv_Target_error, Control_parameter, Accepted_ratio, Max_mass_scaling,
+                        Control_parameter_mass_scaling, Error_factor_mass_scaling, Decrease_factor_mass_scaling,
+                        g_solver.neval, g_solver.max_error, g_solver.mean_error, flags, g_solver.total_mass,
+                        g_solver.max_mass, Solver);
+            g_solver.stepped = true;
+        }
         // contact-update cadence, Code/main.cpp:191-196
         if (Time > Next_contact_update - 1.e-6 * Deltat) {
             Next_contact_update = Next_contact_update + Contact_update_period;

@@ -15,6 +15,9 @@ GOLDEN = os.path.join(HERE, "golden")
 VARIANTS = sorted(f[:-7] for f in os.listdir(GOLDEN)
                   if f.endswith(".tar.gz") and f != "c1.tar.gz" and not f.endswith("_restart.tar.gz")
                   and not f.endswith("_stress.tar.gz"))
+# golden sets run with the adaptive schemes of Solver_step; the others are Euler
+ADAPTIVE_VARIANTS = [v for v in VARIANTS if v in ("tiny_adaptive", "tiny_mass_scaling", "tiny_adaptive_ms")]
+EULER_VARIANTS = [v for v in VARIANTS if v not in ADAPTIVE_VARIANTS]
 RESTARTS = sorted(f[:-15] for f in os.listdir(GOLDEN) if f.endswith("_restart.tar.gz"))
 # nodal stress sets: model files and the reference's stress fields
 STRESSES = sorted(f[:-14] for f in os.listdir(GOLDEN) if f.endswith("_stress.tar.gz"))

@@ -45,6 +45,17 @@ def run_oracle(model_path, steps, out_dir, save_index=0, nthreads=0):
 def device_state(ctx, n_nodes_per_body, n_borders_per_body, border_sizes):
     """The same records as a dump, read through the C-ABI."""
     D = {"time": np.array([ctx.get_info().time])}
+    si = ctx.solver_info()
+    ms = si.scheme in (2, 3)
+    if si.scheme != 0:
+        # the adaptive schemes' step control (Code/Solver.h:285-484)
+        D["dt"] = np.array([si.dt])
+        D["neval"] = np.array([float(si.neval)])
+        D["max_error"] = np.array([si.max_error])
+        D["mean_error"] = np.array([si.mean_error])
+        if ms:
+            D["total_mass"] = np.array([si.total_mass])
+            D["max_mass"] = np.array([si.max_mass])
     for b, n in enumerate(n_nodes_per_body):
         p = "b%d." % b
         for f in native.NODE_FIELDS:
@@ -58,6 +69,10 @@ def device_state(ctx, n_nodes_per_body, n_borders_per_body, border_sizes):
         D[p + "rigid"] = r
         D[p + "works"] = w
         D[p + "energy"] = ctx.body_energy(b)
+        if si.scheme != 0:
+            D[p + "error"] = ctx.body_error(b)[:5 if ms else 3]
+            if ms:
+                D[p + "x_msf"], D[p + "y_msf"] = ctx.mass_scaling(b, n)
         for k in range(n_borders_per_body[b]):
             q = p + "border%d." % k
             ns = border_sizes[b][k]
