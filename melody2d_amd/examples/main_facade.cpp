@@ -15,7 +15,8 @@ int main(int argc, char** argv) {
         melody2d::Solver solver;
         char name[64];
         for (int save = number_save + 1; save <= number_save + saves; save++) {
-            for (int k = 0; k < steps_per_save; k++) solver.Euler_step(model);
+            // the model's scheme: Euler_step or Solver_step (Code/main.cpp:148-166)
+            for (int k = 0; k < steps_per_save; k++) solver.Solver_step(model);
             model.synchronize();
             snprintf(name, sizeof name, "DYNAMIC_%05d.asc", save);
             model.write_dynamic(name);

@@ -22,8 +22,9 @@ int main(int argc, char** argv) {
     char name[64];
     int rc = 0;
     for (int save = number_save + 1; save <= number_save + saves; save++) {
-        // steps_per_save x [Euler_step + contact update every Contact_update_period]
-        mel_status st = mel_euler_steps(ctx, steps_per_save);
+        // steps_per_save x [Euler_step, or Solver_step for the adaptive schemes,
+        // + contact update every Contact_update_period] (Code/main.cpp:148-196)
+        mel_status st = mel_solver_steps(ctx, steps_per_save);
         if (st == MEL_OK) st = mel_synchronize(ctx);
         if (st != MEL_OK) {                                // MEL_E_NAN: Code/main.cpp:227-237
             fprintf(stderr, "step failed: %s\n", mel_last_error(ctx));
