@@ -48,6 +48,7 @@ typedef enum {
     MEL_X_BODY_FORCE, MEL_Y_BODY_FORCE, MEL_X_DIRICHLET_FORCE, MEL_Y_DIRICHLET_FORCE,
     MEL_X_NEUMANN_FORCE, MEL_Y_NEUMANN_FORCE, MEL_X_DAMPING_FORCE, MEL_Y_DAMPING_FORCE,
     MEL_X_FORCE, MEL_Y_FORCE, MEL_X_MASS, MEL_Y_MASS,
+    MEL_X_ALID_FORCE, MEL_Y_ALID_FORCE,     /* Node::x_alid_force (absorbing layers, Code/Body.h:1163-1296) */
     MEL_NODE_FIELD_COUNT
 } mel_node_field;
 

@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_ad_gather(DevModel m, AdArray
 // and the kinematics of its border nodes.
 __device__ void node_block_half(const DevModel& m, const BlockArrays& B, const AdArrays& A, int blk, int half,
                                 double dt) {
-    __shared__ double red[BLOCK_THREADS / 64][5];
+    __shared__ double red[BLOCK_THREADS / 64][6];
     const int t = threadIdx.x;
     const int body = B.blk_body[blk];
     if (m.body_status[body] == 1) return;
@@ -194,10 +194,11 @@ __device__ void node_block_half(const DevModel& m, const BlockArrays& B, const A
     const int nbn = B.blk_bn_n[blk];
     for (int i = t; i < nbn; i += BLOCK_THREADS) bucket_sum_node(m, bns[i]);
     __syncthreads();
-    double w[5] = {0., 0., 0., 0., 0.};
+    double w[6] = {0., 0., 0., 0., 0., 0.};   // internal body dirichlet neumann damping alid
     double gx = m.gravity_x, gy = m.gravity_y;
     if (m.body_status[body] == 2) { gx = 0.; gy = 0.; }
     const double h = dt * 0.5;
+    const bool alid = m.alid_on != 0;
 #pragma unroll
     for (int q = 0; q < NODES_PER_THREAD; q++) {
         int o = t + q * BLOCK_THREADS;
@@ -214,8 +215,9 @@ __device__ void node_block_half(const DevModel& m, const BlockArrays& B, const A
         }
         double bx = m.mx[j] * gx, by = m.my[j] * gy;
         double kx = m.fkx[j], ky = m.fky[j];
-        double fx = ix + cx + 0. + bx + dx + nx + kx + 0.;
-        double fy = iy + cy + 0. + by + dy + ny + ky + 0.;
+        const double lx = alid ? m.fax[j] : 0., ly = alid ? m.fay[j] : 0.;
+        double fx = ix + cx + 0. + bx + dx + nx + kx + lx;
+        double fy = iy + cy + 0. + by + dy + ny + ky + ly;
         double apx = fx * m.imx[j] / A.fsx[j], apy = fy * m.imy[j] / A.fsy[j];
         double vpx = m.vpx[j], vpy = m.vpy[j];
         const double upx = m.upx[j], upy = m.upy[j];
@@ -229,6 +231,7 @@ __device__ void node_block_half(const DevModel& m, const BlockArrays& B, const A
         w[2] += h * (hx * dx + hy * dy);
         w[3] += h * (hx * nx + hy * ny);
         w[4] += h * (hx * kx + hy * ky);
+        w[5] += h * (hx * lx + hy * ly);
         vpx = vpx + h * apx;
         vpy = vpy + h * apy;
         const double nux = upx + h * vpx, nuy = upy + h * vpy;
@@ -238,17 +241,19 @@ __device__ void node_block_half(const DevModel& m, const BlockArrays& B, const A
         m.vpx[j] = vpx; m.vpy[j] = vpy;
         m.apx[j] = apx; m.apy[j] = apy;
     }
+    const int nw = alid ? 6 : 5;
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
+    for (int q = 0; q < 6; q++) {
+        if (q >= nw) break;
         double v = w[q];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
         if ((t & 63) == 0) red[t >> 6][q] = v;
     }
     __syncthreads();
     if (t == 0) {
-        const int slot[5] = {W_INTERNAL, W_BODY, W_DIRICHLET, W_NEUMANN, W_DAMPING};
+        const int slot[6] = {W_INTERNAL, W_BODY, W_DIRICHLET, W_NEUMANN, W_DAMPING, W_ALID};
         double* acc = B.blk_work + 14 * (size_t)blk;
-        for (int q = 0; q < 5; q++) {
+        for (int q = 0; q < nw; q++) {
             double v = 0.;
             for (int wv = 0; wv < BLOCK_THREADS / 64; wv++) v += red[wv][q];
             int s = slot[q];

@@ -85,8 +85,9 @@ __global__ void k_node_outputs(DevModel m, int with_body_forces) {
     double bx = m.mx[j] * gx, by = m.my[j] * gy;
     m.fcx[j] = cx; m.fcy[j] = cy; m.fdx[j] = dx; m.fdy[j] = dy; m.fnx[j] = nx; m.fny[j] = ny;
     m.fbx[j] = bx; m.fby[j] = by;
-    m.fx[j] = m.fix[j] + cx + 0. + bx + dx + nx + m.fkx[j] + 0.;
-    m.fy[j] = m.fiy[j] + cy + 0. + by + dy + ny + m.fky[j] + 0.;
+    const double lx = m.alid_on ? m.fax[j] : 0., ly = m.alid_on ? m.fay[j] : 0.;
+    m.fx[j] = m.fix[j] + cx + 0. + bx + dx + nx + m.fkx[j] + lx;
+    m.fy[j] = m.fiy[j] + cy + 0. + by + dy + ny + m.fky[j] + ly;
 }
 
 }  // namespace mel

@@ -50,6 +50,53 @@ __global__ void k_damping(DevModel m) {
     m.fky[j] = fy;
 }
 
+// Body::Update_alid_forces (Code/Body.h:1163-1296) at time t for one node
+// of a body with absorbing layers: the mass-proportional term from the
+// node's physical velocity, then the stiffness-proportional terms of the
+// matrix entries naming the node, in entry order, from velocity parameters
+// relative to the layers' imposed velocity (pvx, pvy)
+__device__ inline double alid_velocity(const DevModel& m, int h0, int h1, double t, bool& on) {
+    on = h1 > h0;
+    double pv = 0.;
+    for (int n = h0; n + 1 < h1; n++) {
+        double t0 = m.hist_t[n], p0 = m.hist_v[n], t1 = m.hist_t[n + 1], p1 = m.hist_v[n + 1];
+        if ((t0 <= t) & (t <= t1)) { pv = p0 + (p1 - p0) / (t1 - t0) * (t - t0); break; }
+    }
+    return pv;
+}
+
+__global__ void k_alid(DevModel m, double t) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m.n_alid_nodes) return;
+    const int j = m.alid_nodes[q];
+    const int b = m.node_body[j];
+    if (m.body_status[b] == 1) return;
+    const int* h = m.body_alid + 4 * b;
+    bool onx, ony;
+    const double pvx = alid_velocity(m, h[0], h[1], t, onx);
+    const double pvy = alid_velocity(m, h[2], h[3], t, ony);
+    const double beta = m.alid_b[j];
+    double fx = onx ? -beta * m.mx[j] * (m.vx[j] - pvx) : 0.;
+    double fy = ony ? -beta * m.my[j] * (m.vy[j] - pvy) : 0.;
+    for (int k = m.mt_ptr[j]; k < m.mt_ptr[j + 1]; k++) {
+        const int code = m.mt_entry[k];
+        const int e = code >> 1, side = code & 1;
+        const int dof0 = m.m_dof[2 * e], dof1 = m.m_dof[2 * e + 1];
+        const int n0 = dof0 / 2, r0 = dof0 - 2 * n0, n1 = dof1 / 2, r1 = dof1 - 2 * n1;
+        const double d = m.m_stiff[e] / (1. / m.alid_a[n0] + 1. / m.alid_a[n1]);
+        // side 0: node0's direction r0 from node1's r1 velocity parameter;
+        // side 1: node1's direction r1 from node0's r0 one (off-diagonal entries)
+        const int rt = side == 0 ? r0 : r1, rs = side == 0 ? r1 : r0, ns = side == 0 ? n1 : n0;
+        if (!(rt == 0 ? onx : ony)) continue;
+        const double v = rs == 0 ? m.vpx[ns] - pvx : m.vpy[ns] - pvy;
+        const double f = -v * d;
+        if (rt == 0) fx += f;
+        else fy += f;
+    }
+    m.fax[j] = fx;
+    m.fay[j] = fy;
+}
+
 __global__ void k_rigid_nodes(DevModel m, int with_forces) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m.n_rigid_nodes) return;

@@ -16,6 +16,7 @@ constexpr int NB_PER = 4;         // neighbour search: candidate bodies per thre
 __global__ void k_border_length(DevModel m);
 __global__ void k_bc_eval(DevModel m, int plus_dt);
 __global__ void k_damping(DevModel m);
+__global__ void k_alid(DevModel m, double t);
 __global__ void k_rigid_nodes(DevModel m, int with_forces);
 __global__ void k_step_contacts(DevModel m, const int* bc_bn, int n_bc);
 __global__ void k_step_gather(DevModel m, const int* split_bn, int n_split);

@@ -919,7 +919,7 @@ __device__ inline void bn_lengths(const DevModel& m, int bn) {
 // Workgroups past the node blocks move the rigid nodes, evaluate the BCs at
 // t + dt and store the new time.
 __device__ inline void node_block_finish(const DevModel& m, const BlockArrays& B, int blk) {
-    __shared__ double red[BLOCK_THREADS / 64][5];
+    __shared__ double red[BLOCK_THREADS / 64][6];
     const int t = threadIdx.x;
     const int body = B.blk_body[blk];
     if (m.body_status[body] == 1) return;   // inactive body: no update (uniform over the block)
@@ -929,10 +929,11 @@ __device__ inline void node_block_finish(const DevModel& m, const BlockArrays& B
     const int nbn = B.blk_bn_n[blk];        // 0 for blocks of split bodies
     for (int i = t; i < nbn; i += BLOCK_THREADS) bucket_sum_node(m, bns[i]);
     __syncthreads();
-    double w[5] = {0., 0., 0., 0., 0.};   // internal body dirichlet neumann damping
+    double w[6] = {0., 0., 0., 0., 0., 0.};   // internal body dirichlet neumann damping alid
     double gx = m.gravity_x, gy = m.gravity_y;
     if (m.body_status[body] == 2) { gx = 0.; gy = 0.; }
     const double dt = m.dt;
+    const bool alid = m.alid_on != 0;
 #pragma unroll
     for (int q = 0; q < NODES_PER_THREAD; q++) {
         int o = t + q * BLOCK_THREADS;
@@ -949,8 +950,9 @@ __device__ inline void node_block_finish(const DevModel& m, const BlockArrays& B
         }
         double bx = m.mx[j] * gx, by = m.my[j] * gy;
         double kx = m.fkx[j], ky = m.fky[j];
-        double fx = ix + cx + 0. + bx + dx + nx + kx + 0.;
-        double fy = iy + cy + 0. + by + dy + ny + ky + 0.;
+        const double lx = alid ? m.fax[j] : 0., ly = alid ? m.fay[j] : 0.;
+        double fx = ix + cx + 0. + bx + dx + nx + kx + lx;
+        double fy = iy + cy + 0. + by + dy + ny + ky + ly;
         double apx = fx * m.imx[j], apy = fy * m.imy[j];
         double vpx = m.vpx[j], vpy = m.vpy[j];
         double hx = vpx + 0.5 * dt * apx, hy = vpy + 0.5 * dt * apy;
@@ -959,6 +961,7 @@ __device__ inline void node_block_finish(const DevModel& m, const BlockArrays& B
         w[2] += dt * (hx * dx + hy * dy);
         w[3] += dt * (hx * nx + hy * ny);
         w[4] += dt * (hx * kx + hy * ky);
+        w[5] += dt * (hx * lx + hy * ly);
         vpx = vpx + dt * apx;
         vpy = vpy + dt * apy;
         const double nux = m.upx[j] + dt * vpx, nuy = m.upy[j] + dt * vpy;
@@ -970,17 +973,19 @@ __device__ inline void node_block_finish(const DevModel& m, const BlockArrays& B
     }
     // block work increments: fixed-order reduction (wavefront butterflies,
     // then the waves in order), accumulated with compensation across steps
+    const int nw = alid ? 6 : 5;
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
+    for (int q = 0; q < 6; q++) {
+        if (q >= nw) break;
         double v = w[q];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
         if ((t & 63) == 0) red[t >> 6][q] = v;
     }
     __syncthreads();
     if (t == 0) {
-        const int slot[5] = {W_INTERNAL, W_BODY, W_DIRICHLET, W_NEUMANN, W_DAMPING};
+        const int slot[6] = {W_INTERNAL, W_BODY, W_DIRICHLET, W_NEUMANN, W_DAMPING, W_ALID};
         double* acc = B.blk_work + 14 * (size_t)blk;
-        for (int q = 0; q < 5; q++) {
+        for (int q = 0; q < nw; q++) {
             double v = 0.;
             for (int wv = 0; wv < BLOCK_THREADS / 64; wv++) v += red[wv][q];
             // Neumaier accumulation: acc[2s] sum, acc[2s+1] compensation

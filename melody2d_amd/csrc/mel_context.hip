@@ -856,8 +856,9 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     // MELODY2D_FUSED_STEP=0 asks for the launch sequence
     const char* fe = getenv("MELODY2D_FUSED_STEP");
     const bool fused_off = c.opts.fused_step >= 0 ? c.opts.fused_step == 0 : (fe && fe[0] == '0');
-    // (the adaptive schemes run their half steps as the launch sequence)
-    c.two_launch = B.lds_finish_ok && !fused_off && c.model.scheme == SCH_EULER;
+    // (the adaptive schemes run their half steps as the launch sequence, and
+    // so do models with absorbing layers)
+    c.two_launch = B.lds_finish_ok && !fused_off && c.model.scheme == SCH_EULER && !c.d.alid_on;
     // the launch sequence takes the wide-window kernel when Gauss points
     // with more support entries than the default window are common (RPIM
     // supports of 30 nodes; a few long MLS supports read their tail from
@@ -1043,8 +1044,9 @@ void euler_step(mel_ctx& c) {
     // Initialize_contact_forces, Update_contacts, Update_contact_forces
     // (records into buckets) | Update_bc_forces
     TIMED(c, KT_CONTACTS, a, (k_step_contacts<<<n_cwg, BLOCK_THREADS, 0, a>>>(d, B.bc_bn, B.n_bc_bn)));
-    // Update_damping_forces
+    // Update_damping_forces, Update_alid_forces(Time)
     if (d.n_matrix > 0) TIMED(c, KT_DAMPING, a, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, a>>>(d)));
+    if (d.n_alid_nodes > 0) k_alid<<<blocks(d.n_alid_nodes), BLOCK, 0, a>>>(d, c.time);
     // rigid resultants, Newton, Euler | sends to the border nodes of split bodies
     if (d.n_rigid + B.n_split_bn > 0)
         TIMED(c, KT_GATHER, a, (k_step_gather<<<d.n_rigid + blocks(B.n_split_bn, BLOCK_THREADS), BLOCK_THREADS, 0, a>>>(d, B.split_bn, B.n_split_bn)));
@@ -1060,6 +1062,8 @@ void euler_step(mel_ctx& c) {
     TIMED(c, KT_FINISH, s, (k_update_finish<<<B.n_blocks + extra, BLOCK_THREADS, 0, s>>>(d, B, t_next)));
     if (B.n_split_kin > 0)
         k_kinematics_list<<<blocks(B.n_split_kin), BLOCK, 0, s>>>(d, B.split_kin, B.n_split_kin);
+    // the absorbing layers read every node's physical velocity of the end of the step
+    if (d.n_alid_nodes > 0) k_kinematics_list<<<blocks(d.n_alid_nodes), BLOCK, 0, s>>>(d, d.alid_nodes, d.n_alid_nodes);
     c.time = t_next;
     c.outputs_stale = true;
 }
@@ -1233,6 +1237,9 @@ void build(mel_ctx& c) {
     d.gravity_y = m.gravity_y;
     d.dt = m.dt;
     d.self_contact = m.flags[F_NO_MONITORING] ? 0 : 1;
+    d.alid_on = 0;
+    for (const Body& B : m.bodies)
+        if (!B.rigid && B.alid.on) d.alid_on = 1;
     d.law_f32 = c.precision == MEL_FP32_MIXED ? 1 : 0;
     for (auto& b : m.bodies) {
         if (b.status == ST_INACTIVE) throw CapacityError{"inactive bodies are not supported"};
@@ -1346,13 +1353,14 @@ void build(mel_ctx& c) {
     c.nnz_d = (long long)d_idx.size();
     // damping matrices
     std::vector<int> m_dof;
-    std::vector<double> m_damp;
+    std::vector<double> m_damp, m_stiff;
     for (int b = 0; b < NB; b++) {
         Body& B = m.bodies[b];
         for (size_t i = 0; i < B.mat_damping.size(); i++) {
             m_dof.push_back(2 * node_off[b] + B.mat_dof[2 * i]);
             m_dof.push_back(2 * node_off[b] + B.mat_dof[2 * i + 1]);
             m_damp.push_back(B.mat_damping[i]);
+            m_stiff.push_back(B.mat_stiffness[i]);
         }
     }
     int NM = (int)m_damp.size();
@@ -1557,6 +1565,8 @@ void build(mel_ctx& c) {
     double** forces[] = {&d.fix, &d.fiy, &d.fcx, &d.fcy, &d.fbx, &d.fby, &d.fdx, &d.fdy, &d.fnx, &d.fny, &d.fkx,
                          &d.fky, &d.fx, &d.fy};
     for (double** p : forces) *p = c.alloc<double>(N);
+    d.fax = c.alloc<double>(N);
+    d.fay = c.alloc<double>(N);
     c.d.node_body_host = node_body.data();
     build_blocks(c, g_ptr, g_idx, g_phix, g_phiy, g_c0, t_ptr, t_gp, bn_node, node_off, gp_off, perreg, d_ptr, d_idx,
                  bt_ptr, bt_bn);
@@ -1642,6 +1652,35 @@ void build(mel_ctx& c) {
     }
     d.br_xparam = c.upload(br_xparam);
     d.br_yparam = c.upload(br_yparam);
+    {
+        // absorbing layers: per-node coefficients, the bodies' velocity
+        // histories after the borders' ones, the matrix stiffness, the nodes
+        std::vector<int> body_alid(4 * (size_t)NB, 0), alid_nodes;
+        std::vector<double> aa, ab;
+        if (d.alid_on) { aa.assign(N, 0.); ab.assign(N, 0.); }
+        for (int b = 0; b < NB; b++) {
+            const Body& B = m.bodies[b];
+            if (B.rigid || !B.alid.on) continue;
+            const History* hs[2] = {&B.alid.xh, &B.alid.yh};
+            for (int k = 0; k < 2; k++) {
+                body_alid[4 * b + 2 * k] = (int)hist_t.size();
+                hist_t.insert(hist_t.end(), hs[k]->t.begin(), hs[k]->t.end());
+                hist_v.insert(hist_v.end(), hs[k]->v.begin(), hs[k]->v.end());
+                body_alid[4 * b + 2 * k + 1] = (int)hist_t.size();
+            }
+            for (int i = 0; i < B.n_nodes; i++) {
+                aa[node_off[b] + i] = B.alid_alpha[i];
+                ab[node_off[b] + i] = B.alid_beta[i];
+                alid_nodes.push_back(node_off[b] + i);
+            }
+        }
+        d.body_alid = c.upload(body_alid);
+        d.alid_a = c.upload(aa);
+        d.alid_b = c.upload(ab);
+        d.alid_nodes = c.upload(alid_nodes);
+        d.n_alid_nodes = (int)alid_nodes.size();
+        d.m_stiff = c.upload(d.alid_on ? m_stiff : std::vector<double>());
+    }
     d.br_hist_ptr = c.upload(br_hist_ptr);
     d.hist_t = c.upload(hist_t);
     d.hist_v = c.upload(hist_v);
@@ -2038,6 +2077,7 @@ void eval_forces(mel_ctx& c, unsigned what, unsigned char* save_buf = nullptr) {
         if (!external) k_clear_point_forces<<<blocks(d.n_bnodes), BLOCK, 0, s>>>(d, 0, 1);
     }
     if (external && d.n_matrix > 0) k_damping<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d);
+    if (external && d.n_alid_nodes > 0) k_alid<<<blocks(d.n_alid_nodes), BLOCK, 0, s>>>(d, c.time);
     if (!external || d.n_matrix == 0) k_clear_node_forces<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d, 0, 1);
     if (internal) launch_internal_forces(c, s);
     else k_clear_node_forces<<<blocks(d.n_nodes), BLOCK, 0, s>>>(d, 1, 0);
@@ -2081,6 +2121,8 @@ void solver_half(mel_ctx& c, int half, double dt, double t) {
     HIP_OK(hipStreamWaitEvent(a, c.ev_fork, 0));
     TIMED(c, KT_CONTACTS, a, (k_step_contacts<<<n_cwg, BLOCK_THREADS, 0, a>>>(dh, B.bc_bn, B.n_bc_bn)));
     if (d.n_matrix > 0) TIMED(c, KT_DAMPING, a, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, a>>>(d)));
+    // Update_alid_forces(Time) in the first half, (Time + dt/2) in the second
+    if (d.n_alid_nodes > 0) k_alid<<<blocks(d.n_alid_nodes), BLOCK, 0, a>>>(d, half == 1 ? t : t + dt * 0.5);
     if (d.n_rigid + B.n_split_bn > 0)
         TIMED(c, KT_GATHER, a, (k_ad_gather<<<d.n_rigid + blocks(B.n_split_bn, BLOCK_THREADS), BLOCK_THREADS, 0, a>>>(
                                    d, c.ad, B.split_bn, B.n_split_bn, half, dt, t + dt)));
@@ -2091,6 +2133,10 @@ void solver_half(mel_ctx& c, int half, double dt, double t) {
     TIMED(c, KT_FINISH, s, (k_ad_finish<<<B.n_blocks + extra, BLOCK_THREADS, 0, s>>>(d, B, c.ad, half, dt, t + dt)));
     if (B.n_split_kin > 0)
         k_kinematics_list<<<blocks(B.n_split_kin), BLOCK, 0, s>>>(d, B.split_kin, B.n_split_kin);
+    // Update_kinematics of the first half's end, for the absorbing layers of
+    // the second (the second half's end rebuilds every node's)
+    if (half == 1 && d.n_alid_nodes > 0)
+        k_kinematics_list<<<blocks(d.n_alid_nodes), BLOCK, 0, s>>>(d, d.alid_nodes, d.n_alid_nodes);
     HIP_OK(hipGetLastError());
 }
 
@@ -2168,7 +2214,7 @@ std::vector<StateArray> state_arrays(mel_ctx& c, int nc) {
     const size_t N = d.n_nodes, NB = d.n_bodies, NS = d.n_slots, NBR = d.n_borders, NBN = d.n_bnodes;
     double* nf[] = {d.xc, d.yc, d.ux, d.uy, d.vx, d.vy, d.ax, d.ay, d.upx, d.upy, d.vpx, d.vpy, d.apx, d.apy,
                     d.fix, d.fiy, d.fcx, d.fcy, d.fbx, d.fby, d.fdx, d.fdy, d.fnx, d.fny, d.fkx, d.fky,
-                    d.fx, d.fy, d.mx, d.my, d.imx, d.imy};
+                    d.fx, d.fy, d.mx, d.my, d.imx, d.imy, d.fax, d.fay};
     for (double* p : nf) v.push_back({p, N * sizeof(double)});
     v.push_back({d.body_rigid, NB * RIGID_FIELDS * sizeof(double)});
     v.push_back({d.body_mass, NB * sizeof(double)});
@@ -2220,7 +2266,8 @@ double* node_field_ptr(mel_ctx& c, int field) {
     DevModel& d = c.d;
     double* f[] = {d.xc, d.yc, d.ux, d.uy, d.vx, d.vy, d.ax, d.ay, d.upx, d.upy, d.vpx, d.vpy, d.apx, d.apy,
                    d.fix, d.fiy, d.fcx, d.fcy, d.fbx, d.fby, d.fdx, d.fdy, d.fnx, d.fny, d.fkx, d.fky,
-                   d.fx, d.fy, d.mx, d.my};
+                   d.fx, d.fy, d.mx, d.my, d.fax, d.fay};
+    static_assert(sizeof(f) / sizeof(f[0]) == MEL_NODE_FIELD_COUNT, "one array per node field");
     if (field < 0 || field >= MEL_NODE_FIELD_COUNT) return nullptr;
     return f[field];
 }

@@ -187,6 +187,13 @@ struct DevModel {
     int *rm_cnt;                             // per rigid body: elements it is the master of
     int *rm_list;                            // RIGID_LIST_CAP per rigid body: those elements, ascending
     int *rs_range;                           // 2 per rigid body: its slave elements [begin, end)
+    // absorbing layers (Body::Update_alid_forces, Code/Body.h:1163-1296)
+    int alid_on;                             // some deformable body has them
+    double *fax, *fay;                       // Node::x_alid_force, y_alid_force (zero elsewhere)
+    double *alid_a, *alid_b;                 // per node: alid_alpha, alid_beta (ALID models)
+    int *body_alid;                          // per body: x history [h0, h1), y history [h2, h3) in hist_t/hist_v
+    int *alid_nodes, n_alid_nodes;           // the nodes of the bodies with absorbing layers
+    double *m_stiff;                         // per matrix entry: its stiffness (ALID models)
 };
 
 // --------------------------------------------------------------------------

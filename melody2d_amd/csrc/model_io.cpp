@@ -292,6 +292,48 @@ void initialize_segments(Model& m, Body& body) {
     }
 }
 
+// Node::alid_alpha, alid_beta of a deformable body (Code/IO.h:1548-1584):
+// graded through layers of width `range` inside the bounds, the x layers
+// taking the larger value where they overlap the y layers
+void alid_coefficients(Body& b) {
+    b.alid_alpha.assign(b.n_nodes, 0.);
+    b.alid_beta.assign(b.n_nodes, 0.);
+    if (!b.alid.on) return;
+    const Alid& A = b.alid;
+    for (int i = 0; i < b.n_nodes; i++) {
+        const double xini = b.X[i], yini = b.Y[i];
+        double al = 0., be = 0., ka, kb, temp;
+        if (yini < A.ymin + A.range) {
+            ka = (A.alphaout - A.alphain) / std::pow(-A.range, A.exponent);
+            kb = (A.betaout - A.betain) / std::pow(-A.range, A.exponent);
+            al = A.alphain + ka * std::pow(yini - (A.ymin + A.range), A.exponent);
+            be = A.betain + kb * std::pow(yini - (A.ymin + A.range), A.exponent);
+        } else if (yini > A.ymax - A.range) {
+            ka = (A.alphaout - A.alphain) / std::pow(A.range, A.exponent);
+            kb = (A.betaout - A.betain) / std::pow(A.range, A.exponent);
+            al = A.alphain + ka * std::pow(yini - (A.ymax - A.range), A.exponent);
+            be = A.betain + kb * std::pow(yini - (A.ymax - A.range), A.exponent);
+        }
+        if (xini < A.xmin + A.range) {
+            ka = (A.alphaout - A.alphain) / std::pow(-A.range, A.exponent);
+            kb = (A.betaout - A.betain) / std::pow(-A.range, A.exponent);
+            temp = A.alphain + ka * std::pow(xini - (A.xmin + A.range), A.exponent);
+            if (temp > al) al = temp;
+            temp = A.betain + kb * std::pow(xini - (A.xmin + A.range), A.exponent);
+            if (temp > be) be = temp;
+        } else if (xini > A.xmax - A.range) {
+            ka = (A.alphaout - A.alphain) / std::pow(A.range, A.exponent);
+            kb = (A.betaout - A.betain) / std::pow(A.range, A.exponent);
+            temp = A.alphain + ka * std::pow(xini - (A.xmax - A.range), A.exponent);
+            if (temp > al) al = temp;
+            temp = A.betain + kb * std::pow(xini - (A.xmax - A.range), A.exponent);
+            if (temp > be) be = temp;
+        }
+        b.alid_alpha[i] = al;
+        b.alid_beta[i] = be;
+    }
+}
+
 namespace {
 bool load_model_files_impl(const std::string& dir, int save_index, Model& m, std::string& err) {
     m = Model();
@@ -444,14 +486,43 @@ bool load_model_files_impl(const std::string& dir, int save_index, Model& m, std
                     std::getline(in, token);
                     b.borders[i].periodicity = periodicity_of(per);
                     b.borders[i].linear = interp != "Spline";
-                    if (!b.borders[i].linear) { err = "Spline interpolant is not supported"; return false; }
+                    if (!b.borders[i].linear) {
+                        // Closest_2_segment's Spline branch (Code/Tools.h:82-105) never sets
+                        // flag_detect, which its callers read uninitialised
+                        err = "Spline interpolant is not supported (the reference's Spline projection leaves its "
+                              "contact flag unset)";
+                        return false;
+                    }
                     b.borders[i].nodes.resize(nn);
                     read_bc(in, b.borders[i].xbc);
                     read_bc(in, b.borders[i].ybc);
                     std::getline(in, token);
                 }
             }
-            if (starts(line, "ALID")) { err = "ALID absorbing layers are not supported"; return false; }
+            if (starts(line, "ALID")) {
+                // Code/IO.h:1395-1485: twelve values, then the x history,
+                // a separator line when both are given, the y history
+                Body& b = m.bodies.at(index_body);
+                Alid& A = b.alid;
+                int nx = 0, ny = 0;
+                in >> nx >> ny >> A.xmin >> A.xmax >> A.ymin >> A.ymax >> A.range >> A.alphain >> A.alphaout >>
+                    A.betain >> A.betaout >> A.exponent;
+                if (!in || nx < 0 || ny < 0 || nx > (1 << 20) || ny > (1 << 20)) { err = "ALID header"; return false; }
+                auto pairs = [&](History& h, int n) {
+                    for (int j = 0; j < n; j++) {
+                        double t, v;
+                        in >> t >> v;
+                        std::getline(in, token);
+                        h.t.push_back(t);
+                        h.v.push_back(v);
+                    }
+                };
+                pairs(A.xh, nx);
+                if (nx > 0 && ny > 0) std::getline(in, token);
+                pairs(A.yh, ny);
+                A.on = nx > 0 || ny > 0;
+                if (!in) { err = "ALID histories"; return false; }
+            }
         }
     }
     // ---------------- STATIC_DATA.asc (Code/IO.h:1491-1852) ----------------
@@ -491,6 +562,7 @@ bool load_model_files_impl(const std::string& dir, int save_index, Model& m, std
                     else in >> j >> b.X[i] >> b.Y[i];
                     std::getline(in, token);
                 }
+                if (!rigid) alid_coefficients(b);
             }
             if (starts(line, "BORDERS")) {
                 Body& b = m.bodies.at(index_body);

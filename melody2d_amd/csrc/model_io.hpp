@@ -81,6 +81,16 @@ struct ContactElement {                      // Contact_element (Code/Contact_El
     std::vector<double> internal;
 };
 
+// absorbing layers (ALID section of a body, Code/IO.h:1395-1485): bounds
+// xmin xmax ymin ymax, range, alpha in/out, beta in/out, exponent, and the
+// velocity histories of the x and y directions (empty: that direction off)
+struct Alid {
+    bool on = false;                          // Body::flag_alid
+    double xmin = 0, xmax = 0, ymin = 0, ymax = 0, range = 0;
+    double alphain = 0, alphaout = 0, betain = 0, betaout = 0, exponent = 0;
+    History xh, yh;
+};
+
 struct Body {
     bool rigid = false;
     int material = 0;
@@ -101,6 +111,8 @@ struct Body {
     std::vector<int> cells;                  // 3 per triangle
     std::vector<int> mat_dof;                // 2 per matrix entry (upper triangle)
     std::vector<double> mat_stiffness, mat_damping;
+    Alid alid;
+    std::vector<double> alid_alpha, alid_beta;   // per node (Node::alid_alpha, alid_beta, Code/IO.h:1548-1584)
     // rigid
     double xc = 0, yc = 0, mass = 0, inertia = 0, inverse_mass = 0, inverse_inertia = 0;
     // kinematic state as loaded (deformable: per node; rigid: body level)

@@ -26,7 +26,7 @@ NODE_FIELDS = ["x_current", "y_current", "x_disp", "y_disp", "x_vel", "y_vel", "
                "x_internal_force", "y_internal_force", "x_contact_force", "y_contact_force",
                "x_body_force", "y_body_force", "x_dirichlet_force", "y_dirichlet_force",
                "x_neumann_force", "y_neumann_force", "x_damping_force", "y_damping_force",
-               "x_force", "y_force", "x_mass", "y_mass"]
+               "x_force", "y_force", "x_mass", "y_mass", "x_alid_force", "y_alid_force"]
 
 
 class MelError(RuntimeError):

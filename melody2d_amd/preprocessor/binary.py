@@ -58,6 +58,8 @@ def write_binary(model: Model, path: str):
         _rec(f, "model.law_params", np.array([x for *_, p in m.contact_laws for x in p], dtype=float))
         _rec(f, "model.n_bodies", np.array([len(m.bodies)], dtype=np.int32))
         for i, b in enumerate(m.bodies):
+            if b.alid is not None:
+                raise ValueError("the binary container has no ALID records: write the model as ASCII files")
             p = "b%d." % i
             _rec(f, p + "info", np.array([1 if b.rigid else 0, mats.index(b.material), _PER[b.periodicity],
                                           {"active": 0, "inactive": 1, "nogravity": 2}.get(b.status, 0),

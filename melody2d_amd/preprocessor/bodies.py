@@ -78,6 +78,10 @@ class Body:
     d2d_phix: np.ndarray | None = None
     d2d_phiy: np.ndarray | None = None
     matrix: list = field(default_factory=list)   # [(dof0, dof1, stiffness, damping)] upper triangle
+    # absorbing layers (ALID section, ExportStatic.m:514-550): (xmin, xmax,
+    # ymin, ymax, range, alphain, alphaout, betain, betaout, exponent),
+    # x velocity history [(t, v)], y velocity history [(t, v)]
+    alid: tuple | None = None
     # rigid only
     centre: tuple = (0.0, 0.0)
     rigid_mass: float = 0.0

@@ -781,6 +781,17 @@ def variants():
     top.borders[0].ybc = BC("Oriented", [(0.0, -3e4), (1e3, -3e4)])
     m.flags = [f for f in m.flags if f != "NO_LOG"] + ["NO_LOG"]
     v["tiny_bc"] = m
+    # absorbing layers (ALID) on the grain with the matrix: graded layers
+    # along its lower and left sides, an imposed y velocity
+    import copy
+    m = copy.deepcopy(m)
+    b1 = m.bodies[1]
+    xmin, ymin = b1.X.min(axis=0)
+    xmax, ymax = b1.X.max(axis=0)
+    rng = 0.35 * (ymax - ymin)
+    b1.alid = ((xmin, xmax, ymin, ymax, rng, 0.0, 1e-4, 0.0, 2e4, 2.0),
+               [(0.0, 0.0), (1e3, 0.0)], [(0.0, 0.01), (1e3, 0.01)])
+    v["tiny_alid"] = m
     # self-contact disabled (NO_MONITORING)
     m = tiny(h=0.45e-3)
     m.flags = m.flags + ["NO_MONITORING"]

@@ -177,6 +177,14 @@ def write_static(model: Model, d: str):
             out += _bc_lines("X", br.xbc)
             out += _bc_lines("Y", br.ybc)
             out.append(" ")
+        if b.alid is not None:
+            par, xh, yh = b.alid
+            out += ["ALID", " ".join(g(v) for v in [len(xh), len(yh)] + list(par))]
+            for hist in (xh, yh):
+                if hist:
+                    out.append(" ")
+                    out += ["%s %s" % (g(t), g(v)) for t, v in hist]
+            out.append(" ")
         if not b.rigid:
             out += ["INTEGRATION", "%d %d" % (b.n_gauss, b.nregions), " "]
         out += ["TRIANGULATION", "%d" % len(b.tri), " ", " ", " "]

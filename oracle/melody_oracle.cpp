@@ -19,6 +19,7 @@
 //   Body::Update_body_forces   Code/Body.h:909-929
 //   Body::Update_internal_forces Code/Body.h:937-1049 (+ Code/Model.h:19-73)
 //   Body::Update_damping_forces Code/Body.h:1057-1154
+//   Body::Update_alid_forces   Code/Body.h:1163-1296 (+ Code/IO.h:1395-1584)
 //   Body::Initialize_contact_forces Code/Body.h:1305-1333
 //   Body::Update_contacts      Code/Body.h:1341-1634 (+ Code/Tools.h:106-219, 319-369)
 //   Body::Update_contact_forces Code/Body.h:1642-1941 (+ Code/Model.h:81-335)
@@ -60,7 +61,7 @@ namespace {
 struct State {
     // per node
     std::vector<double> xc, yc, ux, uy, vx, vy, ax, ay, upx, upy, vpx, vpy, apx, apy;
-    std::vector<double> fix, fiy, fcx, fcy, fbx, fby, fdx, fdy, fnx, fny, fkx, fky, fx, fy;
+    std::vector<double> fix, fiy, fcx, fcy, fbx, fby, fdx, fdy, fnx, fny, fkx, fky, fx, fy, fax, fay;
     std::vector<double> regx, regy;           // n_regions x n_nodes
     std::vector<double> mx, my, imx, imy;
     // rigid body
@@ -435,6 +436,42 @@ void update_damping_forces(Sim& S, int bi) {
 // --------------------------------------------------------------------------
 // internal forces (Code/Body.h:937-1049, Code/Model.h:19-73)
 // --------------------------------------------------------------------------
+// Body::Update_alid_forces (Code/Body.h:1163-1296): mass-proportional
+// terms from the physical velocities, then the stiffness-proportional terms
+// of the matrix entries in entry order, relative to the layers' imposed
+// velocity at time t
+void update_alid_forces(Sim& S, int bi, double t) {
+    Body& b = S.m.bodies[bi];
+    State& s = S.s[bi];
+    if (b.rigid || !b.alid.on) return;
+    auto pv = [&](const History& h, bool& on) {
+        on = !h.t.empty();
+        for (size_t n = 0; n + 1 < h.t.size(); n++) {
+            double t0 = h.t[n], p0 = h.v[n], t1 = h.t[n + 1], p1 = h.v[n + 1];
+            if ((t0 <= t) & (t <= t1)) return p0 + (p1 - p0) / (t1 - t0) * (t - t0);
+        }
+        return 0.;
+    };
+    bool onx, ony;
+    const double pvx = pv(b.alid.xh, onx), pvy = pv(b.alid.yh, ony);
+    for (int i = 0; i < b.n_nodes; i++) {
+        const double beta = b.alid_beta[i];
+        s.fax[i] = onx ? -beta * s.mx[i] * (s.vx[i] - pvx) : 0.;
+        s.fay[i] = ony ? -beta * s.my[i] * (s.vy[i] - pvy) : 0.;
+    }
+    for (size_t e = 0; e < b.mat_stiffness.size(); e++) {
+        const int dof0 = b.mat_dof[2 * e], dof1 = b.mat_dof[2 * e + 1];
+        const int n0 = dof0 / 2, r0 = dof0 - n0 * 2, n1 = dof1 / 2, r1 = dof1 - n1 * 2;
+        const double d = b.mat_stiffness[e] / (1. / b.alid_alpha[n0] + 1. / b.alid_alpha[n1]);
+        const double vp0[2] = {s.vpx[n0] - pvx, s.vpy[n0] - pvy}, vp1[2] = {s.vpx[n1] - pvx, s.vpy[n1] - pvy};
+        double* f0 = r0 == 0 ? &s.fax[n0] : &s.fay[n0];
+        double* f1 = r1 == 0 ? &s.fax[n1] : &s.fay[n1];
+        const bool on0 = r0 == 0 ? onx : ony, on1 = r1 == 0 ? onx : ony;
+        if (on0) *f0 += -vp1[r1] * d;
+        if (dof0 != dof1 && on1) *f1 += -vp0[r0] * d;
+    }
+}
+
 void update_internal_forces(Sim& S, int bi, int region) {
     Body& b = S.m.bodies[bi];
     State& s = S.s[bi];
@@ -911,8 +948,8 @@ void sum_newton(Sim& S, int bi) {
                 s.regy[(size_t)r * n + i] = 0.;
             }
             s.fix[i] = ix; s.fiy[i] = iy;
-            s.fx[i] = ix + s.fcx[i] + 0. + s.fbx[i] + s.fdx[i] + s.fnx[i] + s.fkx[i] + 0.;
-            s.fy[i] = iy + s.fcy[i] + 0. + s.fby[i] + s.fdy[i] + s.fny[i] + s.fky[i] + 0.;
+            s.fx[i] = ix + s.fcx[i] + 0. + s.fbx[i] + s.fdx[i] + s.fnx[i] + s.fkx[i] + s.fax[i];
+            s.fy[i] = iy + s.fcy[i] + 0. + s.fby[i] + s.fdy[i] + s.fny[i] + s.fky[i] + s.fay[i];
         }
         // Apply_Newton (Code/Node.h:227-231)
         for (int i = 0; i < n; i++) { s.apx[i] = s.fx[i] * s.imx[i] / s.fsx[i]; s.apy[i] = s.fy[i] * s.imy[i] / s.fsy[i]; }
@@ -949,7 +986,7 @@ void euler_kinematics(Sim& S, int bi, double dt) {
             s.works[3] += dt * (vx * s.fdx[i] + vy * s.fdy[i]);
             s.works[4] += dt * (vx * s.fnx[i] + vy * s.fny[i]);
             s.works[5] += dt * (vx * s.fkx[i] + vy * s.fky[i]);
-            s.works[6] += dt * (vx * 0. + vy * 0.);
+            s.works[6] += dt * (vx * s.fax[i] + vy * s.fay[i]);
             s.vpx[i] = s.vpx[i] + dt * s.apx[i];
             s.vpy[i] = s.vpy[i] + dt * s.apy[i];
             s.upx[i] = s.upx[i] + dt * s.vpx[i];
@@ -1352,7 +1389,7 @@ void setup(Sim& S) {
         int n = b.n_nodes;
         auto z = [&](std::vector<double>& v) { v.assign(n, 0.); };
         for (auto* v : {&s.xc, &s.yc, &s.fix, &s.fiy, &s.fcx, &s.fcy, &s.fbx, &s.fby, &s.fdx, &s.fdy, &s.fnx,
-                        &s.fny, &s.fkx, &s.fky, &s.fx, &s.fy})
+                        &s.fny, &s.fkx, &s.fky, &s.fx, &s.fy, &s.fax, &s.fay})
             z(*v);
         s.ux = b.ux; s.uy = b.uy; s.vx = b.vx; s.vy = b.vy; s.ax = b.ax; s.ay = b.ay;
         s.upx = b.upx; s.upy = b.upy; s.vpx = b.vpx; s.vpy = b.vpy; s.apx = b.apx; s.apy = b.apy;
@@ -1429,6 +1466,7 @@ void euler_step(Sim& S) {
             update_bc_forces(S, i);
             update_body_forces(S, i);
             update_damping_forces(S, i);
+            update_alid_forces(S, i, S.time);
         }
     }
     // contact forces: the reference runs them per body in parallel; the
@@ -1626,6 +1664,7 @@ void solver_half(Sim& S, int half, double dt) {
             update_bc_forces(S, i);
             update_body_forces(S, i);
             update_damping_forces(S, i);
+            update_alid_forces(S, i, half == 1 ? t : t + dt * 0.5);
         }
     }
     for (int i = 0; i < NB; i++)
@@ -1880,6 +1919,7 @@ void dump(Sim& S, const std::string& path) {
         w.f64(p + "x_damping_force", s.fkx); w.f64(p + "y_damping_force", s.fky);
         w.f64(p + "x_force", s.fx); w.f64(p + "y_force", s.fy);
         w.f64(p + "x_mass", s.mx); w.f64(p + "y_mass", s.my);
+        w.f64(p + "x_alid_force", s.fax); w.f64(p + "y_alid_force", s.fay);
         w.f64(p + "rigid", {s.cur[0], s.cur[1], s.cur[2], s.disp[0], s.disp[1], s.disp[2], s.vel[0], s.vel[1],
                             s.vel[2], s.acc[0], s.acc[1], s.acc[2], s.rfc[0], s.rfc[1], s.rfc[2], s.rf[0], s.rf[1],
                             s.rf[2], s.mass, s.inertia, s.rfd[0], s.rfd[1], s.rfd[2],

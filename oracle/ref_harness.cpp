@@ -158,6 +158,8 @@ static void dump_state(const string& path, vector<Body>& Bodies, double Time, in
         NODE_FIELD("y_force", N.y_force)
         NODE_FIELD("x_mass", N.x_mass)
         NODE_FIELD("y_mass", N.y_mass)
+        NODE_FIELD("x_alid_force", N.x_alid_force)
+        NODE_FIELD("y_alid_force", N.y_alid_force)
         w.f64(p + "rigid", {B.x_current, B.y_current, B.r_current, B.x_displacement, B.y_displacement,
                            B.r_displacement, B.x_velocity, B.y_velocity, B.r_velocity, B.x_acceleration,
                            B.y_acceleration, B.r_acceleration, B.x_contact_force, B.y_contact_force,

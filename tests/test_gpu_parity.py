@@ -12,7 +12,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import EULER_VARIANTS, VARIANTS
+from conftest import TWO_LAUNCH_VARIANTS, VARIANTS
 from mel_compare import compare, device_state, layout_from_dump, run_oracle
 from melody2d_amd import native
 
@@ -109,7 +109,7 @@ def test_exact_sum_order_matches_oracle(name, golden_dir, tmp_path, monkeypatch)
         assert not bad, "step %d: %s" % (s, sorted(bad.items(), key=lambda kv: -kv[1])[:8])
 
 
-@pytest.mark.parametrize("name", EULER_VARIANTS)
+@pytest.mark.parametrize("name", TWO_LAUNCH_VARIANTS)
 def test_two_launch_step_equals_launch_sequence(name, golden_dir, tmp_path, monkeypatch):
     """The step in two launches (contact elements beside the internal forces,
     then rigid bodies, node-block finish and BC update) and the launch

@@ -42,7 +42,8 @@ def test_goldens_exercise_the_step():
     seen = {"contact": False, "neumann": False, "dirichlet": False, "damping": False, "czm": False,
             "rigid_rotation": False, "periodic_neighbour": False, "periodic_stencil": False,
             "slave_border_1": False, "master_border_3": False, "boundary_work": False,
-            "fatigue_damage": False, "bond_broken_later": False}
+            "fatigue_damage": False, "bond_broken_later": False, "alid": False, "rejected_attempt": False,
+            "mass_scaled": False}
     for name in VARIANTS:
         with tempfile.TemporaryDirectory() as d:
             _, ref = extract_golden(name, d)
@@ -73,6 +74,12 @@ def test_goldens_exercise_the_step():
                     seen["slave_border_1"] |= bool(np.any(ci[:, 0] == 1))
                     seen["master_border_3"] |= bool(np.any(ci[:, 4] == 3))
                     seen["boundary_work"] |= bool(D[p + "energy"][2] != 0)
+                    if p + "x_alid_force" in D:
+                        seen["alid"] |= bool(np.any(D[p + "x_alid_force"] != 0) and D[p + "works"][6] != 0)
+                    if p + "x_msf" in D:
+                        seen["mass_scaled"] |= bool(np.any(D[p + "x_msf"] > 1))
+                if "neval" in D:
+                    seen["rejected_attempt"] |= bool(D["neval"][0] > 1)
     assert all(seen.values()), seen
 
 
