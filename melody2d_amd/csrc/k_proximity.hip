@@ -233,6 +233,37 @@ constexpr int NB_CAND = 2048;   // candidate neighbours per body before ordering
 
 // neighbour lists (Code/Proximity.h:112-169), one workgroup of NB_THREADS
 // per body1 over the candidates of the cell list
+__device__ inline int body_slots(const DevModel& m, int b) {
+    int g0 = m.border_off[b], g1 = g0 + m.body_nborders[b];
+    return m.br_slot0[g1 - 1] + m.br_nslots[g1 - 1] - m.br_slot0[g0];
+}
+
+__device__ inline bool self_enabled(const DevModel& m, int b) {
+    return m.self_contact && !m.body_rigid_flag[b] && m.body_periodicity[b] != 2;
+}
+
+// rows of body b: non-self rows (borderS, neighbour k, nodeS), then self
+// rows (borderS, nodeS, borderM)
+__device__ inline int body_rows(const DevModel& m, int b) {
+    if (m.body_status[b] == 1 || m.body_nborders[b] == 0) return 0;
+    const int ns = body_slots(m, b);
+    int r = m.nb_count[b] * ns;
+    if (self_enabled(m, b)) r += ns * m.body_nborders[b];
+    return r;
+}
+
+// row chunks per body, and the body's row layout for the counting pass:
+// (rows, slots, neighbours, self | borders << 1) and (first border, first slot)
+__device__ inline void body_row_layout(const DevModel& m, int b) {
+    const int rows = body_rows(m, b);
+    m.body_chunks[b] = (rows + ROW_CHUNK - 1) / ROW_CHUNK;
+    const int g0 = m.border_off[b];
+    const int nbr = m.body_nborders[b];
+    m.body_rowrec[b] = make_int4(rows, nbr > 0 ? body_slots(m, b) : 0, m.nb_count[b],
+                                 (self_enabled(m, b) ? 1 : 0) | (nbr << 1));
+    m.body_gs0[b] = make_int2(g0, nbr > 0 ? m.br_slot0[g0] : 0);
+}
+
 __global__ void __launch_bounds__(NB_THREADS) k_prox_neighbours(DevModel m) {
     const int b1 = blockIdx.x;
     const int t = threadIdx.x;
@@ -338,46 +369,15 @@ __global__ void __launch_bounds__(NB_THREADS) k_prox_neighbours(DevModel m) {
     if (t == 0) {
         if (base > m.nb_cap) atomicOr(&m.status_flags[0], 2);
         m.nb_count[b1] = min(base, m.nb_cap);
+        body_row_layout(m, b1);     // reads the count just written
     }
 }
 
-__device__ inline int body_slots(const DevModel& m, int b) {
-    int g0 = m.border_off[b], g1 = g0 + m.body_nborders[b];
-    return m.br_slot0[g1 - 1] + m.br_nslots[g1 - 1] - m.br_slot0[g0];
-}
-
-__device__ inline bool self_enabled(const DevModel& m, int b) {
-    return m.self_contact && !m.body_rigid_flag[b] && m.body_periodicity[b] != 2;
-}
-
-// rows of body b: non-self rows (borderS, neighbour k, nodeS), then self
-// rows (borderS, nodeS, borderM)
-__device__ inline int body_rows(const DevModel& m, int b) {
-    if (m.body_status[b] == 1 || m.body_nborders[b] == 0) return 0;
-    const int ns = body_slots(m, b);
-    int r = m.nb_count[b] * ns;
-    if (self_enabled(m, b)) r += ns * m.body_nborders[b];
-    return r;
-}
-
-// row chunks per body, and the body's row layout for the counting pass:
-// (rows, slots, neighbours, self | borders << 1) and (first border, first slot)
-__device__ inline void body_row_layout(const DevModel& m, int b) {
-    const int rows = body_rows(m, b);
-    m.body_chunks[b] = (rows + ROW_CHUNK - 1) / ROW_CHUNK;
-    const int g0 = m.border_off[b];
-    const int nbr = m.body_nborders[b];
-    m.body_rowrec[b] = make_int4(rows, nbr > 0 ? body_slots(m, b) : 0, m.nb_count[b],
-                                 (self_enabled(m, b) ? 1 : 0) | (nbr << 1));
-    m.body_gs0[b] = make_int2(g0, nbr > 0 ? m.br_slot0[g0] : 0);
-}
-
-// One workgroup: every body's row layout (body_row_layout), the first row
-// chunk of every body and the body of every row chunk; clears the hit list
+// One workgroup: the first row chunk of every body (the bodies' row layouts
+// come from k_prox_neighbours) and the body of every row chunk; clears the
+// hit list
 __global__ void __launch_bounds__(1024) k_prox_chunk_offsets(DevModel m) {
     __shared__ int s_w[16];
-    for (int b = threadIdx.x; b < m.n_bodies; b += blockDim.x) body_row_layout(m, b);
-    __syncthreads();       // the bodies' chunk counts are in place
     wg_scan_array(m.body_chunks, m.chunk_off, m.n_bodies, s_w);
     __syncthreads();
     for (int b = threadIdx.x; b < m.n_bodies; b += blockDim.x)

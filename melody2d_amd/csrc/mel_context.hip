@@ -940,7 +940,9 @@ void update_proximity(mel_ctx& c) {
     std::swap(c.bufA, c.bufB);
     d.C = c.bufA;
     d.Cold = c.bufB;
-    const int n_count = std::min(persistent, std::max(1, c.max_row_chunks));
+    // one counting workgroup per row chunk up to 16 per CU: the hardware
+    // hands the next chunk to whichever workgroup slot frees first
+    const int n_count = std::min(16 * 256, std::max(1, c.max_row_chunks));
     k_prox_count<<<n_count + 64, ROW_CHUNK, 0, s>>>(d, n_count);
     k_prox_elem_offsets<<<1, 1024, 0, s>>>(d);
     k_prox_write<<<persistent, BLOCK, 0, s>>>(d);
