@@ -104,12 +104,16 @@ mel_status mel_autotune(const char* binary_path, mel_precision precision, int hi
                         mel_options* best, double* best_ms_per_step, int* n_tried);
 
 /* Update_proximity (Code/Proximity.h:48-473): neighbour lists, contact
- * candidates with history transfer, self-contact unless NO_MONITORING. */
+ * candidates with history transfer, self-contact unless NO_MONITORING.
+ * (The DEACTIVATION rules run only at the stepping loop's contact updates,
+ * as Code/main.cpp:191-196 calls Update_status.) */
 mel_status mel_update_proximity(mel_ctx* ctx);
 
 /* nsteps x [Euler_step (Code/Solver.h:10-88) + the contact-update cadence of
- * Code/main.cpp:191-196].  Asynchronous with respect to the host except at
- * contact updates; returns MEL_E_NAN / MEL_E_CAPACITY when detected. */
+ * Code/main.cpp:191-196: Update_proximity, then Update_status
+ * (Code/Proximity.h:550-1007) when the model has DEACTIVATION rules].
+ * Asynchronous with respect to the host except at contact updates; returns
+ * MEL_E_NAN / MEL_E_CAPACITY when detected. */
 mel_status mel_euler_steps(mel_ctx* ctx, int nsteps);
 
 /* wait for queued device work; reports deferred errors */
@@ -199,6 +203,13 @@ mel_status mel_get_contact_internals(mel_ctx* ctx, int body, double* out3, size_
 
 /* neighbour triples (body2 border2 shift) of body (Code/Proximity.h:160) */
 mel_status mel_get_neighbours(mel_ctx* ctx, int body, int32_t* out, size_t cap, size_t* count);
+
+/* Body::status as a mel status code (0 active, 1 inactive, 2 nogravity;
+ * Code/Body.h:46, set to inactive by Update_status, Code/Proximity.h:550-1007)
+ * and Body::damage (Code/Body.h:110, formed by Body::Update_damage,
+ * Code/Body.h:2667-2700, for the DEACTIVATION rules that read it); either
+ * pointer may be null */
+mel_status mel_get_body_status(mel_ctx* ctx, int body, int* status, double* damage);
 
 /* 26 body-level values (rigid kinematics and resultants; mass, inertia) and
  * the 7 work accumulators internal contact body dirichlet neumann damping alid

@@ -21,7 +21,7 @@ ORACLE = os.path.join(ROOT, "oracle")
 REFERENCE = "/root/reference"
 
 HIP_SOURCES = ["mel_context.hip", "k_step.hip", "k_block.hip", "k_fused.hip", "k_proximity.hip", "k_rigid_lists.hip",
-               "k_adaptive.hip"]
+               "k_adaptive.hip", "k_status.hip"]
 CXX_SOURCES = ["model_io.cpp", "mel_serve.cpp"]
 
 HIPCC_FLAGS = [

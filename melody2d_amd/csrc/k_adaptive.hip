@@ -169,8 +169,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_ad_gather(DevModel m, AdArray
     int q = (blockIdx.x - m.n_rigid) * blockDim.x + threadIdx.x;
     if (q < n_split) {
         int bn = split_bn[q];
-        bucket_sum_node(m, bn);
         int b = m.node_body[m.bn_node[bn]];
+        bucket_gather(m, bn, b);
         if (!m.body_rigid_flag[b] && m.body_status[b] != 1) bn_lengths(m, bn);
     }
 }
@@ -187,11 +187,14 @@ __device__ void node_block_half(const DevModel& m, const BlockArrays& B, const A
     __shared__ double red[BLOCK_THREADS / 64][6];
     const int t = threadIdx.x;
     const int body = B.blk_body[blk];
-    if (m.body_status[body] == 1) return;
     const int nown = B.blk_nown[blk];
     const int* own = B.own_nodes + B.blk_own_off[blk];
     const int* bns = B.bn_list + B.blk_bn_off[blk];
     const int nbn = B.blk_bn_n[blk];
+    if (m.body_status[body] == 1) {
+        for (int i = t; i < nbn; i += BLOCK_THREADS) bucket_drain_node(m, bns[i]);
+        return;
+    }
     for (int i = t; i < nbn; i += BLOCK_THREADS) bucket_sum_node(m, bns[i]);
     __syncthreads();
     double w[6] = {0., 0., 0., 0., 0., 0.};   // internal body dirichlet neumann damping alid

@@ -70,24 +70,8 @@ __global__ void k_kinematics_list(DevModel m, const int* nodes, int n) {
 __global__ void k_node_outputs(DevModel m, int with_body_forces) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m.n_nodes) return;
-    int b = m.node_body[j];
-    if (m.body_rigid_flag[b]) return;
-    double cx = 0., cy = 0., dx = 0., dy = 0., nx = 0., ny = 0.;
-    for (int e = m.bt_ptr[j]; e < m.bt_ptr[j + 1]; e++) {
-        int bn = m.bt_bn[e];
-        double f = m.bt_phi[e];
-        cx += m.bn_pcx[bn] * f; cy += m.bn_pcy[bn] * f;
-        dx += m.bn_pdx[bn] * f; dy += m.bn_pdy[bn] * f;
-        nx += m.bn_pnx[bn] * f; ny += m.bn_pny[bn] * f;
-    }
-    double gx = m.gravity_x, gy = m.gravity_y;
-    if (m.body_status[b] == 2 || !with_body_forces) { gx = 0.; gy = 0.; }
-    double bx = m.mx[j] * gx, by = m.my[j] * gy;
-    m.fcx[j] = cx; m.fcy[j] = cy; m.fdx[j] = dx; m.fdy[j] = dy; m.fnx[j] = nx; m.fny[j] = ny;
-    m.fbx[j] = bx; m.fby[j] = by;
-    const double lx = m.alid_on ? m.fax[j] : 0., ly = m.alid_on ? m.fay[j] : 0.;
-    m.fx[j] = m.fix[j] + cx + 0. + bx + dx + nx + m.fkx[j] + lx;
-    m.fy[j] = m.fiy[j] + cy + 0. + by + dy + ny + m.fky[j] + ly;
+    if (m.body_rigid_flag[m.node_body[j]]) return;
+    node_outputs(m, j, with_body_forces);
 }
 
 }  // namespace mel

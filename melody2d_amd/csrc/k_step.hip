@@ -31,6 +31,7 @@ __global__ void k_bc_eval(DevModel m, int plus_dt) {
 __global__ void k_damping(DevModel m) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m.n_nodes) return;
+    if (m.body_status[m.node_body[j]] == 1) return;   // inactive body: its damping forces stay
     double fx = 0., fy = 0.;
     for (int q = m.mt_ptr[j]; q < m.mt_ptr[j + 1]; q++) {
         int code = m.mt_entry[q];
@@ -137,8 +138,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_step_gather(DevModel m, const
     int q = (blockIdx.x - m.n_rigid) * blockDim.x + threadIdx.x;
     if (q < n_split) {
         int bn = split_bn[q];
-        bucket_sum_node(m, bn);
         int b = m.node_body[m.bn_node[bn]];
+        bucket_gather(m, bn, b);
         if (!m.body_rigid_flag[b] && m.body_status[b] != 1) bn_lengths(m, bn);
     }
 }
@@ -260,7 +261,10 @@ k_eval_items(DevModel m, const int* bc_bn, int n_bc, int do_contacts, int do_bc)
 
 __global__ void k_bucket_sums(DevModel m) {
     int bn = blockIdx.x * blockDim.x + threadIdx.x;
-    if (bn < m.n_bnodes) bucket_sum_node(m, bn);
+    if (bn >= m.n_bnodes) return;
+    // an inactive body's nodes keep their contact forces (emptied bucket)
+    if (m.body_status[m.node_body[m.bn_node[bn]]] == 1) bucket_total(m, bn);
+    else bucket_sum_node(m, bn);
 }
 
 __global__ void k_clear_point_forces(DevModel m, int contacts, int bcs) {

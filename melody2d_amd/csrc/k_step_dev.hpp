@@ -395,7 +395,8 @@ __device__ __forceinline__ void contact_item(const DevModel& m, int e) {
 }
 
 // point contact force per border node: records summed in record-id order
-__device__ inline double2 bucket_sum_node(const DevModel& m, int bn) {
+// (the bucket is emptied)
+__device__ inline double2 bucket_total(const DevModel& m, int bn) {
     // the first BUCKET_FAST records are loaded together with the count (the
     // bucket's storage is always there; entries past the count are ignored)
     constexpr int BUCKET_FAST = 4;
@@ -432,10 +433,32 @@ __device__ inline double2 bucket_sum_node(const DevModel& m, int bn) {
         }
         for (int k = 0; k < n; k++) { px += m.rec_fx[ids[k]]; py += m.rec_fy[ids[k]]; }
     }
-    m.bn_pcx[bn] = px;
-    m.bn_pcy[bn] = py;
     m.bk_cursor[bn] = 0;
     return make_double2(px, py);
+}
+
+__device__ inline double2 bucket_sum_node(const DevModel& m, int bn) {
+    const double2 p = bucket_total(m, bn);
+    m.bn_pcx[bn] = p.x;
+    m.bn_pcy[bn] = p.y;
+    return p;
+}
+
+// a border node of an inactive body: what it received is added to what it
+// received before (Send_contact_forces still reaches it, nothing resets it)
+// and its point force of the last active step is kept
+__device__ inline void bucket_drain_node(const DevModel& m, int bn) {
+    const double2 p = bucket_total(m, bn);
+    if (m.bn_acx) {
+        m.bn_acx[bn] += p.x;
+        m.bn_acy[bn] += p.y;
+    }
+}
+
+// the contact point forces of a border node of body b
+__device__ inline void bucket_gather(const DevModel& m, int bn, int b) {
+    if (m.body_status[b] == 1) bucket_drain_node(m, bn);
+    else bucket_sum_node(m, bn);
 }
 
 // ---------------------------------------------------------------------------
@@ -626,7 +649,15 @@ __device__ inline RigidForces rigid_forces_block(const DevModel& m, int rigid_in
         const double* r = m.rig_rec + 6 * mlist[q];
         sx += r[3]; sy += r[4]; sm += r[5];
     }
-    if (status == 1) return F;     // inactive body (the same for the whole workgroup)
+    if (m.n_rules > 0) {
+        // the last sends of inactive bodies that named this body as master
+        const int ng = min(m.gh_count[1], m.ghost_cap);
+        for (int q = tid; q < ng; q += nt)
+            if (m.gh_rb[q] == b) { sx += m.gh_rf[3 * q]; sy += m.gh_rf[3 * q + 1]; sm += m.gh_rf[3 * q + 2]; }
+    }
+    // an inactive body only adds what it received to its contact force
+    // (the same for the whole workgroup)
+    if (status == 1 && m.n_rules == 0) return F;
     red[3 * tid] = sx; red[3 * tid + 1] = sy; red[3 * tid + 2] = sm;
     __syncthreads();
     for (int w = nt / 2; w > 0; w >>= 1) {
@@ -634,6 +665,13 @@ __device__ inline RigidForces rigid_forces_block(const DevModel& m, int rigid_in
         __syncthreads();
     }
     if (tid == 0) { F.fcon[0] = red[0]; F.fcon[1] = red[1]; F.fcon[2] = red[2]; }
+    if (status == 1) {
+        if (tid == 0) {
+            double* Rg = m.body_rigid + b * RIGID_FIELDS;
+            for (int q = 0; q < 3; q++) Rg[R_FCON + q] += F.fcon[q];
+        }
+        return F;
+    }
     // boundary conditions: every thread adds the terms of a contiguous run of
     // slots, then the runs are added in slot order (a two-level version of
     // the reference's serial loop)
@@ -922,11 +960,14 @@ __device__ inline void node_block_finish(const DevModel& m, const BlockArrays& B
     __shared__ double red[BLOCK_THREADS / 64][6];
     const int t = threadIdx.x;
     const int body = B.blk_body[blk];
-    if (m.body_status[body] == 1) return;   // inactive body: no update (uniform over the block)
     const int nown = B.blk_nown[blk];
     const int* own = B.own_nodes + B.blk_own_off[blk];
     const int* bns = B.bn_list + B.blk_bn_off[blk];
     const int nbn = B.blk_bn_n[blk];        // 0 for blocks of split bodies
+    if (m.body_status[body] == 1) {         // inactive body: no update (uniform over the block)
+        for (int i = t; i < nbn; i += BLOCK_THREADS) bucket_drain_node(m, bns[i]);
+        return;
+    }
     for (int i = t; i < nbn; i += BLOCK_THREADS) bucket_sum_node(m, bns[i]);
     __syncthreads();
     double w[6] = {0., 0., 0., 0., 0., 0.};   // internal body dirichlet neumann damping alid

@@ -19,6 +19,7 @@
 #include "k_block.hpp"
 #include "k_fused.hpp"
 #include "k_adaptive.hpp"
+#include "k_status.hpp"
 #include "k_rigid_lists.hpp"
 #include "k_step.hpp"
 #include "mel_device.hpp"
@@ -396,6 +397,7 @@ void raise_flags(const int* f) {
         if (f[0] & 4) what += " history index";
         if (f[0] & 8) what += " contact elements";
         if (f[0] & 32) what += " rigid-body element list";
+        if (f[0] & 64) what += " inactive bodies' contact sends";
         throw CapacityError{"device capacity exceeded:" + what};
     }
     if (f[1]) throw NanError{"non-finite displacement parameters (the run is no longer meaningful)"};
@@ -857,8 +859,12 @@ void build_blocks(mel_ctx& c, const std::vector<int>& g_ptr, const std::vector<i
     const char* fe = getenv("MELODY2D_FUSED_STEP");
     const bool fused_off = c.opts.fused_step >= 0 ? c.opts.fused_step == 0 : (fe && fe[0] == '0');
     // (the adaptive schemes run their half steps as the launch sequence, and
-    // so do models with absorbing layers)
-    c.two_launch = B.lds_finish_ok && !fused_off && c.model.scheme == SCH_EULER && !c.d.alid_on;
+    // so do models with absorbing layers, deactivation rules or inactive
+    // bodies)
+    bool any_inactive = false;
+    for (const Body& b : c.model.bodies) any_inactive = any_inactive || b.status == ST_INACTIVE;
+    c.two_launch = B.lds_finish_ok && !fused_off && c.model.scheme == SCH_EULER && !c.d.alid_on &&
+                   c.d.n_rules == 0 && !any_inactive;
     // the launch sequence takes the wide-window kernel when Gauss points
     // with more support entries than the default window are common (RPIM
     // supports of 30 nodes; a few long MLS supports read their tail from
@@ -964,6 +970,13 @@ void initialize_czm(mel_ctx& c) {
     k_init_czm<<<blocks(c.n_contacts), BLOCK, 0, c.stream>>>(c.d);
 }
 
+// the contact update of the stepping loop (Code/main.cpp:191-196):
+// Update_proximity, then Update_status when the model has deactivation rules
+void contact_update(mel_ctx& c) {
+    update_proximity(c);
+    if (c.d.n_rules > 0) k_update_status<<<1, STATUS_THREADS, 0, c.stream>>>(c.d, c.blocks_dev);
+}
+
 // the launches of one step in two launches (the device advances its time)
 void launch_step(mel_ctx& c, int n_cwg) {
     DevModel& d = c.d;
@@ -1046,6 +1059,8 @@ void euler_step(mel_ctx& c) {
     // Initialize_contact_forces, Update_contacts, Update_contact_forces
     // (records into buckets) | Update_bc_forces
     TIMED(c, KT_CONTACTS, a, (k_step_contacts<<<n_cwg, BLOCK_THREADS, 0, a>>>(d, B.bc_bn, B.n_bc_bn)));
+    // Send_contact_forces of the inactive bodies
+    if (d.n_rules > 0) k_ghost_sends<<<64, BLOCK, 0, a>>>(d);
     // Update_damping_forces, Update_alid_forces(Time)
     if (d.n_matrix > 0) TIMED(c, KT_DAMPING, a, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, a>>>(d)));
     if (d.n_alid_nodes > 0) k_alid<<<blocks(d.n_alid_nodes), BLOCK, 0, a>>>(d, c.time);
@@ -1242,11 +1257,10 @@ void build(mel_ctx& c) {
     d.alid_on = 0;
     for (const Body& B : m.bodies)
         if (!B.rigid && B.alid.on) d.alid_on = 1;
+    d.n_rules = (int)m.deactivated.size();
     d.law_f32 = c.precision == MEL_FP32_MIXED ? 1 : 0;
-    for (auto& b : m.bodies) {
-        if (b.status == ST_INACTIVE) throw CapacityError{"inactive bodies are not supported"};
+    for (auto& b : m.bodies)
         if (b.borders.empty()) throw CapacityError{"every body needs at least one border"};
-    }
     for (auto& b : m.bodies)
         if (m.flags[F_UPDATE_STIFFNESS] && !b.mat_dof.empty())
             throw CapacityError{"UPDATE_STIFFNESS_MATRIX is not supported"};
@@ -1720,10 +1734,33 @@ void build(mel_ctx& c) {
     d.C = c.bufA;
     d.Cold = c.bufB;
     upload_loaded_contacts(c);
-    d.rec_bn = c.alloc<int>(3 * (size_t)d.c_cap);
-    d.rec_fx = c.alloc<double>(3 * (size_t)d.c_cap);
-    d.rec_fy = c.alloc<double>(3 * (size_t)d.c_cap);
+    // deactivation rules: the ghost records of inactive bodies take bucket
+    // record ids past the elements' (at most two master records per
+    // element of the bodies they stop)
+    d.ghost_cap = d.n_rules > 0 ? 2 * d.c_cap : 0;
+    d.rec_bn = c.alloc<int>(3 * (size_t)d.c_cap + d.ghost_cap);
+    HIP_OK(hipMemsetAsync(d.rec_bn, 0xff, (3 * (size_t)d.c_cap + d.ghost_cap) * sizeof(int), c.stream));
+    d.rec_fx = c.alloc<double>(3 * (size_t)d.c_cap + d.ghost_cap);
+    d.rec_fy = c.alloc<double>(3 * (size_t)d.c_cap + d.ghost_cap);
     d.rig_rec = c.alloc<double>(6 * (size_t)d.c_cap);
+    if (d.n_rules > 0) {
+        std::vector<double> rules((size_t)RULE_STRIDE * d.n_rules, 0.);
+        for (int i = 0; i < d.n_rules; i++)
+            for (size_t k = 0; k < m.deactivated[i].size() && k < (size_t)RULE_STRIDE; k++)
+                rules[(size_t)RULE_STRIDE * i + k] = m.deactivated[i][k];
+        d.rules = c.upload(rules);
+        d.body_damage = c.alloc<double>(NB);
+        d.body_flip = c.alloc<int>(NB);
+        d.slot_damage = c.alloc<double>(NS);
+        d.gh_count = c.alloc<int>(2);
+        d.gh_bn = c.alloc<int>(d.ghost_cap);
+        d.gh_fx = c.alloc<double>(d.ghost_cap);
+        d.gh_fy = c.alloc<double>(d.ghost_cap);
+        d.gh_rb = c.alloc<int>(d.ghost_cap);
+        d.gh_rf = c.alloc<double>(3 * (size_t)d.ghost_cap);
+        d.bn_acx = c.alloc<double>(d.n_bnodes);
+        d.bn_acy = c.alloc<double>(d.n_bnodes);
+    }
     // proximity scratch
     d.pb_xmin = c.alloc<double>(NBR); d.pb_xmax = c.alloc<double>(NBR);
     d.pb_ymin = c.alloc<double>(NBR); d.pb_ymax = c.alloc<double>(NBR);
@@ -2122,6 +2159,7 @@ void solver_half(mel_ctx& c, int half, double dt, double t) {
     HIP_OK(hipEventRecord(c.ev_fork, s));
     HIP_OK(hipStreamWaitEvent(a, c.ev_fork, 0));
     TIMED(c, KT_CONTACTS, a, (k_step_contacts<<<n_cwg, BLOCK_THREADS, 0, a>>>(dh, B.bc_bn, B.n_bc_bn)));
+    if (d.n_rules > 0) k_ghost_sends<<<64, BLOCK, 0, a>>>(d);
     if (d.n_matrix > 0) TIMED(c, KT_DAMPING, a, (k_damping<<<blocks(d.n_nodes), BLOCK, 0, a>>>(d)));
     // Update_alid_forces(Time) in the first half, (Time + dt/2) in the second
     if (d.n_alid_nodes > 0) k_alid<<<blocks(d.n_alid_nodes), BLOCK, 0, a>>>(d, half == 1 ? t : t + dt * 0.5);
@@ -2233,6 +2271,21 @@ std::vector<StateArray> state_arrays(mel_ctx& c, int nc) {
     if (d.n_rigid > 0) v.push_back({d.rigid_dv, (size_t)d.n_rigid * 8 * sizeof(double)});
     if (c.blocks_dev.n_blocks > 0) v.push_back({c.blocks_dev.blk_work, 14 * (size_t)c.blocks_dev.n_blocks * sizeof(double)});
     v.push_back({d.time, sizeof(double)});
+    if (d.n_rules > 0) {
+        // statuses, damage, the inactive bodies' ghost sends and what their
+        // border nodes received, the node blocks' flags
+        v.push_back({d.body_status, NB * sizeof(int)});
+        v.push_back({d.body_damage, NB * sizeof(double)});
+        v.push_back({d.gh_count, 2 * sizeof(int)});
+        v.push_back({d.gh_bn, (size_t)d.ghost_cap * sizeof(int)});
+        v.push_back({d.gh_fx, (size_t)d.ghost_cap * sizeof(double)});
+        v.push_back({d.gh_fy, (size_t)d.ghost_cap * sizeof(double)});
+        v.push_back({d.gh_rb, (size_t)d.ghost_cap * sizeof(int)});
+        v.push_back({d.gh_rf, 3 * (size_t)d.ghost_cap * sizeof(double)});
+        v.push_back({d.bn_acx, NBN * sizeof(double)});
+        v.push_back({d.bn_acy, NBN * sizeof(double)});
+        if (c.blocks_dev.n_blocks > 0) v.push_back({c.blocks_dev.blk_hdr, 4 * (size_t)c.blocks_dev.n_blocks * sizeof(int4)});
+    }
     if (c.model.scheme != SCH_EULER) {
         // mass-scaling factors and the bodies' adaptive scalars
         v.push_back({c.ad.fsx, N * sizeof(double)});
@@ -2419,10 +2472,10 @@ mel_status mel_euler_steps(mel_ctx* c, int nsteps) {
                     c->next_contact_update = c->next_contact_update + c->model.contact_update_period;
                     if (c->timing && (c->timing_mask >> KT_PROXIMITY & 1u)) {
                         int ev = c->ev_begin(KT_PROXIMITY, c->stream);
-                        update_proximity(*c);
+                        contact_update(*c);
                         c->ev_end(ev, c->stream);
                     } else {
-                        update_proximity(*c);
+                        contact_update(*c);
                     }
                 }
             }
@@ -2437,10 +2490,10 @@ mel_status mel_euler_steps(mel_ctx* c, int nsteps) {
                 c->next_contact_update = c->next_contact_update + c->model.contact_update_period;
                 if (c->timing && (c->timing_mask >> KT_PROXIMITY & 1u)) {
                     int ev = c->ev_begin(KT_PROXIMITY, c->stream);
-                    update_proximity(*c);
+                    contact_update(*c);
                     c->ev_end(ev, c->stream);
                 } else {
-                    update_proximity(*c);
+                    contact_update(*c);
                 }
             }
         }
@@ -2459,7 +2512,7 @@ mel_status mel_solver_steps(mel_ctx* c, int nsteps) {
             // contact-update cadence with the new time step, Code/main.cpp:191-196
             if (c->time > c->next_contact_update - 1.e-6 * c->d.dt) {
                 c->next_contact_update = c->next_contact_update + c->model.contact_update_period;
-                update_proximity(*c);
+                contact_update(*c);
             }
             if (c->nan_stop) {
                 c->err = "a body's total error is NaN (the reference stops the run, Code/main.cpp:227-237)";
@@ -2730,6 +2783,24 @@ mel_status mel_get_neighbours(mel_ctx* c, int body, int32_t* out, size_t cap, si
     });
 }
 
+mel_status mel_get_body_status(mel_ctx* c, int body, int* status, double* damage) {
+    if (!c || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        std::vector<int> st;
+        c->download(st, c->d.body_status + body, 1);
+        if (status) *status = st[0];
+        if (damage) {
+            *damage = 0.;
+            if (c->d.n_rules > 0) {
+                std::vector<double> dm;
+                c->download(dm, c->d.body_damage + body, 1);
+                *damage = dm[0];
+            }
+        }
+        return MEL_OK;
+    });
+}
+
 mel_status mel_get_body_state(mel_ctx* c, int body, double* rigid26, double* works7) {
     if (!c || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
     return guarded(c, [&]() {
@@ -2982,6 +3053,12 @@ mel_status mel_write_dynamic(mel_ctx* c, const char* path) {
         m.number_iteration = c->iteration;
         m.next_contact_update = c->next_contact_update;
         m.dt = c->d.dt;     // the adaptive schemes' current time step (Write_dynamic, Code/IO.h:2396)
+        if (c->d.n_rules > 0) {
+            // statuses the deactivation rules changed (Write_dynamic writes them, Code/IO.h:2429, 2481)
+            std::vector<int> st;
+            c->download(st, c->d.body_status, c->d.n_bodies);
+            for (int b = 0; b < c->d.n_bodies; b++) m.bodies[b].status = st[b];
+        }
         for (int b = 0; b < c->d.n_bodies; b++) {
             Body& B = m.bodies[b];
             size_t n = B.n_nodes;

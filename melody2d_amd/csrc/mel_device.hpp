@@ -194,7 +194,28 @@ struct DevModel {
     int *body_alid;                          // per body: x history [h0, h1), y history [h2, h3) in hist_t/hist_v
     int *alid_nodes, n_alid_nodes;           // the nodes of the bodies with absorbing layers
     double *m_stiff;                         // per matrix entry: its stiffness (ALID models)
+    // deactivation (Update_status, Code/Proximity.h:550-1007)
+    int n_rules;                             // DEACTIVATION rules, RULE_STRIDE entries each as read
+    const double* rules;
+    double* body_damage;                     // Body::damage (Code/Body.h:2667-2700)
+    int* body_flip;                          // per body: deactivated by the last evaluation
+    double* slot_damage;                     // per slot: Update_damage's per-border-node value
+    // An inactive body's last contact sends: Update_contact_forces and
+    // Initialize_contact_forces skip it while Send_contact_forces does not
+    // (Code/Solver.h:36-55), so its last master point forces are added again
+    // every step.  Border-node records take bucket ids past the elements'.
+    int ghost_cap;
+    int* gh_count;                           // [0] border-node records, [1] rigid-master records
+    int* gh_bn;                              // border node
+    double *gh_fx, *gh_fy;
+    int* gh_rb;                              // rigid master body
+    double* gh_rf;                           // its (fx, fy, moment), 3 each
+    // contact point forces an inactive body's border nodes received since it
+    // stopped (its node contact forces are no longer reset, Code/Body.h:1305)
+    double *bn_acx, *bn_acy;
 };
+
+constexpr int RULE_STRIDE = 8;               // entries per DEACTIVATION rule on the device
 
 // --------------------------------------------------------------------------
 // geometry: Closest_2_segment (Linear branch, Code/Tools.h:106-219) and
@@ -517,6 +538,34 @@ __device__ inline void kinematics_node(const DevModel& m, int i) {
     m.ux[i] = dx; m.uy[i] = dy; m.vx[i] = vx; m.vy[i] = vy; m.ax[i] = ax; m.ay[i] = ay;
     m.xc[i] = m.X[i] + dx;
     m.yc[i] = m.Y[i] + dy;
+}
+
+// force fields of a deformable node rebuilt from the last step's
+// components (same expression as the update): Node::x_force and its parts
+// (Code/Node.h:212-225); body forces m g (Body::Update_body_forces) unless
+// with_body_forces is 0.  An inactive body's nodes keep their last total
+// while their contact force goes on receiving sends (bn_acx, zero for
+// active bodies).
+__device__ inline void node_outputs(const DevModel& m, int j, int with_body_forces) {
+    const int b = m.node_body[j];
+    double cx = 0., cy = 0., dx = 0., dy = 0., nx = 0., ny = 0., rx = 0., ry = 0.;
+    for (int e = m.bt_ptr[j]; e < m.bt_ptr[j + 1]; e++) {
+        int bn = m.bt_bn[e];
+        double f = m.bt_phi[e];
+        cx += m.bn_pcx[bn] * f; cy += m.bn_pcy[bn] * f;
+        dx += m.bn_pdx[bn] * f; dy += m.bn_pdy[bn] * f;
+        nx += m.bn_pnx[bn] * f; ny += m.bn_pny[bn] * f;
+        if (m.bn_acx) { rx += m.bn_acx[bn] * f; ry += m.bn_acy[bn] * f; }
+    }
+    double gx = m.gravity_x, gy = m.gravity_y;
+    if (m.body_status[b] == 2 || !with_body_forces) { gx = 0.; gy = 0.; }
+    double bx = m.mx[j] * gx, by = m.my[j] * gy;
+    m.fcx[j] = m.bn_acx ? cx + rx : cx; m.fcy[j] = m.bn_acy ? cy + ry : cy;
+    m.fdx[j] = dx; m.fdy[j] = dy; m.fnx[j] = nx; m.fny[j] = ny;
+    m.fbx[j] = bx; m.fby[j] = by;
+    const double lx = m.alid_on ? m.fax[j] : 0., ly = m.alid_on ? m.fay[j] : 0.;
+    m.fx[j] = m.fix[j] + cx + 0. + bx + dx + nx + m.fkx[j] + lx;
+    m.fy[j] = m.fiy[j] + cy + 0. + by + dy + ny + m.fky[j] + ly;
 }
 
 }  // namespace mel

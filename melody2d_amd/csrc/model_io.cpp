@@ -206,6 +206,7 @@ bool validate_model(const Model& m, std::string& err) {
         }
     }
     const int NB = (int)m.bodies.size();
+    if (!check_deactivation(m, err)) return false;
     for (size_t bi = 0; bi < m.bodies.size(); bi++) {
         const Body& b = m.bodies[bi];
         for (size_t k = 0; k + 2 < b.neighbours.size(); k += 3) {
@@ -235,6 +236,175 @@ bool validate_model(const Model& m, std::string& err) {
 }
 
 }  // namespace
+
+// Read_spy_line (Code/IO.h:11-590).  Every quantity is a keyword, mostly a
+// component keyword, then its indices; the entries are type, component,
+// indices.  Energy and Work read the index before testing the component;
+// the Polar form of MassScaling yields type 11 as the reference writes it.
+bool read_spy_line(std::istream& in, std::vector<double>& out) {
+    auto index_of = [](const std::string& k, std::initializer_list<const char*> names) {
+        int i = 0;
+        for (const char* n : names) {
+            if (k == n) return i;
+            i++;
+        }
+        return -1;
+    };
+    std::string kw, sub;
+    double b = 0, bo = 0, n = 0;
+    in >> kw;
+    if (kw == "Position") {
+        in >> sub;
+        const int c = index_of(sub, {"X", "Y", "R", "Length", "LengthInContact", "Area", "AllX", "AllY"});
+        if (c < 0) return false;
+        if (c <= 2) { in >> b >> n; out = {0., (double)c, b, n}; }
+        else if (c <= 5) { in >> b; out = {0., (double)c, b}; }
+        else out = {0., (double)c};
+        return true;
+    }
+    const int kin = index_of(kw, {"Displacement", "Velocity", "Acceleration"});
+    if (kin >= 0) {
+        in >> sub;
+        const int c = index_of(sub, {"X", "Y", "Mag", "R"});
+        if (c < 0) return false;
+        in >> b >> n;
+        out = {(double)(kin + 1), (double)c, b, n};
+        return true;
+    }
+    if (kw == "Force") {
+        in >> sub;
+        const int c = index_of(sub, {"X", "Y", "Mag", "R"});
+        if (c < 0) return false;
+        in >> sub;
+        const int k = index_of(sub, {"Total", "Internal", "Contact", "Body", "Dirichlet", "Neumann", "Damping", "Alid"});
+        if (k < 0) return false;
+        in >> b >> n;
+        out = {4., (double)c, (double)k, b, n};
+        return true;
+    }
+    if (kw == "Jacobian") {
+        in >> b >> n;
+        out = {5., b, n};
+        return true;
+    }
+    if (kw == "Stress") {
+        in >> sub;
+        const int c = index_of(sub, {"XX", "YY", "XY", "ZZ", "Tresca", "VM", "Major", "Intermediate", "Minor",
+                                     "Spherical"});
+        if (c < 0) return false;
+        in >> b >> n;
+        out = {6., (double)c, b, n};
+        return true;
+    }
+    if (kw == "Contact") {
+        in >> sub;
+        const int c = index_of(sub, {"Gapn", "Gapt", "Xsi", "Xnorm", "Ynorm", "Damage", "Fx", "Fy", "Length", "Master",
+                                     "Px", "Py", "Slave", "Xslave", "Yslave"});
+        if (c < 0) return false;
+        in >> b >> bo >> n;
+        out = {7., (double)c, b, bo, n};
+        return true;
+    }
+    if (kw == "Damage") {
+        in >> b;
+        out = {8., b};
+        return true;
+    }
+    if (kw == "Energy" || kw == "Work") {
+        in >> sub >> b;
+        const int c = kw == "Energy" ? index_of(sub, {"Kinetic", "Deformation"})
+                                     : index_of(sub, {"Internal", "Contact", "Body", "Dirichlet", "Neumann", "Damping",
+                                                      "Alid"});
+        if (c < 0) return false;
+        out = {kw == "Energy" ? 9. : 10., (double)c, b};
+        return true;
+    }
+    if (kw == "Error") {
+        in >> sub;
+        const int c = index_of(sub, {"X", "Y", "Mag", "Norm", "Max"});
+        if (c < 0) return false;
+        in >> b >> n;
+        out = {11., (double)c, b, n};
+        return true;
+    }
+    if (kw == "MassScaling") {
+        in >> sub;
+        if (sub == "Polar") {
+            double x0, x1, y0, y1;
+            in >> sub >> n >> x0 >> x1 >> y0 >> y1;
+            if (sub != "Contact_Normal") return false;
+            out = {11., 0., n, x0, x1, y0, y1};
+            return true;
+        }
+        const int c = index_of(sub, {"X", "Y", "Moy", "dX", "dY", "dMoy", "Xmass", "Ymass", "Mass"});
+        if (c < 0) return false;
+        in >> b >> n;
+        out = {12., (double)c, b, n};
+        return true;
+    }
+    return false;
+}
+
+// Update_status (Code/Proximity.h:550-1007) acts on types 0-4 and 8 and
+// reads, by position: type 0-3 [1] component, [2] body, [3] node (< 0: the
+// body), [4] comparison, [5] threshold; type 4 [1] component, [2] force
+// kind, [3] body, [4] node, [5] comparison, [6] threshold; type 8 [1] body,
+// [2] comparison, [3] threshold.  Its body-level force branch tests the
+// type of body [2] (the force kind) and loops over that body's node count
+// while reading the nodes of body [3]; both must exist for the loop to stay
+// in range.  Position components 3-7 carry fewer entries than the branch
+// reads.  Other types are ignored by the reference and kept as read.
+bool check_deactivation(const Model& m, std::string& err) {
+    const int NB = (int)m.bodies.size();
+    for (size_t i = 0; i < m.deactivated.size(); i++) {
+        const std::vector<double>& r = m.deactivated[i];
+        const std::string where = "DEACTIVATION rule " + std::to_string(i) + ": ";
+        auto body_ok = [&](double v) { return v >= 0. && v < NB; };
+        const int type = (int)r[0];
+        if (type >= 0 && type <= 3) {
+            if (r.size() != 6 || (type == 0 && r[1] > 2.)) {
+                err = where + "the reference reads entries this quantity does not have";
+                return false;
+            }
+            if (!body_ok(r[2])) { err = where + "body out of range"; return false; }
+            if (r[3] >= 0. && r[3] >= m.bodies[(int)r[2]].n_nodes) { err = where + "node out of range"; return false; }
+        } else if (type == 4) {
+            if (r.size() != 7) { err = where + "the reference reads entries this quantity does not have"; return false; }
+            if (!body_ok(r[3])) { err = where + "body out of range"; return false; }
+            const Body& B = m.bodies[(int)r[3]];
+            if (r[4] >= 0.) {
+                if (r[4] >= B.n_nodes) { err = where + "node out of range"; return false; }
+                if (B.rigid) {
+                    err = where + "forces of a rigid body's nodes are not supported (Body::Sum_up_forces copies the "
+                                  "body's forces to them, Code/Body.h:1999-2012; use the body-level form)";
+                    return false;
+                }
+            } else {
+                if (!body_ok(r[2])) {
+                    err = where + "the reference's body-level force branch tests body " + std::to_string((int)r[2]) +
+                          " (the force kind), which does not exist";
+                    return false;
+                }
+                const Body& K = m.bodies[(int)r[2]];
+                if (!K.rigid && (B.rigid || K.n_nodes > B.n_nodes)) {
+                    err = where + "the reference's body-level force branch reads " + std::to_string(K.n_nodes) +
+                          " nodes of a body that has " + std::to_string(B.n_nodes) + " deformable nodes";
+                    return false;
+                }
+                if (K.rigid && !B.rigid) {
+                    err = where + "the reference's body-level force branch reads the rigid-body forces of a deformable "
+                                  "body, which it never sets";
+                    return false;
+                }
+            }
+        } else if (type == 8) {
+            if (r.size() != 4) { err = where + "the reference reads entries this quantity does not have"; return false; }
+            if (!body_ok(r[1])) { err = where + "body out of range"; return false; }
+        }
+    }
+    return true;
+}
+
 
 void initialize_segments(Model& m, Body& body) {
     double period = m.xmax_period - m.xmin_period;
@@ -439,11 +609,34 @@ bool load_model_files_impl(const std::string& dir, int save_index, Model& m, std
                 in >> nb;
                 std::getline(in, token);
             }
-            if (starts(line, "MONITORING") || starts(line, "DEACTIVATION") || starts(line, "SPIES")) {
+            if (starts(line, "MONITORING") || starts(line, "SPIES")) {
                 int n = 0;
                 in >> n;
                 std::getline(in, token);
-                if (n != 0) { err = "monitoring, spies and deactivation entries are not supported"; return false; }
+                if (n != 0) { err = "monitoring and spies entries are not supported"; return false; }
+            }
+            if (starts(line, "DEACTIVATION")) {
+                // Code/IO.h:947-968
+                int n = 0;
+                in >> n;
+                std::getline(in, token);
+                std::vector<double> rule;
+                for (int i = 0; i < n; i++) {
+                    if (!read_spy_line(in, rule)) {
+                        err = "DEACTIVATION rule " + std::to_string(i) + ": unknown quantity keyword";
+                        return false;
+                    }
+                    std::string cmp;
+                    double threshold = 0.;
+                    in >> cmp >> threshold;
+                    if (cmp == "below") rule.push_back(0.);
+                    else if (cmp == "equal") rule.push_back(1.);
+                    else if (cmp == "above") rule.push_back(2.);
+                    else { err = "DEACTIVATION rule " + std::to_string(i) + ": unknown comparison " + cmp; return false; }
+                    rule.push_back(threshold);
+                    m.deactivated.push_back(rule);
+                    std::getline(in, token);
+                }
             }
             if (starts(line, "GRAPHIC")) {
                 for (int i = 0; i < 41; i++) {

@@ -9,6 +9,7 @@
 // Body::Initialize_segments does (Code/Body.h:403-605).
 #pragma once
 #include <cstdint>
+#include <istream>
 #include <string>
 #include <vector>
 
@@ -149,6 +150,10 @@ struct Model {
     int number_save = 0, number_iteration = 0, number_print = 0;
     double next_save = 0, next_print = 0, next_contact_update = 0;
     std::vector<Body> bodies;
+    // DEACTIVATION rules of Code/IO.h:947-968: the spy line of Read_spy_line
+    // (Code/IO.h:11-590) followed by the comparison (0 below, 1 equal, 2
+    // above) and the threshold
+    std::vector<std::vector<double>> deactivated;
 
     int law_index(int mat_s, int mat_m) const {
         return contacts_table[(size_t)mat_s * materials.size() + mat_m];
@@ -159,6 +164,16 @@ struct Model {
         return n;
     }
 };
+
+// Read_spy_line (Code/IO.h:11-590): one spy / monitor / deactivation
+// quantity from the stream into out.  False for a keyword outside its
+// vocabulary (the reference then leaves out as it was).
+bool read_spy_line(std::istream& in, std::vector<double>& out);
+
+// Update_status reads rule entries by position (Code/Proximity.h:550-1007);
+// a rule whose entries it would read out of range, or whose body-level
+// force branch would index nodes the body does not have, is refused.
+bool check_deactivation(const Model& m, std::string& err);
 
 // Parse STATIC_CONTROL.asc, STATIC_DATA.asc and DYNAMIC_<save>.asc from dir.
 bool load_model_files(const std::string& dir, int save_index, Model& m, std::string& err);

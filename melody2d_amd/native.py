@@ -115,6 +115,7 @@ def library():
             "mel_get_neighbours": ([vp, ctypes.c_int, P(ctypes.c_int32), ctypes.c_size_t, P(ctypes.c_size_t)],
                                    ctypes.c_int),
             "mel_get_body_state": ([vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)], ctypes.c_int),
+            "mel_get_body_status": ([vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_double)], ctypes.c_int),
             "mel_get_body_sizes": ([vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
                                     P(ctypes.c_int)], ctypes.c_int),
             "mel_compute_nodal_stresses": ([vp], ctypes.c_int),
@@ -171,7 +172,7 @@ def exported_symbols():
             "mel_get_node_field", "mel_set_node_field", "mel_get_body_sizes", "mel_compute_nodal_stresses", "mel_write_graphic",
             "mel_get_contact_internals",
             "mel_get_nodal_stress", "mel_get_contacts", "mel_get_neighbours",
-            "mel_get_body_state", "mel_get_body_energy", "mel_get_scalars", "mel_eval_forces",
+            "mel_get_body_state", "mel_get_body_status", "mel_get_body_energy", "mel_get_scalars", "mel_eval_forces",
             "mel_eval_forces_noalloc", "mel_get_state_size", "mel_save_state", "mel_load_state",
             "mel_get_border_field", "mel_get_info", "mel_write_dynamic", "mel_get_stream",
             "mel_set_kernel_timing", "mel_get_kernel_timing", "mel_kernel_count", "mel_kernel_name",
@@ -373,6 +374,14 @@ class Context:
         self._check(self.lib.mel_get_body_state(self._h, body, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                                 w.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return r, w
+
+    def body_status(self, body):
+        """(status, damage) of a body: 0 active, 1 inactive, 2 nogravity;
+        Body::damage (mel_get_body_status)."""
+        st = ctypes.c_int()
+        dm = ctypes.c_double()
+        self._check(self.lib.mel_get_body_status(self._h, body, ctypes.byref(st), ctypes.byref(dm)))
+        return st.value, dm.value
 
     def body_energy(self, body):
         """{kinetic, kinetic from velocity parameters, boundary work of the

@@ -800,6 +800,25 @@ def variants():
     v["tiny_periodic"] = tiny_periodic()
     v["tiny_simple"] = tiny_simple()
     v["tiny_holed"] = tiny_holed()
+    # deactivation rules (Update_status, Code/Proximity.h:550-1007) on the
+    # discs bonded to each other and to the platens: the upper disc's damage
+    # crosses its threshold while its bonds still carry forces (its last
+    # sends keep reaching the lower disc and the top platen); the other
+    # rules read positions, velocities, accelerations, forces of nodes and
+    # bodies and never hold, and a Jacobian line the reference ignores
+    m = _set_law(tiny(h=0.45e-3, with_platens=True, overlap=-1e-6), "CZMlinear", [1e13, 2e6, 5e-6, 1e5, 1.0, 2e-6])
+    m.flags = m.flags + ["INITIALIZE_CZM"]
+    m.deactivation = [("Position Y 0 -1", "below", 0.99e-3), ("Velocity Mag 0 5", "above", 10.0),
+                      ("Force Y Contact 3 -1", "above", 1e9), ("Force X Internal 1 -1", "above", 1e12),
+                      ("Acceleration R 2 -1", "above", 1e30), ("Position X 2 -1", "equal", 5.0),
+                      ("Force Y Total 0 3", "below", -1e30), ("Jacobian 0 0", "below", 1.0),
+                      ("Displacement Mag 1 -1", "above", 1.0), ("Damage 1", "above", 0.9)]
+    v["tiny_deact"] = m
+    # a body inactive from the start (the DYNAMIC file's status): no update,
+    # no detection, and the other bodies do not see it
+    m = tiny(h=0.45e-3, with_platens=True)
+    m.bodies[1].status = "inactive"
+    v["tiny_inactive"] = m
     # the adaptive schemes (Code/Solver.h:98-486) on the two discs between
     # platens: error-controlled time steps with rejected attempts, and the
     # two mass-scaling forms (the last one rejects on the mass ratio)

@@ -44,6 +44,10 @@ class Model:
     periodic: tuple = (0.0, 1.0)
     gravity: tuple = (0.0, 0.0)
     flags: list = field(default_factory=lambda: ["UPDATE_MASS_MATRIX", "UPDATE_DAMPING_MATRIX", "NO_LOG"])
+    # DEACTIVATION rules (Code/IO.h:947-968): spy-line text (the words
+    # Read_spy_line reads, e.g. "Position Y 1 -1"), "below" | "equal" |
+    # "above", threshold
+    deactivation: list = field(default_factory=list)
 
     @property
     def n_nodes(self):
@@ -162,7 +166,9 @@ def write_static(model: Model, d: str):
     out += ["PERIODIC_BOUNDARIES", "%s %s" % (g(model.periodic[0]), g(model.periodic[1])), " ", " "]
     out += ["GRAVITY", "%s %s" % (g(model.gravity[0]), g(model.gravity[1])), " ", " "]
     out += ["NUMBER_BODIES", "%d" % len(model.bodies), " ", " "]
-    out += ["MONITORING", "0", " ", " ", "SPIES", "0", " ", " ", "DEACTIVATION", "0", " ", " "]
+    out += ["MONITORING", "0", " ", " ", "SPIES", "0", " ", " ", "DEACTIVATION", "%d" % len(model.deactivation)]
+    out += ["%s %s %s" % (spy, cmp, g(thr)) for spy, cmp, thr in model.deactivation]
+    out += [" ", " "]
     # Code/IO.h:1004-1009 reads 41 graphic flags
     out += ["GRAPHIC"] + ["0 Flag%d" % i for i in range(41)] + [" ", " "]
     for i, b in enumerate(model.bodies):

@@ -91,6 +91,7 @@ struct State {
     double total_error = 0, max_error = 0;
     int node_for_max = 0;
     double mass_ms = 0, factor_ms = 1, delta_factor_ms = 0, max_factor_ms = 0;
+    double damage = 0;                        // Body::damage (Code/Body.h:2667-2700)
 };
 
 struct Sim {
@@ -1927,6 +1928,7 @@ void dump(Sim& S, const std::string& path) {
         w.f64(p + "works", std::vector<double>(s.works, s.works + 7));
         w.scalar(p + "mass", s.mass);
         w.f64(p + "energy", body_energy(S, (int)bi));
+        if (!S.m.deactivated.empty()) w.f64(p + "status", {(double)b.status, s.damage});
         if (adaptive && g_solver.stepped) {
             std::vector<double> er = {s.total_error, s.max_error, (double)s.node_for_max};
             if (ms) { er.push_back(s.mass_ms); er.push_back(s.factor_ms); }
@@ -1962,6 +1964,111 @@ void dump(Sim& S, const std::string& path) {
     }
 }
 
+// --------------------------------------------------------------------------
+// deactivation (Update_status, Code/Proximity.h:550-1007)
+// --------------------------------------------------------------------------
+
+// Body::Update_damage (Code/Body.h:2667-2700): per border node the smallest
+// first internal variable of its contact elements that have one (1 when
+// none), averaged over the border nodes; never decreases
+void update_damage(Sim& S, int bi) {
+    const Body& b = S.m.bodies[bi];
+    State& s = S.s[bi];
+    double total_nodes = 0., damage = 0.;
+    const double previous = s.damage;
+    for (size_t i = 0; i < b.borders.size(); i++) {
+        const int nbn = (int)b.borders[i].nodes.size();
+        total_nodes += nbn;
+        for (int j = 0; j < nbn; j++) {
+            double d = 1.;
+            for (const ContactElement& c : s.contacts)
+                if (c.borderS == (int)i && c.border_nodeS == j && c.nb_internal > 0 && c.internal[0] < d)
+                    d = c.internal[0];
+            damage += d;
+        }
+    }
+    damage = damage / total_nodes;
+    if (damage < previous) damage = previous;
+    s.damage = damage;
+}
+
+// The rules in order.  x, y, r carry over from one rule to the next, as the
+// reference's locals do (a deformable body's r, a force kind without a
+// branch).  Types 0-3 read a node's or the body's position, displacement,
+// velocity, acceleration (body level: x_mass-weighted means over the nodes
+// of a deformable body, the body's own values for a rigid one); type 4 a
+// node's force of one kind, or over the nodes (the reference tests the type
+// and node count of body [2], the force kind, and reads the nodes of body
+// [3]); type 8 the body's damage.  A rule that holds sets the body inactive.
+void update_status(Sim& S) {
+    Model& m = S.m;
+    double x = 0., y = 0., r = 0.;
+    auto compare = [](int cmp, double v, double thr) {
+        return (cmp == 0 && v <= thr) || (cmp == 1 && v == thr) || (cmp == 2 && v >= thr);
+    };
+    for (const std::vector<double>& D : m.deactivated) {
+        const int type = (int)D[0];
+        if (type >= 0 && type <= 3) {
+            const int bi = (int)D[2];
+            const Body& b = m.bodies[bi];
+            const State& s = S.s[bi];
+            const std::vector<double>* nx[4] = {&s.xc, &s.ux, &s.vx, &s.ax};
+            const std::vector<double>* ny[4] = {&s.yc, &s.uy, &s.vy, &s.ay};
+            const double* rb[4] = {s.cur, s.disp, s.vel, s.acc};
+            if (D[3] >= 0) {
+                const int n = (int)D[3];
+                x = (*nx[type])[n];
+                y = (*ny[type])[n];
+            } else if (!b.rigid) {
+                double mass = 0.;
+                x = 0.;
+                y = 0.;
+                for (int j = 0; j < b.n_nodes; j++) {
+                    x += (*nx[type])[j] * s.mx[j];
+                    y += (*ny[type])[j] * s.my[j];
+                    mass += s.mx[j];
+                }
+                x /= mass;
+                y /= mass;
+            } else {
+                x = rb[type][0];
+                y = rb[type][1];
+                r = rb[type][2];
+            }
+            const int comp = (int)D[1];
+            double v;
+            if (type == 0) v = comp == 0 ? x : (comp == 1 ? y : r);
+            else v = comp == 0 ? x : (comp == 1 ? y : (comp == 2 ? std::pow(x * x + y * y, 0.5) : r));
+            if (compare((int)D[4], v, D[5])) m.bodies[bi].status = ST_INACTIVE;
+        } else if (type == 4) {
+            const int kind = (int)D[2], bi = (int)D[3];
+            const State& s = S.s[bi];
+            const std::vector<double>* fx[7] = {&s.fx, &s.fix, &s.fcx, &s.fbx, &s.fdx, &s.fnx, &s.fkx};
+            const std::vector<double>* fy[7] = {&s.fy, &s.fiy, &s.fcy, &s.fby, &s.fdy, &s.fny, &s.fky};
+            if (D[4] >= 0) {
+                const int n = (int)D[4];
+                if (kind <= 6) { x = (*fx[kind])[n]; y = (*fy[kind])[n]; }
+            } else if (!m.bodies[kind].rigid) {
+                x = 0.;
+                y = 0.;
+                for (int j = 0; j < m.bodies[kind].n_nodes; j++)
+                    if (kind <= 6) { x += (*fx[kind])[j]; y += (*fy[kind])[j]; }
+            } else {
+                // the rigid branch has no internal (1) or ALID (7) kind
+                const double* rf[7] = {s.rf, nullptr, s.rfc, s.rfb, s.rfd, s.rfn, s.rfk};
+                if (kind <= 6 && rf[kind]) { x = rf[kind][0]; y = rf[kind][1]; r = rf[kind][2]; }
+            }
+            const int comp = (int)D[1];
+            const double v = comp == 0 ? x : (comp == 1 ? y : (comp == 2 ? std::pow(x * x + y * y, 0.5) : r));
+            if (compare((int)D[5], v, D[6])) m.bodies[bi].status = ST_INACTIVE;
+        } else if (type == 8) {
+            const int bi = (int)D[1];
+            update_damage(S, bi);
+            if (compare((int)D[2], S.s[bi].damage, D[3])) m.bodies[bi].status = ST_INACTIVE;
+        }
+    }
+}
+
 int run(Sim& S, int nsteps, const char* out_dir, const int* dump_steps, int n_dump, double* seconds) {
     auto want = [&](int st) {
         for (int k = 0; k < n_dump; k++)
@@ -1981,6 +2088,7 @@ int run(Sim& S, int nsteps, const char* out_dir, const int* dump_steps, int n_du
         if (S.time > S.next_contact_update - 1.e-6 * S.m.dt) {
             S.next_contact_update = S.next_contact_update + S.m.contact_update_period;
             update_proximity(S);
+            update_status(S);
         }
         if (out_dir && want(st)) {
             std::snprintf(name, sizeof name, "/step_%06d.bin", st);
@@ -2050,8 +2158,6 @@ int oracle_run(const char* path, int save_index, int nsteps, const char* out_dir
     std::string err;
     bool ok = save_index >= 0 ? load_model_files(path, save_index, S.m, err) : load_model_binary(path, S.m, err);
     if (!ok) { g_err = err; return 1; }
-    for (auto& b : S.m.bodies)
-        if (b.status == ST_INACTIVE) { g_err = "inactive bodies are not supported"; return 1; }
     for (auto& b : S.m.bodies)
         if (S.m.flags[F_UPDATE_STIFFNESS] && !b.mat_dof.empty()) { g_err = "UPDATE_STIFFNESS_MATRIX is not supported"; return 1; }
     setup(S);

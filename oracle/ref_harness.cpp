@@ -105,6 +105,8 @@ struct SolverScalars {
     double max_error = 0, mean_error = 0, total_mass = 0, max_mass = 0;
 };
 static SolverScalars g_solver;
+// the model has DEACTIVATION rules: the bodies' status and damage are dumped
+static bool g_has_rules = false;
 
 static void dump_state(const string& path, vector<Body>& Bodies, double Time, int iteration, double Deltat) {
     FilteredWriter w(path);
@@ -170,6 +172,7 @@ static void dump_state(const string& path, vector<Body>& Bodies, double Time, in
                            B.neumann_work, B.damping_work, B.alid_work});
         w.scalar(p + "mass", B.mass);
         w.f64(p + "energy", body_energy(B, Deltat));
+        if (g_has_rules) w.f64(p + "status", {B.status == "inactive" ? 1. : (B.status == "nogravity" ? 2. : 0.), B.damage});
         if (g_solver.adaptive && g_solver.stepped) {
             // Body::Compute_error / Compute_mass_scaling results (Code/Body.h:2524-2585),
             // once a step has set them (the body's mass_mass_scaling has no initial value)
@@ -296,6 +299,7 @@ int main(int argc, char** argv) {
                 Regions, Nb_bodies, Bodies, To_Plot);
     Load_dynamic(Number_save, Number_print, Time, Number_iteration, Deltat, Solver, Next_save, Next_print,
                  Next_contact_update, Xmin_period, Xmax_period, Bodies, flags);
+    g_has_rules = Nb_deactivated > 0;
 
     // initialisation block, Code/main.cpp:103-126
     for (int i = 0; i < Nb_bodies; i++) Bodies[i].Update_borders(Xmin_period, Xmax_period);

@@ -18,8 +18,9 @@ VARIANTS = sorted(f[:-7] for f in os.listdir(GOLDEN)
 # golden sets run with the adaptive schemes of Solver_step; the others are Euler
 ADAPTIVE_VARIANTS = [v for v in VARIANTS if v in ("tiny_adaptive", "tiny_mass_scaling", "tiny_adaptive_ms")]
 EULER_VARIANTS = [v for v in VARIANTS if v not in ADAPTIVE_VARIANTS]
-# models the two-launch step runs (bodies with absorbing layers take the launch sequence)
-TWO_LAUNCH_VARIANTS = [v for v in EULER_VARIANTS if v != "tiny_alid"]
+# models the two-launch step runs (absorbing layers, deactivation rules and
+# inactive bodies take the launch sequence)
+TWO_LAUNCH_VARIANTS = [v for v in EULER_VARIANTS if v not in ("tiny_alid", "tiny_deact", "tiny_inactive")]
 RESTARTS = sorted(f[:-15] for f in os.listdir(GOLDEN) if f.endswith("_restart.tar.gz"))
 # nodal stress sets: model files and the reference's stress fields
 STRESSES = sorted(f[:-14] for f in os.listdir(GOLDEN) if f.endswith("_stress.tar.gz"))

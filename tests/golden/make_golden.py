@@ -7,7 +7,7 @@ compiled from its own sources (oracle/_ref/melody_ref, built by
 together with the reference's state dumps (format of oracle/dump_format.hpp)
 in <variant>.tar.gz: steps 0, 1, 2, 10, 50 and 200, and also 1000 for the
 periodic shear cell, whose top wall needs that long to carry contact points
-across its periodic seam.
+across its periodic seam, and 80 and 90 for the deactivation variant.
 
 The two-disc platen workload (configs.c1, ~800 nodes) is too large to store
 as model files: c1.tar.gz holds only the reference's dumps at steps 0, 1,
@@ -41,7 +41,10 @@ from melody2d_amd.preprocessor import write_model  # noqa: E402
 from melody2d_amd.preprocessor.configs import c1, variants  # noqa: E402
 
 STEPS = [0, 1, 2, 10, 50, 200]
-LONG = {"tiny_periodic": STEPS + [1000]}
+# the deactivation variant also at the contact update that stops the upper
+# disc (step 80) and within the period after it (90), while the lower disc
+# and the top platen still name it as master
+LONG = {"tiny_periodic": STEPS + [1000], "tiny_deact": sorted(STEPS + [80, 90])}
 C1_STEPS = [0, 1, 10, 100, 1000]
 C1_FIELDS = ("x_current,y_current,x_disp_param,y_disp_param,x_vel_param,y_vel_param,x_internal_force,"
              "y_internal_force,x_contact_force,y_contact_force,x_force,y_force,rigid,works,energy,neighbours,"

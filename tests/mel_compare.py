@@ -69,6 +69,8 @@ def device_state(ctx, n_nodes_per_body, n_borders_per_body, border_sizes):
         D[p + "rigid"] = r
         D[p + "works"] = w
         D[p + "energy"] = ctx.body_energy(b)
+        st, dmg = ctx.body_status(b)
+        D[p + "status"] = np.array([float(st), dmg])
         if si.scheme != 0:
             D[p + "error"] = ctx.body_error(b)[:5 if ms else 3]
             if ms:

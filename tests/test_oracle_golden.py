@@ -43,7 +43,7 @@ def test_goldens_exercise_the_step():
             "rigid_rotation": False, "periodic_neighbour": False, "periodic_stencil": False,
             "slave_border_1": False, "master_border_3": False, "boundary_work": False,
             "fatigue_damage": False, "bond_broken_later": False, "alid": False, "rejected_attempt": False,
-            "mass_scaled": False}
+            "mass_scaled": False, "deactivated": False, "damage": False}
     for name in VARIANTS:
         with tempfile.TemporaryDirectory() as d:
             _, ref = extract_golden(name, d)
@@ -78,6 +78,11 @@ def test_goldens_exercise_the_step():
                         seen["alid"] |= bool(np.any(D[p + "x_alid_force"] != 0) and D[p + "works"][6] != 0)
                     if p + "x_msf" in D:
                         seen["mass_scaled"] |= bool(np.any(D[p + "x_msf"] > 1))
+                    if p + "status" in D:
+                        # a body the DEACTIVATION rules stopped, a damage value
+                        # between 0 and 1 (Body::Update_damage)
+                        seen["deactivated"] |= bool(D[p + "status"][0] == 1)
+                        seen["damage"] |= bool(0 < D[p + "status"][1] < 1)
                 if "neval" in D:
                     seen["rejected_attempt"] |= bool(D["neval"][0] > 1)
     assert all(seen.values()), seen
