@@ -10,11 +10,13 @@
 
 namespace mel {
 
+// Update_borders of the initialisation block (Code/main.cpp:106-107), which
+// runs for every body, inactive ones included
 __global__ void k_border_length(DevModel m) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= m.n_slots) return;
     int b = m.br_body[m.sl_border[s]];
-    if (m.body_rigid_flag[b] || m.body_status[b] == 1) return;
+    if (m.body_rigid_flag[b]) return;
     m.sl_len[s] = slot_length(m, s);
 }
 
@@ -98,10 +100,20 @@ __global__ void k_alid(DevModel m, double t) {
     m.fay[j] = fy;
 }
 
+// rigid-body nodes from their bodies' records; with_forces < 0: the
+// positions Load_dynamic forms for every rigid body, inactive ones included
+// (Code/IO.h:2050)
 __global__ void k_rigid_nodes(DevModel m, int with_forces) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m.n_rigid_nodes) return;
-    rigid_node(m, m.rigid_node_ids[q], with_forces);
+    const int i = m.rigid_node_ids[q];
+    if (with_forces >= 0) {
+        rigid_node(m, i, with_forces);
+        return;
+    }
+    const double* R = m.body_rigid + m.node_body[i] * RIGID_FIELDS;
+    const double dr = R[R_DISP + 2] - 0.;
+    rigid_node_at(m, i, R, cos(dr), sin(dr), with_forces, m.X[i], m.Y[i]);
 }
 
 // ---------------------------------------------------------------------------
