@@ -204,6 +204,28 @@ mel_status mel_get_contact_internals(mel_ctx* ctx, int body, double* out3, size_
 /* neighbour triples (body2 border2 shift) of body (Code/Proximity.h:160) */
 mel_status mel_get_neighbours(mel_ctx* ctx, int body, int32_t* out, size_t cap, size_t* count);
 
+/* Monitoring (Code/Monitoring.h:8-559): with on = 1 a row is formed on the
+ * device after every step of mel_euler_steps / mel_solver_steps (the
+ * reference forms one per iteration unless NO_MONITORING, Code/main.cpp:
+ * 172-176): iteration, time, time step, neval, max_error, mean_error (0
+ * under Euler_step, where the reference leaves them uninitialised), one
+ * value per MONITORING entry that yields one (Read_spy_line types 0-10),
+ * then with MONITOR_ENERGY the boundary work, elastic energy (0: the laws
+ * set their energy to 0, Code/Model.h:28, 73) and kinetic energy.  Rows
+ * collect in host memory until read.  MEL_E_UNSUPPORTED when the model sets
+ * NO_MONITORING. */
+mel_status mel_set_monitoring(mel_ctx* ctx, int on);
+
+/* The rows formed since the last read, each as its value count followed by
+ * the values, and clears them; out = NULL only reports the count of
+ * doubles. */
+mel_status mel_get_monitoring(mel_ctx* ctx, double* out, size_t cap, size_t* count);
+
+/* Write_monitoring (Code/IO.h:2644-2666): appends the rows formed since the
+ * last read to path (MONITORING.asc in the reference's run directory), ten
+ * significant digits, and clears them. */
+mel_status mel_write_monitoring(mel_ctx* ctx, const char* path);
+
 /* Body::status as a mel status code (0 active, 1 inactive, 2 nogravity;
  * Code/Body.h:46, set to inactive by Update_status, Code/Proximity.h:550-1007)
  * and Body::damage (Code/Body.h:110, formed by Body::Update_damage,

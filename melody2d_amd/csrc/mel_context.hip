@@ -260,6 +260,16 @@ struct mel_ctx {
     int neval = 0;                // attempts of the last accepted step
     long long attempts = 0;       // attempts since load
     bool nan_stop = false;        // some body's total error was NaN
+    // Monitoring rows (mel_set_monitoring): device rows since the last
+    // drain, the MONITORING entries, the rows drained to the host (each: its
+    // value count, then the values)
+    bool mon_on = false;
+    double* d_mon_rows = nullptr;
+    int* d_mon_n = nullptr;
+    double* d_mon_iter = nullptr;
+    double* d_mon_items = nullptr;
+    int mon_stride = 0, mon_cap = 0, mon_pending = 0;
+    std::vector<double> h_mon;
 
     template <typename T>
     T* alloc(size_t n) {
@@ -975,6 +985,61 @@ void initialize_czm(mel_ctx& c) {
 void contact_update(mel_ctx& c) {
     update_proximity(c);
     if (c.d.n_rules > 0) k_update_status<<<1, STATUS_THREADS, 0, c.stream>>>(c.d, c.blocks_dev);
+}
+
+// Monitoring rows written on the device since the last drain appended to the
+// host's rows (Monitoring keeps them in memory until Write_monitoring)
+void mon_drain(mel_ctx& c) {
+    if (!c.mon_on || c.mon_pending == 0) return;
+    int n = 0;
+    HIP_OK(hipMemcpyAsync(&n, c.d_mon_n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    std::vector<double> rows((size_t)n * c.mon_stride);
+    if (n > 0)
+        HIP_OK(hipMemcpyAsync(rows.data(), c.d_mon_rows, rows.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    const int zero = 0;
+    HIP_OK(hipMemcpyAsync(c.d_mon_n, &zero, sizeof(int), hipMemcpyHostToDevice, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    for (int r = 0; r < n; r++) {
+        const double* q = rows.data() + (size_t)r * c.mon_stride;
+        const int nv = (int)q[0];
+        c.h_mon.push_back(nv);
+        c.h_mon.insert(c.h_mon.end(), q + 1, q + 1 + nv);
+    }
+    c.mon_pending = 0;
+}
+
+// Monitoring (Code/Monitoring.h:8-559) after a step (Code/main.cpp:172-176):
+// the energies it reads first (physical velocities of every node for
+// MONITOR_ENERGY), then the row
+void launch_monitor(mel_ctx& c, double neval, double max_error, double mean_error) {
+    if (!c.mon_on) return;
+    if (c.mon_pending >= c.mon_cap) mon_drain(c);
+    const Model& m = c.model;
+    bool energies = m.flags[F_MONITOR_ENERGY] == 1;
+    for (const std::vector<double>& q : m.monitored) energies = energies || (int)q[0] == 9;
+    if (energies) {
+        if (m.flags[F_MONITOR_ENERGY] == 1 && c.n_kin_all > 0)
+            k_kinematics_list<<<blocks(c.n_kin_all), BLOCK, 0, c.stream>>>(c.d, c.d_kin_all, c.n_kin_all);
+        k_body_energy<<<c.d.n_bodies, BLOCK, 0, c.stream>>>(c.d, c.d_energy);
+    }
+    MonitorArgs a{};
+    a.rows = c.d_mon_rows;
+    a.n_rows = c.d_mon_n;
+    a.stride = c.mon_stride;
+    a.cap = c.mon_cap;
+    a.items = c.d_mon_items;
+    a.n_items = (int)m.monitored.size();
+    a.energy = m.flags[F_MONITOR_ENERGY] == 1;
+    a.body_energy = c.d_energy;
+    a.stress = c.st_out;
+    a.iter = c.d_mon_iter;
+    a.neval = neval;
+    a.max_error = max_error;
+    a.mean_error = mean_error;
+    k_monitor<<<1, STATUS_THREADS, 0, c.stream>>>(c.d, c.blocks_dev, a);
+    HIP_OK(hipGetLastError());
+    c.mon_pending++;
 }
 
 // the launches of one step in two launches (the device advances its time)
@@ -1743,15 +1808,18 @@ void build(mel_ctx& c) {
     d.rec_fx = c.alloc<double>(3 * (size_t)d.c_cap + d.ghost_cap);
     d.rec_fy = c.alloc<double>(3 * (size_t)d.c_cap + d.ghost_cap);
     d.rig_rec = c.alloc<double>(6 * (size_t)d.c_cap);
+    if (d.n_rules > 0 || !m.monitored.empty()) {
+        // Body::damage, formed by Update_status and Monitoring (Body::Update_damage)
+        d.body_damage = c.alloc<double>(NB);
+        d.body_flip = c.alloc<int>(NB);
+        d.slot_damage = c.alloc<double>(NS);
+    }
     if (d.n_rules > 0) {
         std::vector<double> rules((size_t)RULE_STRIDE * d.n_rules, 0.);
         for (int i = 0; i < d.n_rules; i++)
             for (size_t k = 0; k < m.deactivated[i].size() && k < (size_t)RULE_STRIDE; k++)
                 rules[(size_t)RULE_STRIDE * i + k] = m.deactivated[i][k];
         d.rules = c.upload(rules);
-        d.body_damage = c.alloc<double>(NB);
-        d.body_flip = c.alloc<int>(NB);
-        d.slot_damage = c.alloc<double>(NS);
         d.gh_count = c.alloc<int>(2);
         d.gh_bn = c.alloc<int>(d.ghost_cap);
         d.gh_fx = c.alloc<double>(d.ghost_cap);
@@ -2461,8 +2529,10 @@ mel_status mel_euler_steps(mel_ctx* c, int nsteps) {
                     t = t + c->d.dt;
                     n++;
                     if (t > c->next_contact_update - 1.e-6 * c->d.dt) { due = true; break; }
+                    if (c->mon_on) break;      // a Monitoring row after every step
                 }
                 run_steps(*c, n);
+                launch_monitor(*c, 0., 0., 0.);
                 c->time = t;
                 c->iteration += n;
                 c->outputs_stale = true;
@@ -2485,6 +2555,7 @@ mel_status mel_euler_steps(mel_ctx* c, int nsteps) {
         for (int k = 0; k < nsteps; k++) {
             c->iteration++;
             euler_step(*c);
+            launch_monitor(*c, 0., 0., 0.);
             // contact-update cadence, Code/main.cpp:191-196
             if (c->time > c->next_contact_update - 1.e-6 * c->d.dt) {
                 c->next_contact_update = c->next_contact_update + c->model.contact_update_period;
@@ -2509,6 +2580,10 @@ mel_status mel_solver_steps(mel_ctx* c, int nsteps) {
         for (int k = 0; k < nsteps; k++) {
             c->iteration++;
             solver_step(*c);
+            if (c->mon_on) {
+                HIP_OK(hipStreamSynchronize(c->stream));
+                launch_monitor(*c, c->neval, c->h_ad[AO_MAX_ERROR], c->h_ad[AO_MEAN_ERROR]);
+            }
             // contact-update cadence with the new time step, Code/main.cpp:191-196
             if (c->time > c->next_contact_update - 1.e-6 * c->d.dt) {
                 c->next_contact_update = c->next_contact_update + c->model.contact_update_period;
@@ -2783,6 +2858,70 @@ mel_status mel_get_neighbours(mel_ctx* c, int body, int32_t* out, size_t cap, si
     });
 }
 
+mel_status mel_set_monitoring(mel_ctx* c, int on) {
+    if (!c) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        if (!on) {
+            mon_drain(*c);
+            c->mon_on = false;
+            return MEL_OK;
+        }
+        if (c->model.flags[F_NO_MONITORING]) {
+            c->err = "the model sets NO_MONITORING (Code/main.cpp:172 then skips Monitoring)";
+            return MEL_E_UNSUPPORTED;
+        }
+        if (!c->d_mon_rows) {
+            const Model& m = c->model;
+            // iteration, time, time step, neval, max_error, mean_error, one
+            // value per entry at most, three energies
+            c->mon_stride = 1 + 6 + (int)m.monitored.size() + 3;
+            c->mon_cap = 1024;
+            c->d_mon_rows = c->alloc<double>((size_t)c->mon_stride * c->mon_cap);
+            c->d_mon_n = c->alloc<int>(1);
+            c->d_mon_iter = c->alloc<double>(1);
+            std::vector<double> items((size_t)RULE_STRIDE * std::max<size_t>(1, m.monitored.size()), 0.);
+            for (size_t i = 0; i < m.monitored.size(); i++)
+                for (size_t k = 0; k < m.monitored[i].size() && k < (size_t)RULE_STRIDE; k++)
+                    items[RULE_STRIDE * i + k] = m.monitored[i][k];
+            c->d_mon_items = c->upload(items);
+        }
+        if (!c->mon_on) {
+            const double it = c->iteration;
+            HIP_OK(hipMemcpyAsync(c->d_mon_iter, &it, sizeof(double), hipMemcpyHostToDevice, c->stream));
+            HIP_OK(hipStreamSynchronize(c->stream));
+        }
+        c->mon_on = true;
+        return MEL_OK;
+    });
+}
+
+mel_status mel_get_monitoring(mel_ctx* c, double* out, size_t cap, size_t* count) {
+    if (!c || !count) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        mon_drain(*c);
+        *count = c->h_mon.size();
+        if (!out) return MEL_OK;
+        if (cap < c->h_mon.size()) {
+            c->err = "mel_get_monitoring: " + std::to_string(c->h_mon.size()) + " values, room for " + std::to_string(cap);
+            return MEL_E_ARG;
+        }
+        std::copy(c->h_mon.begin(), c->h_mon.end(), out);
+        c->h_mon.clear();
+        return MEL_OK;
+    });
+}
+
+mel_status mel_write_monitoring(mel_ctx* c, const char* path) {
+    if (!c || !path) return MEL_E_ARG;
+    return guarded(c, [&]() {
+        mon_drain(*c);
+        std::string err;
+        if (!write_monitoring_text(path, c->h_mon, err)) { c->err = err; return MEL_E_IO; }
+        c->h_mon.clear();
+        return MEL_OK;
+    });
+}
+
 mel_status mel_get_body_status(mel_ctx* c, int body, int* status, double* damage) {
     if (!c || body < 0 || body >= c->d.n_bodies) return MEL_E_ARG;
     return guarded(c, [&]() {
@@ -2791,7 +2930,7 @@ mel_status mel_get_body_status(mel_ctx* c, int body, int* status, double* damage
         if (status) *status = st[0];
         if (damage) {
             *damage = 0.;
-            if (c->d.n_rules > 0) {
+            if (c->d.body_damage) {
                 std::vector<double> dm;
                 c->download(dm, c->d.body_damage + body, 1);
                 *damage = dm[0];
@@ -2959,6 +3098,13 @@ mel_status mel_load_state(mel_ctx* c, const void* buf, size_t size) {
         c->iteration = (int)h.iteration;
         c->d.dt = h.dt;
         c->outputs_stale = false;
+        if (c->mon_on) {
+            // rows carry the restored iteration numbers
+            mon_drain(*c);
+            const double it = c->iteration;
+            HIP_OK(hipMemcpyAsync(c->d_mon_iter, &it, sizeof(double), hipMemcpyHostToDevice, c->stream));
+            HIP_OK(hipStreamSynchronize(c->stream));
+        }
         return MEL_OK;
     });
 }

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <map>
 #include <sstream>
 #include <stdexcept>
@@ -207,6 +208,7 @@ bool validate_model(const Model& m, std::string& err) {
     }
     const int NB = (int)m.bodies.size();
     if (!check_deactivation(m, err)) return false;
+    if (!check_monitored(m, err)) return false;
     for (size_t bi = 0; bi < m.bodies.size(); bi++) {
         const Body& b = m.bodies[bi];
         for (size_t k = 0; k + 2 < b.neighbours.size(); k += 3) {
@@ -343,6 +345,56 @@ bool read_spy_line(std::istream& in, std::vector<double>& out) {
         return true;
     }
     return false;
+}
+
+// Monitoring (Code/Monitoring.h:8-559) per entry type: 0-3 [1] component,
+// [2] body, [3] node (< 0: the body); 4 [1] component, [2] force kind,
+// [3] body, [4] node - its body-level branch tests the type of body [3] and
+// loops over the node count of body [2] (the force kind) reading the nodes
+// of body [3]; 5 [1] body, [2] node; 6 [1] component, [2] body, [3] node;
+// 7 [1] component, [2] body, [3] border, [4] border node; 8 [1] body;
+// 9 and 10 [1] component, [2] body (< 0: all or none).  Errors (11) and
+// mass scaling (12) have no branch and add nothing.
+bool check_monitored(const Model& m, std::string& err) {
+    const int NB = (int)m.bodies.size();
+    for (size_t i = 0; i < m.monitored.size(); i++) {
+        const std::vector<double>& r = m.monitored[i];
+        const std::string where = "MONITORING entry " + std::to_string(i) + ": ";
+        auto body_ok = [&](double v) { return v >= 0. && v < NB; };
+        auto node_ok = [&](double b, double n) { return n < m.bodies[(int)b].n_nodes; };
+        const int type = (int)r[0];
+        if (type >= 0 && type <= 3) {
+            if (r.size() != 4) { err = where + "the reference reads entries this quantity does not have"; return false; }
+            if (!body_ok(r[2])) { err = where + "body out of range"; return false; }
+            if (r[3] >= 0. && !node_ok(r[2], r[3])) { err = where + "node out of range"; return false; }
+        } else if (type == 4) {
+            if (!body_ok(r[3])) { err = where + "body out of range"; return false; }
+            const Body& B = m.bodies[(int)r[3]];
+            if (r[4] >= 0.) {
+                if (!node_ok(r[3], r[4])) { err = where + "node out of range"; return false; }
+            } else if (!B.rigid) {
+                if (!body_ok(r[2]) || m.bodies[(int)r[2]].n_nodes > B.n_nodes) {
+                    err = where + "the reference's body-level force branch loops over the nodes of body " +
+                          std::to_string((int)r[2]) + " (the force kind) while reading body " +
+                          std::to_string((int)r[3]) + ", which has fewer or does not exist";
+                    return false;
+                }
+            }
+        } else if (type == 5) {
+            if (!body_ok(r[1]) || r[2] < 0. || !node_ok(r[1], r[2])) { err = where + "node out of range"; return false; }
+        } else if (type == 6) {
+            if (!body_ok(r[2]) || r[3] < 0. || !node_ok(r[2], r[3])) { err = where + "node out of range"; return false; }
+        } else if (type == 7 || type == 8) {
+            if (!body_ok(type == 7 ? r[2] : r[1])) { err = where + "body out of range"; return false; }
+        } else if (type == 9 || type == 10) {
+            if (r[2] >= 0. && !body_ok(r[2])) { err = where + "body out of range"; return false; }
+        }
+    }
+    if (!m.monitored.empty() && m.flags[F_MONITOR_BOUNDARIES]) {
+        err = "MONITOR_BOUNDARIES is not supported: its sums start from uninitialised values (Code/Monitoring.h:540-549)";
+        return false;
+    }
+    return true;
 }
 
 // Update_status (Code/Proximity.h:550-1007) acts on types 0-4 and 8 and
@@ -609,11 +661,26 @@ bool load_model_files_impl(const std::string& dir, int save_index, Model& m, std
                 in >> nb;
                 std::getline(in, token);
             }
-            if (starts(line, "MONITORING") || starts(line, "SPIES")) {
+            if (starts(line, "SPIES")) {
                 int n = 0;
                 in >> n;
                 std::getline(in, token);
-                if (n != 0) { err = "monitoring and spies entries are not supported"; return false; }
+                if (n != 0) { err = "spies entries are not supported"; return false; }
+            }
+            if (starts(line, "MONITORING")) {
+                // Code/IO.h:933-945
+                int n = 0;
+                in >> n;
+                std::getline(in, token);
+                std::vector<double> q;
+                for (int i = 0; i < n; i++) {
+                    if (!read_spy_line(in, q)) {
+                        err = "MONITORING entry " + std::to_string(i) + ": unknown quantity keyword";
+                        return false;
+                    }
+                    m.monitored.push_back(q);
+                    std::getline(in, token);
+                }
             }
             if (starts(line, "DEACTIVATION")) {
                 // Code/IO.h:947-968
@@ -1238,6 +1305,22 @@ bool load_model_binary(const std::string& path, Model& m, std::string& err) {
 // ---------------------------------------------------------------------------
 // DYNAMIC writer (layout of Code/IO.h:2351-2636, full precision)
 // ---------------------------------------------------------------------------
+
+bool write_monitoring_text(const std::string& path, const std::vector<double>& rows, std::string& err) {
+    std::ofstream f(path, std::ios::app);
+    if (!f) { err = "cannot open " + path; return false; }
+    f << std::setprecision(10);
+    for (size_t i = 0; i < rows.size();) {
+        const size_t nv = (size_t)rows[i];
+        if (i + 1 + nv > rows.size()) { err = "malformed monitoring rows"; return false; }
+        for (size_t k = 0; k < nv; k++) f << rows[i + 1 + k] << ' ';
+        f << '\n';
+        i += 1 + nv;
+    }
+    f.flush();
+    if (!f) { err = "cannot write " + path; return false; }
+    return true;
+}
 
 bool write_dynamic_text(const std::string& path, const Model& m, std::string& err) {
     FILE* f = std::fopen(path.c_str(), "w");

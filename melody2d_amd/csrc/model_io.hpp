@@ -154,6 +154,8 @@ struct Model {
     // (Code/IO.h:11-590) followed by the comparison (0 below, 1 equal, 2
     // above) and the threshold
     std::vector<std::vector<double>> deactivated;
+    // MONITORING quantities of Code/IO.h:933-945 (Read_spy_line entries)
+    std::vector<std::vector<double>> monitored;
 
     int law_index(int mat_s, int mat_m) const {
         return contacts_table[(size_t)mat_s * materials.size() + mat_m];
@@ -175,6 +177,10 @@ bool read_spy_line(std::istream& in, std::vector<double>& out);
 // force branch would index nodes the body does not have, is refused.
 bool check_deactivation(const Model& m, std::string& err);
 
+// Monitoring (Code/Monitoring.h:8-559) reads the MONITORING entries by
+// position too; the same checks for what it reads.
+bool check_monitored(const Model& m, std::string& err);
+
 // Parse STATIC_CONTROL.asc, STATIC_DATA.asc and DYNAMIC_<save>.asc from dir.
 bool load_model_files(const std::string& dir, int save_index, Model& m, std::string& err);
 
@@ -183,6 +189,11 @@ bool load_model_binary(const std::string& path, Model& m, std::string& err);
 
 // Segment stencils and initial lengths of every border (Code/Body.h:403-605).
 void initialize_segments(Model& m, Body& b);
+
+// Append Monitoring rows (each: its value count, then the values) to
+// `path` as Write_monitoring does (Code/IO.h:2644-2666): ten significant
+// digits, each value followed by a space, one row per line.
+bool write_monitoring_text(const std::string& path, const std::vector<double>& rows, std::string& err);
 
 // Write DYNAMIC_nnnnn-format text of the current state (Code/IO.h:2351-2636 layout,
 // 17 significant digits).

@@ -116,6 +116,9 @@ def library():
                                    ctypes.c_int),
             "mel_get_body_state": ([vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)], ctypes.c_int),
             "mel_get_body_status": ([vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_double)], ctypes.c_int),
+            "mel_set_monitoring": ([vp, ctypes.c_int], ctypes.c_int),
+            "mel_get_monitoring": ([vp, P(ctypes.c_double), ctypes.c_size_t, P(ctypes.c_size_t)], ctypes.c_int),
+            "mel_write_monitoring": ([vp, ctypes.c_char_p], ctypes.c_int),
             "mel_get_body_sizes": ([vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
                                     P(ctypes.c_int)], ctypes.c_int),
             "mel_compute_nodal_stresses": ([vp], ctypes.c_int),
@@ -172,7 +175,8 @@ def exported_symbols():
             "mel_get_node_field", "mel_set_node_field", "mel_get_body_sizes", "mel_compute_nodal_stresses", "mel_write_graphic",
             "mel_get_contact_internals",
             "mel_get_nodal_stress", "mel_get_contacts", "mel_get_neighbours",
-            "mel_get_body_state", "mel_get_body_status", "mel_get_body_energy", "mel_get_scalars", "mel_eval_forces",
+            "mel_get_body_state", "mel_get_body_status", "mel_set_monitoring", "mel_get_monitoring",
+            "mel_write_monitoring", "mel_get_body_energy", "mel_get_scalars", "mel_eval_forces",
             "mel_eval_forces_noalloc", "mel_get_state_size", "mel_save_state", "mel_load_state",
             "mel_get_border_field", "mel_get_info", "mel_write_dynamic", "mel_get_stream",
             "mel_set_kernel_timing", "mel_get_kernel_timing", "mel_kernel_count", "mel_kernel_name",
@@ -382,6 +386,29 @@ class Context:
         dm = ctypes.c_double()
         self._check(self.lib.mel_get_body_status(self._h, body, ctypes.byref(st), ctypes.byref(dm)))
         return st.value, dm.value
+
+    def set_monitoring(self, on=True):
+        """Form a Monitoring row after every step (mel_set_monitoring)."""
+        self._check(self.lib.mel_set_monitoring(self._h, 1 if on else 0))
+
+    def monitoring(self):
+        """The Monitoring rows formed since the last call, as a list of
+        arrays (mel_get_monitoring)."""
+        n = ctypes.c_size_t()
+        self._check(self.lib.mel_get_monitoring(self._h, None, 0, ctypes.byref(n)))
+        buf = np.zeros(max(1, n.value))
+        self._check(self.lib.mel_get_monitoring(self._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                buf.size, ctypes.byref(n)))
+        rows, i = [], 0
+        while i < n.value:
+            k = int(buf[i])
+            rows.append(buf[i + 1:i + 1 + k].copy())
+            i += 1 + k
+        return rows
+
+    def write_monitoring(self, path):
+        """Append the rows formed since the last read to path (mel_write_monitoring)."""
+        self._check(self.lib.mel_write_monitoring(self._h, os.fsencode(path)))
 
     def body_energy(self, body):
         """{kinetic, kinetic from velocity parameters, boundary work of the

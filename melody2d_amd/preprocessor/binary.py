@@ -56,6 +56,8 @@ def write_binary(model: Model, path: str):
         ptr = np.cumsum([0] + [len(p) for *_, p in m.contact_laws]).astype(np.int32)
         _rec(f, "model.law_ptr", ptr)
         _rec(f, "model.law_params", np.array([x for *_, p in m.contact_laws for x in p], dtype=float))
+        if getattr(m, "monitoring", None):
+            raise ValueError("the binary container has no MONITORING records: write the model as ASCII files")
         if getattr(m, "deactivation", None):
             raise ValueError("the binary container has no DEACTIVATION records: write the model as ASCII files")
         _rec(f, "model.n_bodies", np.array([len(m.bodies)], dtype=np.int32))

@@ -814,6 +814,27 @@ def variants():
                       ("Force Y Total 0 3", "below", -1e30), ("Jacobian 0 0", "below", 1.0),
                       ("Displacement Mag 1 -1", "above", 1.0), ("Damage 1", "above", 0.9)]
     v["tiny_deact"] = m
+    # Monitoring rows (Code/Monitoring.h:8-559) on the model with BCs,
+    # damping and a free rigid platen: every entry type, node- and
+    # body-level, the energies of MONITOR_ENERGY (entries chosen away from
+    # values that are rounding noise, such as a body's internal-force sum)
+    m = copy.deepcopy(v["tiny_bc"])
+    m.flags = m.flags + ["MONITOR_ENERGY"]
+    m.monitoring = ["Position X 0 3", "Position Y 1 -1", "Position R 3 -1", "Position R 0 -1",
+                    "Displacement Mag 0 -1", "Displacement R 3 -1", "Displacement Y 2 4",
+                    "Velocity Y 1 5", "Velocity X 3 -1", "Velocity Mag 2 -1",
+                    "Acceleration Mag 1 -1", "Acceleration R 3 -1", "Acceleration X 0 7",
+                    "Force Y Total 0 3", "Force X Contact 1 -1", "Force Mag Contact 0 -1", "Force Y Body 0 -1",
+                    "Force Mag Internal 0 5",
+                    "Force R Neumann 3 -1", "Force X Damping 1 4", "Force Y Alid 0 2", "Force X Internal 2 1",
+                    "Force Y Contact 3 -1", "Force X Total 2 5",
+                    "Jacobian 0 3", "Stress VM 1 2", "Stress Spherical 1 2",
+                    "Contact Gapn 0 0 13", "Contact Xsi 1 0 0", "Contact Gapn 1 0 7", "Contact Fx 0 0 13",
+                    "Contact Ynorm 2 0 11", "Damage 1",
+                    "Energy Kinetic 1", "Energy Kinetic 3", "Energy Kinetic -1", "Energy Deformation 0",
+                    "Work Internal 0", "Work Contact -1", "Work Damping 1", "Work Neumann 3", "Work Body -1",
+                    "Work Dirichlet 2", "Error X 0 1", "MassScaling X 0 1"]
+    v["tiny_monitor"] = m
     # a body inactive from the start (the DYNAMIC file's status): no update,
     # no detection, and the other bodies do not see it
     m = tiny(h=0.45e-3, with_platens=True)

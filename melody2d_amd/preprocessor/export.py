@@ -48,6 +48,8 @@ class Model:
     # Read_spy_line reads, e.g. "Position Y 1 -1"), "below" | "equal" |
     # "above", threshold
     deactivation: list = field(default_factory=list)
+    # MONITORING entries (Code/IO.h:933-945): spy-line text, e.g. "Velocity Y 1 5"
+    monitoring: list = field(default_factory=list)
 
     @property
     def n_nodes(self):
@@ -166,7 +168,8 @@ def write_static(model: Model, d: str):
     out += ["PERIODIC_BOUNDARIES", "%s %s" % (g(model.periodic[0]), g(model.periodic[1])), " ", " "]
     out += ["GRAVITY", "%s %s" % (g(model.gravity[0]), g(model.gravity[1])), " ", " "]
     out += ["NUMBER_BODIES", "%d" % len(model.bodies), " ", " "]
-    out += ["MONITORING", "0", " ", " ", "SPIES", "0", " ", " ", "DEACTIVATION", "%d" % len(model.deactivation)]
+    out += ["MONITORING", "%d" % len(model.monitoring)] + list(model.monitoring) + [" ", " "]
+    out += ["SPIES", "0", " ", " ", "DEACTIVATION", "%d" % len(model.deactivation)]
     out += ["%s %s %s" % (spy, cmp, g(thr)) for spy, cmp, thr in model.deactivation]
     out += [" ", " "]
     # Code/IO.h:1004-1009 reads 41 graphic flags

@@ -100,6 +100,7 @@ struct Sim {
     double time = 0;
     int iteration = 0;
     double next_contact_update = 0;
+    std::vector<double> monitor_row;          // Monitoring's row of the last step
 };
 
 // --------------------------------------------------------------------------
@@ -1898,6 +1899,7 @@ void dump(Sim& S, const std::string& path) {
             w.scalar("max_mass", g_solver.max_mass);
         }
     }
+    if (!S.m.monitored.empty() && !S.m.flags[F_NO_MONITORING] && S.iteration > 0) w.f64("monitor", S.monitor_row);
     for (size_t bi = 0; bi < S.s.size(); bi++) {
         const State& s = S.s[bi];
         const Body& b = S.m.bodies[bi];
@@ -1928,7 +1930,7 @@ void dump(Sim& S, const std::string& path) {
         w.f64(p + "works", std::vector<double>(s.works, s.works + 7));
         w.scalar(p + "mass", s.mass);
         w.f64(p + "energy", body_energy(S, (int)bi));
-        if (!S.m.deactivated.empty()) w.f64(p + "status", {(double)b.status, s.damage});
+        if (!S.m.deactivated.empty() || !S.m.monitored.empty()) w.f64(p + "status", {(double)b.status, s.damage});
         if (adaptive && g_solver.stepped) {
             std::vector<double> er = {s.total_error, s.max_error, (double)s.node_for_max};
             if (ms) { er.push_back(s.mass_ms); er.push_back(s.factor_ms); }
@@ -2069,6 +2071,155 @@ void update_status(Sim& S) {
     }
 }
 
+// --------------------------------------------------------------------------
+// Monitoring (Code/Monitoring.h:8-559): one row per step
+// --------------------------------------------------------------------------
+// The row: iteration, time, time step, and the adaptive schemes' neval,
+// max_error, mean_error (locals the reference leaves uninitialised under
+// Euler_step: 0 here), then one value per entry that yields one, then with
+// MONITOR_ENERGY the boundary work, elastic energy (the laws set their
+// energy to 0, Code/Model.h:28, 73) and kinetic energy.  x, y, r carry over
+// between entries as the reference's locals do (they start uninitialised
+// there; 0 here).
+void monitor(Sim& S) {
+    Model& m = S.m;
+    const int NB = (int)m.bodies.size();
+    std::vector<double> row = {(double)S.iteration, S.time, m.dt, 0., 0., 0.};
+    if (m.scheme != SCH_EULER) { row[3] = g_solver.neval; row[4] = g_solver.max_error; row[5] = g_solver.mean_error; }
+    double x = 0., y = 0., r = 0.;
+    for (const std::vector<double>& D : m.monitored) {
+        const int type = (int)D[0];
+        if (type >= 0 && type <= 3) {
+            const int bi = (int)D[2];
+            const Body& b = m.bodies[bi];
+            const State& s = S.s[bi];
+            const std::vector<double>* nx[4] = {&s.xc, &s.ux, &s.vx, &s.ax};
+            const std::vector<double>* ny[4] = {&s.yc, &s.uy, &s.vy, &s.ay};
+            const double* rb[4] = {s.cur, s.disp, s.vel, s.acc};
+            if (D[3] >= 0) {
+                x = (*nx[type])[(int)D[3]];
+                y = (*ny[type])[(int)D[3]];
+            } else if (!b.rigid) {
+                double mass = 0.;
+                x = 0.;
+                y = 0.;
+                for (int j = 0; j < b.n_nodes; j++) {
+                    x += (*nx[type])[j] * s.mx[j];
+                    y += (*ny[type])[j] * s.my[j];
+                    mass += s.mx[j];
+                }
+                x /= mass;
+                y /= mass;
+            } else {
+                x = rb[type][0];
+                y = rb[type][1];
+                r = rb[type][2];
+            }
+            const int c = (int)D[1];
+            if (type == 0) {
+                if (c == 0) row.push_back(x);
+                else if (c == 1) row.push_back(y);
+                else if (c == 2) row.push_back(r);
+            } else {
+                if (c == 0) row.push_back(x);
+                else if (c == 1) row.push_back(y);
+                else if (c == 2) row.push_back(std::pow(x * x + y * y, 0.5));
+                else if (c == 3) row.push_back(r);
+            }
+        } else if (type == 4) {
+            const int kind = (int)D[2], bi = (int)D[3];
+            const Body& b = m.bodies[bi];
+            const State& s = S.s[bi];
+            const std::vector<double>* fx[8] = {&s.fx, &s.fix, &s.fcx, &s.fbx, &s.fdx, &s.fnx, &s.fkx, &s.fax};
+            const std::vector<double>* fy[8] = {&s.fy, &s.fiy, &s.fcy, &s.fby, &s.fdy, &s.fny, &s.fky, &s.fay};
+            if (D[4] >= 0) {
+                if (kind <= 7) { x = (*fx[kind])[(int)D[4]]; y = (*fy[kind])[(int)D[4]]; }
+            } else if (!b.rigid) {
+                x = 0.;
+                y = 0.;
+                for (int j = 0; j < m.bodies[kind].n_nodes; j++)
+                    if (kind <= 7) { x += (*fx[kind])[j]; y += (*fy[kind])[j]; }
+            } else {
+                const double* rf[8] = {s.rf, nullptr, s.rfc, s.rfb, s.rfd, s.rfn, s.rfk, nullptr};
+                if (kind <= 7 && rf[kind]) { x = rf[kind][0]; y = rf[kind][1]; r = rf[kind][2]; }
+            }
+            const int c = (int)D[1];
+            if (c == 0) row.push_back(x);
+            else if (c == 1) row.push_back(y);
+            else if (c == 2) row.push_back(std::pow(x * x + y * y, 0.5));
+            else if (c == 3) row.push_back(r);
+        } else if (type == 5) {
+            row.push_back(1.);          // Node::jacobian until Compute_nodal_stresses runs
+        } else if (type == 6) {
+            if ((int)D[1] <= 8) row.push_back(0.);    // Node::Sigma* until Compute_nodal_stresses runs
+        } else if (type == 7) {
+            const int bi = (int)D[2], c = (int)D[1];
+            for (const ContactElement& e : S.s[bi].contacts) {
+                if (e.borderS == (int)D[3] && e.border_nodeS == (int)D[4]) {
+                    const double v[6] = {e.gapn, e.gapt, e.xsi, e.xnorm, e.ynorm, e.internal.empty() ? 0. : e.internal[0]};
+                    if (c >= 0 && c <= 5) row.push_back(v[c]);
+                    break;
+                }
+            }
+        } else if (type == 8) {
+            update_damage(S, (int)D[1]);
+            row.push_back(S.s[(int)D[1]].damage);
+        } else if (type == 9) {
+            x = 0.;
+            const int c = (int)D[1], bsel = (int)D[2];
+            auto kinetic = [&](int n) {
+                const Body& b = m.bodies[n];
+                const State& s = S.s[n];
+                if (!b.rigid) {
+                    for (int j = 0; j < b.n_nodes; j++) {
+                        x += 0.5 * s.vpx[j] * s.vpx[j] * s.mx[j];
+                        x += 0.5 * s.vpy[j] * s.vpy[j] * s.my[j];
+                    }
+                } else {
+                    x += 0.5 * s.vel[0] * s.vel[0] * s.mass;
+                    x += 0.5 * s.vel[1] * s.vel[1] * s.mass;
+                    x += 0.5 * s.vel[2] * s.vel[2] * s.inertia;
+                }
+            };
+            // the elastic energies are zero (the laws' energy is 0)
+            if (bsel >= 0 && c == 0) kinetic(bsel);
+            else if (bsel == -1 && c == 0)
+                for (int n = 0; n < NB; n++) kinetic(n);
+            row.push_back(x);
+        } else if (type == 10) {
+            const int c = (int)D[1], bsel = (int)D[2];
+            if (bsel >= 0) {
+                if (c >= 0 && c <= 6) row.push_back(S.s[bsel].works[c]);
+            } else if (bsel == -1) {
+                x = 0.;
+                for (int n = 0; n < NB; n++)
+                    if (c >= 0 && c <= 6) x += S.s[n].works[c];
+                row.push_back(x);
+            }
+        }
+    }
+    if (m.flags[F_MONITOR_ENERGY] == 1) {
+        double dwork = 0., elastic = 0., kinetic = 0.;
+        for (int i = 0; i < NB; i++) {
+            const Body& b = m.bodies[i];
+            const State& s = S.s[i];
+            for (size_t j = 0; j < b.borders.size(); j++)
+                for (size_t k = 0; k < b.borders[j].nodes.size(); k++) {
+                    const int node = b.borders[j].nodes[k];
+                    dwork += (s.xbp[j][k] * s.vx[node] + s.ybp[j][k] * s.vy[node]) * s.length[j][k] * m.dt;
+                }
+            for (int j = 0; j < b.n_nodes; j++) {
+                kinetic += 0.5 * s.vx[j] * s.vx[j] * s.mx[j];
+                kinetic += 0.5 * s.vy[j] * s.vy[j] * s.my[j];
+            }
+        }
+        row.push_back(dwork);
+        row.push_back(elastic);
+        row.push_back(kinetic);
+    }
+    S.monitor_row = row;
+}
+
 int run(Sim& S, int nsteps, const char* out_dir, const int* dump_steps, int n_dump, double* seconds) {
     auto want = [&](int st) {
         for (int k = 0; k < n_dump; k++)
@@ -2085,6 +2236,8 @@ int run(Sim& S, int nsteps, const char* out_dir, const int* dump_steps, int n_du
         S.iteration++;
         if (S.m.scheme == SCH_EULER) euler_step(S);
         else solver_step(S);
+        // Monitoring (Code/main.cpp:172-176), then the contact update
+        if (!S.m.flags[F_NO_MONITORING]) monitor(S);
         if (S.time > S.next_contact_update - 1.e-6 * S.m.dt) {
             S.next_contact_update = S.next_contact_update + S.m.contact_update_period;
             update_proximity(S);

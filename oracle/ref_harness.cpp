@@ -105,8 +105,12 @@ struct SolverScalars {
     double max_error = 0, mean_error = 0, total_mass = 0, max_mass = 0;
 };
 static SolverScalars g_solver;
-// the model has DEACTIVATION rules: the bodies' status and damage are dumped
+// the model has DEACTIVATION rules or MONITORING entries: the bodies'
+// status and damage are dumped
 static bool g_has_rules = false;
+// the model has MONITORING entries: the last step's Monitoring row is dumped
+static bool g_has_monitor = false;
+static vector<double> g_monitor_row;
 
 static void dump_state(const string& path, vector<Body>& Bodies, double Time, int iteration, double Deltat) {
     FilteredWriter w(path);
@@ -114,6 +118,7 @@ static void dump_state(const string& path, vector<Body>& Bodies, double Time, in
     w.scalar("time", Time);
     w.scalar("iteration", (double)iteration);
     w.scalar("n_bodies", (double)Bodies.size());
+    if (g_has_monitor && iteration > 0) w.f64("monitor", g_monitor_row);
     if (g_solver.adaptive) w.scalar("dt", Deltat);
     if (g_solver.adaptive && g_solver.stepped) {
         w.scalar("neval", (double)g_solver.neval);
@@ -277,7 +282,7 @@ int main(int argc, char** argv) {
     double Xmin_plot, Xmax_plot, Ymin_plot, Ymax_plot;
     int Nb_bodies;
     vector<Body> Bodies;
-    int Nb_monitored;
+    int Nb_monitored = 0;
     vector<vector<double>> Monitored;
     int Nb_deactivated = 0;
     vector<vector<double>> Deactivated;
@@ -299,7 +304,9 @@ int main(int argc, char** argv) {
                 Regions, Nb_bodies, Bodies, To_Plot);
     Load_dynamic(Number_save, Number_print, Time, Number_iteration, Deltat, Solver, Next_save, Next_print,
                  Next_contact_update, Xmin_period, Xmax_period, Bodies, flags);
-    g_has_rules = Nb_deactivated > 0;
+    g_has_rules = Nb_deactivated > 0 || Nb_monitored > 0;
+    g_has_monitor = Nb_monitored > 0 && flags[8] == 0;
+    vector<vector<double>> monitoring;
 
     // initialisation block, Code/main.cpp:103-126
     for (int i = 0; i < Nb_bodies; i++) Bodies[i].Update_borders(Xmin_period, Xmax_period);
@@ -341,6 +348,14 @@ int main(int argc, char** argv) {
                         g_solver.neval, g_solver.max_error, g_solver.mean_error, flags, g_solver.total_mass,
                         g_solver.max_mass, Solver);
             g_solver.stepped = true;
+        }
+        // Monitoring, Code/main.cpp:172-176 (the harness's neval, max_error
+        // and mean_error start at 0 where the driver's are uninitialised)
+        if (flags[8] == 0) {
+            Monitoring(Nb_bodies, Bodies, Time, Deltat, Number_iteration, g_solver.neval, g_solver.max_error,
+                       g_solver.mean_error, flags, monitoring, Nb_monitored, Monitored);
+            g_monitor_row = monitoring.back();
+            monitoring.clear();
         }
         // contact-update cadence, Code/main.cpp:191-196
         if (Time > Next_contact_update - 1.e-6 * Deltat) {

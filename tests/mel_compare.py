@@ -124,9 +124,9 @@ def compare(dev, ref, skip=()):
             continue
         if r.dtype.kind == "i":
             out[k] = (0.0 if np.array_equal(d, r) else float("inf"), bool(np.array_equal(d, r)))
-        elif k.endswith(".energy"):
-            # kinetic energies and the boundary work differ in scale: each
-            # component against its own magnitude
+        elif k.endswith(".energy") or k == "monitor":
+            # kinetic energies and the boundary work, and the entries of a
+            # Monitoring row, differ in scale: each against its own magnitude
             out[k] = (max(rel_err(d[i:i + 1], r[i:i + 1]) for i in range(len(r))), bool(np.array_equal(d, r)))
         else:
             out[k] = (rel_err(d, r), bool(np.array_equal(d, r)))

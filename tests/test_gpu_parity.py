@@ -33,9 +33,16 @@ ERR_RECORDS = (".error", "max_error", "mean_error")
 ADAPTIVE_LATE_TOL = {"tiny_adaptive": 1e-10, "tiny_adaptive_ms": 1e-10, "tiny_mass_scaling": 1e-7}
 
 
+# a Monitoring row's entries are single node or body values, each held to
+# its own magnitude rather than to the norm of a field
+MON_TOL = 1e-9
+
+
 def _tol(key, step=0, name=""):
     if any(key.endswith(r) for r in ERR_RECORDS):
         return ERR_TOL
+    if key == "monitor":
+        return MON_TOL
     return ADAPTIVE_LATE_TOL[name] if name in ADAPTIVE_LATE_TOL and step > 50 else TOL
 
 pytestmark = pytest.mark.gpu
@@ -44,6 +51,10 @@ pytestmark = pytest.mark.gpu
 def _run_pair(model_dir, steps, tmp, binary_path=None):
     refs, _ = run_oracle(model_dir, steps, str(tmp / "oracle"))
     ctx = native.Context(binary_path, binary=True) if binary_path else native.Context(model_dir)
+    # models with MONITORING entries: the row of each compared step
+    monitor = "monitor" in refs[max(steps)]
+    if monitor:
+        ctx.set_monitoring(True)
     out = {}
     prev = 0
     for s in steps:
@@ -51,7 +62,12 @@ def _run_pair(model_dir, steps, tmp, binary_path=None):
         prev = s
         ctx.synchronize()
         nn, nbr, sizes = layout_from_dump(refs[s])
-        out[s] = (device_state(ctx, nn, nbr, sizes), refs[s])
+        dev = device_state(ctx, nn, nbr, sizes)
+        if monitor and s > 0:
+            rows = ctx.monitoring()
+            assert len(rows) == s - (steps[steps.index(s) - 1] if steps.index(s) > 0 else 0)
+            dev["monitor"] = rows[-1]
+        out[s] = (dev, refs[s])
     ctx.close()
     return out
 
@@ -67,6 +83,10 @@ def test_device_matches_oracle(name, golden_dir, tmp_path):
         res = compare(dev, ref)
         assert len(res) > 20
         bad = {k: v[0] for k, v in res.items() if not v[0] <= _tol(k, s, name)}
+        if "monitor" in bad:
+            d, r = np.asarray(dev["monitor"]), np.asarray(ref["monitor"])
+            e = [abs(d[i] - r[i]) / max(abs(r[i]), 1e-300) for i in range(len(r))]
+            print("monitor entries", [(int(i), d[i], r[i]) for i in np.argsort(e)[::-1][:4]], flush=True)
         assert not bad, "step %d: %s" % (s, sorted(bad.items(), key=lambda kv: -kv[1])[:8])
 
 

@@ -43,7 +43,7 @@ def test_goldens_exercise_the_step():
             "rigid_rotation": False, "periodic_neighbour": False, "periodic_stencil": False,
             "slave_border_1": False, "master_border_3": False, "boundary_work": False,
             "fatigue_damage": False, "bond_broken_later": False, "alid": False, "rejected_attempt": False,
-            "mass_scaled": False, "deactivated": False, "damage": False}
+            "mass_scaled": False, "deactivated": False, "damage": False, "monitored": False}
     for name in VARIANTS:
         with tempfile.TemporaryDirectory() as d:
             _, ref = extract_golden(name, d)
@@ -83,6 +83,8 @@ def test_goldens_exercise_the_step():
                         # between 0 and 1 (Body::Update_damage)
                         seen["deactivated"] |= bool(D[p + "status"][0] == 1)
                         seen["damage"] |= bool(0 < D[p + "status"][1] < 1)
+                # a Monitoring row with entries beyond its six leading scalars
+                seen["monitored"] |= bool("monitor" in D and len(D["monitor"]) > 6)
                 if "neval" in D:
                     seen["rejected_attempt"] |= bool(D["neval"][0] > 1)
     assert all(seen.values()), seen
