@@ -68,6 +68,12 @@ public:
         works7.assign(7, 0.);
         check(mel_get_body_state(ctx_, index_, rigid26.data(), works7.data()), ctx_);
     }
+    // Body::status (0 active, 1 inactive, 2 nogravity) and Body::damage
+    int status(double* damage = nullptr) const {
+        int st = 0;
+        check(mel_get_body_status(ctx_, index_, &st, damage), ctx_);
+        return st;
+    }
 
 private:
     mel_ctx* ctx_;
@@ -115,6 +121,11 @@ public:
     }
     // Write_dynamic (Code/IO.h:2351-2636)
     void write_dynamic(const std::string& path) { check(mel_write_dynamic(ctx_, path.c_str()), ctx_); }
+    // Monitoring rows after every step from now on (Code/Monitoring.h:8-559);
+    // false when the model sets NO_MONITORING
+    bool start_monitoring() { return mel_set_monitoring(ctx_, 1) == MEL_OK; }
+    // Write_monitoring (Code/IO.h:2644-2666): the rows since the last write, appended
+    void write_monitoring(const std::string& path) { check(mel_write_monitoring(ctx_, path.c_str()), ctx_); }
     void synchronize() { check(mel_synchronize(ctx_), ctx_); }
     mel_ctx* handle() { return ctx_; }
 

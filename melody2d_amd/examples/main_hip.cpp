@@ -19,6 +19,9 @@ int main(int argc, char** argv) {
         fprintf(stderr, "load failed: %s\n", mel_last_error(nullptr));
         return 1;
     }
+    // Monitoring after every iteration unless the model sets NO_MONITORING
+    // (Code/main.cpp:172-176), written at every save (Code/main.cpp:221-222)
+    const bool monitoring = mel_set_monitoring(ctx, 1) == MEL_OK;
     char name[64];
     int rc = 0;
     for (int save = number_save + 1; save <= number_save + saves; save++) {
@@ -32,7 +35,8 @@ int main(int argc, char** argv) {
             break;
         }
         snprintf(name, sizeof name, "DYNAMIC_%05d.asc", save);
-        if (mel_write_dynamic(ctx, name) != MEL_OK) {      // Write_dynamic, Code/IO.h:2351
+        if (mel_write_dynamic(ctx, name) != MEL_OK ||      // Write_dynamic, Code/IO.h:2351
+            (monitoring && mel_write_monitoring(ctx, "MONITORING.asc") != MEL_OK)) {
             fprintf(stderr, "write failed: %s\n", mel_last_error(ctx));
             rc = 3;
             break;
