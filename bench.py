@@ -502,7 +502,7 @@ def main():
     ap.add_argument("--error-steps-after", type=int, default=500,
                     help="steps both forms run from that state for the displacement and energy errors")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run 5 device copies of MIB MiB (scale reference for HBM byte counters)")
+                    help="only run 5 device copies and 5 reductions of MIB MiB (scale references for HBM byte counters)")
     ap.add_argument("--batch-samples", type=int, default=64,
                     help="shear cells of the batched leg (0: skip it; the configuration's full batch is 512)")
     ap.add_argument("--batch-tune-samples", type=int, default=16,
@@ -539,8 +539,13 @@ def main():
         b = torch.empty_like(a)
         for _ in range(5):
             b.copy_(a)
+        # a read-only pass (a reduction reads every byte once, writes one
+        # value): the read counter's scale without a write stream beside it
+        s = torch.zeros((), dtype=torch.float64, device="cuda")
+        for _ in range(5):
+            s += a.sum()
         torch.cuda.synchronize()
-        log("copied %d MiB 5 times" % args.copy_calibration)
+        log("copied %d MiB 5 times, summed it 5 times (%.0f)" % (args.copy_calibration, float(s)))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (samples and grains would be "
