@@ -374,15 +374,19 @@ __global__ void __launch_bounds__(NB_THREADS) k_prox_neighbours(DevModel m) {
 }
 
 // One workgroup: the first row chunk of every body (the bodies' row layouts
-// come from k_prox_neighbours) and the body of every row chunk; clears the
-// hit list
+// come from k_prox_neighbours); clears the hit list
 __global__ void __launch_bounds__(1024) k_prox_chunk_offsets(DevModel m) {
     __shared__ int s_w[16];
     wg_scan_array(m.body_chunks, m.chunk_off, m.n_bodies, s_w);
-    __syncthreads();
-    for (int b = threadIdx.x; b < m.n_bodies; b += blockDim.x)
-        for (int c = m.chunk_off[b]; c < m.chunk_off[b + 1]; c++) m.chunk_body[c] = b;
     if (threadIdx.x == 0) *m.hit_n = 0;
+}
+
+// One workgroup per body: the body of each of its row chunks (a wall's
+// thousands of chunks spread over the workgroup's lanes)
+__global__ void k_prox_chunk_bodies(DevModel m, int cap) {
+    const int b = blockIdx.x;
+    const int c1 = min(m.chunk_off[b + 1], cap);
+    for (int c = m.chunk_off[b] + (int)threadIdx.x; c < c1; c += blockDim.x) m.chunk_body[c] = b;
 }
 
 struct RowInfo {

@@ -38,6 +38,7 @@ __global__ void k_cells_scan_top(DevModel m, int nblocks);
 __global__ void k_cells_scatter(DevModel m);
 __global__ void k_prox_neighbours(DevModel m);
 __global__ void k_prox_chunk_offsets(DevModel m);
+__global__ void k_prox_chunk_bodies(DevModel m, int cap);
 __global__ void k_prox_count(DevModel m, int n_count);
 __global__ void k_prox_elem_offsets(DevModel m);
 __global__ void k_prox_write(DevModel m);

@@ -952,6 +952,7 @@ void update_proximity(mel_ctx& c) {
     k_cells_scatter<<<blocks(d.n_borders), BLOCK, 0, s>>>(d);
     k_prox_neighbours<<<d.n_bodies, NB_THREADS, 0, s>>>(d);
     k_prox_chunk_offsets<<<1, 1024, 0, s>>>(d);
+    k_prox_chunk_bodies<<<d.n_bodies, BLOCK, 0, s>>>(d, c.max_row_chunks);
     // previous elements become the history source
     std::swap(c.bufA, c.bufB);
     d.C = c.bufA;
