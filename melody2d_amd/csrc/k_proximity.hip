@@ -61,6 +61,13 @@ __device__ inline void wg_scan_array(const int* in, int* out, int n, int* s_w) {
 // depend on the order they are taken in.
 __global__ void k_prox_chunk_boxes(DevModel m) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    // the cell list's counters start again (nothing reads them before
+    // k_cells_assign)
+    for (int q = g; q < m.cell_n; q += gridDim.x * blockDim.x) {
+        m.cell_count[q] = 0;
+        m.cell_fill[q] = 0;
+    }
+    if (g == 0) *m.big_count = 0;
     const int c = g / SEG_CHUNK, j = g % SEG_CHUNK;
     const bool live = c < m.n_seg_chunks;
     const int br = live ? m.sc_border[c] : 0;
@@ -374,11 +381,14 @@ __global__ void __launch_bounds__(NB_THREADS) k_prox_neighbours(DevModel m) {
 }
 
 // One workgroup: the first row chunk of every body (the bodies' row layouts
-// come from k_prox_neighbours); clears the hit list
+// come from k_prox_neighbours); clears the hit list and the rigid bodies'
+// master-element counters
 __global__ void __launch_bounds__(1024) k_prox_chunk_offsets(DevModel m) {
     __shared__ int s_w[16];
     wg_scan_array(m.body_chunks, m.chunk_off, m.n_bodies, s_w);
     if (threadIdx.x == 0) *m.hit_n = 0;
+    // the rigid bodies' master-element counters (build_rigid_lists)
+    for (int r = threadIdx.x; r < m.n_rigid; r += blockDim.x) m.rm_cnt[r] = 0;
 }
 
 // One workgroup per body: the body of each of its row chunks (a wall's

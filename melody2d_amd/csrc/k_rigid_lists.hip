@@ -45,8 +45,12 @@ __device__ inline int lower_bound_int(const int* a, int n, int v) {
     return lo;
 }
 
-// one workgroup per rigid body: its list sorted into element order (bitonic
-// sort in LDS over the next power of two), and its slave range
+// lists up to this length are sorted by rank (each entry counts the
+// smaller ones: one pass, no barriers); longer ones by a bitonic network
+constexpr int RANK_SORT_MAX = 1024;
+
+// one workgroup per rigid body: its list sorted into element order (the
+// entries are distinct element ids), and its slave range
 __global__ void __launch_bounds__(RL_BLOCK) k_rm_sort(DevModel m) {
     __shared__ int s[RIGID_LIST_CAP];
     const int r = blockIdx.x;
@@ -55,6 +59,17 @@ __global__ void __launch_bounds__(RL_BLOCK) k_rm_sort(DevModel m) {
     int* lst = m.rm_list + (size_t)r * RIGID_LIST_CAP;
     int p = 1;
     while (p < n) p <<= 1;
+    if (n <= RANK_SORT_MAX) {
+        for (int i = t; i < n; i += RL_BLOCK) s[i] = lst[i];
+        __syncthreads();
+        for (int i = t; i < n; i += RL_BLOCK) {
+            const int v = s[i];
+            int rank = 0;
+            for (int j = 0; j < n; j++) rank += s[j] < v;
+            lst[rank] = v;
+        }
+        p = 0;      // nothing left for the network
+    }
     for (int i = t; i < p; i += RL_BLOCK) s[i] = i < n ? lst[i] : INT_MAX;
     __syncthreads();
     for (int k = 2; k <= p; k <<= 1) {
@@ -70,7 +85,8 @@ __global__ void __launch_bounds__(RL_BLOCK) k_rm_sort(DevModel m) {
             __syncthreads();
         }
     }
-    for (int i = t; i < n; i += RL_BLOCK) lst[i] = s[i];
+    if (p > 0)
+        for (int i = t; i < n; i += RL_BLOCK) lst[i] = s[i];
     if (t == 0) {
         const int nc = *m.c_count;
         const int b = m.rigid_ids[r];
@@ -81,9 +97,9 @@ __global__ void __launch_bounds__(RL_BLOCK) k_rm_sort(DevModel m) {
 
 }  // namespace
 
-hipError_t build_rigid_lists(const DevModel& d, hipStream_t s) {
+hipError_t build_rigid_lists(const DevModel& d, hipStream_t s, bool clear) {
     if (d.n_rigid == 0) return hipSuccess;
-    k_rm_clear<<<(d.n_rigid + RL_BLOCK - 1) / RL_BLOCK, RL_BLOCK, 0, s>>>(d);
+    if (clear) k_rm_clear<<<(d.n_rigid + RL_BLOCK - 1) / RL_BLOCK, RL_BLOCK, 0, s>>>(d);
     k_rm_append<<<std::min((d.c_cap + RL_BLOCK - 1) / RL_BLOCK, 1024), RL_BLOCK, 0, s>>>(d);
     k_rm_sort<<<d.n_rigid, RL_BLOCK, 0, s>>>(d);
     return hipGetLastError();

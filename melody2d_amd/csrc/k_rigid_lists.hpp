@@ -10,6 +10,8 @@ constexpr int RIGID_LIST_CAP = 4096;
 
 // after a contact update (or a restored state): the lists of DevModel
 // rm_cnt / rm_list / rs_range from the current elements, queued on stream s
-hipError_t build_rigid_lists(const DevModel& d, hipStream_t s);
+// clear: zero the per-body counters first (a proximity update has its
+// chunk-offset pass do it)
+hipError_t build_rigid_lists(const DevModel& d, hipStream_t s, bool clear);
 
 }  // namespace mel

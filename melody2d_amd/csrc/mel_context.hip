@@ -941,11 +941,12 @@ void update_proximity(mel_ctx& c) {
     flush_contact_work(c);
     const int persistent = 4 * 256;     // workgroups of the grid-stride loops
     c.prox_after_step = true;
+    // (the chunk-box pass also clears the cell list's counters)
     if (d.n_seg_chunks > 0) k_prox_chunk_boxes<<<blocks((long long)d.n_seg_chunks * SEG_CHUNK), BLOCK, 0, s>>>(d);
+    else k_cells_clear<<<std::min(blocks(d.cell_n), 1024), BLOCK, 0, s>>>(d);
     k_prox_aabb<<<blocks((long long)d.n_borders * 64), BLOCK, 0, s>>>(d);
     // broad phase: the cell list, then the neighbour lists from it
     const int cblocks = (d.cell_n + 1023) / 1024;
-    k_cells_clear<<<std::min(blocks(d.cell_n), 1024), BLOCK, 0, s>>>(d);
     k_cells_assign<<<blocks(d.n_borders), BLOCK, 0, s>>>(d);
     k_cells_scan_blocks<<<cblocks, 1024, 0, s>>>(d);
     k_cells_scan_top<<<1, 1024, 0, s>>>(d, cblocks);
@@ -965,7 +966,7 @@ void update_proximity(mel_ctx& c) {
     k_prox_write<<<persistent, BLOCK, 0, s>>>(d);
     k_prox_history<<<persistent, BLOCK, 0, s>>>(d);
     k_prox_chain<<<persistent, BLOCK, 0, s>>>(d);
-    HIP_OK(build_rigid_lists(d, s));
+    HIP_OK(build_rigid_lists(d, s, false));     // k_prox_chunk_offsets cleared the counters
     HIP_OK(hipMemcpyAsync(c.h_async, d.c_count, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(c.h_async + 1, d.status_flags, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipEventRecord(c.ev_async, s));
@@ -3090,7 +3091,7 @@ mel_status mel_load_state(mel_ctx* c, const void* buf, size_t size) {
             pos += a.bytes;
         }
         HIP_OK(hipMemcpyAsync(c->d.c_count, &nc, sizeof(int), hipMemcpyHostToDevice, c->stream));
-        HIP_OK(build_rigid_lists(c->d, c->stream));
+        HIP_OK(build_rigid_lists(c->d, c->stream, true));
         HIP_OK(hipStreamSynchronize(c->stream));
         c->n_contacts = nc;
         c->nc_hint = nc;
