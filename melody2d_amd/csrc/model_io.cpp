@@ -347,6 +347,30 @@ bool read_spy_line(std::istream& in, std::vector<double>& out) {
     return false;
 }
 
+// n DEACTIVATION rules (Code/IO.h:947-968): a spy line, the comparison
+// (below, equal, above) and the threshold, one rule per line
+bool read_deactivation_rules(std::istream& in, int n, Model& m, std::string& err) {
+    std::string token;
+    std::vector<double> rule;
+    for (int i = 0; i < n; i++) {
+        if (!read_spy_line(in, rule)) {
+            err = "DEACTIVATION rule " + std::to_string(i) + ": unknown quantity keyword";
+            return false;
+        }
+        std::string cmp;
+        double threshold = 0.;
+        in >> cmp >> threshold;
+        if (cmp == "below") rule.push_back(0.);
+        else if (cmp == "equal") rule.push_back(1.);
+        else if (cmp == "above") rule.push_back(2.);
+        else { err = "DEACTIVATION rule " + std::to_string(i) + ": unknown comparison " + cmp; return false; }
+        rule.push_back(threshold);
+        m.deactivated.push_back(rule);
+        std::getline(in, token);
+    }
+    return true;
+}
+
 // Monitoring (Code/Monitoring.h:8-559) per entry type: 0-3 [1] component,
 // [2] body, [3] node (< 0: the body); 4 [1] component, [2] force kind,
 // [3] body, [4] node - its body-level branch tests the type of body [3] and
@@ -683,27 +707,10 @@ bool load_model_files_impl(const std::string& dir, int save_index, Model& m, std
                 }
             }
             if (starts(line, "DEACTIVATION")) {
-                // Code/IO.h:947-968
                 int n = 0;
                 in >> n;
                 std::getline(in, token);
-                std::vector<double> rule;
-                for (int i = 0; i < n; i++) {
-                    if (!read_spy_line(in, rule)) {
-                        err = "DEACTIVATION rule " + std::to_string(i) + ": unknown quantity keyword";
-                        return false;
-                    }
-                    std::string cmp;
-                    double threshold = 0.;
-                    in >> cmp >> threshold;
-                    if (cmp == "below") rule.push_back(0.);
-                    else if (cmp == "equal") rule.push_back(1.);
-                    else if (cmp == "above") rule.push_back(2.);
-                    else { err = "DEACTIVATION rule " + std::to_string(i) + ": unknown comparison " + cmp; return false; }
-                    rule.push_back(threshold);
-                    m.deactivated.push_back(rule);
-                    std::getline(in, token);
-                }
+                if (!read_deactivation_rules(in, n, m, err)) return false;
             }
             if (starts(line, "GRAPHIC")) {
                 for (int i = 0; i < 41; i++) {
@@ -1152,6 +1159,23 @@ bool load_model_binary(const std::string& path, Model& m, std::string& err) {
         need(fl.size() >= (size_t)F_COUNT, "model.flags");
         for (int f = 0; f < F_COUNT; f++) m.flags[f] = fl[f];
         // optional: the scheme and its error control (absent: Euler)
+        if (recs.count("model.deactivation")) {
+            // the rules' text, one character per entry, read as the ASCII
+            // reader reads the DEACTIVATION section
+            auto& tx = I("model.deactivation");
+            std::string text;
+            for (int ch : tx) {
+                need(ch > 0 && ch < 128, "model.deactivation text");
+                text.push_back((char)ch);
+            }
+            std::istringstream rs(text);
+            int n = 0;
+            rs >> n;
+            std::string rest;
+            std::getline(rs, rest);
+            need(n >= 0 && n < (1 << 20), "model.deactivation count");
+            if (!read_deactivation_rules(rs, n, m, err)) return false;
+        }
         if (recs.count("model.solver")) {
             auto& sv = D("model.solver");
             need(sv.size() == 8, "model.solver size");

@@ -177,6 +177,9 @@ bool read_spy_line(std::istream& in, std::vector<double>& out);
 // force branch would index nodes the body does not have, is refused.
 bool check_deactivation(const Model& m, std::string& err);
 
+// n DEACTIVATION rules from `in` into m.deactivated (Code/IO.h:947-968).
+bool read_deactivation_rules(std::istream& in, int n, Model& m, std::string& err);
+
 // Monitoring (Code/Monitoring.h:8-559) reads the MONITORING entries by
 // position too; the same checks for what it reads.
 bool check_monitored(const Model& m, std::string& err);

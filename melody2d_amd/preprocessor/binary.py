@@ -9,7 +9,7 @@ import struct
 
 import numpy as np
 
-from .export import Model
+from .export import Model, g
 
 _MAT = {"ElasticLinear": 0, "NeoHookean": 1}
 _LAW = {"Frictionless": 0, "Cohesion": 1, "MohrCoulomb": 2, "DampedMohrCoulomb": 3, "TwoSlopesMohrCoulomb": 4,
@@ -59,7 +59,10 @@ def write_binary(model: Model, path: str):
         if getattr(m, "monitoring", None):
             raise ValueError("the binary container has no MONITORING records: write the model as ASCII files")
         if getattr(m, "deactivation", None):
-            raise ValueError("the binary container has no DEACTIVATION records: write the model as ASCII files")
+            # the DEACTIVATION section's text, one character per entry
+            text = "%d\n" % len(m.deactivation) + "".join("%s %s %s\n" % (spy, cmp, g(thr))
+                                                           for spy, cmp, thr in m.deactivation)
+            _rec(f, "model.deactivation", np.frombuffer(text.encode("ascii"), dtype=np.uint8).astype(np.int32))
         _rec(f, "model.n_bodies", np.array([len(m.bodies)], dtype=np.int32))
         for i, b in enumerate(m.bodies):
             if b.alid is not None:

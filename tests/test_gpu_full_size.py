@@ -24,10 +24,14 @@ from melody2d_amd import native
 
 pytestmark = pytest.mark.gpu
 
+_MODELS = {}
+
+
 def _make(tmp_path_factory, packing):
     from melody2d_amd.preprocessor import c2
     from melody2d_amd.preprocessor.binary import write_binary
     model = c2(n_grains=1000, packing=packing)
+    _MODELS[packing] = model
     path = str(tmp_path_factory.mktemp("c2_" + packing) / "c2.bin")
     write_binary(model, path)
     rigid = [b.rigid for b in model.bodies]
@@ -65,6 +69,39 @@ def test_full_size_matches_oracle(packing, workload, tmp_path_factory, tmp_path)
         print("  step %d: %d records, %d contact elements" % (s, len(res), n_contacts), flush=True)
     if packing == "deposit":
         assert n_contacts >= 10000
+    ctx.close()
+
+
+def test_full_size_deactivation_matches_oracle(workload, tmp_path_factory, tmp_path):
+    """Deactivation at scale: 40 grains of the deposited pile stop at the
+    first contact update (rules that hold: a velocity magnitude is never
+    below 0), while they press on their neighbours; their last sends go on
+    reaching those neighbours and the walls.  Against the oracle at steps
+    10, 11 and 25: statuses exact, every record within 1e-12."""
+    import copy, os, sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from mel_compare import compare, device_state, layout_from_dump, run_oracle
+    from melody2d_amd.preprocessor.binary import write_binary
+    model = copy.copy(_MODELS["deposit"])
+    model.deactivation = [("Velocity Mag %d -1" % b, "above", 0.0) for b in range(0, 400, 10)]
+    path = str(tmp_path_factory.mktemp("c2_deact") / "c2.bin")
+    write_binary(model, path)
+    print("case full-size deactivation", flush=True)
+    steps = [0, 10, 11, 25]
+    refs, _ = run_oracle(path, steps, str(tmp_path / "oracle"), save_index=-1)
+    ctx = native.Context(path, binary=True)
+    prev = 0
+    for s in steps:
+        ctx.euler_steps(s - prev)
+        prev = s
+        nn, nbr, sizes = layout_from_dump(refs[s])
+        dev = device_state(ctx, nn, nbr, sizes)
+        stopped = [b for b in range(len(nn)) if refs[s]["b%d.status" % b][0] == 1]
+        assert len(stopped) == (40 if s >= 10 else 0)
+        res = compare(dev, refs[s])
+        bad = sorted(((k, v[0]) for k, v in res.items() if not v[0] <= 1e-12), key=lambda kv: -kv[1])
+        assert not bad, "step %d: %d records off, worst %s" % (s, len(bad), bad[:6])
+        print("  step %d: %d records, %d bodies stopped" % (s, len(res), len(stopped)), flush=True)
     ctx.close()
 
 
