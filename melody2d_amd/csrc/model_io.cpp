@@ -1159,6 +1159,27 @@ bool load_model_binary(const std::string& path, Model& m, std::string& err) {
         need(fl.size() >= (size_t)F_COUNT, "model.flags");
         for (int f = 0; f < F_COUNT; f++) m.flags[f] = fl[f];
         // optional: the scheme and its error control (absent: Euler)
+        if (recs.count("model.monitoring")) {
+            // the MONITORING entries' text, one character per entry
+            auto& tx = I("model.monitoring");
+            std::string text;
+            for (int ch : tx) {
+                need(ch > 0 && ch < 128, "model.monitoring text");
+                text.push_back((char)ch);
+            }
+            std::istringstream rs(text);
+            int n = 0;
+            rs >> n;
+            std::string rest;
+            std::getline(rs, rest);
+            need(n >= 0 && n < (1 << 20), "model.monitoring count");
+            std::vector<double> q;
+            for (int i = 0; i < n; i++) {
+                need(read_spy_line(rs, q), "model.monitoring entry");
+                m.monitored.push_back(q);
+                std::getline(rs, rest);
+            }
+        }
         if (recs.count("model.deactivation")) {
             // the rules' text, one character per entry, read as the ASCII
             // reader reads the DEACTIVATION section

@@ -57,7 +57,9 @@ def write_binary(model: Model, path: str):
         _rec(f, "model.law_ptr", ptr)
         _rec(f, "model.law_params", np.array([x for *_, p in m.contact_laws for x in p], dtype=float))
         if getattr(m, "monitoring", None):
-            raise ValueError("the binary container has no MONITORING records: write the model as ASCII files")
+            # the MONITORING section's text, one character per entry
+            text = "%d\n" % len(m.monitoring) + "".join("%s\n" % q for q in m.monitoring)
+            _rec(f, "model.monitoring", np.frombuffer(text.encode("ascii"), dtype=np.uint8).astype(np.int32))
         if getattr(m, "deactivation", None):
             # the DEACTIVATION section's text, one character per entry
             text = "%d\n" % len(m.deactivation) + "".join("%s %s %s\n" % (spy, cmp, g(thr))

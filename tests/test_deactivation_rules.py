@@ -51,3 +51,26 @@ def test_ignored_and_stale_rules_run(tmp_path):
     assert refs[9]["b3.status"][0] == 0
     assert refs[10]["b3.status"][0] == 1
     assert all(refs[10]["b%d.status" % b][0] == 0 for b in range(3))
+
+
+@pytest.mark.parametrize("name", ["tiny_deact", "tiny_monitor"])
+def test_binary_container_carries_rules_and_monitoring(tmp_path, name):
+    """The binary container holds the DEACTIVATION and MONITORING text; a
+    run from it matches the run from the ASCII files record for record."""
+    import numpy as np
+    from melody2d_amd.preprocessor.binary import write_binary
+    v = variants()
+    if name == "tiny_monitor":
+        # the same entries on the platen model (the container holds no
+        # damping matrices)
+        m = copy.deepcopy(v["tiny_platen"])
+        m.monitoring = list(v["tiny_monitor"].monitoring)
+        m.flags = m.flags + ["MONITOR_ENERGY"]
+    else:
+        m = v[name]
+    write_model(m, str(tmp_path / "m"))
+    write_binary(m, str(tmp_path / "m.bin"))
+    a, _ = run_oracle(str(tmp_path / "m"), [0, 100], str(tmp_path / "oa"))
+    b, _ = run_oracle(str(tmp_path / "m.bin"), [0, 100], str(tmp_path / "ob"), save_index=-1)
+    assert set(a[100]) == set(b[100])
+    assert not [k for k in a[100] if not np.array_equal(a[100][k], b[100][k])]
