@@ -814,6 +814,12 @@ def variants():
                       ("Force Y Total 0 3", "below", -1e30), ("Jacobian 0 0", "below", 1.0),
                       ("Displacement Mag 1 -1", "above", 1.0), ("Damage 1", "above", 0.9)]
     v["tiny_deact"] = m
+    # the same under Adaptive_Euler (Solver_step skips the stopped disc in
+    # its half steps, Store and Restore, Code/Solver.h:160-330)
+    m = copy.deepcopy(m)
+    m.scheme = "Adaptive_Euler"
+    m.target_error, m.control_parameter, m.accepted_ratio = 1e-4, 0.2, 2.0
+    v["tiny_deact_adaptive"] = m
     # Monitoring rows (Code/Monitoring.h:8-559) on the model with BCs,
     # damping and a free rigid platen: every entry type, node- and
     # body-level, the energies of MONITOR_ENERGY (entries chosen away from

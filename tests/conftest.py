@@ -16,7 +16,8 @@ VARIANTS = sorted(f[:-7] for f in os.listdir(GOLDEN)
                   if f.endswith(".tar.gz") and f != "c1.tar.gz" and not f.endswith("_restart.tar.gz")
                   and not f.endswith("_stress.tar.gz"))
 # golden sets run with the adaptive schemes of Solver_step; the others are Euler
-ADAPTIVE_VARIANTS = [v for v in VARIANTS if v in ("tiny_adaptive", "tiny_mass_scaling", "tiny_adaptive_ms")]
+ADAPTIVE_VARIANTS = [v for v in VARIANTS
+                     if v in ("tiny_adaptive", "tiny_mass_scaling", "tiny_adaptive_ms", "tiny_deact_adaptive")]
 EULER_VARIANTS = [v for v in VARIANTS if v not in ADAPTIVE_VARIANTS]
 # models the two-launch step runs (absorbing layers, deactivation rules and
 # inactive bodies take the launch sequence)

@@ -30,7 +30,8 @@ ERR_RECORDS = (".error", "max_error", "mean_error")
 # 1e-10; Mass_Scaling also switches every node factor between growing and
 # decaying on the same estimate against a threshold (Code/Node.h:278-306),
 # and its growth term is a pow() that the device rounds on its own: 1e-7
-ADAPTIVE_LATE_TOL = {"tiny_adaptive": 1e-10, "tiny_adaptive_ms": 1e-10, "tiny_mass_scaling": 1e-7}
+ADAPTIVE_LATE_TOL = {"tiny_adaptive": 1e-10, "tiny_adaptive_ms": 1e-10, "tiny_mass_scaling": 1e-7,
+                     "tiny_deact_adaptive": 1e-10}
 
 
 # a Monitoring row's entries are single node or body values, each held to
