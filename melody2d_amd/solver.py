@@ -46,6 +46,10 @@ class Body:
     def energy(self):
         return self._ctx.body_energy(self.index)
 
+    def status(self):
+        """(Body::status: 0 active, 1 inactive, 2 nogravity; Body::damage)."""
+        return self._ctx.body_status(self.index)
+
 
 class Model:
     """A model on one HIP device (owns its context)."""
@@ -89,6 +93,14 @@ class Model:
     def write_dynamic(self, path):
         """Write_dynamic (Code/IO.h:2351-2636)."""
         self._ctx.write_dynamic(path)
+
+    def start_monitoring(self):
+        """Monitoring rows after every step from now on (Code/Monitoring.h:8-559)."""
+        self._ctx.set_monitoring(True)
+
+    def write_monitoring(self, path):
+        """Write_monitoring (Code/IO.h:2644-2666): the rows since the last write, appended."""
+        self._ctx.write_monitoring(path)
 
     def close(self):
         self._ctx.close()
