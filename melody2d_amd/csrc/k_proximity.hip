@@ -9,6 +9,7 @@
 // looks up, per slave slot, the previous elements in their old order.
 #include <climits>
 #include "mel_device.hpp"
+#include "k_rigid_lists.hpp"
 #include "k_step.hpp"
 
 namespace mel {
@@ -741,6 +742,14 @@ __global__ void k_prox_chain(DevModel m) {
     const int n = *m.c_count;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     const ContactArrays& C = m.C;
+    // an element whose master is rigid goes to that body's list (sorted
+    // into element order by build_rigid_lists)
+    const int rr = m.body_rindex[C.bodyM[e]];
+    if (rr >= 0) {
+        const int pos = atomicAdd(&m.rm_cnt[rr], 1);
+        if (pos < RIGID_LIST_CAP) m.rm_list[(size_t)rr * RIGID_LIST_CAP + pos] = e;
+        else atomicOr(&m.status_flags[0], 32);
+    }
     int f = C.fresh[e];
     if (f == 0) continue;
     int b = C.bodyS[e];

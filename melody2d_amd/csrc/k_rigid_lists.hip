@@ -97,10 +97,12 @@ __global__ void __launch_bounds__(RL_BLOCK) k_rm_sort(DevModel m) {
 
 }  // namespace
 
-hipError_t build_rigid_lists(const DevModel& d, hipStream_t s, bool clear) {
+hipError_t build_rigid_lists(const DevModel& d, hipStream_t s, bool collect) {
     if (d.n_rigid == 0) return hipSuccess;
-    if (clear) k_rm_clear<<<(d.n_rigid + RL_BLOCK - 1) / RL_BLOCK, RL_BLOCK, 0, s>>>(d);
-    k_rm_append<<<std::min((d.c_cap + RL_BLOCK - 1) / RL_BLOCK, 1024), RL_BLOCK, 0, s>>>(d);
+    if (collect) {
+        k_rm_clear<<<(d.n_rigid + RL_BLOCK - 1) / RL_BLOCK, RL_BLOCK, 0, s>>>(d);
+        k_rm_append<<<std::min((d.c_cap + RL_BLOCK - 1) / RL_BLOCK, 1024), RL_BLOCK, 0, s>>>(d);
+    }
     k_rm_sort<<<d.n_rigid, RL_BLOCK, 0, s>>>(d);
     return hipGetLastError();
 }

@@ -10,8 +10,9 @@ constexpr int RIGID_LIST_CAP = 4096;
 
 // after a contact update (or a restored state): the lists of DevModel
 // rm_cnt / rm_list / rs_range from the current elements, queued on stream s
-// clear: zero the per-body counters first (a proximity update has its
-// chunk-offset pass do it)
-hipError_t build_rigid_lists(const DevModel& d, hipStream_t s, bool clear);
+// collect: gather the elements per rigid master here (a proximity update
+// has its chunk-offset pass clear the counters and its velocity-chain pass
+// append the elements); then each list is sorted and the slave ranges found
+hipError_t build_rigid_lists(const DevModel& d, hipStream_t s, bool collect);
 
 }  // namespace mel

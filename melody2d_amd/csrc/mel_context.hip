@@ -966,7 +966,7 @@ void update_proximity(mel_ctx& c) {
     k_prox_write<<<persistent, BLOCK, 0, s>>>(d);
     k_prox_history<<<persistent, BLOCK, 0, s>>>(d);
     k_prox_chain<<<persistent, BLOCK, 0, s>>>(d);
-    HIP_OK(build_rigid_lists(d, s, false));     // k_prox_chunk_offsets cleared the counters
+    HIP_OK(build_rigid_lists(d, s, false));     // the chunk-offset and chain passes collected the elements
     HIP_OK(hipMemcpyAsync(c.h_async, d.c_count, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(c.h_async + 1, d.status_flags, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipEventRecord(c.ev_async, s));
