@@ -1,6 +1,6 @@
 """Benchmark of the MELODY 2D explicit step on one MI355X.
 
-Headline workload: biaxial compression of 1000 polydisperse
+Headline workload (`configs.c2`): biaxial compression of 1000 polydisperse
 deformable grains (~200 MLS nodes each, ~208k nodes, ~1.04M Gauss points,
 neo-Hookean, penalty Mohr-Coulomb mu = 0.5, rigid walls, density x100)
 deposited into a pile, so ~1.8e4 contact elements act from the start;
@@ -16,9 +16,9 @@ in the timed region) and the CPU baseline (the FP64 OpenMP oracle on a
 bounded sample of the same workload, on this host's cores), and beside them:
   fp32_mixed  the same workload in the FP32-mixed form, with its force,
               displacement and energy errors against FP64 from one state;
-  plate       the perforated 1M-node RPIM plate: internal forces
+  plate       the perforated 1M-node RPIM plate (`configs.c3`): internal forces
               of one large body, FP64 and FP32-mixed, roofline, CPU baseline;
-  serving     Poisson arrivals of plate jobs through the job
+  serving     Poisson arrivals of plate jobs (`configs.c5`) through the job
               server, p50/p99 latency and aggregate node-steps/s.
 """
 import argparse
@@ -300,7 +300,7 @@ def make_models_main(args):
 
 
 def plate_leg(args, native, tmpdir):
-    """the perforated RPIM plate, internal forces of one large
+    """Configuration C3: the perforated RPIM plate, internal forces of one large
     body (the whole step is timed: the internal-force launch and a small
     update), from the uniaxial 10/s field."""
     import numpy as np
@@ -363,7 +363,7 @@ def plate_leg(args, native, tmpdir):
 
 
 def serve_leg(args, native, tmpdir):
-    """Continuous batching through the job server: Poisson
+    """Continuous batching through the job server (`configs.c5`): Poisson
     arrivals of perforated-plate jobs of log-uniform size, every second one
     two plates bonded by a cohesive law, admitted under an HBM budget and
     run concurrently; the same arrival schedule in FP64 and in the
@@ -414,7 +414,7 @@ def serve_leg(args, native, tmpdir):
 
 
 def batch_leg(args, native, tmpdir):
-    """independent periodic shear cells of uneven sizes in one
+    """Configuration C4: independent periodic shear cells of uneven sizes in one
     context (every step of every sample in the same launches), FP32-mixed,
     launch configuration chosen by the autotuner on the batch's first
     samples; the default configuration timed beside it."""
@@ -549,7 +549,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (samples and grains would be "
-              "sharded across GPUs by HBM footprint; see DESIGN.md).", file=sys.stderr)
+              "sharded across GPUs by HBM footprint).", file=sys.stderr)
         sys.exit(2)
 
     from melody2d_amd import native

@@ -1,4 +1,4 @@
-"""Batched samples in one context: independent periodic shear
+"""Batched samples in one context (`configs.c4`): independent periodic shear
 cells stacked in one model run as one batch of launches per step; every
 sample's state is bit for bit the state it reaches alone (the same sample
 built alone at the same place, with the batch's time step), in both

@@ -7,7 +7,7 @@ C-ABI, against the oracle and against the mathematics.
 * The energy balance: with no contact force the kinetic energy from velocity
   parameters changes by the sum of the works (Code/Body.h:2058-2066) to
   rounding, on the device as on the oracle.
-* The two-disc platen workload (~800 nodes) through 1000 steps
+* The two-disc platen workload (`configs.c1`, ~800 nodes) through 1000 steps
   against the oracle: contact sets and neighbour lists exactly; every
   floating-point record within 1e-12 through step 100; at step 1000
   kinematic records (positions, displacements, velocities, lengths, works,
@@ -129,7 +129,7 @@ def test_body_energy_matches_oracle(name, golden_dir, tmp_path):
 
 
 def test_c1_trajectory_1000_steps(tmp_path):
-    """SURVEY C1 through 1000 steps: device vs oracle (and vs the
+    """Configuration C1 through 1000 steps: device vs oracle (and vs the
     reference's own dumps where the model rebuilds bit for bit)."""
     print("case c1 1000 steps", flush=True)
     d = str(tmp_path / "c1")
